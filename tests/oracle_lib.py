@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU oracle (oracle/libbbt_oracle.so).
+
+Test infrastructure: only the tests, the smoke check and the benchmark's
+CPU-baseline leg load this module, as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "libbbt_oracle.so")
+
+SIMPLE, LENGTH, HARMONIC, BINOMIAL = 0, 1, 2, 3
+STD, ORDERED, BESTHIT, SCORES = 0, 1, 2, 3
+
+
+class OrcFilter(C.Structure):
+    _fields_ = [("bits", C.c_void_p), ("m", C.c_uint64), ("h", C.c_uint32),
+                ("k", C.c_uint32), ("fpr", C.c_double)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("scorer", C.c_uint32), ("streak", C.c_uint32),
+                ("threshold", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = C.CDLL(LIB_PATH)
+        u64p = C.POINTER(C.c_uint64)
+        L.orc_nthash.restype = C.c_int64
+        L.orc_nthash.argtypes = [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                 u64p, u64p, C.c_uint64]
+        L.orc_nthash_window.argtypes = [C.c_char_p, C.c_uint32, u64p, u64p]
+        L.orc_multihash.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, u64p]
+        L.orc_sthash.restype = C.c_int64
+        L.orc_sthash.argtypes = [C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint32,
+                                 C.c_uint32, u64p, u64p, C.c_uint64]
+        L.orc_bf_contains.restype = C.c_int
+        L.orc_bf_contains.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, u64p,
+                                      C.POINTER(C.c_uint32)]
+        L.orc_bf_insert.restype = C.c_int
+        L.orc_bf_insert.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, u64p]
+        L.orc_bf_insert_seq.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32,
+                                        C.c_uint32, C.c_char_p, C.c_uint64, u64p,
+                                        u64p]
+        L.orc_popcount.restype = C.c_uint64
+        L.orc_popcount.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_calc_optimal_size.restype = C.c_uint64
+        L.orc_calc_optimal_size.argtypes = [C.c_uint64, C.c_double, C.c_uint32]
+        L.orc_calc_optimal_hashnum.restype = C.c_uint32
+        L.orc_calc_optimal_hashnum.argtypes = [C.c_double]
+        L.orc_binom_pmf.restype = C.c_double
+        L.orc_binom_pmf.argtypes = [C.c_uint64, C.c_uint64, C.c_double]
+        L.orc_binom_sf.restype = C.c_double
+        L.orc_binom_sf.argtypes = [C.c_int64, C.c_uint64, C.c_double]
+        L.orc_binom_isf_up.restype = C.c_uint64
+        L.orc_binom_isf_up.argtypes = [C.c_double, C.c_uint64, C.c_double]
+        L.orc_calc_min_count.restype = C.c_uint64
+        L.orc_calc_min_count.argtypes = [C.c_uint64, C.c_double, C.c_double]
+        L.orc_eval_read.restype = C.c_int
+        L.orc_eval_read.argtypes = [C.POINTER(OrcFilter), C.c_char_p, C.c_uint64,
+                                    C.POINTER(OrcParams), u64p, u64p]
+        L.orc_eval_score.restype = C.c_double
+        L.orc_eval_score.argtypes = [C.POINTER(OrcFilter), C.c_char_p, C.c_uint64,
+                                     C.POINTER(OrcParams), u64p, u64p]
+        L.orc_classify.restype = C.c_int
+        L.orc_classify.argtypes = [C.POINTER(OrcFilter), C.c_uint32, C.c_void_p,
+                                   u64p, C.c_uint64, C.c_int, C.c_int, C.c_uint32,
+                                   C.POINTER(OrcParams), C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, u64p]
+        L.orc_summary.restype = C.c_uint64
+        L.orc_summary.argtypes = [C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p,
+                                  C.c_uint64, C.c_int, C.c_int, C.c_char_p,
+                                  C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def _u64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def nthash(seq, k, h):
+    """(positions, hashes[n, h]) for every N-free window of seq."""
+    if isinstance(seq, str):
+        seq = seq.encode()
+    cap = max(1, len(seq))
+    pos = np.zeros(cap, np.uint64)
+    hv = np.zeros(cap * h, np.uint64)
+    n = lib().orc_nthash(seq, len(seq), k, h, _u64(pos), _u64(hv), cap)
+    assert n >= 0
+    return pos[:n].copy(), hv[: n * h].reshape(n, h).copy()
+
+
+def nthash_window(win, k):
+    if isinstance(win, str):
+        win = win.encode()
+    f, r = C.c_uint64(), C.c_uint64()
+    lib().orc_nthash_window(win, k, C.byref(f), C.byref(r))
+    return f.value, r.value
+
+
+def multihash(h0, k, h):
+    out = np.zeros(h, np.uint64)
+    lib().orc_multihash(h0, k, h, _u64(out))
+    return [int(x) for x in out]
+
+
+def sthash(seq, seeds, k):
+    if isinstance(seq, str):
+        seq = seq.encode()
+    cap = max(1, len(seq))
+    pos = np.zeros(cap, np.uint64)
+    hv = np.zeros(cap * len(seeds), np.uint64)
+    n = lib().orc_sthash(seq, len(seq), "".join(seeds).encode(), len(seeds), k,
+                         _u64(pos), _u64(hv), cap)
+    assert n >= 0
+    return pos[:n].copy(), hv[: n * len(seeds)].reshape(n, len(seeds)).copy()
+
+
+class Filter:
+    """A Bloom bit array owned by numpy, in the oracle's layout."""
+
+    def __init__(self, m, h, k, bits=None):
+        self.m, self.h, self.k = int(m), int(h), int(k)
+        nbytes = (self.m + 7) // 8
+        self.bits = np.zeros(nbytes, np.uint8) if bits is None else bits
+        self._fpr = None
+
+    def insert_seq(self, seq):
+        if isinstance(seq, str):
+            seq = seq.encode()
+        new, red = C.c_uint64(0), C.c_uint64(0)
+        lib().orc_bf_insert_seq(self.bits.ctypes.data, self.m, self.h, self.k, seq,
+                                len(seq), C.byref(new), C.byref(red))
+        self._fpr = None
+        return new.value, red.value
+
+    def insert_hashes(self, hv):
+        hv = np.ascontiguousarray(hv, np.uint64)
+        self._fpr = None
+        return lib().orc_bf_insert(self.bits.ctypes.data, self.m, self.h, _u64(hv))
+
+    def contains_hashes(self, hv):
+        hv = np.ascontiguousarray(hv, np.uint64)
+        lk = C.c_uint32(0)
+        r = lib().orc_bf_contains(self.bits.ctypes.data, self.m, self.h, _u64(hv),
+                                  C.byref(lk))
+        return bool(r), lk.value
+
+    def popcount(self):
+        return int(lib().orc_popcount(self.bits.ctypes.data, self.bits.size))
+
+    @property
+    def fpr(self):
+        if self._fpr is None:
+            self._fpr = (self.popcount() / self.m) ** self.h
+        return self._fpr
+
+    def struct(self):
+        return OrcFilter(self.bits.ctypes.data, self.m, self.h, self.k, self.fpr)
+
+
+def params(scorer=SIMPLE, threshold=0.15, streak=3):
+    return OrcParams(scorer, streak, threshold)
+
+
+def eval_read(filt, seq, p):
+    if isinstance(seq, str):
+        seq = seq.encode()
+    pr, lk = C.c_uint64(0), C.c_uint64(0)
+    fs = filt.struct()
+    r = lib().orc_eval_read(C.byref(fs), seq, len(seq), C.byref(p), C.byref(pr),
+                            C.byref(lk))
+    return bool(r), pr.value, lk.value
+
+
+def eval_score(filt, seq, p):
+    if isinstance(seq, str):
+        seq = seq.encode()
+    pr, lk = C.c_uint64(0), C.c_uint64(0)
+    fs = filt.struct()
+    r = lib().orc_eval_score(C.byref(fs), seq, len(seq), C.byref(p), C.byref(pr),
+                             C.byref(lk))
+    return r, pr.value, lk.value
+
+
+def pack_reads(reads):
+    """Concatenate reads (bytes/str) into (uint8 buffer, uint64 offsets)."""
+    bs = [r.encode() if isinstance(r, str) else bytes(r) for r in reads]
+    off = np.zeros(len(bs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(b) for b in bs], dtype=np.uint64)
+    buf = np.frombuffer(b"".join(bs) + b"\0", np.uint8).copy()
+    return buf, off
+
+
+def classify(filters, buf, off, paired=False, inclusive=False, mode=STD, p=None,
+             nthreads=0, want_scores=False):
+    """Oracle classification of a packed batch -> (hits, scores, cls, counters)."""
+    p = p or params()
+    nreads = len(off) - 1
+    nf = len(filters)
+    arr = (OrcFilter * nf)(*[f.struct() for f in filters])
+    hits = np.zeros(nreads * nf, np.uint8)
+    scores = np.zeros(nreads * nf, np.float64) if want_scores else None
+    nrec = nreads // 2 if paired else nreads
+    cls = np.zeros(max(1, nrec), np.uint32)
+    counters = np.zeros(2, np.uint64)
+    rc = lib().orc_classify(arr, nf, buf.ctypes.data, _u64(off), nreads, int(paired),
+                            int(inclusive), mode, C.byref(p), nthreads,
+                            hits.ctypes.data,
+                            scores.ctypes.data if scores is not None else None,
+                            cls.ctypes.data, _u64(counters))
+    assert rc == 0
+    return (hits.reshape(nreads, nf), None if scores is None else scores.reshape(nreads, nf),
+            cls[:nrec], counters)
+
+
+def summary(filter_ids, hits, paired=False, inclusive=False):
+    nreads, nf = hits.shape
+    ids = (C.c_char_p * nf)(*[s.encode() for s in filter_ids])
+    h = np.ascontiguousarray(hits, np.uint8)
+    n = lib().orc_summary(ids, nf, h.ctypes.data, nreads, int(paired), int(inclusive),
+                          None, 0)
+    buf = C.create_string_buffer(int(n) + 1)
+    lib().orc_summary(ids, nf, h.ctypes.data, nreads, int(paired), int(inclusive), buf,
+                      n + 1)
+    return buf.value.decode()
