@@ -1,0 +1,60 @@
+"""Builds libbbt_hip.so (gfx950 kernels + C ABI) in-tree with hipcc, and the
+C++ front end.  Run as `python -m biobloom_amd.build`."""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIB = os.path.join(PKG, "libbbt_hip.so")
+CLI = os.path.join(PKG, "biobloomcategorizer")
+MAKER = os.path.join(PKG, "biobloommaker")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ARCH = os.environ.get("BBT_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are disabled for all device code.
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib():
+    srcs = [os.path.join(CSRC, "bbt_hip.hip"), os.path.join(CSRC, "bbt_host.cpp")]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *DEVICE_FLAGS, "-shared", "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+
+
+def build_cli():
+    """The restated biobloomcategorizer / biobloommaker front ends."""
+    if not os.path.isdir(HOST):
+        return
+    common = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+              "-Wall", f"-Wl,-rpath,{PKG}", "-L", PKG, "-lbbt_hip", "-lz"]
+    for name, out in [("biobloomcategorizer.cpp", CLI), ("biobloommaker.cpp", MAKER)]:
+        src = os.path.join(HOST, name)
+        if os.path.exists(src):
+            _run([*common[:1], "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                  "-Wall", "-o", out, src, os.path.join(HOST, "front_end.cpp"),
+                  f"-Wl,-rpath,{PKG}", "-L", PKG, "-lbbt_hip", "-lz"])
+
+
+def build():
+    build_lib()
+    build_cli()
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,451 @@
+/*
+ * bbt_device.h -- the per-(read, filter) SeqEval walk as a lane state machine.
+ *
+ * Shared by the gfx950 kernels and by the host-side self test (the same code
+ * compiled for the CPU), so the walk can be checked without a device.
+ *
+ * Reference behaviour restated here:
+ *   ntHashIterator window order, N handling, canonical hash and multi-hash
+ *       (btl_bloomfilter, absent; call site Common/SeqEval.h:50)
+ *   BloomFilter::contains -- h bit tests, stop at the first clear bit
+ *       (btl_bloomfilter, absent; call sites Common/SeqEval.h:54,79)
+ *   evalSimple      Common/SeqEval.h:37-114
+ *   evalHarmonic    Common/SeqEval.h:116-193
+ *   evalBinomial    Common/SeqEval.h:218-290
+ *   evalMinMatchLen Common/SeqEval.h:295-332
+ *   eval*Score      Common/SeqEval.h:334-491
+ *
+ * The walk consumes one base per step (rolling the hash through N-runs from a
+ * zeroed state), and stops whenever it needs a bit of the filter: the caller
+ * issues that load and hands the byte back through consume().  Every lane
+ * therefore has at most one filter load in flight, and performs exactly the
+ * bit lookups the reference's contains() calls would, in the same order.
+ */
+#ifndef BBT_DEVICE_H
+#define BBT_DEVICE_H
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define BBT_HD __host__ __device__ __forceinline__
+#else
+#define BBT_HD inline
+#endif
+
+namespace bbt {
+
+/* ntHash v1 seeds (Mohamadi et al. 2016) and multi-hash constants. */
+constexpr uint64_t kSeedA = 0x3c8bfbb395c60474ULL;
+constexpr uint64_t kSeedC = 0x3193c18562a02b4cULL;
+constexpr uint64_t kSeedG = 0x20323ed082572324ULL;
+constexpr uint64_t kSeedT = 0x295549f54be24456ULL;
+constexpr uint64_t kMultiSeed = 0x90b45d39fb6da1faULL;
+constexpr unsigned kMultiShift = 27;
+constexpr unsigned kMaxHash = 16;
+
+enum Scorer : uint32_t { SIMPLE = 0, LENGTH = 1, HARMONIC = 2, BINOMIAL = 3 };
+
+BBT_HD uint64_t rotl64(uint64_t v, unsigned s) {
+    s &= 63;
+    return s ? (v << s) | (v >> (64 - s)) : v;
+}
+BBT_HD uint64_t rotl1(uint64_t v) { return (v << 1) | (v >> 63); }
+BBT_HD uint64_t rotr1(uint64_t v) { return (v >> 1) | (v << 63); }
+
+/* ASCII -> 2-bit code with (c >> 1) & 3: A 0, C 1, T 2, G 3 in either case;
+ * the complement of a code is code ^ 2. */
+BBT_HD bool base_valid(unsigned c) {
+    unsigned u = c & 0xDFu;
+    return u == 'A' || u == 'C' || u == 'G' || u == 'T';
+}
+BBT_HD unsigned base_code(unsigned c) { return (c >> 1) & 3u; }
+
+/* seed of a code, forward strand */
+BBT_HD uint64_t seed_fwd(unsigned code) {
+    return (code & 2) ? ((code & 1) ? kSeedG : kSeedT) : ((code & 1) ? kSeedC : kSeedA);
+}
+/* seed of the complement base */
+BBT_HD uint64_t seed_rc(unsigned code) { return seed_fwd(code ^ 2u); }
+
+BBT_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return uint64_t((unsigned __int128)a * b >> 64);
+#endif
+}
+
+/* Exact n % d for a fixed 64-bit divisor: Granlund & Montgomery, "Division
+ * by invariant integers using multiplication" (PLDI 1994), Fig. 4.1. */
+struct FastMod {
+    uint64_t d;
+    uint64_t magic;
+    uint32_t sh1;
+    uint32_t sh2;
+};
+
+inline FastMod fastmod_make(uint64_t d) {
+    FastMod f;
+    f.d = d;
+    unsigned l = 0;
+    while (l < 64 && (uint64_t(1) << l) < d)
+        ++l; /* l = ceil(log2 d) */
+    unsigned __int128 two_l = (unsigned __int128)1 << l;
+    f.magic = uint64_t((((two_l - d) << 64) / d) + 1);
+    f.sh1 = l < 1 ? l : 1;
+    f.sh2 = l > 0 ? l - 1 : 0;
+    return f;
+}
+
+BBT_HD uint64_t fastmod(uint64_t n, uint64_t d, uint64_t magic, uint32_t sh1,
+                        uint32_t sh2) {
+    uint64_t t1 = mulhi64(magic, n);
+    uint64_t q = (t1 + ((n - t1) >> sh1)) >> sh2;
+    return n - q * d;
+}
+
+/* Device-resident description of one Bloom filter. */
+struct FilterDev {
+    const uint8_t *bits;     /* m/8 bytes, bit p = byte p>>3, mask 1<<(p&7) */
+    uint64_t m;              /* bits */
+    uint64_t magic;          /* FastMod of m */
+    uint32_t sh1, sh2;
+    uint32_t h, k;
+    double fpr;              /* (popcount/m)^h, for the binomial scorer */
+    const uint32_t *binom;   /* calcMinCount per frame length, or null */
+    uint32_t binom_len;      /* entries in binom */
+    uint32_t pad;
+    uint64_t mult[kMaxHash]; /* mult[j] = j ^ (k * kMultiSeed), j >= 1 */
+};
+
+/* Per-walk parameters that are uniform over a launch. */
+struct WalkParams {
+    double threshold;  /* opt::score after CLI interpretation */
+    uint32_t streak;   /* opt::streakThreshold */
+    uint32_t min_len;  /* LENGTH: (unsigned) round(threshold) */
+};
+
+/* Walk outcome codes. */
+enum : uint32_t { W_RUNNING = 0, W_ACCEPT = 1, W_REJECT = 2, W_DONE = 3 };
+
+/*
+ * Lane state machine for one (read, filter) walk.
+ *   SC   scorer (Scorer)
+ *   EXH  false: evalRead (early accept/reject); true: evalScore (exhaustive)
+ *   K    compile-time k, or 0 for a runtime k
+ */
+template <int SC, bool EXH, int K>
+struct Walker {
+    /* read */
+    const uint8_t *s;
+    uint32_t len;
+    uint32_t i; /* next base */
+    uint64_t word;
+    uint64_t waddr;
+    /* rolling hash */
+    uint64_t fwd, rc, win;
+    uint32_t run;
+    uint32_t k;
+    /* filter */
+    const FilterDev *F;
+    const uint8_t *bits;
+    uint64_t m, magic;
+    uint32_t sh1, sh2, h;
+    /* probe in progress */
+    uint64_t h0;
+    uint32_t j;
+    uint32_t pending;
+    const uint8_t *addr;
+    uint32_t mask;
+    /* automaton */
+    uint32_t p, prev;
+    uint32_t state;
+    uint32_t first;
+    uint32_t skipping, skipEnd;
+    uint32_t streak, anti, anti_thr;
+    uint32_t score2;  /* SIMPLE: 2*score; BINOMIAL: count; LENGTH: matchLen */
+    uint32_t thr2;    /* SIMPLE: ceil(2*thres); BINOMIAL: thres; LENGTH: minLen */
+    double dscore;    /* HARMONIC score */
+    double dthr;      /* HARMONIC thres */
+    uint32_t streak_thr;
+    /* instrumentation: contains() calls and bit lookups */
+    uint32_t probes, lookups;
+
+    BBT_HD unsigned kk() const { return K ? unsigned(K) : k; }
+
+    BBT_HD void init(const uint8_t *seq, uint32_t length, const FilterDev *f,
+                     const WalkParams &wp) {
+        s = seq;
+        len = length;
+        i = 0;
+        waddr = ~uint64_t(0);
+        word = 0;
+        fwd = rc = win = 0;
+        run = 0;
+        F = f;
+        bits = f->bits;
+        m = f->m;
+        magic = f->magic;
+        sh1 = f->sh1;
+        sh2 = f->sh2;
+        h = f->h;
+        k = f->k;
+        pending = 0;
+        prev = 0;
+        first = 1;
+        skipping = 0;
+        skipEnd = 0;
+        streak = 0;
+        anti = 0;
+        score2 = 0;
+        dscore = 0.0;
+        streak_thr = wp.streak;
+        state = W_RUNNING;
+        const unsigned kv = kk();
+        /* frame count as the size_t expression seqLen - k + 1 (SeqEval.h:28-35) */
+        const uint64_t nframes = uint64_t(length) - uint64_t(kv) + 1;
+        if (SC == SIMPLE || SC == HARMONIC) {
+            const double n = double(nframes);
+            const double thres = wp.threshold * n;
+            const double at = floor((1.0 - wp.threshold) * n);
+            /* a negative antiThres rejects at the first miss, as 0 does */
+            anti_thr = at >= 4294967295.0 ? 0xFFFFFFFFu : (at <= 0 ? 0u : uint32_t(at));
+            if (SC == SIMPLE) {
+                const double t2 = ceil(2.0 * thres);
+                thr2 = t2 >= 4294967295.0 ? 0xFFFFFFFFu : (t2 <= 0 ? 0u : uint32_t(t2));
+            }
+            dthr = thres;
+        } else if (SC == BINOMIAL) {
+            if (length < kv) {
+                state = EXH ? W_DONE : W_REJECT;
+                thr2 = 0;
+                anti_thr = 0;
+            } else {
+                const uint32_t fl = uint32_t(nframes);
+                thr2 = (!EXH && f->binom && fl < f->binom_len) ? f->binom[fl] : 1u;
+                anti_thr = fl - thr2; /* unsigned, as SeqEval.h:227 */
+            }
+        } else {
+            thr2 = wp.min_len;
+        }
+    }
+
+    /* byte i of the read through an 8-byte aligned register window */
+    BBT_HD unsigned base_at(uint32_t idx) {
+        uint64_t a = reinterpret_cast<uint64_t>(s) + idx;
+        uint64_t wa = a & ~uint64_t(7);
+        if (wa != waddr) {
+            waddr = wa;
+            word = *reinterpret_cast<const uint64_t *>(wa);
+        }
+        return unsigned(word >> ((a & 7) * 8)) & 0xFFu;
+    }
+
+    BBT_HD uint64_t hash_j(uint32_t jj) const {
+        if (jj == 0)
+            return h0;
+        uint64_t t = h0 * F->mult[jj];
+        return t ^ (t >> kMultiShift);
+    }
+
+    BBT_HD void set_addr() {
+        uint64_t pos = fastmod(hash_j(j), m, magic, sh1, sh2);
+        addr = bits + (pos >> 3);
+        mask = 1u << unsigned(pos & 7);
+    }
+
+    BBT_HD void finish_end() {
+        if (EXH)
+            state = W_DONE;
+        else
+            state = W_REJECT;
+    }
+
+    /* Advance until a filter byte is needed (returns true, addr/mask set) or
+     * the walk has ended (returns false). */
+    BBT_HD bool advance() {
+        if (pending)
+            return true;
+        if (state != W_RUNNING)
+            return false;
+        const unsigned kv = kk();
+        while (true) {
+            if (i >= len) {
+                finish_end();
+                return false;
+            }
+            const unsigned c = base_at(i);
+            ++i;
+            if (!base_valid(c)) {
+                run = 0;
+                fwd = rc = 0;
+                continue;
+            }
+            const unsigned code = base_code(c);
+            if (run >= kv) {
+                const unsigned out = unsigned(win >> (2 * (kv - 1))) & 3u;
+                fwd = rotl1(fwd) ^ rotl64(seed_fwd(out), kv) ^ seed_fwd(code);
+                rc = rotr1(rc ^ seed_rc(out) ^ rotl64(seed_rc(code), kv));
+            } else {
+                fwd = rotl1(fwd) ^ seed_fwd(code);
+                rc = rotr1(rc ^ rotl64(seed_rc(code), kv));
+                ++run;
+            }
+            win = (win << 2) | code;
+            if (run < kv)
+                continue;
+            p = i - kv; /* start of the N-free window ending at base i-1 */
+            if (SC == LENGTH) {
+                if (!EXH) {
+                    /* SeqEval.h:306: quit when the rest cannot reach minLen */
+                    if (uint64_t(len) - p + score2 < uint64_t(thr2)) {
+                        state = W_REJECT;
+                        return false;
+                    }
+                }
+                if (p != prev + 1)
+                    score2 = 0;
+            } else if (!EXH) {
+                if (skipping) {
+                    if (p < skipEnd) { /* SeqEval.h:103-108 */
+                        if (anti_thr <= ++anti) {
+                            state = W_REJECT;
+                            return false;
+                        }
+                        prev = p;
+                        continue;
+                    }
+                    skipping = 0;
+                }
+                if (!first && p != prev + 1) { /* SeqEval.h:71-77 */
+                    anti += p - prev - 1;
+                    if (anti_thr <= anti) {
+                        state = W_REJECT;
+                        return false;
+                    }
+                    streak = 0;
+                }
+            } else {
+                if (skipping) {
+                    if (p < skipEnd) {
+                        prev = p;
+                        continue;
+                    }
+                    skipping = 0;
+                }
+                if (p != prev + 1)
+                    streak = 0;
+            }
+            /* contains(*itr) */
+            h0 = rc < fwd ? rc : fwd;
+            j = 0;
+            ++probes;
+            pending = 1;
+            set_addr();
+            return true;
+        }
+    }
+
+    /* Hand back the byte loaded from addr. */
+    BBT_HD void consume(unsigned byte) {
+        ++lookups;
+        const bool bit = (byte & mask) != 0;
+        if (bit && j + 1 < h) {
+            ++j;
+            set_addr();
+            return;
+        }
+        pending = 0;
+        on_result(bit);
+    }
+
+    BBT_HD void on_result(bool hit) {
+        const unsigned kv = kk();
+        if (SC == LENGTH) {
+            score2 = hit ? (score2 == 0 ? kv : score2 + 1) : 0;
+            if (!EXH && score2 >= thr2)
+                state = W_ACCEPT;
+            prev = p;
+            return;
+        }
+        if (!EXH && first) { /* SeqEval.h:52-66 */
+            first = 0;
+            if (hit) {
+                if (add_hit())
+                    return;
+                streak = 1;
+            } else if (anti_thr <= ++anti) {
+                state = W_REJECT;
+                return;
+            }
+            prev = p;
+            return;
+        }
+        if (hit) {
+            if (add_hit())
+                return;
+            prev = p;
+            ++streak;
+            return;
+        }
+        if (streak < streak_thr) {
+            if (!EXH && anti_thr <= ++anti) {
+                state = W_REJECT;
+                return;
+            }
+        } else {
+            /* jump-skip: this window and the following ones below p + k
+             * count as misses without probes (SeqEval.h:99-109) */
+            skipping = 1;
+            skipEnd = p + kv;
+            if (!EXH && anti_thr <= ++anti) {
+                state = W_REJECT;
+                return;
+            }
+        }
+        prev = p;
+        streak = 0;
+    }
+
+    /* adds a hit to the score; true if the walk accepted */
+    BBT_HD bool add_hit() {
+        if (SC == HARMONIC) {
+            if (streak == 0)
+                dscore += 0.5;
+            else
+                dscore += 1.0 - 1.0 / (1.0 + double(streak));
+            if (!EXH && dthr <= dscore) {
+                state = W_ACCEPT;
+                return true;
+            }
+            return false;
+        }
+        if (SC == BINOMIAL)
+            score2 += 1;
+        else
+            score2 += streak == 0 ? 1u : 2u;
+        if (!EXH && thr2 <= score2) {
+            state = W_ACCEPT;
+            return true;
+        }
+        return false;
+    }
+
+    /* Result of a finished walk: evalRead's bool, or evalScore's value (for
+     * BINOMIAL, the hit count; the caller applies the binomial tail). */
+    BBT_HD bool accepted() const { return state == W_ACCEPT; }
+    BBT_HD double score_value() const {
+        const double n = double(uint64_t(len) - uint64_t(kk()) + 1);
+        if (SC == SIMPLE)
+            return (double(score2) * 0.5) / n;
+        if (SC == HARMONIC)
+            return dscore / n;
+        return double(score2);
+    }
+};
+
+} // namespace bbt
+
+#endif

@@ -1,0 +1,971 @@
+/*
+ * bbt_hip.hip -- gfx950 kernels and the C ABI of libbbt_hip.
+ *
+ * Kernels
+ *   k_walk_plain       one lane per (read, filter) walk, one launch per batch
+ *   k_walk_persistent  persistent waves pulling (read, filter) walks from a
+ *                      device work queue; idle lanes are refilled per wave so
+ *                      every lane keeps one filter load in flight
+ *   k_ordered          ORDERED mode: one lane per record, filters in order
+ *   k_classify         hit lists -> ResultsManager class + summary counters
+ *   k_insert           filter construction: every k-mer of a sequence buffer
+ *
+ * Reference behaviour is restated in bbt_device.h (the walk) and below
+ * (modes: BioBloomClassifier.cpp:1145-1237; classes: ResultsManager.hpp:
+ * 40-189).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bbt_hip.h"
+#include "bbt_device.h"
+#include "bbt_host.h"
+
+using namespace bbt;
+
+/* ======================================================================= */
+/* error reporting                                                          */
+
+static thread_local std::string g_err;
+
+static bbt_status fail(bbt_status code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return fail(BBT_ERR_HIP, std::string(#expr) + ": " +                \
+                                         hipGetErrorString(e_));                \
+    } while (0)
+
+/* ======================================================================= */
+/* kernels                                                                  */
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct EvalArgs {
+    const uint8_t *seq;
+    const uint64_t *off;
+    const uint32_t *order;
+    const FilterDev *filters;
+    uint8_t *hits;
+    double *scores;
+    unsigned long long *counters;
+    unsigned long long *queue;
+    uint64_t nitems;
+    uint32_t nf;
+    uint32_t refill_min;
+    WalkParams wp;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t probes,
+                                             uint32_t lookups) {
+    if (!ctr)
+        return;
+    unsigned long long a = probes, b = lookups;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(ctr + 0, a);
+        atomicAdd(ctr + 1, b);
+    }
+}
+
+template <int SC, bool EXH, int K>
+__device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uint32_t f,
+                                           const Walker<SC, EXH, K> &w) {
+    const uint64_t idx = read * a.nf + f;
+    if (EXH)
+        a.scores[idx] = w.score_value();
+    else
+        a.hits[idx] = w.accepted() ? 1 : 0;
+}
+
+template <int SC, bool EXH, int K>
+__device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item,
+                                           Walker<SC, EXH, K> &w, uint64_t &read,
+                                           uint32_t &f) {
+    const uint64_t r = item / a.nf;
+    f = uint32_t(item - r * a.nf);
+    read = a.order ? a.order[r] : r;
+    const uint64_t b = a.off[read];
+    w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+}
+
+template <int SC, bool EXH, int K>
+__global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
+    const uint64_t item = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    Walker<SC, EXH, K> w;
+    w.probes = 0;
+    w.lookups = 0;
+    if (item < a.nitems) {
+        uint64_t read;
+        uint32_t f;
+        start_walk(a, item, w, read, f);
+        while (w.advance())
+            w.consume(*w.addr);
+        store_walk(a, read, f, w);
+    }
+    add_counters(a.counters, w.probes, w.lookups);
+}
+
+template <int SC, bool EXH, int K>
+__global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    Walker<SC, EXH, K> w;
+    w.probes = 0;
+    w.lookups = 0;
+    bool active = false;
+    bool drained = false;
+    uint64_t read = 0;
+    uint32_t f = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (!drained && nidle != 0 && (nidle >= a.refill_min || nidle == 64)) {
+            const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
+            unsigned long long base = 0;
+            if (lane == leader)
+                base = atomicAdd(a.queue, (unsigned long long)nidle);
+            base = __shfl(base, int(leader));
+            if (base + nidle >= a.nitems)
+                drained = true;
+            if (!active) {
+                const uint64_t item = base + uint64_t(__popcll(idle & lt_mask));
+                if (item < a.nitems) {
+                    start_walk(a, item, w, read, f);
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained)
+                break;
+            continue;
+        }
+        if (active) {
+            if (w.advance()) {
+                const unsigned byte = *w.addr;
+                w.consume(byte);
+            } else {
+                store_walk(a, read, f, w);
+                active = false;
+            }
+        }
+    }
+    add_counters(a.counters, w.probes, w.lookups);
+}
+
+template <int SC, int K>
+__device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
+                         Walker<SC, false, K> &w) {
+    const uint64_t b = a.off[read];
+    w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+    while (w.advance())
+        w.consume(*w.addr);
+    return w.accepted();
+}
+
+/* evaluateReadOrdered / evaluateReadOrderedPair (BioBloomClassifier.cpp:
+ * 1145-1180): the first filter that passes, with the same short-circuit. */
+template <int SC, int K>
+__global__ __launch_bounds__(kBlock) void k_ordered(EvalArgs a, uint64_t nrec,
+                                                    uint32_t paired, uint32_t inclusive) {
+    const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    Walker<SC, false, K> w;
+    w.probes = 0;
+    w.lookups = 0;
+    if (rec < nrec) {
+        const uint64_t r1 = paired ? 2 * rec : rec;
+        for (uint32_t f = 0; f < a.nf; ++f) {
+            bool pass;
+            if (!paired)
+                pass = walk_one(a, r1, f, w);
+            else if (inclusive)
+                pass = walk_one(a, r1, f, w) || walk_one(a, r1 + 1, f, w);
+            else
+                pass = walk_one(a, r1, f, w) && walk_one(a, r1 + 1, f, w);
+            if (pass) {
+                a.hits[r1 * a.nf + f] = 1;
+                if (paired)
+                    a.hits[(r1 + 1) * a.nf + f] = 1;
+                break;
+            }
+        }
+    }
+    add_counters(a.counters, w.probes, w.lookups);
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        p[i] = 0;
+}
+
+/* evaluateReadBestHit (BioBloomClassifier.cpp:1194-1218): the filters with
+ * the maximal evalScore, kept only when that maximum is > 0. */
+__device__ void best_hits(const double *srow, uint8_t *hrow, uint32_t nf) {
+    double mx = 0;
+    for (uint32_t i = 0; i < nf; ++i) {
+        const double s = srow[i];
+        if (mx < s)
+            mx = s;
+    }
+    for (uint32_t i = 0; i < nf; ++i)
+        hrow[i] = (mx > 0 && srow[i] == mx) ? 1 : 0;
+}
+
+/* ResultsManager::updateSummaryData (ResultsManager.hpp:66-189). */
+__global__ __launch_bounds__(kBlock) void k_classify(uint8_t *hits, const double *scores,
+                                                     uint32_t nf, uint64_t nrec,
+                                                     uint32_t paired, uint32_t inclusive,
+                                                     uint32_t besthit, uint32_t *cls,
+                                                     unsigned long long *summary) {
+    extern __shared__ unsigned int cnt[]; /* 2*nf + 2 */
+    const uint32_t ncnt = 2 * nf + 2;
+    if (summary)
+        for (uint32_t i = threadIdx.x; i < ncnt; i += blockDim.x)
+            cnt[i] = 0;
+    __syncthreads();
+    const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (rec < nrec) {
+        const uint64_t r1 = paired ? 2 * rec : rec;
+        uint8_t *h1 = hits + r1 * nf;
+        uint8_t *h2 = paired ? h1 + nf : nullptr;
+        if (besthit) {
+            best_hits(scores + r1 * nf, h1, nf);
+            if (paired)
+                best_hits(scores + (r1 + 1) * nf, h2, nf);
+        }
+        uint32_t c = nf;
+        for (uint32_t i = 0; i < nf; ++i) {
+            bool in = paired ? (inclusive ? (h1[i] | h2[i]) != 0 : (h1[i] & h2[i]) != 0)
+                             : h1[i] != 0;
+            if (in) {
+                c = (c == nf) ? i : nf + 1;
+                if (summary)
+                    atomicAdd(&cnt[i], 1u);
+            }
+        }
+        if (cls)
+            cls[rec] = c;
+        if (summary) {
+            if (c == nf)
+                atomicAdd(&cnt[2 * nf + 1], 1u);
+            else if (c == nf + 1)
+                atomicAdd(&cnt[2 * nf], 1u);
+            else
+                atomicAdd(&cnt[nf + c], 1u);
+        }
+    }
+    __syncthreads();
+    if (summary)
+        for (uint32_t i = threadIdx.x; i < ncnt; i += blockDim.x)
+            if (cnt[i])
+                atomicAdd(summary + i, (unsigned long long)cnt[i]);
+}
+
+/* BloomFilterGenerator::loadFilter (BloomFilterGenerator.h:143-188): insert
+ * every N-free k-mer.  Thread t owns window starts [t*chunk, (t+1)*chunk). */
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t nbytes,
+                                                   FilterDev f, uint32_t chunk,
+                                                   unsigned long long *counts) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t p0 = t * chunk;
+    uint32_t nnew = 0, nold = 0;
+    if (p0 < nbytes) {
+        const unsigned kv = K ? unsigned(K) : f.k;
+        const uint64_t last = std::min<uint64_t>(p0 + chunk + kv - 1, nbytes);
+        unsigned int *words = reinterpret_cast<unsigned int *>(const_cast<uint8_t *>(f.bits));
+        uint64_t fwd = 0, rc = 0, win = 0;
+        uint32_t run = 0;
+        for (uint64_t i = p0; i < last; ++i) {
+            const unsigned c = seq[i];
+            if (!base_valid(c)) {
+                run = 0;
+                fwd = rc = 0;
+                continue;
+            }
+            const unsigned code = base_code(c);
+            if (run >= kv) {
+                const unsigned out = unsigned(win >> (2 * (kv - 1))) & 3u;
+                fwd = rotl1(fwd) ^ rotl64(seed_fwd(out), kv) ^ seed_fwd(code);
+                rc = rotr1(rc ^ seed_rc(out) ^ rotl64(seed_rc(code), kv));
+            } else {
+                fwd = rotl1(fwd) ^ seed_fwd(code);
+                rc = rotr1(rc ^ rotl64(seed_rc(code), kv));
+                ++run;
+            }
+            win = (win << 2) | code;
+            if (run < kv)
+                continue;
+            const uint64_t h0 = rc < fwd ? rc : fwd;
+            bool found = true;
+            for (uint32_t j = 0; j < f.h; ++j) {
+                uint64_t hv = h0;
+                if (j) {
+                    hv = h0 * f.mult[j];
+                    hv ^= hv >> kMultiShift;
+                }
+                const uint64_t pos = fastmod(hv, f.m, f.magic, f.sh1, f.sh2);
+                const unsigned bit = 1u << unsigned(pos & 31);
+                const unsigned old = atomicOr(words + (pos >> 5), bit);
+                found = found && (old & bit);
+            }
+            if (found)
+                ++nold;
+            else
+                ++nnew;
+        }
+    }
+    unsigned long long a = nnew, b = nold;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if (lane_id() == 0 && counts) {
+        atomicAdd(counts + 0, a);
+        atomicAdd(counts + 1, b);
+    }
+}
+
+__global__ void k_popcount(const uint64_t *w, uint64_t nwords, unsigned long long *out) {
+    unsigned long long c = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nwords;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        c += __popcll(w[i]);
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o);
+    if (lane_id() == 0)
+        atomicAdd(out, c);
+}
+
+} // namespace
+
+/* ======================================================================= */
+/* filter set                                                               */
+
+struct bbt_filterset {
+    int device = 0;
+    std::vector<FilterDev> desc;          /* host copy, device pointers inside */
+    std::vector<void *> dbits;
+    std::vector<std::string> ids;
+    std::vector<bbt_filter_info> info;
+    FilterDev *ddesc = nullptr;
+    bool dirty = true;
+    /* BINOMIAL thresholds per filter: calcMinCount for every frame length */
+    std::vector<uint32_t *> dbinom;
+    double binom_min_fpr = -1;
+    uint32_t binom_frames = 0;
+    /* work queue counter and per-call scratch */
+    unsigned long long *dscratch = nullptr; /* [1..2] insert counts, [3] popcount, [4..5] walk queues */
+    int cu_count = 0;
+    /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_walk_ms = -1;
+    std::mutex mu;
+};
+
+static void fill_desc(FilterDev &d, void *bits, uint64_t m, uint32_t h, uint32_t k) {
+    std::memset(&d, 0, sizeof(d));
+    d.bits = static_cast<const uint8_t *>(bits);
+    d.m = m;
+    FastMod fm = fastmod_make(m);
+    d.magic = fm.magic;
+    d.sh1 = fm.sh1;
+    d.sh2 = fm.sh2;
+    d.h = h;
+    d.k = k;
+    for (uint32_t j = 1; j < kMaxHash; ++j)
+        d.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
+}
+
+static bbt_status sync_desc(bbt_filterset *fs) {
+    if (!fs->dirty)
+        return BBT_OK;
+    const size_t n = fs->desc.size();
+    if (!fs->ddesc)
+        HIP_OK(hipMalloc(&fs->ddesc, sizeof(FilterDev) * std::max<size_t>(n, 1)));
+    for (size_t i = 0; i < n; ++i) {
+        fs->desc[i].fpr = fs->info[i].fpr;
+        fs->desc[i].binom = i < fs->dbinom.size() ? fs->dbinom[i] : nullptr;
+        fs->desc[i].binom_len = fs->dbinom.empty() ? 0 : fs->binom_frames + 1;
+    }
+    HIP_OK(hipMemcpy(fs->ddesc, fs->desc.data(), sizeof(FilterDev) * n,
+                     hipMemcpyHostToDevice));
+    fs->dirty = false;
+    return BBT_OK;
+}
+
+static bbt_status alloc_set(uint32_t n, const uint64_t *m, const uint32_t *h,
+                            const uint32_t *k, bbt_filterset **out) {
+    if (!out || n == 0 || !m || !h || !k)
+        return fail(BBT_ERR_ARG, "bbt_filters_create: empty filter list");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (m[i] == 0 || m[i] % 64 != 0)
+            return fail(BBT_ERR_ARG, "filter size must be a positive multiple of 64 bits");
+        if (h[i] == 0 || h[i] > kMaxHash)
+            return fail(BBT_ERR_ARG, "hash count must be in 1..16");
+        if (k[i] == 0 || k[i] > 32)
+            return fail(BBT_ERR_UNSUPPORTED, "k-mer size must be in 1..32");
+    }
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    bbt_filterset *fs = new bbt_filterset();
+    fs->device = dev;
+    HIP_OK(hipDeviceGetAttribute(&fs->cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+    for (uint32_t i = 0; i < n; ++i) {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, m[i] / 8);
+        if (e != hipSuccess) {
+            bbt_filters_free(fs);
+            return fail(BBT_ERR_NOMEM, "device allocation for filter bits failed");
+        }
+        e = hipMemset(p, 0, m[i] / 8);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            bbt_filters_free(fs);
+            return fail(BBT_ERR_HIP, hipGetErrorString(e));
+        }
+        fs->dbits.push_back(p);
+        FilterDev d;
+        fill_desc(d, p, m[i], h[i], k[i]);
+        fs->desc.push_back(d);
+        bbt_filter_info inf;
+        std::memset(&inf, 0, sizeof(inf));
+        inf.m = m[i];
+        inf.h = h[i];
+        inf.k = k[i];
+        fs->info.push_back(inf);
+        fs->ids.push_back("filter" + std::to_string(i));
+    }
+    if (hipMalloc(&fs->dscratch, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        bbt_filters_free(fs);
+        return fail(BBT_ERR_NOMEM, "device allocation for scratch failed");
+    }
+    *out = fs;
+    return BBT_OK;
+}
+
+static void set_fpr(bbt_filterset *fs, uint32_t idx, uint64_t pop) {
+    bbt_filter_info &inf = fs->info[idx];
+    inf.popcount = pop;
+    /* the FPR the binomial scorer uses: (popcount / m)^h */
+    inf.fpr = std::pow(double(pop) / double(inf.m), double(inf.h));
+    fs->dirty = true;
+}
+
+extern "C" {
+
+const char *bbt_last_error(void) { return g_err.c_str(); }
+
+bbt_status bbt_filters_create(uint32_t n, const uint64_t *m, const uint32_t *h,
+                              const uint32_t *k, bbt_filterset **out) {
+    return alloc_set(n, m, h, k, out);
+}
+
+bbt_status bbt_filters_load(const char *const *paths, uint32_t n, bbt_filterset **out) {
+    if (!paths || n == 0 || !out)
+        return fail(BBT_ERR_ARG, "Error: Need Filter File (-f)");
+    std::vector<BfHeader> hdrs(n);
+    std::vector<std::vector<uint8_t>> bits(n);
+    std::vector<InfoFile> infos(n);
+    std::string err;
+    for (uint32_t i = 0; i < n; ++i) {
+        std::string bf = paths[i] ? paths[i] : "";
+        /* <x>.bf -> <x>.txt as BioBloomClassifier.cpp:957 */
+        if (bf.size() < 2)
+            return fail(BBT_ERR_ARG, "Error: " + bf + " File cannot be opened");
+        std::string txt = bf.substr(0, bf.size() - 2) + "txt";
+        if (!read_info_file(txt, infos[i], err))
+            return fail(BBT_ERR_IO, err);
+        if (!read_bf_file(bf, hdrs[i], bits[i], err))
+            return fail(BBT_ERR_FORMAT, err);
+    }
+    std::vector<uint64_t> m(n);
+    std::vector<uint32_t> h(n), k(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        m[i] = hdrs[i].size;
+        h[i] = hdrs[i].nhash;
+        k[i] = hdrs[i].kmer;
+    }
+    bbt_filterset *fs = nullptr;
+    bbt_status st = alloc_set(n, m.data(), h.data(), k.data(), &fs);
+    if (st != BBT_OK)
+        return st;
+    for (uint32_t i = 0; i < n; ++i) {
+        st = bbt_filter_upload(fs, i, bits[i].data(), bits[i].size());
+        if (st != BBT_OK) {
+            bbt_filters_free(fs);
+            return st;
+        }
+        fs->ids[i] = infos[i].filter_id;
+        fs->info[i].desired_fpr = infos[i].desired_fpr;
+    }
+    *out = fs;
+    return BBT_OK;
+}
+
+bbt_status bbt_filters_free(bbt_filterset *fs) {
+    if (!fs)
+        return BBT_OK;
+    for (void *p : fs->dbits)
+        if (p)
+            (void)hipFree(p);
+    for (uint32_t *p : fs->dbinom)
+        if (p)
+            (void)hipFree(p);
+    if (fs->ddesc)
+        (void)hipFree(fs->ddesc);
+    if (fs->dscratch)
+        (void)hipFree(fs->dscratch);
+    if (fs->ev0)
+        (void)hipEventDestroy(fs->ev0);
+    if (fs->ev1)
+        (void)hipEventDestroy(fs->ev1);
+    delete fs;
+    return BBT_OK;
+}
+
+uint32_t bbt_filterset_size(const bbt_filterset *fs) {
+    return fs ? uint32_t(fs->desc.size()) : 0;
+}
+
+bbt_status bbt_filter_info_get(const bbt_filterset *fs, uint32_t idx, bbt_filter_info *out) {
+    if (!fs || idx >= fs->info.size() || !out)
+        return fail(BBT_ERR_ARG, "bbt_filter_info_get: bad index");
+    *out = fs->info[idx];
+    return BBT_OK;
+}
+
+const char *bbt_filter_id(const bbt_filterset *fs, uint32_t idx) {
+    if (!fs || idx >= fs->ids.size())
+        return nullptr;
+    return fs->ids[idx].c_str();
+}
+
+bbt_status bbt_filter_set_id(bbt_filterset *fs, uint32_t idx, const char *id) {
+    if (!fs || idx >= fs->ids.size() || !id)
+        return fail(BBT_ERR_ARG, "bbt_filter_set_id: bad argument");
+    fs->ids[idx] = id;
+    return BBT_OK;
+}
+
+const void *bbt_filter_device_bits(const bbt_filterset *fs, uint32_t idx) {
+    if (!fs || idx >= fs->dbits.size())
+        return nullptr;
+    return fs->dbits[idx];
+}
+
+bbt_status bbt_filter_upload(bbt_filterset *fs, uint32_t idx, const void *host_bits,
+                             uint64_t nbytes) {
+    if (!fs || idx >= fs->desc.size() || !host_bits)
+        return fail(BBT_ERR_ARG, "bbt_filter_upload: bad argument");
+    if (nbytes != fs->info[idx].m / 8)
+        return fail(BBT_ERR_ARG, "bbt_filter_upload: size does not match the filter");
+    HIP_OK(hipMemcpy(fs->dbits[idx], host_bits, nbytes, hipMemcpyHostToDevice));
+    const uint8_t *b = static_cast<const uint8_t *>(host_bits);
+    uint64_t pop = 0;
+    for (uint64_t i = 0; i + 8 <= nbytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        pop += uint64_t(__builtin_popcountll(w));
+    }
+    set_fpr(fs, idx, pop);
+    return BBT_OK;
+}
+
+bbt_status bbt_filter_download(const bbt_filterset *fs, uint32_t idx, void *host_bits,
+                               uint64_t nbytes) {
+    if (!fs || idx >= fs->desc.size() || !host_bits)
+        return fail(BBT_ERR_ARG, "bbt_filter_download: bad argument");
+    if (nbytes != fs->info[idx].m / 8)
+        return fail(BBT_ERR_ARG, "bbt_filter_download: size does not match the filter");
+    HIP_OK(hipMemcpy(host_bits, fs->dbits[idx], nbytes, hipMemcpyDeviceToHost));
+    return BBT_OK;
+}
+
+bbt_status bbt_filter_refresh(bbt_filterset *fs, uint32_t idx, void *stream) {
+    if (!fs || idx >= fs->desc.size())
+        return fail(BBT_ERR_ARG, "bbt_filter_refresh: bad index");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(fs->mu);
+    unsigned long long *pop = fs->dscratch + 3;
+    k_zero_u64<<<1, 64, 0, st>>>(pop, 1);
+    const uint64_t nwords = fs->info[idx].m / 64;
+    k_popcount<<<1024, kBlock, 0, st>>>(static_cast<const uint64_t *>(fs->dbits[idx]),
+                                        nwords, pop);
+    HIP_OK(hipGetLastError());
+    unsigned long long hp = 0;
+    HIP_OK(hipMemcpyAsync(&hp, pop, sizeof(hp), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    set_fpr(fs, idx, hp);
+    return BBT_OK;
+}
+
+bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
+                             uint64_t nbytes, uint64_t *counts, void *stream) {
+    if (!fs || idx >= fs->desc.size() || (!dseq && nbytes))
+        return fail(BBT_ERR_ARG, "bbt_filter_insert: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(fs->mu);
+    unsigned long long *cnt = fs->dscratch + 1;
+    k_zero_u64<<<1, 64, 0, st>>>(cnt, 2);
+    const uint32_t chunk = 256;
+    const uint64_t threads = (nbytes + chunk - 1) / chunk;
+    const uint64_t blocks = (threads + kBlock - 1) / kBlock;
+    if (blocks > 0x7FFFFFFFull)
+        return fail(BBT_ERR_ARG, "bbt_filter_insert: sequence too large for one call");
+    const FilterDev &d = fs->desc[idx];
+    if (blocks) {
+        if (d.k == 25)
+            k_insert<25><<<dim3(uint32_t(blocks)), kBlock, 0, st>>>(
+                reinterpret_cast<const uint8_t *>(dseq), nbytes, d, chunk, cnt);
+        else
+            k_insert<0><<<dim3(uint32_t(blocks)), kBlock, 0, st>>>(
+                reinterpret_cast<const uint8_t *>(dseq), nbytes, d, chunk, cnt);
+        HIP_OK(hipGetLastError());
+    }
+    unsigned long long hc[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (counts) {
+        counts[0] = hc[0];
+        counts[1] = hc[1];
+    }
+    return BBT_OK;
+}
+
+bbt_status bbt_filter_store(const bbt_filterset *fs, uint32_t idx, const char *prefix,
+                            double desired_fpr, uint64_t expected_entries,
+                            uint64_t num_entries, uint64_t redundant,
+                            const char *seq_sources) {
+    if (!fs || idx >= fs->desc.size() || !prefix)
+        return fail(BBT_ERR_ARG, "bbt_filter_store: bad argument");
+    const bbt_filter_info &inf = fs->info[idx];
+    std::vector<uint8_t> bits(inf.m / 8);
+    HIP_OK(hipMemcpy(bits.data(), fs->dbits[idx], bits.size(), hipMemcpyDeviceToHost));
+    BfHeader hdr;
+    std::memset(&hdr, 0, sizeof(hdr));
+    hdr.size = inf.m;
+    hdr.nhash = inf.h;
+    hdr.kmer = inf.k;
+    hdr.dFPR = desired_fpr;
+    hdr.nEntry = num_entries;
+    hdr.tEntry = expected_entries;
+    std::string err;
+    std::string p = prefix;
+    if (!write_bf_file(p + ".bf", hdr, bits.data(), bits.size(), err))
+        return fail(BBT_ERR_IO, err);
+    InfoFile ini;
+    ini.filter_id = fs->ids[idx];
+    ini.kmer_size = inf.k;
+    ini.desired_fpr = desired_fpr;
+    ini.hash_num = inf.h;
+    ini.expected_entries = expected_entries;
+    ini.sequence_sources = seq_sources ? seq_sources : "";
+    ini.size = inf.m;
+    ini.num_entries = num_entries;
+    ini.approx_fpr = calc_approx_fpr(inf.m, num_entries, inf.h);
+    ini.redundant_sequences = redundant;
+    ini.redundant_fpr = 0;
+    if (!write_info_file(p + ".txt", ini, err))
+        return fail(BBT_ERR_IO, err);
+    return BBT_OK;
+}
+
+uint64_t bbt_calc_optimal_size(uint64_t entries, double fpr, uint32_t h) {
+    return calc_optimal_size(entries, fpr, h);
+}
+
+uint32_t bbt_calc_optimal_hashnum(double fpr) { return calc_optimal_hashnum(fpr); }
+
+uint64_t bbt_calc_min_count(uint64_t frame_len, double bf_fpr, double min_fpr) {
+    return calc_min_count(frame_len, bf_fpr, min_fpr);
+}
+
+double bbt_binomial_score(uint64_t count, uint64_t frame_len, double bf_fpr) {
+    /* evalScore BINOMIAL: log10(P(X > count)) * -10 (SeqEval.h:545,558) */
+    return std::log10(binom_sf(int64_t(count), frame_len, bf_fpr)) * -10;
+}
+
+bbt_status bbt_summary_text(const bbt_filterset *fs, const uint64_t *summary,
+                            uint64_t nrec, char *buf, uint64_t cap, uint64_t *needed) {
+    if (!fs || !summary)
+        return fail(BBT_ERR_ARG, "bbt_summary_text: bad argument");
+    std::string s = summary_text(fs->ids, summary, nrec);
+    if (needed)
+        *needed = s.size();
+    if (buf && cap) {
+        uint64_t n = std::min<uint64_t>(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return BBT_OK;
+}
+
+} // extern "C"
+
+/* ======================================================================= */
+/* classification dispatch                                                  */
+
+namespace {
+
+struct Launch {
+    EvalArgs a;
+    hipStream_t st;
+    uint32_t path;
+    int cu;
+};
+
+template <int SC, bool EXH, int K>
+bbt_status launch_walk(const Launch &L) {
+    if (L.a.nitems == 0)
+        return BBT_OK;
+    if (L.path == BBT_PATH_PLAIN) {
+        const uint64_t blocks = (L.a.nitems + kBlock - 1) / kBlock;
+        if (blocks > 0x7FFFFFFFull)
+            return fail(BBT_ERR_ARG, "batch too large for one launch");
+        k_walk_plain<SC, EXH, K><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a);
+    } else {
+        int per_cu = 0;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_walk_persistent<SC, EXH, K>, kBlock, 0));
+        per_cu = std::max(1, per_cu);
+        uint64_t blocks = uint64_t(L.cu) * uint64_t(per_cu);
+        const uint64_t need = (L.a.nitems + kBlock - 1) / kBlock;
+        blocks = std::max<uint64_t>(1, std::min(blocks, need));
+        k_walk_persistent<SC, EXH, K><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a);
+    }
+    HIP_OK(hipGetLastError());
+    return BBT_OK;
+}
+
+template <int SC, bool EXH>
+bbt_status launch_k(const Launch &L, uint32_t k) {
+    return k == 25 ? launch_walk<SC, EXH, 25>(L) : launch_walk<SC, EXH, 0>(L);
+}
+
+bbt_status launch_scorer(const Launch &L, uint32_t scorer, bool exh, uint32_t k) {
+    switch (scorer) {
+    case BBT_SIMPLE:
+        return exh ? launch_k<SIMPLE, true>(L, k) : launch_k<SIMPLE, false>(L, k);
+    case BBT_HARMONIC:
+        return exh ? launch_k<HARMONIC, true>(L, k) : launch_k<HARMONIC, false>(L, k);
+    case BBT_LENGTH:
+        return exh ? launch_k<LENGTH, true>(L, k) : launch_k<LENGTH, false>(L, k);
+    case BBT_BINOMIAL:
+        return exh ? launch_k<BINOMIAL, true>(L, k) : launch_k<BINOMIAL, false>(L, k);
+    }
+    return fail(BBT_ERR_ARG, "unknown scorer");
+}
+
+template <int SC, int K>
+bbt_status launch_ordered_k(const Launch &L, uint64_t nrec, uint32_t paired,
+                            uint32_t inclusive) {
+    const uint64_t blocks = (nrec + kBlock - 1) / kBlock;
+    if (blocks)
+        k_ordered<SC, K><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a, nrec, paired,
+                                                                      inclusive);
+    HIP_OK(hipGetLastError());
+    return BBT_OK;
+}
+
+template <int SC>
+bbt_status launch_ordered_sc(const Launch &L, uint64_t nrec, uint32_t paired,
+                             uint32_t inclusive, uint32_t k) {
+    return k == 25 ? launch_ordered_k<SC, 25>(L, nrec, paired, inclusive)
+                   : launch_ordered_k<SC, 0>(L, nrec, paired, inclusive);
+}
+
+bbt_status launch_ordered(const Launch &L, uint32_t scorer, uint64_t nrec, uint32_t paired,
+                          uint32_t inclusive, uint32_t k) {
+    switch (scorer) {
+    case BBT_SIMPLE:
+        return launch_ordered_sc<SIMPLE>(L, nrec, paired, inclusive, k);
+    case BBT_HARMONIC:
+        return launch_ordered_sc<HARMONIC>(L, nrec, paired, inclusive, k);
+    case BBT_LENGTH:
+        return launch_ordered_sc<LENGTH>(L, nrec, paired, inclusive, k);
+    case BBT_BINOMIAL:
+        return launch_ordered_sc<BINOMIAL>(L, nrec, paired, inclusive, k);
+    }
+    return fail(BBT_ERR_ARG, "unknown scorer");
+}
+
+/* calcMinCount tables for every frame length up to max_frames */
+bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) {
+    if (!fs->dbinom.empty() && fs->binom_min_fpr == min_fpr &&
+        fs->binom_frames >= max_frames)
+        return BBT_OK;
+    for (uint32_t *p : fs->dbinom)
+        if (p)
+            (void)hipFree(p);
+    fs->dbinom.clear();
+    for (size_t f = 0; f < fs->desc.size(); ++f) {
+        std::vector<uint32_t> t(size_t(max_frames) + 1, 1u);
+        for (uint32_t fl = 1; fl <= max_frames; ++fl)
+            t[fl] = uint32_t(calc_min_count(fl, fs->info[f].fpr, min_fpr));
+        uint32_t *d = nullptr;
+        HIP_OK(hipMalloc(&d, t.size() * sizeof(uint32_t)));
+        HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        fs->dbinom.push_back(d);
+    }
+    fs->binom_min_fpr = min_fpr;
+    fs->binom_frames = max_frames;
+    fs->dirty = true;
+    return BBT_OK;
+}
+
+} // namespace
+
+extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
+                                     const bbt_eval_params *p, bbt_eval_out *out,
+                                     void *stream) {
+    if (!fs || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: null argument");
+    const uint32_t nf = uint32_t(fs->desc.size());
+    if (in->nreads == 0)
+        return BBT_OK;
+    if (!in->seq || !in->off)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: reads not given");
+    if (reinterpret_cast<uintptr_t>(in->seq) % 8 != 0 || in->seq_bytes % 8 != 0)
+        return fail(BBT_ERR_ARG,
+                    "bbt_eval_batch: seq must be 8-byte aligned with a size multiple of 8");
+    if (p->paired && (in->nreads & 1))
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: paired batch needs an even read count");
+    if (p->mode > BBT_SCORES || p->scorer > BBT_BINOMIAL || p->path > BBT_PATH_BATCHED)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: bad mode, scorer or path");
+    if (!out->hits)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: hits output is required");
+    const bool need_scores = p->mode == BBT_BESTHIT || p->mode == BBT_SCORES;
+    if (need_scores && !out->scores)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: scores output required for this mode");
+    if (need_scores && p->scorer == BBT_BINOMIAL)
+        return fail(BBT_ERR_UNSUPPORTED,
+                    "binomial evalScore (best-hit / with-score modes) is host-only");
+    uint32_t common_k = fs->desc[0].k;
+    for (const FilterDev &d : fs->desc)
+        if (d.k != common_k)
+            common_k = 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(fs->mu);
+    if (p->scorer == BBT_BINOMIAL) {
+        uint32_t kmin = 64;
+        for (const FilterDev &d : fs->desc)
+            kmin = std::min(kmin, d.k);
+        const uint32_t frames = in->max_len >= kmin ? in->max_len - kmin + 1 : 0;
+        bbt_status s = ensure_binom(fs, p->threshold, frames);
+        if (s != BBT_OK)
+            return s;
+    }
+    bbt_status s = sync_desc(fs);
+    if (s != BBT_OK)
+        return s;
+
+    Launch L;
+    std::memset(&L, 0, sizeof(L));
+    L.st = st;
+    L.path = p->path;
+    L.cu = fs->cu_count;
+    EvalArgs &a = L.a;
+    a.seq = reinterpret_cast<const uint8_t *>(in->seq);
+    a.off = in->off;
+    a.order = in->order;
+    a.filters = fs->ddesc;
+    a.hits = out->hits;
+    a.scores = out->scores;
+    a.counters = reinterpret_cast<unsigned long long *>(out->counters);
+    a.nf = nf;
+    a.refill_min = 8;
+    a.wp.threshold = p->threshold;
+    a.wp.streak = p->streak;
+    a.wp.min_len = uint32_t(std::round(p->threshold));
+    a.nitems = in->nreads * nf;
+
+    const uint64_t nrec = p->paired ? in->nreads / 2 : in->nreads;
+    const uint32_t kk = common_k == 25 ? 25u : 0u;
+    const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
+    /* work-queue counters of the (up to two) walks of this call */
+    k_zero_u64<<<1, 64, 0, st>>>(fs->dscratch + 4, 2);
+    a.queue = fs->dscratch + 4;
+    if (timed) {
+        if (!fs->ev0)
+            HIP_OK(hipEventCreate(&fs->ev0));
+        if (!fs->ev1)
+            HIP_OK(hipEventCreate(&fs->ev1));
+        if (p->mode == BBT_ORDERED)
+            k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
+        HIP_OK(hipEventRecord(fs->ev0, st));
+    }
+    switch (p->mode) {
+    case BBT_ORDERED: {
+        if (!timed)
+            k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
+        s = launch_ordered(L, p->scorer, nrec, p->paired, p->inclusive, kk);
+        break;
+    }
+    case BBT_BESTHIT:
+        s = launch_scorer(L, p->scorer, true, kk);
+        break;
+    case BBT_SCORES:
+        s = launch_scorer(L, p->scorer, false, kk);
+        a.queue = fs->dscratch + 5;
+        if (s == BBT_OK)
+            s = launch_scorer(L, p->scorer, true, kk);
+        break;
+    default:
+        s = launch_scorer(L, p->scorer, false, kk);
+        break;
+    }
+    if (s != BBT_OK)
+        return s;
+    if (timed)
+        HIP_OK(hipEventRecord(fs->ev1, st));
+    if (out->cls || out->summary || p->mode == BBT_BESTHIT) {
+        const uint64_t blocks = (nrec + kBlock - 1) / kBlock;
+        const size_t shm = out->summary ? (2 * size_t(nf) + 2) * sizeof(unsigned int) : 0;
+        if (shm > 64 * 1024)
+            return fail(BBT_ERR_UNSUPPORTED, "too many filters for device summary counters");
+        k_classify<<<dim3(uint32_t(blocks)), kBlock, shm, st>>>(
+            out->hits, out->scores, nf, nrec, p->paired, p->inclusive,
+            p->mode == BBT_BESTHIT, out->cls,
+            reinterpret_cast<unsigned long long *>(out->summary));
+        HIP_OK(hipGetLastError());
+    }
+    if (timed) {
+        HIP_OK(hipEventSynchronize(fs->ev1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, fs->ev0, fs->ev1));
+        fs->last_walk_ms = ms;
+    }
+    return BBT_OK;
+}
+
+extern "C" double bbt_last_walk_ms(const bbt_filterset *fs) {
+    return fs ? fs->last_walk_ms : -1.0;
+}

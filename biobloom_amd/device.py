@@ -1,0 +1,115 @@
+"""Device-side batches for libbbt_hip: reads staged in HBM, filters built on
+the device, classification outputs.  torch provides device memory and
+streams only; every computation runs in libbbt_hip's kernels."""
+import numpy as np
+import torch
+
+from . import lib as B
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+class DeviceReads:
+    """Reads packed back to back in device memory (bbt_reads)."""
+
+    def __init__(self, buf, off, device="cuda", order=None):
+        off = np.ascontiguousarray(off, np.uint64)
+        total = int(off[-1])
+        nbytes = ((total + 7) // 8) * 8 + 8
+        self.seq = torch.zeros(nbytes, dtype=torch.uint8, device=device)
+        if total:
+            self.seq[:total].copy_(torch.from_numpy(np.ascontiguousarray(buf[:total])))
+        self.off = torch.from_numpy(off.view(np.int64)).to(device)
+        self.nreads = len(off) - 1
+        lens = np.diff(off)
+        self.max_len = int(lens.max()) if len(lens) else 0
+        self.lens = lens
+        self.order = None
+        if order is not None:
+            self.order = torch.from_numpy(np.ascontiguousarray(order, np.int32)).to(device)
+
+    @classmethod
+    def from_matrix(cls, reads2d, device="cuda"):
+        n, L = reads2d.shape
+        off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+        return cls(reads2d.reshape(-1), off, device)
+
+    def struct(self):
+        return B.Reads(self.seq.data_ptr(), self.seq.numel(), self.off.data_ptr(),
+                       self.nreads, self.max_len, 0,
+                       self.order.data_ptr() if self.order is not None else None)
+
+
+def length_order(lens):
+    """Processing order of reads: longest first (LPT), stable."""
+    return np.argsort(-np.asarray(lens, np.int64), kind="stable").astype(np.int32)
+
+
+def build_filters(genomes, k, fpr, ids=None, device="cuda", stream=None):
+    """biobloommaker on the device: one filter per genome (list of uint8
+    arrays, records separated by a non-ACGT byte), sized as the reference's
+    BloomFilterInfo (expected entries = sum of len - k + 1)."""
+    L = B.load()
+    h = L.bbt_calc_optimal_hashnum(fpr)
+    ms = []
+    for g in genomes:
+        n = max(1, len(g) - k + 1)
+        ms.append(L.bbt_calc_optimal_size(n, fpr, h))
+    fs = B.FilterSet.create(ms, [h] * len(ms), [k] * len(ms))
+    st = _stream_handle(stream)
+    counts = []
+    for i, g in enumerate(genomes):
+        dg = torch.from_numpy(np.ascontiguousarray(g)).to(device)
+        counts.append(fs.insert(i, dg.data_ptr(), dg.numel(), st))
+        fs.refresh(i, st)
+        fs.set_id(i, ids[i] if ids else f"g{i}")
+        del dg
+    return fs, counts
+
+
+class Outputs:
+    def __init__(self, nreads, nf, paired, want_scores, device="cuda"):
+        nrec = nreads // 2 if paired else nreads
+        self.hits = torch.zeros(max(1, nreads * nf), dtype=torch.uint8, device=device)
+        self.scores = (torch.zeros(max(1, nreads * nf), dtype=torch.float64, device=device)
+                       if want_scores else None)
+        self.cls = torch.zeros(max(1, nrec), dtype=torch.int32, device=device)
+        self.summary = torch.zeros(2 * nf + 2, dtype=torch.int64, device=device)
+        self.counters = torch.zeros(4, dtype=torch.int64, device=device)
+        self.nreads, self.nf, self.nrec = nreads, nf, nrec
+
+    def struct(self):
+        return B.EvalOut(self.hits.data_ptr(),
+                         self.scores.data_ptr() if self.scores is not None else None,
+                         self.cls.data_ptr(), self.summary.data_ptr(),
+                         self.counters.data_ptr())
+
+    def zero_counts(self):
+        self.summary.zero_()
+        self.counters.zero_()
+
+    def host(self):
+        return {
+            "hits": self.hits[: self.nreads * self.nf].cpu().numpy().reshape(self.nreads, self.nf),
+            "scores": (None if self.scores is None else
+                       self.scores[: self.nreads * self.nf].cpu().numpy().reshape(self.nreads, self.nf)),
+            "cls": self.cls[: self.nrec].cpu().numpy().astype(np.uint32),
+            "summary": self.summary.cpu().numpy().astype(np.uint64),
+            "counters": self.counters.cpu().numpy().astype(np.uint64),
+        }
+
+
+def classify(fs, dreads, params, want_scores=None, stream=None, out=None):
+    """Runs one bbt_eval_batch and returns host copies of every output."""
+    if want_scores is None:
+        want_scores = params.mode in (B.BESTHIT, B.SCORES)
+    if out is None:
+        out = Outputs(dreads.nreads, len(fs), bool(params.paired), want_scores,
+                      dreads.seq.device)
+    out.zero_counts()
+    fs.eval_batch(dreads.struct(), params, out.struct(), _stream_handle(stream))
+    torch.cuda.synchronize()
+    return out.host()

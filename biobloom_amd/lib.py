@@ -1,0 +1,200 @@
+"""ctypes binding of libbbt_hip (include/bbt_hip.h).
+
+The library is required: importing a classifier without the built library
+raises, there is no host fallback.  Device buffers are passed as integer
+device pointers (e.g. torch tensors' data_ptr()) and streams as integer
+hipStream_t handles.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libbbt_hip.so")
+
+SIMPLE, LENGTH, HARMONIC, BINOMIAL = 0, 1, 2, 3
+STD, ORDERED, BESTHIT, SCORES = 0, 1, 2, 3
+PATH_PLAIN, PATH_BATCHED = 0, 1
+FLAG_TIME_WALK = 1
+
+
+class BbtError(RuntimeError):
+    pass
+
+
+class FilterInfo(C.Structure):
+    _fields_ = [("m", C.c_uint64), ("h", C.c_uint32), ("k", C.c_uint32),
+                ("fpr", C.c_double), ("popcount", C.c_uint64),
+                ("desired_fpr", C.c_double)]
+
+
+class EvalParams(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("scorer", C.c_uint32), ("threshold", C.c_double),
+                ("streak", C.c_uint32), ("paired", C.c_uint32),
+                ("inclusive", C.c_uint32), ("path", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class Reads(C.Structure):
+    _fields_ = [("seq", C.c_void_p), ("seq_bytes", C.c_uint64), ("off", C.c_void_p),
+                ("nreads", C.c_uint64), ("max_len", C.c_uint32),
+                ("reserved", C.c_uint32), ("order", C.c_void_p)]
+
+
+class EvalOut(C.Structure):
+    _fields_ = [("hits", C.c_void_p), ("scores", C.c_void_p), ("cls", C.c_void_p),
+                ("summary", C.c_void_p), ("counters", C.c_void_p)]
+
+
+# exported symbols and their signatures: (restype, argtypes)
+_SIGS = {
+    "bbt_last_error": (C.c_char_p, []),
+    "bbt_filters_load": (C.c_int32, [C.POINTER(C.c_char_p), C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
+    "bbt_filters_create": (C.c_int32, [C.c_uint32, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_void_p)]),
+    "bbt_filters_free": (C.c_int32, [C.c_void_p]),
+    "bbt_filterset_size": (C.c_uint32, [C.c_void_p]),
+    "bbt_filter_info_get": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(FilterInfo)]),
+    "bbt_filter_id": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "bbt_filter_set_id": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_char_p]),
+    "bbt_filter_device_bits": (C.c_void_p, [C.c_void_p, C.c_uint32]),
+    "bbt_filter_upload": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "bbt_filter_download": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "bbt_filter_insert": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint64), C.c_void_p]),
+    "bbt_filter_refresh": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "bbt_filter_store": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_double,
+                                     C.c_uint64, C.c_uint64, C.c_uint64, C.c_char_p]),
+    "bbt_calc_optimal_size": (C.c_uint64, [C.c_uint64, C.c_double, C.c_uint32]),
+    "bbt_calc_optimal_hashnum": (C.c_uint32, [C.c_double]),
+    "bbt_eval_batch": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
+                                   C.POINTER(EvalOut), C.c_void_p]),
+    "bbt_last_walk_ms": (C.c_double, [C.c_void_p]),
+    "bbt_summary_text": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64,
+                                     C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "bbt_calc_min_count": (C.c_uint64, [C.c_uint64, C.c_double, C.c_double]),
+    "bbt_binomial_score": (C.c_double, [C.c_uint64, C.c_uint64, C.c_double]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libbbt_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BbtError(f"{LIB_PATH} is missing: build it with "
+                           "`python -m biobloom_amd.build` (no host fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        msg = load().bbt_last_error()
+        raise BbtError(f"bbt status {status}: {msg.decode() if msg else ''}")
+
+
+class FilterSet:
+    """A set of Bloom filters resident in device memory (bbt_filterset)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def load(cls, bf_paths):
+        L = load()
+        arr = (C.c_char_p * len(bf_paths))(*[p.encode() for p in bf_paths])
+        h = C.c_void_p()
+        check(L.bbt_filters_load(arr, len(bf_paths), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def create(cls, m, h, k):
+        L = load()
+        n = len(m)
+        h_ = C.c_void_p()
+        check(L.bbt_filters_create(n, (C.c_uint64 * n)(*m), (C.c_uint32 * n)(*h),
+                                   (C.c_uint32 * n)(*k), C.byref(h_)))
+        return cls(h_.value)
+
+    def close(self):
+        if self._h:
+            load().bbt_filters_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return load().bbt_filterset_size(self._h)
+
+    def info(self, i):
+        out = FilterInfo()
+        check(load().bbt_filter_info_get(self._h, i, C.byref(out)))
+        return out
+
+    def filter_id(self, i):
+        return load().bbt_filter_id(self._h, i).decode()
+
+    def set_id(self, i, name):
+        check(load().bbt_filter_set_id(self._h, i, name.encode()))
+
+    def device_bits(self, i):
+        return load().bbt_filter_device_bits(self._h, i)
+
+    def upload(self, i, host_bits):
+        check(load().bbt_filter_upload(self._h, i, host_bits.ctypes.data, host_bits.nbytes))
+
+    def download(self, i, out):
+        check(load().bbt_filter_download(self._h, i, out.ctypes.data, out.nbytes))
+        return out
+
+    def insert(self, i, dseq_ptr, nbytes, stream=0):
+        counts = (C.c_uint64 * 2)()
+        check(load().bbt_filter_insert(self._h, i, C.c_void_p(dseq_ptr), nbytes, counts,
+                                       C.c_void_p(stream)))
+        return int(counts[0]), int(counts[1])
+
+    def refresh(self, i, stream=0):
+        check(load().bbt_filter_refresh(self._h, i, C.c_void_p(stream)))
+
+    def store(self, i, prefix, desired_fpr, expected, num_entries, redundant, sources):
+        check(load().bbt_filter_store(self._h, i, prefix.encode(), desired_fpr, expected,
+                                      num_entries, redundant, sources.encode()))
+
+    def eval_batch(self, reads, params, out, stream=0):
+        check(load().bbt_eval_batch(self._h, C.byref(reads), C.byref(params),
+                                    C.byref(out), C.c_void_p(stream)))
+
+    def last_walk_ms(self):
+        return load().bbt_last_walk_ms(self._h)
+
+    def summary_text(self, summary_counts, nrec):
+        L = load()
+        arr = (C.c_uint64 * len(summary_counts))(*[int(x) for x in summary_counts])
+        need = C.c_uint64()
+        check(L.bbt_summary_text(self._h, arr, nrec, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value + 1)
+        check(L.bbt_summary_text(self._h, arr, nrec, buf, need.value + 1, None))
+        return buf.value.decode()
+
+
+def params(mode=STD, scorer=SIMPLE, threshold=0.15, streak=3, paired=False,
+           inclusive=False, path=PATH_BATCHED, flags=0):
+    return EvalParams(mode, scorer, threshold, streak, int(paired), int(inclusive), path,
+                      flags, 0)

@@ -1,0 +1,169 @@
+/*
+ * bbt_hip.h -- C ABI of the MI355X read-classification library (libbbt_hip).
+ *
+ * Plain C types and pointers only.  Device buffers are HIP device pointers;
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ * Every entry point returns a bbt_status and never exits or throws; the
+ * message of the last failure on the calling thread is bbt_last_error().
+ *
+ * Reference interfaces each entry point replaces (BioBloom Tools 2.3.4):
+ *   bbt_filters_load   BioBloomClassifier::loadFilters
+ *                      (BioBloomCategorizer/BioBloomClassifier.cpp:950-971):
+ *                      <x>.txt via BloomFilterInfo(Common/BloomFilterInfo.cpp:31-53)
+ *                      and new BloomFilter(<x>.bf) per path, in order
+ *   bbt_eval_batch     per read, BioBloomClassifier::evaluateRead /
+ *                      evaluateReadPair (BioBloomClassifier.h:531-571), which
+ *                      call SeqEval::evalRead / evalScore (Common/SeqEval.h:
+ *                      493-564) per filter, followed by
+ *                      ResultsManager::updateSummaryData
+ *                      (BioBloomCategorizer/ResultsManager.hpp:66-189)
+ *   bbt_summary_text   ResultsManager::getResultsSummary (ResultsManager.hpp:
+ *                      191-237), the text of <prefix>_summary.tsv
+ *   bbt_filter_insert  BloomFilterGenerator::loadFilter
+ *                      (BioBloomMaker/BloomFilterGenerator.h:143-188)
+ *   bbt_filter_store   BloomFilter::storeFilter + BloomFilterInfo::printInfoFile
+ *                      (BloomFilterGenerator.cpp:71, BloomFilterInfo.cpp:81-116)
+ *   bbt_calc_optimal_size / bbt_calc_optimal_hashnum
+ *                      BloomFilterInfo::calcOptimalSize / calcOptimalHashNum
+ *                      (Common/BloomFilterInfo.h:57-76)
+ */
+#ifndef BBT_HIP_H
+#define BBT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bbt_status;
+#define BBT_OK 0
+#define BBT_ERR_ARG 1         /* invalid argument */
+#define BBT_ERR_IO 2          /* file cannot be opened / read / written */
+#define BBT_ERR_FORMAT 3      /* malformed .bf / .txt */
+#define BBT_ERR_HIP 4         /* HIP runtime error */
+#define BBT_ERR_NOMEM 5       /* device allocation failed */
+#define BBT_ERR_UNSUPPORTED 6 /* combination not available on the device path */
+
+/* scorer ids: opt::ScoringMethod (Common/Options.h:35) */
+#define BBT_SIMPLE 0
+#define BBT_LENGTH 1
+#define BBT_HARMONIC 2
+#define BBT_BINOMIAL 3
+/* mode ids: opt::FilteringMode (BioBloomCategorizer/Options.h:118-120) */
+#define BBT_STD 0
+#define BBT_ORDERED 1
+#define BBT_BESTHIT 2
+#define BBT_SCORES 3
+/* execution paths */
+#define BBT_PATH_PLAIN 0   /* one kernel launch per batch, one lane per walk */
+#define BBT_PATH_BATCHED 1 /* persistent work-queue kernel, graph-replayed */
+
+/* flags */
+#define BBT_FLAG_TIME_WALK 1u /* time the walk kernel(s) with HIP events on the
+                                 stream; the call then waits for them and
+                                 bbt_last_walk_ms() returns the duration */
+
+typedef struct bbt_filterset bbt_filterset;
+
+typedef struct {
+    uint64_t m;         /* filter bits */
+    uint32_t h;         /* hash functions */
+    uint32_t k;         /* k-mer size */
+    double fpr;         /* (popcount / m)^h  (getFPRPrecompute) */
+    uint64_t popcount;  /* set bits */
+    double desired_fpr; /* .txt desired_false_positve_rate (0 if unknown) */
+} bbt_filter_info;
+
+typedef struct {
+    uint32_t mode;      /* BBT_STD .. BBT_SCORES */
+    uint32_t scorer;    /* BBT_SIMPLE .. BBT_BINOMIAL */
+    double threshold;   /* opt::score after the categorizer's interpretation
+                           (BioBloomCategorizer.cpp:416-441) */
+    uint32_t streak;    /* opt::streakThreshold (-r), default 3 */
+    uint32_t paired;    /* reads 2j and 2j+1 are mates; one record per pair */
+    uint32_t inclusive; /* -i: pair hit if either mate hits */
+    uint32_t path;      /* BBT_PATH_PLAIN or BBT_PATH_BATCHED */
+    uint32_t flags;     /* BBT_FLAG_* */
+    uint32_t reserved;
+} bbt_eval_params;
+
+typedef struct {
+    const char *seq;       /* device: bases of all reads back to back (ASCII) */
+    uint64_t seq_bytes;    /* readable bytes at seq; must be >= off[nreads]
+                              rounded up to a multiple of 8 */
+    const uint64_t *off;   /* device: nreads + 1 offsets into seq */
+    uint64_t nreads;
+    uint32_t max_len;      /* upper bound on read length */
+    uint32_t reserved;
+    const uint32_t *order; /* device, optional: processing order of reads
+                              (a permutation of 0..nreads-1), or NULL */
+} bbt_reads;
+
+typedef struct {
+    uint8_t *hits;      /* device: nreads * nf bytes, 1 = filter in hit list */
+    double *scores;     /* device: nreads * nf evalScore values, or NULL;
+                           required for BBT_BESTHIT and BBT_SCORES */
+    uint32_t *cls;      /* device: per record class index (filter id,
+                           nf = noMatch, nf+1 = multiMatch), or NULL */
+    uint64_t *summary;  /* device: 2*nf + 2 counters ADDED TO: hits per filter,
+                           unique hits per filter, multiMatch, noMatch; or NULL */
+    uint64_t *counters; /* device: 4 counters ADDED TO: contains() calls, bit
+                           lookups, reserved, reserved; or NULL */
+} bbt_eval_out;
+
+const char *bbt_last_error(void);
+
+/* ---- filters ------------------------------------------------------------ */
+bbt_status bbt_filters_load(const char *const *bf_paths, uint32_t n,
+                            bbt_filterset **out);
+bbt_status bbt_filters_create(uint32_t n, const uint64_t *m, const uint32_t *h,
+                              const uint32_t *k, bbt_filterset **out);
+bbt_status bbt_filters_free(bbt_filterset *fs);
+uint32_t bbt_filterset_size(const bbt_filterset *fs);
+bbt_status bbt_filter_info_get(const bbt_filterset *fs, uint32_t idx,
+                               bbt_filter_info *out);
+const char *bbt_filter_id(const bbt_filterset *fs, uint32_t idx);
+bbt_status bbt_filter_set_id(bbt_filterset *fs, uint32_t idx, const char *id);
+/* device bit array of filter idx (read-only use by callers) */
+const void *bbt_filter_device_bits(const bbt_filterset *fs, uint32_t idx);
+bbt_status bbt_filter_upload(bbt_filterset *fs, uint32_t idx, const void *host_bits,
+                             uint64_t nbytes);
+bbt_status bbt_filter_download(const bbt_filterset *fs, uint32_t idx, void *host_bits,
+                               uint64_t nbytes);
+/* Insert every k-mer of the device sequence buffer dseq[0..nbytes) into filter
+ * idx on the device; records are separated by any non-ACGT byte.  counts
+ * (host, optional) receives {k-mers newly set, k-mers already present}. */
+bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
+                             uint64_t nbytes, uint64_t *counts, void *stream);
+/* Recomputes popcount and the precomputed FPR after uploads / inserts. */
+bbt_status bbt_filter_refresh(bbt_filterset *fs, uint32_t idx, void *stream);
+/* Writes <path_prefix>.bf and <path_prefix>.txt. */
+bbt_status bbt_filter_store(const bbt_filterset *fs, uint32_t idx,
+                            const char *path_prefix, double desired_fpr,
+                            uint64_t expected_entries, uint64_t num_entries,
+                            uint64_t redundant, const char *seq_sources);
+uint64_t bbt_calc_optimal_size(uint64_t entries, double fpr, uint32_t h);
+uint32_t bbt_calc_optimal_hashnum(double fpr);
+
+/* ---- classification ----------------------------------------------------- */
+bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
+                          const bbt_eval_params *p, bbt_eval_out *out,
+                          void *stream);
+/* Duration of the walk kernel(s) of the last bbt_eval_batch on fs that was
+ * called with BBT_FLAG_TIME_WALK, in milliseconds (-1 if none). */
+double bbt_last_walk_ms(const bbt_filterset *fs);
+/* summary.tsv text from the host copy of bbt_eval_out.summary counters;
+ * writes at most cap bytes (NUL-terminated), *needed = full length. */
+bbt_status bbt_summary_text(const bbt_filterset *fs, const uint64_t *summary,
+                            uint64_t nrec, char *buf, uint64_t cap,
+                            uint64_t *needed);
+/* Binomial helpers used by the BINOMIAL scorer (SeqEval.h:199-216). */
+uint64_t bbt_calc_min_count(uint64_t frame_len, double bf_fpr, double min_fpr);
+double bbt_binomial_score(uint64_t count, uint64_t frame_len, double bf_fpr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""GPU parity: libbbt_hip through its C ABI against the CPU oracle, on the
+same seeded inputs.  Bit-exact for hits, classes, summary counters, scores
+and the contains()/lookup counters."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from biobloom_amd import device as D
+    from biobloom_amd import lib as B
+    B.load()
+    return D, B
+
+
+def oracle_filters_like(fs, B):
+    out = []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        f = O.Filter(inf.m, inf.h, inf.k)
+        fs.download(i, f.bits)
+        out.append(f)
+    return out
+
+
+def small_genomes(rng, n, glen, records=3):
+    gs = []
+    for _ in range(n):
+        parts = [rng.choice(np.frombuffer(b"ACGT", np.uint8), size=glen // records)
+                 for _ in range(records)]
+        sep = np.frombuffer(b"\n", np.uint8)
+        g = parts[0]
+        for p in parts[1:]:
+            g = np.concatenate([g, sep, p])
+        gs.append(g)
+    return gs
+
+
+@pytest.mark.parametrize("k", [25, 11, 32])
+def test_device_filter_build_matches_oracle(dev, k):
+    D, B = dev
+    print(f"[gpu] filter build k={k}", flush=True)
+    rng = np.random.default_rng(k)
+    genomes = small_genomes(rng, 3, 40_000)
+    genomes[1][rng.integers(0, len(genomes[1]), 200)] = ord("N")
+    fs, counts = D.build_filters(genomes, k, 0.075)
+    for i, g in enumerate(genomes):
+        inf = fs.info(i)
+        of = O.Filter(inf.m, inf.h, inf.k)
+        new, red = of.insert_seq(bytes(g))
+        got = np.zeros_like(of.bits)
+        fs.download(i, got)
+        assert np.array_equal(got, of.bits)
+        assert sum(counts[i]) == new + red
+        assert inf.popcount == of.popcount()
+        assert inf.fpr == of.fpr
+
+
+PATHS = [0, 1]
+
+
+def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
+                scorer=0, thr=0.15, streak=3, order=None):
+    dr = D.DeviceReads(buf, off, order=order)
+    p_or = O.params(scorer, thr, streak)
+    want_scores = mode in (O.BESTHIT, O.SCORES)
+    oh, osc, ocls, octr = O.classify(ofs, buf, off, paired, inclusive, mode, p_or,
+                                     want_scores=want_scores)
+    for path in PATHS:
+        print(f"[gpu] mode={mode} scorer={scorer} thr={thr} streak={streak} "
+              f"paired={paired} incl={inclusive} path={path} n={len(off)-1}", flush=True)
+        p = B.params(mode, scorer, thr, streak, paired, inclusive, path)
+        res = D.classify(fs, dr, p)
+        assert np.array_equal(res["hits"], oh), "hits differ"
+        assert np.array_equal(res["cls"], ocls), "classes differ"
+        if want_scores:
+            assert np.array_equal(res["scores"].view(np.uint64), osc.view(np.uint64))
+        assert int(res["counters"][0]) == int(octr[0]), "contains() calls differ"
+        assert int(res["counters"][1]) == int(octr[1]), "bit lookups differ"
+        ids = [fs.filter_id(i) for i in range(len(fs))]
+        nrec = len(ocls)
+        txt = fs.summary_text(res["summary"], nrec)
+        want = O.summary(ids, oh, paired, inclusive)
+        assert txt == want
+    return oh
+
+
+@pytest.fixture(scope="module")
+def multi(dev):
+    D, B = dev
+    from biobloom_amd import synth
+    w = synth.workload_multi(scale=0.004, nfilters=4, genome_len=300_000)
+    fs, _ = D.build_filters(w["genomes"], 25, 0.075)
+    ofs = oracle_filters_like(fs, B)
+    reads = w["reads"]
+    # edge cases appended as an extra pair block: empty, short, N-only, lowercase
+    extra = [b"", b"ACGT", b"N" * 200, bytes(reads[0]).lower(), b"A" * 24, b"C" * 25,
+             bytes(reads[1][:60]) + b"NNNNN" + bytes(reads[1][60:]), b"ACGTN" * 40]
+    buf, off = O.pack_reads([bytes(r) for r in reads] + extra)
+    return fs, ofs, buf, off
+
+
+def test_std_single_end_all_scorers(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.2), (O.LENGTH, 40.0),
+                        (O.BINOMIAL, 1e-10), (O.SIMPLE, 0.0), (O.SIMPLE, 0.9)]:
+        check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
+
+
+def test_std_streak_variants(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    for streak in [0, 1, 5]:
+        check_batch(D, B, fs, ofs, buf, off, streak=streak)
+
+
+def test_paired_end(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    for inc in [False, True]:
+        check_batch(D, B, fs, ofs, buf, off, paired=True, inclusive=inc)
+        check_batch(D, B, fs, ofs, buf, off, paired=True, inclusive=inc, mode=O.ORDERED)
+
+
+def test_modes(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    check_batch(D, B, fs, ofs, buf, off, mode=O.ORDERED)
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+        check_batch(D, B, fs, ofs, buf, off, mode=O.BESTHIT, scorer=scorer, thr=thr)
+        check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=scorer, thr=thr)
+
+
+def test_length_order_does_not_change_results(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    order = D.length_order(np.diff(off))
+    check_batch(D, B, fs, ofs, buf, off, order=order)
+
+
+def test_uneven_lengths(dev):
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(3)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 200_000)) for _ in range(2)]
+    longs, _ = synth.long_reads(rng, genomes, 60, median=2000, hi=20_000)
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 400, 150, 0.01, 0.7)
+    items = longs + [s for s in shorts]
+    buf, off = O.pack_reads([bytes(x) for x in items])
+    fs, _ = D.build_filters(genomes, 25, 0.05)
+    ofs = oracle_filters_like(fs, B)
+    check_batch(D, B, fs, ofs, buf, off)
+    check_batch(D, B, fs, ofs, buf, off, order=D.length_order(np.diff(off)))
+
+
+def test_store_and_load_roundtrip(dev, tmp_path):
+    D, B = dev
+    rng = np.random.default_rng(5)
+    genomes = small_genomes(rng, 2, 30_000)
+    fs, counts = D.build_filters(genomes, 25, 0.075, ids=["alpha", "beta"])
+    paths = []
+    for i in range(2):
+        prefix = str(tmp_path / f"f{i}")
+        inf = fs.info(i)
+        fs.store(i, prefix, 0.075, len(genomes[i]) - 24, counts[i][0], counts[i][1],
+                 f"g{i}.fa")
+        paths.append(prefix + ".bf")
+    fs2 = B.FilterSet.load(paths)
+    assert [fs2.filter_id(i) for i in range(2)] == ["alpha", "beta"]
+    for i in range(2):
+        a = np.zeros(fs.info(i).m // 8, np.uint8)
+        b = np.zeros_like(a)
+        fs.download(i, a)
+        fs2.download(i, b)
+        assert np.array_equal(a, b)
+        assert fs2.info(i).fpr == fs.info(i).fpr

@@ -1,0 +1,138 @@
+"""The device walk (biobloom_amd/csrc/bbt_device.h) compiled for the CPU and
+checked against the oracle: decisions, scores (bit-exact doubles), contains()
+calls and bit lookups, for every scorer, plus the exact 64-bit modulo."""
+import ctypes as C
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+NATIVE = os.path.join(HERE, "native")
+LIB = os.path.join(NATIVE, "libwalker_host.so")
+
+
+def build():
+    src = os.path.join(NATIVE, "walker_host.cpp")
+    if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(
+            os.path.getmtime(src),
+            os.path.getmtime(os.path.join(ROOT, "biobloom_amd", "csrc", "bbt_device.h"))):
+        return
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "biobloom_amd", "csrc"), "-o", LIB, src],
+                   check=True)
+
+
+@pytest.fixture(scope="module")
+def W():
+    build()
+    L = C.CDLL(LIB)
+    p = C.c_void_p
+    L.walker_host_eval.restype = C.c_int
+    L.walker_host_eval.argtypes = [p, C.c_uint64, C.c_uint32, C.c_uint32, p, C.c_uint32,
+                                   p, p, C.c_uint64, C.c_uint32, C.c_int, C.c_double,
+                                   C.c_uint32, p, p, p]
+    L.walker_host_fastmod.argtypes = [C.c_uint64, p, C.c_uint64, p]
+    return L
+
+
+def padded(buf):
+    n = (len(buf) + 15) // 8 * 8
+    out = np.zeros(n, np.uint8)
+    out[: len(buf)] = buf
+    return out
+
+
+def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None):
+    buf, off = O.pack_reads(reads)
+    buf = padded(buf)
+    n = len(reads)
+    hits = np.zeros(n, np.uint8)
+    scores = np.zeros(n, np.float64)
+    ctr = np.zeros(2, np.uint64)
+    b = binom if binom is not None else np.zeros(1, np.uint32)
+    rc = W.walker_host_eval(f.bits.ctypes.data, f.m, f.h, f.k, b.ctypes.data,
+                            len(b) if binom is not None else 0, buf.ctypes.data,
+                            off.ctypes.data, n, scorer, int(exh), thr, streak,
+                            hits.ctypes.data, scores.ctypes.data, ctr.ctypes.data)
+    assert rc == 0
+    return hits, scores, ctr
+
+
+def test_fastmod_exact(W):
+    rng = random.Random(0)
+    ds = [1, 2, 3, 7, 64, 127, 128, 1000, 27387648, 109550784, 624697664,
+          16_430_000_000, (1 << 33) + 64, (1 << 40) - 64, (1 << 63) - 1, (1 << 63),
+          (1 << 64) - 1]
+    ns = [0, 1, 2, (1 << 64) - 1, (1 << 63), (1 << 32) - 1, 1 << 32]
+    ns += [rng.getrandbits(64) for _ in range(2000)]
+    arr = np.array(ns, np.uint64)
+    for d in ds + [rng.getrandbits(rng.randint(1, 64)) | 1 for _ in range(200)]:
+        out = np.zeros_like(arr)
+        W.walker_host_fastmod(d, arr.ctypes.data, len(arr), out.ctypes.data)
+        assert [int(x) for x in out] == [n % d for n in ns], d
+
+
+def make_filter_and_reads(rng, k, h, m, nreads=150):
+    genome = "".join(rng.choice("ACGT") for _ in range(3000))
+    f = O.Filter(m, h, k)
+    f.insert_seq(genome)
+    reads = []
+    for _ in range(nreads):
+        L = rng.choice([0, 1, k - 1, k, k + 1, 60, 150, 151, 400])
+        if rng.random() < 0.5 and L < 2900:
+            a = rng.randrange(0, 3000 - L)
+            r = list(genome[a:a + L])
+            for i in range(L):
+                x = rng.random()
+                if x < 0.02:
+                    r[i] = rng.choice("ACGT")
+                elif x < 0.03:
+                    r[i] = "N"
+            s = "".join(r)
+        else:
+            s = "".join(rng.choice("ACGTNacgt") if rng.random() < 0.05 else
+                        rng.choice("ACGT") for _ in range(L))
+        reads.append(s)
+    return f, reads
+
+
+@pytest.mark.parametrize("k,h,m", [(25, 3, 1 << 16), (25, 4, 1 << 14), (11, 2, 4096),
+                                   (5, 1, 1 << 12), (31, 3, 1 << 18), (32, 5, 1 << 17)])
+def test_walker_matches_oracle(W, k, h, m):
+    rng = random.Random(k * 1000 + h)
+    f, reads = make_filter_and_reads(rng, k, h, m)
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.SIMPLE, 0.0), (O.SIMPLE, 0.5),
+                        (O.HARMONIC, 0.15), (O.LENGTH, 30.0), (O.BINOMIAL, 1e-5)]:
+        for streak in [3, 0, 1]:
+            p = O.params(scorer, thr, streak)
+            binom = None
+            if scorer == O.BINOMIAL:
+                binom = np.array([1] + [O.lib().orc_calc_min_count(fl, f.fpr, thr)
+                                        for fl in range(1, 401)], np.uint32)
+            hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, streak, binom)
+            want, probes, lookups = [], 0, 0
+            for r in reads:
+                ok, pr, lk = O.eval_read(f, r, p)
+                want.append(int(ok))
+                probes += pr
+                lookups += lk
+            assert hits.tolist() == want
+            assert (int(ctr[0]), int(ctr[1])) == (probes, lookups)
+            if scorer == O.BINOMIAL:
+                continue
+            _, scores, ctr = run_walker(W, f, reads, scorer, True, thr, streak)
+            ws, probes = [], 0
+            for r in reads:
+                sc, pr, _ = O.eval_score(f, r, p)
+                ws.append(sc)
+                probes += pr
+            got = scores.view(np.uint64).tolist()
+            exp = np.array(ws, np.float64).view(np.uint64).tolist()
+            assert got == exp
+            assert int(ctr[0]) == probes
