@@ -1,0 +1,236 @@
+"""Benchmark of the MI355X read-classification path.
+
+One step = one bbt_eval_batch over the whole batch of reads already resident
+in HBM: every (read, filter) SeqEval walk, the ResultsManager classes and the
+summary counters.  Default workload ("multi"): 10 filters from 20 Mbp
+genomes (k=25, FPR 0.075, h=3), 2M read pairs 2x150 bp, paired-end
+intersection mode, default SIMPLE scorer (-s 0.15, -r 3).
+
+Prints one JSON line: reads/s of the batched persistent path (value), the
+plain per-batch path beside it, k-mer probes/s, the roofline of the walk
+kernel and the CPU oracle baseline on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "k-mer probes/s + reads/s on 1 MI355X (batched vs plain GPU); p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+LINE_BYTES = 64  # one random 64 B line per bit lookup
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name, scale):
+    from biobloom_amd import synth
+    t = time.time()
+    if name == "multi":
+        w = synth.workload_multi(scale=scale)
+        reads = w["reads"]
+        buf = np.ascontiguousarray(reads).reshape(-1)
+        off = np.arange(reads.shape[0] + 1, dtype=np.uint64) * np.uint64(reads.shape[1])
+    elif name == "single":
+        w = synth.workload_single(scale=scale)
+        reads = w["reads"]
+        buf = np.ascontiguousarray(reads).reshape(-1)
+        off = np.arange(reads.shape[0] + 1, dtype=np.uint64) * np.uint64(reads.shape[1])
+    elif name == "uneven":
+        w = synth.workload_uneven(scale=scale)
+        buf, off = synth.pack(w["reads_list"])
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    log(f"workload {name}: {len(off) - 1} reads, {int(off[-1])} bases, "
+        f"{len(w['genomes'])} genomes ({time.time() - t:.1f}s)")
+    return w, buf, off
+
+
+def time_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline(fs, buf, off, paired, sample_records, threads, B, gpu_hits):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    ofs = []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        f = O.Filter(inf.m, inf.h, inf.k)
+        fs.download(i, f.bits)
+        ofs.append(f)
+    nreads = sample_records * (2 if paired else 1)
+    sbuf = buf[: int(off[nreads])]
+    soff = off[: nreads + 1]
+    O.classify(ofs, sbuf, soff, paired=paired, nthreads=threads)  # warm
+    t = time.time()
+    hits, _, _, _ = O.classify(ofs, sbuf, soff, paired=paired, nthreads=threads)
+    dt = time.time() - t
+    parity = bool(np.array_equal(hits, gpu_hits[:nreads]))
+    return nreads / dt, dt, parity
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_walk_{workload}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="multi", choices=["multi", "single", "uneven"])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-sample", type=int, default=100_000,
+                    help="records (pairs for paired workloads) in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", default="both", choices=["both", "batched", "plain"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(reads shard across GPUs with replicated filters, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from biobloom_amd import device as D
+    from biobloom_amd import lib as B
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+    torch.cuda.set_device(0)
+    B.load()
+
+    w, buf, off = make_workload(args.workload, args.scale)
+    paired = bool(w["paired"])
+    t = time.time()
+    fs, _ = D.build_filters(w["genomes"], w["k"], w["fpr"])
+    torch.cuda.synchronize()
+    log(f"filters built on device ({time.time() - t:.2f}s): "
+        + ", ".join(f"m={fs.info(i).m} h={fs.info(i).h}" for i in range(min(2, len(fs)))))
+    order = D.length_order(np.diff(off)) if args.workload == "uneven" else None
+    dr = D.DeviceReads(buf, off, order=order)
+    nreads = dr.nreads
+    nrec = nreads // 2 if paired else nreads
+    out = D.Outputs(nreads, len(fs), paired, False)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    rs = dr.struct()
+    os_ = out.struct()
+
+    def step(path, flags=0):
+        out.zero_counts()
+        fs.eval_batch(rs, B.params(paired=paired, path=path, flags=flags), os_, stream)
+
+    results = {}
+    paths = {"batched": B.PATH_BATCHED, "plain": B.PATH_PLAIN}
+    for name, path in paths.items():
+        if args.path not in ("both", name):
+            continue
+        ms = time_steps(lambda: step(path), args.steps, args.warmup, torch)
+        walk = []
+        for _ in range(3):
+            step(path, B.FLAG_TIME_WALK)
+            walk.append(fs.last_walk_ms())
+        torch.cuda.synchronize()
+        host = out.host()
+        results[name] = {"ms": ms, "walk_ms": float(np.mean(walk)),
+                         "probes": int(host["counters"][0]),
+                         "lookups": int(host["counters"][1]), "hits": host["hits"],
+                         "summary": host["summary"]}
+        log(f"{name}: {ms:.3f} ms/step, walk kernel {np.mean(walk):.3f} ms, "
+            f"{nreads / ms * 1e3 / 1e6:.2f} M reads/s")
+    main_name = "batched" if "batched" in results else "plain"
+    R = results[main_name]
+    if len(results) == 2:
+        a, b = results["batched"], results["plain"]
+        assert np.array_equal(a["hits"], b["hits"]), "batched and plain paths disagree"
+        assert a["lookups"] == b["lookups"]
+    reads_per_s = nreads / (R["ms"] / 1e3)
+    probes_per_s = R["probes"] / (R["ms"] / 1e3)
+    alg_bytes = R["lookups"] * LINE_BYTES
+    achieved = alg_bytes / (R["walk_ms"] / 1e3) / 1e9
+    traffic = load_traffic(args.workload)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sample = min(args.cpu_sample, nrec)
+        v, dt, parity = cpu_baseline(fs, buf, off, paired, sample, threads, B, R["hits"])
+        cpu = {"value": v, "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"first {sample} {'pairs' if paired else 'reads'} of the "
+                         f"workload, all filters, {dt:.2f}s",
+               "parity_with_gpu": parity}
+        log(f"cpu oracle: {v / 1e6:.3f} M reads/s on {threads} threads, parity={parity}")
+
+    line = {
+        "metric": METRIC,
+        "value": reads_per_s,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": {"multi": "multi-filter batch: 10 filters x 20 Mbp, k=25 FPR 0.075, "
+                                  "2M pairs 2x150 bp, paired intersection, SIMPLE -s 0.15",
+                         "single": "single filter 5 Mbp k=25 FPR 0.075, 100k SE 150 bp",
+                         "uneven": "uneven lengths: 4 filters x 100 Mbp k=25 FPR 0.05, "
+                                   "200k log-normal long reads + 2M x 150 bp"}[args.workload],
+            "scale": args.scale,
+            "reads": nreads,
+            "filters": len(fs),
+            "path": main_name,
+        },
+        "probes_per_s": probes_per_s,
+        "pairs_per_s": reads_per_s / 2 if paired else None,
+        "probes_per_step": R["probes"],
+        "lookups_per_step": R["lookups"],
+        "plain_reads_per_s": (nreads / (results["plain"]["ms"] / 1e3)
+                              if "plain" in results else None),
+        "speedup_vs_plain": (results["plain"]["ms"] / results["batched"]["ms"]
+                             if len(results) == 2 else None),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_walk_persistent" if main_name == "batched" else "k_walk_plain",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "walk_ms": R["walk_ms"],
+            "algorithmic_bytes": alg_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,16 +38,13 @@ def build_lib():
 
 def build_cli():
     """The restated biobloomcategorizer / biobloommaker front ends."""
-    if not os.path.isdir(HOST):
-        return
-    common = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-              "-Wall", f"-Wl,-rpath,{PKG}", "-L", PKG, "-lbbt_hip", "-lz"]
     for name, out in [("biobloomcategorizer.cpp", CLI), ("biobloommaker.cpp", MAKER)]:
         src = os.path.join(HOST, name)
-        if os.path.exists(src):
-            _run([*common[:1], "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-                  "-Wall", "-o", out, src, os.path.join(HOST, "front_end.cpp"),
-                  f"-Wl,-rpath,{PKG}", "-L", PKG, "-lbbt_hip", "-lz"])
+        if not os.path.exists(src):
+            continue
+        _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-Wall",
+              "-o", out, src, os.path.join(HOST, "front_end.cpp"),
+              "-Wl,-rpath,$ORIGIN", "-L", PKG, "-lbbt_hip", "-lz"])
 
 
 def build():

@@ -692,7 +692,7 @@ bbt_status bbt_filter_store(const bbt_filterset *fs, uint32_t idx, const char *p
     ini.num_entries = num_entries;
     ini.approx_fpr = calc_approx_fpr(inf.m, num_entries, inf.h);
     ini.redundant_sequences = redundant;
-    ini.redundant_fpr = 0;
+    ini.redundant_fpr = calc_redundancy_fpr(inf.m, num_entries, inf.h);
     if (!write_info_file(p + ".txt", ini, err))
         return fail(BBT_ERR_IO, err);
     return BBT_OK;

@@ -191,6 +191,16 @@ double calc_approx_fpr(uint64_t size, uint64_t entries, unsigned h) {
                     double(h));
 }
 
+double calc_redundancy_fpr(uint64_t size, uint64_t entries, unsigned h) {
+    /* BloomFilterInfo::calcRedunancyFPR (Common/BloomFilterInfo.cpp:183-191) */
+    if (entries == 0)
+        return 0;
+    double total = std::log(calc_approx_fpr(size, 1, h));
+    for (uint64_t i = 2; i < entries; ++i)
+        total = std::log(std::exp(total) + calc_approx_fpr(size, i, h));
+    return std::exp(total) / double(entries);
+}
+
 /* ------------------------------------------------------------ binomial */
 /* Binomial probabilities by Loader's saddle-point method (C. Loader, "Fast
  * and accurate computation of binomial probabilities", 2000). */

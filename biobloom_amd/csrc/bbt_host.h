@@ -55,6 +55,7 @@ bool write_bf_file(const std::string &path, const BfHeader &hdr, const uint8_t *
 uint64_t calc_optimal_size(uint64_t entries, double fpr, unsigned h);
 unsigned calc_optimal_hashnum(double fpr);
 double calc_approx_fpr(uint64_t size, uint64_t entries, unsigned h);
+double calc_redundancy_fpr(uint64_t size, uint64_t entries, unsigned h);
 
 /* Binomial(n, p): P(X > x), and the smallest x with P(X > x) <= q. */
 double binom_sf(int64_t x, uint64_t n, double p);
