@@ -51,6 +51,10 @@ BBT_HD uint64_t rotl64(uint64_t v, unsigned s) {
     s &= 63;
     return s ? (v << s) | (v >> (64 - s)) : v;
 }
+BBT_HD uint64_t rotr64(uint64_t v, unsigned s) {
+    s &= 63;
+    return s ? (v >> s) | (v << (64 - s)) : v;
+}
 BBT_HD uint64_t rotl1(uint64_t v) { return (v << 1) | (v >> 63); }
 BBT_HD uint64_t rotr1(uint64_t v) { return (v >> 1) | (v << 63); }
 
@@ -68,6 +72,20 @@ BBT_HD uint64_t seed_fwd(unsigned code) {
 }
 /* seed of the complement base */
 BBT_HD uint64_t seed_rc(unsigned code) { return seed_fwd(code ^ 2u); }
+
+/*
+ * Prefix form of the window hashes, used where one wave hashes 64
+ * consecutive windows at once.  With position-rotated contributions
+ *   H_i = XOR_{j<=i} rotr(seed_fwd(c_j), j),  G_i = XOR_{j<=i} rotl(seed_rc(c_j), j)
+ * (rotations mod 64), the window [p, i] with i = p + k - 1 hashes to
+ *   fwd = rotl(H_i ^ H_{p-1}, i),  rc = rotr(G_i ^ G_{p-1}, p),
+ * equal to XOR_j rotl(seed_fwd(c_j), i - j) and XOR_j rotl(seed_rc(c_j), j - p)
+ * of the rolling form; both prefixes are plain XOR scans across lanes.
+ */
+BBT_HD uint64_t pfx_fwd(unsigned code, uint32_t i) { return rotr64(seed_fwd(code), i & 63); }
+BBT_HD uint64_t pfx_rc(unsigned code, uint32_t i) { return rotl64(seed_rc(code), i & 63); }
+BBT_HD uint64_t pfx_win_fwd(uint64_t Hi, uint64_t Hb, uint32_t i) { return rotl64(Hi ^ Hb, i & 63); }
+BBT_HD uint64_t pfx_win_rc(uint64_t Gi, uint64_t Gb, uint32_t p) { return rotr64(Gi ^ Gb, p & 63); }
 
 BBT_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -172,6 +190,8 @@ struct Walker {
     uint32_t streak_thr;
     /* instrumentation: contains() calls and bit lookups */
     uint32_t probes, lookups;
+    /* set when a BINOMIAL threshold table does not cover the read */
+    uint32_t bad;
 
     BBT_HD unsigned kk() const { return K ? unsigned(K) : k; }
 
@@ -203,6 +223,7 @@ struct Walker {
         dscore = 0.0;
         streak_thr = wp.streak;
         state = W_RUNNING;
+        bad = 0;
         const unsigned kv = kk();
         /* frame count as the size_t expression seqLen - k + 1 (SeqEval.h:28-35) */
         const uint64_t nframes = uint64_t(length) - uint64_t(kv) + 1;
@@ -224,7 +245,16 @@ struct Walker {
                 anti_thr = 0;
             } else {
                 const uint32_t fl = uint32_t(nframes);
-                thr2 = (!EXH && f->binom && fl < f->binom_len) ? f->binom[fl] : 1u;
+                if (EXH) {
+                    thr2 = 1u;
+                } else if (f->binom && fl < f->binom_len) {
+                    thr2 = f->binom[fl];
+                } else {
+                    /* no threshold for this length: report, do not guess */
+                    bad = 1;
+                    state = W_REJECT;
+                    thr2 = 1u;
+                }
                 anti_thr = fl - thr2; /* unsigned, as SeqEval.h:227 */
             }
         } else {
@@ -297,47 +327,11 @@ struct Walker {
             if (run < kv)
                 continue;
             p = i - kv; /* start of the N-free window ending at base i-1 */
-            if (SC == LENGTH) {
-                if (!EXH) {
-                    /* SeqEval.h:306: quit when the rest cannot reach minLen */
-                    if (uint64_t(len) - p + score2 < uint64_t(thr2)) {
-                        state = W_REJECT;
-                        return false;
-                    }
-                }
-                if (p != prev + 1)
-                    score2 = 0;
-            } else if (!EXH) {
-                if (skipping) {
-                    if (p < skipEnd) { /* SeqEval.h:103-108 */
-                        if (anti_thr <= ++anti) {
-                            state = W_REJECT;
-                            return false;
-                        }
-                        prev = p;
-                        continue;
-                    }
-                    skipping = 0;
-                }
-                if (!first && p != prev + 1) { /* SeqEval.h:71-77 */
-                    anti += p - prev - 1;
-                    if (anti_thr <= anti) {
-                        state = W_REJECT;
-                        return false;
-                    }
-                    streak = 0;
-                }
-            } else {
-                if (skipping) {
-                    if (p < skipEnd) {
-                        prev = p;
-                        continue;
-                    }
-                    skipping = 0;
-                }
-                if (p != prev + 1)
-                    streak = 0;
-            }
+            const int pre = window_pre(p);
+            if (pre == PRE_END)
+                return false;
+            if (pre == PRE_NOPROBE)
+                continue;
             /* contains(*itr) */
             h0 = rc < fwd ? rc : fwd;
             j = 0;
@@ -346,6 +340,58 @@ struct Walker {
             set_addr();
             return true;
         }
+    }
+
+    /* Window-level part of the walk, before contains(): returns PRE_PROBE
+     * when window p must be probed, PRE_NOPROBE when it is consumed without
+     * a probe (inside a jump-skip), PRE_END when the walk has ended. */
+    enum { PRE_PROBE = 0, PRE_NOPROBE = 1, PRE_END = 2 };
+    BBT_HD int window_pre(uint32_t pw) {
+        p = pw;
+        if (SC == LENGTH) {
+            if (!EXH) {
+                /* SeqEval.h:306: quit when the rest cannot reach minLen */
+                if (uint64_t(len) - pw + score2 < uint64_t(thr2)) {
+                    state = W_REJECT;
+                    return PRE_END;
+                }
+            }
+            if (pw != prev + 1)
+                score2 = 0;
+            return PRE_PROBE;
+        }
+        if (!EXH) {
+            if (skipping) {
+                if (pw < skipEnd) { /* SeqEval.h:103-108 */
+                    if (anti_thr <= ++anti) {
+                        state = W_REJECT;
+                        return PRE_END;
+                    }
+                    prev = pw;
+                    return PRE_NOPROBE;
+                }
+                skipping = 0;
+            }
+            if (!first && pw != prev + 1) { /* SeqEval.h:71-77 */
+                anti += pw - prev - 1;
+                if (anti_thr <= anti) {
+                    state = W_REJECT;
+                    return PRE_END;
+                }
+                streak = 0;
+            }
+            return PRE_PROBE;
+        }
+        if (skipping) {
+            if (pw < skipEnd) {
+                prev = pw;
+                return PRE_NOPROBE;
+            }
+            skipping = 0;
+        }
+        if (pw != prev + 1)
+            streak = 0;
+        return PRE_PROBE;
     }
 
     /* Hand back the byte loaded from addr. */

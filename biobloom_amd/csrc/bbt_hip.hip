@@ -55,6 +55,8 @@ static bbt_status fail(bbt_status code, const std::string &msg) {
 namespace {
 
 constexpr int kBlock = 256;
+/* default length from which the batched path walks a read with a whole wave */
+constexpr uint32_t kDefaultWaveMin = 1024;
 
 struct EvalArgs {
     const uint8_t *seq;
@@ -64,7 +66,11 @@ struct EvalArgs {
     uint8_t *hits;
     double *scores;
     unsigned long long *counters;
-    unsigned long long *queue;
+    unsigned long long *queue;  /* lane-item queue */
+    unsigned long long *lqueue; /* wave-item queue */
+    const uint32_t *long_list;  /* reads walked by a whole wave, longest first */
+    const uint32_t *short_list; /* reads walked one per lane */
+    const unsigned long long *list_counts; /* [0] long reads, [1] short reads */
     uint64_t nitems;
     uint32_t nf;
     uint32_t refill_min;
@@ -88,6 +94,16 @@ __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t p
     }
 }
 
+__device__ __forceinline__ void add_issued(unsigned long long *ctr, uint32_t issued) {
+    if (!ctr)
+        return;
+    unsigned long long a = issued;
+    for (int o = 32; o > 0; o >>= 1)
+        a += __shfl_xor(a, o);
+    if (lane_id() == 0)
+        atomicAdd(ctr + 2, a);
+}
+
 template <int SC, bool EXH, int K>
 __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uint32_t f,
                                            const Walker<SC, EXH, K> &w) {
@@ -96,6 +112,8 @@ __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uin
         a.scores[idx] = w.score_value();
     else
         a.hits[idx] = w.accepted() ? 1 : 0;
+    if (w.bad && a.counters)
+        atomicOr(a.counters + 3, 1ull);
 }
 
 template <int SC, bool EXH, int K>
@@ -104,7 +122,7 @@ __device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item,
                                            uint32_t &f) {
     const uint64_t r = item / a.nf;
     f = uint32_t(item - r * a.nf);
-    read = a.order ? a.order[r] : r;
+    read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
 }
@@ -126,6 +144,113 @@ __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
     add_counters(a.counters, w.probes, w.lookups);
 }
 
+/*
+ * One (read, filter) walk by a whole wavefront, 64 windows per step: the
+ * window hashes come from prefix-XOR scans across lanes (bbt_device.h,
+ * pfx_*), every window the walk may probe is probed at once (each lane
+ * stops at its first clear bit), and the walk is then replayed in window
+ * order over the chunk's valid / hit masks with the same window_pre /
+ * on_result steps as the lane walk.  The replay counts contains() calls and
+ * lookups exactly as the reference would make them; lookups of windows the
+ * walk skips or never reaches are counted separately as issued-only.
+ */
+template <int SC, bool EXH, int K>
+__device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
+                          Walker<SC, EXH, K> &w, uint32_t &issued) {
+    const uint32_t lane = lane_id();
+    const uint64_t b = a.off[read];
+    const uint32_t len = uint32_t(a.off[read + 1] - b);
+    const uint8_t *s = a.seq + b;
+    w.init(s, len, a.filters + f, a.wp);
+    w.probes = 0;
+    w.lookups = 0;
+    const unsigned kv = w.kk();
+    const FilterDev *F = a.filters + f;
+    uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
+    int lastN = -1;
+    for (uint32_t b0 = 0; b0 < len && w.state == W_RUNNING; b0 += 64) {
+        const uint32_t i = b0 + lane;
+        const unsigned c = i < len ? unsigned(s[i]) : 0u;
+        const bool ok = i < len && base_valid(c);
+        const unsigned code = base_code(c);
+        uint64_t H = ok ? pfx_fwd(code, i) : 0;
+        uint64_t G = ok ? pfx_rc(code, i) : 0;
+        int ln = ok ? -1 : int(i);
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t th = __shfl_up(H, unsigned(d));
+            const uint64_t tg = __shfl_up(G, unsigned(d));
+            const int tl = __shfl_up(ln, unsigned(d));
+            if (int(lane) >= d) {
+                H ^= th;
+                G ^= tg;
+                ln = ln > tl ? ln : tl;
+            }
+        }
+        H ^= Hc;
+        G ^= Gc;
+        ln = ln > lastN ? ln : lastN;
+        /* prefix just before the window: base i - k, here or in the previous chunk */
+        const int src = int(lane) - int(kv);
+        const uint64_t hb_cur = __shfl(H, src & 63), gb_cur = __shfl(G, src & 63);
+        const uint64_t hb_prev = __shfl(Hp, (src + 64) & 63), gb_prev = __shfl(Gp, (src + 64) & 63);
+        const uint64_t Hb = src >= 0 ? hb_cur : hb_prev;
+        const uint64_t Gb = src >= 0 ? gb_cur : gb_prev;
+        const int64_t pw = int64_t(i) - int64_t(kv) + 1;
+        const bool valid = pw >= 0 && i < len && int64_t(ln) < pw;
+        const uint64_t wf = pfx_win_fwd(H, Hb, i);
+        const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
+        const uint64_t h0 = wr < wf ? wr : wf;
+        Hc = __shfl(H, 63);
+        Gc = __shfl(G, 63);
+        lastN = __shfl(ln, 63);
+        Hp = H;
+        Gp = G;
+        const uint64_t vmask = __ballot(valid);
+        if (!vmask)
+            continue;
+        /* windows inside the current jump-skip are consumed without a probe */
+        bool need = valid && !(w.skipping && uint32_t(pw) < w.skipEnd);
+        bool allset = need;
+        uint32_t lk = 0;
+        for (uint32_t jj = 0; __any(need); ++jj) {
+            if (need) {
+                uint64_t hv = h0;
+                if (jj) {
+                    hv = h0 * F->mult[jj];
+                    hv ^= hv >> kMultiShift;
+                }
+                const uint64_t pos = fastmod(hv, w.m, w.magic, w.sh1, w.sh2);
+                const unsigned byte = w.bits[pos >> 3];
+                ++lk;
+                if (!(byte & (1u << unsigned(pos & 7)))) {
+                    allset = false;
+                    need = false;
+                } else if (jj + 1 >= w.h) {
+                    need = false;
+                }
+            }
+        }
+        issued += lk;
+        const uint64_t hmask = __ballot(allset);
+        for (uint64_t vm = vmask; vm; vm &= vm - 1) {
+            const int L = __ffsll((unsigned long long)vm) - 1;
+            const uint32_t p = b0 + uint32_t(L) - kv + 1;
+            const int pre = w.window_pre(p);
+            if (pre == w.PRE_END)
+                break;
+            if (pre == w.PRE_NOPROBE)
+                continue;
+            ++w.probes;
+            w.lookups += uint32_t(__shfl(int(lk), L));
+            w.on_result(((hmask >> L) & 1) != 0);
+            if (w.state != W_RUNNING)
+                break;
+        }
+    }
+    if (w.state == W_RUNNING)
+        w.finish_end();
+}
+
 template <int SC, bool EXH, int K>
 __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     const uint32_t lane = lane_id();
@@ -133,6 +258,33 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     Walker<SC, EXH, K> w;
     w.probes = 0;
     w.lookups = 0;
+    uint32_t issued = 0;
+    uint32_t wave_probes = 0, wave_lookups = 0; /* lane 0 only */
+    const uint64_t nlong = a.list_counts ? a.list_counts[0] * a.nf : 0;
+    const uint64_t nitems = a.list_counts ? a.list_counts[1] * a.nf : a.nitems;
+
+    /* phase 1: long reads, a whole wave per (read, filter), longest first */
+    while (nlong) {
+        unsigned long long it = 0;
+        if (lane == 0)
+            it = atomicAdd(a.lqueue, 1ull);
+        it = __shfl(it, 0);
+        if (it >= nlong)
+            break;
+        const uint64_t r = it / a.nf;
+        const uint32_t f = uint32_t(it - r * a.nf);
+        const uint64_t read = a.long_list[r];
+        wave_walk(a, read, f, w, issued);
+        if (lane == 0) {
+            store_walk(a, read, f, w);
+            wave_probes += w.probes;
+            wave_lookups += w.lookups;
+        }
+        w.probes = 0;
+        w.lookups = 0;
+    }
+
+    /* phase 2: one walk per lane, idle lanes refilled from the queue */
     bool active = false;
     bool drained = false;
     uint64_t read = 0;
@@ -146,11 +298,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             if (lane == leader)
                 base = atomicAdd(a.queue, (unsigned long long)nidle);
             base = __shfl(base, int(leader));
-            if (base + nidle >= a.nitems)
+            if (base + nidle >= nitems)
                 drained = true;
             if (!active) {
                 const uint64_t item = base + uint64_t(__popcll(idle & lt_mask));
-                if (item < a.nitems) {
+                if (item < nitems) {
                     start_walk(a, item, w, read, f);
                     active = true;
                 }
@@ -171,7 +323,82 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             }
         }
     }
-    add_counters(a.counters, w.probes, w.lookups);
+    issued += w.lookups;
+    add_counters(a.counters, w.probes + wave_probes, w.lookups + wave_lookups);
+    add_issued(a.counters, issued);
+}
+
+/* Length buckets for the batched path: reads of at least wave_min bases go
+ * to the wave list by floor(log2(len)), longest bucket first; the rest to
+ * the lane list.  cnt[0..31] bucket sizes, cnt[32] short reads. */
+__global__ __launch_bounds__(kBlock) void k_len_count(const uint64_t *off, uint64_t nreads,
+                                                      uint32_t wave_min,
+                                                      unsigned long long *cnt) {
+    __shared__ unsigned int h[33];
+    for (int t = threadIdx.x; t < 33; t += blockDim.x)
+        h[t] = 0;
+    __syncthreads();
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nreads;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t len = off[r + 1] - off[r];
+        if (len >= wave_min && len > 0)
+            atomicAdd(&h[min(31, 63 - __clzll((long long)len))], 1u);
+        else
+            atomicAdd(&h[32], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 33; t += blockDim.x)
+        if (h[t])
+            atomicAdd(cnt + t, (unsigned long long)h[t]);
+}
+
+__global__ void k_len_scan(unsigned long long *cnt, unsigned long long *base,
+                           unsigned long long *list_counts) {
+    if (threadIdx.x != 0)
+        return;
+    unsigned long long pos = 0;
+    for (int b = 31; b >= 0; --b) {
+        base[b] = pos;
+        pos += cnt[b];
+    }
+    list_counts[0] = pos;
+    list_counts[1] = cnt[32];
+    base[32] = 0;
+    for (int b = 0; b < 33; ++b)
+        cnt[b] = 0; /* reused as scatter cursors */
+}
+
+__global__ __launch_bounds__(kBlock) void k_len_scatter(const uint64_t *off, uint64_t nreads,
+                                                        uint32_t wave_min,
+                                                        const unsigned long long *base,
+                                                        unsigned long long *cursor,
+                                                        uint32_t *long_list,
+                                                        uint32_t *short_list) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t start = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (uint64_t r0 = start - lane; r0 < nreads; r0 += stride) {
+        const uint64_t r = r0 + lane;
+        const bool in = r < nreads;
+        const uint64_t len = in ? off[r + 1] - off[r] : 0;
+        const bool lng = in && len >= wave_min && len > 0;
+        const bool sht = in && !lng;
+        const uint64_t sm = __ballot(sht);
+        unsigned long long sb = 0;
+        if (sm) {
+            const uint32_t leader = uint32_t(__ffsll((unsigned long long)sm) - 1);
+            if (lane == leader)
+                sb = atomicAdd(cursor + 32, (unsigned long long)__popcll(sm));
+            sb = __shfl(sb, int(leader));
+        }
+        if (sht)
+            short_list[sb + __popcll(sm & lt_mask)] = uint32_t(r);
+        if (lng) {
+            const int bk = min(31, 63 - __clzll((long long)len));
+            long_list[base[bk] + atomicAdd(cursor + bk, 1ull)] = uint32_t(r);
+        }
+    }
 }
 
 template <int SC, int K>
@@ -384,6 +611,11 @@ struct bbt_filterset {
     /* work queue counter and per-call scratch */
     unsigned long long *dscratch = nullptr; /* [1..2] insert counts, [3] popcount, [4..5] walk queues */
     int cu_count = 0;
+    /* batched path: [0..1] lane queues, [2..3] wave queues, [4..5] list
+     * sizes, [8..40] bucket counts / cursors, [48..80] bucket bases */
+    unsigned long long *dctl = nullptr;
+    uint32_t *dlists = nullptr; /* 2 * lists_cap read indices */
+    uint64_t lists_cap = 0;
     /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_walk_ms = -1;
@@ -543,6 +775,10 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->ddesc);
     if (fs->dscratch)
         (void)hipFree(fs->dscratch);
+    if (fs->dctl)
+        (void)hipFree(fs->dctl);
+    if (fs->dlists)
+        (void)hipFree(fs->dlists);
     if (fs->ev0)
         (void)hipEventDestroy(fs->ev0);
     if (fs->ev1)
@@ -910,9 +1146,34 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     const uint64_t nrec = p->paired ? in->nreads / 2 : in->nreads;
     const uint32_t kk = common_k == 25 ? 25u : 0u;
     const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
-    /* work-queue counters of the (up to two) walks of this call */
-    k_zero_u64<<<1, 64, 0, st>>>(fs->dscratch + 4, 2);
-    a.queue = fs->dscratch + 4;
+    /* work queues of the (up to two) walks of this call, and for the batched
+     * path the length buckets: long reads walked by whole waves */
+    if (!fs->dctl)
+        HIP_OK(hipMalloc(&fs->dctl, 128 * sizeof(unsigned long long)));
+    k_zero_u64<<<1, 64, 0, st>>>(fs->dctl, 48);
+    a.queue = fs->dctl + 0;
+    a.lqueue = fs->dctl + 2;
+    const uint32_t wave_min = p->wave_min_len ? p->wave_min_len : kDefaultWaveMin;
+    if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED) {
+        if (fs->lists_cap < in->nreads) {
+            if (fs->dlists)
+                HIP_OK(hipFree(fs->dlists));
+            fs->dlists = nullptr;
+            fs->lists_cap = 0;
+            HIP_OK(hipMalloc(&fs->dlists, 2 * in->nreads * sizeof(uint32_t)));
+            fs->lists_cap = in->nreads;
+        }
+        const uint32_t gb = uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));
+        k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, wave_min, fs->dctl + 8);
+        k_len_scan<<<1, 64, 0, st>>>(fs->dctl + 8, fs->dctl + 48, fs->dctl + 4);
+        k_len_scatter<<<gb, kBlock, 0, st>>>(in->off, in->nreads, wave_min, fs->dctl + 48,
+                                              fs->dctl + 8, fs->dlists,
+                                              fs->dlists + fs->lists_cap);
+        HIP_OK(hipGetLastError());
+        a.long_list = fs->dlists;
+        a.short_list = fs->dlists + fs->lists_cap;
+        a.list_counts = fs->dctl + 4;
+    }
     if (timed) {
         if (!fs->ev0)
             HIP_OK(hipEventCreate(&fs->ev0));
@@ -934,7 +1195,8 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
         break;
     case BBT_SCORES:
         s = launch_scorer(L, p->scorer, false, kk);
-        a.queue = fs->dscratch + 5;
+        a.queue = fs->dctl + 1;
+        a.lqueue = fs->dctl + 3;
         if (s == BBT_OK)
             s = launch_scorer(L, p->scorer, true, kk);
         break;

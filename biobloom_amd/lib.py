@@ -31,7 +31,7 @@ class EvalParams(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("scorer", C.c_uint32), ("threshold", C.c_double),
                 ("streak", C.c_uint32), ("paired", C.c_uint32),
                 ("inclusive", C.c_uint32), ("path", C.c_uint32), ("flags", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("wave_min_len", C.c_uint32)]
 
 
 class Reads(C.Structure):
@@ -195,6 +195,6 @@ class FilterSet:
 
 
 def params(mode=STD, scorer=SIMPLE, threshold=0.15, streak=3, paired=False,
-           inclusive=False, path=PATH_BATCHED, flags=0):
+           inclusive=False, path=PATH_BATCHED, flags=0, wave_min_len=0):
     return EvalParams(mode, scorer, threshold, streak, int(paired), int(inclusive), path,
-                      flags, 0)
+                      flags, wave_min_len)

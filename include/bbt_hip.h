@@ -86,7 +86,9 @@ typedef struct {
     uint32_t inclusive; /* -i: pair hit if either mate hits */
     uint32_t path;      /* BBT_PATH_PLAIN or BBT_PATH_BATCHED */
     uint32_t flags;     /* BBT_FLAG_* */
-    uint32_t reserved;
+    uint32_t wave_min_len; /* batched path: reads at least this long are walked
+                              by a whole wavefront, 64 windows per step
+                              (0 = library default) */
 } bbt_eval_params;
 
 typedef struct {
@@ -109,8 +111,11 @@ typedef struct {
                            nf = noMatch, nf+1 = multiMatch), or NULL */
     uint64_t *summary;  /* device: 2*nf + 2 counters ADDED TO: hits per filter,
                            unique hits per filter, multiMatch, noMatch; or NULL */
-    uint64_t *counters; /* device: 4 counters ADDED TO: contains() calls, bit
-                           lookups, reserved, reserved; or NULL */
+    uint64_t *counters; /* device: 4 counters ADDED TO: contains() calls and bit
+                           lookups the reference walk makes, bit lookups
+                           issued (>= the former when whole waves probe ahead),
+                           error flag (1: a BINOMIAL read longer than
+                           max_len); or NULL */
 } bbt_eval_out;
 
 const char *bbt_last_error(void);

@@ -3,6 +3,7 @@
  * the tests can check its logic against the oracle without a GPU.  Test
  * infrastructure only.
  */
+#include <algorithm>
 #include <cstring>
 
 #include "bbt_device.h"
@@ -28,9 +29,145 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
         ctr[1] += w.lookups;
     }
 }
+/* The wave-cooperative walk of the long-read kernel, with the 64 lanes
+ * emulated by loops: prefix-XOR window hashes per chunk of 64 bases,
+ * speculative probing of every window the walk may probe, then the walk
+ * replayed over the chunk's valid / hit masks. */
+template <int SC, bool EXH>
+void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t nreads,
+              const WalkParams &wp, uint8_t *hits, double *scores, uint64_t *ctr) {
+    for (uint64_t r = 0; r < nreads; ++r) {
+        Walker<SC, EXH, 0> w;
+        const uint8_t *s = seq + off[r];
+        const uint32_t len = uint32_t(off[r + 1] - off[r]);
+        w.init(s, len, &f, wp);
+        w.probes = 0;
+        w.lookups = 0;
+        const unsigned kv = f.k;
+        uint64_t Hc = 0, Gc = 0, Hp[64] = {0}, Gp[64] = {0};
+        int64_t lastN = -1;
+        for (uint32_t b0 = 0; b0 < len && w.state == W_RUNNING; b0 += 64) {
+            uint64_t H[64], G[64], h0[64];
+            int64_t ln[64];
+            bool valid[64];
+            uint32_t lk[64];
+            bool allset[64];
+            for (int L = 0; L < 64; ++L) {
+                const uint32_t i = b0 + L;
+                const unsigned c = i < len ? s[i] : 0;
+                const bool ok = i < len && base_valid(c);
+                H[L] = (ok ? pfx_fwd(base_code(c), i) : 0) ^ (L ? H[L - 1] : Hc);
+                G[L] = (ok ? pfx_rc(base_code(c), i) : 0) ^ (L ? G[L - 1] : Gc);
+                const int64_t me = ok ? -1 : int64_t(i);
+                ln[L] = std::max(me, L ? ln[L - 1] : lastN);
+            }
+            for (int L = 0; L < 64; ++L) {
+                const uint32_t i = b0 + L;
+                const int src = L - int(kv);
+                const uint64_t Hb = src >= 0 ? H[src] : Hp[src + 64];
+                const uint64_t Gb = src >= 0 ? G[src] : Gp[src + 64];
+                const int64_t pw = int64_t(i) - kv + 1;
+                valid[L] = pw >= 0 && i < len && ln[L] < pw;
+                const uint64_t wf = pfx_win_fwd(H[L], Hb, i);
+                const uint64_t wr = pfx_win_rc(G[L], Gb, uint32_t(pw));
+                h0[L] = wr < wf ? wr : wf;
+                bool need = valid[L] && !(w.skipping && uint32_t(pw) < w.skipEnd);
+                allset[L] = need;
+                lk[L] = 0;
+                for (uint32_t j = 0; need; ++j) {
+                    uint64_t hv = h0[L];
+                    if (j) {
+                        hv = h0[L] * f.mult[j];
+                        hv ^= hv >> kMultiShift;
+                    }
+                    const uint64_t pos = fastmod(hv, f.m, f.magic, f.sh1, f.sh2);
+                    ++lk[L];
+                    if (!(f.bits[pos >> 3] & (1u << (pos & 7)))) {
+                        allset[L] = false;
+                        need = false;
+                    } else if (j + 1 >= f.h) {
+                        need = false;
+                    }
+                }
+                ctr[2] += lk[L];
+            }
+            Hc = H[63];
+            Gc = G[63];
+            lastN = ln[63];
+            for (int L = 0; L < 64; ++L) {
+                Hp[L] = H[L];
+                Gp[L] = G[L];
+            }
+            for (int L = 0; L < 64; ++L) {
+                if (!valid[L])
+                    continue;
+                const uint32_t p = b0 + L - kv + 1;
+                const int pre = w.window_pre(p);
+                if (pre == w.PRE_END)
+                    break;
+                if (pre == w.PRE_NOPROBE)
+                    continue;
+                ++w.probes;
+                w.lookups += lk[L];
+                w.on_result(allset[L]);
+                if (w.state != W_RUNNING)
+                    break;
+            }
+        }
+        if (w.state == W_RUNNING)
+            w.finish_end();
+        if (EXH)
+            scores[r] = w.score_value();
+        else
+            hits[r] = w.accepted() ? 1 : 0;
+        ctr[0] += w.probes;
+        ctr[1] += w.lookups;
+    }
+}
 } // namespace
 
 extern "C" {
+
+int walker_host_wave_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
+                          const uint32_t *binom, uint32_t binom_len, const uint8_t *seq,
+                          const uint64_t *off, uint64_t nreads, uint32_t scorer, int exh,
+                          double threshold, uint32_t streak, uint8_t *hits, double *scores,
+                          uint64_t *ctr) {
+    FilterDev f;
+    std::memset(&f, 0, sizeof(f));
+    f.bits = bits;
+    f.m = m;
+    FastMod fm = fastmod_make(m);
+    f.magic = fm.magic;
+    f.sh1 = fm.sh1;
+    f.sh2 = fm.sh2;
+    f.h = h;
+    f.k = k;
+    f.binom = binom;
+    f.binom_len = binom_len;
+    for (uint32_t j = 1; j < kMaxHash; ++j)
+        f.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
+    WalkParams wp;
+    wp.threshold = threshold;
+    wp.streak = streak;
+    wp.min_len = uint32_t(threshold + 0.5);
+#define BBT_WAVE_CASE(S)                                                              \
+    case S:                                                                           \
+        exh ? run_wave<S, true>(f, seq, off, nreads, wp, hits, scores, ctr)           \
+            : run_wave<S, false>(f, seq, off, nreads, wp, hits, scores, ctr);         \
+        break;
+    switch (scorer) {
+        BBT_WAVE_CASE(SIMPLE)
+        BBT_WAVE_CASE(HARMONIC)
+        BBT_WAVE_CASE(LENGTH)
+        BBT_WAVE_CASE(BINOMIAL)
+    default:
+        return 1;
+    }
+#undef BBT_WAVE_CASE
+    return 0;
+}
+
 
 /* One filter, all reads; binom[] (frame length -> threshold) optional. */
 int walker_host_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,

@@ -68,7 +68,7 @@ PATHS = [0, 1]
 
 
 def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
-                scorer=0, thr=0.15, streak=3, order=None):
+                scorer=0, thr=0.15, streak=3, order=None, wave_min=0):
     dr = D.DeviceReads(buf, off, order=order)
     p_or = O.params(scorer, thr, streak)
     want_scores = mode in (O.BESTHIT, O.SCORES)
@@ -76,8 +76,10 @@ def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
                                      want_scores=want_scores)
     for path in PATHS:
         print(f"[gpu] mode={mode} scorer={scorer} thr={thr} streak={streak} "
-              f"paired={paired} incl={inclusive} path={path} n={len(off)-1}", flush=True)
-        p = B.params(mode, scorer, thr, streak, paired, inclusive, path)
+              f"paired={paired} incl={inclusive} path={path} wave_min={wave_min} "
+              f"n={len(off)-1}", flush=True)
+        p = B.params(mode, scorer, thr, streak, paired, inclusive, path,
+                     wave_min_len=wave_min)
         res = D.classify(fs, dr, p)
         assert np.array_equal(res["hits"], oh), "hits differ"
         assert np.array_equal(res["cls"], ocls), "classes differ"
@@ -85,6 +87,8 @@ def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
             assert np.array_equal(res["scores"].view(np.uint64), osc.view(np.uint64))
         assert int(res["counters"][0]) == int(octr[0]), "contains() calls differ"
         assert int(res["counters"][1]) == int(octr[1]), "bit lookups differ"
+        assert int(res["counters"][2]) >= int(octr[1]), "issued lookups below reference"
+        assert int(res["counters"][3]) == 0, "device error flag set"
         ids = [fs.filter_id(i) for i in range(len(fs))]
         nrec = len(ocls)
         txt = fs.summary_text(res["summary"], nrec)
@@ -140,6 +144,21 @@ def test_modes(dev, multi):
         check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=scorer, thr=thr)
 
 
+def test_whole_wave_walks_all_scorers(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.2), (O.LENGTH, 40.0),
+                        (O.BINOMIAL, 1e-10)]:
+        check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr, wave_min=26)
+    check_batch(D, B, fs, ofs, buf, off, streak=0, wave_min=26)
+    check_batch(D, B, fs, ofs, buf, off, paired=True, wave_min=26)
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+        check_batch(D, B, fs, ofs, buf, off, mode=O.BESTHIT, scorer=scorer, thr=thr,
+                    wave_min=26)
+        check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=scorer, thr=thr,
+                    wave_min=100)
+
+
 def test_length_order_does_not_change_results(dev, multi):
     D, B = dev
     fs, ofs, buf, off = multi
@@ -160,6 +179,8 @@ def test_uneven_lengths(dev):
     ofs = oracle_filters_like(fs, B)
     check_batch(D, B, fs, ofs, buf, off)
     check_batch(D, B, fs, ofs, buf, off, order=D.length_order(np.diff(off)))
+    for scorer, thr in [(O.HARMONIC, 0.15), (O.LENGTH, 60.0), (O.BINOMIAL, 1e-10)]:
+        check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
 
 
 def test_store_and_load_roundtrip(dev, tmp_path):

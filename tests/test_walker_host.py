@@ -37,6 +37,8 @@ def W():
     L.walker_host_eval.argtypes = [p, C.c_uint64, C.c_uint32, C.c_uint32, p, C.c_uint32,
                                    p, p, C.c_uint64, C.c_uint32, C.c_int, C.c_double,
                                    C.c_uint32, p, p, p]
+    L.walker_host_wave_eval.restype = C.c_int
+    L.walker_host_wave_eval.argtypes = L.walker_host_eval.argtypes
     L.walker_host_fastmod.argtypes = [C.c_uint64, p, C.c_uint64, p]
     return L
 
@@ -48,15 +50,16 @@ def padded(buf):
     return out
 
 
-def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None):
+def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None, wave=False):
     buf, off = O.pack_reads(reads)
     buf = padded(buf)
     n = len(reads)
     hits = np.zeros(n, np.uint8)
     scores = np.zeros(n, np.float64)
-    ctr = np.zeros(2, np.uint64)
+    ctr = np.zeros(3, np.uint64)
     b = binom if binom is not None else np.zeros(1, np.uint32)
-    rc = W.walker_host_eval(f.bits.ctypes.data, f.m, f.h, f.k, b.ctypes.data,
+    fn = W.walker_host_wave_eval if wave else W.walker_host_eval
+    rc = fn(f.bits.ctypes.data, f.m, f.h, f.k, b.ctypes.data,
                             len(b) if binom is not None else 0, buf.ctypes.data,
                             off.ctypes.data, n, scorer, int(exh), thr, streak,
                             hits.ctypes.data, scores.ctypes.data, ctr.ctypes.data)
@@ -82,6 +85,7 @@ def make_filter_and_reads(rng, k, h, m, nreads=150):
     genome = "".join(rng.choice("ACGT") for _ in range(3000))
     f = O.Filter(m, h, k)
     f.insert_seq(genome)
+    f.genome = genome
     reads = []
     for _ in range(nreads):
         L = rng.choice([0, 1, k - 1, k, k + 1, 60, 150, 151, 400])
@@ -102,20 +106,32 @@ def make_filter_and_reads(rng, k, h, m, nreads=150):
     return f, reads
 
 
+@pytest.mark.parametrize("wave", [False, True])
 @pytest.mark.parametrize("k,h,m", [(25, 3, 1 << 16), (25, 4, 1 << 14), (11, 2, 4096),
                                    (5, 1, 1 << 12), (31, 3, 1 << 18), (32, 5, 1 << 17)])
-def test_walker_matches_oracle(W, k, h, m):
+def test_walker_matches_oracle(W, k, h, m, wave):
     rng = random.Random(k * 1000 + h)
     f, reads = make_filter_and_reads(rng, k, h, m)
+    if wave:
+        long = []
+        for _ in range(12):
+            a = rng.randrange(0, 1500)
+            seg = list(f.genome[a:a + rng.randint(300, 1400)])
+            for i in range(len(seg)):
+                if rng.random() < 0.05:
+                    seg[i] = rng.choice("ACGTN")
+            long.append("".join(seg))
+        reads = reads + long
     for scorer, thr in [(O.SIMPLE, 0.15), (O.SIMPLE, 0.0), (O.SIMPLE, 0.5),
                         (O.HARMONIC, 0.15), (O.LENGTH, 30.0), (O.BINOMIAL, 1e-5)]:
         for streak in [3, 0, 1]:
             p = O.params(scorer, thr, streak)
             binom = None
             if scorer == O.BINOMIAL:
+                top = max(len(r) for r in reads)
                 binom = np.array([1] + [O.lib().orc_calc_min_count(fl, f.fpr, thr)
-                                        for fl in range(1, 401)], np.uint32)
-            hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, streak, binom)
+                                        for fl in range(1, top + 1)], np.uint32)
+            hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, streak, binom, wave)
             want, probes, lookups = [], 0, 0
             for r in reads:
                 ok, pr, lk = O.eval_read(f, r, p)
@@ -124,9 +140,11 @@ def test_walker_matches_oracle(W, k, h, m):
                 lookups += lk
             assert hits.tolist() == want
             assert (int(ctr[0]), int(ctr[1])) == (probes, lookups)
+            if wave:
+                assert int(ctr[2]) >= lookups  # speculative probes only add
             if scorer == O.BINOMIAL:
                 continue
-            _, scores, ctr = run_walker(W, f, reads, scorer, True, thr, streak)
+            _, scores, ctr = run_walker(W, f, reads, scorer, True, thr, streak, None, wave)
             ws, probes = [], 0
             for r in reads:
                 sc, pr, _ = O.eval_score(f, r, p)
