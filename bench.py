@@ -107,6 +107,15 @@ def main():
                     help="records (pairs for paired workloads) in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--path", default="both", choices=["both", "batched", "plain"])
+    ap.add_argument("--wave-min", type=int, default=0,
+                    help="batched path: reads this long are walked by a whole wave")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the batched path's scheduling knobs on this batch first")
+    ap.add_argument("--no-gather", action="store_true",
+                    help="skip the random-gather reference measurement")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -121,6 +130,15 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "read_bytes": n, "write_bytes": n}))
+        return
     B.load()
 
     w, buf, off = make_workload(args.workload, args.scale)
@@ -130,8 +148,9 @@ def main():
     torch.cuda.synchronize()
     log(f"filters built on device ({time.time() - t:.2f}s): "
         + ", ".join(f"m={fs.info(i).m} h={fs.info(i).h}" for i in range(min(2, len(fs)))))
-    order = D.length_order(np.diff(off)) if args.workload == "uneven" else None
-    dr = D.DeviceReads(buf, off, order=order)
+    # reads stay in input order: the batched path buckets them by length on the
+    # device, the plain path walks them as they come
+    dr = D.DeviceReads(buf, off)
     nreads = dr.nreads
     nrec = nreads // 2 if paired else nreads
     out = D.Outputs(nreads, len(fs), paired, False)
@@ -141,7 +160,14 @@ def main():
 
     def step(path, flags=0):
         out.zero_counts()
-        fs.eval_batch(rs, B.params(paired=paired, path=path, flags=flags), os_, stream)
+        fs.eval_batch(rs, B.params(paired=paired, path=path, flags=flags,
+                                   wave_min_len=args.wave_min), os_, stream)
+
+    tuning = None
+    if args.autotune:
+        t = time.time()
+        tuning = fs.autotune(rs, B.params(paired=paired), os_, stream)
+        log(f"autotune ({time.time() - t:.1f}s): {tuning}")
 
     results = {}
     paths = {"batched": B.PATH_BATCHED, "plain": B.PATH_PLAIN}
@@ -157,8 +183,10 @@ def main():
         host = out.host()
         results[name] = {"ms": ms, "walk_ms": float(np.mean(walk)),
                          "probes": int(host["counters"][0]),
-                         "lookups": int(host["counters"][1]), "hits": host["hits"],
+                         "lookups": int(host["counters"][1]),
+                         "issued": int(host["counters"][2]), "hits": host["hits"],
                          "summary": host["summary"]}
+        assert int(host["counters"][3]) == 0, "device error flag set"
         log(f"{name}: {ms:.3f} ms/step, walk kernel {np.mean(walk):.3f} ms, "
             f"{nreads / ms * 1e3 / 1e6:.2f} M reads/s")
     main_name = "batched" if "batched" in results else "plain"
@@ -172,6 +200,15 @@ def main():
     alg_bytes = R["lookups"] * LINE_BYTES
     achieved = alg_bytes / (R["walk_ms"] / 1e3) / 1e9
     traffic = load_traffic(args.workload)
+    gather = {}
+    if not args.no_gather:
+        fs_bytes = sum(fs.info(i).m // 8 for i in range(len(fs)))
+        fs_pow2 = 1 << max(12, fs_bytes.bit_length() - 1)
+        for label, nbytes in (("hbm_8GiB", 8 << 30), ("filter_set_size", fs_pow2)):
+            lps, _ = B.measure_gather(nbytes, 64, stream)
+            gather[label] = {"bytes": nbytes, "lines_per_s": lps,
+                             "gbs_at_64B": lps * LINE_BYTES / 1e9}
+            log(f"random gather over {nbytes >> 20} MiB: {lps / 1e9:.1f} G lines/s")
 
     cpu = None
     if not args.no_cpu:
@@ -212,6 +249,7 @@ def main():
         "pairs_per_s": reads_per_s / 2 if paired else None,
         "probes_per_step": R["probes"],
         "lookups_per_step": R["lookups"],
+        "issued_lookups_per_step": R["issued"],
         "plain_reads_per_s": (nreads / (results["plain"]["ms"] / 1e3)
                               if "plain" in results else None),
         "speedup_vs_plain": (results["plain"]["ms"] / results["batched"]["ms"]
@@ -226,7 +264,11 @@ def main():
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "walk_ms": R["walk_ms"],
             "algorithmic_bytes": alg_bytes,
+            "random_gather": gather or None,
+            "frac_of_random_gather": (achieved / gather["filter_set_size"]["gbs_at_64B"]
+                                      if gather else None),
         },
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -87,6 +87,16 @@ BBT_HD uint64_t pfx_rc(unsigned code, uint32_t i) { return rotl64(seed_rc(code),
 BBT_HD uint64_t pfx_win_fwd(uint64_t Hi, uint64_t Hb, uint32_t i) { return rotl64(Hi ^ Hb, i & 63); }
 BBT_HD uint64_t pfx_win_rc(uint64_t Gi, uint64_t Gb, uint32_t p) { return rotr64(Gi ^ Gb, p & 63); }
 
+/* j-th value of ntHash's multi-hash expansion of the canonical hash h0:
+ * t = h0 * (j ^ k*multiSeed); t ^ (t >> 27); j = 0 is h0 itself.  kseed is
+ * k * multiSeed, a constant per filter. */
+BBT_HD uint64_t multi_hash(uint64_t h0, uint32_t j, uint64_t kseed) {
+    if (j == 0)
+        return h0;
+    const uint64_t t = h0 * (uint64_t(j) ^ kseed);
+    return t ^ (t >> kMultiShift);
+}
+
 BBT_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __umul64hi(a, b);
@@ -160,14 +170,14 @@ struct Walker {
     const uint8_t *s;
     uint32_t len;
     uint32_t i; /* next base */
-    uint64_t word;
-    uint64_t waddr;
+    uint64_t word;  /* 8 aligned bytes holding base i */
+    uint64_t next;  /* the following 8 bytes, loaded one word ahead */
+    uint64_t waddr; /* address of word */
     /* rolling hash */
     uint64_t fwd, rc, win;
     uint32_t run;
     uint32_t k;
     /* filter */
-    const FilterDev *F;
     const uint8_t *bits;
     uint64_t m, magic;
     uint32_t sh1, sh2, h;
@@ -200,11 +210,13 @@ struct Walker {
         s = seq;
         len = length;
         i = 0;
-        waddr = ~uint64_t(0);
-        word = 0;
+        waddr = reinterpret_cast<uint64_t>(seq) & ~uint64_t(7);
+        word = length ? *reinterpret_cast<const uint64_t *>(waddr) : 0;
+        next = waddr + 8 < reinterpret_cast<uint64_t>(seq) + length
+                   ? *reinterpret_cast<const uint64_t *>(waddr + 8)
+                   : 0;
         fwd = rc = win = 0;
         run = 0;
-        F = f;
         bits = f->bits;
         m = f->m;
         magic = f->magic;
@@ -263,22 +275,24 @@ struct Walker {
     }
 
     /* byte i of the read through an 8-byte aligned register window */
+    /* byte idx of the read; idx advances by one per call, so the word
+     * after the current one is loaded a word ahead of its use (loads stay
+     * inside the read) */
     BBT_HD unsigned base_at(uint32_t idx) {
-        uint64_t a = reinterpret_cast<uint64_t>(s) + idx;
-        uint64_t wa = a & ~uint64_t(7);
+        const uint64_t a = reinterpret_cast<uint64_t>(s) + idx;
+        const uint64_t wa = a & ~uint64_t(7);
         if (wa != waddr) {
             waddr = wa;
-            word = *reinterpret_cast<const uint64_t *>(wa);
+            word = next;
+            next = wa + 8 < reinterpret_cast<uint64_t>(s) + len
+                       ? *reinterpret_cast<const uint64_t *>(wa + 8)
+                       : 0;
         }
         return unsigned(word >> ((a & 7) * 8)) & 0xFFu;
     }
 
-    BBT_HD uint64_t hash_j(uint32_t jj) const {
-        if (jj == 0)
-            return h0;
-        uint64_t t = h0 * F->mult[jj];
-        return t ^ (t >> kMultiShift);
-    }
+    BBT_HD uint64_t kseed() const { return uint64_t(kk()) * kMultiSeed; }
+    BBT_HD uint64_t hash_j(uint32_t jj) const { return multi_hash(h0, jj, kseed()); }
 
     BBT_HD void set_addr() {
         uint64_t pos = fastmod(hash_j(j), m, magic, sh1, sh2);

@@ -79,6 +79,20 @@ struct EvalArgs {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+/* value of lane 0 / lane l, known wave-uniform to the compiler (scalar) */
+__device__ __forceinline__ uint32_t uni32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+}
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return (uint64_t(uni32(uint32_t(v >> 32))) << 32) | uni32(uint32_t(v));
+}
+__device__ __forceinline__ uint32_t lane32(uint32_t v, uint32_t l) {
+    return uint32_t(__builtin_amdgcn_readlane(int(v), int(l)));
+}
+__device__ __forceinline__ uint64_t lane64(uint64_t v, uint32_t l) {
+    return (uint64_t(lane32(uint32_t(v >> 32), l)) << 32) | lane32(uint32_t(v), l);
+}
+
 __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t probes,
                                              uint32_t lookups) {
     if (!ctr)
@@ -142,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
         store_walk(a, read, f, w);
     }
     add_counters(a.counters, w.probes, w.lookups);
+    add_issued(a.counters, w.lookups);
 }
 
 /*
@@ -200,9 +215,9 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         const uint64_t wf = pfx_win_fwd(H, Hb, i);
         const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
         const uint64_t h0 = wr < wf ? wr : wf;
-        Hc = __shfl(H, 63);
-        Gc = __shfl(G, 63);
-        lastN = __shfl(ln, 63);
+        Hc = lane64(H, 63);
+        Gc = lane64(G, 63);
+        lastN = int(lane32(uint32_t(ln), 63));
         Hp = H;
         Gp = G;
         const uint64_t vmask = __ballot(valid);
@@ -214,11 +229,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         uint32_t lk = 0;
         for (uint32_t jj = 0; __any(need); ++jj) {
             if (need) {
-                uint64_t hv = h0;
-                if (jj) {
-                    hv = h0 * F->mult[jj];
-                    hv ^= hv >> kMultiShift;
-                }
+                const uint64_t hv = multi_hash(h0, jj, w.kseed());
                 const uint64_t pos = fastmod(hv, w.m, w.magic, w.sh1, w.sh2);
                 const unsigned byte = w.bits[pos >> 3];
                 ++lk;
@@ -241,7 +252,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
             if (pre == w.PRE_NOPROBE)
                 continue;
             ++w.probes;
-            w.lookups += uint32_t(__shfl(int(lk), L));
+            w.lookups += lane32(lk, uint32_t(L));
             w.on_result(((hmask >> L) & 1) != 0);
             if (w.state != W_RUNNING)
                 break;
@@ -263,25 +274,25 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     const uint64_t nlong = a.list_counts ? a.list_counts[0] * a.nf : 0;
     const uint64_t nitems = a.list_counts ? a.list_counts[1] * a.nf : a.nitems;
 
-    /* phase 1: long reads, a whole wave per (read, filter), longest first */
+    /* phase 1: long reads, a whole wave per (read, filter), longest first;
+     * the walk state is wave-uniform (scalar registers) */
     while (nlong) {
         unsigned long long it = 0;
         if (lane == 0)
             it = atomicAdd(a.lqueue, 1ull);
-        it = __shfl(it, 0);
+        it = uni64(it);
         if (it >= nlong)
             break;
         const uint64_t r = it / a.nf;
         const uint32_t f = uint32_t(it - r * a.nf);
-        const uint64_t read = a.long_list[r];
-        wave_walk(a, read, f, w, issued);
+        const uint64_t read = uni64(a.long_list[r]);
+        Walker<SC, EXH, K> ww;
+        wave_walk(a, read, f, ww, issued);
         if (lane == 0) {
-            store_walk(a, read, f, w);
-            wave_probes += w.probes;
-            wave_lookups += w.lookups;
+            store_walk(a, read, f, ww);
+            wave_probes += ww.probes;
+            wave_lookups += ww.lookups;
         }
-        w.probes = 0;
-        w.lookups = 0;
     }
 
     /* phase 2: one walk per lane, idle lanes refilled from the queue */
@@ -439,6 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_ordered(EvalArgs a, uint64_t nrec,
         }
     }
     add_counters(a.counters, w.probes, w.lookups);
+    add_issued(a.counters, w.lookups);
 }
 
 __global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v) {
@@ -553,11 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t 
             const uint64_t h0 = rc < fwd ? rc : fwd;
             bool found = true;
             for (uint32_t j = 0; j < f.h; ++j) {
-                uint64_t hv = h0;
-                if (j) {
-                    hv = h0 * f.mult[j];
-                    hv ^= hv >> kMultiShift;
-                }
+                const uint64_t hv = multi_hash(h0, j, uint64_t(kv) * kMultiSeed);
                 const uint64_t pos = fastmod(hv, f.m, f.magic, f.sh1, f.sh2);
                 const unsigned bit = 1u << unsigned(pos & 31);
                 const unsigned old = atomicOr(words + (pos >> 5), bit);
@@ -578,6 +586,30 @@ __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t 
         atomicAdd(counts + 0, a);
         atomicAdd(counts + 1, b);
     }
+}
+
+/* Independent random byte loads over a power-of-two buffer, 8 in flight per
+ * lane: the random-line rate that bounds a Bloom probe. */
+__global__ __launch_bounds__(kBlock) void k_rand_gather(const uint8_t *buf, uint64_t mask,
+                                                        uint32_t rounds,
+                                                        unsigned long long *sink) {
+    uint64_t x = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 0x9E3779B97F4A7C15ull + 1;
+    unsigned acc = 0;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        unsigned v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            v[u] = buf[x & mask];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            acc += v[u];
+    }
+    if (acc == 0xFFFFFFFFu)
+        sink[0] = acc;
 }
 
 __global__ void k_popcount(const uint64_t *w, uint64_t nwords, unsigned long long *out) {
@@ -616,6 +648,11 @@ struct bbt_filterset {
     unsigned long long *dctl = nullptr;
     uint32_t *dlists = nullptr; /* 2 * lists_cap read indices */
     uint64_t lists_cap = 0;
+    /* scheduling knobs (bbt_set_tuning / bbt_autotune) */
+    bbt_tuning tune = {0, 0, 0, 0};
+    /* replayed batch (BBT_FLAG_GRAPH) */
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
     /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_walk_ms = -1;
@@ -779,6 +816,8 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->dctl);
     if (fs->dlists)
         (void)hipFree(fs->dlists);
+    if (fs->graph_exec)
+        (void)hipGraphExecDestroy(fs->graph_exec);
     if (fs->ev0)
         (void)hipEventDestroy(fs->ev0);
     if (fs->ev1)
@@ -976,7 +1015,22 @@ struct Launch {
     hipStream_t st;
     uint32_t path;
     int cu;
+    uint32_t blocks_per_cu; /* persistent grid: 0 = as many as fit */
 };
+
+/* resident blocks per CU of one persistent-kernel instantiation, queried once */
+template <int SC, bool EXH, int K>
+int persistent_occupancy() {
+    static int cached = -1;
+    if (cached < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, k_walk_persistent<SC, EXH, K>, kBlock, 0) != hipSuccess)
+            per_cu = 1;
+        cached = std::max(1, per_cu);
+    }
+    return cached;
+}
 
 template <int SC, bool EXH, int K>
 bbt_status launch_walk(const Launch &L) {
@@ -988,10 +1042,9 @@ bbt_status launch_walk(const Launch &L) {
             return fail(BBT_ERR_ARG, "batch too large for one launch");
         k_walk_plain<SC, EXH, K><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a);
     } else {
-        int per_cu = 0;
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_walk_persistent<SC, EXH, K>, kBlock, 0));
-        per_cu = std::max(1, per_cu);
+        int per_cu = persistent_occupancy<SC, EXH, K>();
+        if (L.blocks_per_cu)
+            per_cu = std::min<int>(per_cu, int(L.blocks_per_cu));
         uint64_t blocks = uint64_t(L.cu) * uint64_t(per_cu);
         const uint64_t need = (L.a.nitems + kBlock - 1) / kBlock;
         blocks = std::max<uint64_t>(1, std::min(blocks, need));
@@ -1079,14 +1132,19 @@ bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) 
 
 } // namespace
 
-extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
-                                     const bbt_eval_params *p, bbt_eval_out *out,
-                                     void *stream) {
-    if (!fs || !in || !p || !out)
-        return fail(BBT_ERR_ARG, "bbt_eval_batch: null argument");
+/* Everything of one bbt_eval_batch that is host work (checks, tables,
+ * allocations) and the argument block of its kernels. */
+struct Prepared {
+    Launch L;
+    uint32_t nf;
+    uint64_t nrec;
+    uint32_t kk;
+    uint32_t wave_min;
+};
+
+static bbt_status prepare(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
+                          const bbt_eval_out *out, hipStream_t st, Prepared &P) {
     const uint32_t nf = uint32_t(fs->desc.size());
-    if (in->nreads == 0)
-        return BBT_OK;
     if (!in->seq || !in->off)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: reads not given");
     if (reinterpret_cast<uintptr_t>(in->seq) % 8 != 0 || in->seq_bytes % 8 != 0)
@@ -1104,12 +1162,12 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     if (need_scores && p->scorer == BBT_BINOMIAL)
         return fail(BBT_ERR_UNSUPPORTED,
                     "binomial evalScore (best-hit / with-score modes) is host-only");
+    if (out->summary && (2 * size_t(nf) + 2) * sizeof(unsigned int) > 64 * 1024)
+        return fail(BBT_ERR_UNSUPPORTED, "too many filters for device summary counters");
     uint32_t common_k = fs->desc[0].k;
     for (const FilterDev &d : fs->desc)
         if (d.k != common_k)
             common_k = 0;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    std::lock_guard<std::mutex> g(fs->mu);
     if (p->scorer == BBT_BINOMIAL) {
         uint32_t kmin = 64;
         for (const FilterDev &d : fs->desc)
@@ -1122,12 +1180,22 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     bbt_status s = sync_desc(fs);
     if (s != BBT_OK)
         return s;
-
-    Launch L;
-    std::memset(&L, 0, sizeof(L));
+    if (!fs->dctl)
+        HIP_OK(hipMalloc(&fs->dctl, 128 * sizeof(unsigned long long)));
+    if (p->path == BBT_PATH_BATCHED && fs->lists_cap < in->nreads) {
+        if (fs->dlists)
+            HIP_OK(hipFree(fs->dlists));
+        fs->dlists = nullptr;
+        fs->lists_cap = 0;
+        HIP_OK(hipMalloc(&fs->dlists, 2 * in->nreads * sizeof(uint32_t)));
+        fs->lists_cap = in->nreads;
+    }
+    std::memset(&P, 0, sizeof(P));
+    Launch &L = P.L;
     L.st = st;
     L.path = p->path;
     L.cu = fs->cu_count;
+    L.blocks_per_cu = fs->tune.blocks_per_cu;
     EvalArgs &a = L.a;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
     a.off = in->off;
@@ -1137,71 +1205,67 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     a.scores = out->scores;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.nf = nf;
-    a.refill_min = 8;
+    a.refill_min = fs->tune.refill_min ? fs->tune.refill_min : 8;
     a.wp.threshold = p->threshold;
     a.wp.streak = p->streak;
     a.wp.min_len = uint32_t(std::round(p->threshold));
     a.nitems = in->nreads * nf;
+    P.nf = nf;
+    P.nrec = p->paired ? in->nreads / 2 : in->nreads;
+    P.kk = common_k == 25 ? 25u : 0u;
+    P.wave_min = p->wave_min_len ? p->wave_min_len
+                                 : (fs->tune.wave_min_len ? fs->tune.wave_min_len
+                                                          : kDefaultWaveMin);
+    return BBT_OK;
+}
 
-    const uint64_t nrec = p->paired ? in->nreads / 2 : in->nreads;
-    const uint32_t kk = common_k == 25 ? 25u : 0u;
-    const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
+/* The kernels of one batch, in stream order; capturable (no memset nodes:
+ * counters are cleared by a fill kernel). */
+static bbt_status enqueue(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
+                          const bbt_eval_out *out, Prepared &P, bool timed) {
+    Launch &L = P.L;
+    EvalArgs &a = L.a;
+    hipStream_t st = L.st;
+    const uint32_t nf = P.nf;
+    bbt_status s = BBT_OK;
     /* work queues of the (up to two) walks of this call, and for the batched
      * path the length buckets: long reads walked by whole waves */
-    if (!fs->dctl)
-        HIP_OK(hipMalloc(&fs->dctl, 128 * sizeof(unsigned long long)));
     k_zero_u64<<<1, 64, 0, st>>>(fs->dctl, 48);
     a.queue = fs->dctl + 0;
     a.lqueue = fs->dctl + 2;
-    const uint32_t wave_min = p->wave_min_len ? p->wave_min_len : kDefaultWaveMin;
     if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED) {
-        if (fs->lists_cap < in->nreads) {
-            if (fs->dlists)
-                HIP_OK(hipFree(fs->dlists));
-            fs->dlists = nullptr;
-            fs->lists_cap = 0;
-            HIP_OK(hipMalloc(&fs->dlists, 2 * in->nreads * sizeof(uint32_t)));
-            fs->lists_cap = in->nreads;
-        }
-        const uint32_t gb = uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));
-        k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, wave_min, fs->dctl + 8);
+        const uint32_t gb =
+            uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));
+        k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min, fs->dctl + 8);
         k_len_scan<<<1, 64, 0, st>>>(fs->dctl + 8, fs->dctl + 48, fs->dctl + 4);
-        k_len_scatter<<<gb, kBlock, 0, st>>>(in->off, in->nreads, wave_min, fs->dctl + 48,
-                                              fs->dctl + 8, fs->dlists,
+        k_len_scatter<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min,
+                                              fs->dctl + 48, fs->dctl + 8, fs->dlists,
                                               fs->dlists + fs->lists_cap);
         HIP_OK(hipGetLastError());
         a.long_list = fs->dlists;
         a.short_list = fs->dlists + fs->lists_cap;
         a.list_counts = fs->dctl + 4;
     }
-    if (timed) {
-        if (!fs->ev0)
-            HIP_OK(hipEventCreate(&fs->ev0));
-        if (!fs->ev1)
-            HIP_OK(hipEventCreate(&fs->ev1));
-        if (p->mode == BBT_ORDERED)
-            k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
+    if (p->mode == BBT_ORDERED)
+        k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
+    if (timed)
         HIP_OK(hipEventRecord(fs->ev0, st));
-    }
     switch (p->mode) {
-    case BBT_ORDERED: {
-        if (!timed)
-            k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
-        s = launch_ordered(L, p->scorer, nrec, p->paired, p->inclusive, kk);
+    case BBT_ORDERED:
+        s = launch_ordered(L, p->scorer, P.nrec, p->paired, p->inclusive, P.kk);
         break;
-    }
     case BBT_BESTHIT:
-        s = launch_scorer(L, p->scorer, true, kk);
+        s = launch_scorer(L, p->scorer, true, P.kk);
         break;
     case BBT_SCORES:
-        s = launch_scorer(L, p->scorer, false, kk);
+        s = launch_scorer(L, p->scorer, false, P.kk);
         a.queue = fs->dctl + 1;
         a.lqueue = fs->dctl + 3;
         if (s == BBT_OK)
-            s = launch_scorer(L, p->scorer, true, kk);
+            s = launch_scorer(L, p->scorer, true, P.kk);
         break;
     default:
-        s = launch_scorer(L, p->scorer, false, kk);
+        s = launch_scorer(L, p->scorer, false, P.kk);
         break;
     }
     if (s != BBT_OK)
@@ -1209,15 +1273,85 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     if (timed)
         HIP_OK(hipEventRecord(fs->ev1, st));
     if (out->cls || out->summary || p->mode == BBT_BESTHIT) {
-        const uint64_t blocks = (nrec + kBlock - 1) / kBlock;
+        const uint64_t blocks = (P.nrec + kBlock - 1) / kBlock;
         const size_t shm = out->summary ? (2 * size_t(nf) + 2) * sizeof(unsigned int) : 0;
-        if (shm > 64 * 1024)
-            return fail(BBT_ERR_UNSUPPORTED, "too many filters for device summary counters");
         k_classify<<<dim3(uint32_t(blocks)), kBlock, shm, st>>>(
-            out->hits, out->scores, nf, nrec, p->paired, p->inclusive,
+            out->hits, out->scores, nf, P.nrec, p->paired, p->inclusive,
             p->mode == BBT_BESTHIT, out->cls,
             reinterpret_cast<unsigned long long *>(out->summary));
         HIP_OK(hipGetLastError());
+    }
+    return BBT_OK;
+}
+
+/* identity of a captured batch: every pointer and value the kernels see */
+static std::string graph_key(const bbt_reads *in, const bbt_eval_params *p,
+                             const bbt_eval_out *out, const Prepared &P, hipStream_t st) {
+    std::string k;
+    auto add = [&k](const void *v, size_t n) { k.append(static_cast<const char *>(v), n); };
+    add(in, sizeof(*in));
+    add(p, sizeof(*p));
+    add(out, sizeof(*out));
+    add(&P.wave_min, sizeof(P.wave_min));
+    add(&P.L.a.refill_min, sizeof(P.L.a.refill_min));
+    add(&P.L.blocks_per_cu, sizeof(P.L.blocks_per_cu));
+    add(&st, sizeof(st));
+    return k;
+}
+
+extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
+                                     const bbt_eval_params *p, bbt_eval_out *out,
+                                     void *stream) {
+    if (!fs || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: null argument");
+    if (in->nreads == 0)
+        return BBT_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(fs->mu);
+    Prepared P;
+    bbt_status s = prepare(fs, in, p, out, st, P);
+    if (s != BBT_OK)
+        return s;
+    const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
+    if (timed) {
+        if (!fs->ev0)
+            HIP_OK(hipEventCreate(&fs->ev0));
+        if (!fs->ev1)
+            HIP_OK(hipEventCreate(&fs->ev1));
+    }
+    const bool graph = (p->flags & BBT_FLAG_GRAPH) != 0 && !timed;
+    if (!graph) {
+        s = enqueue(fs, in, p, out, P, timed);
+        if (s != BBT_OK)
+            return s;
+    } else {
+        const std::string key = graph_key(in, p, out, P, st);
+        if (!fs->graph_exec || key != fs->graph_key) {
+            if (fs->graph_exec) {
+                (void)hipGraphExecDestroy(fs->graph_exec);
+                fs->graph_exec = nullptr;
+            }
+            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            s = enqueue(fs, in, p, out, P, false);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &gr);
+            if (s != BBT_OK) {
+                if (gr)
+                    (void)hipGraphDestroy(gr);
+                return s;
+            }
+            if (e != hipSuccess)
+                return fail(BBT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&fs->graph_exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                fs->graph_exec = nullptr;
+                return fail(BBT_ERR_HIP, std::string("graph instantiate: ") +
+                                             hipGetErrorString(e));
+            }
+            fs->graph_key = key;
+        }
+        HIP_OK(hipGraphLaunch(fs->graph_exec, st));
     }
     if (timed) {
         HIP_OK(hipEventSynchronize(fs->ev1));
@@ -1226,6 +1360,117 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
         fs->last_walk_ms = ms;
     }
     return BBT_OK;
+}
+
+extern "C" bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void *stream,
+                                         double *lines_per_s, double *ms_out) {
+    if (nbytes < 4096 || (nbytes & (nbytes - 1)) || !rounds || !lines_per_s)
+        return fail(BBT_ERR_ARG, "bbt_measure_gather: nbytes must be a power of two >= 4096");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int dev = 0, cu = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    uint8_t *buf = nullptr;
+    unsigned long long *sink = nullptr;
+    if (hipMalloc(&buf, nbytes) != hipSuccess)
+        return fail(BBT_ERR_NOMEM, "bbt_measure_gather: allocation failed");
+    if (hipMalloc(&sink, 8) != hipSuccess) {
+        (void)hipFree(buf);
+        return fail(BBT_ERR_NOMEM, "bbt_measure_gather: allocation failed");
+    }
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    const uint32_t blocks = uint32_t(cu) * 8;
+    k_fill_u8<<<4096, kBlock, 0, st>>>(buf, nbytes, 1);
+    k_rand_gather<<<blocks, kBlock, 0, st>>>(buf, nbytes - 1, rounds / 8 + 1, sink); /* warm */
+    HIP_OK(hipEventRecord(e0, st));
+    k_rand_gather<<<blocks, kBlock, 0, st>>>(buf, nbytes - 1, rounds, sink);
+    HIP_OK(hipEventRecord(e1, st));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    (void)hipFree(sink);
+    const double loads = double(blocks) * kBlock * double(rounds) * 8;
+    *lines_per_s = loads / (double(ms) * 1e-3);
+    if (ms_out)
+        *ms_out = ms;
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t) {
+    if (!fs || !t)
+        return fail(BBT_ERR_ARG, "bbt_set_tuning: null argument");
+    std::lock_guard<std::mutex> g(fs->mu);
+    fs->tune = *t;
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_get_tuning(const bbt_filterset *fs, bbt_tuning *t) {
+    if (!fs || !t)
+        return fail(BBT_ERR_ARG, "bbt_get_tuning: null argument");
+    *t = fs->tune;
+    return BBT_OK;
+}
+
+/* Coordinate search over the knobs, each candidate timed on the full batch
+ * by the walk-kernel events. */
+extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
+                                   const bbt_eval_params *p, bbt_eval_out *out,
+                                   void *stream, bbt_tuning *best) {
+    if (!fs || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_autotune: null argument");
+    bbt_eval_params q = *p;
+    q.path = BBT_PATH_BATCHED;
+    q.flags = BBT_FLAG_TIME_WALK;
+    q.wave_min_len = 0;
+    bbt_tuning cur = fs->tune;
+    auto trial = [&](const bbt_tuning &t, double &ms) -> bbt_status {
+        bbt_status s = bbt_set_tuning(fs, &t);
+        if (s == BBT_OK)
+            s = bbt_eval_batch(fs, in, &q, out, stream);
+        ms = fs->last_walk_ms;
+        return s;
+    };
+    double best_ms = 0;
+    bbt_status s = trial(cur, best_ms);
+    if (s != BBT_OK)
+        return s;
+    const uint32_t waves[] = {256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
+    const uint32_t refills[] = {1, 4, 8, 16, 32};
+    const uint32_t bpcs[] = {0, 1, 2, 3, 4};
+    for (int dim = 0; dim < 3; ++dim) {
+        const uint32_t *vals = dim == 0 ? waves : (dim == 1 ? refills : bpcs);
+        const int nv = dim == 0 ? 6 : 5;
+        for (int i = 0; i < nv; ++i) {
+            bbt_tuning t = cur;
+            if (dim == 0) {
+                if (vals[i] != 0x7FFFFFFFu && vals[i] > in->max_len)
+                    continue; /* no read would change path */
+                t.wave_min_len = vals[i];
+            } else if (dim == 1) {
+                t.refill_min = vals[i];
+            } else {
+                t.blocks_per_cu = vals[i];
+            }
+            double ms = 0;
+            s = trial(t, ms);
+            if (s != BBT_OK)
+                return s;
+            if (ms < best_ms) {
+                best_ms = ms;
+                cur = t;
+            }
+        }
+    }
+    s = bbt_set_tuning(fs, &cur);
+    if (best)
+        *best = cur;
+    fs->last_walk_ms = best_ms;
+    return s;
 }
 
 extern "C" double bbt_last_walk_ms(const bbt_filterset *fs) {

@@ -15,6 +15,7 @@ SIMPLE, LENGTH, HARMONIC, BINOMIAL = 0, 1, 2, 3
 STD, ORDERED, BESTHIT, SCORES = 0, 1, 2, 3
 PATH_PLAIN, PATH_BATCHED = 0, 1
 FLAG_TIME_WALK = 1
+FLAG_GRAPH = 2
 
 
 class BbtError(RuntimeError):
@@ -32,6 +33,11 @@ class EvalParams(C.Structure):
                 ("streak", C.c_uint32), ("paired", C.c_uint32),
                 ("inclusive", C.c_uint32), ("path", C.c_uint32), ("flags", C.c_uint32),
                 ("wave_min_len", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("wave_min_len", C.c_uint32), ("refill_min", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Reads(C.Structure):
@@ -71,6 +77,12 @@ _SIGS = {
     "bbt_eval_batch": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
                                    C.POINTER(EvalOut), C.c_void_p]),
     "bbt_last_walk_ms": (C.c_double, [C.c_void_p]),
+    "bbt_measure_gather": (C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bbt_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "bbt_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "bbt_autotune": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
+                                 C.POINTER(EvalOut), C.c_void_p, C.POINTER(Tuning)]),
     "bbt_summary_text": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64,
                                      C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bbt_calc_min_count": (C.c_uint64, [C.c_uint64, C.c_double, C.c_double]),
@@ -181,6 +193,23 @@ class FilterSet:
         check(load().bbt_eval_batch(self._h, C.byref(reads), C.byref(params),
                                     C.byref(out), C.c_void_p(stream)))
 
+    def set_tuning(self, wave_min_len=0, refill_min=0, blocks_per_cu=0):
+        check(load().bbt_set_tuning(self._h, C.byref(Tuning(wave_min_len, refill_min,
+                                                             blocks_per_cu, 0))))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(load().bbt_get_tuning(self._h, C.byref(t)))
+        return {"wave_min_len": t.wave_min_len, "refill_min": t.refill_min,
+                "blocks_per_cu": t.blocks_per_cu}
+
+    def autotune(self, reads, params, out, stream=0):
+        t = Tuning()
+        check(load().bbt_autotune(self._h, C.byref(reads), C.byref(params), C.byref(out),
+                                  C.c_void_p(stream), C.byref(t)))
+        return {"wave_min_len": t.wave_min_len, "refill_min": t.refill_min,
+                "blocks_per_cu": t.blocks_per_cu, "walk_ms": self.last_walk_ms()}
+
     def last_walk_ms(self):
         return load().bbt_last_walk_ms(self._h)
 
@@ -192,6 +221,14 @@ class FilterSet:
         buf = C.create_string_buffer(need.value + 1)
         check(L.bbt_summary_text(self._h, arr, nrec, buf, need.value + 1, None))
         return buf.value.decode()
+
+
+def measure_gather(nbytes, rounds=64, stream=0):
+    """Random-line load rate (lines/s) over a device buffer of nbytes."""
+    r, ms = C.c_double(), C.c_double()
+    check(load().bbt_measure_gather(nbytes, rounds, C.c_void_p(stream), C.byref(r),
+                                    C.byref(ms)))
+    return r.value, ms.value
 
 
 def params(mode=STD, scorer=SIMPLE, threshold=0.15, streak=3, paired=False,

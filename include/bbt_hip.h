@@ -64,8 +64,20 @@ typedef int32_t bbt_status;
 #define BBT_FLAG_TIME_WALK 1u /* time the walk kernel(s) with HIP events on the
                                  stream; the call then waits for them and
                                  bbt_last_walk_ms() returns the duration */
+#define BBT_FLAG_GRAPH 2u     /* batch as one captured hipGraph: captured on the
+                                 first call with a given set of buffers and
+                                 parameters, replayed by later identical calls */
 
 typedef struct bbt_filterset bbt_filterset;
+
+/* Scheduling knobs of the batched path; 0 = library default.  They change
+ * only how work is spread over the GPU, never a result. */
+typedef struct {
+    uint32_t wave_min_len;  /* reads this long are walked by a whole wavefront */
+    uint32_t refill_min;    /* idle lanes a wave gathers before refilling */
+    uint32_t blocks_per_cu; /* persistent blocks (256 threads) per CU */
+    uint32_t reserved;
+} bbt_tuning;
 
 typedef struct {
     uint64_t m;         /* filter bits */
@@ -156,6 +168,20 @@ uint32_t bbt_calc_optimal_hashnum(double fpr);
 bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
                           const bbt_eval_params *p, bbt_eval_out *out,
                           void *stream);
+/* Scheduling knobs of a filter set (no counterpart in the reference). */
+bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t);
+bbt_status bbt_get_tuning(const bbt_filterset *fs, bbt_tuning *t);
+/* Times candidate knob settings of the batched path on this batch (each
+ * candidate runs the full batch; outputs are overwritten, counters added
+ * to), keeps the fastest in fs and returns it in *best. */
+bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
+                        bbt_eval_out *out, void *stream, bbt_tuning *best);
+/* Measurement utility (no counterpart in the reference): rate of
+ * independent random byte loads over a fresh nbytes device buffer (power of
+ * two), 8 in flight per lane, rounds x 8 loads per lane; each load touches
+ * one random cache line. */
+bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void *stream,
+                              double *lines_per_s, double *ms);
 /* Duration of the walk kernel(s) of the last bbt_eval_batch on fs that was
  * called with BBT_FLAG_TIME_WALK, in milliseconds (-1 if none). */
 double bbt_last_walk_ms(const bbt_filterset *fs);

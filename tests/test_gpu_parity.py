@@ -204,3 +204,48 @@ def test_store_and_load_roundtrip(dev, tmp_path):
         fs2.download(i, b)
         assert np.array_equal(a, b)
         assert fs2.info(i).fpr == fs.info(i).fpr
+
+
+def test_graph_replay_matches_eager(dev, multi):
+    D, B = dev
+    fs, ofs, buf, off = multi
+    dr = D.DeviceReads(buf, off)
+    oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True)
+    out = D.Outputs(dr.nreads, len(fs), True, False)
+    for streak in [3, 1]:
+        if streak == 1:
+            oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True,
+                                           p=O.params(O.SIMPLE, 0.15, 1))
+        for rep in range(4):
+            print(f"[gpu] graph replay streak={streak} rep={rep}", flush=True)
+            out.hits.fill_(7)
+            res = D.classify(fs, dr, B.params(paired=True, streak=streak,
+                                              flags=B.FLAG_GRAPH), out=out)
+            assert np.array_equal(res["hits"], oh)
+            assert np.array_equal(res["cls"], ocls)
+            assert int(res["counters"][1]) == int(octr[1])
+            eager = D.classify(fs, dr, B.params(paired=True, streak=streak))
+            assert np.array_equal(eager["summary"], res["summary"])
+
+
+def test_autotune_keeps_results(dev):
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(3, 7)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 150_000)) for _ in range(2)]
+    longs, _ = synth.long_reads(rng, genomes, 40, median=3000, hi=12_000)
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 2000, 150, 0.01, 0.7)
+    buf, off = O.pack_reads([bytes(x) for x in longs + list(shorts)])
+    fs, _ = D.build_filters(genomes, 25, 0.05)
+    ofs = oracle_filters_like(fs, B)
+    oh, _, ocls, octr = O.classify(ofs, buf, off)
+    dr = D.DeviceReads(buf, off)
+    out = D.Outputs(dr.nreads, len(fs), False, False)
+    print("[gpu] autotune", flush=True)
+    best = fs.autotune(dr.struct(), B.params(), out.struct(),
+                       int(torch.cuda.current_stream().cuda_stream))
+    assert best["walk_ms"] > 0
+    assert fs.get_tuning()["wave_min_len"] == best["wave_min_len"]
+    res = D.classify(fs, dr, B.params())
+    assert np.array_equal(res["hits"], oh) and np.array_equal(res["cls"], ocls)
+    assert int(res["counters"][1]) == int(octr[1])
