@@ -34,11 +34,8 @@ using namespace bbt;
 /* ======================================================================= */
 /* error reporting                                                          */
 
-static thread_local std::string g_err;
-
 static bbt_status fail(bbt_status code, const std::string &msg) {
-    g_err = msg;
-    return code;
+    return set_error(code, msg);
 }
 
 #define HIP_OK(expr)                                                            \
@@ -78,20 +75,6 @@ struct EvalArgs {
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-/* value of lane 0 / lane l, known wave-uniform to the compiler (scalar) */
-__device__ __forceinline__ uint32_t uni32(uint32_t v) {
-    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
-}
-__device__ __forceinline__ uint64_t uni64(uint64_t v) {
-    return (uint64_t(uni32(uint32_t(v >> 32))) << 32) | uni32(uint32_t(v));
-}
-__device__ __forceinline__ uint32_t lane32(uint32_t v, uint32_t l) {
-    return uint32_t(__builtin_amdgcn_readlane(int(v), int(l)));
-}
-__device__ __forceinline__ uint64_t lane64(uint64_t v, uint32_t l) {
-    return (uint64_t(lane32(uint32_t(v >> 32), l)) << 32) | lane32(uint32_t(v), l);
-}
 
 __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t probes,
                                              uint32_t lookups) {
@@ -180,7 +163,6 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
     w.probes = 0;
     w.lookups = 0;
     const unsigned kv = w.kk();
-    const FilterDev *F = a.filters + f;
     uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
     int lastN = -1;
     for (uint32_t b0 = 0; b0 < len && w.state == W_RUNNING; b0 += 64) {
@@ -215,9 +197,9 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         const uint64_t wf = pfx_win_fwd(H, Hb, i);
         const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
         const uint64_t h0 = wr < wf ? wr : wf;
-        Hc = lane64(H, 63);
-        Gc = lane64(G, 63);
-        lastN = int(lane32(uint32_t(ln), 63));
+        Hc = __shfl(H, 63);
+        Gc = __shfl(G, 63);
+        lastN = __shfl(ln, 63);
         Hp = H;
         Gp = G;
         const uint64_t vmask = __ballot(valid);
@@ -252,7 +234,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
             if (pre == w.PRE_NOPROBE)
                 continue;
             ++w.probes;
-            w.lookups += lane32(lk, uint32_t(L));
+            w.lookups += uint32_t(__shfl(int(lk), L));
             w.on_result(((hmask >> L) & 1) != 0);
             if (w.state != W_RUNNING)
                 break;
@@ -275,17 +257,17 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     const uint64_t nitems = a.list_counts ? a.list_counts[1] * a.nf : a.nitems;
 
     /* phase 1: long reads, a whole wave per (read, filter), longest first;
-     * the walk state is wave-uniform (scalar registers) */
+     * every lane holds the same walk state */
     while (nlong) {
         unsigned long long it = 0;
         if (lane == 0)
             it = atomicAdd(a.lqueue, 1ull);
-        it = uni64(it);
+        it = __shfl(it, 0);
         if (it >= nlong)
             break;
         const uint64_t r = it / a.nf;
         const uint32_t f = uint32_t(it - r * a.nf);
-        const uint64_t read = uni64(a.long_list[r]);
+        const uint64_t read = a.long_list[r];
         Walker<SC, EXH, K> ww;
         wave_walk(a, read, f, ww, issued);
         if (lane == 0) {
@@ -653,6 +635,9 @@ struct bbt_filterset {
     /* replayed batch (BBT_FLAG_GRAPH) */
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
+    /* captures run on this stream: the caller's may be the null stream,
+     * which cannot be captured; the graph is launched on the caller's */
+    hipStream_t cap_stream = nullptr;
     /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_walk_ms = -1;
@@ -750,7 +735,7 @@ static void set_fpr(bbt_filterset *fs, uint32_t idx, uint64_t pop) {
 
 extern "C" {
 
-const char *bbt_last_error(void) { return g_err.c_str(); }
+const char *bbt_last_error(void) { return last_error(); }
 
 bbt_status bbt_filters_create(uint32_t n, const uint64_t *m, const uint32_t *h,
                               const uint32_t *k, bbt_filterset **out) {
@@ -818,6 +803,8 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->dlists);
     if (fs->graph_exec)
         (void)hipGraphExecDestroy(fs->graph_exec);
+    if (fs->cap_stream)
+        (void)hipStreamDestroy(fs->cap_stream);
     if (fs->ev0)
         (void)hipEventDestroy(fs->ev0);
     if (fs->ev1)
@@ -1331,10 +1318,14 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
                 (void)hipGraphExecDestroy(fs->graph_exec);
                 fs->graph_exec = nullptr;
             }
-            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            if (!fs->cap_stream)
+                HIP_OK(hipStreamCreateWithFlags(&fs->cap_stream, hipStreamNonBlocking));
+            P.L.st = fs->cap_stream;
+            HIP_OK(hipStreamBeginCapture(fs->cap_stream, hipStreamCaptureModeRelaxed));
             s = enqueue(fs, in, p, out, P, false);
+            P.L.st = st;
             hipGraph_t gr = nullptr;
-            hipError_t e = hipStreamEndCapture(st, &gr);
+            hipError_t e = hipStreamEndCapture(fs->cap_stream, &gr);
             if (s != BBT_OK) {
                 if (gr)
                     (void)hipGraphDestroy(gr);

@@ -334,4 +334,13 @@ std::string summary_text(const std::vector<std::string> &ids, const uint64_t *sm
     return o.str();
 }
 
+static thread_local std::string g_last_error;
+
+int32_t set_error(int32_t code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+
 } // namespace bbt

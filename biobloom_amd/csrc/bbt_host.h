@@ -65,6 +65,10 @@ uint64_t calc_min_count(uint64_t frame_len, double bf_fpr, double min_fpr);
 std::string summary_text(const std::vector<std::string> &ids, const uint64_t *summary,
                          uint64_t nrec);
 
+/* Message of the last failure on the calling thread (bbt_last_error). */
+int32_t set_error(int32_t code, const std::string &msg);
+const char *last_error();
+
 } // namespace bbt
 
 #endif

@@ -1,0 +1,1035 @@
+/*
+ * bbt_mibf.hip -- multi-index Bloom filter (MIBF) on gfx950: build and query.
+ *
+ * Reference structure (BioBloom Tools 2.3.4; the MIBloomFilter classes are in
+ * the absent btl_bloomfilter submodule):
+ *   build  BioBloomMaker/MIBFGen.hpp:131-309 -- bit-vector pass, ID pass
+ *          (random ownership), saturation pass ("if target frame does not
+ *          have a single representative, mark frame as saturated")
+ *   query  BioBloomCategorizer/MIBFClassifier.hpp:773-796 (classify), 79-94
+ *          (allowed misses, frame probabilities), 930-939 (getMinCount)
+ * The exact definitions restated here are listed in the oracle's header
+ * (ownership by the largest (mix32(h ^ id*C), id), saturation per frame,
+ * per-frame ID counting, ranking by nonSat/count/solid/totalNonSat/total/id).
+ *
+ * Device layout of the bit vector: 64-byte lines of one 64-bit rank sample
+ * (set bits before the line) and seven 64-bit words, so a bit test and its
+ * rank1 come from one line; the ID array is rank-ordered, 32-bit
+ * (bit 31 = saturated).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bbt_hip.h"
+#include "bbt_device.h"
+#include "bbt_host.h"
+
+using namespace bbt;
+
+namespace {
+
+constexpr uint32_t kSatBit = 0x80000000u;
+constexpr uint32_t kIdMask = 0x7FFFFFFFu;
+constexpr uint32_t kMaxSeeds = 8;
+constexpr uint32_t kWordsPerLine = 7;
+
+bbt_status mfail(bbt_status c, const std::string &m) { return set_error(c, m); }
+
+#define MHIP(expr)                                                                   \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return mfail(BBT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+/* Uniform description of a MIBF for the kernels. */
+struct MibfDev {
+    const uint64_t *lines; /* nlines * 8 words */
+    const uint32_t *data;
+    uint64_t m;
+    uint64_t magic;
+    uint32_t sh1, sh2;
+    uint32_t S, span;
+    uint32_t nids;
+    uint32_t pad;
+    uint32_t care[kMaxSeeds]; /* bit j set: seed position j is a care position */
+};
+
+__device__ __forceinline__ uint32_t mix32(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return uint32_t(x >> 32);
+}
+
+/* bit p and rank1(p) from the interleaved lines */
+__device__ __forceinline__ bool bv_rank(const uint64_t *lines, uint64_t p, uint64_t &rank) {
+    const uint64_t w = p >> 6;
+    const uint64_t line = w / kWordsPerLine;
+    const uint32_t slot = uint32_t(w - line * kWordsPerLine);
+    const uint64_t *L = lines + line * 8;
+    uint64_t r = L[0];
+    for (uint32_t s = 0; s < slot; ++s)
+        r += uint64_t(__popcll(L[1 + s]));
+    const uint64_t word = L[1 + slot];
+    const uint32_t b = uint32_t(p & 63);
+    rank = r + uint64_t(__popcll(b ? word & (~uint64_t(0) >> (64 - b)) : 0));
+    return (word >> b) & 1;
+}
+
+/* Spaced-seed hashes of the window of span k starting at p, from the full
+ * window values (fwd, rc) with the don't-care bases XORed back out. */
+__device__ __forceinline__ uint64_t seed_hash(const uint8_t *s, uint64_t p, uint32_t k,
+                                              uint32_t care, uint64_t fwd, uint64_t rc) {
+    for (uint32_t j = 0; j < k; ++j) {
+        if ((care >> j) & 1)
+            continue;
+        const unsigned code = base_code(s[p + j]);
+        fwd ^= rotl64(seed_fwd(code), k - 1 - j);
+        rc ^= rotl64(seed_rc(code), j);
+    }
+    return rc < fwd ? rc : fwd;
+}
+
+/* ---------------------------------------------------------------- build */
+
+/* hashes of every N-free window of records (separated by non-ACGT bytes):
+ * the callback gets (window start, seed index, hash) */
+struct FrameCursor {
+    uint64_t fwd = 0, rc = 0, win = 0;
+    uint32_t run = 0;
+};
+
+/* Thread t owns window starts [t*chunk, (t+1)*chunk) of the whole buffer. */
+template <typename Fn>
+__device__ void for_frames(const uint8_t *seq, uint64_t nbytes, uint32_t k, uint32_t chunk,
+                           const uint32_t *care, uint32_t S, Fn fn) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t p0 = t * chunk;
+    if (p0 >= nbytes)
+        return;
+    const uint64_t last = std::min<uint64_t>(p0 + chunk + k - 1, nbytes);
+    FrameCursor c;
+    for (uint64_t i = p0; i < last; ++i) {
+        const unsigned ch = seq[i];
+        if (!base_valid(ch)) {
+            c.run = 0;
+            c.fwd = c.rc = 0;
+            continue;
+        }
+        const unsigned code = base_code(ch);
+        if (c.run >= k) {
+            const unsigned out = unsigned(c.win >> (2 * (k - 1))) & 3u;
+            c.fwd = rotl1(c.fwd) ^ rotl64(seed_fwd(out), k) ^ seed_fwd(code);
+            c.rc = rotr1(c.rc ^ seed_rc(out) ^ rotl64(seed_rc(code), k));
+        } else {
+            c.fwd = rotl1(c.fwd) ^ seed_fwd(code);
+            c.rc = rotr1(c.rc ^ rotl64(seed_rc(code), k));
+            ++c.run;
+        }
+        c.win = (c.win << 2) | code;
+        if (c.run < k)
+            continue;
+        const uint64_t p = i + 1 - k;
+        for (uint32_t sd = 0; sd < S; ++sd)
+            fn(p, sd, seed_hash(seq, p, k, care[sd], c.fwd, c.rc));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mibf_bits(const uint8_t *seq, uint64_t nbytes,
+                                                   MibfDev d, uint64_t *bits, uint32_t chunk) {
+    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+               [&](uint64_t, uint32_t, uint64_t h) {
+                   const uint64_t p = fastmod(h, d.m, d.magic, d.sh1, d.sh2);
+                   atomicOr(reinterpret_cast<unsigned long long *>(bits) + (p >> 6),
+                            1ull << (p & 63));
+               });
+}
+
+/* lines: copy the words, leave the per-line popcount in word 0 for the scan */
+__global__ void k_mibf_lines(const uint64_t *bits, uint64_t nwords, uint64_t nlines,
+                             uint64_t *lines) {
+    for (uint64_t l = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; l < nlines;
+         l += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t pc = 0;
+        for (uint32_t s = 0; s < kWordsPerLine; ++s) {
+            const uint64_t w = l * kWordsPerLine + s;
+            const uint64_t v = w < nwords ? bits[w] : 0;
+            lines[l * 8 + 1 + s] = v;
+            pc += uint64_t(__popcll(v));
+        }
+        lines[l * 8] = pc;
+    }
+}
+
+/* exclusive scan of the per-line counts in place (one block, chunked) */
+__global__ __launch_bounds__(1024) void k_mibf_scan(uint64_t *lines, uint64_t nlines,
+                                                    unsigned long long *total) {
+    __shared__ unsigned long long part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (nlines + 1023) / 1024;
+    const uint64_t a = std::min<uint64_t>(nlines, t * per), b = std::min<uint64_t>(nlines, a + per);
+    unsigned long long s = 0;
+    for (uint64_t l = a; l < b; ++l)
+        s += lines[l * 8];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long run = 0;
+        for (uint32_t i = 0; i < 1024; ++i) {
+            const unsigned long long v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    unsigned long long run = part[t];
+    for (uint64_t l = a; l < b; ++l) {
+        const unsigned long long v = lines[l * 8];
+        lines[l * 8] = run;
+        run += v;
+    }
+}
+
+/* Records are separated by one non-ACGT byte; rec_of maps a window start to
+ * its record (binary search in the record start offsets). */
+__device__ __forceinline__ uint32_t record_of(const uint64_t *rec_off, uint64_t nrec,
+                                              uint64_t p) {
+    uint64_t lo = 0, hi = nrec;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (rec_off[mid] <= p)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return uint32_t(lo);
+}
+
+__global__ __launch_bounds__(256) void k_mibf_own(const uint8_t *seq, uint64_t nbytes,
+                                                  const uint64_t *rec_off, const uint32_t *rec_id,
+                                                  uint64_t nrec, MibfDev d,
+                                                  unsigned long long *best, uint32_t chunk) {
+    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+               [&](uint64_t p, uint32_t, uint64_t h) {
+                   const uint64_t id = rec_id[record_of(rec_off, nrec, p)];
+                   uint64_t rank;
+                   bv_rank(d.lines, fastmod(h, d.m, d.magic, d.sh1, d.sh2), rank);
+                   const unsigned long long key =
+                       (uint64_t(mix32(h ^ (id * 0x9E3779B97F4A7C15ULL))) << 32) | id;
+                   atomicMax(best + rank, key);
+               });
+}
+
+__global__ void k_mibf_settle(const unsigned long long *best, uint64_t n, uint32_t *data) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        data[i] = uint32_t(best[i]) & kIdMask;
+}
+
+/* saturation: a frame whose seed slots all hold other IDs marks them all */
+__global__ __launch_bounds__(256) void k_mibf_sat(const uint8_t *seq, uint64_t nbytes,
+                                                  const uint64_t *rec_off, const uint32_t *rec_id,
+                                                  uint64_t nrec, MibfDev d, uint32_t *data,
+                                                  uint32_t chunk) {
+    uint64_t pos_cur = ~uint64_t(0);
+    uint64_t ranks[kMaxSeeds];
+    bool rep = false;
+    uint32_t nseen = 0;
+    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+               [&](uint64_t p, uint32_t sd, uint64_t h) {
+                   if (p != pos_cur) {
+                       pos_cur = p;
+                       rep = false;
+                       nseen = 0;
+                   }
+                   const uint32_t id = rec_id[record_of(rec_off, nrec, p)];
+                   uint64_t rank;
+                   bv_rank(d.lines, fastmod(h, d.m, d.magic, d.sh1, d.sh2), rank);
+                   ranks[nseen++] = rank;
+                   if ((data[rank] & kIdMask) == id)
+                       rep = true;
+                   if (sd + 1 == d.S && !rep)
+                       for (uint32_t j = 0; j < nseen; ++j)
+                           atomicOr(data + ranks[j], kSatBit);
+               });
+}
+
+/* slots per ID (hist[id]) and saturated slots (hist[nids + 1]) */
+__global__ void k_mibf_hist(const uint32_t *data, uint64_t n, uint32_t nids,
+                            unsigned long long *hist) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t id = data[i] & kIdMask;
+        if (id <= nids)
+            atomicAdd(hist + id, 1ull);
+        if (data[i] & kSatBit)
+            atomicAdd(hist + nids + 1, 1ull);
+    }
+}
+
+/* every record must end with a byte that is not A/C/G/T (any case), so that
+ * no window spans two records; err[0] counts the records that do not */
+__global__ void k_mibf_check_sep(const uint8_t *seq, const uint64_t *rec_off, uint64_t nrec,
+                                 unsigned long long *err) {
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrec;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t a = rec_off[r], b = rec_off[r + 1];
+        if (b <= a || base_valid(seq[b - 1]))
+            atomicAdd(err, 1ull);
+    }
+}
+
+/* ---------------------------------------------------------------- query */
+
+struct QueryArgs {
+    MibfDev d;
+    const uint8_t *seq;
+    const uint64_t *off;
+    uint64_t nreads;
+    const uint32_t *min_count; /* [frameCount][nids + 1] */
+    uint32_t max_frames;       /* rows of min_count - 1 */
+    uint32_t allowed_miss;
+    double multi_thresh;
+    uint32_t min_nonsat;
+    uint32_t max_hits;
+    uint32_t *nhits;  /* per read */
+    uint32_t *hits;   /* per read max_hits x 6: id count nonsat total totalNonSat solid */
+    uint32_t *sat;    /* per read */
+    unsigned long long *counters; /* bv lookups, id gathers, overflow, unsupported */
+    unsigned long long *queue;
+};
+
+/* record of one ID in one frame, sortable by ID: id | occ | nonsat-occ */
+__device__ __forceinline__ uint64_t make_rec(uint32_t id, uint32_t occ, uint32_t ns) {
+    return (uint64_t(id) << 32) | (occ << 8) | ns;
+}
+
+template <uint32_t CAP>
+__global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
+    __shared__ uint64_t rec[CAP];
+    __shared__ uint32_t cand[64 * 6];
+    __shared__ uint32_t ncand_s;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t S = a.d.S, k = a.d.span;
+    unsigned long long bvl = 0, gat = 0;
+    while (true) {
+        unsigned long long it = 0;
+        if (lane == 0)
+            it = atomicAdd(a.queue, 1ull);
+        it = __shfl(it, 0);
+        if (it >= a.nreads)
+            break;
+        const uint64_t r = it;
+        const uint64_t b = a.off[r];
+        const uint32_t len = uint32_t(a.off[r + 1] - b);
+        const uint8_t *s = a.seq + b;
+        uint32_t satc = 0, nrec = 0;
+        bool over = false;
+        if (len >= k) {
+            uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
+            int lastN = -1;
+            for (uint32_t b0 = 0; b0 < len; b0 += 64) {
+                const uint32_t i = b0 + lane;
+                const unsigned c = i < len ? unsigned(s[i]) : 0u;
+                const bool ok = i < len && base_valid(c);
+                const unsigned code = base_code(c);
+                uint64_t H = ok ? pfx_fwd(code, i) : 0;
+                uint64_t G = ok ? pfx_rc(code, i) : 0;
+                int ln = ok ? -1 : int(i);
+                for (int dd = 1; dd < 64; dd <<= 1) {
+                    const uint64_t th = __shfl_up(H, unsigned(dd));
+                    const uint64_t tg = __shfl_up(G, unsigned(dd));
+                    const int tl = __shfl_up(ln, unsigned(dd));
+                    if (int(lane) >= dd) {
+                        H ^= th;
+                        G ^= tg;
+                        ln = ln > tl ? ln : tl;
+                    }
+                }
+                H ^= Hc;
+                G ^= Gc;
+                ln = ln > lastN ? ln : lastN;
+                const int src = int(lane) - int(k);
+                const uint64_t hb_cur = __shfl(H, src & 63), gb_cur = __shfl(G, src & 63);
+                const uint64_t hb_prev = __shfl(Hp, (src + 64) & 63);
+                const uint64_t gb_prev = __shfl(Gp, (src + 64) & 63);
+                const uint64_t Hb = src >= 0 ? hb_cur : hb_prev;
+                const uint64_t Gb = src >= 0 ? gb_cur : gb_prev;
+                const int64_t pw = int64_t(i) - int64_t(k) + 1;
+                const bool valid = pw >= 0 && i < len && int64_t(ln) < pw;
+                const uint64_t wf = pfx_win_fwd(H, Hb, i);
+                const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
+                Hc = __shfl(H, 63);
+                Gc = __shfl(G, 63);
+                lastN = __shfl(ln, 63);
+                Hp = H;
+                Gp = G;
+                /* per frame: seed slots, their IDs, distinct-ID records */
+                uint32_t raw[kMaxSeeds];
+                uint32_t nh = 0;
+                if (valid) {
+                    for (uint32_t sd = 0; sd < S; ++sd) {
+                        const uint64_t h = seed_hash(s, uint64_t(pw), k, a.d.care[sd], wf, wr);
+                        uint64_t rank;
+                        ++bvl;
+                        if (bv_rank(a.d.lines, fastmod(h, a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
+                                    rank)) {
+                            raw[nh++] = a.d.data[rank];
+                            ++gat;
+                        }
+                    }
+                }
+                const bool eval = valid && nh + a.allowed_miss >= S;
+                uint32_t myrec = 0;
+                uint64_t out[kMaxSeeds];
+                if (eval) {
+                    bool allsat = nh > 0;
+                    for (uint32_t x = 0; x < nh; ++x)
+                        if (!(raw[x] & kSatBit))
+                            allsat = false;
+                    satc += allsat ? 1 : 0;
+                    for (uint32_t x = 0; x < nh; ++x) {
+                        const uint32_t id = raw[x] & kIdMask;
+                        bool seen = false;
+                        for (uint32_t y = 0; y < x; ++y)
+                            if ((raw[y] & kIdMask) == id)
+                                seen = true;
+                        if (seen)
+                            continue;
+                        uint32_t occ = 0, ns = 0;
+                        for (uint32_t y = x; y < nh; ++y)
+                            if ((raw[y] & kIdMask) == id) {
+                                ++occ;
+                                if (!(raw[y] & kSatBit))
+                                    ++ns;
+                            }
+                        out[myrec++] = make_rec(id, occ, ns);
+                    }
+                }
+                /* append this chunk's records: exclusive scan of myrec */
+                uint32_t pre = myrec;
+                for (int dd = 1; dd < 64; dd <<= 1) {
+                    const uint32_t t = __shfl_up(pre, unsigned(dd));
+                    if (int(lane) >= dd)
+                        pre += t;
+                }
+                const uint32_t total = __shfl(pre, 63);
+                const uint32_t base = nrec + pre - myrec;
+                for (uint32_t x = 0; x < myrec; ++x)
+                    if (base + x < CAP)
+                        rec[base + x] = out[x];
+                nrec += total;
+            }
+        }
+        /* satCount: sum over lanes */
+        for (int o = 32; o > 0; o >>= 1)
+            satc += __shfl_xor(satc, o);
+        if (nrec > CAP) {
+            over = true;
+            nrec = 0;
+        }
+        __syncthreads();
+        /* bitonic sort of the records by key (IDs first) */
+        uint32_t n2 = 1;
+        while (n2 < nrec)
+            n2 <<= 1;
+        for (uint32_t x = nrec + lane; x < n2; x += 64)
+            rec[x] = ~uint64_t(0);
+        __syncthreads();
+        for (uint32_t sz = 2; sz <= n2; sz <<= 1) {
+            for (uint32_t st = sz >> 1; st > 0; st >>= 1) {
+                for (uint32_t x = lane; x < n2; x += 64) {
+                    const uint32_t y = x ^ st;
+                    if (y > x) {
+                        const bool up = (x & sz) == 0;
+                        const uint64_t vx = rec[x], vy = rec[y];
+                        if ((vx > vy) == up) {
+                            rec[x] = vy;
+                            rec[y] = vx;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        /* per-ID totals: each run of equal IDs is reduced by the lane at its head */
+        if (lane == 0)
+            ncand_s = 0;
+        __syncthreads();
+        const uint32_t fc = len >= k ? len - k + 1 : 0;
+        const uint32_t row = fc <= a.max_frames ? fc : 0;
+        for (uint32_t x = lane; x < nrec; x += 64) {
+            const uint32_t id = uint32_t(rec[x] >> 32);
+            if (x > 0 && uint32_t(rec[x - 1] >> 32) == id)
+                continue;
+            uint32_t cnt = 0, nsat = 0, tot = 0, tns = 0, solid = 0;
+            for (uint32_t y = x; y < nrec && uint32_t(rec[y] >> 32) == id; ++y) {
+                const uint32_t occ = uint32_t(rec[y] >> 8) & 0xFF, ns = uint32_t(rec[y]) & 0xFF;
+                cnt += 1;
+                nsat += ns ? 1 : 0;
+                tot += occ;
+                tns += ns;
+                solid += occ == S ? 1 : 0;
+            }
+            const uint32_t mc = (id <= a.d.nids && row) ? a.min_count[uint64_t(row) * (a.d.nids + 1) + id]
+                                                        : 0xFFFFFFFFu;
+            if (cnt >= mc && nsat >= a.min_nonsat) {
+                const uint32_t slot = atomicAdd(&ncand_s, 1u);
+                if (slot < 64) {
+                    uint32_t *c6 = cand + slot * 6;
+                    c6[0] = id;
+                    c6[1] = cnt;
+                    c6[2] = nsat;
+                    c6[3] = tot;
+                    c6[4] = tns;
+                    c6[5] = solid;
+                }
+            }
+        }
+        __syncthreads();
+        /* ranking and output by lane 0 (few candidates) */
+        if (lane == 0) {
+            uint32_t nc = ncand_s;
+            if (nc > 64) {
+                over = true;
+                nc = 64;
+            }
+            uint32_t order[64];
+            for (uint32_t x = 0; x < nc; ++x)
+                order[x] = x;
+            for (uint32_t x = 1; x < nc; ++x) { /* insertion sort, rank order */
+                const uint32_t v = order[x];
+                const uint32_t *cv = cand + v * 6;
+                uint32_t y = x;
+                while (y > 0) {
+                    const uint32_t *cu = cand + order[y - 1] * 6;
+                    bool before;
+                    if (cv[2] != cu[2]) before = cv[2] > cu[2];
+                    else if (cv[1] != cu[1]) before = cv[1] > cu[1];
+                    else if (cv[5] != cu[5]) before = cv[5] > cu[5];
+                    else if (cv[4] != cu[4]) before = cv[4] > cu[4];
+                    else if (cv[3] != cu[3]) before = cv[3] > cu[3];
+                    else before = cv[0] < cu[0];
+                    if (!before)
+                        break;
+                    order[y] = order[y - 1];
+                    --y;
+                }
+                order[y] = v;
+            }
+            uint32_t nout = 0;
+            if (nc) {
+                const double bar = a.multi_thresh * double(cand[order[0] * 6 + 2]);
+                for (uint32_t x = 0; x < nc; ++x) {
+                    const uint32_t *cv = cand + order[x] * 6;
+                    if (double(cv[2]) < bar)
+                        break;
+                    if (nout < a.max_hits)
+                        for (uint32_t f = 0; f < 6; ++f)
+                            a.hits[(r * a.max_hits + nout) * 6 + f] = cv[f];
+                    else
+                        atomicAdd(a.counters + 2, 1ull);
+                    ++nout;
+                }
+            }
+            a.nhits[r] = over ? 0xFFFFFFFFu : nout;
+            a.sat[r] = satc;
+            if (over)
+                atomicAdd(a.counters + 3, 1ull);
+        }
+        __syncthreads();
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        bvl += __shfl_xor(bvl, o);
+        gat += __shfl_xor(gat, o);
+    }
+    if (lane == 0) {
+        atomicAdd(a.counters + 0, bvl);
+        atomicAdd(a.counters + 1, gat);
+    }
+}
+
+__global__ void k_mibf_zero(unsigned long long *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        p[i] = 0;
+}
+
+} // namespace
+
+/* ======================================================================= */
+
+struct bbt_mibf {
+    MibfDev d;
+    uint64_t *dlines = nullptr;
+    uint32_t *ddata = nullptr;
+    uint64_t nlines = 0, pop = 0, pop_sat = 0;
+    std::vector<std::string> seeds;
+    std::vector<std::string> names;  /* names[0] = "repeat", then IDs 1..n */
+    std::vector<uint64_t> id_slots;  /* slots per ID (index = ID) */
+    /* minCount table for (score, allowed_miss), rows 0..min_frames */
+    uint32_t *dmin = nullptr;
+    double min_score = -1;
+    uint32_t min_miss = 0xFFFFFFFFu, min_frames = 0;
+    unsigned long long *dctl = nullptr; /* [0] scan total, [1] separator errors, [8] queue */
+    int cu = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds) {
+    if (!seeds || nseeds == 0 || nseeds > kMaxSeeds)
+        return mfail(BBT_ERR_ARG, "MIBF: 1..8 spaced seeds required");
+    const size_t span = std::strlen(seeds[0]);
+    if (span == 0 || span > 32)
+        return mfail(BBT_ERR_UNSUPPORTED, "MIBF: seed span must be 1..32");
+    std::memset(&mf->d, 0, sizeof(mf->d));
+    for (uint32_t i = 0; i < nseeds; ++i) {
+        const std::string sd = seeds[i];
+        if (sd.size() != span)
+            return mfail(BBT_ERR_ARG, "MIBF: all seeds must have the same span");
+        uint32_t care = 0;
+        for (size_t j = 0; j < span; ++j) {
+            if (sd[j] != '0' && sd[j] != '1')
+                return mfail(BBT_ERR_ARG, "MIBF: seeds are strings of 0 and 1");
+            if (sd[j] == '1')
+                care |= 1u << j;
+        }
+        mf->d.care[i] = care;
+        mf->seeds.push_back(sd);
+    }
+    mf->d.S = nseeds;
+    mf->d.span = uint32_t(span);
+    int dev = 0;
+    MHIP(hipGetDevice(&dev));
+    MHIP(hipDeviceGetAttribute(&mf->cu, hipDeviceAttributeMultiprocessorCount, dev));
+    MHIP(hipMalloc(&mf->dctl, 16 * sizeof(unsigned long long)));
+    return BBT_OK;
+}
+
+void set_m(bbt_mibf *mf, uint64_t m) {
+    mf->d.m = m;
+    const FastMod fm = fastmod_make(m);
+    mf->d.magic = fm.magic;
+    mf->d.sh1 = fm.sh1;
+    mf->d.sh2 = fm.sh2;
+    mf->nlines = (m / 64 + kWordsPerLine - 1) / kWordsPerLine;
+}
+
+/* slot statistics from the device ID array */
+bbt_status mibf_stats(bbt_mibf *mf, hipStream_t st) {
+    const uint32_t n = mf->d.nids;
+    unsigned long long *hist = nullptr;
+    MHIP(hipMalloc(&hist, (size_t(n) + 2) * 8));
+    bool ok = hipMemsetAsync(hist, 0, (size_t(n) + 2) * 8, st) == hipSuccess;
+    if (ok && mf->pop)
+        k_mibf_hist<<<4096, 256, 0, st>>>(mf->ddata, mf->pop, n, hist);
+    std::vector<uint64_t> h(size_t(n) + 2, 0);
+    ok = ok && hipMemcpyAsync(h.data(), hist, h.size() * 8, hipMemcpyDeviceToHost, st) ==
+                   hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    (void)hipFree(hist);
+    if (!ok)
+        return mfail(BBT_ERR_HIP, "MIBF: slot statistics failed");
+    mf->id_slots.assign(h.begin(), h.begin() + n + 1);
+    mf->pop_sat = h[n + 1];
+    return BBT_OK;
+}
+
+constexpr char kMagic[8] = {'B', 'B', 'T', 'M', 'I', 'B', 'F', '1'};
+
+struct MibfHeader {
+    char magic[8];
+    uint32_t nseeds, span, nids, reserved;
+    uint64_t m, pop;
+};
+static_assert(sizeof(MibfHeader) == 40, "MibfHeader layout");
+
+} // namespace
+
+extern "C" {
+
+uint64_t bbt_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy) {
+    const uint64_t v = uint64_t(-double(entries) * double(nseeds) / std::log(1.0 - occupancy));
+    return v + (64 - v % 64);
+}
+
+bbt_status bbt_mibf_free(bbt_mibf *mf) {
+    if (!mf)
+        return BBT_OK;
+    (void)hipFree(mf->dlines);
+    (void)hipFree(mf->ddata);
+    (void)hipFree(mf->dmin);
+    (void)hipFree(mf->dctl);
+    delete mf;
+    return BBT_OK;
+}
+
+bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint32_t *drec_id,
+                          const uint64_t *rec_off, uint64_t nrec, const char *const *names,
+                          uint32_t nids, const char *const *seeds, uint32_t nseeds,
+                          double occupancy, uint64_t entries, uint64_t m_override,
+                          bbt_mibf **out, void *stream) {
+    if (!out || !dseq || !drec_off || !drec_id || !rec_off || nrec == 0 || nids == 0)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_build: bad argument");
+    if (nids >= kIdMask)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_build: IDs must fit in 31 bits");
+    if (!(occupancy > 0 && occupancy < 1) && !m_override)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_build: occupancy must be in (0, 1)");
+    *out = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    bbt_mibf *mf = new bbt_mibf();
+    bbt_status s = mibf_init(mf, seeds, nseeds);
+    if (s != BBT_OK) {
+        bbt_mibf_free(mf);
+        return s;
+    }
+    const uint32_t k = mf->d.span;
+    const uint64_t nbytes = rec_off[nrec];
+    if (!entries) /* windows per record, as MIBFGen counts them (len - k + 1) */
+        for (uint64_t r = 0; r < nrec; ++r) {
+            const uint64_t len = rec_off[r + 1] - rec_off[r] - 1;
+            entries += len >= k ? len - k + 1 : 0;
+        }
+    const uint64_t m = m_override ? m_override : bbt_mibf_calc_size(entries, nseeds, occupancy);
+    if (m == 0 || m % 64) {
+        bbt_mibf_free(mf);
+        return mfail(BBT_ERR_ARG, "bbt_mibf_build: m must be a positive multiple of 64");
+    }
+    set_m(mf, m);
+    mf->d.nids = nids;
+    mf->names.push_back("repeat");
+    for (uint32_t i = 0; i < nids; ++i)
+        mf->names.push_back(names && names[i] ? names[i] : std::to_string(i + 1));
+    uint64_t *bits = nullptr;
+    unsigned long long *best = nullptr;
+    auto done = [&](bbt_status e) {
+        (void)hipFree(bits);
+        (void)hipFree(best);
+        if (e != BBT_OK)
+            bbt_mibf_free(mf);
+        else
+            *out = mf;
+        return e;
+    };
+    const uint8_t *seq = reinterpret_cast<const uint8_t *>(dseq);
+    const uint32_t chunk = 256;
+    const uint64_t threads = (nbytes + chunk - 1) / chunk;
+    const uint32_t blocks = uint32_t(std::max<uint64_t>(1, (threads + 255) / 256));
+    unsigned long long ctl[2] = {0, 0};
+    if (hipMemsetAsync(mf->dctl, 0, 16 * 8, st) != hipSuccess)
+        return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: memset failed"));
+    k_mibf_check_sep<<<64, 256, 0, st>>>(seq, drec_off, nrec, mf->dctl + 1);
+    if (hipMalloc(&bits, (m / 64) * 8) != hipSuccess ||
+        hipMalloc(&mf->dlines, mf->nlines * 64) != hipSuccess)
+        return done(mfail(BBT_ERR_NOMEM, "bbt_mibf_build: bit vector allocation failed"));
+    mf->d.lines = mf->dlines;
+    if (hipMemsetAsync(bits, 0, (m / 64) * 8, st) != hipSuccess)
+        return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: memset failed"));
+    /* pass 1: bit vector (MIBFGen.hpp:144 generateBV) */
+    k_mibf_bits<<<blocks, 256, 0, st>>>(seq, nbytes, mf->d, bits, chunk);
+    k_mibf_lines<<<4096, 256, 0, st>>>(bits, m / 64, mf->nlines, mf->dlines);
+    k_mibf_scan<<<1, 1024, 0, st>>>(mf->dlines, mf->nlines, mf->dctl);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(ctl, mf->dctl, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: bit vector pass failed"));
+    if (ctl[1])
+        return done(mfail(BBT_ERR_ARG, "bbt_mibf_build: " + std::to_string(ctl[1]) +
+                                           " record(s) do not end with a separator byte"));
+    (void)hipFree(bits);
+    bits = nullptr;
+    mf->pop = ctl[0];
+    const uint64_t slots = std::max<uint64_t>(1, mf->pop);
+    if (hipMalloc(&mf->ddata, slots * 4) != hipSuccess ||
+        hipMalloc(&best, slots * 8) != hipSuccess)
+        return done(mfail(BBT_ERR_NOMEM, "bbt_mibf_build: ID array allocation failed"));
+    mf->d.data = mf->ddata;
+    if (hipMemsetAsync(best, 0, slots * 8, st) != hipSuccess)
+        return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: memset failed"));
+    /* pass 2: IDs (MIBFGen.hpp:155-230 insertMIBF), order-independent ownership */
+    k_mibf_own<<<blocks, 256, 0, st>>>(seq, nbytes, drec_off, drec_id, nrec, mf->d, best, chunk);
+    k_mibf_settle<<<4096, 256, 0, st>>>(best, mf->pop, mf->ddata);
+    /* pass 3: saturation (MIBFGen.hpp:200-262 insertSaturation) */
+    k_mibf_sat<<<blocks, 256, 0, st>>>(seq, nbytes, drec_off, drec_id, nrec, mf->d, mf->ddata,
+                                       chunk);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: ID passes failed"));
+    return done(mibf_stats(mf, st));
+}
+
+bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info) {
+    if (!mf || !info)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_info: null argument");
+    info->m = mf->d.m;
+    info->pop = mf->pop;
+    info->pop_saturated = mf->pop_sat;
+    info->nids = mf->d.nids;
+    info->nseeds = mf->d.S;
+    info->span = mf->d.span;
+    info->reserved = 0;
+    return BBT_OK;
+}
+
+const char *bbt_mibf_name(const bbt_mibf *mf, uint32_t id) {
+    if (!mf || id >= mf->names.size())
+        return nullptr;
+    return mf->names[id].c_str();
+}
+
+const char *bbt_mibf_seed(const bbt_mibf *mf, uint32_t i) {
+    if (!mf || i >= mf->seeds.size())
+        return nullptr;
+    return mf->seeds[i].c_str();
+}
+
+bbt_status bbt_mibf_download(const bbt_mibf *mf, uint64_t *bv, uint32_t *data) {
+    if (!mf)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_download: null");
+    if (bv) {
+        std::vector<uint64_t> lines(mf->nlines * 8);
+        MHIP(hipMemcpy(lines.data(), mf->dlines, lines.size() * 8, hipMemcpyDeviceToHost));
+        for (uint64_t w = 0; w < mf->d.m / 64; ++w)
+            bv[w] = lines[(w / kWordsPerLine) * 8 + 1 + w % kWordsPerLine];
+    }
+    if (data && mf->pop)
+        MHIP(hipMemcpy(data, mf->ddata, mf->pop * 4, hipMemcpyDeviceToHost));
+    return BBT_OK;
+}
+
+bbt_status bbt_mibf_store(const bbt_mibf *mf, const char *prefix) {
+    if (!mf || !prefix)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_store: null argument");
+    std::vector<uint64_t> bv(mf->d.m / 64);
+    std::vector<uint32_t> data(mf->pop);
+    bbt_status s = bbt_mibf_download(mf, bv.data(), data.data());
+    if (s != BBT_OK)
+        return s;
+    const std::string base(prefix);
+    std::ofstream f(base + ".bf", std::ios::binary);
+    MibfHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kMagic, 8);
+    h.nseeds = mf->d.S;
+    h.span = mf->d.span;
+    h.nids = mf->d.nids;
+    h.m = mf->d.m;
+    h.pop = mf->pop;
+    f.write(reinterpret_cast<const char *>(&h), sizeof(h));
+    for (const std::string &sd : mf->seeds)
+        f.write(sd.data(), std::streamsize(sd.size()));
+    f.write(reinterpret_cast<const char *>(bv.data()), std::streamsize(bv.size() * 8));
+    f.write(reinterpret_cast<const char *>(data.data()), std::streamsize(data.size() * 4));
+    if (!f)
+        return mfail(BBT_ERR_IO, "bbt_mibf_store: cannot write " + base + ".bf");
+    /* ID file (MIBFGen.hpp:424-430): "<id>\t<name>" per line */
+    std::ofstream ids(base + "_ids.txt");
+    for (uint32_t i = 1; i < mf->names.size(); ++i)
+        ids << i << "\t" << mf->names[i] << "\n";
+    if (!ids)
+        return mfail(BBT_ERR_IO, "bbt_mibf_store: cannot write " + base + "_ids.txt");
+    return BBT_OK;
+}
+
+bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream) {
+    if (!path || !out)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_load: null argument");
+    *out = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return mfail(BBT_ERR_IO, std::string("bbt_mibf_load: cannot open ") + path);
+    MibfHeader h;
+    f.read(reinterpret_cast<char *>(&h), sizeof(h));
+    if (!f || std::memcmp(h.magic, kMagic, 8) != 0 || h.nseeds == 0 || h.nseeds > kMaxSeeds ||
+        h.span == 0 || h.span > 32 || h.m == 0 || h.m % 64 || h.pop > h.m || h.nids == 0)
+        return mfail(BBT_ERR_FORMAT, std::string("bbt_mibf_load: not a multi-index filter: ") + path);
+    std::vector<std::string> seeds(h.nseeds, std::string(h.span, '0'));
+    for (std::string &sd : seeds)
+        f.read(&sd[0], h.span);
+    std::vector<uint64_t> bv(h.m / 64);
+    std::vector<uint32_t> data(h.pop);
+    f.read(reinterpret_cast<char *>(bv.data()), std::streamsize(bv.size() * 8));
+    f.read(reinterpret_cast<char *>(data.data()), std::streamsize(data.size() * 4));
+    if (!f)
+        return mfail(BBT_ERR_FORMAT, std::string("bbt_mibf_load: truncated file ") + path);
+    uint64_t pop = 0;
+    for (uint64_t w : bv)
+        pop += uint64_t(__builtin_popcountll(w));
+    if (pop != h.pop)
+        return mfail(BBT_ERR_FORMAT, "bbt_mibf_load: popcount does not match the header");
+    /* names from the ID file beside the filter: <prefix>_ids.txt for <prefix>.bf
+     * (MIBFClassifier.hpp:52-76) */
+    std::string p(path);
+    const std::string stem = p.size() > 3 && p.compare(p.size() - 3, 3, ".bf") == 0
+                                 ? p.substr(0, p.size() - 3)
+                                 : p;
+    std::ifstream idf(stem + "_ids.txt");
+    if (!idf)
+        return mfail(BBT_ERR_IO, "bbt_mibf_load: " + stem + "_ids.txt cannot be opened; "
+                                 "a corresponding id file is needed");
+    bbt_mibf *mf = new bbt_mibf();
+    std::vector<const char *> sp;
+    for (const std::string &sd : seeds)
+        sp.push_back(sd.c_str());
+    bbt_status s = mibf_init(mf, sp.data(), h.nseeds);
+    if (s != BBT_OK) {
+        bbt_mibf_free(mf);
+        return s;
+    }
+    set_m(mf, h.m);
+    mf->d.nids = h.nids;
+    mf->pop = h.pop;
+    mf->names.assign(h.nids + 1, std::string());
+    mf->names[0] = "repeat";
+    std::string line;
+    while (std::getline(idf, line)) {
+        const size_t tab = line.find_first_of(" \t");
+        if (tab == std::string::npos)
+            continue;
+        const unsigned long id = std::strtoul(line.c_str(), nullptr, 10);
+        size_t b = line.find_first_not_of(" \t", tab);
+        if (id >= 1 && id <= h.nids && b != std::string::npos)
+            mf->names[id] = line.substr(b, line.find_first_of(" \t", b) - b);
+    }
+    std::vector<uint64_t> lines(mf->nlines * 8, 0);
+    uint64_t run = 0;
+    for (uint64_t l = 0; l < mf->nlines; ++l) {
+        lines[l * 8] = run;
+        for (uint32_t t = 0; t < kWordsPerLine; ++t) {
+            const uint64_t w = l * kWordsPerLine + t;
+            const uint64_t v = w < bv.size() ? bv[w] : 0;
+            lines[l * 8 + 1 + t] = v;
+            run += uint64_t(__builtin_popcountll(v));
+        }
+    }
+    if (hipMalloc(&mf->dlines, lines.size() * 8) != hipSuccess ||
+        hipMalloc(&mf->ddata, std::max<uint64_t>(1, h.pop) * 4) != hipSuccess) {
+        bbt_mibf_free(mf);
+        return mfail(BBT_ERR_NOMEM, "bbt_mibf_load: device allocation failed");
+    }
+    mf->d.lines = mf->dlines;
+    mf->d.data = mf->ddata;
+    if (hipMemcpyAsync(mf->dlines, lines.data(), lines.size() * 8, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        (h.pop && hipMemcpyAsync(mf->ddata, data.data(), h.pop * 4, hipMemcpyHostToDevice, st) !=
+                      hipSuccess)) {
+        bbt_mibf_free(mf);
+        return mfail(BBT_ERR_HIP, "bbt_mibf_load: upload failed");
+    }
+    s = mibf_stats(mf, st);
+    if (s != BBT_OK) {
+        bbt_mibf_free(mf);
+        return s;
+    }
+    *out = mf;
+    return BBT_OK;
+}
+
+bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, double *probs) {
+    if (!mf || !probs)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_frame_probs: null argument");
+    const uint32_t S = mf->d.S;
+    const double occ = double(mf->pop) / double(mf->d.m);
+    probs[0] = 0;
+    for (uint32_t id = 1; id <= mf->d.nids; ++id) {
+        const double f = mf->pop ? double(mf->id_slots[id]) / double(mf->pop) : 0.0;
+        double P = 0;
+        const uint32_t jmin = allowed_miss >= S ? 0 : S - allowed_miss;
+        for (uint32_t j = jmin; j <= S; ++j) {
+            double c = 1;
+            for (uint32_t t = 0; t < j; ++t)
+                c = c * double(S - t) / double(t + 1);
+            P += c * std::pow(occ, double(j)) * std::pow(1 - occ, double(S - j)) *
+                 (1 - std::pow(1 - f, double(j)));
+        }
+        probs[id] = P;
+    }
+    return BBT_OK;
+}
+
+uint32_t bbt_mibf_min_count(uint64_t frames, double prob, double score, uint32_t nids) {
+    const uint64_t q = binom_isf_up(score / double(nids), frames, prob);
+    return q > 1 ? uint32_t(q) : 1u;
+}
+
+bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mibf_params *p,
+                                bbt_mibf_out *out, void *stream) {
+    if (!mf || !in || !p || !out || !out->nhits || !out->hits || !out->sat || !out->counters)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: bad argument");
+    if (in->nreads == 0)
+        return BBT_OK;
+    if (p->max_hits == 0 || p->max_hits > 64)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..64");
+    if (in->max_len > (1u << 20))
+        return mfail(BBT_ERR_UNSUPPORTED, "bbt_mibf_query_batch: reads longer than 1 Mbp");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(mf->mu);
+    const uint32_t k = mf->d.span;
+    const uint32_t frames = in->max_len >= k ? in->max_len - k + 1 : 0;
+    if (uint64_t(frames) * mf->d.S > 8192)
+        return mfail(BBT_ERR_UNSUPPORTED,
+                     "bbt_mibf_query_batch: frames x seeds of the longest read exceed 8192");
+    const uint32_t miss = p->allowed_miss;
+    if (!mf->dmin || mf->min_score != p->score || mf->min_miss != miss ||
+        mf->min_frames < frames) {
+        /* getMinCount (MIBFClassifier.hpp:930-939) for every frame count up to
+         * the longest read of the batch */
+        std::vector<double> probs(mf->d.nids + 1);
+        bbt_mibf_frame_probs(mf, miss, probs.data());
+        const uint32_t cols = mf->d.nids + 1;
+        std::vector<uint32_t> t(size_t(frames + 1) * cols, 1);
+        for (uint32_t fc = 1; fc <= frames; ++fc)
+            for (uint32_t id = 1; id <= mf->d.nids; ++id)
+                t[size_t(fc) * cols + id] = bbt_mibf_min_count(fc, probs[id], p->score, mf->d.nids);
+        (void)hipFree(mf->dmin);
+        mf->dmin = nullptr;
+        MHIP(hipMalloc(&mf->dmin, t.size() * 4));
+        MHIP(hipMemcpy(mf->dmin, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+        mf->min_score = p->score;
+        mf->min_miss = miss;
+        mf->min_frames = frames;
+    }
+    QueryArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.d = mf->d;
+    a.seq = reinterpret_cast<const uint8_t *>(in->seq);
+    a.off = in->off;
+    a.nreads = in->nreads;
+    a.min_count = mf->dmin;
+    a.max_frames = mf->min_frames;
+    a.allowed_miss = miss;
+    a.multi_thresh = p->multi_thresh;
+    a.min_nonsat = p->min_nonsat;
+    a.max_hits = p->max_hits;
+    a.nhits = out->nhits;
+    a.hits = out->hits;
+    a.sat = out->sat;
+    a.counters = reinterpret_cast<unsigned long long *>(out->counters);
+    a.queue = mf->dctl + 8;
+    k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 1);
+    const uint64_t recs = uint64_t(frames) * mf->d.S;
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(in->nreads, uint64_t(mf->cu) * 16));
+    if (recs <= 1024)
+        k_mibf_query<1024><<<blocks, 64, 0, st>>>(a);
+    else if (recs <= 4096)
+        k_mibf_query<4096><<<blocks, 64, 0, st>>>(a);
+    else
+        k_mibf_query<8192><<<blocks, 64, 0, st>>>(a);
+    MHIP(hipGetLastError());
+    return BBT_OK;
+}
+
+} // extern "C"

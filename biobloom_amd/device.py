@@ -113,3 +113,64 @@ def classify(fs, dreads, params, want_scores=None, stream=None, out=None):
     fs.eval_batch(dreads.struct(), params, out.struct(), _stream_handle(stream))
     torch.cuda.synchronize()
     return out.host()
+
+
+def pack_records(records, sep=b"N"):
+    """Reference records (bytes/str) packed back to back, each followed by one
+    separator byte, as bbt_mibf_build takes them: (uint8 buffer, offsets)."""
+    parts, off = [], [0]
+    for r in records:
+        b = r.encode() if isinstance(r, str) else bytes(r)
+        parts.append(b + sep)
+        off.append(off[-1] + len(b) + 1)
+    buf = np.frombuffer(b"".join(parts), np.uint8) if parts else np.zeros(0, np.uint8)
+    return buf, np.asarray(off, np.uint64)
+
+
+def build_mibf(records, ids, names, seeds, occupancy=0.5, m=0, device="cuda", stream=None):
+    """biobloommimaker on the device: records (str/bytes) with their IDs
+    (1..len(names)), one ID per record or several records per ID."""
+    buf, off = pack_records(records)
+    dseq = torch.from_numpy(buf.copy()).to(device)
+    doff = torch.from_numpy(off.view(np.int64)).to(device)
+    did = torch.from_numpy(np.asarray(ids, np.uint32).view(np.int32)).to(device)
+    mf = B.Mibf.build(dseq.data_ptr(), doff.data_ptr(), did.data_ptr(), off.tolist(), names,
+                      seeds, occupancy=occupancy, m=m, stream=_stream_handle(stream))
+    torch.cuda.synchronize()
+    return mf
+
+
+class MibfOutputs:
+    def __init__(self, nreads, max_hits, device="cuda"):
+        n = max(1, nreads)
+        self.nhits = torch.zeros(n, dtype=torch.int32, device=device)
+        self.hits = torch.zeros(n * max_hits * 6, dtype=torch.int32, device=device)
+        self.sat = torch.zeros(n, dtype=torch.int32, device=device)
+        self.counters = torch.zeros(4, dtype=torch.int64, device=device)
+        self.nreads, self.max_hits = nreads, max_hits
+
+    def struct(self):
+        return B.MibfOut(self.nhits.data_ptr(), self.hits.data_ptr(), self.sat.data_ptr(),
+                         self.counters.data_ptr())
+
+    def host(self):
+        """(per read list of hit tuples in rank order, per read satCount,
+        counters, raw nhits)"""
+        n, mh = self.nreads, self.max_hits
+        nh = self.nhits[:n].cpu().numpy().view(np.uint32)
+        hits = self.hits[: n * mh * 6].cpu().numpy().view(np.uint32).reshape(n, mh, 6)
+        rows = []
+        for r in range(n):
+            c = int(nh[r]) if nh[r] != 0xFFFFFFFF else 0
+            rows.append([tuple(int(x) for x in hits[r, j]) for j in range(min(c, mh))])
+        return (rows, self.sat[:n].cpu().numpy().view(np.uint32).tolist(),
+                self.counters.cpu().numpy().astype(np.uint64), nh.copy())
+
+
+def mibf_classify(mf, dreads, params, stream=None, out=None):
+    if out is None:
+        out = MibfOutputs(dreads.nreads, params.max_hits, dreads.seq.device)
+    out.counters.zero_()
+    mf.query_batch(dreads.struct(), params, out.struct(), _stream_handle(stream))
+    torch.cuda.synchronize()
+    return out.host()

@@ -51,6 +51,23 @@ class EvalOut(C.Structure):
                 ("summary", C.c_void_p), ("counters", C.c_void_p)]
 
 
+class MibfInfo(C.Structure):
+    _fields_ = [("m", C.c_uint64), ("pop", C.c_uint64), ("pop_saturated", C.c_uint64),
+                ("nids", C.c_uint32), ("nseeds", C.c_uint32), ("span", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class MibfParams(C.Structure):
+    _fields_ = [("score", C.c_double), ("multi_thresh", C.c_double),
+                ("allowed_miss", C.c_uint32), ("min_nonsat", C.c_uint32),
+                ("max_hits", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class MibfOut(C.Structure):
+    _fields_ = [("nhits", C.c_void_p), ("hits", C.c_void_p), ("sat", C.c_void_p),
+                ("counters", C.c_void_p)]
+
+
 # exported symbols and their signatures: (restype, argtypes)
 _SIGS = {
     "bbt_last_error": (C.c_char_p, []),
@@ -87,6 +104,24 @@ _SIGS = {
                                      C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bbt_calc_min_count": (C.c_uint64, [C.c_uint64, C.c_double, C.c_double]),
     "bbt_binomial_score": (C.c_double, [C.c_uint64, C.c_uint64, C.c_double]),
+    "bbt_mibf_build": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_uint64), C.c_uint64,
+                                   C.POINTER(C.c_char_p), C.c_uint32,
+                                   C.POINTER(C.c_char_p), C.c_uint32, C.c_double,
+                                   C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p),
+                                   C.c_void_p]),
+    "bbt_mibf_store": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "bbt_mibf_load": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p]),
+    "bbt_mibf_free": (C.c_int32, [C.c_void_p]),
+    "bbt_mibf_info": (C.c_int32, [C.c_void_p, C.POINTER(MibfInfo)]),
+    "bbt_mibf_name": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "bbt_mibf_seed": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "bbt_mibf_download": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bbt_mibf_calc_size": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_double]),
+    "bbt_mibf_frame_probs": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
+    "bbt_mibf_min_count": (C.c_uint32, [C.c_uint64, C.c_double, C.c_double, C.c_uint32]),
+    "bbt_mibf_query_batch": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(MibfParams),
+                                         C.POINTER(MibfOut), C.c_void_p]),
 }
 
 _lib = None
@@ -235,3 +270,77 @@ def params(mode=STD, scorer=SIMPLE, threshold=0.15, streak=3, paired=False,
            inclusive=False, path=PATH_BATCHED, flags=0, wave_min_len=0):
     return EvalParams(mode, scorer, threshold, streak, int(paired), int(inclusive), path,
                       flags, wave_min_len)
+
+
+class Mibf:
+    """A multi-index Bloom filter resident in device memory (bbt_mibf)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def build(cls, dseq_ptr, drec_off_ptr, drec_id_ptr, rec_off, names, seeds,
+              occupancy=0.5, entries=0, m=0, stream=0):
+        """rec_off: host sequence of nrec + 1 offsets; every record ends with
+        one non-ACGT separator byte."""
+        L = load()
+        nrec = len(rec_off) - 1
+        off = (C.c_uint64 * len(rec_off))(*[int(x) for x in rec_off])
+        nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        sd = (C.c_char_p * len(seeds))(*[x.encode() for x in seeds])
+        h = C.c_void_p()
+        check(L.bbt_mibf_build(C.c_void_p(dseq_ptr), C.c_void_p(drec_off_ptr),
+                               C.c_void_p(drec_id_ptr), off, nrec, nm, len(names), sd,
+                               len(seeds), occupancy, entries, m, C.byref(h),
+                               C.c_void_p(stream)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path, stream=0):
+        h = C.c_void_p()
+        check(load().bbt_mibf_load(path.encode(), C.byref(h), C.c_void_p(stream)))
+        return cls(h.value)
+
+    def close(self):
+        if self._h:
+            load().bbt_mibf_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = MibfInfo()
+        check(load().bbt_mibf_info(self._h, C.byref(out)))
+        return out
+
+    def name(self, i):
+        r = load().bbt_mibf_name(self._h, i)
+        return None if r is None else r.decode()
+
+    def seeds(self):
+        n = self.info().nseeds
+        return [load().bbt_mibf_seed(self._h, i).decode() for i in range(n)]
+
+    def download(self, bv, data):
+        check(load().bbt_mibf_download(self._h, bv.ctypes.data, data.ctypes.data))
+
+    def store(self, prefix):
+        check(load().bbt_mibf_store(self._h, prefix.encode()))
+
+    def frame_probs(self, allowed_miss):
+        n = self.info().nids + 1
+        arr = (C.c_double * n)()
+        check(load().bbt_mibf_frame_probs(self._h, allowed_miss, arr))
+        return list(arr)
+
+    def query_batch(self, reads, params, out, stream=0):
+        check(load().bbt_mibf_query_batch(self._h, C.byref(reads), C.byref(params),
+                                          C.byref(out), C.c_void_p(stream)))
+
+
+def mibf_params(score=0.1, multi_thresh=1.0, allowed_miss=0, min_nonsat=1, max_hits=8):
+    return MibfParams(score, multi_thresh, allowed_miss, min_nonsat, max_hits, 0)

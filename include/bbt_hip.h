@@ -26,6 +26,20 @@
  *   bbt_calc_optimal_size / bbt_calc_optimal_hashnum
  *                      BloomFilterInfo::calcOptimalSize / calcOptimalHashNum
  *                      (Common/BloomFilterInfo.h:57-76)
+ *   bbt_mibf_build     MIBFGen::generate (BioBloomMaker/MIBFGen.hpp:131-309):
+ *                      bit-vector pass, ID pass, saturation pass
+ *   bbt_mibf_store / bbt_mibf_load
+ *                      MIBloomFilter::store + MIBFGen::writeIDs
+ *                      (MIBFGen.hpp:284-292, 424-430); MIBFClassifier
+ *                      constructor (BioBloomCategorizer/MIBFClassifier.hpp:
+ *                      49-95), which reads <prefix>.bf and <prefix>_ids.txt
+ *   bbt_mibf_frame_probs
+ *                      MIBloomFilter::calcFrameProbs (MIBFClassifier.hpp:86-87)
+ *   bbt_mibf_min_count MIBFClassifier::getMinCount (MIBFClassifier.hpp:930-939)
+ *   bbt_mibf_query_batch
+ *                      per read, MIBFClassifier::classify (MIBFClassifier.hpp:
+ *                      773-796): MIBFQuerySupport::query over the read's
+ *                      spaced-seed frames, best hit first
  */
 #ifndef BBT_HIP_H
 #define BBT_HIP_H
@@ -193,6 +207,76 @@ bbt_status bbt_summary_text(const bbt_filterset *fs, const uint64_t *summary,
 /* Binomial helpers used by the BINOMIAL scorer (SeqEval.h:199-216). */
 uint64_t bbt_calc_min_count(uint64_t frame_len, double bf_fpr, double min_fpr);
 double bbt_binomial_score(uint64_t count, uint64_t frame_len, double bf_fpr);
+
+/* ---- multi-index Bloom filter ------------------------------------------- */
+/* One bit vector of m bits shared by all references; each set bit owns one
+ * 32-bit slot (rank order) holding a reference ID (bits 0..30, IDs 1..n) and
+ * a saturation flag (bit 31).  Hashes are spaced-seed ntHash values, one per
+ * seed per window; a seed is a string of '0'/'1' ('1' = position used). */
+typedef struct bbt_mibf bbt_mibf;
+
+typedef struct {
+    uint64_t m;             /* bits */
+    uint64_t pop;           /* set bits = ID slots */
+    uint64_t pop_saturated; /* slots with the saturation flag */
+    uint32_t nids;
+    uint32_t nseeds;
+    uint32_t span;          /* seed length (window size) */
+    uint32_t reserved;
+} bbt_mibf_info_t;
+
+typedef struct {
+    double score;           /* opt::score: significance spread over the IDs */
+    double multi_thresh;    /* report candidates with nonSat >= this x best's */
+    uint32_t allowed_miss;  /* seeds of a frame that may miss (S - frameMatches) */
+    uint32_t min_nonsat;    /* opt::minCountNonSatCount */
+    uint32_t max_hits;      /* hit records per read in bbt_mibf_out.hits (1..64) */
+    uint32_t reserved;
+} bbt_mibf_params;
+
+typedef struct {
+    uint32_t *nhits;    /* device, per read: reported candidates (may exceed
+                           max_hits; 0xFFFFFFFF = not evaluated, see counters[3]) */
+    uint32_t *hits;     /* device, per read max_hits records of 6 words in rank
+                           order: id, count, nonSat, total, totalNonSat, solid */
+    uint32_t *sat;      /* device, per read: frames whose hit slots are all
+                           saturated */
+    uint64_t *counters; /* device, 4 counters ADDED TO: bit lookups, ID slot
+                           reads, candidates beyond max_hits, reads not
+                           evaluated (more than 64 candidates) */
+} bbt_mibf_out;
+
+/* Builds on the device from reference records packed back to back in dseq,
+ * record r at [rec_off[r], rec_off[r+1]) ENDING WITH one non-ACGT separator
+ * byte (rejected otherwise).  drec_off / drec_id are device copies of the
+ * nrec + 1 offsets and of the record IDs (1..nids: one per record, or one per
+ * file); rec_off is the host copy.  names: nids strings (ID i = names[i-1]).
+ * entries = 0 counts len - span + 1 per record as MIBFGen does; m_override = 0
+ * sizes the vector for the occupancy. */
+bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint32_t *drec_id,
+                          const uint64_t *rec_off, uint64_t nrec, const char *const *names,
+                          uint32_t nids, const char *const *seeds, uint32_t nseeds,
+                          double occupancy, uint64_t entries, uint64_t m_override,
+                          bbt_mibf **out, void *stream);
+/* Writes <prefix>.bf and <prefix>_ids.txt. */
+bbt_status bbt_mibf_store(const bbt_mibf *mf, const char *prefix);
+/* Reads <x>.bf and the ID file <x>_ids.txt beside it. */
+bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream);
+bbt_status bbt_mibf_free(bbt_mibf *mf);
+bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info);
+/* name of ID id (0 = "repeat"), or NULL */
+const char *bbt_mibf_name(const bbt_mibf *mf, uint32_t id);
+const char *bbt_mibf_seed(const bbt_mibf *mf, uint32_t i);
+/* host copies: bv = m/64 words of the bit vector, data = pop slots */
+bbt_status bbt_mibf_download(const bbt_mibf *mf, uint64_t *bv, uint32_t *data);
+uint64_t bbt_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy);
+/* probs[0..nids]: chance that a random frame reports each ID */
+bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, double *probs);
+uint32_t bbt_mibf_min_count(uint64_t frames, double prob, double score, uint32_t nids);
+/* Classifies a batch of reads (bbt_reads as for bbt_eval_batch; order is
+ * ignored); one wavefront per read. */
+bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mibf_params *p,
+                                bbt_mibf_out *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,27 @@ uint64_t orc_summary(const char *const *filter_ids, uint32_t nf,
                      const uint8_t *hits, uint64_t nreads, int paired,
                      int inclusive, char *out, uint64_t cap);
 
+/* ---- multi-index Bloom filter (bbt_mibf.cpp) ------------------------- */
+typedef struct {
+    uint32_t id, count, nonsat, total, total_nonsat, solid;
+} orc_mibf_hit;
+
+uint64_t orc_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy);
+void orc_mibf_set_bits(const char *seq, uint64_t len, const char *seeds, uint32_t nseeds,
+                       uint32_t span, uint64_t m, uint64_t *bv);
+uint64_t orc_mibf_popcount(const uint64_t *bv, uint64_t m);
+void orc_mibf_assign(const char *seq, const uint64_t *rec_off, const uint32_t *rec_id,
+                     uint64_t nrec, const char *seeds, uint32_t nseeds, uint32_t span,
+                     uint64_t m, const uint64_t *bv, uint32_t *data);
+void orc_mibf_frame_probs(const uint32_t *data, uint64_t pop, uint64_t m, uint32_t nseeds,
+                          uint32_t allowed_miss, uint32_t nids, double *probs);
+int orc_mibf_query(const uint64_t *bv, const uint32_t *data, uint64_t m, const char *seeds,
+                   uint32_t nseeds, uint32_t span, const double *probs, uint32_t nids,
+                   double score, uint32_t allowed_miss, double multi_thresh,
+                   uint32_t min_nonsat, const char *seq, const uint64_t *off,
+                   uint64_t nreads, uint32_t max_hits, orc_mibf_hit *hits,
+                   uint32_t *nhits, uint32_t *satcount, uint64_t *counters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -236,3 +236,92 @@ def summary(filter_ids, hits, paired=False, inclusive=False):
     lib().orc_summary(ids, nf, h.ctypes.data, nreads, int(paired), int(inclusive), buf,
                       n + 1)
     return buf.value.decode()
+
+
+# ---------------------------------------------------------------- MIBF
+
+class MibfHit(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("count", C.c_uint32), ("nonsat", C.c_uint32),
+                ("total", C.c_uint32), ("total_nonsat", C.c_uint32), ("solid", C.c_uint32)]
+
+
+_mibf_ready = False
+
+
+def _mibf_sigs():
+    global _mibf_ready
+    if _mibf_ready:
+        return lib()
+    L = lib()
+    vp, u64p = C.c_void_p, C.POINTER(C.c_uint64)
+    L.orc_mibf_calc_size.restype = C.c_uint64
+    L.orc_mibf_calc_size.argtypes = [C.c_uint64, C.c_uint32, C.c_double]
+    L.orc_mibf_set_bits.argtypes = [C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint32,
+                                    C.c_uint32, C.c_uint64, vp]
+    L.orc_mibf_popcount.restype = C.c_uint64
+    L.orc_mibf_popcount.argtypes = [vp, C.c_uint64]
+    L.orc_mibf_assign.argtypes = [vp, vp, vp, C.c_uint64, C.c_char_p, C.c_uint32,
+                                  C.c_uint32, C.c_uint64, vp, vp]
+    L.orc_mibf_frame_probs.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, vp]
+    L.orc_mibf_query.restype = C.c_int
+    L.orc_mibf_query.argtypes = [vp, vp, C.c_uint64, C.c_char_p, C.c_uint32, C.c_uint32,
+                                 vp, C.c_uint32, C.c_double, C.c_uint32, C.c_double,
+                                 C.c_uint32, vp, vp, C.c_uint64, C.c_uint32, vp, vp, vp,
+                                 vp]
+    _mibf_ready = True
+    return L
+
+
+class Mibf:
+    """Oracle MIBF built from (id, sequence) records (ids 1..n)."""
+
+    def __init__(self, records, seeds, occupancy=0.5, m=None):
+        L = _mibf_sigs()
+        self.seeds = list(seeds)
+        self.S, self.k = len(seeds), len(seeds[0])
+        sd = "".join(seeds).encode()
+        entries = sum(max(0, len(s) - self.k + 1) for _, s in records)
+        self.m = int(m or L.orc_mibf_calc_size(entries, self.S, occupancy))
+        self.bv = np.zeros(self.m // 64, np.uint64)
+        for _, s in records:
+            b = s.encode() if isinstance(s, str) else bytes(s)
+            L.orc_mibf_set_bits(b, len(b), sd, self.S, self.k, self.m, self.bv.ctypes.data)
+        self.pop = int(L.orc_mibf_popcount(self.bv.ctypes.data, self.m))
+        buf, off = pack_reads([s for _, s in records])
+        ids = np.array([i for i, _ in records], np.uint32)
+        self.data = np.zeros(max(1, self.pop), np.uint32)
+        L.orc_mibf_assign(buf.ctypes.data, off.ctypes.data, ids.ctypes.data, len(records), sd,
+                          self.S, self.k, self.m, self.bv.ctypes.data, self.data.ctypes.data)
+        self.nids = int(ids.max()) if len(ids) else 0
+
+    def frame_probs(self, allowed_miss):
+        out = np.zeros(self.nids + 1, np.float64)
+        _mibf_sigs().orc_mibf_frame_probs(self.data.ctypes.data, self.pop, self.m, self.S,
+                                          allowed_miss, self.nids, out.ctypes.data)
+        return out
+
+    def query(self, reads, score=1e-10, allowed_miss=None, multi=1.0, min_nonsat=0,
+              max_hits=16):
+        L = _mibf_sigs()
+        a = self.S - 1 if allowed_miss is None else allowed_miss
+        probs = self.frame_probs(a)
+        buf, off = pack_reads(reads)
+        n = len(reads)
+        hits = (MibfHit * max(1, n * max_hits))()
+        nh = np.zeros(max(1, n), np.uint32)
+        sat = np.zeros(max(1, n), np.uint32)
+        ctr = np.zeros(3, np.uint64)
+        L.orc_mibf_query(self.bv.ctypes.data, self.data.ctypes.data, self.m,
+                         "".join(self.seeds).encode(), self.S, self.k, probs.ctypes.data,
+                         self.nids, score, a, multi, min_nonsat, buf.ctypes.data,
+                         off.ctypes.data, n, max_hits, hits, nh.ctypes.data, sat.ctypes.data,
+                         ctr.ctypes.data)
+        res = []
+        for r in range(n):
+            row = []
+            for j in range(min(int(nh[r]), max_hits)):
+                h = hits[r * max_hits + j]
+                row.append((h.id, h.count, h.nonsat, h.total, h.total_nonsat, h.solid))
+            res.append(row)
+        return res, sat[:n].tolist(), ctr

@@ -1,0 +1,207 @@
+"""Multi-index Bloom filter on the device against the oracle
+(oracle/bbt_mibf.cpp): the built bit vector and ID slots bit for bit, the
+per-read query results (candidates in rank order with all five statistics,
+satCount, lookup counters), the stored file, and the argument checks."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from test_oracle_mibf import make_refs, seeds_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from biobloom_amd import device as D
+    from biobloom_amd import lib as B
+    B.load()
+    return D, B
+
+
+def reads_from(rng, recs, n, lens=(40, 100, 150), err=0.02, junk=0.3):
+    reads = []
+    for _ in range(n):
+        if rng.random() > junk:
+            _, ref = rng.choice(recs)
+            L = rng.choice(lens)
+            a = rng.randrange(0, len(ref) - L)
+            r = list(ref[a:a + L])
+            for i in range(len(r)):
+                if rng.random() < err:
+                    r[i] = rng.choice("ACGTN")
+            reads.append("".join(r))
+        else:
+            reads.append("".join(rng.choice("ACGT") for _ in range(rng.choice([5, 10, 120]))))
+    return reads
+
+
+def device_reads(D, reads):
+    buf, off = O.pack_reads(reads)
+    return D.DeviceReads(buf, off)
+
+
+def download(mf):
+    info = mf.info()
+    bv = np.zeros(info.m // 64, np.uint64)
+    data = np.zeros(max(1, info.pop), np.uint32)
+    mf.download(bv, data)
+    return info, bv, data
+
+
+@pytest.mark.parametrize("case", [(15, 11, 0.5, 1), (25, 20, 0.3, 2), (21, 15, 0.7, 3)])
+def test_build_matches_oracle(dev, case):
+    D, B = dev
+    span, weight, occ, seed = case
+    rng = random.Random(100 + seed)
+    seeds = seeds_for(rng, span=span, weight=weight)
+    recs = make_refs(rng, n=6, L=3000)
+    print(f"[gpu] mibf build span={span} occ={occ}", flush=True)
+    om = O.Mibf(recs, seeds, occupancy=occ)
+    names = [f"ref{i}" for i, _ in recs]
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs], names, seeds, occupancy=occ)
+    info, bv, data = download(mf)
+    assert info.m == om.m, "occupancy sizing differs"
+    assert info.pop == om.pop
+    assert np.array_equal(bv, om.bv)
+    assert np.array_equal(data[: om.pop], om.data[: om.pop])
+    assert info.pop_saturated == int(np.count_nonzero(om.data[: om.pop] & 0x80000000))
+    assert info.pop_saturated > 0
+    assert mf.name(0) == "repeat" and mf.name(3) == "ref3"
+    assert mf.seeds() == seeds
+    for a in range(len(seeds)):
+        assert np.allclose(mf.frame_probs(a), om.frame_probs(a), rtol=1e-12, atol=0)
+
+
+def test_build_ids_by_file(dev):
+    """Several records per ID (MIBFGen's idByFile mode)."""
+    D, B = dev
+    rng = random.Random(7)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=1200)
+    ids = [1, 2, 1, 3, 2, 3]
+    om = O.Mibf([(i, s) for i, (_, s) in zip(ids, recs)], seeds)
+    mf = D.build_mibf([s for _, s in recs], ids, ["a.fa", "b.fa", "c.fa"], seeds, m=om.m)
+    info, bv, data = download(mf)
+    assert info.nids == 3
+    assert np.array_equal(bv, om.bv)
+    assert np.array_equal(data[: om.pop], om.data[: om.pop])
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_query_matches_oracle(dev, seed):
+    D, B = dev
+    rng = random.Random(200 + seed)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=7, L=2500)
+    occ = rng.choice([0.3, 0.5])
+    om = O.Mibf(recs, seeds, occupancy=occ)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = reads_from(rng, recs, 300)
+    dr = device_reads(D, reads)
+    for score, a, multi, mns in [(1e-10, 2, 1.0, 0), (1e-3, 1, 0.5, 0), (1e-10, 0, 1.0, 0),
+                                 (0.1, 2, 0.0, 2)]:
+        print(f"[gpu] mibf query seed={seed} score={score} miss={a} multi={multi}", flush=True)
+        want, wsat, wctr = om.query(reads, score=score, allowed_miss=a, multi=multi,
+                                    min_nonsat=mns, max_hits=8)
+        got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(score, multi, a, mns, 8))
+        assert gsat == wsat
+        for r in range(len(reads)):
+            assert got[r] == want[r], (r, reads[r])
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[2] == wctr[2]
+        assert gctr[3] == 0
+    assert sum(len(r) for r in want) > 0
+
+
+@pytest.mark.parametrize("L", [700, 1300, 2600])
+def test_query_long_reads(dev, L):
+    """Longer reads take the larger per-read record stores (frames x seeds up
+    to 8192)."""
+    D, B = dev
+    rng = random.Random(L)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=4, L=6000)
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = reads_from(rng, recs, 40, lens=(L, L // 2, 64), junk=0.1)
+    dr = device_reads(D, reads)
+    print(f"[gpu] mibf long reads L={L}", flush=True)
+    want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=1, multi=0.5, max_hits=8)
+    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 0.5, 1, 0, 8))
+    assert gsat == wsat
+    assert got == want
+    assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+
+
+def test_max_hits_overflow_counted(dev):
+    """Candidates beyond max_hits are counted, the first max_hits kept."""
+    D, B = dev
+    rng = random.Random(9)
+    seeds = seeds_for(rng)
+    base = "".join(rng.choice("ACGT") for _ in range(800))
+    recs = [(i + 1, base) for i in range(5)]  # five identical references
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = [base[a:a + 150] for a in range(0, 600, 50)]
+    dr = device_reads(D, reads)
+    want, wsat, wctr = om.query(reads, score=0.5, allowed_miss=2, multi=0.0, max_hits=2)
+    got, gsat, gctr, nh = D.mibf_classify(mf, dr, B.mibf_params(0.5, 0.0, 2, 0, 2))
+    assert got == want and gsat == wsat
+    assert wctr[2] > 0 and gctr[2] == wctr[2]
+
+
+def test_store_load_roundtrip(dev, tmp_path):
+    D, B = dev
+    rng = random.Random(11)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=4, L=1500)
+    names = ["chrA", "chrB", "chrC", "chrD"]
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs], names, seeds)
+    prefix = str(tmp_path / "multi")
+    mf.store(prefix)
+    ids_txt = open(prefix + "_ids.txt").read().splitlines()
+    assert ids_txt == [f"{i}\t{n}" for i, n in enumerate(names, 1)]
+    mf2 = B.Mibf.load(prefix + ".bf")
+    i1, bv1, d1 = download(mf)
+    i2, bv2, d2 = download(mf2)
+    assert (i1.m, i1.pop, i1.pop_saturated, i1.nids) == (i2.m, i2.pop, i2.pop_saturated, i2.nids)
+    assert np.array_equal(bv1, bv2) and np.array_equal(d1, d2)
+    assert [mf2.name(i) for i in range(5)] == ["repeat"] + names
+    reads = reads_from(rng, recs, 50)
+    dr = device_reads(D, reads)
+    p = B.mibf_params(1e-10, 1.0, 2, 0, 4)
+    assert D.mibf_classify(mf, dr, p)[:2] == D.mibf_classify(mf2, dr, p)[:2]
+
+
+def test_load_errors(dev, tmp_path):
+    D, B = dev
+    bad = tmp_path / "x.bf"
+    bad.write_bytes(b"not a filter at all, no header" * 4)
+    with pytest.raises(B.BbtError, match="status 3"):
+        B.Mibf.load(str(bad))
+    with pytest.raises(B.BbtError, match="status 2"):
+        B.Mibf.load(str(tmp_path / "missing.bf"))
+
+
+def test_records_without_separator_rejected(dev):
+    torch = pytest.importorskip("torch")
+    D, B = dev
+    seq = np.frombuffer(b"ACGTACGTACGTACGTACGTACGTACGTACGTN", np.uint8)
+    off = [0, 10, len(seq)]  # record 0 ends in 'G'
+    dseq = torch.from_numpy(seq.copy()).cuda()
+    doff = torch.tensor(off, dtype=torch.int64).cuda()
+    did = torch.tensor([1, 2], dtype=torch.int32).cuda()
+    with pytest.raises(B.BbtError, match="separator"):
+        B.Mibf.build(dseq.data_ptr(), doff.data_ptr(), did.data_ptr(), off, ["a", "b"],
+                     ["1101"], occupancy=0.5)
+    with pytest.raises(B.BbtError, match="status 1"):
+        B.Mibf.build(dseq.data_ptr(), doff.data_ptr(), did.data_ptr(), off, ["a", "b"],
+                     ["1101", "111"], occupancy=0.5)
