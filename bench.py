@@ -9,6 +9,11 @@ intersection mode, default SIMPLE scorer (-s 0.15, -r 3).
 Prints one JSON line: reads/s of the batched persistent path (value), the
 plain per-batch path beside it, k-mer probes/s, the roofline of the walk
 kernel and the CPU oracle baseline on a bounded sample.
+
+--workload mibf: the multi-index Bloom filter instead (200 x 5 Mbp
+references, 3 spaced seeds span 25 weight 20, 2M x 150 bp reads, the
+biobloommicategorizer defaults -s 100 -a 1 -m 1 -c 0); one step is one
+bbt_mibf_query_batch over all reads; probes are (frame, seed) lookups.
 """
 import argparse
 import json
@@ -88,6 +93,115 @@ def cpu_baseline(fs, buf, off, paired, sample_records, threads, B, gpu_hits):
     return nreads / dt, dt, parity
 
 
+def cpu_baseline_mibf(mf, buf, off, nsample, params, gpu_rows, gpu_sat):
+    """Oracle MIBF query (OpenMP over reads) on the first nsample reads,
+    against host copies of the device-built bit vector and ID array."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    info = mf.info()
+    bv = np.zeros(info.m // 64, np.uint64)
+    data = np.zeros(max(1, info.pop), np.uint32)
+    mf.download(bv, data)
+    seeds = mf.seeds() or O.Kmer(info.span, info.nseeds)
+    om = O.Mibf.from_arrays(bv, data, info.m, seeds, info.nids)
+    sbuf = buf[: int(off[nsample])]
+    soff = off[: nsample + 1]
+    reads = [bytes(sbuf[int(soff[i]):int(soff[i + 1])]) for i in range(nsample)]
+    kw = dict(score=params.score, allowed_miss=params.allowed_miss,
+              multi=params.multi_thresh, min_nonsat=params.min_nonsat,
+              max_hits=params.max_hits)
+    om.query(reads[: min(1000, nsample)], **kw)  # warm
+    t = time.time()
+    rows, sat, _ = om.query(reads, **kw)
+    dt = time.time() - t
+    parity = rows == gpu_rows[:nsample] and sat == gpu_sat[:nsample]
+    return nsample / dt, dt, bool(parity)
+
+
+def bench_mibf(args, torch, D, B):
+    from biobloom_amd import synth
+    t = time.time()
+    w = synth.workload_mibf(scale=args.scale)
+    reads = w["reads"]
+    buf = np.ascontiguousarray(reads).reshape(-1)
+    off = np.arange(reads.shape[0] + 1, dtype=np.uint64) * np.uint64(reads.shape[1])
+    nreads = reads.shape[0]
+    log(f"workload mibf: {len(w['refs'])} references x {len(w['refs'][0])} bp, "
+        f"{nreads} reads, seeds {w['seeds']} ({time.time() - t:.1f}s)")
+    t = time.time()
+    mf = D.build_mibf(w["refs"], list(range(1, len(w["refs"]) + 1)), w["names"], w["seeds"],
+                      occupancy=0.5)
+    info = mf.info()
+    log(f"MIBF built on device ({time.time() - t:.2f}s): m={info.m} pop={info.pop} "
+        f"saturated={info.pop_saturated} ids={info.nids}")
+    del w
+    dr = D.DeviceReads(buf, off)
+    # biobloommicategorizer defaults: -s 100 (score 1e-10), -a 1 (allowed
+    # misses = 3 - 1), -m 1, -c 0 (BioBloomMICategorizer.cpp:132, Options)
+    params = B.mibf_params(score=1e-10, multi_thresh=1.0, allowed_miss=info.nseeds - 1,
+                           min_nonsat=0, max_hits=8)
+    out = D.MibfOutputs(nreads, params.max_hits)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    rs, os_ = dr.struct(), out.struct()
+
+    def step():
+        mf.query_batch(rs, params, os_, stream)
+
+    out.counters.zero_()
+    ms = time_steps(step, args.steps, args.warmup, torch)
+    out.counters.zero_()
+    step()
+    torch.cuda.synchronize()
+    rows, sat, ctr, nh = out.host()
+    lookups, gathers = int(ctr[0]), int(ctr[1])
+    assert int(ctr[3]) == 0, "records not evaluated"
+    classified = sum(1 for r in rows if r)
+    reads_per_s = nreads / (ms / 1e3)
+    alg_bytes = (lookups + gathers) * LINE_BYTES
+    achieved = alg_bytes / (ms / 1e3) / 1e9
+    log(f"query: {ms:.3f} ms/step, {reads_per_s / 1e6:.2f} M reads/s, "
+        f"{lookups / (ms / 1e3) / 1e9:.2f} G lookups/s, classified {classified}/{nreads}")
+    gather = {}
+    if not args.no_gather:
+        lps, _ = B.measure_gather(8 << 30, 64, stream)
+        gather = {"hbm_8GiB": {"bytes": 8 << 30, "lines_per_s": lps,
+                               "gbs_at_64B": lps * LINE_BYTES / 1e9}}
+        log(f"random gather over 8192 MiB: {lps / 1e9:.1f} G lines/s")
+    cpu = None
+    if not args.no_cpu:
+        nsample = min(args.cpu_sample, nreads)
+        v, dt, parity = cpu_baseline_mibf(mf, buf, off, nsample, params, rows, sat)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": v, "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"first {nsample} reads of the workload, {dt:.2f}s",
+               "parity_with_gpu": parity}
+        log(f"cpu oracle: {v / 1e6:.4f} M reads/s on {threads} threads, parity={parity}")
+    traffic = load_traffic("mibf")
+    line = {
+        "metric": METRIC, "value": reads_per_s, "unit": "reads/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "multi-index Bloom filter: 200 x 5 Mbp references (5% shared "
+                               "10 kbp segments), 3 spaced seeds span 25 weight 20, "
+                               "occupancy 0.5, 32-bit IDs, 2M x 150 bp reads",
+                   "scale": args.scale, "reads": nreads, "ids": info.nids, "m": info.m,
+                   "pop": info.pop},
+        "probes_per_s": lookups / (ms / 1e3),
+        "probes_per_step": lookups,
+        "id_gathers_per_step": gathers,
+        "classified_reads": classified,
+        "roofline": {"bound": "hbm", "kernel": "k_mibf_query", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "algorithmic_bytes": alg_bytes, "random_gather": gather or None,
+                     "frac_of_random_gather": (achieved / gather["hbm_8GiB"]["gbs_at_64B"]
+                                               if gather else None)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_walk_{workload}.json")
     if os.path.exists(p):
@@ -101,7 +215,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="multi", choices=["multi", "single", "uneven"])
+    ap.add_argument("--workload", default="multi",
+                    choices=["multi", "single", "uneven", "mibf"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cpu-sample", type=int, default=100_000,
                     help="records (pairs for paired workloads) in the CPU baseline sample")
@@ -140,6 +255,8 @@ def main():
         print(json.dumps({"copy_bytes": n, "read_bytes": n, "write_bytes": n}))
         return
     B.load()
+    if args.workload == "mibf":
+        return bench_mibf(args, torch, D, B)
 
     w, buf, off = make_workload(args.workload, args.scale)
     paired = bool(w["paired"])

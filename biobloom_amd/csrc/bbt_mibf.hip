@@ -59,8 +59,8 @@ struct MibfDev {
     uint32_t sh1, sh2;
     uint32_t S, span;
     uint32_t nids;
-    uint32_t pad;
-    uint32_t care[kMaxSeeds]; /* bit j set: seed position j is a care position */
+    uint32_t kmer;            /* 1: k-mer mode, S ntHash values per k-mer */
+    uint32_t care[kMaxSeeds]; /* spaced seeds: bit j set = position j is used */
 };
 
 __device__ __forceinline__ uint32_t mix32(uint64_t x) {
@@ -87,10 +87,15 @@ __device__ __forceinline__ bool bv_rank(const uint64_t *lines, uint64_t p, uint6
     return (word >> b) & 1;
 }
 
-/* Spaced-seed hashes of the window of span k starting at p, from the full
- * window values (fwd, rc) with the don't-care bases XORed back out. */
-__device__ __forceinline__ uint64_t seed_hash(const uint8_t *s, uint64_t p, uint32_t k,
-                                              uint32_t care, uint64_t fwd, uint64_t rc) {
+/* Hash sd of the window of span k starting at p, from the full window values
+ * (fwd, rc).  Spaced seeds: the don't-care bases XORed back out, canonical
+ * min.  k-mer mode: the ntHash multi-hash sd of the canonical value. */
+__device__ __forceinline__ uint64_t frame_hash(const MibfDev &d, const uint8_t *s, uint64_t p,
+                                               uint32_t sd, uint64_t fwd, uint64_t rc) {
+    const uint32_t k = d.span;
+    if (d.kmer)
+        return multi_hash(rc < fwd ? rc : fwd, sd, uint64_t(k) * kMultiSeed);
+    const uint32_t care = d.care[sd];
     for (uint32_t j = 0; j < k; ++j) {
         if ((care >> j) & 1)
             continue;
@@ -112,8 +117,9 @@ struct FrameCursor {
 
 /* Thread t owns window starts [t*chunk, (t+1)*chunk) of the whole buffer. */
 template <typename Fn>
-__device__ void for_frames(const uint8_t *seq, uint64_t nbytes, uint32_t k, uint32_t chunk,
-                           const uint32_t *care, uint32_t S, Fn fn) {
+__device__ void for_frames(const uint8_t *seq, uint64_t nbytes, const MibfDev &d,
+                           uint32_t chunk, Fn fn) {
+    const uint32_t k = d.span, S = d.S;
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint64_t p0 = t * chunk;
     if (p0 >= nbytes)
@@ -142,13 +148,13 @@ __device__ void for_frames(const uint8_t *seq, uint64_t nbytes, uint32_t k, uint
             continue;
         const uint64_t p = i + 1 - k;
         for (uint32_t sd = 0; sd < S; ++sd)
-            fn(p, sd, seed_hash(seq, p, k, care[sd], c.fwd, c.rc));
+            fn(p, sd, frame_hash(d, seq, p, sd, c.fwd, c.rc));
     }
 }
 
 __global__ __launch_bounds__(256) void k_mibf_bits(const uint8_t *seq, uint64_t nbytes,
                                                    MibfDev d, uint64_t *bits, uint32_t chunk) {
-    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+    for_frames(seq, nbytes, d, chunk,
                [&](uint64_t, uint32_t, uint64_t h) {
                    const uint64_t p = fastmod(h, d.m, d.magic, d.sh1, d.sh2);
                    atomicOr(reinterpret_cast<unsigned long long *>(bits) + (p >> 6),
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(256) void k_mibf_own(const uint8_t *seq, uint64_t n
                                                   const uint64_t *rec_off, const uint32_t *rec_id,
                                                   uint64_t nrec, MibfDev d,
                                                   unsigned long long *best, uint32_t chunk) {
-    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+    for_frames(seq, nbytes, d, chunk,
                [&](uint64_t p, uint32_t, uint64_t h) {
                    const uint64_t id = rec_id[record_of(rec_off, nrec, p)];
                    uint64_t rank;
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(256) void k_mibf_sat(const uint8_t *seq, uint64_t n
     uint64_t ranks[kMaxSeeds];
     bool rep = false;
     uint32_t nseen = 0;
-    for_frames(seq, nbytes, d.span, chunk, d.care, d.S,
+    for_frames(seq, nbytes, d, chunk,
                [&](uint64_t p, uint32_t sd, uint64_t h) {
                    if (p != pos_cur) {
                        pos_cur = p;
@@ -266,16 +272,40 @@ __global__ __launch_bounds__(256) void k_mibf_sat(const uint8_t *seq, uint64_t n
                });
 }
 
-/* slots per ID (hist[id]) and saturated slots (hist[nids + 1]) */
-__global__ void k_mibf_hist(const uint32_t *data, uint64_t n, uint32_t nids,
-                            unsigned long long *hist) {
+/* slots per ID (hist[id]) and saturated slots (hist[nids + 1]); with few
+ * IDs each block counts in LDS first (thousands of slots share one ID) */
+constexpr uint32_t kHistLds = 4096;
+
+__global__ __launch_bounds__(256) void k_mibf_hist(const uint32_t *data, uint64_t n,
+                                                   uint32_t nids, unsigned long long *hist) {
+    __shared__ unsigned int local[kHistLds];
+    const bool lds = nids + 2 <= kHistLds;
+    if (lds) {
+        for (uint32_t i = threadIdx.x; i < nids + 2; i += blockDim.x)
+            local[i] = 0;
+        __syncthreads();
+    }
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t id = data[i] & kIdMask;
-        if (id <= nids)
-            atomicAdd(hist + id, 1ull);
-        if (data[i] & kSatBit)
-            atomicAdd(hist + nids + 1, 1ull);
+        const uint32_t v = data[i];
+        const uint32_t id = v & kIdMask;
+        if (lds) {
+            if (id <= nids)
+                atomicAdd(local + id, 1u);
+            if (v & kSatBit)
+                atomicAdd(local + nids + 1, 1u);
+        } else {
+            if (id <= nids)
+                atomicAdd(hist + id, 1ull);
+            if (v & kSatBit)
+                atomicAdd(hist + nids + 1, 1ull);
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nids + 2; i += blockDim.x)
+            if (local[i])
+                atomicAdd(hist + i, (unsigned long long)local[i]);
     }
 }
 
@@ -297,29 +327,51 @@ struct QueryArgs {
     MibfDev d;
     const uint8_t *seq;
     const uint64_t *off;
-    uint64_t nreads;
+    uint64_t nrec;             /* reads, or pairs (reads 2j, 2j+1) */
     const uint32_t *min_count; /* [frameCount][nids + 1] */
     uint32_t max_frames;       /* rows of min_count - 1 */
     uint32_t allowed_miss;
     double multi_thresh;
     uint32_t min_nonsat;
     uint32_t max_hits;
-    uint32_t *nhits;  /* per read */
-    uint32_t *hits;   /* per read max_hits x 6: id count nonsat total totalNonSat solid */
-    uint32_t *sat;    /* per read */
-    unsigned long long *counters; /* bv lookups, id gathers, overflow, unsupported */
+    uint32_t paired, best_agree;
+    uint32_t *nhits;  /* per record */
+    uint32_t *hits;   /* per record max_hits x kHitWords */
+    uint32_t *sat;    /* per record */
+    unsigned long long *counters; /* bv lookups, id gathers, overflow, not evaluated */
     unsigned long long *queue;
 };
 
-/* record of one ID in one frame, sortable by ID: id | occ | nonsat-occ */
-__device__ __forceinline__ uint64_t make_rec(uint32_t id, uint32_t occ, uint32_t ns) {
-    return (uint64_t(id) << 32) | (occ << 8) | ns;
+constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
+constexpr uint32_t kMaxCand = 64;
+
+/* one distinct ID of one frame, sortable by ID:
+ * id << 32 | clean frame << 16 | slots holding it << 8 | unsaturated ones */
+__device__ __forceinline__ uint64_t make_rec(uint32_t id, uint32_t clean, uint32_t occ,
+                                             uint32_t ns) {
+    return (uint64_t(id) << 32) | (clean << 16) | (occ << 8) | ns;
 }
 
+/* rank order of two candidates (MIBFQuerySupport ranking as restated in the
+ * oracle): nonSat, count, solid, totalNonSat, total descending, then ID */
+__device__ __forceinline__ bool ranks_before(const uint32_t *x, const uint32_t *y) {
+    if (x[2] != y[2]) return x[2] > y[2];
+    if (x[1] != y[1]) return x[1] > y[1];
+    if (x[5] != y[5]) return x[5] > y[5];
+    if (x[4] != y[4]) return x[4] > y[4];
+    if (x[3] != y[3]) return x[3] > y[3];
+    return x[0] < y[0];
+}
+
+/* One wavefront per record.  Each mate is walked 64 windows per step: the
+ * window hashes come from prefix-XOR scans, every lane resolves its frame's
+ * S slots (bit + rank from one line, then the ID), and the distinct IDs of
+ * each frame are appended to an LDS store that is sorted by ID and reduced
+ * per ID at the end of the record. */
 template <uint32_t CAP>
 __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
     __shared__ uint64_t rec[CAP];
-    __shared__ uint32_t cand[64 * 6];
+    __shared__ uint32_t cand[kMaxCand * kHitWords];
     __shared__ uint32_t ncand_s;
     const uint32_t lane = threadIdx.x;
     const uint32_t S = a.d.S, k = a.d.span;
@@ -329,15 +381,26 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
         if (lane == 0)
             it = atomicAdd(a.queue, 1ull);
         it = __shfl(it, 0);
-        if (it >= a.nreads)
+        if (it >= a.nrec)
             break;
         const uint64_t r = it;
-        const uint64_t b = a.off[r];
-        const uint32_t len = uint32_t(a.off[r + 1] - b);
-        const uint8_t *s = a.seq + b;
+        const uint64_t first = a.paired ? 2 * r : r;
+        const uint32_t nmates = a.paired ? 2 : 1;
+        /* frameCount: len - k + 1; for a pair, the nesting of
+         * MIBFClassifier.hpp:804-805 (mate 2 only when mate 1 is too short) */
+        const uint32_t len1 = uint32_t(a.off[first + 1] - a.off[first]);
+        uint32_t fc = len1 >= k ? len1 - k + 1 : 0;
+        if (a.paired && len1 < k) {
+            const uint32_t len2 = uint32_t(a.off[first + 2] - a.off[first + 1]);
+            fc = len2 >= k ? len2 - k + 1 : 0;
+        }
         uint32_t satc = 0, nrec = 0;
-        bool over = false;
-        if (len >= k) {
+        for (uint32_t mate = 0; mate < nmates && fc; ++mate) {
+            const uint64_t b = a.off[first + mate];
+            const uint32_t len = uint32_t(a.off[first + mate + 1] - b);
+            const uint8_t *s = a.seq + b;
+            if (len < k)
+                continue;
             uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
             int lastN = -1;
             for (uint32_t b0 = 0; b0 < len; b0 += 64) {
@@ -376,12 +439,12 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 lastN = __shfl(ln, 63);
                 Hp = H;
                 Gp = G;
-                /* per frame: seed slots, their IDs, distinct-ID records */
+                /* this lane's frame: its S slots and their IDs */
                 uint32_t raw[kMaxSeeds];
                 uint32_t nh = 0;
                 if (valid) {
                     for (uint32_t sd = 0; sd < S; ++sd) {
-                        const uint64_t h = seed_hash(s, uint64_t(pw), k, a.d.care[sd], wf, wr);
+                        const uint64_t h = frame_hash(a.d, s, uint64_t(pw), sd, wf, wr);
                         uint64_t rank;
                         ++bvl;
                         if (bv_rank(a.d.lines, fastmod(h, a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
@@ -395,10 +458,13 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 uint32_t myrec = 0;
                 uint64_t out[kMaxSeeds];
                 if (eval) {
-                    bool allsat = nh > 0;
-                    for (uint32_t x = 0; x < nh; ++x)
-                        if (!(raw[x] & kSatBit))
+                    bool allsat = nh > 0, anysat = false;
+                    for (uint32_t x = 0; x < nh; ++x) {
+                        if (raw[x] & kSatBit)
+                            anysat = true;
+                        else
                             allsat = false;
+                    }
                     satc += allsat ? 1 : 0;
                     for (uint32_t x = 0; x < nh; ++x) {
                         const uint32_t id = raw[x] & kIdMask;
@@ -415,10 +481,10 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                                 if (!(raw[y] & kSatBit))
                                     ++ns;
                             }
-                        out[myrec++] = make_rec(id, occ, ns);
+                        out[myrec++] = make_rec(id, anysat ? 0u : 1u, occ, ns);
                     }
                 }
-                /* append this chunk's records: exclusive scan of myrec */
+                /* append this step's records: exclusive scan of myrec */
                 uint32_t pre = myrec;
                 for (int dd = 1; dd < 64; dd <<= 1) {
                     const uint32_t t = __shfl_up(pre, unsigned(dd));
@@ -433,15 +499,15 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 nrec += total;
             }
         }
-        /* satCount: sum over lanes */
         for (int o = 32; o > 0; o >>= 1)
             satc += __shfl_xor(satc, o);
-        if (nrec > CAP) {
+        bool over = false;
+        if (nrec > CAP) { /* cannot happen when the host sized CAP by frames x S */
             over = true;
             nrec = 0;
         }
         __syncthreads();
-        /* bitonic sort of the records by key (IDs first) */
+        /* bitonic sort of the records (IDs in the high bits) */
         uint32_t n2 = 1;
         while (n2 < nrec)
             n2 <<= 1;
@@ -464,81 +530,77 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 __syncthreads();
             }
         }
-        /* per-ID totals: each run of equal IDs is reduced by the lane at its head */
         if (lane == 0)
             ncand_s = 0;
         __syncthreads();
-        const uint32_t fc = len >= k ? len - k + 1 : 0;
+        /* per-ID totals: the lane at the head of each run of one ID reduces it */
         const uint32_t row = fc <= a.max_frames ? fc : 0;
         for (uint32_t x = lane; x < nrec; x += 64) {
             const uint32_t id = uint32_t(rec[x] >> 32);
             if (x > 0 && uint32_t(rec[x - 1] >> 32) == id)
                 continue;
-            uint32_t cnt = 0, nsat = 0, tot = 0, tns = 0, solid = 0;
+            uint32_t cnt = 0, nsat = 0, tot = 0, tns = 0, solid = 0, nsf = 0;
             for (uint32_t y = x; y < nrec && uint32_t(rec[y] >> 32) == id; ++y) {
-                const uint32_t occ = uint32_t(rec[y] >> 8) & 0xFF, ns = uint32_t(rec[y]) & 0xFF;
+                const uint32_t v = uint32_t(rec[y]);
+                const uint32_t occ = (v >> 8) & 0xFF, ns = v & 0xFF;
                 cnt += 1;
                 nsat += ns ? 1 : 0;
                 tot += occ;
                 tns += ns;
                 solid += occ == S ? 1 : 0;
+                nsf += (v >> 16) & 1;
             }
-            const uint32_t mc = (id <= a.d.nids && row) ? a.min_count[uint64_t(row) * (a.d.nids + 1) + id]
-                                                        : 0xFFFFFFFFu;
+            const uint32_t mc = (id >= 1 && id <= a.d.nids && row)
+                                    ? a.min_count[uint64_t(row) * (a.d.nids + 1) + id]
+                                    : 0xFFFFFFFFu;
             if (cnt >= mc && nsat >= a.min_nonsat) {
                 const uint32_t slot = atomicAdd(&ncand_s, 1u);
-                if (slot < 64) {
-                    uint32_t *c6 = cand + slot * 6;
-                    c6[0] = id;
-                    c6[1] = cnt;
-                    c6[2] = nsat;
-                    c6[3] = tot;
-                    c6[4] = tns;
-                    c6[5] = solid;
+                if (slot < kMaxCand) {
+                    uint32_t *c7 = cand + slot * kHitWords;
+                    c7[0] = id;
+                    c7[1] = cnt;
+                    c7[2] = nsat;
+                    c7[3] = tot;
+                    c7[4] = tns;
+                    c7[5] = solid;
+                    c7[6] = nsf;
                 }
             }
         }
         __syncthreads();
-        /* ranking and output by lane 0 (few candidates) */
+        /* ranking and output by lane 0 (a handful of candidates) */
         if (lane == 0) {
             uint32_t nc = ncand_s;
-            if (nc > 64) {
+            if (nc > kMaxCand) {
                 over = true;
-                nc = 64;
+                nc = 0;
             }
-            uint32_t order[64];
-            for (uint32_t x = 0; x < nc; ++x)
-                order[x] = x;
-            for (uint32_t x = 1; x < nc; ++x) { /* insertion sort, rank order */
-                const uint32_t v = order[x];
-                const uint32_t *cv = cand + v * 6;
+            uint32_t order[kMaxCand];
+            for (uint32_t x = 0; x < nc; ++x) {
                 uint32_t y = x;
-                while (y > 0) {
-                    const uint32_t *cu = cand + order[y - 1] * 6;
-                    bool before;
-                    if (cv[2] != cu[2]) before = cv[2] > cu[2];
-                    else if (cv[1] != cu[1]) before = cv[1] > cu[1];
-                    else if (cv[5] != cu[5]) before = cv[5] > cu[5];
-                    else if (cv[4] != cu[4]) before = cv[4] > cu[4];
-                    else if (cv[3] != cu[3]) before = cv[3] > cu[3];
-                    else before = cv[0] < cu[0];
-                    if (!before)
-                        break;
+                while (y > 0 && ranks_before(cand + x * kHitWords, cand + order[y - 1] * kHitWords)) {
                     order[y] = order[y - 1];
                     --y;
                 }
-                order[y] = v;
+                order[y] = x;
+            }
+            if (nc && a.best_agree) {
+                const uint32_t *t = cand + order[0] * kHitWords;
+                for (uint32_t x = 0; x < nc; ++x)
+                    for (uint32_t f = 1; f < kHitWords; ++f)
+                        if (cand[x * kHitWords + f] > t[f])
+                            nc = 0;
             }
             uint32_t nout = 0;
             if (nc) {
-                const double bar = a.multi_thresh * double(cand[order[0] * 6 + 2]);
+                const double bar = a.multi_thresh * double(cand[order[0] * kHitWords + 2]);
                 for (uint32_t x = 0; x < nc; ++x) {
-                    const uint32_t *cv = cand + order[x] * 6;
+                    const uint32_t *cv = cand + order[x] * kHitWords;
                     if (double(cv[2]) < bar)
                         break;
                     if (nout < a.max_hits)
-                        for (uint32_t f = 0; f < 6; ++f)
-                            a.hits[(r * a.max_hits + nout) * 6 + f] = cv[f];
+                        for (uint32_t f = 0; f < kHitWords; ++f)
+                            a.hits[(r * a.max_hits + nout) * kHitWords + f] = cv[f];
                     else
                         atomicAdd(a.counters + 2, 1ull);
                     ++nout;
@@ -589,14 +651,16 @@ struct bbt_mibf {
 
 namespace {
 
-bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds) {
-    if (!seeds || nseeds == 0 || nseeds > kMaxSeeds)
-        return mfail(BBT_ERR_ARG, "MIBF: 1..8 spaced seeds required");
-    const size_t span = std::strlen(seeds[0]);
+bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds,
+                     uint32_t kmer_size) {
+    if (nseeds == 0 || nseeds > kMaxSeeds)
+        return mfail(BBT_ERR_ARG, "MIBF: 1..8 spaced seeds or hash functions required");
+    const size_t span = seeds ? std::strlen(seeds[0]) : kmer_size;
     if (span == 0 || span > 32)
-        return mfail(BBT_ERR_UNSUPPORTED, "MIBF: seed span must be 1..32");
+        return mfail(BBT_ERR_UNSUPPORTED, "MIBF: seed span / k-mer size must be 1..32");
     std::memset(&mf->d, 0, sizeof(mf->d));
-    for (uint32_t i = 0; i < nseeds; ++i) {
+    mf->d.kmer = seeds ? 0 : 1;
+    for (uint32_t i = 0; seeds && i < nseeds; ++i) {
         const std::string sd = seeds[i];
         if (sd.size() != span)
             return mfail(BBT_ERR_ARG, "MIBF: all seeds must have the same span");
@@ -610,6 +674,9 @@ bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds) {
         mf->d.care[i] = care;
         mf->seeds.push_back(sd);
     }
+    if (!seeds)
+        for (uint32_t i = 0; i < nseeds; ++i)
+            mf->d.care[i] = span == 32 ? 0xFFFFFFFFu : (1u << span) - 1;
     mf->d.S = nseeds;
     mf->d.span = uint32_t(span);
     int dev = 0;
@@ -652,7 +719,7 @@ constexpr char kMagic[8] = {'B', 'B', 'T', 'M', 'I', 'B', 'F', '1'};
 
 struct MibfHeader {
     char magic[8];
-    uint32_t nseeds, span, nids, reserved;
+    uint32_t nseeds, span, nids, kmer; /* kmer = 1: no seed strings follow */
     uint64_t m, pop;
 };
 static_assert(sizeof(MibfHeader) == 40, "MibfHeader layout");
@@ -680,8 +747,8 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
 bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint32_t *drec_id,
                           const uint64_t *rec_off, uint64_t nrec, const char *const *names,
                           uint32_t nids, const char *const *seeds, uint32_t nseeds,
-                          double occupancy, uint64_t entries, uint64_t m_override,
-                          bbt_mibf **out, void *stream) {
+                          uint32_t kmer_size, double occupancy, uint64_t entries,
+                          uint64_t m_override, bbt_mibf **out, void *stream) {
     if (!out || !dseq || !drec_off || !drec_id || !rec_off || nrec == 0 || nids == 0)
         return mfail(BBT_ERR_ARG, "bbt_mibf_build: bad argument");
     if (nids >= kIdMask)
@@ -691,7 +758,7 @@ bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint
     *out = nullptr;
     hipStream_t st = static_cast<hipStream_t>(stream);
     bbt_mibf *mf = new bbt_mibf();
-    bbt_status s = mibf_init(mf, seeds, nseeds);
+    bbt_status s = mibf_init(mf, seeds, nseeds, kmer_size);
     if (s != BBT_OK) {
         bbt_mibf_free(mf);
         return s;
@@ -779,7 +846,7 @@ bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info) {
     info->nids = mf->d.nids;
     info->nseeds = mf->d.S;
     info->span = mf->d.span;
-    info->reserved = 0;
+    info->kmer_mode = mf->d.kmer;
     return BBT_OK;
 }
 
@@ -825,6 +892,7 @@ bbt_status bbt_mibf_store(const bbt_mibf *mf, const char *prefix) {
     h.nseeds = mf->d.S;
     h.span = mf->d.span;
     h.nids = mf->d.nids;
+    h.kmer = mf->d.kmer;
     h.m = mf->d.m;
     h.pop = mf->pop;
     f.write(reinterpret_cast<const char *>(&h), sizeof(h));
@@ -854,9 +922,10 @@ bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream) {
     MibfHeader h;
     f.read(reinterpret_cast<char *>(&h), sizeof(h));
     if (!f || std::memcmp(h.magic, kMagic, 8) != 0 || h.nseeds == 0 || h.nseeds > kMaxSeeds ||
-        h.span == 0 || h.span > 32 || h.m == 0 || h.m % 64 || h.pop > h.m || h.nids == 0)
+        h.span == 0 || h.span > 32 || h.m == 0 || h.m % 64 || h.pop > h.m || h.nids == 0 ||
+        h.kmer > 1)
         return mfail(BBT_ERR_FORMAT, std::string("bbt_mibf_load: not a multi-index filter: ") + path);
-    std::vector<std::string> seeds(h.nseeds, std::string(h.span, '0'));
+    std::vector<std::string> seeds(h.kmer ? 0 : h.nseeds, std::string(h.span, '0'));
     for (std::string &sd : seeds)
         f.read(&sd[0], h.span);
     std::vector<uint64_t> bv(h.m / 64);
@@ -884,7 +953,7 @@ bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream) {
     std::vector<const char *> sp;
     for (const std::string &sd : seeds)
         sp.push_back(sd.c_str());
-    bbt_status s = mibf_init(mf, sp.data(), h.nseeds);
+    bbt_status s = mibf_init(mf, h.kmer ? nullptr : sp.data(), h.nseeds, h.span);
     if (s != BBT_OK) {
         bbt_mibf_free(mf);
         return s;
@@ -969,19 +1038,23 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
                                 bbt_mibf_out *out, void *stream) {
     if (!mf || !in || !p || !out || !out->nhits || !out->hits || !out->sat || !out->counters)
         return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: bad argument");
-    if (in->nreads == 0)
+    const bool paired = (p->flags & BBT_MIBF_PAIRED) != 0;
+    if (paired && in->nreads % 2)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: paired batch with an odd read count");
+    const uint64_t nrec = paired ? in->nreads / 2 : in->nreads;
+    if (nrec == 0)
         return BBT_OK;
-    if (p->max_hits == 0 || p->max_hits > 64)
+    if (p->max_hits == 0 || p->max_hits > kMaxCand)
         return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..64");
-    if (in->max_len > (1u << 20))
-        return mfail(BBT_ERR_UNSUPPORTED, "bbt_mibf_query_batch: reads longer than 1 Mbp");
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> g(mf->mu);
     const uint32_t k = mf->d.span;
     const uint32_t frames = in->max_len >= k ? in->max_len - k + 1 : 0;
-    if (uint64_t(frames) * mf->d.S > 8192)
+    const uint64_t recs = uint64_t(frames) * mf->d.S * (paired ? 2 : 1);
+    if (recs > 8192)
         return mfail(BBT_ERR_UNSUPPORTED,
-                     "bbt_mibf_query_batch: frames x seeds of the longest read exceed 8192");
+                     "bbt_mibf_query_batch: frames x hashes of the longest record exceed 8192 "
+                     "(reads up to ~2.7 kbp with 3 seeds)");
     const uint32_t miss = p->allowed_miss;
     if (!mf->dmin || mf->min_score != p->score || mf->min_miss != miss ||
         mf->min_frames < frames) {
@@ -1007,21 +1080,22 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     a.d = mf->d;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
     a.off = in->off;
-    a.nreads = in->nreads;
+    a.nrec = nrec;
     a.min_count = mf->dmin;
     a.max_frames = mf->min_frames;
     a.allowed_miss = miss;
     a.multi_thresh = p->multi_thresh;
     a.min_nonsat = p->min_nonsat;
     a.max_hits = p->max_hits;
+    a.paired = paired ? 1 : 0;
+    a.best_agree = (p->flags & BBT_MIBF_BEST_AGREE) ? 1 : 0;
     a.nhits = out->nhits;
     a.hits = out->hits;
     a.sat = out->sat;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.queue = mf->dctl + 8;
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 1);
-    const uint64_t recs = uint64_t(frames) * mf->d.S;
-    const uint32_t blocks = uint32_t(std::min<uint64_t>(in->nreads, uint64_t(mf->cu) * 16));
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
     if (recs <= 1024)
         k_mibf_query<1024><<<blocks, 64, 0, st>>>(a);
     else if (recs <= 4096)

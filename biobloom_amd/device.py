@@ -127,24 +127,32 @@ def pack_records(records, sep=b"N"):
     return buf, np.asarray(off, np.uint64)
 
 
-def build_mibf(records, ids, names, seeds, occupancy=0.5, m=0, device="cuda", stream=None):
+def build_mibf(records, ids, names, seeds, occupancy=0.5, m=0, device="cuda", stream=None,
+               kmer=None):
     """biobloommimaker on the device: records (str/bytes) with their IDs
-    (1..len(names)), one ID per record or several records per ID."""
+    (1..len(names)), one ID per record or several records per ID.  seeds:
+    spaced seeds, or None with kmer = (k, hash values) for k-mer mode."""
     buf, off = pack_records(records)
     dseq = torch.from_numpy(buf.copy()).to(device)
     doff = torch.from_numpy(off.view(np.int64)).to(device)
     did = torch.from_numpy(np.asarray(ids, np.uint32).view(np.int32)).to(device)
     mf = B.Mibf.build(dseq.data_ptr(), doff.data_ptr(), did.data_ptr(), off.tolist(), names,
-                      seeds, occupancy=occupancy, m=m, stream=_stream_handle(stream))
+                      seeds, occupancy=occupancy, m=m, stream=_stream_handle(stream),
+                      kmer=kmer)
     torch.cuda.synchronize()
     return mf
 
 
 class MibfOutputs:
-    def __init__(self, nreads, max_hits, device="cuda"):
-        n = max(1, nreads)
+    """Per-record outputs of bbt_mibf_query_batch (a record is a read, or a
+    pair in paired mode)."""
+
+    def __init__(self, nrec, max_hits, device="cuda"):
+        n = max(1, nrec)
+        nreads = nrec
         self.nhits = torch.zeros(n, dtype=torch.int32, device=device)
-        self.hits = torch.zeros(n * max_hits * 6, dtype=torch.int32, device=device)
+        self.hits = torch.zeros(n * max_hits * B.MIBF_HIT_WORDS, dtype=torch.int32,
+                                device=device)
         self.sat = torch.zeros(n, dtype=torch.int32, device=device)
         self.counters = torch.zeros(4, dtype=torch.int64, device=device)
         self.nreads, self.max_hits = nreads, max_hits
@@ -156,9 +164,9 @@ class MibfOutputs:
     def host(self):
         """(per read list of hit tuples in rank order, per read satCount,
         counters, raw nhits)"""
-        n, mh = self.nreads, self.max_hits
+        n, mh, w = self.nreads, self.max_hits, B.MIBF_HIT_WORDS
         nh = self.nhits[:n].cpu().numpy().view(np.uint32)
-        hits = self.hits[: n * mh * 6].cpu().numpy().view(np.uint32).reshape(n, mh, 6)
+        hits = self.hits[: n * mh * w].cpu().numpy().view(np.uint32).reshape(n, mh, w)
         rows = []
         for r in range(n):
             c = int(nh[r]) if nh[r] != 0xFFFFFFFF else 0
@@ -169,7 +177,8 @@ class MibfOutputs:
 
 def mibf_classify(mf, dreads, params, stream=None, out=None):
     if out is None:
-        out = MibfOutputs(dreads.nreads, params.max_hits, dreads.seq.device)
+        nrec = dreads.nreads // 2 if params.flags & B.MIBF_PAIRED else dreads.nreads
+        out = MibfOutputs(nrec, params.max_hits, dreads.seq.device)
     out.counters.zero_()
     mf.query_batch(dreads.struct(), params, out.struct(), _stream_handle(stream))
     torch.cuda.synchronize()

@@ -54,13 +54,13 @@ class EvalOut(C.Structure):
 class MibfInfo(C.Structure):
     _fields_ = [("m", C.c_uint64), ("pop", C.c_uint64), ("pop_saturated", C.c_uint64),
                 ("nids", C.c_uint32), ("nseeds", C.c_uint32), ("span", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("kmer_mode", C.c_uint32)]
 
 
 class MibfParams(C.Structure):
     _fields_ = [("score", C.c_double), ("multi_thresh", C.c_double),
                 ("allowed_miss", C.c_uint32), ("min_nonsat", C.c_uint32),
-                ("max_hits", C.c_uint32), ("reserved", C.c_uint32)]
+                ("max_hits", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class MibfOut(C.Structure):
@@ -107,9 +107,9 @@ _SIGS = {
     "bbt_mibf_build": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_uint64), C.c_uint64,
                                    C.POINTER(C.c_char_p), C.c_uint32,
-                                   C.POINTER(C.c_char_p), C.c_uint32, C.c_double,
-                                   C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p),
-                                   C.c_void_p]),
+                                   C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
+                                   C.c_double, C.c_uint64, C.c_uint64,
+                                   C.POINTER(C.c_void_p), C.c_void_p]),
     "bbt_mibf_store": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "bbt_mibf_load": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p]),
     "bbt_mibf_free": (C.c_int32, [C.c_void_p]),
@@ -280,18 +280,24 @@ class Mibf:
 
     @classmethod
     def build(cls, dseq_ptr, drec_off_ptr, drec_id_ptr, rec_off, names, seeds,
-              occupancy=0.5, entries=0, m=0, stream=0):
+              occupancy=0.5, entries=0, m=0, stream=0, kmer=None):
         """rec_off: host sequence of nrec + 1 offsets; every record ends with
-        one non-ACGT separator byte."""
+        one non-ACGT separator byte.  seeds: list of '0'/'1' strings, or None
+        with kmer = (k, number of hash values) for k-mer mode."""
         L = load()
         nrec = len(rec_off) - 1
         off = (C.c_uint64 * len(rec_off))(*[int(x) for x in rec_off])
         nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
-        sd = (C.c_char_p * len(seeds))(*[x.encode() for x in seeds])
+        if seeds is None:
+            k, nh = kmer
+            sd = None
+        else:
+            k, nh = 0, len(seeds)
+            sd = (C.c_char_p * len(seeds))(*[x.encode() for x in seeds])
         h = C.c_void_p()
         check(L.bbt_mibf_build(C.c_void_p(dseq_ptr), C.c_void_p(drec_off_ptr),
                                C.c_void_p(drec_id_ptr), off, nrec, nm, len(names), sd,
-                               len(seeds), occupancy, entries, m, C.byref(h),
+                               nh, k, occupancy, entries, m, C.byref(h),
                                C.c_void_p(stream)))
         return cls(h.value)
 
@@ -322,8 +328,11 @@ class Mibf:
         return None if r is None else r.decode()
 
     def seeds(self):
-        n = self.info().nseeds
-        return [load().bbt_mibf_seed(self._h, i).decode() for i in range(n)]
+        """spaced seeds, or None in k-mer mode"""
+        info = self.info()
+        if info.kmer_mode:
+            return None
+        return [load().bbt_mibf_seed(self._h, i).decode() for i in range(info.nseeds)]
 
     def download(self, bv, data):
         check(load().bbt_mibf_download(self._h, bv.ctypes.data, data.ctypes.data))
@@ -342,5 +351,12 @@ class Mibf:
                                           C.byref(out), C.c_void_p(stream)))
 
 
-def mibf_params(score=0.1, multi_thresh=1.0, allowed_miss=0, min_nonsat=1, max_hits=8):
-    return MibfParams(score, multi_thresh, allowed_miss, min_nonsat, max_hits, 0)
+MIBF_PAIRED = 1
+MIBF_BEST_AGREE = 2
+MIBF_HIT_WORDS = 7
+
+
+def mibf_params(score=0.1, multi_thresh=1.0, allowed_miss=0, min_nonsat=1, max_hits=8,
+                paired=False, best_agree=False):
+    flags = (MIBF_PAIRED if paired else 0) | (MIBF_BEST_AGREE if best_agree else 0)
+    return MibfParams(score, multi_thresh, allowed_miss, min_nonsat, max_hits, flags)

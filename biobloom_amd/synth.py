@@ -12,6 +12,11 @@ Workloads (reference flags in parentheses):
   uneven   4 x 100 Mbp genomes, k=25 FPR 0.05; 200k log-normal long reads
            (median 2 kbp, 100 bp-50 kbp, 5% subst 1.5% ins 1.5% del)
            interleaved with 2M x 150 bp reads
+  mibf     200 x 5 Mbp references ref000..ref199, each with 5% of its sequence
+           overwritten by 10 kbp segments copied from other references'
+           original sequence; 3 spaced seeds of span 25 and weight 20 closed
+           under reversal; 2M x 150 bp reads, 90% from the references (1%
+           substitutions), 10% random
 """
 import numpy as np
 
@@ -193,3 +198,45 @@ def workload_uneven(scale=1.0):
     reads = [items[i] for i in perm]
     return {"genomes": genomes, "k": 25, "fpr": 0.05, "reads_list": reads,
             "paired": False}
+
+
+def mibf_seeds(rng, span=25, weight=20):
+    """Three spaced seeds closed under reversal: s0 random with both ends
+    set, s2 = reverse(s0), s1 a palindrome with the centre unset and
+    weight/2 of the mirrored pairs (the end pair included) set."""
+    inner = np.zeros(span - 2, np.uint8)
+    inner[rng.choice(span - 2, size=weight - 2, replace=False)] = 1
+    s0 = "1" + "".join(map(str, inner)) + "1"
+    half = span // 2
+    pairs = np.zeros(half, np.uint8)
+    pairs[0] = 1
+    pairs[1 + rng.choice(half - 1, size=weight // 2 - 1, replace=False)] = 1
+    left = "".join(map(str, pairs))
+    s1 = left + ("0" if span % 2 else "") + left[::-1]
+    return [s0, s1, s0[::-1]]
+
+
+def workload_mibf(scale=1.0, nrefs=200, ref_len=5_000_000, seg=10_000):
+    rng = rng_for(4)
+    rlen = max(4 * seg, int(ref_len * min(1.0, scale)))
+    rlen -= rlen % seg
+    orig = [codes_to_ascii(rng_for(4, 1000 + g).integers(0, 4, size=rlen, dtype=np.uint8))
+            for g in range(nrefs)]
+    nseg = max(1, int(round(0.05 * rlen / seg)))
+    refs = []
+    for g in range(nrefs):
+        r = orig[g].copy()
+        slots = rng.choice(rlen // seg, size=nseg, replace=False)
+        for sl in slots:
+            other = int(rng.integers(0, nrefs - 1))
+            other += other >= g
+            src = int(rng.integers(0, rlen // seg)) * seg
+            r[sl * seg:(sl + 1) * seg] = orig[other][src:src + seg]
+        refs.append(r)
+    del orig
+    seeds = mibf_seeds(rng)
+    n = max(10, int(2_000_000 * scale))
+    reads, src, gsel = fixed_len_reads(rng, refs, n, 150, 0.01, 0.9)
+    names = [f"ref{g:03d}" for g in range(nrefs)]
+    return {"refs": refs, "names": names, "seeds": seeds, "reads": reads, "src": src,
+            "gsel": gsel, "paired": False, "genomes": refs}

@@ -221,33 +221,40 @@ typedef struct {
     uint64_t pop_saturated; /* slots with the saturation flag */
     uint32_t nids;
     uint32_t nseeds;
-    uint32_t span;          /* seed length (window size) */
-    uint32_t reserved;
+    uint32_t span;          /* seed length / k-mer size (window size) */
+    uint32_t kmer_mode;     /* 1: nseeds ntHash values per k-mer, no seeds */
 } bbt_mibf_info_t;
+
+#define BBT_MIBF_PAIRED 1     /* reads 2j, 2j+1 are mates: one record per pair */
+#define BBT_MIBF_BEST_AGREE 2 /* -b: drop results whose best hit is ambiguous */
 
 typedef struct {
     double score;           /* opt::score: significance spread over the IDs */
     double multi_thresh;    /* report candidates with nonSat >= this x best's */
     uint32_t allowed_miss;  /* seeds of a frame that may miss (S - frameMatches) */
     uint32_t min_nonsat;    /* opt::minCountNonSatCount */
-    uint32_t max_hits;      /* hit records per read in bbt_mibf_out.hits (1..64) */
-    uint32_t reserved;
+    uint32_t max_hits;      /* hit records per record in bbt_mibf_out.hits (1..64) */
+    uint32_t flags;         /* BBT_MIBF_* */
 } bbt_mibf_params;
 
 typedef struct {
-    uint32_t *nhits;    /* device, per read: reported candidates (may exceed
-                           max_hits; 0xFFFFFFFF = not evaluated, see counters[3]) */
-    uint32_t *hits;     /* device, per read max_hits records of 6 words in rank
-                           order: id, count, nonSat, total, totalNonSat, solid */
-    uint32_t *sat;      /* device, per read: frames whose hit slots are all
-                           saturated */
+    uint32_t *nhits;    /* device, per record (read or pair): reported candidates
+                           (may exceed max_hits; 0xFFFFFFFF = not evaluated,
+                           see counters[3]) */
+    uint32_t *hits;     /* device, per record max_hits records of 7 words in rank
+                           order: id, count, nonSat, total, totalNonSat, solid,
+                           nonSatFrame */
+    uint32_t *sat;      /* device, per record: frames whose hit slots are all
+                           saturated (satCount) */
     uint64_t *counters; /* device, 4 counters ADDED TO: bit lookups, ID slot
-                           reads, candidates beyond max_hits, reads not
+                           reads, candidates beyond max_hits, records not
                            evaluated (more than 64 candidates) */
 } bbt_mibf_out;
 
 /* Builds on the device from reference records packed back to back in dseq,
- * record r at [rec_off[r], rec_off[r+1]) ENDING WITH one non-ACGT separator
+ * hashing with nseeds spaced seeds, or, with seeds = NULL, in k-mer mode with
+ * nseeds ntHash values per kmer_size-mer (biobloommimaker without -S);
+ * records: record r at [rec_off[r], rec_off[r+1]) ENDING WITH one non-ACGT separator
  * byte (rejected otherwise).  drec_off / drec_id are device copies of the
  * nrec + 1 offsets and of the record IDs (1..nids: one per record, or one per
  * file); rec_off is the host copy.  names: nids strings (ID i = names[i-1]).
@@ -256,8 +263,8 @@ typedef struct {
 bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint32_t *drec_id,
                           const uint64_t *rec_off, uint64_t nrec, const char *const *names,
                           uint32_t nids, const char *const *seeds, uint32_t nseeds,
-                          double occupancy, uint64_t entries, uint64_t m_override,
-                          bbt_mibf **out, void *stream);
+                          uint32_t kmer_size, double occupancy, uint64_t entries,
+                          uint64_t m_override, bbt_mibf **out, void *stream);
 /* Writes <prefix>.bf and <prefix>_ids.txt. */
 bbt_status bbt_mibf_store(const bbt_mibf *mf, const char *prefix);
 /* Reads <x>.bf and the ID file <x>_ids.txt beside it. */
@@ -266,6 +273,7 @@ bbt_status bbt_mibf_free(bbt_mibf *mf);
 bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info);
 /* name of ID id (0 = "repeat"), or NULL */
 const char *bbt_mibf_name(const bbt_mibf *mf, uint32_t id);
+/* spaced seed i, or NULL (k-mer mode, i out of range) */
 const char *bbt_mibf_seed(const bbt_mibf *mf, uint32_t i);
 /* host copies: bv = m/64 words of the bit vector, data = pop slots */
 bbt_status bbt_mibf_download(const bbt_mibf *mf, uint64_t *bv, uint32_t *data);
@@ -273,8 +281,9 @@ uint64_t bbt_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy)
 /* probs[0..nids]: chance that a random frame reports each ID */
 bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, double *probs);
 uint32_t bbt_mibf_min_count(uint64_t frames, double prob, double score, uint32_t nids);
-/* Classifies a batch of reads (bbt_reads as for bbt_eval_batch; order is
- * ignored); one wavefront per read. */
+/* Classifies a batch of reads or pairs (bbt_reads as for bbt_eval_batch;
+ * order is ignored); one wavefront per record.  Records of up to 8192
+ * frames x hashes (both mates of a pair together). */
 bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mibf_params *p,
                                 bbt_mibf_out *out, void *stream);
 

@@ -14,6 +14,9 @@
  * the specification the device path is checked against ("parity unpinned"
  * against upstream):
  *
+ *  hashing     spaced-seed mode: one stHash value per seed per N-free window;
+ *              k-mer mode (no seeds): the S ntHash values of each k-mer
+ *              (sntHashIterator is ntHashIterator, Common/sntHashIterator.hpp).
  *  bit vector  m bits, m = calcOptimalSize(entries, #seeds, occupancy) with
  *              the occupancy form m = -n*S/ln(1-occ) rounded up as
  *              BloomFilterInfo::calcOptimalSize rounds; bit p of seed hash h
@@ -29,14 +32,21 @@
  *              at least S - allowedMiss bits are set; each distinct ID among
  *              the hit slots counts once per frame (count), once more if one
  *              of its slots is unsaturated (nonSat), by slots (total,
- *              totalNonSat), and as solid when all S slots hold it.  A frame
- *              whose hit slots are all saturated adds to satCount.
+ *              totalNonSat), as solid when all S slots hold it, and as a
+ *              non-saturated frame when no hit slot of the frame is saturated
+ *              (nonSatFrame).  A frame whose hit slots are all saturated adds
+ *              to satCount.  A pair pools the frames of both mates; its
+ *              frameCount follows the nesting of MIBFClassifier.hpp:804-805
+ *              (mate 1's frames, or mate 2's when mate 1 is shorter than k).
  *  candidates  count >= minCount[id] (binomial quantile at score/(#IDs) on
  *              the ID's per-frame probability, at least 1, frameCount =
  *              len - span + 1) and nonSat >= minNoSat; ranked by nonSat,
  *              count, solid, totalNonSat, total (descending) then ID; the best
  *              and every candidate with nonSat >= multiThresh * best.nonSat
- *              are reported in rank order.
+ *              are reported in rank order.  bestHitAgree (-b): no result
+ *              unless the best candidate leads (ties allowed) in every
+ *              statistic.  The streak option (-r) stops no query early here:
+ *              every frame is evaluated.
  *  frame prob  P(id) = sum_{j=S-a..S} C(S,j) occ^j (1-occ)^(S-j) (1-(1-f_id)^j)
  *              with occ = popcount/m, f_id = share of ID slots holding id.
  */
@@ -53,7 +63,10 @@ namespace {
 const uint32_t kSat = 0x80000000u;
 const uint32_t kIdMask = 0x7FFFFFFFu;
 
-/* spaced-seed hashes of every N-free window of s: frames[] starts, hv[i*S+j] */
+/* hashes of every N-free window of s: window starts, hv[i*S+j].  Spaced-seed
+ * mode (stHashIterator, one hash per seed) when seeds is set; k-mer mode
+ * (ntHashIterator with S hashes per k-mer, Common/sntHashIterator.hpp:18-35)
+ * when seeds is NULL. */
 void seed_hashes(const char *s, uint64_t len, const char *seeds, uint32_t S, uint32_t k,
                  std::vector<uint64_t> &pos, std::vector<uint64_t> &hv) {
     pos.clear();
@@ -62,7 +75,8 @@ void seed_hashes(const char *s, uint64_t len, const char *seeds, uint32_t S, uin
         return;
     int64_t n = int64_t(len);
     std::vector<uint64_t> p(static_cast<size_t>(n)), h(static_cast<size_t>(n) * S);
-    int64_t cnt = orc_sthash(s, len, seeds, S, k, p.data(), h.data(), uint64_t(n));
+    int64_t cnt = seeds ? orc_sthash(s, len, seeds, S, k, p.data(), h.data(), uint64_t(n))
+                        : orc_nthash(s, len, k, S, p.data(), h.data(), uint64_t(n));
     pos.assign(p.begin(), p.begin() + cnt);
     hv.assign(h.begin(), h.begin() + cnt * S);
 }
@@ -184,74 +198,97 @@ void orc_mibf_frame_probs(const uint32_t *data, uint64_t pop, uint64_t m, uint32
 int orc_mibf_query(const uint64_t *bv, const uint32_t *data, uint64_t m, const char *seeds,
                    uint32_t S, uint32_t k, const double *probs, uint32_t nids, double score,
                    uint32_t allowed_miss, double multi_thresh, uint32_t min_nonsat,
-                   const char *seq, const uint64_t *off, uint64_t nreads, uint32_t max_hits,
-                   orc_mibf_hit *hits, uint32_t *nhits, uint32_t *satcount,
-                   uint64_t *counters) {
+                   int paired, int best_agree, const char *seq, const uint64_t *off,
+                   uint64_t nreads, uint32_t max_hits, orc_mibf_hit *hits,
+                   uint32_t *nhits, uint32_t *satcount, uint64_t *counters) {
     Rank R(bv, m);
+    const uint64_t nrec = paired ? nreads / 2 : nreads;
+    /* frameCount of each record; MIBFClassifier.hpp:804-805 nests the
+     * conditionals, so a pair counts mate 2 only when mate 1 is shorter than k */
+    std::vector<uint64_t> fcs(nrec);
+    for (uint64_t r = 0; r < nrec; ++r) {
+        const uint64_t a0 = paired ? 2 * r : r;
+        const uint64_t len1 = off[a0 + 1] - off[a0];
+        uint64_t fc = len1 >= k ? len1 - k + 1 : 0;
+        if (paired && len1 < k) {
+            const uint64_t len2 = off[a0 + 2] - off[a0 + 1];
+            fc = len2 >= k ? len2 - k + 1 : 0;
+        }
+        fcs[r] = fc;
+    }
+    /* minCount tables (getMinCount, MIBFClassifier.hpp:930-939) per frameCount */
     std::map<uint64_t, std::vector<uint32_t>> minCount;
-    std::vector<uint64_t> pos, hv;
+    for (uint64_t fc : fcs) {
+        if (fc == 0 || minCount.count(fc))
+            continue;
+        std::vector<uint32_t> t(size_t(nids) + 1, 1);
+        for (uint32_t id = 1; id <= nids; ++id) {
+            uint64_t q = orc_binom_isf_up(score / double(nids), fc, probs[id]);
+            t[id] = q > 1 ? uint32_t(q) : 1;
+        }
+        minCount.emplace(fc, std::move(t));
+    }
     uint64_t bvl = 0, gathers = 0, overflow = 0;
-    for (uint64_t r = 0; r < nreads; ++r) {
-        const char *s = seq + off[r];
-        const uint64_t len = off[r + 1] - off[r];
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bvl, gathers, overflow)
+    for (uint64_t r = 0; r < nrec; ++r) {
         nhits[r] = 0;
         satcount[r] = 0;
-        if (len < k)
+        const uint64_t frameCount = fcs[r];
+        if (frameCount == 0)
             continue;
-        const uint64_t frameCount = len - k + 1;
-        auto it = minCount.find(frameCount);
-        if (it == minCount.end()) {
-            std::vector<uint32_t> t(size_t(nids) + 1, 1);
-            for (uint32_t id = 1; id <= nids; ++id) {
-                uint64_t q = orc_binom_isf_up(score / double(nids), frameCount, probs[id]);
-                t[id] = q > 1 ? uint32_t(q) : 1;
-            }
-            it = minCount.emplace(frameCount, std::move(t)).first;
-        }
-        const std::vector<uint32_t> &mc = it->second;
-        seed_hashes(s, len, seeds, S, k, pos, hv);
+        const std::vector<uint32_t> &mc = minCount.find(frameCount)->second;
+        const uint64_t a0 = paired ? 2 * r : r;
+        std::vector<uint64_t> pos, hv;
         std::map<uint32_t, orc_mibf_hit> acc;
-        for (size_t f = 0; f < pos.size(); ++f) {
-            uint32_t raw[32];
-            uint32_t nh = 0;
-            for (uint32_t j = 0; j < S; ++j) {
-                const uint64_t p = hv[f * S + j] % m;
-                ++bvl;
-                if (bit(bv, p)) {
-                    raw[nh++] = data[R.rank1(bv, p)];
-                    ++gathers;
-                }
-            }
-            if (nh + allowed_miss < S)
-                continue;
-            bool allsat = nh > 0;
-            for (uint32_t a = 0; a < nh; ++a)
-                if (!(raw[a] & kSat))
-                    allsat = false;
-            if (allsat)
-                ++satcount[r];
-            for (uint32_t a = 0; a < nh; ++a) {
-                const uint32_t id = raw[a] & kIdMask;
-                bool seen = false;
-                for (uint32_t b = 0; b < a; ++b)
-                    if ((raw[b] & kIdMask) == id)
-                        seen = true;
-                if (seen)
-                    continue;
-                uint32_t occ = 0, ns = 0;
-                for (uint32_t b = a; b < nh; ++b)
-                    if ((raw[b] & kIdMask) == id) {
-                        ++occ;
-                        if (!(raw[b] & kSat))
-                            ++ns;
+        for (uint64_t mate = a0; mate < a0 + (paired ? 2 : 1); ++mate) {
+            const char *s = seq + off[mate];
+            seed_hashes(s, off[mate + 1] - off[mate], seeds, S, k, pos, hv);
+            for (size_t f = 0; f < pos.size(); ++f) {
+                uint32_t raw[32];
+                uint32_t nh = 0;
+                for (uint32_t j = 0; j < S; ++j) {
+                    const uint64_t p = hv[f * S + j] % m;
+                    ++bvl;
+                    if (bit(bv, p)) {
+                        raw[nh++] = data[R.rank1(bv, p)];
+                        ++gathers;
                     }
-                orc_mibf_hit &h = acc[id];
-                h.id = id;
-                h.count += 1;
-                h.nonsat += ns ? 1 : 0;
-                h.total += occ;
-                h.total_nonsat += ns;
-                h.solid += occ == S ? 1 : 0;
+                }
+                if (nh + allowed_miss < S)
+                    continue;
+                bool allsat = nh > 0, anysat = false;
+                for (uint32_t a = 0; a < nh; ++a) {
+                    if (!(raw[a] & kSat))
+                        allsat = false;
+                    else
+                        anysat = true;
+                }
+                if (allsat)
+                    ++satcount[r];
+                for (uint32_t a = 0; a < nh; ++a) {
+                    const uint32_t id = raw[a] & kIdMask;
+                    bool seen = false;
+                    for (uint32_t b = 0; b < a; ++b)
+                        if ((raw[b] & kIdMask) == id)
+                            seen = true;
+                    if (seen)
+                        continue;
+                    uint32_t occ = 0, ns = 0;
+                    for (uint32_t b = a; b < nh; ++b)
+                        if ((raw[b] & kIdMask) == id) {
+                            ++occ;
+                            if (!(raw[b] & kSat))
+                                ++ns;
+                        }
+                    orc_mibf_hit &h = acc[id];
+                    h.id = id;
+                    h.count += 1;
+                    h.nonsat += ns ? 1 : 0;
+                    h.total += occ;
+                    h.total_nonsat += ns;
+                    h.solid += occ == S ? 1 : 0;
+                    h.nonsat_frame += anysat ? 0 : 1;
+                }
             }
         }
         std::vector<orc_mibf_hit> cand;
@@ -270,6 +307,18 @@ int orc_mibf_query(const uint64_t *bv, const uint32_t *data, uint64_t m, const c
             if (a.total != b.total) return a.total > b.total;
             return a.id < b.id;
         });
+        if (best_agree) {
+            /* best hit must lead (ties allowed) in every statistic */
+            const orc_mibf_hit &t = cand[0];
+            bool agree = true;
+            for (const orc_mibf_hit &h : cand)
+                if (h.count > t.count || h.nonsat > t.nonsat || h.total > t.total ||
+                    h.total_nonsat > t.total_nonsat || h.solid > t.solid ||
+                    h.nonsat_frame > t.nonsat_frame)
+                    agree = false;
+            if (!agree)
+                continue;
+        }
         const double bar = multi_thresh * double(cand[0].nonsat);
         uint32_t n = 0;
         for (const orc_mibf_hit &h : cand) {

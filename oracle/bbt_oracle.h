@@ -112,8 +112,10 @@ uint64_t orc_summary(const char *const *filter_ids, uint32_t nf,
                      int inclusive, char *out, uint64_t cap);
 
 /* ---- multi-index Bloom filter (bbt_mibf.cpp) ------------------------- */
+/* seeds: nseeds concatenated '0'/'1' strings of length span (spaced-seed
+ * mode), or NULL for k-mer mode (nseeds ntHash values per span-mer). */
 typedef struct {
-    uint32_t id, count, nonsat, total, total_nonsat, solid;
+    uint32_t id, count, nonsat, total, total_nonsat, solid, nonsat_frame;
 } orc_mibf_hit;
 
 uint64_t orc_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy);
@@ -128,7 +130,8 @@ void orc_mibf_frame_probs(const uint32_t *data, uint64_t pop, uint64_t m, uint32
 int orc_mibf_query(const uint64_t *bv, const uint32_t *data, uint64_t m, const char *seeds,
                    uint32_t nseeds, uint32_t span, const double *probs, uint32_t nids,
                    double score, uint32_t allowed_miss, double multi_thresh,
-                   uint32_t min_nonsat, const char *seq, const uint64_t *off,
+                   uint32_t min_nonsat, int paired, int best_agree, const char *seq,
+                   const uint64_t *off,
                    uint64_t nreads, uint32_t max_hits, orc_mibf_hit *hits,
                    uint32_t *nhits, uint32_t *satcount, uint64_t *counters);
 

@@ -20,7 +20,10 @@ def mix32(x):
 
 
 def frames(seq, seeds):
-    pos, hv = O.sthash(seq, seeds, len(seeds[0]))
+    if isinstance(seeds, O.Kmer):
+        pos, hv = O.nthash(seq, seeds.k, seeds.h)
+    else:
+        pos, hv = O.sthash(seq, seeds, len(seeds[0]))
     return [int(p) for p in pos], [[int(v) for v in row] for row in hv]
 
 
@@ -69,37 +72,53 @@ def frame_probs(data, pop, m, S, a, nids):
     return out
 
 
-def query(order, data, m, seeds, probs, nids, score, a, multi, min_nonsat, seq):
-    S, k = len(seeds), len(seeds[0])
-    if len(seq) < k:
+def query(order, data, m, seeds, probs, nids, score, a, multi, min_nonsat, seq, seq2=None,
+          best_agree=False):
+    """Hits of one read (or of the pair seq, seq2) and its satCount."""
+    S, k, _ = O.seed_shape(seeds)
+    if seq2 is None:
+        fc = len(seq) - k + 1 if len(seq) >= k else 0
+        mates = [seq]
+    else:
+        # the precedence of MIBFClassifier.hpp:804-805
+        fc = (len(seq) - k + 1 if len(seq) >= k else
+              (len(seq2) - k + 1 if len(seq2) >= k else 0))
+        mates = [seq, seq2]
+    if fc == 0:
         return [], 0
-    fc = len(seq) - k + 1
     slot = {p: i for i, p in enumerate(order)}
     acc = {}
     sat = 0
-    pos, hv = frames(seq, seeds)
-    for row in hv:
-        raw = [data[slot[h % m]] for h in row if (h % m) in slot]
-        if len(raw) + a < S:
-            continue
-        if raw and all(r & SAT for r in raw):
-            sat += 1
-        for iid in {r & MASK for r in raw}:
-            occ = [r for r in raw if (r & MASK) == iid]
-            ns = sum(1 for r in occ if not r & SAT)
-            h = acc.setdefault(iid, [0, 0, 0, 0, 0])
-            h[0] += 1
-            h[1] += 1 if ns else 0
-            h[2] += len(occ)
-            h[3] += ns
-            h[4] += 1 if len(occ) == S else 0
+    for s in mates:
+        pos, hv = frames(s, seeds)
+        for row in hv:
+            raw = [data[slot[h % m]] for h in row if (h % m) in slot]
+            if len(raw) + a < S:
+                continue
+            if raw and all(r & SAT for r in raw):
+                sat += 1
+            clean = not any(r & SAT for r in raw)
+            for iid in {r & MASK for r in raw}:
+                occ = [r for r in raw if (r & MASK) == iid]
+                ns = sum(1 for r in occ if not r & SAT)
+                h = acc.setdefault(iid, [0, 0, 0, 0, 0, 0])
+                h[0] += 1
+                h[1] += 1 if ns else 0
+                h[2] += len(occ)
+                h[3] += ns
+                h[4] += 1 if len(occ) == S else 0
+                h[5] += 1 if clean else 0
     cand = []
-    for iid, (cnt, nsat, tot, tns, solid) in acc.items():
+    for iid, (cnt, nsat, tot, tns, solid, nsf) in acc.items():
         mc = max(1, int(O.lib().orc_binom_isf_up(score / nids, fc, probs[iid])))
         if cnt >= mc and nsat >= min_nonsat:
-            cand.append((iid, cnt, nsat, tot, tns, solid))
+            cand.append((iid, cnt, nsat, tot, tns, solid, nsf))
     cand.sort(key=lambda c: (-c[2], -c[1], -c[5], -c[4], -c[3], c[0]))
     if not cand:
         return [], sat
+    if best_agree:
+        top = cand[0]
+        if any(c[f] > top[f] for c in cand for f in range(1, 7)):
+            return [], sat
     bar = multi * cand[0][2]
     return [c for c in cand if c[2] >= bar], sat

@@ -242,7 +242,8 @@ def summary(filter_ids, hits, paired=False, inclusive=False):
 
 class MibfHit(C.Structure):
     _fields_ = [("id", C.c_uint32), ("count", C.c_uint32), ("nonsat", C.c_uint32),
-                ("total", C.c_uint32), ("total_nonsat", C.c_uint32), ("solid", C.c_uint32)]
+                ("total", C.c_uint32), ("total_nonsat", C.c_uint32), ("solid", C.c_uint32),
+                ("nonsat_frame", C.c_uint32)]
 
 
 _mibf_ready = False
@@ -267,20 +268,37 @@ def _mibf_sigs():
     L.orc_mibf_query.restype = C.c_int
     L.orc_mibf_query.argtypes = [vp, vp, C.c_uint64, C.c_char_p, C.c_uint32, C.c_uint32,
                                  vp, C.c_uint32, C.c_double, C.c_uint32, C.c_double,
-                                 C.c_uint32, vp, vp, C.c_uint64, C.c_uint32, vp, vp, vp,
-                                 vp]
+                                 C.c_uint32, C.c_int, C.c_int, vp, vp, C.c_uint64,
+                                 C.c_uint32, vp, vp, vp, vp]
     _mibf_ready = True
     return L
 
 
+class Kmer:
+    """k-mer mode of a MIBF: h ntHash values per k-mer instead of spaced seeds."""
+
+    def __init__(self, k, h):
+        self.k, self.h = k, h
+
+    def __len__(self):
+        return self.h
+
+
+def seed_shape(seeds):
+    """(number of hashes per frame, frame span, seed bytes or None)"""
+    if isinstance(seeds, Kmer):
+        return seeds.h, seeds.k, None
+    return len(seeds), len(seeds[0]), "".join(seeds).encode()
+
+
 class Mibf:
-    """Oracle MIBF built from (id, sequence) records (ids 1..n)."""
+    """Oracle MIBF built from (id, sequence) records (ids 1..n); seeds is a
+    list of '0'/'1' strings or a Kmer."""
 
     def __init__(self, records, seeds, occupancy=0.5, m=None):
         L = _mibf_sigs()
-        self.seeds = list(seeds)
-        self.S, self.k = len(seeds), len(seeds[0])
-        sd = "".join(seeds).encode()
+        self.seeds = seeds
+        self.S, self.k, sd = seed_shape(seeds)
         entries = sum(max(0, len(s) - self.k + 1) for _, s in records)
         self.m = int(m or L.orc_mibf_calc_size(entries, self.S, occupancy))
         self.bv = np.zeros(self.m // 64, np.uint64)
@@ -295,6 +313,19 @@ class Mibf:
                           self.S, self.k, self.m, self.bv.ctypes.data, self.data.ctypes.data)
         self.nids = int(ids.max()) if len(ids) else 0
 
+    @classmethod
+    def from_arrays(cls, bv, data, m, seeds, nids):
+        """Wraps an existing bit vector (m/64 words) and ID array."""
+        self = cls.__new__(cls)
+        self.seeds = seeds
+        self.S, self.k, _ = seed_shape(seeds)
+        self.m = int(m)
+        self.bv = np.ascontiguousarray(bv, np.uint64)
+        self.pop = int(_mibf_sigs().orc_mibf_popcount(self.bv.ctypes.data, self.m))
+        self.data = np.ascontiguousarray(data, np.uint32)
+        self.nids = int(nids)
+        return self
+
     def frame_probs(self, allowed_miss):
         out = np.zeros(self.nids + 1, np.float64)
         _mibf_sigs().orc_mibf_frame_probs(self.data.ctypes.data, self.pop, self.m, self.S,
@@ -302,26 +333,30 @@ class Mibf:
         return out
 
     def query(self, reads, score=1e-10, allowed_miss=None, multi=1.0, min_nonsat=0,
-              max_hits=16):
+              max_hits=16, paired=False, best_agree=False):
+        """Per record (read, or pair of reads 2j/2j+1): hit tuples
+        (id, count, nonsat, total, total_nonsat, solid, nonsat_frame) in rank
+        order; satCount per record; counters."""
         L = _mibf_sigs()
         a = self.S - 1 if allowed_miss is None else allowed_miss
         probs = self.frame_probs(a)
         buf, off = pack_reads(reads)
-        n = len(reads)
+        n = len(reads) // 2 if paired else len(reads)
         hits = (MibfHit * max(1, n * max_hits))()
         nh = np.zeros(max(1, n), np.uint32)
         sat = np.zeros(max(1, n), np.uint32)
         ctr = np.zeros(3, np.uint64)
         L.orc_mibf_query(self.bv.ctypes.data, self.data.ctypes.data, self.m,
-                         "".join(self.seeds).encode(), self.S, self.k, probs.ctypes.data,
-                         self.nids, score, a, multi, min_nonsat, buf.ctypes.data,
-                         off.ctypes.data, n, max_hits, hits, nh.ctypes.data, sat.ctypes.data,
-                         ctr.ctypes.data)
+                         seed_shape(self.seeds)[2], self.S, self.k, probs.ctypes.data,
+                         self.nids, score, a, multi, min_nonsat, int(paired), int(best_agree),
+                         buf.ctypes.data, off.ctypes.data, len(reads), max_hits, hits,
+                         nh.ctypes.data, sat.ctypes.data, ctr.ctypes.data)
         res = []
         for r in range(n):
             row = []
             for j in range(min(int(nh[r]), max_hits)):
                 h = hits[r * max_hits + j]
-                row.append((h.id, h.count, h.nonsat, h.total, h.total_nonsat, h.solid))
+                row.append((h.id, h.count, h.nonsat, h.total, h.total_nonsat, h.solid,
+                            h.nonsat_frame))
             res.append(row)
         return res, sat[:n].tolist(), ctr

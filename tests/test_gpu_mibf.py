@@ -205,3 +205,76 @@ def test_records_without_separator_rejected(dev):
     with pytest.raises(B.BbtError, match="status 1"):
         B.Mibf.build(dseq.data_ptr(), doff.data_ptr(), did.data_ptr(), off, ["a", "b"],
                      ["1101", "111"], occupancy=0.5)
+
+
+def test_kmer_mode_matches_oracle(dev, tmp_path):
+    """biobloommimaker without -S: h ntHash values per k-mer."""
+    D, B = dev
+    rng = random.Random(31)
+    recs = make_refs(rng, n=5, L=2000)
+    km = O.Kmer(25, 4)
+    om = O.Mibf(recs, km)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], None, kmer=(25, 4))
+    info, bv, data = download(mf)
+    assert info.kmer_mode == 1 and info.nseeds == 4 and info.span == 25
+    assert mf.seeds() is None
+    assert info.m == om.m
+    assert np.array_equal(bv, om.bv)
+    assert np.array_equal(data[: om.pop], om.data[: om.pop])
+    reads = reads_from(rng, recs, 200)
+    dr = device_reads(D, reads)
+    print("[gpu] mibf k-mer mode query", flush=True)
+    want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=0, multi=1.0, max_hits=8)
+    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))
+    assert got == want and gsat == wsat
+    assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+    mf.store(str(tmp_path / "km"))
+    mf2 = B.Mibf.load(str(tmp_path / "km.bf"))
+    assert mf2.info().kmer_mode == 1
+    assert D.mibf_classify(mf2, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))[:2] == (got, gsat)
+
+
+@pytest.mark.parametrize("best_agree", [False, True])
+def test_pairs_match_oracle(dev, best_agree):
+    """Pairs pool both mates' frames with the reference's frameCount nesting
+    (short first mates included); bestHitAgree on and off."""
+    D, B = dev
+    rng = random.Random(50 + best_agree)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=3000)
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = []
+    for i in range(300):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - 400)
+        l1 = 5 if i % 7 == 0 else rng.choice([30, 100, 150])
+        l2 = 8 if i % 11 == 0 else rng.choice([60, 150, 250])
+        reads.append(ref[a:a + l1])
+        reads.append(ref[a + 150:a + 150 + l2])
+    dr = device_reads(D, reads)
+    for score, a, multi in [(1e-10, 2, 1.0), (1e-3, 1, 0.0)]:
+        print(f"[gpu] mibf pairs best_agree={best_agree} score={score} miss={a}", flush=True)
+        want, wsat, wctr = om.query(reads, score=score, allowed_miss=a, multi=multi,
+                                    max_hits=8, paired=True, best_agree=best_agree)
+        got, gsat, gctr, _ = D.mibf_classify(
+            mf, dr, B.mibf_params(score, multi, a, 0, 8, paired=True, best_agree=best_agree))
+        assert len(got) == len(reads) // 2
+        assert gsat == wsat
+        assert got == want
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[2] == wctr[2]
+    assert sum(len(r) for r in want) > 0
+
+
+def test_odd_paired_batch_rejected(dev):
+    D, B = dev
+    rng = random.Random(3)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=2, L=600)
+    mf = D.build_mibf([s for _, s in recs], [1, 2], ["a", "b"], seeds)
+    dr = device_reads(D, ["ACGT" * 40] * 3)
+    out = D.MibfOutputs(2, 4)
+    with pytest.raises(B.BbtError, match="odd"):
+        mf.query_batch(dr.struct(), B.mibf_params(paired=True, max_hits=4), out.struct())

@@ -90,8 +90,61 @@ def test_query_matches_model(seed):
             want, wsat = M.query(order, data, mb.m, seeds, probs, mb.nids, score, a, multi,
                                  0, r)
             assert st == wsat
-            assert row == [(c[0], c[1], c[2], c[3], c[4], c[5]) for c in want]
+            assert row == [tuple(c) for c in want]
     assert sum(len(r) for r in res) > 0
+    assert any(c[6] > 0 for row in res for c in row), "no non-saturated frame counted"
+
+
+@pytest.mark.parametrize("best_agree", [False, True])
+def test_pairs_and_best_agree_match_model(best_agree):
+    """Pairs pool both mates' frames; frameCount follows the nested
+    conditional of MIBFClassifier.hpp:804-805 (mate 2 only when mate 1 is
+    shorter than k), which a short first mate exercises."""
+    rng = random.Random(40 + best_agree)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=2000)
+    mb = O.Mibf(recs, seeds)
+    order, data = M.build(recs, seeds, mb.m)
+    reads = []
+    for i in range(80):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - 300)
+        l1 = rng.choice([10, 100, 150]) if i % 4 else 5
+        l2 = rng.choice([8, 120, 150])
+        reads.append(ref[a:a + l1])
+        reads.append(ref[a + 150:a + 150 + l2])
+    for score, a, multi in [(1e-10, 2, 1.0), (1e-3, 1, 0.3)]:
+        res, sat, _ = mb.query(reads, score=score, allowed_miss=a, multi=multi, paired=True,
+                               best_agree=best_agree)
+        assert len(res) == len(reads) // 2
+        probs = M.frame_probs(data, mb.pop, mb.m, 3, a, mb.nids)
+        for j, (row, st) in enumerate(zip(res, sat)):
+            want, wsat = M.query(order, data, mb.m, seeds, probs, mb.nids, score, a, multi, 0,
+                                 reads[2 * j], reads[2 * j + 1], best_agree=best_agree)
+            assert st == wsat and row == [tuple(c) for c in want], j
+        assert sum(len(r) for r in res) > 0
+
+
+def test_best_agree_drops_ambiguous_best_hit():
+    """bestHitAgree empties exactly the results whose rank-first candidate
+    trails another candidate in some statistic, and keeps all others."""
+    rng = random.Random(42)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=2000)
+    mb = O.Mibf(recs, seeds)
+    reads = []
+    for _ in range(200):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - 150)
+        reads.append(ref[a:a + rng.choice([40, 100, 150])])
+    plain, _, _ = mb.query(reads, score=1e-3, allowed_miss=2, multi=0.0)
+    agree, _, _ = mb.query(reads, score=1e-3, allowed_miss=2, multi=0.0, best_agree=True)
+    dropped = 0
+    for p, q in zip(plain, agree):
+        ambiguous = bool(p) and any(c[f] > p[0][f] for c in p for f in range(1, 7))
+        assert q == ([] if ambiguous else p)
+        dropped += ambiguous
+    assert dropped > 0
 
 
 def test_reads_classify_to_their_reference():
@@ -108,3 +161,26 @@ def test_reads_classify_to_their_reference():
     res, _, _ = mb.query(reads)
     best = [row[0][0] if row else 0 for row in res]
     assert sum(b == t for b, t in zip(best, truth)) >= 90
+
+
+def test_kmer_mode_matches_model():
+    """k-mer mode (no spaced seeds): h ntHash values per k-mer."""
+    rng = random.Random(21)
+    recs = make_refs(rng, n=4, L=1200)
+    km = O.Kmer(25, 3)
+    mb = O.Mibf(recs, km)
+    order, data = M.build(recs, km, mb.m)
+    got_bits = np.flatnonzero(np.unpackbits(mb.bv.view(np.uint8), bitorder="little"))
+    assert got_bits.tolist() == order
+    assert mb.data[: mb.pop].tolist() == data
+    reads = []
+    for _ in range(30):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - 150)
+        reads.append(ref[a:a + 150])
+    res, sat, _ = mb.query(reads, score=1e-10, allowed_miss=0)
+    probs = M.frame_probs(data, mb.pop, mb.m, 3, 0, mb.nids)
+    for r, row, st in zip(reads, res, sat):
+        want, wsat = M.query(order, data, mb.m, km, probs, mb.nids, 1e-10, 0, 1.0, 0, r)
+        assert st == wsat and row == [tuple(c) for c in want]
+    assert sum(len(r) for r in res) >= 25
