@@ -10,6 +10,11 @@ Prints one JSON line: reads/s of the batched persistent path (value), the
 plain per-batch path beside it, k-mer probes/s, the roofline of the walk
 kernel and the CPU oracle baseline on a bounded sample.
 
+--workload serve: the serving loop (one 3 Gbp filter + 8 contaminant
+filters resident in HBM; requests of 1k-20k reads arrive Poisson at each
+--qps); reads/s and p50/p99 request latency of continuous batching
+(bbt_server) against the plain per-request launch path.
+
 --workload mibf: the multi-index Bloom filter instead (200 x 5 Mbp
 references, 3 spaced seeds span 25 weight 20, 2M x 150 bp reads, the
 biobloommicategorizer defaults -s 100 -a 1 -m 1 -c 0); one step is one
@@ -202,6 +207,99 @@ def bench_mibf(args, torch, D, B):
     print(json.dumps(line), flush=True)
 
 
+def serve_run(B, fs, params, pool, mode, qps, seconds, max_requests, seed, want):
+    """Poisson arrivals at qps submitted in real time; returns latency and
+    throughput figures and whether every result equals the request alone."""
+    plain = mode == "plain"
+    srv = B.Server(fs, params, slots=1 if plain else 4,
+                   mode=B.SERVE_PLAIN if plain else B.SERVE_BATCHED,
+                   device_bytes=4 << 30, max_wave_reads=1 << 18, max_wave_bases=512 << 20)
+    n = max(20, min(max_requests, int(qps * seconds)))
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    ids = []
+    t0 = time.perf_counter()
+    for i in range(n):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        pi = i % len(pool)
+        ids.append((srv.submit(*pool[pi]), pi))
+    srv.drain(900.0)
+    lat, reads, first, last, equal = [], 0, None, 0.0, True
+    for rid, pi in ids:
+        nr = len(pool[pi][1]) - 1
+        h, c, st = srv.result(rid, nr)
+        srv.release(rid)
+        lat.append((st["done_us"] - st["submit_us"]) / 1e3)
+        reads += nr
+        first = st["submit_us"] if first is None else min(first, st["submit_us"])
+        last = max(last, st["done_us"])
+        equal &= bool(np.array_equal(h, want[pi][0]) and np.array_equal(c, want[pi][1]))
+    stats = srv.stats()
+    srv.close()
+    p50, p99 = np.percentile(lat, [50, 99])
+    return {"qps": qps, "requests": n, "reads": reads, "p50_ms": float(p50),
+            "p99_ms": float(p99), "mean_ms": float(np.mean(lat)),
+            "reads_per_s": reads / ((last - first) / 1e6),
+            "offered_reads_per_s": reads / arrivals[-1] if arrivals[-1] > 0 else None,
+            "waves": stats["waves"], "mean_wave_requests": stats["mean_wave_requests"],
+            "max_waves_in_flight": stats["max_waves_in_flight"],
+            "equal_to_alone": equal}
+
+
+def bench_serve(args, torch, D, B):
+    from biobloom_amd import synth
+    t = time.time()
+    w = synth.workload_serve(scale=args.scale)
+    pool = w["requests"]
+    log(f"workload serve: genome {len(w['genomes'][0])} bp + {len(w['genomes']) - 1} "
+        f"contaminants, request pool {len(pool)} (mean "
+        f"{np.mean([len(o) - 1 for _, o in pool]):.0f} reads) ({time.time() - t:.1f}s)")
+    t = time.time()
+    fs, _ = D.build_filters(w["genomes"], w["k"], w["fpr"])
+    torch.cuda.synchronize()
+    big = fs.info(0)
+    log(f"filters built on device ({time.time() - t:.1f}s): big m={big.m} "
+        f"({big.m / 8 / 2**30:.2f} GiB), h={big.h}")
+    del w
+    params = B.params()
+    # each pool request alone through bbt_eval_batch: the results every
+    # served copy must reproduce
+    want = []
+    t = time.time()
+    for buf, off in pool:
+        r = D.classify(fs, D.DeviceReads(buf, off), params)
+        want.append((r["hits"], r["cls"]))
+    log(f"pool classified alone ({time.time() - t:.1f}s)")
+    qps_list = [float(x) for x in args.qps.split(",")]
+    runs = {"plain": [], "batched": []}
+    for mode in ("plain", "batched"):
+        for i, qps in enumerate(qps_list):
+            r = serve_run(B, fs, params, pool, mode, qps, args.serve_seconds,
+                          args.max_requests, 1000 + i, want)
+            runs[mode].append(r)
+            log(f"{mode} qps={qps:g}: {r['reads_per_s'] / 1e6:.2f} M reads/s, p50 "
+                f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, waves {r['waves']}, "
+                f"{r['mean_wave_requests']:.2f} req/wave, equal={r['equal_to_alone']}")
+    top = runs["batched"][-1]
+    line = {
+        "metric": METRIC, "value": top["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
+        "steps": top["requests"], "warmup": 0,
+        "ms_per_step": top["mean_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "serving: 3 Gbp genome (50% 300 bp repeats, 10% divergence) "
+                               "+ 8 x 20 Mbp contaminant filters, k=25 FPR 0.075; Poisson "
+                               "requests of 1k-20k reads, 80% 150 bp / 20% log-normal long",
+                   "scale": args.scale, "qps": qps_list, "filters": len(fs),
+                   "big_filter_bits": big.m},
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+        "serving": runs,
+        "alone_equals_mixed": all(r["equal_to_alone"] for m in runs.values() for r in m),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_walk_{workload}.json")
     if os.path.exists(p):
@@ -216,7 +314,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi",
-                    choices=["multi", "single", "uneven", "mibf"])
+                    choices=["multi", "single", "uneven", "mibf", "serve"])
+    ap.add_argument("--qps", default="10,50,200,400",
+                    help="serve: request arrival rates to sweep")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="serve: arrival window per rate")
+    ap.add_argument("--max-requests", type=int, default=400)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cpu-sample", type=int, default=100_000,
                     help="records (pairs for paired workloads) in the CPU baseline sample")
@@ -257,6 +360,8 @@ def main():
     B.load()
     if args.workload == "mibf":
         return bench_mibf(args, torch, D, B)
+    if args.workload == "serve":
+        return bench_serve(args, torch, D, B)
 
     w, buf, off = make_workload(args.workload, args.scale)
     paired = bool(w["paired"])

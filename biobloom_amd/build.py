@@ -31,7 +31,7 @@ def _run(cmd):
 
 def build_lib():
     srcs = [os.path.join(CSRC, "bbt_hip.hip"), os.path.join(CSRC, "bbt_mibf.hip"),
-            os.path.join(CSRC, "bbt_host.cpp")]
+            os.path.join(CSRC, "bbt_serve.cpp"), os.path.join(CSRC, "bbt_host.cpp")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *DEVICE_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)

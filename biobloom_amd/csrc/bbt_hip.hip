@@ -610,6 +610,42 @@ __global__ void k_popcount(const uint64_t *w, uint64_t nwords, unsigned long lon
 /* ======================================================================= */
 /* filter set                                                               */
 
+/* Device work queues and replay state of one stream of bbt_eval_batch calls.
+ * A filter set owns one (bbt_eval_batch); callers that evaluate batches on
+ * several streams at once give each stream its own (bbt_eval_batch_ws). */
+struct bbt_workspace {
+    /* batched path: [0..1] lane queues, [2..3] wave queues, [4..5] list
+     * sizes, [8..40] bucket counts / cursors, [48..80] bucket bases */
+    unsigned long long *dctl = nullptr;
+    uint32_t *dlists = nullptr; /* 2 * lists_cap read indices */
+    uint64_t lists_cap = 0;
+    /* replayed batch (BBT_FLAG_GRAPH) */
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    /* captures run on this stream: the caller's may be the null stream,
+     * which cannot be captured; the graph is launched on the caller's */
+    hipStream_t cap_stream = nullptr;
+    /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_walk_ms = -1;
+};
+
+static void workspace_release(bbt_workspace *w) {
+    if (w->dctl)
+        (void)hipFree(w->dctl);
+    if (w->dlists)
+        (void)hipFree(w->dlists);
+    if (w->graph_exec)
+        (void)hipGraphExecDestroy(w->graph_exec);
+    if (w->cap_stream)
+        (void)hipStreamDestroy(w->cap_stream);
+    if (w->ev0)
+        (void)hipEventDestroy(w->ev0);
+    if (w->ev1)
+        (void)hipEventDestroy(w->ev1);
+    *w = bbt_workspace();
+}
+
 struct bbt_filterset {
     int device = 0;
     std::vector<FilterDev> desc;          /* host copy, device pointers inside */
@@ -625,22 +661,9 @@ struct bbt_filterset {
     /* work queue counter and per-call scratch */
     unsigned long long *dscratch = nullptr; /* [1..2] insert counts, [3] popcount, [4..5] walk queues */
     int cu_count = 0;
-    /* batched path: [0..1] lane queues, [2..3] wave queues, [4..5] list
-     * sizes, [8..40] bucket counts / cursors, [48..80] bucket bases */
-    unsigned long long *dctl = nullptr;
-    uint32_t *dlists = nullptr; /* 2 * lists_cap read indices */
-    uint64_t lists_cap = 0;
     /* scheduling knobs (bbt_set_tuning / bbt_autotune) */
     bbt_tuning tune = {0, 0, 0, 0};
-    /* replayed batch (BBT_FLAG_GRAPH) */
-    hipGraphExec_t graph_exec = nullptr;
-    std::string graph_key;
-    /* captures run on this stream: the caller's may be the null stream,
-     * which cannot be captured; the graph is launched on the caller's */
-    hipStream_t cap_stream = nullptr;
-    /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_walk_ms = -1;
+    bbt_workspace ws; /* of bbt_eval_batch */
     std::mutex mu;
 };
 
@@ -661,6 +684,8 @@ static void fill_desc(FilterDev &d, void *bits, uint64_t m, uint32_t h, uint32_t
 static bbt_status sync_desc(bbt_filterset *fs) {
     if (!fs->dirty)
         return BBT_OK;
+    /* kernels of earlier batches on other streams may still read them */
+    HIP_OK(hipDeviceSynchronize());
     const size_t n = fs->desc.size();
     if (!fs->ddesc)
         HIP_OK(hipMalloc(&fs->ddesc, sizeof(FilterDev) * std::max<size_t>(n, 1)));
@@ -797,18 +822,7 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->ddesc);
     if (fs->dscratch)
         (void)hipFree(fs->dscratch);
-    if (fs->dctl)
-        (void)hipFree(fs->dctl);
-    if (fs->dlists)
-        (void)hipFree(fs->dlists);
-    if (fs->graph_exec)
-        (void)hipGraphExecDestroy(fs->graph_exec);
-    if (fs->cap_stream)
-        (void)hipStreamDestroy(fs->cap_stream);
-    if (fs->ev0)
-        (void)hipEventDestroy(fs->ev0);
-    if (fs->ev1)
-        (void)hipEventDestroy(fs->ev1);
+    workspace_release(&fs->ws);
     delete fs;
     return BBT_OK;
 }
@@ -1098,6 +1112,7 @@ bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) 
     if (!fs->dbinom.empty() && fs->binom_min_fpr == min_fpr &&
         fs->binom_frames >= max_frames)
         return BBT_OK;
+    HIP_OK(hipDeviceSynchronize()); /* in-flight batches read the old tables */
     for (uint32_t *p : fs->dbinom)
         if (p)
             (void)hipFree(p);
@@ -1129,8 +1144,9 @@ struct Prepared {
     uint32_t wave_min;
 };
 
-static bbt_status prepare(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
-                          const bbt_eval_out *out, hipStream_t st, Prepared &P) {
+static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
+                          const bbt_eval_params *p, const bbt_eval_out *out, hipStream_t st,
+                          Prepared &P) {
     const uint32_t nf = uint32_t(fs->desc.size());
     if (!in->seq || !in->off)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: reads not given");
@@ -1167,15 +1183,18 @@ static bbt_status prepare(bbt_filterset *fs, const bbt_reads *in, const bbt_eval
     bbt_status s = sync_desc(fs);
     if (s != BBT_OK)
         return s;
-    if (!fs->dctl)
-        HIP_OK(hipMalloc(&fs->dctl, 128 * sizeof(unsigned long long)));
-    if (p->path == BBT_PATH_BATCHED && fs->lists_cap < in->nreads) {
-        if (fs->dlists)
-            HIP_OK(hipFree(fs->dlists));
-        fs->dlists = nullptr;
-        fs->lists_cap = 0;
-        HIP_OK(hipMalloc(&fs->dlists, 2 * in->nreads * sizeof(uint32_t)));
-        fs->lists_cap = in->nreads;
+    if (!W.dctl)
+        HIP_OK(hipMalloc(&W.dctl, 128 * sizeof(unsigned long long)));
+    if (p->path == BBT_PATH_BATCHED && W.lists_cap < in->nreads) {
+        /* the lists may still be read by this workspace's previous batch */
+        if (W.dlists) {
+            HIP_OK(hipStreamSynchronize(st));
+            HIP_OK(hipFree(W.dlists));
+        }
+        W.dlists = nullptr;
+        W.lists_cap = 0;
+        HIP_OK(hipMalloc(&W.dlists, 2 * in->nreads * sizeof(uint32_t)));
+        W.lists_cap = in->nreads;
     }
     std::memset(&P, 0, sizeof(P));
     Launch &L = P.L;
@@ -1208,8 +1227,9 @@ static bbt_status prepare(bbt_filterset *fs, const bbt_reads *in, const bbt_eval
 
 /* The kernels of one batch, in stream order; capturable (no memset nodes:
  * counters are cleared by a fill kernel). */
-static bbt_status enqueue(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
-                          const bbt_eval_out *out, Prepared &P, bool timed) {
+static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
+                          const bbt_eval_params *p, const bbt_eval_out *out, Prepared &P,
+                          bool timed) {
     Launch &L = P.L;
     EvalArgs &a = L.a;
     hipStream_t st = L.st;
@@ -1217,26 +1237,26 @@ static bbt_status enqueue(bbt_filterset *fs, const bbt_reads *in, const bbt_eval
     bbt_status s = BBT_OK;
     /* work queues of the (up to two) walks of this call, and for the batched
      * path the length buckets: long reads walked by whole waves */
-    k_zero_u64<<<1, 64, 0, st>>>(fs->dctl, 48);
-    a.queue = fs->dctl + 0;
-    a.lqueue = fs->dctl + 2;
+    k_zero_u64<<<1, 64, 0, st>>>(W.dctl, 48);
+    a.queue = W.dctl + 0;
+    a.lqueue = W.dctl + 2;
     if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED) {
         const uint32_t gb =
             uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));
-        k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min, fs->dctl + 8);
-        k_len_scan<<<1, 64, 0, st>>>(fs->dctl + 8, fs->dctl + 48, fs->dctl + 4);
+        k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min, W.dctl + 8);
+        k_len_scan<<<1, 64, 0, st>>>(W.dctl + 8, W.dctl + 48, W.dctl + 4);
         k_len_scatter<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min,
-                                              fs->dctl + 48, fs->dctl + 8, fs->dlists,
-                                              fs->dlists + fs->lists_cap);
+                                              W.dctl + 48, W.dctl + 8, W.dlists,
+                                              W.dlists + W.lists_cap);
         HIP_OK(hipGetLastError());
-        a.long_list = fs->dlists;
-        a.short_list = fs->dlists + fs->lists_cap;
-        a.list_counts = fs->dctl + 4;
+        a.long_list = W.dlists;
+        a.short_list = W.dlists + W.lists_cap;
+        a.list_counts = W.dctl + 4;
     }
     if (p->mode == BBT_ORDERED)
         k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
     if (timed)
-        HIP_OK(hipEventRecord(fs->ev0, st));
+        HIP_OK(hipEventRecord(W.ev0, st));
     switch (p->mode) {
     case BBT_ORDERED:
         s = launch_ordered(L, p->scorer, P.nrec, p->paired, p->inclusive, P.kk);
@@ -1246,8 +1266,8 @@ static bbt_status enqueue(bbt_filterset *fs, const bbt_reads *in, const bbt_eval
         break;
     case BBT_SCORES:
         s = launch_scorer(L, p->scorer, false, P.kk);
-        a.queue = fs->dctl + 1;
-        a.lqueue = fs->dctl + 3;
+        a.queue = W.dctl + 1;
+        a.lqueue = W.dctl + 3;
         if (s == BBT_OK)
             s = launch_scorer(L, p->scorer, true, P.kk);
         break;
@@ -1258,7 +1278,7 @@ static bbt_status enqueue(bbt_filterset *fs, const bbt_reads *in, const bbt_eval
     if (s != BBT_OK)
         return s;
     if (timed)
-        HIP_OK(hipEventRecord(fs->ev1, st));
+        HIP_OK(hipEventRecord(W.ev1, st));
     if (out->cls || out->summary || p->mode == BBT_BESTHIT) {
         const uint64_t blocks = (P.nrec + kBlock - 1) / kBlock;
         const size_t shm = out->summary ? (2 * size_t(nf) + 2) * sizeof(unsigned int) : 0;
@@ -1286,46 +1306,40 @@ static std::string graph_key(const bbt_reads *in, const bbt_eval_params *p,
     return k;
 }
 
-extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
-                                     const bbt_eval_params *p, bbt_eval_out *out,
-                                     void *stream) {
-    if (!fs || !in || !p || !out)
-        return fail(BBT_ERR_ARG, "bbt_eval_batch: null argument");
-    if (in->nreads == 0)
-        return BBT_OK;
-    hipStream_t st = static_cast<hipStream_t>(stream);
+static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
+                            const bbt_eval_params *p, bbt_eval_out *out, hipStream_t st) {
     std::lock_guard<std::mutex> g(fs->mu);
     Prepared P;
-    bbt_status s = prepare(fs, in, p, out, st, P);
+    bbt_status s = prepare(fs, W, in, p, out, st, P);
     if (s != BBT_OK)
         return s;
     const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
     if (timed) {
-        if (!fs->ev0)
-            HIP_OK(hipEventCreate(&fs->ev0));
-        if (!fs->ev1)
-            HIP_OK(hipEventCreate(&fs->ev1));
+        if (!W.ev0)
+            HIP_OK(hipEventCreate(&W.ev0));
+        if (!W.ev1)
+            HIP_OK(hipEventCreate(&W.ev1));
     }
     const bool graph = (p->flags & BBT_FLAG_GRAPH) != 0 && !timed;
     if (!graph) {
-        s = enqueue(fs, in, p, out, P, timed);
+        s = enqueue(fs, W, in, p, out, P, timed);
         if (s != BBT_OK)
             return s;
     } else {
         const std::string key = graph_key(in, p, out, P, st);
-        if (!fs->graph_exec || key != fs->graph_key) {
-            if (fs->graph_exec) {
-                (void)hipGraphExecDestroy(fs->graph_exec);
-                fs->graph_exec = nullptr;
+        if (!W.graph_exec || key != W.graph_key) {
+            if (W.graph_exec) {
+                (void)hipGraphExecDestroy(W.graph_exec);
+                W.graph_exec = nullptr;
             }
-            if (!fs->cap_stream)
-                HIP_OK(hipStreamCreateWithFlags(&fs->cap_stream, hipStreamNonBlocking));
-            P.L.st = fs->cap_stream;
-            HIP_OK(hipStreamBeginCapture(fs->cap_stream, hipStreamCaptureModeRelaxed));
-            s = enqueue(fs, in, p, out, P, false);
+            if (!W.cap_stream)
+                HIP_OK(hipStreamCreateWithFlags(&W.cap_stream, hipStreamNonBlocking));
+            P.L.st = W.cap_stream;
+            HIP_OK(hipStreamBeginCapture(W.cap_stream, hipStreamCaptureModeRelaxed));
+            s = enqueue(fs, W, in, p, out, P, false);
             P.L.st = st;
             hipGraph_t gr = nullptr;
-            hipError_t e = hipStreamEndCapture(fs->cap_stream, &gr);
+            hipError_t e = hipStreamEndCapture(W.cap_stream, &gr);
             if (s != BBT_OK) {
                 if (gr)
                     (void)hipGraphDestroy(gr);
@@ -1333,24 +1347,59 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
             }
             if (e != hipSuccess)
                 return fail(BBT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&fs->graph_exec, gr, nullptr, nullptr, 0);
+            e = hipGraphInstantiate(&W.graph_exec, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             if (e != hipSuccess) {
-                fs->graph_exec = nullptr;
+                W.graph_exec = nullptr;
                 return fail(BBT_ERR_HIP, std::string("graph instantiate: ") +
                                              hipGetErrorString(e));
             }
-            fs->graph_key = key;
+            W.graph_key = key;
         }
-        HIP_OK(hipGraphLaunch(fs->graph_exec, st));
+        HIP_OK(hipGraphLaunch(W.graph_exec, st));
     }
     if (timed) {
-        HIP_OK(hipEventSynchronize(fs->ev1));
+        HIP_OK(hipEventSynchronize(W.ev1));
         float ms = 0;
-        HIP_OK(hipEventElapsedTime(&ms, fs->ev0, fs->ev1));
-        fs->last_walk_ms = ms;
+        HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+        W.last_walk_ms = ms;
     }
     return BBT_OK;
+}
+
+extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
+                                     const bbt_eval_params *p, bbt_eval_out *out,
+                                     void *stream) {
+    if (!fs || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: null argument");
+    if (in->nreads == 0)
+        return BBT_OK;
+    return eval_impl(fs, fs->ws, in, p, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" bbt_status bbt_workspace_create(bbt_workspace **out) {
+    if (!out)
+        return fail(BBT_ERR_ARG, "bbt_workspace_create: null argument");
+    *out = new bbt_workspace();
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_workspace_free(bbt_workspace *ws) {
+    if (!ws)
+        return BBT_OK;
+    workspace_release(ws);
+    delete ws;
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_eval_batch_ws(bbt_filterset *fs, bbt_workspace *ws,
+                                        const bbt_reads *in, const bbt_eval_params *p,
+                                        bbt_eval_out *out, void *stream) {
+    if (!fs || !ws || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch_ws: null argument");
+    if (in->nreads == 0)
+        return BBT_OK;
+    return eval_impl(fs, *ws, in, p, out, static_cast<hipStream_t>(stream));
 }
 
 extern "C" bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void *stream,
@@ -1423,7 +1472,7 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
         bbt_status s = bbt_set_tuning(fs, &t);
         if (s == BBT_OK)
             s = bbt_eval_batch(fs, in, &q, out, stream);
-        ms = fs->last_walk_ms;
+        ms = fs->ws.last_walk_ms;
         return s;
     };
     double best_ms = 0;
@@ -1460,10 +1509,10 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     s = bbt_set_tuning(fs, &cur);
     if (best)
         *best = cur;
-    fs->last_walk_ms = best_ms;
+    fs->ws.last_walk_ms = best_ms;
     return s;
 }
 
 extern "C" double bbt_last_walk_ms(const bbt_filterset *fs) {
-    return fs ? fs->last_walk_ms : -1.0;
+    return fs ? fs->ws.last_walk_ms : -1.0;
 }

@@ -68,6 +68,26 @@ class MibfOut(C.Structure):
                 ("counters", C.c_void_p)]
 
 
+SERVE_BATCHED, SERVE_PLAIN = 0, 1
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("mode", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("max_wave_reads", C.c_uint64), ("max_wave_bases", C.c_uint64)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("nreads", C.c_uint64), ("wave", C.c_uint64), ("submit_us", C.c_double),
+                ("admit_us", C.c_double), ("done_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("waves", C.c_uint64), ("requests_done", C.c_uint64),
+                ("reads_done", C.c_uint64), ("max_waves_in_flight", C.c_uint64),
+                ("arena_bytes", C.c_uint64), ("arena_high_water", C.c_uint64),
+                ("mean_wave_reads", C.c_double), ("mean_wave_requests", C.c_double)]
+
+
 # exported symbols and their signatures: (restype, argtypes)
 _SIGS = {
     "bbt_last_error": (C.c_char_p, []),
@@ -96,6 +116,21 @@ _SIGS = {
     "bbt_last_walk_ms": (C.c_double, [C.c_void_p]),
     "bbt_measure_gather": (C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bbt_workspace_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
+    "bbt_workspace_free": (C.c_int32, [C.c_void_p]),
+    "bbt_eval_batch_ws": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Reads),
+                                      C.POINTER(EvalParams), C.POINTER(EvalOut), C.c_void_p]),
+    "bbt_server_create": (C.c_int32, [C.c_void_p, C.POINTER(EvalParams),
+                                      C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "bbt_server_submit": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]),
+    "bbt_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double]),
+    "bbt_server_drain": (C.c_int32, [C.c_void_p, C.c_double]),
+    "bbt_server_result": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                      C.POINTER(RequestStats)]),
+    "bbt_server_release": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "bbt_server_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    "bbt_server_destroy": (C.c_int32, [C.c_void_p]),
     "bbt_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     "bbt_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     "bbt_autotune": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
@@ -360,3 +395,85 @@ def mibf_params(score=0.1, multi_thresh=1.0, allowed_miss=0, min_nonsat=1, max_h
                 paired=False, best_agree=False):
     flags = (MIBF_PAIRED if paired else 0) | (MIBF_BEST_AGREE if best_agree else 0)
     return MibfParams(score, multi_thresh, allowed_miss, min_nonsat, max_hits, flags)
+
+
+class Workspace:
+    """Work queues for batches on one stream (bbt_workspace)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(load().bbt_workspace_create(C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().bbt_workspace_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def eval_batch(self, fs, reads, params, out, stream=0):
+        check(load().bbt_eval_batch_ws(fs.handle, self._h, C.byref(reads), C.byref(params),
+                                       C.byref(out), C.c_void_p(stream)))
+
+
+class Server:
+    """Request-serving loop over a resident filter set (bbt_server)."""
+
+    def __init__(self, fs, params, slots=4, mode=SERVE_BATCHED, device_bytes=1 << 30,
+                 max_wave_reads=1 << 18, max_wave_bases=64 << 20):
+        self.fs = fs  # keeps the filter set alive
+        self.nf = len(fs)
+        self.paired = bool(params.paired)
+        cfg = ServerConfig(slots, mode, device_bytes, max_wave_reads, max_wave_bases)
+        h = C.c_void_p()
+        check(load().bbt_server_create(fs.handle, C.byref(params), C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().bbt_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, buf, off):
+        """buf: uint8 numpy array, off: uint64 numpy offsets (nreads + 1)."""
+        rid = C.c_uint64()
+        check(load().bbt_server_submit(self._h, buf.ctypes.data, off.ctypes.data,
+                                       len(off) - 1, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        check(load().bbt_server_wait(self._h, rid, timeout))
+
+    def drain(self, timeout=-1.0):
+        check(load().bbt_server_drain(self._h, timeout))
+
+    def result(self, rid, nreads):
+        import numpy as np
+        nrec = nreads // 2 if self.paired else nreads
+        hits = np.zeros(max(1, nreads * self.nf), np.uint8)
+        cls = np.zeros(max(1, nrec), np.uint32)
+        st = RequestStats()
+        check(load().bbt_server_result(self._h, rid, hits.ctypes.data, cls.ctypes.data,
+                                       C.byref(st)))
+        return (hits[: nreads * self.nf].reshape(nreads, self.nf), cls[:nrec],
+                {"wave": st.wave, "submit_us": st.submit_us, "admit_us": st.admit_us,
+                 "done_us": st.done_us})
+
+    def release(self, rid):
+        check(load().bbt_server_release(self._h, rid))
+
+    def stats(self):
+        st = ServerStats()
+        check(load().bbt_server_get_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}

@@ -12,6 +12,11 @@ Workloads (reference flags in parentheses):
   uneven   4 x 100 Mbp genomes, k=25 FPR 0.05; 200k log-normal long reads
            (median 2 kbp, 100 bp-50 kbp, 5% subst 1.5% ins 1.5% del)
            interleaved with 2M x 150 bp reads
+  serve    one 3 Gbp genome (24 records, half of it copies of 1000 random
+           300 bp repeat consensuses with 10% divergence) and 8 x 20 Mbp
+           contaminant genomes, k=25 FPR 0.075; a pool of requests of
+           1k-20k reads each, 80% 150 bp / 20% log-normal long reads, from the
+           big genome (50%), the contaminants (10%) or random (40%)
   mibf     200 x 5 Mbp references ref000..ref199, each with 5% of its sequence
            overwritten by 10 kbp segments copied from other references'
            original sequence; 3 spaced seeds of span 25 and weight 20 closed
@@ -240,3 +245,65 @@ def workload_mibf(scale=1.0, nrefs=200, ref_len=5_000_000, seg=10_000):
     names = [f"ref{g:03d}" for g in range(nrefs)]
     return {"refs": refs, "names": names, "seeds": seeds, "reads": reads, "src": src,
             "gsel": gsel, "paired": False, "genomes": refs}
+
+
+def repeat_genome(rng, n, nrep=1000, rep_len=300, frac=0.5, div=0.10, records=24,
+                  chunk=1 << 20):
+    """n bases: uniform random, then a `frac` share of the rep_len slots
+    overwritten by copies of nrep random consensuses with i.i.d. `div`
+    substitutions per copy; records - 1 'N' bytes split it into records."""
+    g = codes_to_ascii(rng.integers(0, 4, size=n, dtype=np.uint8))
+    cons = codes_to_ascii(rng.integers(0, 4, size=(nrep, rep_len), dtype=np.uint8))
+    nslots = n // rep_len
+    view = g[: nslots * rep_len].reshape(nslots, rep_len)
+    chosen = np.flatnonzero(rng.random(nslots) < frac)
+    for a in range(0, chosen.size, chunk):
+        idx = chosen[a:a + chunk]
+        cp = cons[rng.integers(0, nrep, size=idx.size)]
+        mask = rng.random(cp.shape) < div
+        cur = np.searchsorted(ASCII, cp[mask])
+        cp[mask] = ASCII[(cur + rng.integers(1, 4, size=cur.size)) % 4]
+        view[idx] = cp
+    for r in range(1, records):
+        g[r * (n // records)] = ord("N")
+    return g
+
+
+def workload_serve(scale=1.0, npool=48, min_reads=1000, max_reads=20_000):
+    rng = rng_for(5)
+    big = repeat_genome(rng_for(5, 1), max(2_000_000, int(3_000_000_000 * min(1.0, scale))))
+    clen = max(200_000, int(20_000_000 * min(1.0, scale)))
+    contams = [codes_to_ascii(rng_for(5, 100 + c).integers(0, 4, size=clen, dtype=np.uint8))
+               for c in range(8)]
+    pool = []
+    for q in range(npool):
+        n = int(rng.integers(min_reads, max_reads + 1))
+        nlong = int(rng.binomial(n, 0.2))
+        nshort = n - nlong
+        origin = rng.choice(3, size=n, p=[0.5, 0.1, 0.4])
+        items = []
+        # short reads: big genome, contaminants, random
+        s_or = origin[:nshort]
+        for src, gens in ((0, [big]), (1, contams)):
+            k = int(np.count_nonzero(s_or == src))
+            if k:
+                r, _, _ = fixed_len_reads(rng, gens, k, 150, 0.01, 1.0)
+                items.extend(list(r))
+        k = int(np.count_nonzero(s_or == 2))
+        if k:
+            items.extend(list(codes_to_ascii(rng.integers(0, 4, size=(k, 150), dtype=np.uint8))))
+        l_or = origin[nshort:]
+        for src, gens in ((0, [big]), (1, contams)):
+            k = int(np.count_nonzero(l_or == src))
+            if k:
+                r, _ = long_reads(rng, gens, k)
+                items.extend(r)
+        k = int(np.count_nonzero(l_or == 2))
+        if k:
+            lens = np.clip(np.round(rng.lognormal(np.log(2000), 0.75, size=k)), 100, 50_000)
+            items.extend(codes_to_ascii(rng.integers(0, 4, size=int(L), dtype=np.uint8))
+                         for L in lens)
+        perm = rng.permutation(len(items))
+        pool.append(pack([items[i] for i in perm]))
+    return {"genomes": [big] + contams, "k": 25, "fpr": 0.075, "requests": pool,
+            "paired": False}

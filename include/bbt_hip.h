@@ -182,6 +182,16 @@ uint32_t bbt_calc_optimal_hashnum(double fpr);
 bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
                           const bbt_eval_params *p, bbt_eval_out *out,
                           void *stream);
+/* Work queues and replay state for one stream of batches.  bbt_eval_batch
+ * uses the filter set's own; callers running batches on several streams at
+ * once give each stream a workspace of its own (no counterpart in the
+ * reference, whose filters are shared read-only by OpenMP threads,
+ * BioBloomClassifier.cpp:59). */
+typedef struct bbt_workspace bbt_workspace;
+bbt_status bbt_workspace_create(bbt_workspace **out);
+bbt_status bbt_workspace_free(bbt_workspace *ws);
+bbt_status bbt_eval_batch_ws(bbt_filterset *fs, bbt_workspace *ws, const bbt_reads *in,
+                             const bbt_eval_params *p, bbt_eval_out *out, void *stream);
 /* Scheduling knobs of a filter set (no counterpart in the reference). */
 bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t);
 bbt_status bbt_get_tuning(const bbt_filterset *fs, bbt_tuning *t);
@@ -207,6 +217,58 @@ bbt_status bbt_summary_text(const bbt_filterset *fs, const uint64_t *summary,
 /* Binomial helpers used by the BINOMIAL scorer (SeqEval.h:199-216). */
 uint64_t bbt_calc_min_count(uint64_t frame_len, double bf_fpr, double min_fpr);
 double bbt_binomial_score(uint64_t count, uint64_t frame_len, double bf_fpr);
+
+/* ---- serving ------------------------------------------------------------ */
+/* A request loop over a resident filter set (no counterpart in the reference,
+ * which classifies one input stream per process: BioBloomClassifier::filter,
+ * BioBloomClassifier.cpp:33-90).  Requests (host reads) are queued FIFO and
+ * admitted into launches while a slot is free and their reads and outputs fit
+ * in a device arena of device_bytes; requests arriving while launches run
+ * join the next launch (continuous batching), up to `slots` launches in
+ * flight on their own streams.  BBT_SERVE_PLAIN: one request per launch with
+ * the plain kernel, in order (the per-request baseline).  Results are the
+ * per-read hits and per-record classes of bbt_eval_batch. */
+typedef struct bbt_server bbt_server;
+enum { BBT_SERVE_BATCHED = 0, BBT_SERVE_PLAIN = 1 };
+
+typedef struct {
+    uint32_t slots;          /* launches in flight (1..8; 1 in plain mode) */
+    uint32_t mode;           /* BBT_SERVE_* */
+    uint64_t device_bytes;   /* arena for the reads and outputs in flight */
+    uint64_t max_wave_reads; /* cap on reads merged into one launch */
+    uint64_t max_wave_bases; /* cap on bases per launch (sizes pinned staging) */
+} bbt_server_config;
+
+typedef struct {
+    uint64_t nreads;
+    uint64_t wave;           /* launch that carried the request (1-based) */
+    double submit_us, admit_us, done_us; /* since server creation */
+} bbt_request_stats;
+
+typedef struct {
+    uint64_t waves, requests_done, reads_done, max_waves_in_flight;
+    uint64_t arena_bytes, arena_high_water;
+    double mean_wave_reads, mean_wave_requests;
+} bbt_server_stats;
+
+/* p: the classification parameters of every request (BBT_STD or BBT_ORDERED;
+ * path and flags are set by the server).  fs must outlive the server. */
+bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
+                             const bbt_server_config *cfg, bbt_server **out);
+/* Copies the request (host seq, nreads + 1 offsets into it) and queues it. */
+bbt_status bbt_server_submit(bbt_server *s, const char *seq, const uint64_t *off,
+                             uint64_t nreads, uint64_t *req_id);
+/* Blocks until the request is done (timeout_s < 0: no limit). */
+bbt_status bbt_server_wait(bbt_server *s, uint64_t req_id, double timeout_s);
+/* Blocks until every queued request is done. */
+bbt_status bbt_server_drain(bbt_server *s, double timeout_s);
+/* hits: nreads * nf bytes, cls: per record; either may be NULL. */
+bbt_status bbt_server_result(bbt_server *s, uint64_t req_id, uint8_t *hits, uint32_t *cls,
+                             bbt_request_stats *stats);
+bbt_status bbt_server_release(bbt_server *s, uint64_t req_id);
+bbt_status bbt_server_get_stats(bbt_server *s, bbt_server_stats *out);
+/* Stops admitting, finishes the launches in flight, frees everything. */
+bbt_status bbt_server_destroy(bbt_server *s);
 
 /* ---- multi-index Bloom filter ------------------------------------------- */
 /* One bit vector of m bits shared by all references; each set bit owns one

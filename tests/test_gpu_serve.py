@@ -1,0 +1,161 @@
+"""Serving loop (bbt_server): every request's results are the same whether it
+is classified alone or merged with others into shared launches, under a
+device-memory budget that forces requests to wait, in plain and batched
+modes, single- and paired-end; and the server's argument checks."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from biobloom_amd import device as D
+    from biobloom_amd import lib as B
+    from biobloom_amd import synth
+    B.load()
+    rng = np.random.default_rng(7)
+    genomes = [synth.codes_to_ascii(rng.integers(0, 4, size=400_000, dtype=np.uint8))
+               for _ in range(3)]
+    fs, _ = D.build_filters(genomes, 25, 0.075, ids=["g0", "g1", "g2"])
+    return torch, D, B, synth, fs, genomes
+
+
+def make_requests(synth, genomes, n, seed, paired=False):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        nr = int(rng.integers(20, 600)) * (2 if paired else 1)
+        items = []
+        for j in range(nr):
+            if rng.random() < 0.2 and not paired:
+                L = int(rng.integers(200, 3000))
+            else:
+                L = 150
+            if rng.random() < 0.6:
+                g = genomes[int(rng.integers(0, len(genomes)))]
+                a = int(rng.integers(0, len(g) - L))
+                items.append(g[a:a + L].copy())
+            else:
+                items.append(synth.codes_to_ascii(rng.integers(0, 4, size=L, dtype=np.uint8)))
+        buf, off = synth.pack(items)
+        reqs.append((buf, off))
+    return reqs
+
+
+def alone(D, B, fs, buf, off, params):
+    dr = D.DeviceReads(buf, off)
+    res = D.classify(fs, dr, params)
+    return res["hits"], res["cls"]
+
+
+def run_server(srv, reqs):
+    ids = [srv.submit(buf, off) for buf, off in reqs]
+    srv.drain(120.0)
+    out = []
+    for rid, (buf, off) in zip(ids, reqs):
+        h, c, st = srv.result(rid, len(off) - 1)
+        srv.release(rid)
+        out.append((h, c, st))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_alone_equals_mixed(env, mode):
+    torch, D, B, synth, fs, genomes = env
+    params = B.params()
+    reqs = make_requests(synth, genomes, 40, 1)
+    m = B.SERVE_PLAIN if mode == "plain" else B.SERVE_BATCHED
+    print(f"[gpu] server {mode}: 40 requests", flush=True)
+    srv = B.Server(fs, params, slots=3, mode=m, device_bytes=64 << 20,
+                   max_wave_reads=1 << 14, max_wave_bases=8 << 20)
+    got = run_server(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["requests_done"] == len(reqs)
+    for (buf, off), (h, c, info) in zip(reqs, got):
+        wh, wc = alone(D, B, fs, buf, off, params)
+        assert np.array_equal(h, wh) and np.array_equal(c, wc)
+        assert info["done_us"] >= info["admit_us"] >= info["submit_us"]
+    if mode == "plain":
+        assert st["mean_wave_requests"] == 1.0
+    else:
+        # requests submitted back to back are merged into shared launches
+        assert st["mean_wave_requests"] > 1.0
+        assert st["max_waves_in_flight"] >= 1
+
+
+def test_results_match_oracle(env):
+    torch, D, B, synth, fs, genomes = env
+    params = B.params()
+    reqs = make_requests(synth, genomes, 6, 2)
+    ofs = []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        f = O.Filter(inf.m, inf.h, inf.k)
+        fs.download(i, f.bits)
+        ofs.append(f)
+    srv = B.Server(fs, params, slots=2, device_bytes=32 << 20, max_wave_reads=1 << 13,
+                   max_wave_bases=4 << 20)
+    got = run_server(srv, reqs)
+    srv.close()
+    for (buf, off), (h, c, _) in zip(reqs, got):
+        oh, _, ocls, _ = O.classify(ofs, buf, off)
+        assert np.array_equal(h, oh) and np.array_equal(c, ocls)
+
+
+def test_device_memory_admission(env):
+    """An arena that holds about one launch: requests wait for memory held by
+    launches in flight, and all finish with the same results."""
+    torch, D, B, synth, fs, genomes = env
+    params = B.params()
+    reqs = make_requests(synth, genomes, 24, 3)
+    wave_reads, wave_bases = 1 << 12, 2 << 20
+    need = wave_bases + 8 * wave_reads + wave_reads * len(fs) + 4 * wave_reads + 4096
+    srv = B.Server(fs, params, slots=4, device_bytes=need, max_wave_reads=wave_reads,
+                   max_wave_bases=wave_bases)
+    got = run_server(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["requests_done"] == len(reqs)
+    assert st["arena_high_water"] <= st["arena_bytes"]
+    for (buf, off), (h, c, _) in zip(reqs, got):
+        wh, wc = alone(D, B, fs, buf, off, params)
+        assert np.array_equal(h, wh) and np.array_equal(c, wc)
+
+
+def test_paired_requests(env):
+    torch, D, B, synth, fs, genomes = env
+    params = B.params(paired=True)
+    reqs = make_requests(synth, genomes, 12, 4, paired=True)
+    srv = B.Server(fs, params, slots=2, device_bytes=64 << 20, max_wave_reads=1 << 14,
+                   max_wave_bases=8 << 20)
+    got = run_server(srv, reqs)
+    srv.close()
+    for (buf, off), (h, c, _) in zip(reqs, got):
+        wh, wc = alone(D, B, fs, buf, off, params)
+        assert np.array_equal(h, wh) and np.array_equal(c, wc)
+        assert len(c) == (len(off) - 1) // 2
+
+
+def test_server_argument_checks(env):
+    torch, D, B, synth, fs, genomes = env
+    with pytest.raises(B.BbtError, match="device_bytes"):
+        B.Server(fs, B.params(), device_bytes=4096, max_wave_reads=1 << 14,
+                 max_wave_bases=8 << 20)
+    with pytest.raises(B.BbtError, match="configuration"):
+        B.Server(fs, B.params(), slots=0)
+    srv = B.Server(fs, B.params(paired=True), slots=1, device_bytes=16 << 20,
+                   max_wave_reads=1024, max_wave_bases=1 << 20)
+    buf, off = synth.pack([np.frombuffer(b"ACGT" * 40, np.uint8)] * 3)
+    with pytest.raises(B.BbtError, match="odd"):
+        srv.submit(buf, off)
+    big, boff = synth.pack([np.frombuffer(b"ACGT" * 40, np.uint8)] * 2000)
+    with pytest.raises(B.BbtError, match="larger than one wave"):
+        srv.submit(big, boff)
+    srv.close()
