@@ -12,6 +12,8 @@ HOST = os.path.join(PKG, "host")
 LIB = os.path.join(PKG, "libbbt_hip.so")
 CLI = os.path.join(PKG, "biobloomcategorizer")
 MAKER = os.path.join(PKG, "biobloommaker")
+MIMAKER = os.path.join(PKG, "biobloommimaker")
+MICAT = os.path.join(PKG, "biobloommicategorizer")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("BBT_OFFLOAD_ARCH", "gfx950")
@@ -38,8 +40,11 @@ def build_lib():
 
 
 def build_cli():
-    """The restated biobloomcategorizer / biobloommaker front ends."""
-    for name, out in [("biobloomcategorizer.cpp", CLI), ("biobloommaker.cpp", MAKER)]:
+    """The restated biobloomcategorizer / biobloommaker / biobloommimaker /
+    biobloommicategorizer front ends."""
+    for name, out in [("biobloomcategorizer.cpp", CLI), ("biobloommaker.cpp", MAKER),
+                      ("biobloommimaker.cpp", MIMAKER),
+                      ("biobloommicategorizer.cpp", MICAT)]:
         src = os.path.join(HOST, name)
         if not os.path.exists(src):
             continue

@@ -1007,6 +1007,22 @@ bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream) {
     return BBT_OK;
 }
 
+bbt_status bbt_mibf_summary_text(const bbt_mibf *mf, const uint64_t *summary, uint64_t nrec,
+                                 char *buf, uint64_t cap, uint64_t *needed) {
+    if (!mf || !summary)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_summary_text: bad argument");
+    /* ResultsManager<ID>(m_fullIDs) rows: "repeat" and every ID name */
+    const std::string s = summary_text(mf->names, summary, nrec);
+    if (needed)
+        *needed = s.size();
+    if (buf && cap) {
+        const size_t n = std::min<size_t>(s.size(), size_t(cap - 1));
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return BBT_OK;
+}
+
 bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, double *probs) {
     if (!mf || !probs)
         return mfail(BBT_ERR_ARG, "bbt_mibf_frame_probs: null argument");

@@ -1,0 +1,499 @@
+/*
+ * biobloommicategorizer -- the reference's multi-index Bloom filter
+ * categorizer (BioBloomCategorizer/BioBloomMICategorizer.cpp:120-344,
+ * MIBFClassifier.hpp:47-939) over libbbt_hip.  Options, the per-read output
+ * lines on stdout (TSV, or FASTA/FASTQ with the hits in the header,
+ * MIBFClassifier.hpp:652-736) and <prefix>_summary.tsv (ResultsManager over
+ * "repeat" and the IDs) follow the reference; reads are classified in device
+ * batches by bbt_mibf_query_batch.
+ *
+ * Differences kept on purpose: reads are counted for the summary in every
+ * thread setting (the reference counts them only in its multi-threaded
+ * reader); -r (streak) does not stop a query early, every frame is
+ * evaluated; --debug is not available.
+ */
+#include <getopt.h>
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "front_end.h"
+
+using namespace std;
+
+namespace {
+
+enum OutType { TSV = 0, FASTA = 1, FASTQ = 2 };
+
+struct Opt {
+    string prefix, filter;
+    bool paired = false, hitOnly = false, bestAgree = false;
+    double score = std::pow(10.0, -10.0);
+    unsigned frameMatches = 1, streak = 3, minNoSat = 0;
+    double multi = 1.0;
+    uint64_t interval = 10000000;
+    int out = TSV;
+    int verbose = 0;
+};
+
+constexpr uint32_t kMaxHits = 64;
+constexpr size_t kBatch = 1 << 17;
+
+void usage() {
+    cerr << "Usage: biobloommicategorizer [OPTION]... -f [FILTER] [FILE]...\n"
+            "Usage: biobloommicategorizer [OPTION]... -f [FILTER1] -e [FILE1.fq] [FILE2.fq]\n"
+            "The input format may be FASTA, FASTQ, and compressed with gz. Output will be to\n"
+            "stdout with a summary file outputted to a prefix.\n"
+            "  -p, --prefix=N         Output prefix to use. Otherwise will output to current\n"
+            "                         directory.\n"
+            "  -f, --filter=N         Path of miBF '.bf' file\n"
+            "  -e, --paired_mode      Uses paired-end information.\n"
+            "  -s, --min_FPR=N        Minimum -10*log(FPR) threshold for a match. [100.0]\n"
+            "  -t, --threads=N        The number of threads to use. [1]\n"
+            "  -i, --hitOnly          Return only results that hit the filter.\n"
+            "      --fa               Output categorized reads in Fasta output.\n"
+            "      --fq               Output categorized reads in Fastq output.\n"
+            "      --version          Display version information.\n"
+            "  -h, --help             Display this dialog.\n"
+            "  -v, --verbose          Display verbose output\n"
+            "  -I, --interval         the interval to report file processing status [10000000]\n"
+            "Advanced options:\n"
+            "  -a, --frameMatches=N   Min number seed matches needed in a frame to match [1]\n"
+            "                         Ignored if k-mers used when indexing.\n"
+            "  -m, --multi=N          Multi Match threshold. [1.0]\n"
+            "  -r, --streak=N         Number of additional hits needed to skip classification. [3]\n"
+            "  -c, --minNoSat=N       Minimum count of non saturated matches. Increasing this value\n"
+            "                         will filter out repetitive or low quality sequences.[0]\n"
+            "  -b, --bestHitAgree     Filters out all matches where best hit is ambiguous because\n"
+            "                         match count metrics do not agree.\n";
+    exit(EXIT_SUCCESS);
+}
+
+bool fexists(const string &f) {
+    ifstream in(f.c_str());
+    return in.good();
+}
+
+/* the device side of one batch: reads, outputs, pinned host copies */
+struct Device {
+    hipStream_t st = nullptr;
+    char *dseq = nullptr;
+    uint64_t *doff = nullptr;
+    uint32_t *dnh = nullptr, *dhits = nullptr, *dsat = nullptr;
+    uint64_t *dctr = nullptr;
+    size_t seq_cap = 0, rec_cap = 0;
+    vector<uint32_t> nh, hits, sat;
+
+    void reserve(size_t bases, size_t nreads) {
+        if (bases + 8 > seq_cap) {
+            (void)hipFree(dseq);
+            seq_cap = std::max(bases + 8, seq_cap * 2);
+            if (hipMalloc(&dseq, seq_cap) != hipSuccess)
+                die("device allocation failed");
+        }
+        if (nreads > rec_cap) {
+            (void)hipFree(doff);
+            (void)hipFree(dnh);
+            (void)hipFree(dhits);
+            (void)hipFree(dsat);
+            rec_cap = std::max(nreads, rec_cap * 2);
+            if (hipMalloc(&doff, (rec_cap + 1) * 8) != hipSuccess ||
+                hipMalloc(&dnh, rec_cap * 4) != hipSuccess ||
+                hipMalloc(&dhits, rec_cap * kMaxHits * 7 * 4) != hipSuccess ||
+                hipMalloc(&dsat, rec_cap * 4) != hipSuccess)
+                die("device allocation failed");
+        }
+        if (!dctr && hipMalloc(&dctr, 32) != hipSuccess)
+            die("device allocation failed");
+    }
+    static void die(const string &m) {
+        cerr << "Error: " << m << endl;
+        exit(1);
+    }
+};
+
+class MIBFClassifier {
+  public:
+    MIBFClassifier(const Opt &o) : m_o(o) {
+        /* MIBFClassifier.hpp:49-95 */
+        if (bbt_mibf_load(o.filter.c_str(), &m_mf, nullptr) != BBT_OK) {
+            cerr << "Error: " << bbt_last_error() << endl;
+            exit(1);
+        }
+        bbt_mibf_info(m_mf, &m_info);
+        for (uint32_t i = 0; i <= m_info.nids; ++i)
+            m_names.push_back(bbt_mibf_name(m_mf, i));
+        m_allowedMiss = (o.frameMatches > m_info.nseeds || m_info.kmer_mode)
+                            ? 0
+                            : m_info.nseeds - o.frameMatches;
+        m_above.assign(m_info.nids + 1, 0);
+        m_unique.assign(m_info.nids + 1, 0);
+        if (hipStreamCreateWithFlags(&m_dev.st, hipStreamNonBlocking) != hipSuccess)
+            Device::die("stream creation failed");
+    }
+    ~MIBFClassifier() { bbt_mibf_free(m_mf); }
+
+    void filter(const vector<string> &files) {
+        cerr << "Filtering Start" << endl;
+        for (const string &f : files) {
+            bbt_fe::SeqReader rd(f);
+            if (!rd.ok()) {
+                cerr << "file " << f << " cannot be opened" << endl;
+                exit(1);
+            }
+            vector<bbt_fe::SeqRecord> batch;
+            bbt_fe::SeqRecord rec;
+            while (rd.read(rec) >= 0) {
+                batch.push_back(rec);
+                if (++m_numRead % m_o.interval == 0)
+                    cerr << "Currently Reading Read Number: " << m_numRead << endl;
+                if (batch.size() == kBatch) {
+                    run(batch, false);
+                    batch.clear();
+                }
+            }
+            if (!batch.empty())
+                run(batch, false);
+        }
+        finish();
+    }
+
+    void filterPair(const string &f1, const string &f2) {
+        bbt_fe::SeqReader r1(f1), r2(f2);
+        if (!r1.ok() || !r2.ok()) {
+            cerr << "file " << (r1.ok() ? f2 : f1) << " cannot be opened" << endl;
+            exit(1);
+        }
+        vector<bbt_fe::SeqRecord> batch;
+        bbt_fe::SeqRecord a, b;
+        while (r1.read(a) >= 0 && r2.read(b) >= 0) {
+            batch.push_back(a);
+            batch.push_back(b);
+            if (++m_numRead % m_o.interval == 0)
+                cerr << "Currently Reading Read Number: " << m_numRead << endl;
+            if (batch.size() >= 2 * kBatch) {
+                run(batch, true);
+                batch.clear();
+            }
+        }
+        if (!batch.empty())
+            run(batch, true);
+        finish();
+    }
+
+  private:
+    void run(const vector<bbt_fe::SeqRecord> &recs, bool paired) {
+        string seq;
+        vector<uint64_t> off{0};
+        uint32_t max_len = 0;
+        for (const auto &r : recs) {
+            seq += r.seq;
+            off.push_back(seq.size());
+            max_len = std::max<uint32_t>(max_len, uint32_t(r.seq.size()));
+        }
+        const size_t nreads = recs.size(), nrec = paired ? nreads / 2 : nreads;
+        Device &D = m_dev;
+        D.reserve(seq.size(), nreads);
+        const size_t padded = (seq.size() + 7) / 8 * 8 + 8;
+        seq.resize(padded, '\0');
+        uint64_t zero[4] = {0, 0, 0, 0};
+        if (hipMemcpyAsync(D.dseq, seq.data(), padded, hipMemcpyHostToDevice, D.st) !=
+                hipSuccess ||
+            hipMemcpyAsync(D.doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, D.st) !=
+                hipSuccess ||
+            hipMemcpyAsync(D.dctr, zero, 32, hipMemcpyHostToDevice, D.st) != hipSuccess)
+            Device::die("upload failed");
+        bbt_reads in;
+        std::memset(&in, 0, sizeof(in));
+        in.seq = D.dseq;
+        in.seq_bytes = padded;
+        in.off = D.doff;
+        in.nreads = nreads;
+        in.max_len = max_len;
+        bbt_mibf_params p;
+        std::memset(&p, 0, sizeof(p));
+        p.score = m_o.score;
+        p.multi_thresh = m_o.multi;
+        p.allowed_miss = m_allowedMiss;
+        p.min_nonsat = m_o.minNoSat;
+        p.max_hits = kMaxHits;
+        p.flags = (paired ? BBT_MIBF_PAIRED : 0) | (m_o.bestAgree ? BBT_MIBF_BEST_AGREE : 0);
+        bbt_mibf_out out{D.dnh, D.dhits, D.dsat, D.dctr};
+        if (bbt_mibf_query_batch(m_mf, &in, &p, &out, D.st) != BBT_OK) {
+            cerr << "Error: " << bbt_last_error() << endl;
+            exit(1);
+        }
+        D.nh.resize(nrec);
+        D.hits.resize(nrec * kMaxHits * 7);
+        D.sat.resize(nrec);
+        uint64_t ctr[4];
+        if (hipMemcpyAsync(D.nh.data(), D.dnh, nrec * 4, hipMemcpyDeviceToHost, D.st) !=
+                hipSuccess ||
+            hipMemcpyAsync(D.hits.data(), D.dhits, nrec * kMaxHits * 7 * 4,
+                           hipMemcpyDeviceToHost, D.st) != hipSuccess ||
+            hipMemcpyAsync(D.sat.data(), D.dsat, nrec * 4, hipMemcpyDeviceToHost, D.st) !=
+                hipSuccess ||
+            hipMemcpyAsync(ctr, D.dctr, 32, hipMemcpyDeviceToHost, D.st) != hipSuccess ||
+            hipStreamSynchronize(D.st) != hipSuccess)
+            Device::die("download failed");
+        if (ctr[2] && !m_warned) {
+            cerr << "Warning: reads with more than " << kMaxHits
+                 << " reported candidates; the output lists the first " << kMaxHits << endl;
+            m_warned = true;
+        }
+        string outStr;
+        for (size_t r = 0; r < nrec; ++r) {
+            const uint32_t n = std::min(D.nh[r], kMaxHits);
+            const uint32_t *h = D.hits.data() + r * kMaxHits * 7;
+            if (!paired) {
+                /* filterSingleRead (MIBFClassifier.hpp:738-751) */
+                update_summary(h, n);
+                if (m_o.hitOnly && n == 0)
+                    continue;
+                outStr.clear();
+                format(recs[r], D.sat[r], h, n, outStr);
+            } else {
+                /* filterPairedRead (:753-771): hitOnly returns before the
+                 * summary update */
+                if (m_o.hitOnly && n == 0)
+                    continue;
+                update_summary(h, n);
+                outStr.clear();
+                format(recs[2 * r], D.sat[r], h, n, outStr);
+                format(recs[2 * r + 1], D.sat[r], h, n, outStr);
+            }
+            cout << outStr;
+        }
+    }
+
+    /* ResultsManager::updateSummaryData over QueryResults (ResultsManager.hpp:40-64) */
+    void update_summary(const uint32_t *h, uint32_t n) {
+        for (uint32_t i = 0; i < n; ++i)
+            ++m_above[h[i * 7]];
+        if (n == 0)
+            ++m_noMatch;
+        else if (n == 1)
+            ++m_unique[h[0]];
+        else
+            ++m_multi;
+    }
+
+    /* appendResults (MIBFClassifier.hpp:652-673): name, nonSatFrameCount,
+     * solidCount, count, nonSatCount, totalNonSatCount, totalCount */
+    void append(const uint32_t *h, uint32_t n, string &o) {
+        o += "\t";
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t *c = h + i * 7;
+            if (i)
+                o += ";";
+            o += m_names[c[0]];
+            o += "," + to_string(c[6]) + "," + to_string(c[5]) + "," + to_string(c[1]) + "," +
+                 to_string(c[2]) + "," + to_string(c[4]) + "," + to_string(c[3]);
+        }
+    }
+
+    /* formatOutStr (MIBFClassifier.hpp:675-736) */
+    void format(const bbt_fe::SeqRecord &rd, uint32_t sat, const uint32_t *h, uint32_t n,
+                string &o) {
+        const string head = rd.name + (rd.comment.empty() ? "" : " " + rd.comment);
+        if (m_o.out == FASTA || m_o.out == FASTQ) {
+            o += m_o.out == FASTA ? ">" : "@";
+            o += head;
+            if (n) {
+                o += "\t" + m_names[h[0]];
+                append(h, n, o);
+            }
+            o += "\n" + rd.seq;
+            if (m_o.out == FASTQ)
+                o += "\n+\n" + rd.qual;
+        } else if (n == 0) {
+            o += "*\t" + head + "\t" + to_string(sat) + "\t*";
+        } else {
+            o += m_names[h[0]] + "\t" + head + "\t" + to_string(sat);
+            append(h, n, o);
+        }
+        o += "\n";
+    }
+
+    void finish() {
+        cerr << "Total Reads:" << m_numRead << endl;
+        cerr << "Writing file: " << m_o.prefix << "_summary.tsv" << endl;
+        vector<uint64_t> sm(2 * (m_info.nids + 1) + 2);
+        for (uint32_t i = 0; i <= m_info.nids; ++i) {
+            sm[i] = m_above[i];
+            sm[m_info.nids + 1 + i] = m_unique[i];
+        }
+        sm[2 * (m_info.nids + 1)] = m_multi;
+        sm[2 * (m_info.nids + 1) + 1] = m_noMatch;
+        uint64_t need = 0;
+        bbt_mibf_summary_text(m_mf, sm.data(), m_numRead, nullptr, 0, &need);
+        string text(need + 1, '\0');
+        bbt_mibf_summary_text(m_mf, sm.data(), m_numRead, &text[0], need + 1, nullptr);
+        text.resize(need);
+        ofstream so(m_o.prefix + "_summary.tsv");
+        so << text;
+        cout.flush();
+    }
+
+    Opt m_o;
+    bbt_mibf *m_mf = nullptr;
+    bbt_mibf_info_t m_info{};
+    vector<string> m_names;
+    uint32_t m_allowedMiss = 0;
+    Device m_dev;
+    uint64_t m_numRead = 0, m_multi = 0, m_noMatch = 0;
+    vector<uint64_t> m_above, m_unique;
+    bool m_warned = false;
+};
+
+} // namespace
+
+int main(int argc, char *argv[]) {
+    Opt o;
+    int fq = 0, fa = 0, tsv = 0, version = 0, debug = 0;
+    bool die = false;
+    static struct option lo[] = {{"prefix", required_argument, NULL, 'p'},
+                                 {"filter", required_argument, NULL, 'f'},
+                                 {"paired", no_argument, NULL, 'e'},
+                                 {"min_FPR", required_argument, NULL, 's'},
+                                 {"help", no_argument, NULL, 'h'},
+                                 {"interval", required_argument, NULL, 'I'},
+                                 {"threads", required_argument, NULL, 't'},
+                                 {"frameMatches", required_argument, NULL, 'a'},
+                                 {"fq", no_argument, &fq, 1},
+                                 {"fa", no_argument, &fa, 1},
+                                 {"tsv", no_argument, &tsv, 1},
+                                 {"hitOnly", no_argument, NULL, 'i'},
+                                 {"version", no_argument, &version, 1},
+                                 {"multi", required_argument, NULL, 'm'},
+                                 {"streak", required_argument, NULL, 'r'},
+                                 {"minNoSat", required_argument, NULL, 'c'},
+                                 {"bestHitAgree", no_argument, NULL, 'b'},
+                                 {"stdout", no_argument, NULL, 'd'},
+                                 {"inverse", no_argument, NULL, 'n'},
+                                 {"debug", no_argument, &debug, 1},
+                                 {"verbose", no_argument, NULL, 'v'},
+                                 {NULL, 0, NULL, 0}};
+    int c, idx = 0;
+    while ((c = getopt_long(argc, argv, "p:f:es:hI:t:f:m:r:dnvc:a:bi", lo, &idx)) != -1) {
+        stringstream cv(optarg ? optarg : "");
+        switch (c) {
+        case 's': {
+            double raw = 0;
+            cv >> raw;
+            o.score = std::pow(10.0, -(raw / 10.0));
+            break;
+        }
+        case 'f': o.filter = optarg; break;
+        case 'p': o.prefix = optarg; break;
+        case 'h': usage(); break;
+        case 'I':
+            if (!(cv >> o.interval)) {
+                cerr << "Error - Invalid parameters! I: " << optarg << endl;
+                return 0;
+            }
+            break;
+        case 'e': o.paired = true; break;
+        case 'a':
+            if (!(cv >> o.frameMatches)) {
+                cerr << "Error - Invalid parameter! a: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
+        case 'b': o.bestAgree = true; break;
+        case 't': break;
+        case 'v': ++o.verbose; break;
+        case 'r':
+            if (!(cv >> o.streak)) {
+                cerr << "Error - Invalid parameter! r: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
+        case 'c':
+            if (!(cv >> o.minNoSat)) {
+                cerr << "Error - Invalid parameter! c: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
+        case 'm':
+            if (!(cv >> o.multi)) {
+                cerr << "Error - Invalid parameter! m: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
+        case 'd':
+        case 'n': break; /* parsed, unused, as in the reference */
+        case 'i': o.hitOnly = true; break;
+        case 0: break;
+        case '?': die = true; break;
+        default: die = true;
+        }
+    }
+    if (version) {
+        cerr << "biobloommicategorizer (biobloom_amd)" << endl;
+        return 0;
+    }
+    vector<string> inputs;
+    while (optind < argc)
+        inputs.push_back(argv[optind++]);
+    if (o.paired && inputs.size() != 2) {
+        cerr << "Usage of paired end mode:\n"
+             << "BioBloomCategorizer [OPTION]... -f \"[FILTER1]...\" [FILEPAIR1] [FILEPAIR2]\n"
+             << endl;
+        exit(1);
+    }
+    if (o.streak < o.minNoSat) {
+        cerr << "-r should be greater or equalt to -c" << endl;
+        exit(1);
+    }
+    if (inputs.empty()) {
+        cerr << "Error: Need Input File" << endl;
+        die = true;
+    }
+    if (o.filter.empty()) {
+        cerr << "Error: Need Filter File (-f)" << endl;
+        die = true;
+    }
+    if (die) {
+        cerr << "Try '--help' for more information.\n";
+        exit(EXIT_FAILURE);
+    }
+    if (o.prefix.find('/') != string::npos) {
+        const string dir = o.prefix.substr(0, o.prefix.find_last_of('/'));
+        struct stat sb;
+        if (stat(dir.c_str(), &sb) != 0) {
+            cerr << "Error: Output folder does not exist. " << dir << endl;
+            exit(1);
+        } else if (!S_ISDIR(sb.st_mode)) {
+            cerr << "Error: Output folder - file exists with this name. " << dir << endl;
+            exit(1);
+        }
+    }
+    if (fa && fq) {
+        cerr << "Error: fasta (--fa) and fastq (--fq) outputs types cannot be both set" << endl;
+        exit(1);
+    }
+    o.out = fq ? FASTQ : (fa ? FASTA : TSV);
+    (void)tsv;
+    if (debug) {
+        cerr << "Error: --debug output is not available in this build" << endl;
+        exit(1);
+    }
+    if (!fexists(o.filter)) {
+        cerr << "Error: " + o.filter + " File cannot be opened" << endl;
+        exit(1);
+    }
+    MIBFClassifier BMC(o);
+    if (o.paired)
+        BMC.filterPair(inputs[0], inputs[1]);
+    else
+        BMC.filter(inputs);
+    return 0;
+}

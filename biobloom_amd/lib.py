@@ -153,6 +153,8 @@ _SIGS = {
     "bbt_mibf_seed": (C.c_char_p, [C.c_void_p, C.c_uint32]),
     "bbt_mibf_download": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bbt_mibf_calc_size": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_double]),
+    "bbt_mibf_summary_text": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64,
+                                          C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bbt_mibf_frame_probs": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
     "bbt_mibf_min_count": (C.c_uint32, [C.c_uint64, C.c_double, C.c_double, C.c_uint32]),
     "bbt_mibf_query_batch": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(MibfParams),
@@ -374,6 +376,15 @@ class Mibf:
 
     def store(self, prefix):
         check(load().bbt_mibf_store(self._h, prefix.encode()))
+
+    def summary_text(self, summary_counts, nrec):
+        L = load()
+        arr = (C.c_uint64 * len(summary_counts))(*[int(x) for x in summary_counts])
+        need = C.c_uint64()
+        check(L.bbt_mibf_summary_text(self._h, arr, nrec, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value + 1)
+        check(L.bbt_mibf_summary_text(self._h, arr, nrec, buf, need.value + 1, None))
+        return buf.value.decode()
 
     def frame_probs(self, allowed_miss):
         n = self.info().nids + 1

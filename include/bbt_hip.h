@@ -340,6 +340,12 @@ const char *bbt_mibf_seed(const bbt_mibf *mf, uint32_t i);
 /* host copies: bv = m/64 words of the bit vector, data = pop slots */
 bbt_status bbt_mibf_download(const bbt_mibf *mf, uint64_t *bv, uint32_t *data);
 uint64_t bbt_mibf_calc_size(uint64_t entries, uint32_t nseeds, double occupancy);
+/* <prefix>_summary.tsv text of biobloommicategorizer (ResultsManager<ID>
+ * over "repeat" and the ID names, ResultsManager.hpp:40-64, 191-237) from
+ * host counters: summary[0..nids] hits per ID, [nids+1..2nids+1] unique hits
+ * per ID, [2nids+2] multiMatch, [2nids+3] noMatch. */
+bbt_status bbt_mibf_summary_text(const bbt_mibf *mf, const uint64_t *summary, uint64_t nrec,
+                                 char *buf, uint64_t cap, uint64_t *needed);
 /* probs[0..nids]: chance that a random frame reports each ID */
 bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, double *probs);
 uint32_t bbt_mibf_min_count(uint64_t frames, double prob, double score, uint32_t nids);
