@@ -244,6 +244,8 @@ def serve_run(B, fs, params, pool, mode, qps, seconds, max_requests, seed, want)
             "reads_per_s": reads / ((last - first) / 1e6),
             "offered_reads_per_s": reads / arrivals[-1] if arrivals[-1] > 0 else None,
             "waves": stats["waves"], "mean_wave_requests": stats["mean_wave_requests"],
+            "submit_ms": stats["submit_ms"], "pack_ms": stats["pack_ms"],
+            "device_ms": stats["device_ms"], "span_ms": (last - first) / 1e3,
             "max_waves_in_flight": stats["max_waves_in_flight"],
             "equal_to_alone": equal}
 
@@ -281,7 +283,9 @@ def bench_serve(args, torch, D, B):
             runs[mode].append(r)
             log(f"{mode} qps={qps:g}: {r['reads_per_s'] / 1e6:.2f} M reads/s, p50 "
                 f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, waves {r['waves']}, "
-                f"{r['mean_wave_requests']:.2f} req/wave, equal={r['equal_to_alone']}")
+                f"{r['mean_wave_requests']:.2f} req/wave, equal={r['equal_to_alone']}; "
+                f"span {r['span_ms']:.0f} ms: submit {r['submit_ms']:.0f}, pack "
+                f"{r['pack_ms']:.0f}, device {r['device_ms']:.0f} ms")
     top = runs["batched"][-1]
     line = {
         "metric": METRIC, "value": top["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
@@ -327,6 +331,10 @@ def main():
     ap.add_argument("--path", default="both", choices=["both", "batched", "plain"])
     ap.add_argument("--wave-min", type=int, default=0,
                     help="batched path: reads this long are walked by a whole wave")
+    ap.add_argument("--refill-min", type=int, default=0,
+                    help="batched path: idle lanes that trigger a queue refill")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="batched path: persistent blocks per CU (0 = occupancy)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the batched path's scheduling knobs on this batch first")
     ap.add_argument("--no-gather", action="store_true",
@@ -386,6 +394,10 @@ def main():
                                    wave_min_len=args.wave_min), os_, stream)
 
     tuning = None
+    if args.refill_min or args.blocks_per_cu:
+        fs.set_tuning(wave_min_len=args.wave_min, refill_min=args.refill_min,
+                      blocks_per_cu=args.blocks_per_cu)
+        tuning = fs.get_tuning()
     if args.autotune:
         t = time.time()
         tuning = fs.autotune(rs, B.params(paired=paired), os_, stream)

@@ -493,6 +493,107 @@ struct Walker {
         return false;
     }
 
+    /* The walk over one chunk of up to 64 consecutive windows whose contains()
+     * answers are already known: lane L of the chunk is window
+     * b0 + L - k + 1, vmask marks the N-free windows, hmask the windows whose
+     * bits were all set, and lk(a, b) returns the bit lookups issued for lanes
+     * [a, b) (the lookups a reference contains() makes: 1..h, stopping at the
+     * first clear bit).  Three kinds of runs are consumed in one step each,
+     * with the same effect as window_pre / on_result applied window by window:
+     *   - windows inside a jump-skip (SeqEval.h:99-109): anti + 1 each, no probe;
+     *   - contiguous misses while streak < streakThreshold (SeqEval.h:94-98):
+     *     anti + 1 each, streak 0;
+     *   - contiguous hits, SIMPLE and BINOMIAL (SeqEval.h:78-92, 236-262):
+     *     +1 then +2 per hit (2 x score), or +1 per hit, accepting at the
+     *     first window that reaches the threshold.
+     * Everything else (the first window, gaps after N, the miss that starts a
+     * jump-skip, LENGTH, HARMONIC hits) goes through the single-window
+     * steps. */
+    template <typename LK>
+    BBT_HD void replay(uint32_t b0, uint64_t vmask, uint64_t hmask, const LK &lk) {
+        const unsigned kv = kk();
+        uint64_t vm = vmask;
+        while (vm && state == W_RUNNING) {
+            const uint32_t L = uint32_t(__builtin_ctzll(vm));
+            const uint32_t pw = b0 + L - kv + 1;
+            if (SC != LENGTH && (EXH || !first)) {
+                if (skipping && pw < skipEnd) {
+                    const uint32_t span = skipEnd - pw;
+                    const uint64_t sm =
+                        vm & (span >= 64 - L ? ~uint64_t(0) << L
+                                             : ((uint64_t(1) << span) - 1) << L);
+                    const uint32_t c = uint32_t(__builtin_popcountll(sm));
+                    if (!EXH) {
+                        if (anti_thr <= anti + c) {
+                            state = W_REJECT;
+                            return;
+                        }
+                        anti += c;
+                    }
+                    prev = b0 + uint32_t(63 - __builtin_clzll(sm)) - kv + 1;
+                    vm &= ~sm;
+                    continue;
+                }
+                skipping = 0;
+                if (pw == prev + 1) {
+                    const bool hit = (hmask >> L) & 1;
+                    const uint64_t same = (hit ? hmask : ~hmask) & vm;
+                    const uint64_t x = ~(same >> L);
+                    const uint32_t r = x ? uint32_t(__builtin_ctzll(x)) : 64 - L;
+                    const uint64_t run =
+                        r >= 64 ? ~uint64_t(0) : ((uint64_t(1) << r) - 1) << L;
+                    if (!hit && streak < streak_thr) {
+                        if (!EXH && anti_thr <= anti + r) {
+                            const uint32_t n = anti_thr > anti ? anti_thr - anti : 1u;
+                            probes += n;
+                            lookups += lk(L, L + n);
+                            state = W_REJECT;
+                            return;
+                        }
+                        if (!EXH)
+                            anti += r;
+                        probes += r;
+                        lookups += lk(L, L + r);
+                        streak = 0;
+                        prev = pw + r - 1;
+                        vm &= ~run;
+                        continue;
+                    }
+                    if (hit && (SC == SIMPLE || SC == BINOMIAL)) {
+                        /* score after n hits: SIMPLE 2n - (streak == 0), BINOMIAL n */
+                        const uint32_t z = (SC == SIMPLE && streak == 0) ? 1u : 0u;
+                        const uint32_t per = SC == SIMPLE ? 2u : 1u;
+                        if (!EXH && uint64_t(score2) + uint64_t(per) * r - z >= thr2) {
+                            const uint32_t d = thr2 - score2; /* >= 1: not accepted yet */
+                            const uint32_t n = SC == SIMPLE ? (d + z + 1) / 2 : d;
+                            score2 += per * n - z;
+                            probes += n;
+                            lookups += lk(L, L + n);
+                            state = W_ACCEPT;
+                            return;
+                        }
+                        score2 += per * r - z;
+                        streak += r;
+                        probes += r;
+                        lookups += lk(L, L + r);
+                        prev = pw + r - 1;
+                        vm &= ~run;
+                        continue;
+                    }
+                }
+            }
+            vm &= vm - 1;
+            const int pre = window_pre(pw);
+            if (pre == PRE_END)
+                return;
+            if (pre == PRE_NOPROBE)
+                continue;
+            ++probes;
+            lookups += lk(L, L + 1);
+            on_result(((hmask >> L) & 1) != 0);
+        }
+    }
+
     /* Result of a finished walk: evalRead's bool, or evalScore's value (for
      * BINOMIAL, the hit count; the caller applies the binomial tail). */
     BBT_HD bool accepted() const { return state == W_ACCEPT; }

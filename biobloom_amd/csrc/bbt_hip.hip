@@ -225,20 +225,19 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         }
         issued += lk;
         const uint64_t hmask = __ballot(allset);
-        for (uint64_t vm = vmask; vm; vm &= vm - 1) {
-            const int L = __ffsll((unsigned long long)vm) - 1;
-            const uint32_t p = b0 + uint32_t(L) - kv + 1;
-            const int pre = w.window_pre(p);
-            if (pre == w.PRE_END)
-                break;
-            if (pre == w.PRE_NOPROBE)
-                continue;
-            ++w.probes;
-            w.lookups += uint32_t(__shfl(int(lk), L));
-            w.on_result(((hmask >> L) & 1) != 0);
-            if (w.state != W_RUNNING)
-                break;
+        /* lookups of lanes [x, y) from an inclusive prefix sum across lanes */
+        uint32_t lkp = lk;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(lkp, unsigned(d));
+            if (int(lane) >= d)
+                lkp += t;
         }
+        const auto lk_range = [&](uint32_t x, uint32_t y) -> uint32_t {
+            const uint32_t hi = uint32_t(__shfl(int(lkp), int(y - 1)));
+            const uint32_t lo = x ? uint32_t(__shfl(int(lkp), int(x - 1))) : 0u;
+            return hi - lo;
+        };
+        w.replay(b0, vmask, hmask, lk_range);
     }
     if (w.state == W_RUNNING)
         w.finish_end();

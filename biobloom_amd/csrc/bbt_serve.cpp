@@ -158,7 +158,7 @@ struct WaveLayout {
 struct Slot {
     hipStream_t st = nullptr;
     bbt_workspace *ws = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr, done = nullptr; /* around the launch's device work */
     char *h_seq = nullptr;
     uint64_t *h_off = nullptr;
     uint8_t *h_hits = nullptr;
@@ -190,6 +190,7 @@ struct bbt_server {
     bool stop = false;
     uint64_t next_id = 1, waves = 0, done_reqs = 0, done_reads = 0, max_inflight = 0;
     uint64_t wave_reads_sum = 0, wave_reqs_sum = 0;
+    double submit_ms = 0, pack_ms = 0, device_ms = 0; /* totals */
     std::string error;
     Clock::time_point t0;
 
@@ -216,6 +217,7 @@ void fail_wave(bbt_server *S, Slot &s, const std::string &msg) {
 
 bbt_status bbt_server::launch(Slot &s) {
     const bool paired = params.paired != 0;
+    const Clock::time_point tp = Clock::now();
     /* pack the wave's reads into the pinned staging buffers */
     uint64_t nreads = 0, bases = 0;
     uint32_t max_len = 0;
@@ -232,6 +234,12 @@ bbt_status bbt_server::launch(Slot &s) {
     const WaveLayout L = WaveLayout::of(nreads, bases, nrec, nf);
     std::memset(s.h_seq + bases, 0, L.seq_bytes - bases);
     char *blk = darena + s.block_off;
+    const double packed = std::chrono::duration<double, std::milli>(Clock::now() - tp).count();
+    {
+        std::lock_guard<std::mutex> g(mu);
+        pack_ms += packed;
+    }
+    SHIP(hipEventRecord(s.start, s.st));
     SHIP(hipMemcpyAsync(blk + L.seq, s.h_seq, L.seq_bytes, hipMemcpyHostToDevice, s.st));
     SHIP(hipMemcpyAsync(blk + L.off, s.h_off, (nreads + 1) * 8, hipMemcpyHostToDevice, s.st));
     bbt_reads in;
@@ -369,6 +377,9 @@ void bbt_server::complete_loop() {
                 error = std::string("wave failed: ") + hipGetErrorString(q);
                 fail_wave(this, s, error);
             } else {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, s.start, s.done) == hipSuccess)
+                    device_ms += ms;
                 uint64_t rd = 0, rc = 0;
                 for (auto &r : s.reqs) {
                     r->hits.assign(s.h_hits + rd * nf, s.h_hits + (rd + r->nreads) * nf);
@@ -430,7 +441,8 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
     S->slots.resize(S->cfg.slots);
     for (Slot &s : S->slots) {
         SHIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-        SHIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        SHIP(hipEventCreate(&s.start));
+        SHIP(hipEventCreate(&s.done));
         bbt_status st = bbt_workspace_create(&s.ws);
         if (st != BBT_OK)
             return st;
@@ -454,6 +466,7 @@ bbt_status bbt_server_submit(bbt_server *S, const char *seq, const uint64_t *off
         return sfail(BBT_ERR_ARG, "bbt_server_submit: empty request");
     if (S->params.paired && nreads % 2)
         return sfail(BBT_ERR_ARG, "bbt_server_submit: paired request with an odd read count");
+    const Clock::time_point t_in = Clock::now();
     auto r = std::make_shared<Request>();
     const uint64_t b0 = off[0], bases = off[nreads] - off[0];
     if (nreads > S->cfg.max_wave_reads || bases > S->cfg.max_wave_bases)
@@ -472,7 +485,8 @@ bbt_status bbt_server_submit(bbt_server *S, const char *seq, const uint64_t *off
     {
         std::lock_guard<std::mutex> g(S->mu);
         r->id = S->next_id++;
-        r->t_submit = Clock::now();
+        r->t_submit = t_in;
+        S->submit_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_in).count();
         S->reqs[r->id] = r;
         S->pending.push_back(r);
     }
@@ -572,6 +586,9 @@ bbt_status bbt_server_get_stats(bbt_server *S, bbt_server_stats *out) {
     out->arena_high_water = S->arena.high_water();
     out->mean_wave_reads = S->waves ? double(S->wave_reads_sum) / double(S->waves) : 0.0;
     out->mean_wave_requests = S->waves ? double(S->wave_reqs_sum) / double(S->waves) : 0.0;
+    out->submit_ms = S->submit_ms;
+    out->pack_ms = S->pack_ms;
+    out->device_ms = S->device_ms;
     return BBT_OK;
 }
 
@@ -603,6 +620,8 @@ bbt_status bbt_server_destroy(bbt_server *S) {
             bbt_workspace_free(s.ws);
         if (s.done)
             (void)hipEventDestroy(s.done);
+        if (s.start)
+            (void)hipEventDestroy(s.start);
         if (s.st)
             (void)hipStreamDestroy(s.st);
         (void)hipHostFree(s.h_seq);

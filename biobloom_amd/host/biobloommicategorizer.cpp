@@ -91,10 +91,11 @@ struct Device {
     size_t seq_cap = 0, rec_cap = 0;
     vector<uint32_t> nh, hits, sat;
 
-    void reserve(size_t bases, size_t nreads) {
-        if (bases + 8 > seq_cap) {
+    /* bytes: the padded sequence bytes of the batch */
+    void reserve(size_t bytes, size_t nreads) {
+        if (bytes > seq_cap) {
             (void)hipFree(dseq);
-            seq_cap = std::max(bases + 8, seq_cap * 2);
+            seq_cap = std::max(bytes, seq_cap * 2);
             if (hipMalloc(&dseq, seq_cap) != hipSuccess)
                 die("device allocation failed");
         }
@@ -200,8 +201,8 @@ class MIBFClassifier {
         }
         const size_t nreads = recs.size(), nrec = paired ? nreads / 2 : nreads;
         Device &D = m_dev;
-        D.reserve(seq.size(), nreads);
         const size_t padded = (seq.size() + 7) / 8 * 8 + 8;
+        D.reserve(padded, nreads);
         seq.resize(padded, '\0');
         uint64_t zero[4] = {0, 0, 0, 0};
         if (hipMemcpyAsync(D.dseq, seq.data(), padded, hipMemcpyHostToDevice, D.st) !=

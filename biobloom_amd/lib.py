@@ -85,7 +85,8 @@ class ServerStats(C.Structure):
     _fields_ = [("waves", C.c_uint64), ("requests_done", C.c_uint64),
                 ("reads_done", C.c_uint64), ("max_waves_in_flight", C.c_uint64),
                 ("arena_bytes", C.c_uint64), ("arena_high_water", C.c_uint64),
-                ("mean_wave_reads", C.c_double), ("mean_wave_requests", C.c_double)]
+                ("mean_wave_reads", C.c_double), ("mean_wave_requests", C.c_double),
+                ("submit_ms", C.c_double), ("pack_ms", C.c_double), ("device_ms", C.c_double)]
 
 
 # exported symbols and their signatures: (restype, argtypes)

@@ -249,6 +249,9 @@ typedef struct {
     uint64_t waves, requests_done, reads_done, max_waves_in_flight;
     uint64_t arena_bytes, arena_high_water;
     double mean_wave_reads, mean_wave_requests;
+    double submit_ms;  /* host time spent copying requests in bbt_server_submit */
+    double pack_ms;    /* host time packing launches into pinned staging */
+    double device_ms;  /* device time of the launches: copies in, kernels, copies out */
 } bbt_server_stats;
 
 /* p: the classification parameters of every request (BBT_STD or BBT_ORDERED;

@@ -98,21 +98,16 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
                 Hp[L] = H[L];
                 Gp[L] = G[L];
             }
+            uint64_t vmask = 0, hmask = 0;
+            uint32_t lkp[65];
+            lkp[0] = 0;
             for (int L = 0; L < 64; ++L) {
-                if (!valid[L])
-                    continue;
-                const uint32_t p = b0 + L - kv + 1;
-                const int pre = w.window_pre(p);
-                if (pre == w.PRE_END)
-                    break;
-                if (pre == w.PRE_NOPROBE)
-                    continue;
-                ++w.probes;
-                w.lookups += lk[L];
-                w.on_result(allset[L]);
-                if (w.state != W_RUNNING)
-                    break;
+                vmask |= uint64_t(valid[L]) << L;
+                hmask |= uint64_t(allset[L]) << L;
+                lkp[L + 1] = lkp[L] + lk[L];
             }
+            w.replay(b0, vmask, hmask,
+                     [&](uint32_t x, uint32_t y) -> uint32_t { return lkp[y] - lkp[x]; });
         }
         if (w.state == W_RUNNING)
             w.finish_end();
