@@ -53,7 +53,7 @@ namespace {
 
 constexpr int kBlock = 256;
 /* default length from which the batched path walks a read with a whole wave */
-constexpr uint32_t kDefaultWaveMin = 1024;
+constexpr uint32_t kDefaultWaveMin = 256;
 
 struct EvalArgs {
     const uint8_t *seq;
@@ -1478,12 +1478,12 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     bbt_status s = trial(cur, best_ms);
     if (s != BBT_OK)
         return s;
-    const uint32_t waves[] = {256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
+    const uint32_t waves[] = {64, 128, 256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
     const uint32_t refills[] = {1, 4, 8, 16, 32};
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
     for (int dim = 0; dim < 3; ++dim) {
         const uint32_t *vals = dim == 0 ? waves : (dim == 1 ? refills : bpcs);
-        const int nv = dim == 0 ? 6 : 5;
+        const int nv = dim == 0 ? 8 : 5;
         for (int i = 0; i < nv; ++i) {
             bbt_tuning t = cur;
             if (dim == 0) {

@@ -7,14 +7,16 @@
  * the filters resident in device memory and serves independent requests
  * (a batch of reads each) with continuous batching:
  *
- *   submit      copies the request's reads into host memory and queues it
- *               (FIFO); any thread may submit.
+ *   submit      copies the request's reads into a pinned host arena (waiting
+ *               while the arena is full) and queues the request (FIFO); any
+ *               thread may submit.
  *   dispatcher  forms a launch ("wave") from the head of the queue whenever a
  *               slot is free: requests are merged while the wave stays within
  *               the read / base caps and its device block (reads + outputs)
- *               fits in the device arena -- admission by device memory.  The
- *               wave is staged through the slot's pinned buffers, copied to
- *               the device, classified by the batched persistent path
+ *               fits in the device arena -- admission by device memory.  Each
+ *               request's bases are copied to the device straight from the
+ *               pinned arena (the offsets are rebased through the slot's
+ *               pinned buffer), classified by the batched persistent path
  *               (bbt_eval_batch_ws on the slot's stream and workspace) and its
  *               per-read hits and per-record classes copied back.  Up to
  *               `slots` waves are in flight on separate streams, so copies of
@@ -122,7 +124,8 @@ enum ReqState { PENDING = 0, ADMITTED = 1, DONE = 2, FAILED = 3 };
 
 struct Request {
     uint64_t id = 0;
-    std::vector<char> seq;
+    uint64_t host_off = 0, bases = 0; /* the bases, in the pinned host arena */
+    bool host_held = false;
     std::vector<uint64_t> off; /* nreads + 1, from 0 */
     uint64_t nreads = 0, nrec = 0;
     uint32_t max_len = 0;
@@ -159,7 +162,6 @@ struct Slot {
     hipStream_t st = nullptr;
     bbt_workspace *ws = nullptr;
     hipEvent_t start = nullptr, done = nullptr; /* around the launch's device work */
-    char *h_seq = nullptr;
     uint64_t *h_off = nullptr;
     uint8_t *h_hits = nullptr;
     uint32_t *h_cls = nullptr;
@@ -180,6 +182,9 @@ struct bbt_server {
     uint32_t nf = 0;
     char *darena = nullptr;
     Arena arena;
+    char *harena = nullptr; /* pinned: bases of queued and in-flight requests */
+    Arena hostmem;
+    std::condition_variable cv_host;
     std::vector<Slot> slots;
     std::deque<std::shared_ptr<Request>> pending;
     std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
@@ -197,6 +202,15 @@ struct bbt_server {
     void dispatch_loop();
     void complete_loop();
     bbt_status launch(Slot &s);
+    /* with mu held: the wave's pinned host ranges go back to the arena */
+    void release_host(Slot &s) {
+        for (auto &r : s.reqs)
+            if (r->host_held) {
+                hostmem.release(r->host_off, r->bases);
+                r->host_held = false;
+            }
+        cv_host.notify_all();
+    }
 };
 
 namespace {
@@ -223,16 +237,14 @@ bbt_status bbt_server::launch(Slot &s) {
     uint32_t max_len = 0;
     s.h_off[0] = 0;
     for (auto &r : s.reqs) {
-        std::memcpy(s.h_seq + bases, r->seq.data(), r->seq.size());
         for (uint64_t i = 0; i < r->nreads; ++i)
             s.h_off[nreads + i + 1] = bases + r->off[i + 1];
         nreads += r->nreads;
-        bases += r->seq.size();
+        bases += r->bases;
         max_len = std::max(max_len, r->max_len);
     }
     const uint64_t nrec = paired ? nreads / 2 : nreads;
     const WaveLayout L = WaveLayout::of(nreads, bases, nrec, nf);
-    std::memset(s.h_seq + bases, 0, L.seq_bytes - bases);
     char *blk = darena + s.block_off;
     const double packed = std::chrono::duration<double, std::milli>(Clock::now() - tp).count();
     {
@@ -240,7 +252,15 @@ bbt_status bbt_server::launch(Slot &s) {
         pack_ms += packed;
     }
     SHIP(hipEventRecord(s.start, s.st));
-    SHIP(hipMemcpyAsync(blk + L.seq, s.h_seq, L.seq_bytes, hipMemcpyHostToDevice, s.st));
+    /* the bases of each request, straight from the pinned arena (the padding
+     * past the last base is never read) */
+    uint64_t at = 0;
+    for (auto &r : s.reqs) {
+        if (r->bases)
+            SHIP(hipMemcpyAsync(blk + L.seq + at, harena + r->host_off, r->bases,
+                                hipMemcpyHostToDevice, s.st));
+        at += r->bases;
+    }
     SHIP(hipMemcpyAsync(blk + L.off, s.h_off, (nreads + 1) * 8, hipMemcpyHostToDevice, s.st));
     bbt_reads in;
     std::memset(&in, 0, sizeof(in));
@@ -298,9 +318,9 @@ void bbt_server::dispatch_loop() {
         const size_t maxreq = cfg.mode == BBT_SERVE_PLAIN ? 1 : pending.size();
         for (size_t i = 0; i < pending.size() && i < maxreq; ++i) {
             const Request &r = *pending[i];
-            if (nreads + r.nreads > cfg.max_wave_reads || bases + r.seq.size() > cfg.max_wave_bases)
+            if (nreads + r.nreads > cfg.max_wave_reads || bases + r.bases > cfg.max_wave_bases)
                 break;
-            const WaveLayout L = WaveLayout::of(nreads + r.nreads, bases + r.seq.size(),
+            const WaveLayout L = WaveLayout::of(nreads + r.nreads, bases + r.bases,
                                                 paired ? (nreads + r.nreads) / 2 : nreads + r.nreads,
                                                 nf);
             uint64_t off = 0;
@@ -308,7 +328,7 @@ void bbt_server::dispatch_loop() {
                 break;
             arena.release(off, L.total);
             nreads += r.nreads;
-            bases += r.seq.size();
+            bases += r.bases;
             take = i + 1;
             block = L.total;
         }
@@ -342,6 +362,7 @@ void bbt_server::dispatch_loop() {
         if (st != BBT_OK) {
             error = last_error();
             fail_wave(this, s, error);
+            release_host(s);
             arena.release(s.block_off, s.block_len);
             s.reqs.clear();
             s.busy = false;
@@ -392,6 +413,7 @@ void bbt_server::complete_loop() {
                     done_reads += r->nreads;
                 }
             }
+            release_host(s);
             arena.release(s.block_off, s.block_len);
             s.reqs.clear();
             s.busy = false;
@@ -438,6 +460,12 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
                                       std::to_string(Lmax.total) + " bytes)");
     SHIP(hipMalloc(&S->darena, cfg->device_bytes));
     S->arena.reset(cfg->device_bytes);
+    const uint64_t hb = cfg->host_bytes ? cfg->host_bytes
+                                        : align_up(bs, kAlign) * (uint64_t(S->cfg.slots) + 2);
+    if (hb < align_up(bs, kAlign))
+        return sfail(BBT_ERR_ARG, "bbt_server_create: host_bytes below one full wave");
+    SHIP(hipHostMalloc(reinterpret_cast<void **>(&S->harena), hb));
+    S->hostmem.reset(hb);
     S->slots.resize(S->cfg.slots);
     for (Slot &s : S->slots) {
         SHIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
@@ -446,7 +474,6 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
         bbt_status st = bbt_workspace_create(&s.ws);
         if (st != BBT_OK)
             return st;
-        SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_seq), align_up(bs, 8) + 8));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_off), (rd + 1) * 8));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_hits), std::max<uint64_t>(1, rd * S->nf)));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_cls), std::max<uint64_t>(1, rd) * 4));
@@ -471,7 +498,6 @@ bbt_status bbt_server_submit(bbt_server *S, const char *seq, const uint64_t *off
     const uint64_t b0 = off[0], bases = off[nreads] - off[0];
     if (nreads > S->cfg.max_wave_reads || bases > S->cfg.max_wave_bases)
         return sfail(BBT_ERR_ARG, "bbt_server_submit: request larger than one wave");
-    r->seq.assign(seq + b0, seq + b0 + bases);
     r->off.resize(nreads + 1);
     for (uint64_t i = 0; i <= nreads; ++i) {
         if (i && off[i] < off[i - 1])
@@ -482,6 +508,17 @@ bbt_status bbt_server_submit(bbt_server *S, const char *seq, const uint64_t *off
     }
     r->nreads = nreads;
     r->nrec = S->params.paired ? nreads / 2 : nreads;
+    r->bases = bases;
+    {
+        /* room in the pinned arena: requests wait here while queued and
+         * in-flight requests hold it (host-side admission) */
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_host.wait(lk, [&] { return S->stop || S->hostmem.alloc(bases, r->host_off); });
+        if (S->stop)
+            return sfail(BBT_ERR_ARG, "bbt_server_submit: server is stopping");
+        r->host_held = true;
+    }
+    std::memcpy(S->harena + r->host_off, seq + b0, bases);
     {
         std::lock_guard<std::mutex> g(S->mu);
         r->id = S->next_id++;
@@ -599,6 +636,7 @@ bbt_status bbt_server_destroy(bbt_server *S) {
         std::lock_guard<std::mutex> g(S->mu);
         S->stop = true;
     }
+    S->cv_host.notify_all();
     S->cv_dispatch.notify_all();
     S->cv_complete.notify_all();
     if (S->dispatcher.joinable())
@@ -624,12 +662,12 @@ bbt_status bbt_server_destroy(bbt_server *S) {
             (void)hipEventDestroy(s.start);
         if (s.st)
             (void)hipStreamDestroy(s.st);
-        (void)hipHostFree(s.h_seq);
         (void)hipHostFree(s.h_off);
         (void)hipHostFree(s.h_hits);
         (void)hipHostFree(s.h_cls);
     }
     (void)hipFree(S->darena);
+    (void)hipHostFree(S->harena);
     delete S;
     return BBT_OK;
 }

@@ -73,7 +73,8 @@ SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("mode", C.c_uint32), ("device_bytes", C.c_uint64),
-                ("max_wave_reads", C.c_uint64), ("max_wave_bases", C.c_uint64)]
+                ("max_wave_reads", C.c_uint64), ("max_wave_bases", C.c_uint64),
+                ("host_bytes", C.c_uint64)]
 
 
 class RequestStats(C.Structure):
@@ -437,11 +438,12 @@ class Server:
     """Request-serving loop over a resident filter set (bbt_server)."""
 
     def __init__(self, fs, params, slots=4, mode=SERVE_BATCHED, device_bytes=1 << 30,
-                 max_wave_reads=1 << 18, max_wave_bases=64 << 20):
+                 max_wave_reads=1 << 18, max_wave_bases=64 << 20, host_bytes=0):
         self.fs = fs  # keeps the filter set alive
         self.nf = len(fs)
         self.paired = bool(params.paired)
-        cfg = ServerConfig(slots, mode, device_bytes, max_wave_reads, max_wave_bases)
+        cfg = ServerConfig(slots, mode, device_bytes, max_wave_reads, max_wave_bases,
+                           host_bytes)
         h = C.c_void_p()
         check(load().bbt_server_create(fs.handle, C.byref(params), C.byref(cfg), C.byref(h)))
         self._h = h

@@ -236,7 +236,10 @@ typedef struct {
     uint32_t mode;           /* BBT_SERVE_* */
     uint64_t device_bytes;   /* arena for the reads and outputs in flight */
     uint64_t max_wave_reads; /* cap on reads merged into one launch */
-    uint64_t max_wave_bases; /* cap on bases per launch (sizes pinned staging) */
+    uint64_t max_wave_bases; /* cap on bases per launch and per request */
+    uint64_t host_bytes;     /* pinned host arena holding the bases of queued and
+                                in-flight requests; submit waits while it is full
+                                (0 = (slots + 2) x max_wave_bases) */
 } bbt_server_config;
 
 typedef struct {

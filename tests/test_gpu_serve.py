@@ -159,3 +159,21 @@ def test_server_argument_checks(env):
     with pytest.raises(B.BbtError, match="larger than one wave"):
         srv.submit(big, boff)
     srv.close()
+
+
+def test_host_memory_admission(env):
+    """A pinned host arena that holds about one request: submit waits until
+    launched requests release it; every result still equals the alone run."""
+    torch, D, B, synth, fs, genomes = env
+    params = B.params()
+    reqs = make_requests(synth, genomes, 16, 5)
+    biggest = max(int(off[-1]) for _, off in reqs)
+    srv = B.Server(fs, params, slots=2, device_bytes=64 << 20, max_wave_reads=1 << 14,
+                   max_wave_bases=biggest + 4096, host_bytes=biggest + 4096)
+    got = run_server(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["requests_done"] == len(reqs)
+    for (buf, off), (h, c, _) in zip(reqs, got):
+        wh, wc = alone(D, B, fs, buf, off, params)
+        assert np.array_equal(h, wh) and np.array_equal(c, wc)
