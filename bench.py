@@ -319,7 +319,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi",
                     choices=["multi", "single", "uneven", "mibf", "serve"])
-    ap.add_argument("--qps", default="10,50,200,400",
+    ap.add_argument("--qps", default="10,50,200,400,800,1600",
                     help="serve: request arrival rates to sweep")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="serve: arrival window per rate")

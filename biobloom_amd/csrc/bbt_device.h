@@ -144,7 +144,10 @@ struct FilterDev {
     double fpr;              /* (popcount/m)^h, for the binomial scorer */
     const uint32_t *binom;   /* calcMinCount per frame length, or null */
     uint32_t binom_len;      /* entries in binom */
-    uint32_t pad;
+    uint32_t bscore_frames;  /* frame lengths covered by bscore */
+    /* BINOMIAL evalScore: -10 log10 P(X > c), X ~ Bin(fl, fpr), at
+     * [fl (fl + 1) / 2 + c] for 1 <= fl <= bscore_frames, 0 <= c <= fl */
+    const double *bscore;
     uint64_t mult[kMaxHash]; /* mult[j] = j ^ (k * kMultiSeed), j >= 1 */
 };
 

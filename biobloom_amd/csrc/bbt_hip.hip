@@ -54,6 +54,8 @@ namespace {
 constexpr int kBlock = 256;
 /* default length from which the batched path walks a read with a whole wave */
 constexpr uint32_t kDefaultWaveMin = 256;
+/* idle lanes of a wave that trigger a refill from the queue */
+constexpr uint32_t kDefaultRefillMin = 32;
 
 struct EvalArgs {
     const uint8_t *seq;
@@ -105,11 +107,29 @@ template <int SC, bool EXH, int K>
 __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uint32_t f,
                                            const Walker<SC, EXH, K> &w) {
     const uint64_t idx = read * a.nf + f;
-    if (EXH)
-        a.scores[idx] = w.score_value();
-    else
+    bool bad = w.bad != 0;
+    if (EXH) {
+        double v = w.score_value();
+        if (SC == BINOMIAL) {
+            /* evalBinomialScore then log10(.) * -10 (SeqEval.h:450-491, 558):
+             * a read shorter than k has tail 1, i.e. -0 */
+            const FilterDev &F = a.filters[f];
+            const uint32_t kv = w.kk();
+            if (w.len < kv) {
+                v = -0.0;
+            } else {
+                const uint64_t fl = uint64_t(w.len) - kv + 1;
+                if (F.bscore && fl <= F.bscore_frames)
+                    v = F.bscore[fl * (fl + 1) / 2 + w.score2];
+                else
+                    bad = true;
+            }
+        }
+        a.scores[idx] = v;
+    } else {
         a.hits[idx] = w.accepted() ? 1 : 0;
-    if (w.bad && a.counters)
+    }
+    if (bad && a.counters)
         atomicOr(a.counters + 3, 1ull);
 }
 
@@ -657,6 +677,10 @@ struct bbt_filterset {
     std::vector<uint32_t *> dbinom;
     double binom_min_fpr = -1;
     uint32_t binom_frames = 0;
+    /* BINOMIAL evalScore tables per filter (and the FPR they were made for) */
+    std::vector<double *> dbscore;
+    std::vector<double> bscore_fpr;
+    uint32_t bscore_frames = 0;
     /* work queue counter and per-call scratch */
     unsigned long long *dscratch = nullptr; /* [1..2] insert counts, [3] popcount, [4..5] walk queues */
     int cu_count = 0;
@@ -692,6 +716,8 @@ static bbt_status sync_desc(bbt_filterset *fs) {
         fs->desc[i].fpr = fs->info[i].fpr;
         fs->desc[i].binom = i < fs->dbinom.size() ? fs->dbinom[i] : nullptr;
         fs->desc[i].binom_len = fs->dbinom.empty() ? 0 : fs->binom_frames + 1;
+        fs->desc[i].bscore = i < fs->dbscore.size() ? fs->dbscore[i] : nullptr;
+        fs->desc[i].bscore_frames = fs->dbscore.empty() ? 0 : fs->bscore_frames;
     }
     HIP_OK(hipMemcpy(fs->ddesc, fs->desc.data(), sizeof(FilterDev) * n,
                      hipMemcpyHostToDevice));
@@ -815,6 +841,9 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         if (p)
             (void)hipFree(p);
     for (uint32_t *p : fs->dbinom)
+        if (p)
+            (void)hipFree(p);
+    for (double *p : fs->dbscore)
         if (p)
             (void)hipFree(p);
     if (fs->ddesc)
@@ -1131,6 +1160,43 @@ bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) 
     return BBT_OK;
 }
 
+/* BINOMIAL evalScore values for every frame length up to max_frames and
+ * every hit count: log10(P(X > c)) * -10, computed by the same host
+ * binomial tail as the thresholds; remade when a filter's FPR changed. */
+constexpr uint32_t kBinomScoreMaxFrames = 1024;
+
+bbt_status ensure_bscore(bbt_filterset *fs, uint32_t max_frames) {
+    bool fresh = !fs->dbscore.empty() && fs->bscore_frames >= max_frames &&
+                 fs->bscore_fpr.size() == fs->desc.size();
+    for (size_t f = 0; fresh && f < fs->desc.size(); ++f)
+        fresh = fs->bscore_fpr[f] == fs->info[f].fpr;
+    if (fresh)
+        return BBT_OK;
+    HIP_OK(hipDeviceSynchronize()); /* in-flight batches read the old tables */
+    for (double *p : fs->dbscore)
+        if (p)
+            (void)hipFree(p);
+    fs->dbscore.clear();
+    fs->bscore_fpr.clear();
+    const uint32_t F = std::max(max_frames, fs->bscore_frames);
+    const size_t n = (size_t(F) + 1) * (size_t(F) + 2) / 2;
+    for (size_t f = 0; f < fs->desc.size(); ++f) {
+        const double fpr = fs->info[f].fpr;
+        std::vector<double> t(n, 0.0);
+        for (uint32_t fl = 1; fl <= F; ++fl)
+            for (uint32_t c = 0; c <= fl; ++c)
+                t[size_t(fl) * (fl + 1) / 2 + c] = std::log10(binom_sf(int64_t(c), fl, fpr)) * -10;
+        double *d = nullptr;
+        HIP_OK(hipMalloc(&d, n * sizeof(double)));
+        HIP_OK(hipMemcpy(d, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        fs->dbscore.push_back(d);
+        fs->bscore_fpr.push_back(fpr);
+    }
+    fs->bscore_frames = F;
+    fs->dirty = true;
+    return BBT_OK;
+}
+
 } // namespace
 
 /* Everything of one bbt_eval_batch that is host work (checks, tables,
@@ -1161,9 +1227,19 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     const bool need_scores = p->mode == BBT_BESTHIT || p->mode == BBT_SCORES;
     if (need_scores && !out->scores)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: scores output required for this mode");
-    if (need_scores && p->scorer == BBT_BINOMIAL)
-        return fail(BBT_ERR_UNSUPPORTED,
-                    "binomial evalScore (best-hit / with-score modes) is host-only");
+    if (need_scores && p->scorer == BBT_BINOMIAL) {
+        uint32_t kmin = 64;
+        for (const FilterDev &d : fs->desc)
+            kmin = std::min(kmin, d.k);
+        const uint32_t frames = in->max_len >= kmin ? in->max_len - kmin + 1 : 0;
+        if (frames > kBinomScoreMaxFrames)
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "binomial evalScore (best-hit / with-score modes) covers reads of up "
+                        "to 1024 k-mers");
+        bbt_status s = ensure_bscore(fs, frames);
+        if (s != BBT_OK)
+            return s;
+    }
     if (out->summary && (2 * size_t(nf) + 2) * sizeof(unsigned int) > 64 * 1024)
         return fail(BBT_ERR_UNSUPPORTED, "too many filters for device summary counters");
     uint32_t common_k = fs->desc[0].k;
@@ -1210,7 +1286,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.scores = out->scores;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.nf = nf;
-    a.refill_min = fs->tune.refill_min ? fs->tune.refill_min : 8;
+    a.refill_min = fs->tune.refill_min ? fs->tune.refill_min : kDefaultRefillMin;
     a.wp.threshold = p->threshold;
     a.wp.streak = p->streak;
     a.wp.min_len = uint32_t(std::round(p->threshold));

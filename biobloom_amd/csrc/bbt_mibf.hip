@@ -61,6 +61,8 @@ struct MibfDev {
     uint32_t nids;
     uint32_t kmer;            /* 1: k-mer mode, S ntHash values per k-mer */
     uint32_t care[kMaxSeeds]; /* spaced seeds: bit j set = position j is used */
+    uint32_t dc_union;        /* positions that are don't-care in some seed */
+    uint32_t pad;
 };
 
 __device__ __forceinline__ uint32_t mix32(uint64_t x) {
@@ -82,6 +84,27 @@ __device__ __forceinline__ bool bv_rank(const uint64_t *lines, uint64_t p, uint6
     for (uint32_t s = 0; s < slot; ++s)
         r += uint64_t(__popcll(L[1 + s]));
     const uint64_t word = L[1 + slot];
+    const uint32_t b = uint32_t(p & 63);
+    rank = r + uint64_t(__popcll(b ? word & (~uint64_t(0) >> (64 - b)) : 0));
+    return (word >> b) & 1;
+}
+
+/* The same from the whole 64-byte line, loaded as four independent 16-byte
+ * loads with every word selected by constant index (no loop on the slot):
+ * the loads of several lookups can all be in flight together. */
+__device__ __forceinline__ bool bv_rank_line(const uint64_t *lines, uint64_t p, uint64_t &rank) {
+    const uint64_t w = p >> 6;
+    const uint64_t line = w / kWordsPerLine;
+    const uint32_t slot = uint32_t(w - line * kWordsPerLine);
+    const ulonglong2 *q = reinterpret_cast<const ulonglong2 *>(lines + line * 8);
+    const ulonglong2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const uint64_t W[8] = {q0.x, q0.y, q1.x, q1.y, q2.x, q2.y, q3.x, q3.y};
+    uint64_t r = W[0], word = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < kWordsPerLine; ++t) {
+        r += t < slot ? uint64_t(__popcll(W[1 + t])) : 0;
+        word = t == slot ? W[1 + t] : word;
+    }
     const uint32_t b = uint32_t(p & 63);
     rank = r + uint64_t(__popcll(b ? word & (~uint64_t(0) >> (64 - b)) : 0));
     return (word >> b) & 1;
@@ -344,6 +367,10 @@ struct QueryArgs {
 
 constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
 constexpr uint32_t kMaxCand = 64;
+/* up to this many IDs (with the repeat ID 0), per-ID totals are accumulated
+ * directly in LDS, three packed 16-bit pairs per ID, instead of sorting the
+ * frame records */
+constexpr uint32_t kDirectIds = 1024;
 
 /* one distinct ID of one frame, sortable by ID:
  * id << 32 | clean frame << 16 | slots holding it << 8 | unsaturated ones */
@@ -368,14 +395,23 @@ __device__ __forceinline__ bool ranks_before(const uint32_t *x, const uint32_t *
  * S slots (bit + rank from one line, then the ID), and the distinct IDs of
  * each frame are appended to an LDS store that is sorted by ID and reduced
  * per ID at the end of the record. */
-template <uint32_t CAP>
+template <uint32_t CAP, uint32_t SMAX, bool DIRECT>
 __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
-    __shared__ uint64_t rec[CAP];
+    __shared__ uint64_t rec[DIRECT ? 1 : CAP];
     __shared__ uint32_t cand[kMaxCand * kHitWords];
     __shared__ uint32_t ncand_s;
+    /* DIRECT: per ID, (count | nonSat << 16), (total | totalNonSat << 16),
+     * (solid | nonSatFrame << 16) */
+    extern __shared__ uint32_t acc[];
     const uint32_t lane = threadIdx.x;
     const uint32_t S = a.d.S, k = a.d.span;
+    const uint32_t nid1 = a.d.nids + 1;
     unsigned long long bvl = 0, gat = 0;
+    if (DIRECT) {
+        for (uint32_t x = lane; x < 3 * nid1; x += 64)
+            acc[x] = 0;
+        __syncthreads();
+    }
     while (true) {
         unsigned long long it = 0;
         if (lane == 0)
@@ -403,6 +439,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 continue;
             uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
             int lastN = -1;
+            unsigned cprev = 0; /* base codes of the previous 64 bases, one per lane */
             for (uint32_t b0 = 0; b0 < len; b0 += 64) {
                 const uint32_t i = b0 + lane;
                 const unsigned c = i < len ? unsigned(s[i]) : 0u;
@@ -439,50 +476,119 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 lastN = __shfl(ln, 63);
                 Hp = H;
                 Gp = G;
-                /* this lane's frame: its S slots and their IDs */
-                uint32_t raw[kMaxSeeds];
-                uint32_t nh = 0;
-                if (valid) {
-                    for (uint32_t sd = 0; sd < S; ++sd) {
-                        const uint64_t h = frame_hash(a.d, s, uint64_t(pw), sd, wf, wr);
-                        uint64_t rank;
-                        ++bvl;
-                        if (bv_rank(a.d.lines, fastmod(h, a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
-                                    rank)) {
-                            raw[nh++] = a.d.data[rank];
-                            ++gat;
-                        }
+                /* this lane's frame: the S hashes (spaced seeds: the window
+                 * values with the don't-care bases XORed out, each base taken
+                 * from the lane that holds it), all S line loads, then the ID
+                 * gathers */
+                uint64_t hs[SMAX];
+                if (a.d.kmer) {
+                    const uint64_t h0 = wr < wf ? wr : wf;
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        hs[sd] = multi_hash(h0, sd, uint64_t(k) * kMultiSeed);
+                } else {
+                    uint64_t fw[SMAX], rv[SMAX];
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                        fw[sd] = wf;
+                        rv[sd] = wr;
+                    }
+                    for (uint32_t dcm = a.d.dc_union; dcm; dcm &= dcm - 1) {
+                        const uint32_t j = uint32_t(__builtin_ctz(dcm));
+                        const uint32_t back = k - 1 - j; /* base pw + j = i - back */
+                        const unsigned cur = unsigned(__shfl_up(int(code), back));
+                        const unsigned prv =
+                            unsigned(__shfl(int(cprev), (int(lane) - int(back)) & 63));
+                        const unsigned cj = lane >= back ? cur : prv;
+                        const uint64_t tf = rotl64(seed_fwd(cj), back);
+                        const uint64_t tr = rotl64(seed_rc(cj), j);
+#pragma unroll
+                        for (uint32_t sd = 0; sd < SMAX; ++sd)
+                            if (sd < S && !((a.d.care[sd] >> j) & 1)) {
+                                fw[sd] ^= tf;
+                                rv[sd] ^= tr;
+                            }
+                    }
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        hs[sd] = rv[sd] < fw[sd] ? rv[sd] : fw[sd];
+                }
+                cprev = code;
+                uint64_t rk[SMAX];
+                uint32_t present = 0;
+#pragma unroll
+                for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                    rk[sd] = 0;
+                    if (sd < S && valid) {
+                        const bool bit = bv_rank_line(
+                            a.d.lines, fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
+                            rk[sd]);
+                        present |= uint32_t(bit) << sd;
                     }
                 }
+                uint32_t raw[SMAX];
+#pragma unroll
+                for (uint32_t sd = 0; sd < SMAX; ++sd)
+                    raw[sd] = ((present >> sd) & 1) ? a.d.data[rk[sd]] : 0u;
+                const uint32_t nh = uint32_t(__popc(present));
+                bvl += valid ? S : 0;
+                gat += nh;
                 const bool eval = valid && nh + a.allowed_miss >= S;
-                uint32_t myrec = 0;
-                uint64_t out[kMaxSeeds];
-                if (eval) {
-                    bool allsat = nh > 0, anysat = false;
-                    for (uint32_t x = 0; x < nh; ++x) {
+                /* the distinct IDs of the frame, in seed order of first
+                 * occurrence; emit(x, record) for each */
+                bool allsat = nh > 0, anysat = false;
+#pragma unroll
+                for (uint32_t x = 0; x < SMAX; ++x)
+                    if ((present >> x) & 1) {
                         if (raw[x] & kSatBit)
                             anysat = true;
                         else
                             allsat = false;
                     }
-                    satc += allsat ? 1 : 0;
-                    for (uint32_t x = 0; x < nh; ++x) {
+                const auto each_id = [&](auto emit) {
+                    uint32_t n = 0;
+#pragma unroll
+                    for (uint32_t x = 0; x < SMAX; ++x) {
+                        if (!((present >> x) & 1))
+                            continue;
                         const uint32_t id = raw[x] & kIdMask;
                         bool seen = false;
+#pragma unroll
                         for (uint32_t y = 0; y < x; ++y)
-                            if ((raw[y] & kIdMask) == id)
+                            if (((present >> y) & 1) && (raw[y] & kIdMask) == id)
                                 seen = true;
                         if (seen)
                             continue;
                         uint32_t occ = 0, ns = 0;
-                        for (uint32_t y = x; y < nh; ++y)
-                            if ((raw[y] & kIdMask) == id) {
+#pragma unroll
+                        for (uint32_t y = x; y < SMAX; ++y)
+                            if (((present >> y) & 1) && (raw[y] & kIdMask) == id) {
                                 ++occ;
                                 if (!(raw[y] & kSatBit))
                                     ++ns;
                             }
-                        out[myrec++] = make_rec(id, anysat ? 0u : 1u, occ, ns);
+                        emit(n++, make_rec(id, anysat ? 0u : 1u, occ, ns));
                     }
+                    return n;
+                };
+                uint32_t myrec = 0;
+                if (eval) {
+                    satc += allsat ? 1 : 0;
+                    myrec = each_id([](uint32_t, uint64_t) {});
+                }
+                if (DIRECT) {
+                    if (eval)
+                        each_id([&](uint32_t, uint64_t v) {
+                            const uint32_t id = uint32_t(v >> 32);
+                            const uint32_t occ = (uint32_t(v) >> 8) & 0xFF, ns = uint32_t(v) & 0xFF;
+                            if (id < nid1) {
+                                atomicAdd(acc + 3 * id, 1u | ((ns ? 1u : 0u) << 16));
+                                atomicAdd(acc + 3 * id + 1, occ | (ns << 16));
+                                atomicAdd(acc + 3 * id + 2,
+                                          (occ == S ? 1u : 0u) | (((uint32_t(v) >> 16) & 1) << 16));
+                            }
+                        });
+                    continue;
                 }
                 /* append this step's records: exclusive scan of myrec */
                 uint32_t pre = myrec;
@@ -493,16 +599,51 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 }
                 const uint32_t total = __shfl(pre, 63);
                 const uint32_t base = nrec + pre - myrec;
-                for (uint32_t x = 0; x < myrec; ++x)
-                    if (base + x < CAP)
-                        rec[base + x] = out[x];
+                if (eval)
+                    each_id([&](uint32_t x, uint64_t v) {
+                        if (base + x < CAP)
+                            rec[base + x] = v;
+                    });
                 nrec += total;
             }
         }
         for (int o = 32; o > 0; o >>= 1)
             satc += __shfl_xor(satc, o);
         bool over = false;
-        if (nrec > CAP) { /* cannot happen when the host sized CAP by frames x S */
+        const uint32_t row = fc <= a.max_frames ? fc : 0;
+        if (DIRECT) {
+            if (lane == 0)
+                ncand_s = 0;
+            __syncthreads();
+            /* candidates straight from the per-ID totals; each lane clears
+             * the IDs it read for the next record */
+            for (uint32_t id = lane; id < nid1; id += 64) {
+                const uint32_t A = acc[3 * id], Bv = acc[3 * id + 1], Cv = acc[3 * id + 2];
+                if (!A)
+                    continue;
+                acc[3 * id] = 0;
+                acc[3 * id + 1] = 0;
+                acc[3 * id + 2] = 0;
+                const uint32_t cnt = A & 0xFFFF, nsat = A >> 16;
+                const uint32_t mc = (id >= 1 && row)
+                                        ? a.min_count[uint64_t(row) * nid1 + id]
+                                        : 0xFFFFFFFFu;
+                if (cnt >= mc && nsat >= a.min_nonsat) {
+                    const uint32_t slot = atomicAdd(&ncand_s, 1u);
+                    if (slot < kMaxCand) {
+                        uint32_t *c7 = cand + slot * kHitWords;
+                        c7[0] = id;
+                        c7[1] = cnt;
+                        c7[2] = nsat;
+                        c7[3] = Bv & 0xFFFF;
+                        c7[4] = Bv >> 16;
+                        c7[5] = Cv & 0xFFFF;
+                        c7[6] = Cv >> 16;
+                    }
+                }
+            }
+            nrec = 0;
+        } else if (nrec > CAP) { /* cannot happen when the host sized CAP by frames x S */
             over = true;
             nrec = 0;
         }
@@ -530,11 +671,12 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 __syncthreads();
             }
         }
-        if (lane == 0)
-            ncand_s = 0;
-        __syncthreads();
+        if (!DIRECT) {
+            if (lane == 0)
+                ncand_s = 0;
+            __syncthreads();
+        }
         /* per-ID totals: the lane at the head of each run of one ID reduces it */
-        const uint32_t row = fc <= a.max_frames ? fc : 0;
         for (uint32_t x = lane; x < nrec; x += 64) {
             const uint32_t id = uint32_t(rec[x] >> 32);
             if (x > 0 && uint32_t(rec[x - 1] >> 32) == id)
@@ -677,6 +819,10 @@ bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds,
     if (!seeds)
         for (uint32_t i = 0; i < nseeds; ++i)
             mf->d.care[i] = span == 32 ? 0xFFFFFFFFu : (1u << span) - 1;
+    const uint32_t all = span == 32 ? 0xFFFFFFFFu : (1u << span) - 1;
+    mf->d.dc_union = 0;
+    for (uint32_t i = 0; i < nseeds; ++i)
+        mf->d.dc_union |= all & ~mf->d.care[i];
     mf->d.S = nseeds;
     mf->d.span = uint32_t(span);
     int dev = 0;
@@ -1112,12 +1258,20 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     a.queue = mf->dctl + 8;
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 1);
     const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
-    if (recs <= 1024)
-        k_mibf_query<1024><<<blocks, 64, 0, st>>>(a);
+    const bool few = mf->d.S <= 4;
+    if (mf->d.nids + 1 <= kDirectIds) {
+        const size_t shm = size_t(3) * (mf->d.nids + 1) * sizeof(uint32_t);
+        few ? k_mibf_query<1, 4, true><<<blocks, 64, shm, st>>>(a)
+            : k_mibf_query<1, 8, true><<<blocks, 64, shm, st>>>(a);
+    } else if (recs <= 1024)
+        few ? k_mibf_query<1024, 4, false><<<blocks, 64, 0, st>>>(a)
+            : k_mibf_query<1024, 8, false><<<blocks, 64, 0, st>>>(a);
     else if (recs <= 4096)
-        k_mibf_query<4096><<<blocks, 64, 0, st>>>(a);
+        few ? k_mibf_query<4096, 4, false><<<blocks, 64, 0, st>>>(a)
+            : k_mibf_query<4096, 8, false><<<blocks, 64, 0, st>>>(a);
     else
-        k_mibf_query<8192><<<blocks, 64, 0, st>>>(a);
+        few ? k_mibf_query<8192, 4, false><<<blocks, 64, 0, st>>>(a)
+            : k_mibf_query<8192, 8, false><<<blocks, 64, 0, st>>>(a);
     MHIP(hipGetLastError());
     return BBT_OK;
 }

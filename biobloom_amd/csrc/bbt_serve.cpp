@@ -460,8 +460,9 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
                                       std::to_string(Lmax.total) + " bytes)");
     SHIP(hipMalloc(&S->darena, cfg->device_bytes));
     S->arena.reset(cfg->device_bytes);
-    const uint64_t hb = cfg->host_bytes ? cfg->host_bytes
-                                        : align_up(bs, kAlign) * (uint64_t(S->cfg.slots) + 2);
+    const uint64_t hb = align_up(cfg->host_bytes ? cfg->host_bytes
+                                                 : bs * (uint64_t(S->cfg.slots) + 2),
+                                 kAlign);
     if (hb < align_up(bs, kAlign))
         return sfail(BBT_ERR_ARG, "bbt_server_create: host_bytes below one full wave");
     SHIP(hipHostMalloc(reinterpret_cast<void **>(&S->harena), hb));

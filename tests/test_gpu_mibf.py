@@ -278,3 +278,35 @@ def test_odd_paired_batch_rejected(dev):
     out = D.MibfOutputs(2, 4)
     with pytest.raises(B.BbtError, match="odd"):
         mf.query_batch(dr.struct(), B.mibf_params(paired=True, max_hits=4), out.struct())
+
+
+def test_many_ids_sort_path(dev):
+    """More than 1023 IDs: per-ID totals come from sorting the frame records
+    instead of the direct per-ID counters."""
+    D, B = dev
+    rng = random.Random(61)
+    seeds = seeds_for(rng)
+    base = ["".join(rng.choice("ACGT") for _ in range(400)) for _ in range(1100)]
+    recs = [(i + 1, s) for i, s in enumerate(base)]
+    # a few references share a segment so frames see several IDs
+    for i in range(0, 1100, 50):
+        j = (i + 7) % 1100
+        recs[i] = (recs[i][0], recs[i][1][:100] + recs[j][1][100:250] + recs[i][1][250:])
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    assert mf.info().nids == 1100
+    reads = []
+    for _ in range(200):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, 250)
+        reads.append(ref[a:a + 150])
+    reads += ["".join(rng.choice("ACGT") for _ in range(150)) for _ in range(20)]
+    dr = device_reads(D, reads)
+    print("[gpu] mibf 1100 ids", flush=True)
+    want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5, max_hits=8)
+    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8))
+    assert gsat == wsat
+    assert got == want
+    assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+    assert sum(len(r) for r in want) > 100

@@ -139,7 +139,8 @@ def test_modes(dev, multi):
     D, B = dev
     fs, ofs, buf, off = multi
     check_batch(D, B, fs, ofs, buf, off, mode=O.ORDERED)
-    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0),
+                        (O.BINOMIAL, 1e-10)]:
         check_batch(D, B, fs, ofs, buf, off, mode=O.BESTHIT, scorer=scorer, thr=thr)
         check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=scorer, thr=thr)
 
@@ -152,7 +153,8 @@ def test_whole_wave_walks_all_scorers(dev, multi):
         check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr, wave_min=26)
     check_batch(D, B, fs, ofs, buf, off, streak=0, wave_min=26)
     check_batch(D, B, fs, ofs, buf, off, paired=True, wave_min=26)
-    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+    for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0),
+                        (O.BINOMIAL, 1e-10)]:
         check_batch(D, B, fs, ofs, buf, off, mode=O.BESTHIT, scorer=scorer, thr=thr,
                     wave_min=26)
         check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=scorer, thr=thr,
