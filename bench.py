@@ -199,6 +199,10 @@ def bench_mibf(args, torch, D, B):
         "roofline": {"bound": "hbm", "kernel": "k_mibf_query", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_gbs": (traffic["bytes_per_launch"] / (ms / 1e3) / 1e9
+                                     if traffic else None),
+                     "traffic_frac": (traffic["bytes_per_launch"] / (ms / 1e3) / 1e9
+                                      / HBM_PEAK_GBS if traffic else None),
                      "algorithmic_bytes": alg_bytes, "random_gather": gather or None,
                      "frac_of_random_gather": (achieved / gather["hbm_8GiB"]["gbs_at_64B"]
                                                if gather else None)},
@@ -496,6 +500,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
+            # measured line traffic (scaled FETCH_SIZE + WRITE_SIZE of this kernel on
+            # this workload) over this run's kernel time: each 64 B lookup pulls a
+            # 128 B line, so this is the HBM-side figure
+            "traffic_gbs": (traffic["bytes_per_launch"] / (R["walk_ms"] / 1e3) / 1e9
+                            if traffic else None),
+            "traffic_frac": (traffic["bytes_per_launch"] / (R["walk_ms"] / 1e3) / 1e9
+                             / HBM_PEAK_GBS if traffic else None),
             "walk_ms": R["walk_ms"],
             "algorithmic_bytes": alg_bytes,
             "random_gather": gather or None,

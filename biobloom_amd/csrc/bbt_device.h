@@ -205,6 +205,9 @@ struct Walker {
     uint32_t probes, lookups;
     /* set when a BINOMIAL threshold table does not cover the read */
     uint32_t bad;
+    /* -D: low-complexity window starts, bit (dbase + p) of dm (null: no dust) */
+    const uint32_t *dm;
+    uint64_t dbase;
 
     BBT_HD unsigned kk() const { return K ? unsigned(K) : k; }
 
@@ -239,6 +242,8 @@ struct Walker {
         streak_thr = wp.streak;
         state = W_RUNNING;
         bad = 0;
+        dm = nullptr;
+        dbase = 0;
         const unsigned kv = kk();
         /* frame count as the size_t expression seqLen - k + 1 (SeqEval.h:28-35) */
         const uint64_t nframes = uint64_t(length) - uint64_t(kv) + 1;
@@ -292,6 +297,13 @@ struct Walker {
                        : 0;
         }
         return unsigned(word >> ((a & 7) * 8)) & 0xFFu;
+    }
+
+    BBT_HD bool low_complexity(uint32_t pw) const {
+        if (!dm)
+            return false;
+        const uint64_t g = dbase + pw;
+        return (dm[g >> 5] >> (g & 31)) & 1u;
     }
 
     BBT_HD uint64_t kseed() const { return uint64_t(kk()) * kMultiSeed; }
@@ -349,6 +361,13 @@ struct Walker {
                 return false;
             if (pre == PRE_NOPROBE)
                 continue;
+            if (low_complexity(p)) {
+                /* -D: a miss without a contains() call (SeqEval.h:53) */
+                on_result(false);
+                if (state != W_RUNNING)
+                    return false;
+                continue;
+            }
             /* contains(*itr) */
             h0 = rc < fwd ? rc : fwd;
             j = 0;
@@ -511,9 +530,11 @@ struct Walker {
      *     first window that reaches the threshold.
      * Everything else (the first window, gaps after N, the miss that starts a
      * jump-skip, LENGTH, HARMONIC hits) goes through the single-window
-     * steps. */
+     * steps.  dmask marks the windows -D treats as misses without a
+     * contains() call (their hmask bits are clear and lk counts nothing). */
     template <typename LK>
-    BBT_HD void replay(uint32_t b0, uint64_t vmask, uint64_t hmask, const LK &lk) {
+    BBT_HD void replay(uint32_t b0, uint64_t vmask, uint64_t hmask, uint64_t dmask,
+                       const LK &lk) {
         const unsigned kv = kk();
         uint64_t vm = vmask;
         while (vm && state == W_RUNNING) {
@@ -548,14 +569,16 @@ struct Walker {
                     if (!hit && streak < streak_thr) {
                         if (!EXH && anti_thr <= anti + r) {
                             const uint32_t n = anti_thr > anti ? anti_thr - anti : 1u;
-                            probes += n;
+                            const uint64_t first_n =
+                                n >= 64 ? run : run & (((uint64_t(1) << n) - 1) << L);
+                            probes += n - uint32_t(__builtin_popcountll(first_n & dmask));
                             lookups += lk(L, L + n);
                             state = W_REJECT;
                             return;
                         }
                         if (!EXH)
                             anti += r;
-                        probes += r;
+                        probes += r - uint32_t(__builtin_popcountll(run & dmask));
                         lookups += lk(L, L + r);
                         streak = 0;
                         prev = pw + r - 1;
@@ -591,8 +614,10 @@ struct Walker {
                 return;
             if (pre == PRE_NOPROBE)
                 continue;
-            ++probes;
-            lookups += lk(L, L + 1);
+            if (!((dmask >> L) & 1)) { /* a dust window is a miss without a probe */
+                ++probes;
+                lookups += lk(L, L + 1);
+            }
             on_result(((hmask >> L) & 1) != 0);
         }
     }

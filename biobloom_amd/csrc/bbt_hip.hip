@@ -27,6 +27,7 @@
 
 #include "../../include/bbt_hip.h"
 #include "bbt_device.h"
+#include "bbt_dust.h"
 #include "bbt_host.h"
 
 using namespace bbt;
@@ -70,6 +71,7 @@ struct EvalArgs {
     const uint32_t *long_list;  /* reads walked by a whole wave, longest first */
     const uint32_t *short_list; /* reads walked one per lane */
     const unsigned long long *list_counts; /* [0] long reads, [1] short reads */
+    const uint32_t *dust;       /* -D: bit (off[read] + p) = window p is low complexity */
     uint64_t nitems;
     uint32_t nf;
     uint32_t refill_min;
@@ -142,6 +144,8 @@ __device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item,
     read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+    w.dm = a.dust;
+    w.dbase = b;
 }
 
 template <int SC, bool EXH, int K>
@@ -180,6 +184,8 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
     const uint32_t len = uint32_t(a.off[read + 1] - b);
     const uint8_t *s = a.seq + b;
     w.init(s, len, a.filters + f, a.wp);
+    w.dm = a.dust;
+    w.dbase = b;
     w.probes = 0;
     w.lookups = 0;
     const unsigned kv = w.kk();
@@ -225,8 +231,11 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         const uint64_t vmask = __ballot(valid);
         if (!vmask)
             continue;
-        /* windows inside the current jump-skip are consumed without a probe */
-        bool need = valid && !(w.skipping && uint32_t(pw) < w.skipEnd);
+        /* windows inside the current jump-skip are consumed without a probe,
+         * and so are -D's low-complexity windows */
+        const bool low = valid && w.low_complexity(uint32_t(pw));
+        const uint64_t dmask = __ballot(low);
+        bool need = valid && !low && !(w.skipping && uint32_t(pw) < w.skipEnd);
         bool allset = need;
         uint32_t lk = 0;
         for (uint32_t jj = 0; __any(need); ++jj) {
@@ -257,7 +266,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
             const uint32_t lo = x ? uint32_t(__shfl(int(lkp), int(x - 1))) : 0u;
             return hi - lo;
         };
-        w.replay(b0, vmask, hmask, lk_range);
+        w.replay(b0, vmask, hmask, dmask, lk_range);
     }
     if (w.state == W_RUNNING)
         w.finish_end();
@@ -418,6 +427,8 @@ __device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
                          Walker<SC, false, K> &w) {
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+    w.dm = a.dust;
+    w.dbase = b;
     while (w.advance())
         w.consume(*w.addr);
     return w.accepted();
@@ -647,6 +658,10 @@ struct bbt_workspace {
     /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_walk_ms = -1;
+    /* -D: the batch's low-complexity bitmap and the pre-pass's scratch */
+    uint32_t *dmask = nullptr;
+    uint64_t dmask_words = 0;
+    DustScratch *dscratch = nullptr;
 };
 
 static void workspace_release(bbt_workspace *w) {
@@ -662,6 +677,10 @@ static void workspace_release(bbt_workspace *w) {
         (void)hipEventDestroy(w->ev0);
     if (w->ev1)
         (void)hipEventDestroy(w->ev1);
+    if (w->dmask)
+        (void)hipFree(w->dmask);
+    if (w->dscratch)
+        (void)hipFree(w->dscratch);
     *w = bbt_workspace();
 }
 
@@ -1260,6 +1279,23 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         return s;
     if (!W.dctl)
         HIP_OK(hipMalloc(&W.dctl, 128 * sizeof(unsigned long long)));
+    if (p->dust) {
+        if (p->dust_w < 3 || p->dust_w > uint32_t(kDustMaxW) || p->dust_t > 1000000)
+            return fail(BBT_ERR_UNSUPPORTED, "dust window (-W) must be 3..64");
+        const uint64_t words = in->seq_bytes / 32 + 2;
+        if (W.dmask_words < words) {
+            if (W.dmask) {
+                HIP_OK(hipStreamSynchronize(st));
+                HIP_OK(hipFree(W.dmask));
+            }
+            W.dmask = nullptr;
+            W.dmask_words = 0;
+            HIP_OK(hipMalloc(&W.dmask, words * sizeof(uint32_t)));
+            W.dmask_words = words;
+        }
+        if (!W.dscratch)
+            HIP_OK(hipMalloc(&W.dscratch, size_t(kDustThreads) * sizeof(DustScratch)));
+    }
     if (p->path == BBT_PATH_BATCHED && W.lists_cap < in->nreads) {
         /* the lists may still be read by this workspace's previous batch */
         if (W.dlists) {
@@ -1315,6 +1351,19 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     k_zero_u64<<<1, 64, 0, st>>>(W.dctl, 48);
     a.queue = W.dctl + 0;
     a.lqueue = W.dctl + 2;
+    a.dust = nullptr;
+    if (p->dust) {
+        /* -D pre-pass: clear the bitmap, then sdust every read */
+        k_fill_u8<<<1024, kBlock, 0, st>>>(reinterpret_cast<uint8_t *>(W.dmask),
+                                            W.dmask_words * sizeof(uint32_t), 0);
+        k_zero_u64<<<1, 64, 0, st>>>(W.dctl + 100, 2);
+        k_dust<<<kDustThreads / 64, 64, 0, st>>>(
+            a.seq, in->off, in->nreads, int(p->dust_t), int(p->dust_w), W.dscratch, W.dmask,
+            W.dctl + 100, out->counters ? reinterpret_cast<unsigned long long *>(out->counters) + 3
+                                        : W.dctl + 101);
+        HIP_OK(hipGetLastError());
+        a.dust = W.dmask;
+    }
     if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED) {
         const uint32_t gb =
             uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));

@@ -2,8 +2,8 @@
  * biobloomcategorizer -- the reference CLI (BioBloomCategorizer/
  * BioBloomCategorizer.cpp:145-533) over libbbt_hip: the same options,
  * threshold interpretation, checks and dispatch; classification runs on the
- * GPU.  Options of the reference not on this path (-D dust, smart-pair
- * single-file input) are rejected with a message.
+ * GPU, -D dust masking included (a device sdust pre-pass).  Smart-pair
+ * single-file input is not on this path and is rejected with a message.
  */
 #include <getopt.h>
 #include <sys/stat.h>
@@ -185,11 +185,18 @@ int main(int argc, char *argv[]) {
             }
             break;
         case 'D':
-        case 'T':
-        case 'W':
-            cerr << "Error: dust filtering (-D) is not available on this path" << endl;
-            die = true;
+            opt.dust = true;
             break;
+        case 'T': {
+            stringstream convert(optarg);
+            convert >> opt.dust_t;
+            break;
+        }
+        case 'W': {
+            stringstream convert(optarg);
+            convert >> opt.dust_w;
+            break;
+        }
         case '?': die = true; break;
         }
     }

@@ -279,6 +279,9 @@ void BioBloomClassifier::run_batch(std::vector<SeqRecord> &recs, bool paired) {
     p.paired = paired;
     p.inclusive = m_opt.inclusive;
     p.path = m_opt.path;
+    p.dust = m_opt.dust ? 1u : 0u;
+    p.dust_t = m_opt.dust_t;
+    p.dust_w = m_opt.dust_w;
     bbt_eval_out out;
     std::memset(&out, 0, sizeof(out));
     out.hits = b.d_hits;

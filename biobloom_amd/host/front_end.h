@@ -33,6 +33,9 @@ struct Options {
     uint32_t path = BBT_PATH_BATCHED;
     uint64_t batch_reads = 1u << 18; /* reads per device batch */
     bool read_status = false;        /* write <prefix>_readStatus.tsv */
+    bool dust = false;               /* opt::dust (-D) */
+    uint32_t dust_t = 20;            /* opt::dustT (-T) */
+    uint32_t dust_w = 64;            /* opt::dustWindow (-W) */
 };
 
 /* One FASTA/FASTQ record as kseq_read delivers it (Common/kseq.h:177-219). */

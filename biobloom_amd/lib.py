@@ -32,7 +32,8 @@ class EvalParams(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("scorer", C.c_uint32), ("threshold", C.c_double),
                 ("streak", C.c_uint32), ("paired", C.c_uint32),
                 ("inclusive", C.c_uint32), ("path", C.c_uint32), ("flags", C.c_uint32),
-                ("wave_min_len", C.c_uint32)]
+                ("wave_min_len", C.c_uint32), ("dust", C.c_uint32), ("dust_t", C.c_uint32),
+                ("dust_w", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -306,9 +307,10 @@ def measure_gather(nbytes, rounds=64, stream=0):
 
 
 def params(mode=STD, scorer=SIMPLE, threshold=0.15, streak=3, paired=False,
-           inclusive=False, path=PATH_BATCHED, flags=0, wave_min_len=0):
+           inclusive=False, path=PATH_BATCHED, flags=0, wave_min_len=0, dust=False,
+           dust_t=20, dust_w=64):
     return EvalParams(mode, scorer, threshold, streak, int(paired), int(inclusive), path,
-                      flags, wave_min_len)
+                      flags, wave_min_len, int(dust), dust_t, dust_w, 0)
 
 
 class Mibf:

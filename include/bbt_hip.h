@@ -115,6 +115,11 @@ typedef struct {
     uint32_t wave_min_len; /* batched path: reads at least this long are walked
                               by a whole wavefront, 64 windows per step
                               (0 = library default) */
+    uint32_t dust;      /* -D: windows starting in an sdust low-complexity interval
+                           are misses without a contains() call (SeqEval.h:496) */
+    uint32_t dust_t;    /* -T, sdust threshold [20] */
+    uint32_t dust_w;    /* -W, sdust window [64], at most 64 here */
+    uint32_t reserved;
 } bbt_eval_params;
 
 typedef struct {

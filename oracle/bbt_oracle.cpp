@@ -170,12 +170,52 @@ inline bool bf_contains(const orc_filter &f, const uint64_t *hv, uint64_t &looku
 struct Counters {
     uint64_t probes = 0;
     uint64_t lookups = 0;
+    /* -D: the read's low-complexity intervals and the in-order query cursor
+     * of SDust::isLowComp (Common/SDust.hpp:38-48) */
+    const std::vector<uint32_t> *dust_start = nullptr, *dust_finish = nullptr;
+    size_t dust_cur = 0;
+
+    bool low_complexity(uint64_t pos) {
+        const std::vector<uint32_t> &st = *dust_start, &fi = *dust_finish;
+        while (dust_cur < st.size() && pos >= fi[dust_cur])
+            ++dust_cur;
+        return dust_cur < st.size() && pos >= st[dust_cur] && pos < fi[dust_cur];
+    }
 };
 
+/* `!(sduster && sduster->isLowComp(pos)) && filter.contains(*itr)`
+ * (SeqEval.h:53 and the same test in every scorer): a window starting in a
+ * low-complexity interval is a miss without a contains() call. */
 inline bool probe(const orc_filter &f, const NtIter &itr, Counters &c) {
+    if (c.dust_start && c.low_complexity(itr.pos()))
+        return false;
     ++c.probes;
     return bf_contains(f, itr.hashes(), c.lookups);
 }
+
+/* With -D, evalRead / evalScore build an SDust of the read per call
+ * (SeqEval.h:496-498, 537-539): the intervals of sdust(read, T, W). */
+struct DustScope {
+    Counters &c;
+    std::vector<uint32_t> st, fi;
+    DustScope(Counters &cc, const orc_eval_params &p, const char *s, uint64_t len) : c(cc) {
+        if (!p.dust)
+            return;
+        st.resize(len + 1);
+        fi.resize(len + 1);
+        const int64_t n = orc_sdust(s, len, int(p.dust_t), int(p.dust_w), st.data(), fi.data(),
+                                    len + 1);
+        st.resize(size_t(n));
+        fi.resize(size_t(n));
+        c.dust_start = &st;
+        c.dust_finish = &fi;
+        c.dust_cur = 0;
+    }
+    ~DustScope() {
+        c.dust_start = nullptr;
+        c.dust_finish = nullptr;
+    }
+};
 
 /* ------------------------------------------------------------------ */
 /* SeqEval scorers (Common/SeqEval.h).  `n` below is the size_t expression
@@ -393,6 +433,7 @@ unsigned eval_min_match_len_score(const orc_filter &f, const char *s,
 /* evalRead dispatcher (:493-529); -D (dust) is not part of this path. */
 bool eval_read(const orc_filter &f, const char *s, uint64_t len,
                const orc_eval_params &p, Counters &c) {
+    DustScope dust(c, p, s, len);
     switch (p.scorer) {
     case ORC_LENGTH:
         return eval_min_match_len(f, s, len, unsigned(std::round(p.threshold)), c);
@@ -408,6 +449,7 @@ bool eval_read(const orc_filter &f, const char *s, uint64_t len,
 /* evalScore dispatcher (:534-564). */
 double eval_score(const orc_filter &f, const char *s, uint64_t len,
                   const orc_eval_params &p, Counters &c) {
+    DustScope dust(c, p, s, len);
     switch (p.scorer) {
     case ORC_LENGTH:
         return double(eval_min_match_len_score(f, s, len, c));

@@ -17,6 +17,8 @@
  *   ResultsManager                              ResultsManager.hpp:40-237
  *   filter sizing                               Common/BloomFilterInfo.h:57-76
  *   binomial thresholds                         Common/SeqEval.h:199-216
+ *   dust masking (-D)                           Common/sdust.c, Common/SDust.hpp,
+ *                                               SeqEval.h:496-529
  *
  * Parity status of each function is recorded beside it in bbt_oracle.cpp.
  */
@@ -83,6 +85,10 @@ typedef struct {
     uint32_t scorer;   /* ORC_SIMPLE .. */
     uint32_t streak;   /* opt::streakThreshold */
     double threshold;  /* opt::score after CLI interpretation */
+    uint32_t dust;     /* opt::dust (-D) */
+    uint32_t dust_t;   /* opt::dustT (-T) */
+    uint32_t dust_w;   /* opt::dustWindow (-W) */
+    uint32_t pad;
 } orc_eval_params;
 
 int orc_eval_read(const orc_filter *f, const char *seq, uint64_t len,
@@ -110,6 +116,12 @@ int orc_classify(const orc_filter *filters, uint32_t nf, const char *seq,
 uint64_t orc_summary(const char *const *filter_ids, uint32_t nf,
                      const uint8_t *hits, uint64_t nreads, int paired,
                      int inclusive, char *out, uint64_t cap);
+
+/* ---- dust (bbt_dust.cpp) ------------------------------------------------ */
+/* sdust(seq, len, T, W): merged low-complexity base intervals [start, finish);
+ * returns their number, or -1 if cap is too small. */
+int64_t orc_sdust(const char *seq, uint64_t len, int T, int W, uint32_t *starts,
+                  uint32_t *finishes, uint64_t cap);
 
 /* ---- multi-index Bloom filter (bbt_mibf.cpp) ------------------------- */
 /* seeds: nseeds concatenated '0'/'1' strings of length span (spaced-seed

@@ -11,6 +11,9 @@
 using namespace bbt;
 
 namespace {
+/* -D bitmap of the batch (bit off[read] + p), or null */
+const uint32_t *g_dust = nullptr;
+
 template <int SC, bool EXH>
 void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t nreads,
          const WalkParams &wp, uint8_t *hits, double *scores, uint64_t *ctr) {
@@ -19,6 +22,8 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
         w.probes = 0;
         w.lookups = 0;
         w.init(seq + off[r], uint32_t(off[r + 1] - off[r]), &f, wp);
+        w.dm = g_dust;
+        w.dbase = off[r];
         while (w.advance())
             w.consume(*w.addr);
         if (EXH)
@@ -41,6 +46,8 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
         const uint8_t *s = seq + off[r];
         const uint32_t len = uint32_t(off[r + 1] - off[r]);
         w.init(s, len, &f, wp);
+        w.dm = g_dust;
+        w.dbase = off[r];
         w.probes = 0;
         w.lookups = 0;
         const unsigned kv = f.k;
@@ -52,6 +59,7 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
             bool valid[64];
             uint32_t lk[64];
             bool allset[64];
+            bool low[64];
             for (int L = 0; L < 64; ++L) {
                 const uint32_t i = b0 + L;
                 const unsigned c = i < len ? s[i] : 0;
@@ -71,7 +79,8 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
                 const uint64_t wf = pfx_win_fwd(H[L], Hb, i);
                 const uint64_t wr = pfx_win_rc(G[L], Gb, uint32_t(pw));
                 h0[L] = wr < wf ? wr : wf;
-                bool need = valid[L] && !(w.skipping && uint32_t(pw) < w.skipEnd);
+                low[L] = valid[L] && w.low_complexity(uint32_t(pw));
+                bool need = valid[L] && !low[L] && !(w.skipping && uint32_t(pw) < w.skipEnd);
                 allset[L] = need;
                 lk[L] = 0;
                 for (uint32_t j = 0; need; ++j) {
@@ -98,15 +107,16 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
                 Hp[L] = H[L];
                 Gp[L] = G[L];
             }
-            uint64_t vmask = 0, hmask = 0;
+            uint64_t vmask = 0, hmask = 0, dmask = 0;
             uint32_t lkp[65];
             lkp[0] = 0;
             for (int L = 0; L < 64; ++L) {
                 vmask |= uint64_t(valid[L]) << L;
                 hmask |= uint64_t(allset[L]) << L;
+                dmask |= uint64_t(low[L]) << L;
                 lkp[L + 1] = lkp[L] + lk[L];
             }
-            w.replay(b0, vmask, hmask,
+            w.replay(b0, vmask, hmask, dmask,
                      [&](uint32_t x, uint32_t y) -> uint32_t { return lkp[y] - lkp[x]; });
         }
         if (w.state == W_RUNNING)
@@ -127,7 +137,8 @@ int walker_host_wave_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t 
                           const uint32_t *binom, uint32_t binom_len, const uint8_t *seq,
                           const uint64_t *off, uint64_t nreads, uint32_t scorer, int exh,
                           double threshold, uint32_t streak, uint8_t *hits, double *scores,
-                          uint64_t *ctr) {
+                          uint64_t *ctr, const uint32_t *dust) {
+    g_dust = dust;
     FilterDev f;
     std::memset(&f, 0, sizeof(f));
     f.bits = bits;
@@ -169,7 +180,8 @@ int walker_host_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
                      const uint32_t *binom, uint32_t binom_len, const uint8_t *seq,
                      const uint64_t *off, uint64_t nreads, uint32_t scorer, int exh,
                      double threshold, uint32_t streak, uint8_t *hits, double *scores,
-                     uint64_t *ctr) {
+                     uint64_t *ctr, const uint32_t *dust) {
+    g_dust = dust;
     FilterDev f;
     std::memset(&f, 0, sizeof(f));
     f.bits = bits;

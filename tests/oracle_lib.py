@@ -24,7 +24,8 @@ class OrcFilter(C.Structure):
 
 class OrcParams(C.Structure):
     _fields_ = [("scorer", C.c_uint32), ("streak", C.c_uint32),
-                ("threshold", C.c_double)]
+                ("threshold", C.c_double), ("dust", C.c_uint32), ("dust_t", C.c_uint32),
+                ("dust_w", C.c_uint32), ("pad", C.c_uint32)]
 
 
 _lib = None
@@ -171,8 +172,8 @@ class Filter:
         return OrcFilter(self.bits.ctypes.data, self.m, self.h, self.k, self.fpr)
 
 
-def params(scorer=SIMPLE, threshold=0.15, streak=3):
-    return OrcParams(scorer, streak, threshold)
+def params(scorer=SIMPLE, threshold=0.15, streak=3, dust=False, dust_t=20, dust_w=64):
+    return OrcParams(scorer, streak, threshold, int(dust), dust_t, dust_w, 0)
 
 
 def eval_read(filt, seq, p):
@@ -236,6 +237,23 @@ def summary(filter_ids, hits, paired=False, inclusive=False):
     lib().orc_summary(ids, nf, h.ctypes.data, nreads, int(paired), int(inclusive), buf,
                       n + 1)
     return buf.value.decode()
+
+
+# ---------------------------------------------------------------- dust
+
+def sdust(seq, T=20, W=64):
+    """Merged low-complexity intervals [(start, finish)] of the oracle's sdust."""
+    L = lib()
+    L.orc_sdust.restype = C.c_int64
+    L.orc_sdust.argtypes = [C.c_char_p, C.c_uint64, C.c_int, C.c_int, C.c_void_p,
+                            C.c_void_p, C.c_uint64]
+    b = seq.encode() if isinstance(seq, str) else bytes(seq)
+    cap = len(b) + 1
+    st = np.zeros(cap, np.uint32)
+    fi = np.zeros(cap, np.uint32)
+    n = L.orc_sdust(b, len(b), T, W, st.ctypes.data, fi.ctypes.data, cap)
+    assert n >= 0
+    return list(zip(st[:n].tolist(), fi[:n].tolist()))
 
 
 # ---------------------------------------------------------------- MIBF

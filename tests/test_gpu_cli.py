@@ -130,3 +130,28 @@ def test_missing_filter_file_exits_nonzero(data, tmp_path):
     r = subprocess.run([CAT, "-p", str(tmp_path / "x"), "-f", str(tmp_path / "nope.bf"),
                         str(d / "all.fq")], capture_output=True, timeout=60)
     assert r.returncode != 0 and b"cannot be opened" in r.stderr
+
+
+def test_dust_option(data, tmp_path):
+    """-D -T -W reach the device sdust pre-pass: the status of every read
+    equals the oracle's dust walks on reads with low-complexity stretches."""
+    d, reads, filters = data
+    rng = np.random.default_rng(31)
+    dreads = []
+    for j, (n, s) in enumerate(reads[:600]):
+        a = int(rng.integers(0, len(s)))
+        dreads.append((n, s[:a] + ["A", "CA", "GTT"][j % 3] * int(rng.integers(10, 40)) + s[a:]))
+    write_fastq(tmp_path / "dust.fq", dreads)
+    buf, off = O.pack_reads([s for _, s in dreads])
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    for T, W in [(20, 64), (10, 40)]:
+        print(f"[gpu] biobloomcategorizer -D -T {T} -W {W}", flush=True)
+        bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+        subprocess.run([CAT, "-p", str(tmp_path / "dust"), "-f", bfs, "-D", "-T", str(T),
+                        "-W", str(W), "--status", str(tmp_path / "dust.fq")],
+                       check=True, timeout=120)
+        hits, _, cls, _ = O.classify(filters, buf, off, p=O.params(dust=True, dust_t=T,
+                                                                    dust_w=W))
+        status = open(tmp_path / "dust_readStatus.tsv").read().splitlines()
+        assert status == [f"{n}\t{names[c]}" for (n, _), c in zip(dreads, cls)]
+        assert open(tmp_path / "dust_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)

@@ -68,9 +68,10 @@ PATHS = [0, 1]
 
 
 def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
-                scorer=0, thr=0.15, streak=3, order=None, wave_min=0):
+                scorer=0, thr=0.15, streak=3, order=None, wave_min=0, dust=None):
     dr = D.DeviceReads(buf, off, order=order)
-    p_or = O.params(scorer, thr, streak)
+    dt, dw = dust if dust else (20, 64)
+    p_or = O.params(scorer, thr, streak, bool(dust), dt, dw)
     want_scores = mode in (O.BESTHIT, O.SCORES)
     oh, osc, ocls, octr = O.classify(ofs, buf, off, paired, inclusive, mode, p_or,
                                      want_scores=want_scores)
@@ -79,7 +80,7 @@ def check_batch(D, B, fs, ofs, buf, off, paired=False, inclusive=False, mode=0,
               f"paired={paired} incl={inclusive} path={path} wave_min={wave_min} "
               f"n={len(off)-1}", flush=True)
         p = B.params(mode, scorer, thr, streak, paired, inclusive, path,
-                     wave_min_len=wave_min)
+                     wave_min_len=wave_min, dust=bool(dust), dust_t=dt, dust_w=dw)
         res = D.classify(fs, dr, p)
         assert np.array_equal(res["hits"], oh), "hits differ"
         assert np.array_equal(res["cls"], ocls), "classes differ"
@@ -251,3 +252,32 @@ def test_autotune_keeps_results(dev):
     res = D.classify(fs, dr, B.params())
     assert np.array_equal(res["hits"], oh) and np.array_equal(res["cls"], ocls)
     assert int(res["counters"][1]) == int(octr[1])
+
+
+def test_dust(dev, multi):
+    """-D: the sdust pre-pass on the device and the walks that skip its
+    windows, against the oracle, on reads with planted low-complexity runs."""
+    D, B = dev
+    fs, ofs, buf, off = multi
+    rng = np.random.default_rng(12)
+    reads = [bytes(buf[int(off[i]):int(off[i + 1])]) for i in range(len(off) - 1)]
+    units = [b"A", b"AC", b"CAG", b"ACGT", b"T"]
+    planted = []
+    for i, r in enumerate(reads[:400]):
+        a = int(rng.integers(0, max(1, len(r))))
+        u = units[i % len(units)]
+        planted.append(r[:a] + u * int(rng.integers(5, 40)) + r[a:])
+    planted += [b"A" * 300, b"AC" * 150 + b"N" + b"G" * 60, b"ACGTTGCA" + b"N" + b"A" * 40]
+    if len(planted) % 2:
+        planted.append(b"CA" * 40)
+    pbuf, poff = O.pack_reads(planted)
+    for dust in [(20, 64), (10, 32)]:
+        check_batch(D, B, fs, ofs, pbuf, poff, dust=dust)
+        check_batch(D, B, fs, ofs, pbuf, poff, dust=dust, wave_min=26)
+    check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), paired=True)
+    check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), mode=O.ORDERED)
+    for scorer, thr in [(O.HARMONIC, 0.15), (O.LENGTH, 30.0), (O.BINOMIAL, 1e-10)]:
+        check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), scorer=scorer, thr=thr)
+    check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), mode=O.BESTHIT)
+    check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), mode=O.SCORES, scorer=O.LENGTH,
+                thr=30.0)

@@ -139,3 +139,39 @@ def test_score_of_read_one_shorter_than_k_is_nan():
     assert math.isnan(sc) and probes == 0
     sc, _, _ = O.eval_score(full_filter(25, 3), "A" * 10, O.params(O.SIMPLE))
     assert sc == 0.0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_dust_walks_match_model(seed):
+    """-D: a window starting in an sdust interval is a miss without a
+    contains() call (SeqEval.h:53, :78 and the same test in every scorer)."""
+    rng = random.Random(300 + seed)
+    checked = 0
+    for _ in range(120):
+        f, s = make_case(rng)
+        if rng.random() < 0.7:  # plant low-complexity stretches
+            a = rng.randrange(0, max(1, len(s) - 1))
+            s = s[:a] + rng.choice(["A", "AC", "GTT"]) * rng.randint(8, 25) + s[a:]
+        T, W = rng.choice([(20, 64), (10, 32), (30, 64)])
+        low = {p for a, b in O.sdust(s, T, W) for p in range(a, b)}
+        pos, hits, looks = windows_and_hits(f, s)
+        masked = [h and p not in low for p, h in zip(pos, hits)]
+        t = rng.choice([0.05, 0.15, 0.5])
+        streak = rng.choice([0, 1, 3])
+        for harmonic, scorer in [(False, O.SIMPLE), (True, O.HARMONIC)]:
+            ok, probes, lookups = O.eval_read(f, s, O.params(scorer, t, streak, True, T, W))
+            mok, w = M.simple(len(s), f.k, pos, masked, t, streak, harmonic)
+            called = [i for i in w.probes if pos[i] not in low]
+            assert ok == mok
+            assert probes == len(called)
+            assert lookups == sum(looks[i] for i in called)
+        mlen = rng.choice([f.k, f.k + 3, 40])
+        ok, probes, _ = O.eval_read(f, s, O.params(O.LENGTH, float(mlen), 3, True, T, W))
+        mok, w = M.min_match_len(len(s), f.k, pos, masked, mlen)
+        assert (ok, probes) == (mok, len([i for i in w.probes if pos[i] not in low]))
+        sc, probes, _ = O.eval_score(f, s, O.params(O.SIMPLE, 0.15, streak, True, T, W))
+        msc, _, w = M.tiling_score(len(s), f.k, pos, masked, streak, "simple")
+        assert (math.isnan(sc) and math.isnan(msc)) or sc == msc
+        assert probes == len([i for i in w.probes if pos[i] not in low])
+        checked += bool(low & set(pos))
+    assert checked > 50

@@ -36,7 +36,7 @@ def W():
     L.walker_host_eval.restype = C.c_int
     L.walker_host_eval.argtypes = [p, C.c_uint64, C.c_uint32, C.c_uint32, p, C.c_uint32,
                                    p, p, C.c_uint64, C.c_uint32, C.c_int, C.c_double,
-                                   C.c_uint32, p, p, p]
+                                   C.c_uint32, p, p, p, p]
     L.walker_host_wave_eval.restype = C.c_int
     L.walker_host_wave_eval.argtypes = L.walker_host_eval.argtypes
     L.walker_host_fastmod.argtypes = [C.c_uint64, p, C.c_uint64, p]
@@ -50,7 +50,7 @@ def padded(buf):
     return out
 
 
-def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None, wave=False):
+def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None, wave=False, dust=None):
     buf, off = O.pack_reads(reads)
     buf = padded(buf)
     n = len(reads)
@@ -62,7 +62,8 @@ def run_walker(W, f, reads, scorer, exh, thr, streak, binom=None, wave=False):
     rc = fn(f.bits.ctypes.data, f.m, f.h, f.k, b.ctypes.data,
                             len(b) if binom is not None else 0, buf.ctypes.data,
                             off.ctypes.data, n, scorer, int(exh), thr, streak,
-                            hits.ctypes.data, scores.ctypes.data, ctr.ctypes.data)
+                            hits.ctypes.data, scores.ctypes.data, ctr.ctypes.data,
+                            None if dust is None else dust.ctypes.data)
     assert rc == 0
     return hits, scores, ctr
 
@@ -154,3 +155,48 @@ def test_walker_matches_oracle(W, k, h, m, wave):
             exp = np.array(ws, np.float64).view(np.uint64).tolist()
             assert got == exp
             assert int(ctr[0]) == probes
+
+
+def dust_bitmap(reads, T, W):
+    """bit off[read] + p set for the oracle's sdust intervals (clipped to the read)"""
+    _, off = O.pack_reads(reads)
+    words = np.zeros(int(off[-1]) // 32 + 2, np.uint32)
+    for r, s in enumerate(reads):
+        for a, b in O.sdust(s, T, W):
+            for p in range(a, min(b, len(s))):
+                g = int(off[r]) + p
+                words[g >> 5] |= np.uint32(1 << (g & 31))
+    return words
+
+
+@pytest.mark.parametrize("wave", [False, True])
+def test_walker_dust_matches_oracle(W, wave):
+    """-D: the device walk (lane and whole-wave) with the sdust bitmap equals
+    the oracle's dust walks, every scorer, counts included."""
+    rng = random.Random(808 + wave)
+    f, reads = make_filter_and_reads(rng, 25, 3, 1 << 15)
+    planted = []
+    for r in reads:
+        a = rng.randrange(0, max(1, len(r)))
+        planted.append(r[:a] + rng.choice(["A", "CA", "GGT", "ACGT"]) * rng.randint(6, 40) + r[a:])
+    reads = reads + planted
+    if wave:
+        reads += [f.genome[:600] + "AC" * 100 + f.genome[600:1100]]
+    for T, Wd in [(20, 64), (10, 32)]:
+        bm = dust_bitmap(reads, T, Wd)
+        for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+            for streak in [3, 0]:
+                p = O.params(scorer, thr, streak, True, T, Wd)
+                hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, streak, None, wave, bm)
+                want, probes, lookups = [], 0, 0
+                for r in reads:
+                    ok, pr, lk = O.eval_read(f, r, p)
+                    want.append(int(ok))
+                    probes += pr
+                    lookups += lk
+                assert hits.tolist() == want
+                assert (int(ctr[0]), int(ctr[1])) == (probes, lookups)
+                _, scores, ctr = run_walker(W, f, reads, scorer, True, thr, streak, None, wave, bm)
+                ws = [O.eval_score(f, r, p)[0] for r in reads]
+                got = scores.view(np.uint64).tolist()
+                assert got == np.array(ws, np.float64).view(np.uint64).tolist()
