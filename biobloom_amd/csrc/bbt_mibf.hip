@@ -368,9 +368,10 @@ struct QueryArgs {
 constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
 constexpr uint32_t kMaxCand = 64;
 /* up to this many IDs (with the repeat ID 0), per-ID totals are accumulated
- * directly in LDS, three packed 16-bit pairs per ID, instead of sorting the
- * frame records */
+ * directly in LDS, six 32-bit counters per ID, instead of sorting the frame
+ * records; such a filter takes records of any length */
 constexpr uint32_t kDirectIds = 1024;
+constexpr uint32_t kDirectWords = 6; /* count nonSat total totalNonSat solid nonSatFrame */
 
 /* one distinct ID of one frame, sortable by ID:
  * id << 32 | clean frame << 16 | slots holding it << 8 | unsaturated ones */
@@ -400,15 +401,14 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
     __shared__ uint64_t rec[DIRECT ? 1 : CAP];
     __shared__ uint32_t cand[kMaxCand * kHitWords];
     __shared__ uint32_t ncand_s;
-    /* DIRECT: per ID, (count | nonSat << 16), (total | totalNonSat << 16),
-     * (solid | nonSatFrame << 16) */
+    /* DIRECT: per ID, count, nonSat, total, totalNonSat, solid, nonSatFrame */
     extern __shared__ uint32_t acc[];
     const uint32_t lane = threadIdx.x;
     const uint32_t S = a.d.S, k = a.d.span;
     const uint32_t nid1 = a.d.nids + 1;
     unsigned long long bvl = 0, gat = 0;
     if (DIRECT) {
-        for (uint32_t x = lane; x < 3 * nid1; x += 64)
+        for (uint32_t x = lane; x < kDirectWords * nid1; x += 64)
             acc[x] = 0;
         __syncthreads();
     }
@@ -582,10 +582,17 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                             const uint32_t id = uint32_t(v >> 32);
                             const uint32_t occ = (uint32_t(v) >> 8) & 0xFF, ns = uint32_t(v) & 0xFF;
                             if (id < nid1) {
-                                atomicAdd(acc + 3 * id, 1u | ((ns ? 1u : 0u) << 16));
-                                atomicAdd(acc + 3 * id + 1, occ | (ns << 16));
-                                atomicAdd(acc + 3 * id + 2,
-                                          (occ == S ? 1u : 0u) | (((uint32_t(v) >> 16) & 1) << 16));
+                                uint32_t *A = acc + kDirectWords * id;
+                                atomicAdd(A, 1u);
+                                atomicAdd(A + 2, occ);
+                                if (ns) {
+                                    atomicAdd(A + 1, 1u);
+                                    atomicAdd(A + 3, ns);
+                                }
+                                if (occ == S)
+                                    atomicAdd(A + 4, 1u);
+                                if ((uint32_t(v) >> 16) & 1)
+                                    atomicAdd(A + 5, 1u);
                             }
                         });
                     continue;
@@ -618,13 +625,14 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
             /* candidates straight from the per-ID totals; each lane clears
              * the IDs it read for the next record */
             for (uint32_t id = lane; id < nid1; id += 64) {
-                const uint32_t A = acc[3 * id], Bv = acc[3 * id + 1], Cv = acc[3 * id + 2];
-                if (!A)
+                uint32_t *A = acc + kDirectWords * id;
+                const uint32_t cnt = A[0];
+                if (!cnt)
                     continue;
-                acc[3 * id] = 0;
-                acc[3 * id + 1] = 0;
-                acc[3 * id + 2] = 0;
-                const uint32_t cnt = A & 0xFFFF, nsat = A >> 16;
+                const uint32_t nsat = A[1], tot = A[2], tns = A[3], solid = A[4], nsf = A[5];
+#pragma unroll
+                for (uint32_t x = 0; x < kDirectWords; ++x)
+                    A[x] = 0;
                 const uint32_t mc = (id >= 1 && row)
                                         ? a.min_count[uint64_t(row) * nid1 + id]
                                         : 0xFFFFFFFFu;
@@ -635,10 +643,10 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                         c7[0] = id;
                         c7[1] = cnt;
                         c7[2] = nsat;
-                        c7[3] = Bv & 0xFFFF;
-                        c7[4] = Bv >> 16;
-                        c7[5] = Cv & 0xFFFF;
-                        c7[6] = Cv >> 16;
+                        c7[3] = tot;
+                        c7[4] = tns;
+                        c7[5] = solid;
+                        c7[6] = nsf;
                     }
                 }
             }
@@ -1213,10 +1221,15 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     const uint32_t k = mf->d.span;
     const uint32_t frames = in->max_len >= k ? in->max_len - k + 1 : 0;
     const uint64_t recs = uint64_t(frames) * mf->d.S * (paired ? 2 : 1);
-    if (recs > 8192)
+    const bool direct = mf->d.nids + 1 <= kDirectIds;
+    if (!direct && recs > 8192)
         return mfail(BBT_ERR_UNSUPPORTED,
-                     "bbt_mibf_query_batch: frames x hashes of the longest record exceed 8192 "
-                     "(reads up to ~2.7 kbp with 3 seeds)");
+                     "bbt_mibf_query_batch: with more than 1023 IDs, frames x hashes of the "
+                     "longest record may not exceed 8192 (reads up to ~2.7 kbp with 3 seeds)");
+    if (uint64_t(frames + 1) * (mf->d.nids + 1) * 4 > (uint64_t(1) << 30))
+        return mfail(BBT_ERR_UNSUPPORTED,
+                     "bbt_mibf_query_batch: minimum-count table above 1 GiB (reads too long "
+                     "for this many IDs)");
     const uint32_t miss = p->allowed_miss;
     if (!mf->dmin || mf->min_score != p->score || mf->min_miss != miss ||
         mf->min_frames < frames) {
@@ -1259,8 +1272,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 1);
     const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
     const bool few = mf->d.S <= 4;
-    if (mf->d.nids + 1 <= kDirectIds) {
-        const size_t shm = size_t(3) * (mf->d.nids + 1) * sizeof(uint32_t);
+    if (direct) {
+        const size_t shm = size_t(kDirectWords) * (mf->d.nids + 1) * sizeof(uint32_t);
         few ? k_mibf_query<1, 4, true><<<blocks, 64, shm, st>>>(a)
             : k_mibf_query<1, 8, true><<<blocks, 64, shm, st>>>(a);
     } else if (recs <= 1024)

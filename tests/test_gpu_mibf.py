@@ -310,3 +310,35 @@ def test_many_ids_sort_path(dev):
     assert got == want
     assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
     assert sum(len(r) for r in want) > 100
+
+
+def test_query_reads_beyond_the_record_store(dev):
+    """With at most 1023 IDs the per-ID totals live in LDS and records of any
+    length are evaluated: reads of 5-20 kbp, single and paired."""
+    D, B = dev
+    rng = random.Random(71)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=4, L=30000)
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = []
+    for L in [5000, 9000, 20000, 150, 20000, 7000]:
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - L)
+        r = list(ref[a:a + L])
+        for i in range(len(r)):
+            if rng.random() < 0.03:
+                r[i] = rng.choice("ACGTN")
+        reads.append("".join(r))
+    dr = device_reads(D, reads)
+    print("[gpu] mibf 5-20 kbp reads", flush=True)
+    for paired in [False, True]:
+        want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5,
+                                    max_hits=8, paired=paired)
+        got, gsat, gctr, _ = D.mibf_classify(
+            mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8, paired=paired))
+        assert gsat == wsat
+        assert got == want
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[3] == 0
+    assert all(row for row in want)
