@@ -540,14 +540,16 @@ __global__ __launch_bounds__(kBlock) void k_classify(uint8_t *hits, const double
 }
 
 /* BloomFilterGenerator::loadFilter (BloomFilterGenerator.h:143-188): insert
- * every N-free k-mer.  Thread t owns window starts [t*chunk, (t+1)*chunk). */
-template <int K>
+ * every N-free k-mer.  Thread t owns window starts [t*chunk, (t+1)*chunk).
+ * SUB: loadFilterSubtract (BloomFilterGenerator.h:217-284), a k-mer that the
+ * subtract filter contains is counted as removed and not inserted. */
+template <int K, bool SUB>
 __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t nbytes,
-                                                   FilterDev f, uint32_t chunk,
+                                                   FilterDev f, FilterDev sub, uint32_t chunk,
                                                    unsigned long long *counts) {
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint64_t p0 = t * chunk;
-    uint32_t nnew = 0, nold = 0;
+    uint32_t nnew = 0, nold = 0, nsub = 0;
     if (p0 < nbytes) {
         const unsigned kv = K ? unsigned(K) : f.k;
         const uint64_t last = std::min<uint64_t>(p0 + chunk + kv - 1, nbytes);
@@ -575,6 +577,19 @@ __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t 
             if (run < kv)
                 continue;
             const uint64_t h0 = rc < fwd ? rc : fwd;
+            if (SUB) {
+                /* BloomFilter::contains on the subtract filter (same h and k) */
+                bool in = true;
+                for (uint32_t j = 0; j < sub.h && in; ++j) {
+                    const uint64_t pos = fastmod(multi_hash(h0, j, uint64_t(kv) * kMultiSeed),
+                                                 sub.m, sub.magic, sub.sh1, sub.sh2);
+                    in = (sub.bits[pos >> 3] >> (pos & 7)) & 1u;
+                }
+                if (in) {
+                    ++nsub;
+                    continue;
+                }
+            }
             bool found = true;
             for (uint32_t j = 0; j < f.h; ++j) {
                 const uint64_t hv = multi_hash(h0, j, uint64_t(kv) * kMultiSeed);
@@ -589,14 +604,18 @@ __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t 
                 ++nnew;
         }
     }
-    unsigned long long a = nnew, b = nold;
+    unsigned long long a = nnew, b = nold, c = nsub;
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o);
         b += __shfl_xor(b, o);
+        if (SUB)
+            c += __shfl_xor(c, o);
     }
     if (lane_id() == 0 && counts) {
         atomicAdd(counts + 0, a);
         atomicAdd(counts + 1, b);
+        if (SUB)
+            atomicAdd(counts + 2, c);
     }
 }
 
@@ -701,7 +720,7 @@ struct bbt_filterset {
     std::vector<double> bscore_fpr;
     uint32_t bscore_frames = 0;
     /* work queue counter and per-call scratch */
-    unsigned long long *dscratch = nullptr; /* [1..2] insert counts, [3] popcount, [4..5] walk queues */
+    unsigned long long *dscratch = nullptr; /* [3] popcount, [4..6] insert counts */
     int cu_count = 0;
     /* scheduling knobs (bbt_set_tuning / bbt_autotune) */
     bbt_tuning tune = {0, 0, 0, 0};
@@ -950,37 +969,82 @@ bbt_status bbt_filter_refresh(bbt_filterset *fs, uint32_t idx, void *stream) {
     return BBT_OK;
 }
 
-bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
-                             uint64_t nbytes, uint64_t *counts, void *stream) {
-    if (!fs || idx >= fs->desc.size() || (!dseq && nbytes))
-        return fail(BBT_ERR_ARG, "bbt_filter_insert: bad argument");
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    std::lock_guard<std::mutex> g(fs->mu);
-    unsigned long long *cnt = fs->dscratch + 1;
-    k_zero_u64<<<1, 64, 0, st>>>(cnt, 2);
+/* k_insert over dseq[0..nbytes), with the subtract filter when sub != null;
+ * hc receives {new, already present, removed}.  Caller holds fs->mu. */
+static bbt_status insert_impl(bbt_filterset *fs, uint32_t idx, const FilterDev *sub,
+                              const char *dseq, uint64_t nbytes, unsigned long long hc[3],
+                              hipStream_t st) {
+    unsigned long long *cnt = fs->dscratch + 4;
+    k_zero_u64<<<1, 64, 0, st>>>(cnt, 3);
     const uint32_t chunk = 256;
     const uint64_t threads = (nbytes + chunk - 1) / chunk;
     const uint64_t blocks = (threads + kBlock - 1) / kBlock;
     if (blocks > 0x7FFFFFFFull)
         return fail(BBT_ERR_ARG, "bbt_filter_insert: sequence too large for one call");
     const FilterDev &d = fs->desc[idx];
+    const uint8_t *sq = reinterpret_cast<const uint8_t *>(dseq);
     if (blocks) {
-        if (d.k == 25)
-            k_insert<25><<<dim3(uint32_t(blocks)), kBlock, 0, st>>>(
-                reinterpret_cast<const uint8_t *>(dseq), nbytes, d, chunk, cnt);
+        const dim3 g{static_cast<uint32_t>(blocks)};
+        if (sub)
+            d.k == 25 ? k_insert<25, true><<<g, kBlock, 0, st>>>(sq, nbytes, d, *sub, chunk, cnt)
+                      : k_insert<0, true><<<g, kBlock, 0, st>>>(sq, nbytes, d, *sub, chunk, cnt);
         else
-            k_insert<0><<<dim3(uint32_t(blocks)), kBlock, 0, st>>>(
-                reinterpret_cast<const uint8_t *>(dseq), nbytes, d, chunk, cnt);
+            d.k == 25 ? k_insert<25, false><<<g, kBlock, 0, st>>>(sq, nbytes, d, d, chunk, cnt)
+                      : k_insert<0, false><<<g, kBlock, 0, st>>>(sq, nbytes, d, d, chunk, cnt);
         HIP_OK(hipGetLastError());
     }
-    unsigned long long hc[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hc, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    if (counts) {
+    return BBT_OK;
+}
+
+bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
+                             uint64_t nbytes, uint64_t *counts, void *stream) {
+    if (!fs || idx >= fs->desc.size() || (!dseq && nbytes))
+        return fail(BBT_ERR_ARG, "bbt_filter_insert: bad argument");
+    std::lock_guard<std::mutex> g(fs->mu);
+    unsigned long long hc[3] = {0, 0, 0};
+    const bbt_status rc = insert_impl(fs, idx, nullptr, dseq, nbytes, hc,
+                                      static_cast<hipStream_t>(stream));
+    if (rc == BBT_OK && counts) {
         counts[0] = hc[0];
         counts[1] = hc[1];
     }
-    return BBT_OK;
+    return rc;
+}
+
+bbt_status bbt_filter_insert_subtract(bbt_filterset *fs, uint32_t idx,
+                                      const bbt_filterset *sub, uint32_t sub_idx,
+                                      const char *dseq, uint64_t nbytes, uint64_t *counts,
+                                      void *stream) {
+    if (!fs || idx >= fs->desc.size() || !sub || sub_idx >= sub->desc.size() ||
+        (!dseq && nbytes))
+        return fail(BBT_ERR_ARG, "bbt_filter_insert_subtract: bad argument");
+    const FilterDev &d = fs->desc[idx], &s = sub->desc[sub_idx];
+    /* loadFilterSubtract's checks (BloomFilterGenerator.h:222-236) */
+    if (d.h != s.h)
+        return fail(BBT_ERR_ARG, "Error: Subtraction filter's hash number " +
+                                     std::to_string(d.h) +
+                                     " is a different size from than output filter's hash "
+                                     "number " + std::to_string(s.h));
+    if (d.k != s.k)
+        return fail(BBT_ERR_ARG, "Error: Subtraction filter's k-mer size " +
+                                     std::to_string(d.k) +
+                                     " is a different size from than output filter's k-mer "
+                                     "size " + std::to_string(s.k));
+    if (s.bits == d.bits)
+        return fail(BBT_ERR_ARG, "bbt_filter_insert_subtract: subtract filter is the output");
+    std::lock_guard<std::mutex> g(fs->mu);
+    unsigned long long hc[3] = {0, 0, 0};
+    const FilterDev sd = s;
+    const bbt_status rc = insert_impl(fs, idx, &sd, dseq, nbytes, hc,
+                                      static_cast<hipStream_t>(stream));
+    if (rc == BBT_OK && counts) {
+        counts[0] = hc[0];
+        counts[1] = hc[1];
+        counts[2] = hc[2];
+    }
+    return rc;
 }
 
 bbt_status bbt_filter_store(const bbt_filterset *fs, uint32_t idx, const char *prefix,

@@ -2,8 +2,8 @@
  * biobloomcategorizer -- the reference CLI (BioBloomCategorizer/
  * BioBloomCategorizer.cpp:145-533) over libbbt_hip: the same options,
  * threshold interpretation, checks and dispatch; classification runs on the
- * GPU, -D dust masking included (a device sdust pre-pass).  Smart-pair
- * single-file input is not on this path and is rejected with a message.
+ * GPU, -D dust masking included (a device sdust pre-pass), and -e with one
+ * input file pairs mates by name within it (smart pair).
  */
 #include <getopt.h>
 #include <sys/stat.h>
@@ -45,6 +45,7 @@ static void printHelpDialog() {
             "  -S, --score_type=N     'harmonic' or 'binomial' scoring. [simple]\n"
             "      --status           Also write <prefix>_readStatus.tsv (read -> class).\n"
             "      --plain            Use the plain one-launch-per-batch kernel.\n"
+            "      --batch_reads=N    Reads (or pairs) per device batch. [262144]\n"
             "  -h, --help             Display this dialog.\n";
     exit(EXIT_SUCCESS);
 }
@@ -109,6 +110,7 @@ int main(int argc, char *argv[]) {
         {"dust", no_argument, NULL, 'D'},
         {"T_dust", required_argument, NULL, 'T'},
         {"window_dust", required_argument, NULL, 'W'},
+        {"batch_reads", required_argument, NULL, 1000},
         {NULL, 0, NULL, 0}};
     int c, idx = 0;
     while ((c = getopt_long(argc, argv, "f:p:hegl:vs:r:t:cdiwI:nm:S:bDT:W:", long_options,
@@ -128,6 +130,14 @@ int main(int argc, char *argv[]) {
             if (!(cv >> opt.file_interval)) {
                 cerr << "Error - Invalid parameters! I: " << optarg << endl;
                 return 0;
+            }
+            break;
+        }
+        case 1000: {
+            stringstream cv(optarg);
+            if (!(cv >> opt.batch_reads) || opt.batch_reads == 0) {
+                cerr << "Error - Invalid parameters! batch_reads: " << optarg << endl;
+                die = true;
             }
             break;
         }
@@ -207,14 +217,15 @@ int main(int argc, char *argv[]) {
     vector<string> inputFiles;
     while (optind < argc)
         inputFiles.push_back(argv[optind++]);
+    /* BioBloomCategorizer.cpp:357-370: one input file with -e is a smart pair */
+    bool smartPair = false;
     if (paired) {
         if (inputFiles.size() == 1) {
-            cerr << "Error: single-file (smart pair) input is not available on this path"
-                 << endl;
-            die = true;
+            smartPair = true;
         } else if (inputFiles.size() != 2 && fileListFilename.empty()) {
             cerr << "Usage of paired end mode:\n"
                  << "BioBloomCategorizer [OPTION]... -f \"[FILTER1]...\" [FILEPAIR1] [FILEPAIR2]\n"
+                 << "or BioBloomCategorizer [OPTION]... -f \"[FILTER1]...\" [SMARTPAIR]\n"
                  << endl;
             die = true;
         }
@@ -298,8 +309,12 @@ int main(int argc, char *argv[]) {
     if (paired) {
         if (opt.inclusive)
             bbc.setInclusive();
-        if (!outputType.empty())
+        if (!outputType.empty() && smartPair)
+            bbc.filterPairPrint(inputFiles[0], outputType);
+        else if (!outputType.empty())
             bbc.filterPairPrint(inputFiles[0], inputFiles[1], outputType);
+        else if (smartPair)
+            bbc.filterPair(inputFiles[0]);
         else if (!fileListFilename.empty())
             bbc.filterPair(list1, list2);
         else

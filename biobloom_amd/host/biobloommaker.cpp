@@ -2,8 +2,9 @@
  * biobloommaker -- the reference maker (BioBloomMaker/BioBloomMaker.cpp:
  * 110-547, standard mode) over libbbt_hip: the same sizing
  * (BloomFilterInfo.h:57-76), k-mer insertion on the GPU, and the .bf/.txt
- * outputs.  Progressive, bait and subtract modes are not on this path and
- * are rejected.
+ * outputs, and -s (a filter whose k-mers are left out, BloomFilterGenerator.cpp:
+ * 1175-1199).  Progressive and bait modes are not on this path and are
+ * rejected.
  */
 #include <getopt.h>
 #include <hip/hip_runtime.h>
@@ -26,12 +27,13 @@ static void usage() {
             "  -g, --hash_num=N       Set number of hash functions to use in filter.\n"
             "  -k, --kmer_size=N      K-mer size to use to create filter. [25]\n"
             "  -n, --num_ele=N        Set the number of expected elements.\n"
+            "  -s, --subtract=N       Path to a filter whose k-mers are not added.\n"
             "  -h, --help             Display this dialog.\n";
     exit(EXIT_SUCCESS);
 }
 
 int main(int argc, char *argv[]) {
-    string prefix, outDir;
+    string prefix, outDir, subtractFilter;
     double fpr = 0.0078125;
     unsigned hashNum = 0, k = 25;
     uint64_t entryNum = 0;
@@ -61,10 +63,14 @@ int main(int argc, char *argv[]) {
         case 'k': cv >> k; break;
         case 'n': cv >> entryNum; break;
         case 'h': usage(); break;
-        case 'r':
         case 's':
-            cerr << "Error: progressive / subtract filters are not available on this path"
-                 << endl;
+            if (!(cv >> subtractFilter)) {
+                cerr << "Error - Invalid set of bloom filter parameters! s: " << optarg << endl;
+                return 0;
+            }
+            break;
+        case 'r':
+            cerr << "Error: progressive filters are not available on this path" << endl;
             die = true;
             break;
         default: die = true;
@@ -129,10 +135,26 @@ int main(int argc, char *argv[]) {
         cerr << "device upload failed" << endl;
         exit(1);
     }
-    uint64_t counts[2] = {0, 0};
-    if (bbt_filter_insert(fs, 0, static_cast<const char *>(d), all.size(), counts, nullptr) !=
-            BBT_OK ||
-        bbt_filter_refresh(fs, 0, nullptr) != BBT_OK || bbt_filter_set_id(fs, 0, prefix.c_str())) {
+    uint64_t counts[3] = {0, 0, 0};
+    bbt_status rc;
+    if (subtractFilter.empty()) {
+        rc = bbt_filter_insert(fs, 0, static_cast<const char *>(d), all.size(), counts, nullptr);
+    } else {
+        /* generate(filename, subtractFilter): the subtract filter with its .txt */
+        bbt_filterset *sub = nullptr;
+        const char *sp = subtractFilter.c_str();
+        if (bbt_filters_load(&sp, 1, &sub) != BBT_OK) {
+            cerr << bbt_last_error() << endl;
+            exit(1);
+        }
+        rc = bbt_filter_insert_subtract(fs, 0, sub, 0, static_cast<const char *>(d), all.size(),
+                                        counts, nullptr);
+        bbt_filters_free(sub);
+        if (rc == BBT_OK)
+            cerr << "Total Number of K-mers not added: " << counts[2] << endl;
+    }
+    if (rc != BBT_OK || bbt_filter_refresh(fs, 0, nullptr) != BBT_OK ||
+        bbt_filter_set_id(fs, 0, prefix.c_str())) {
         cerr << bbt_last_error() << endl;
         exit(1);
     }

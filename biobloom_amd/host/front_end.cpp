@@ -16,6 +16,7 @@
 #include <cstring>
 #include <iostream>
 #include <sstream>
+#include <unordered_map>
 
 namespace bbt_fe {
 
@@ -540,12 +541,68 @@ void BioBloomClassifier::pair_loop(const std::string &f1, const std::string &f2,
     }
 }
 
+/* Single-file ("smart pair") input, BioBloomClassifier.cpp:464-562 and
+ * 572-690: a record's key is its name less the last two characters when the
+ * name ends in '1' or '2' (the key is also the name printed); the first record
+ * of a key waits, the next record with that key completes the pair as mate 2,
+ * and records still waiting at the end of the file are not classified.  Pairs
+ * are classified in the order they complete, the reference's order on one
+ * thread.  A name shorter than two characters is kept whole (the reference
+ * would take a negative length there). */
+void BioBloomClassifier::smart_pair_loop(const std::string &file, const std::string &type) {
+    SeqReader rd(file);
+    if (!rd.ok())
+        die("file " + file + " cannot be opened");
+    std::unordered_map<std::string, SeqRecord> waiting;
+    std::vector<SeqRecord> recs;
+    SeqRecord rec;
+    bool more = true;
+    while (more) {
+        recs.clear();
+        while (recs.size() < 2 * m_opt.batch_reads) {
+            if (rd.read(rec) < 0) {
+                more = false;
+                break;
+            }
+            const size_t L = rec.name.size();
+            if (L >= 2 && (rec.name[L - 1] == '1' || rec.name[L - 1] == '2'))
+                rec.name.resize(L - 2);
+            auto it = waiting.find(rec.name);
+            if (it == waiting.end()) {
+                std::string key = rec.name;
+                waiting.emplace(std::move(key), std::move(rec));
+            } else {
+                recs.push_back(std::move(it->second));
+                recs.push_back(std::move(rec));
+                waiting.erase(it);
+            }
+            rec = SeqRecord();
+        }
+        m_pairs += recs.size() / 2;
+        run_batch(recs, true);
+        emit_batch(recs, true, type);
+    }
+}
+
 void BioBloomClassifier::filterPair(const std::string &file1, const std::string &file2) {
     filterPairPrint(file1, file2, "");
 }
 
+void BioBloomClassifier::filterPair(const std::string &file) {
+    pair_run(file, nullptr, "");
+}
+
+void BioBloomClassifier::filterPairPrint(const std::string &file, const std::string &type) {
+    pair_run(file, nullptr, type);
+}
+
 void BioBloomClassifier::filterPairPrint(const std::string &file1, const std::string &file2,
                                          const std::string &type) {
+    pair_run(file1, &file2, type);
+}
+
+void BioBloomClassifier::pair_run(const std::string &file1, const std::string *file2,
+                                  const std::string &type) {
     if (!type.empty()) {
         m_gz = m_postfix == ".gz";
         std::vector<std::string> names = m_ids;
@@ -560,7 +617,10 @@ void BioBloomClassifier::filterPairPrint(const std::string &file1, const std::st
         m_status = open_one(m_prefix + "_readStatus.tsv", false);
     std::cerr << "Filtering Start" << "\n";
     m_pairs = 0;
-    pair_loop(file1, file2, type);
+    if (file2)
+        pair_loop(file1, *file2, type);
+    else
+        smart_pair_loop(file1, type);
     close_outputs(type);
     std::cerr << "Total Reads:" << m_pairs << std::endl;
     write_summary(m_pairs);

@@ -78,6 +78,9 @@ class BioBloomClassifier {
     void filterPair(const std::string &file1, const std::string &file2);
     void filterPairPrint(const std::string &file1, const std::string &file2,
                          const std::string &outputType);
+    /* one file holding both mates ("smart pair") */
+    void filterPair(const std::string &file);
+    void filterPairPrint(const std::string &file, const std::string &outputType);
     void filterPair(const std::vector<std::string> &files1,
                     const std::vector<std::string> &files2);
     void setOrderedFilter();
@@ -95,6 +98,8 @@ class BioBloomClassifier {
     void write_summary(uint64_t totalReads);
     void progress(uint64_t &total, uint64_t add);
     void pair_loop(const std::string &f1, const std::string &f2, const std::string &type);
+    void smart_pair_loop(const std::string &file, const std::string &type);
+    void pair_run(const std::string &file1, const std::string *file2, const std::string &type);
 
     bbt_filterset *m_fs = nullptr;
     uint32_t m_nf = 0;

@@ -109,6 +109,9 @@ _SIGS = {
     "bbt_filter_download": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "bbt_filter_insert": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                       C.POINTER(C.c_uint64), C.c_void_p]),
+    "bbt_filter_insert_subtract": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                               C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                               C.c_void_p]),
     "bbt_filter_refresh": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "bbt_filter_store": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_double,
                                      C.c_uint64, C.c_uint64, C.c_uint64, C.c_char_p]),
@@ -256,6 +259,14 @@ class FilterSet:
         check(load().bbt_filter_insert(self._h, i, C.c_void_p(dseq_ptr), nbytes, counts,
                                        C.c_void_p(stream)))
         return int(counts[0]), int(counts[1])
+
+    def insert_subtract(self, i, sub, sub_i, dseq_ptr, nbytes, stream=0):
+        """Insert every k-mer that filter sub_i of FilterSet sub does not
+        contain -> (new, already present, removed)."""
+        counts = (C.c_uint64 * 3)()
+        check(load().bbt_filter_insert_subtract(self._h, i, sub._h, sub_i, C.c_void_p(dseq_ptr),
+                                                nbytes, counts, C.c_void_p(stream)))
+        return int(counts[0]), int(counts[1]), int(counts[2])
 
     def refresh(self, i, stream=0):
         check(load().bbt_filter_refresh(self._h, i, C.c_void_p(stream)))

@@ -21,6 +21,10 @@
  *                      191-237), the text of <prefix>_summary.tsv
  *   bbt_filter_insert  BloomFilterGenerator::loadFilter
  *                      (BioBloomMaker/BloomFilterGenerator.h:143-188)
+ *   bbt_filter_insert_subtract
+ *                      BloomFilterGenerator::loadFilterSubtract
+ *                      (BloomFilterGenerator.h:217-284), the maker's -s mode
+ *                      (BloomFilterGenerator.cpp:1175-1199)
  *   bbt_filter_store   BloomFilter::storeFilter + BloomFilterInfo::printInfoFile
  *                      (BloomFilterGenerator.cpp:71, BloomFilterInfo.cpp:81-116)
  *   bbt_calc_optimal_size / bbt_calc_optimal_hashnum
@@ -173,6 +177,14 @@ bbt_status bbt_filter_download(const bbt_filterset *fs, uint32_t idx, void *host
  * (host, optional) receives {k-mers newly set, k-mers already present}. */
 bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
                              uint64_t nbytes, uint64_t *counts, void *stream);
+/* As bbt_filter_insert, but a k-mer that filter sub_idx of sub contains is
+ * not inserted.  The two filters must share h and k (BBT_ERR_ARG with the
+ * reference's message otherwise).  counts (host, optional) receives {k-mers
+ * newly set, k-mers already present, k-mers removed by the subtract filter}. */
+bbt_status bbt_filter_insert_subtract(bbt_filterset *fs, uint32_t idx,
+                                      const bbt_filterset *sub, uint32_t sub_idx,
+                                      const char *dseq, uint64_t nbytes, uint64_t *counts,
+                                      void *stream);
 /* Recomputes popcount and the precomputed FPR after uploads / inserts. */
 bbt_status bbt_filter_refresh(bbt_filterset *fs, uint32_t idx, void *stream);
 /* Writes <path_prefix>.bf and <path_prefix>.txt. */

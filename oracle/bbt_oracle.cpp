@@ -759,6 +759,26 @@ void orc_bf_insert_seq(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
     }
 }
 
+void orc_bf_insert_seq_subtract(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
+                                const uint8_t *sub_bits, uint64_t sub_m, const char *seq,
+                                uint64_t len, uint64_t *n_new, uint64_t *n_redundant,
+                                uint64_t *n_removed) {
+    /* BloomFilterGenerator::loadFilterSubtract (BloomFilterGenerator.h:238-270):
+     * a k-mer the subtract filter contains is counted and not inserted */
+    orc_filter sub{sub_bits, sub_m, h, k, 0};
+    for (NtIter it(seq, len, h, k); !it.end(); it.next()) {
+        uint64_t l = 0;
+        if (bf_contains(sub, it.hashes(), l)) {
+            ++*n_removed;
+            continue;
+        }
+        if (orc_bf_insert(bits, m, h, it.hashes()))
+            ++*n_redundant;
+        else
+            ++*n_new;
+    }
+}
+
 uint64_t orc_popcount(const uint8_t *bits, uint64_t nbytes) {
     uint64_t c = 0;
     for (uint64_t i = 0; i < nbytes; ++i)

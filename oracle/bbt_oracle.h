@@ -62,6 +62,12 @@ int orc_bf_insert(uint8_t *bits, uint64_t m, uint32_t h, const uint64_t *hashes)
 void orc_bf_insert_seq(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
                        const char *seq, uint64_t len, uint64_t *n_new,
                        uint64_t *n_redundant);
+/* loadFilterSubtract: as orc_bf_insert_seq, skipping (and counting in
+ * *n_removed) every k-mer the filter sub_bits/sub_m (same h, k) contains */
+void orc_bf_insert_seq_subtract(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
+                                const uint8_t *sub_bits, uint64_t sub_m, const char *seq,
+                                uint64_t len, uint64_t *n_new, uint64_t *n_redundant,
+                                uint64_t *n_removed);
 uint64_t orc_popcount(const uint8_t *bits, uint64_t nbytes);
 uint64_t orc_calc_optimal_size(uint64_t entries, double fpr, uint32_t h);
 uint32_t orc_calc_optimal_hashnum(double fpr);

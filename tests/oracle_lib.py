@@ -54,6 +54,9 @@ def lib():
         L.orc_bf_insert_seq.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32,
                                         C.c_uint32, C.c_char_p, C.c_uint64, u64p,
                                         u64p]
+        L.orc_bf_insert_seq_subtract.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p, C.c_uint64,
+                                                 C.c_char_p, C.c_uint64, u64p, u64p, u64p]
         L.orc_popcount.restype = C.c_uint64
         L.orc_popcount.argtypes = [C.c_void_p, C.c_uint64]
         L.orc_calc_optimal_size.restype = C.c_uint64
@@ -146,6 +149,18 @@ class Filter:
                                 len(seq), C.byref(new), C.byref(red))
         self._fpr = None
         return new.value, red.value
+
+    def insert_seq_subtract(self, seq, sub):
+        """loadFilterSubtract: insert the k-mers of seq that filter sub does
+        not contain -> (new, redundant, removed)."""
+        if isinstance(seq, str):
+            seq = seq.encode()
+        new, red, rem = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        lib().orc_bf_insert_seq_subtract(self.bits.ctypes.data, self.m, self.h, self.k,
+                                         sub.bits.ctypes.data, sub.m, seq, len(seq),
+                                         C.byref(new), C.byref(red), C.byref(rem))
+        self._fpr = None
+        return new.value, red.value, rem.value
 
     def insert_hashes(self, hv):
         hv = np.ascontiguousarray(hv, np.uint64)

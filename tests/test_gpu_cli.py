@@ -155,3 +155,86 @@ def test_dust_option(data, tmp_path):
         status = open(tmp_path / "dust_readStatus.tsv").read().splitlines()
         assert status == [f"{n}\t{names[c]}" for (n, _), c in zip(dreads, cls)]
         assert open(tmp_path / "dust_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)
+
+
+def smart_pair_model(items):
+    """The pairing rule of single-file paired input (BioBloomClassifier.cpp:
+    497-551): a name ending in '1' or '2' loses its last two characters; the
+    first record of a name waits, the next one with that name completes the
+    pair (earlier record first); pairs come out in the order they complete and
+    records left waiting are dropped."""
+    waiting, pairs = {}, []
+    for name, seq in items:
+        key = name[:-2] if len(name) >= 2 and name[-1] in "12" else name
+        if key in waiting:
+            pairs.append((key, waiting.pop(key), seq))
+        else:
+            waiting[key] = seq
+    return pairs
+
+
+@pytest.mark.parametrize("batch", [100_000, 97])
+def test_smart_pair_single_file(data, tmp_path, batch):
+    """-e with one file: mates found by name anywhere in the file (either mate
+    first), unpaired records dropped, names printed without the /1 /2 suffix;
+    summary and --fq partitions equal the oracle on the completed pairs."""
+    d, reads, filters = data
+    rng = np.random.default_rng(11)
+    items = []
+    for i in range(1200):
+        items.append((f"p{i}/1", reads[2 * i][1]))
+        items.append((f"p{i}/2", reads[2 * i + 1][1]))
+    for j in range(40):
+        items.append((f"lone{j}/1", reads[2400 + j][1]))
+    items.append(("x", reads[2500][1]))
+    items.append(("x", reads[2501][1]))
+    order = rng.permutation(len(items))
+    items = [items[i] for i in order]
+    write_fastq(tmp_path / "smart.fq", items)
+    pairs = smart_pair_model(items)
+    assert len(pairs) == 1201
+    bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    for inc in [False, True]:
+        args = [CAT, "-p", str(tmp_path / "sp"), "-f", bfs, "-e", "--fq",
+                f"--batch_reads={batch}"] + (["-i"] if inc else []) + [str(tmp_path / "smart.fq")]
+        print(f"[gpu] biobloomcategorizer smart pair inclusive={inc} batch={batch}", flush=True)
+        subprocess.run(args, check=True, timeout=120)
+        buf, off = O.pack_reads([s for p in pairs for s in p[1:]])
+        hits, _, cls, _ = O.classify(filters, buf, off, paired=True, inclusive=inc)
+        assert open(tmp_path / "sp_summary.tsv").read() == O.summary(
+            ["g0", "g1", "g2"], hits, paired=True, inclusive=inc)
+        for ci, n in enumerate(names):
+            for m in (1, 2):
+                want = "".join(f"@{key} c\n{p[m]}\n+\n{'I' * len(p[m])}\n"
+                               for p, c in zip(pairs, cls) for key in [p[0]] if c == ci)
+                assert open(tmp_path / f"sp_{n}_{m}.fq").read() == want
+
+
+def test_maker_subtract(data, tmp_path):
+    """biobloommaker -s: k-mers the subtract filter contains are left out;
+    bits and the removed count equal the oracle's loadFilterSubtract, and a
+    subtract filter with another hash number is refused."""
+    d, _, filters = data
+    recs = []
+    for i in (0, 1):
+        body = open(d / f"g{i}.fa").read().split(">")[1:]
+        recs.append(("".join(body[0].split("\n")[1:])))
+    write_fasta(tmp_path / "mix.fa", [("a", recs[0][:20000]), ("b", recs[1][:20000])])
+    print("[gpu] biobloommaker -s g0.bf", flush=True)
+    r = subprocess.run([MAKER, "-p", "mix", "-o", str(tmp_path), "-k", "25", "-f", "0.075",
+                        "-s", str(d / "g0.bf"), str(tmp_path / "mix.fa")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = load_bf(str(tmp_path / "mix.bf"))
+    want = O.Filter(got.m, got.h, got.k)
+    removed = 0
+    for s in (recs[0][:20000], recs[1][:20000]):
+        removed += want.insert_seq_subtract(s, filters[0])[2]
+    assert np.array_equal(got.bits, want.bits)
+    assert f"Total Number of K-mers not added: {removed}" in r.stderr
+    assert removed >= 20000 - 24
+    r = subprocess.run([MAKER, "-p", "bad", "-o", str(tmp_path), "-k", "25", "-g", "5",
+                        "-s", str(d / "g0.bf"), str(tmp_path / "mix.fa")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "hash number" in r.stderr

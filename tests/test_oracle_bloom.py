@@ -86,3 +86,44 @@ def test_sizing_matches_documented_geometry():
         v = int(-n * h / math.log(1 - p ** (1 / h)))
         assert L.orc_calc_optimal_size(n, p, h) == v + 64 - v % 64
         assert L.orc_calc_optimal_size(n, p, h) % 64 == 0
+
+
+def test_insert_subtract_pinned_per_window():
+    """loadFilterSubtract restated: the result equals inserting, window by
+    window, exactly the hashes the subtract filter does not contain (hashes
+    and contains() pinned separately); an empty subtract filter changes
+    nothing, one holding the same sequence removes every k-mer."""
+    rng = np.random.default_rng(3)
+    k, h = 25, 3
+    seq = "".join(rng.choice(list("ACGTN"), 3000, p=[.248, .248, .248, .248, .008]))
+    other = "".join(rng.choice(list("ACGT"), 2000))
+    sub = O.Filter(4096 * 8, h, k)
+    sub.insert_seq(seq[:1200] + "N" + other)
+    got = O.Filter(8192 * 8, h, k)
+    new, red, rem = got.insert_seq_subtract(seq, sub)
+    want = O.Filter(8192 * 8, h, k)
+    pos, hv = O.nthash(seq, k, h)
+    w_new = w_red = w_rem = 0
+    for row in hv:
+        if sub.contains_hashes(row)[0]:
+            w_rem += 1
+            continue
+        if want.insert_hashes(row):
+            w_red += 1
+        else:
+            w_new += 1
+    assert np.array_equal(got.bits, want.bits)
+    assert (new, red, rem) == (w_new, w_red, w_rem)
+    assert rem >= int((pos <= 1200 - k).sum()) > 0 and new + red + rem == len(hv)
+    # an empty subtract filter is plain insertion
+    plain, none = O.Filter(8192 * 8, h, k), O.Filter(64, h, k)
+    a = O.Filter(8192 * 8, h, k)
+    assert a.insert_seq_subtract(seq, none)[2] == 0
+    plain.insert_seq(seq)
+    assert np.array_equal(a.bits, plain.bits)
+    # a subtract filter holding the sequence leaves the output empty
+    full = O.Filter(8192 * 8, h, k)
+    full.insert_seq(seq)
+    z = O.Filter(4096 * 8, h, k)
+    assert z.insert_seq_subtract(seq, full) == (0, 0, len(hv))
+    assert z.popcount() == 0
