@@ -15,6 +15,12 @@ filters resident in HBM; requests of 1k-20k reads arrive Poisson at each
 --qps); reads/s and p50/p99 request latency of continuous batching
 (bbt_server) against the plain per-request launch path.
 
+--workload e2e: the same reads and filters end to end through the
+biobloomcategorizer CLI: FASTQ files on disk (plain, and gzip with --e2e-gz)
+-> parallel parse -> device batches -> summary.tsv; reads/s over the CLI's
+filtering time, with the summary checked against an in-process
+bbt_eval_batch of the same reads.
+
 --workload mibf: the multi-index Bloom filter instead (200 x 5 Mbp
 references, 3 spaced seeds span 25 weight 20, 2M x 150 bp reads, the
 biobloommicategorizer defaults -s 100 -a 1 -m 1 -c 0); one step is one
@@ -308,6 +314,105 @@ def bench_serve(args, torch, D, B):
     print(json.dumps(line), flush=True)
 
 
+def write_fastq(path, reads, mate, gz=False):
+    """FASTQ with fixed-width names @r<9 digits>/<mate>, qualities 'I'."""
+    import gzip
+    n, L = reads.shape
+    w = 2 * L + 18
+    a = np.empty((n, w), np.uint8)
+    idx = np.arange(n, dtype=np.int64)
+    a[:, 0] = ord("@")
+    a[:, 1] = ord("r")
+    for d in range(9):
+        a[:, 2 + d] = 48 + (idx // 10 ** (8 - d)) % 10
+    a[:, 11] = ord("/")
+    a[:, 12] = ord(str(mate))
+    a[:, 13] = 10
+    a[:, 14:14 + L] = reads
+    a[:, 14 + L] = 10
+    a[:, 15 + L] = ord("+")
+    a[:, 16 + L] = 10
+    a[:, 17 + L:17 + 2 * L] = ord("I")
+    a[:, 17 + 2 * L] = 10
+    if gz:
+        with gzip.open(path, "wb", compresslevel=1) as f:
+            f.write(a.tobytes())
+    else:
+        a.tofile(path)
+    return a.nbytes
+
+
+def bench_e2e(args, torch, D, B):
+    import subprocess
+    import tempfile
+    w, buf, off = make_workload("multi", args.scale)
+    t = time.time()
+    fs, counts = D.build_filters(w["genomes"], w["k"], w["fpr"])
+    torch.cuda.synchronize()
+    log(f"filters built on device ({time.time() - t:.2f}s)")
+    reads = w["reads"]
+    npairs = reads.shape[0] // 2
+    # the in-process result the CLI's summary must equal
+    dr = D.DeviceReads(buf, off)
+    res = D.classify(fs, dr, B.params(paired=True))
+    want = fs.summary_text(res["summary"], npairs)
+    del dr
+    cli = os.path.join(ROOT, "biobloom_amd", "biobloomcategorizer")
+    tmp = tempfile.mkdtemp(prefix="bbt_e2e_", dir=os.environ.get("TMPDIR", "/tmp"))
+    bfs = []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        pre = os.path.join(tmp, f"g{i}")
+        glen = len(w["genomes"][i])
+        fs.store(i, pre, w["fpr"], glen - w["k"] + 1, counts[i][0], counts[i][1], f"g{i}.fa")
+        bfs.append(pre + ".bf")
+    runs = []
+    variants = [("fastq", False)] + ([("fastq.gz", True)] if args.e2e_gz else [])
+    for ext, gz in variants:
+        t = time.time()
+        f1, f2 = os.path.join(tmp, f"r1.{ext}"), os.path.join(tmp, f"r2.{ext}")
+        nbytes = write_fastq(f1, reads[0::2], 1, gz) + write_fastq(f2, reads[1::2], 2, gz)
+        disk = os.path.getsize(f1) + os.path.getsize(f2)
+        log(f"wrote {ext} pair files: {nbytes / 1e6:.0f} MB text, {disk / 1e6:.0f} MB on "
+            f"disk ({time.time() - t:.1f}s)")
+        for threads in args.e2e_threads:
+            prefix = os.path.join(tmp, f"out_{ext}_{threads}")
+            cmd = [cli, "-e", "-p", prefix, "-f", " ".join(bfs), "-t", str(threads), f1, f2]
+            t = time.time()
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            wall = time.time() - t
+            if r.returncode != 0:
+                raise SystemExit(f"biobloomcategorizer failed: {r.stderr[-2000:]}")
+            ft = [float(x.split()[2]) for x in r.stderr.splitlines()
+                  if x.startswith("Filtering time:")][0]
+            got = open(prefix + "_summary.tsv").read()
+            item = {"input": ext, "threads": threads, "filtering_s": ft, "wall_s": wall,
+                    "reads_per_s": 2 * npairs / ft, "pairs_per_s": npairs / ft,
+                    "text_gb_per_s": nbytes / ft / 1e9, "summary_equal": got == want}
+            runs.append(item)
+            log(f"e2e {ext} -t {threads}: filtering {ft:.2f} s (wall {wall:.2f} s), "
+                f"{item['reads_per_s'] / 1e6:.2f} M reads/s, {item['text_gb_per_s']:.2f} GB/s "
+                f"of FASTQ, summary equal={item['summary_equal']}")
+        os.remove(f1)
+        os.remove(f2)
+    best = max((r for r in runs if r["input"] == "fastq"), key=lambda r: r["reads_per_s"])
+    line = {
+        "metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": best["filtering_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "end to end: biobloomcategorizer -e on two FASTQ files "
+                               "(2M pairs 2x150 bp) with 10 filters from 20 Mbp genomes, "
+                               "k=25 FPR 0.075; parse + device batches + summary.tsv",
+                   "scale": args.scale, "pairs": npairs},
+        "e2e": runs,
+        "summary_equal": all(r["summary_equal"] for r in runs),
+    }
+    print(json.dumps(line), flush=True)
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_walk_{workload}.json")
     if os.path.exists(p):
@@ -322,7 +427,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi",
-                    choices=["multi", "single", "uneven", "mibf", "serve"])
+                    choices=["multi", "single", "uneven", "mibf", "serve", "e2e"])
+    ap.add_argument("--e2e-threads", type=lambda v: [int(x) for x in v.split(",")],
+                    default=[16, 4, 1], help="e2e: CLI parse-thread counts to run")
+    ap.add_argument("--e2e-gz", action="store_true", help="e2e: also gzip inputs")
     ap.add_argument("--qps", default="10,50,200,400,800,1600",
                     help="serve: request arrival rates to sweep")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
@@ -374,6 +482,8 @@ def main():
         return bench_mibf(args, torch, D, B)
     if args.workload == "serve":
         return bench_serve(args, torch, D, B)
+    if args.workload == "e2e":
+        return bench_e2e(args, torch, D, B)
 
     w, buf, off = make_workload(args.workload, args.scale)
     paired = bool(w["paired"])

@@ -32,7 +32,7 @@ static void printHelpDialog() {
             "  -s, --score=N          Score threshold (0..1) or minimum match length (>1).\n"
             "  -b, --best_hit         The best hit is used rather than the score threshold.\n"
             "  -w, --with_score       Output multimatches with scores in the order of filter.\n"
-            "  -t, --threads=N        Accepted for compatibility; the GPU does the work.\n"
+            "  -t, --threads=N        Threads that parse the input. [all, at most 16]\n"
             "  -g, --gz_output        Outputs all output files in compressed gzip.\n"
             "      --fa               Output categorized reads in Fasta files.\n"
             "      --fq               Output categorized reads in Fastq files.\n"
@@ -46,6 +46,7 @@ static void printHelpDialog() {
             "      --status           Also write <prefix>_readStatus.tsv (read -> class).\n"
             "      --plain            Use the plain one-launch-per-batch kernel.\n"
             "      --batch_reads=N    Reads (or pairs) per device batch. [262144]\n"
+            "      --ingest_block=N   Bytes of input parsed per block. [16777216]\n"
             "  -h, --help             Display this dialog.\n";
     exit(EXIT_SUCCESS);
 }
@@ -111,6 +112,7 @@ int main(int argc, char *argv[]) {
         {"T_dust", required_argument, NULL, 'T'},
         {"window_dust", required_argument, NULL, 'W'},
         {"batch_reads", required_argument, NULL, 1000},
+        {"ingest_block", required_argument, NULL, 1001},
         {NULL, 0, NULL, 0}};
     int c, idx = 0;
     while ((c = getopt_long(argc, argv, "f:p:hegl:vs:r:t:cdiwI:nm:S:bDT:W:", long_options,
@@ -141,12 +143,19 @@ int main(int argc, char *argv[]) {
             }
             break;
         }
+        case 1001: {
+            stringstream cv(optarg);
+            if (!(cv >> opt.ingest_block) || opt.ingest_block == 0) {
+                cerr << "Error - Invalid parameters! ingest_block: " << optarg << endl;
+                die = true;
+            }
+            break;
+        }
         case 'e': paired = true; break;
         case 'i': opt.inclusive = true; break;
         case 't': {
             stringstream cv(optarg);
-            unsigned t;
-            if (!(cv >> t)) {
+            if (!(cv >> opt.threads)) {
                 cerr << "Error - Invalid parameter! t: " << optarg << endl;
                 exit(EXIT_FAILURE);
             }

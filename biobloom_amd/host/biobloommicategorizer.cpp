@@ -144,7 +144,7 @@ class MIBFClassifier {
     void filter(const vector<string> &files) {
         cerr << "Filtering Start" << endl;
         for (const string &f : files) {
-            bbt_fe::SeqReader rd(f);
+            bbt_fe::ParallelSeqReader rd(f);
             if (!rd.ok()) {
                 cerr << "file " << f << " cannot be opened" << endl;
                 exit(1);
@@ -152,7 +152,7 @@ class MIBFClassifier {
             vector<bbt_fe::SeqRecord> batch;
             bbt_fe::SeqRecord rec;
             while (rd.read(rec) >= 0) {
-                batch.push_back(rec);
+                batch.push_back(std::move(rec));
                 if (++m_numRead % m_o.interval == 0)
                     cerr << "Currently Reading Read Number: " << m_numRead << endl;
                 if (batch.size() == kBatch) {
@@ -167,7 +167,7 @@ class MIBFClassifier {
     }
 
     void filterPair(const string &f1, const string &f2) {
-        bbt_fe::SeqReader r1(f1), r2(f2);
+        bbt_fe::ParallelSeqReader r1(f1), r2(f2);
         if (!r1.ok() || !r2.ok()) {
             cerr << "file " << (r1.ok() ? f2 : f1) << " cannot be opened" << endl;
             exit(1);
@@ -175,8 +175,8 @@ class MIBFClassifier {
         vector<bbt_fe::SeqRecord> batch;
         bbt_fe::SeqRecord a, b;
         while (r1.read(a) >= 0 && r2.read(b) >= 0) {
-            batch.push_back(a);
-            batch.push_back(b);
+            batch.push_back(std::move(a));
+            batch.push_back(std::move(b));
             if (++m_numRead % m_o.interval == 0)
                 cerr << "Currently Reading Read Number: " << m_numRead << endl;
             if (batch.size() >= 2 * kBatch) {

@@ -1,21 +1,24 @@
 /*
  * front_end.cpp -- BioBloomClassifier over libbbt_hip (see front_end.h).
  *
- * Reads are parsed on the host in input order, gathered into device batches
- * (pinned staging -> HBM) and classified by bbt_eval_batch; the
- * ResultsManager counters accumulate on the device across batches.  Output
- * text follows BioBloomClassifier.h:115-529 and ResultsManager.hpp:191-237.
+ * Reads are parsed by ParallelSeqReader (parsing threads run ahead of the
+ * device), gathered into device batches (pinned staging -> HBM) and
+ * classified by bbt_eval_batch in a two-slot pipeline; the ResultsManager
+ * counters accumulate on the device across batches.  Output text follows
+ * BioBloomClassifier.h:115-529 and ResultsManager.hpp:191-237.
  */
 #include "front_end.h"
 
 #include <hip/hip_runtime.h>
 
 #include <cctype>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <sstream>
+#include <thread>
 #include <unordered_map>
 
 namespace bbt_fe {
@@ -38,116 +41,25 @@ static void bbt_check(bbt_status s) {
 }
 
 /* ===================================================================== */
-/* SeqReader                                                              */
-
-SeqReader::SeqReader(const std::string &path) : m_buf(1 << 16) {
-    m_fp = gzopen(path.c_str(), "r");
-}
-
-SeqReader::~SeqReader() {
-    if (m_fp)
-        gzclose(m_fp);
-}
-
-int SeqReader::getc_() {
-    if (m_err)
-        return -3;
-    if (m_begin >= m_end) {
-        if (m_eof)
-            return -1;
-        m_begin = 0;
-        m_end = gzread(m_fp, m_buf.data(), unsigned(m_buf.size()));
-        if (m_end == 0) {
-            m_eof = true;
-            return -1;
-        }
-        if (m_end < 0) {
-            m_eof = m_err = true;
-            return -3;
-        }
-    }
-    return (unsigned char)m_buf[m_begin++];
-}
-
-/* Appends bytes up to a delimiter ('\n', or any isspace byte when
- * stop_at_space); the delimiter is consumed and reported.  Returns false
- * when nothing could be read at end of file. */
-bool SeqReader::getline_(std::string &out, bool stop_at_space, int *delim) {
-    bool got = false;
-    if (delim)
-        *delim = 0;
-    while (true) {
-        int c = getc_();
-        if (c < 0)
-            break;
-        got = true;
-        if (stop_at_space ? std::isspace(c) != 0 : c == '\n') {
-            if (delim)
-                *delim = c;
-            return true;
-        }
-        out.push_back(char(c));
-    }
-    return got;
-}
-
-int64_t SeqReader::read(SeqRecord &rec) {
-    int c;
-    if (m_last == 0) {
-        while ((c = getc_()) >= 0 && c != '>' && c != '@') {
-        }
-        if (c < 0)
-            return c;
-        m_last = c;
-    }
-    rec.name.clear();
-    rec.comment.clear();
-    rec.seq.clear();
-    rec.qual.clear();
-    int d = 0;
-    if (!getline_(rec.name, true, &d))
-        return m_err ? -3 : -1;
-    if (d != '\n')
-        getline_(rec.comment, false, nullptr);
-    while ((c = getc_()) >= 0 && c != '>' && c != '+' && c != '@') {
-        if (c == '\n')
-            continue;
-        rec.seq.push_back(char(c));
-        getline_(rec.seq, false, nullptr);
-    }
-    if (c == '>' || c == '@')
-        m_last = c;
-    rec.fastq = (c == '+');
-    if (!rec.fastq)
-        return int64_t(rec.seq.size());
-    while ((c = getc_()) >= 0 && c != '\n') {
-    }
-    if (c == -1)
-        return -2;
-    while (rec.qual.size() < rec.seq.size() && getline_(rec.qual, false, nullptr)) {
-    }
-    if (m_err)
-        return -3;
-    m_last = 0;
-    if (rec.seq.size() != rec.qual.size())
-        return -2;
-    return int64_t(rec.seq.size());
-}
-
-/* ===================================================================== */
 /* device batches                                                         */
 
+/* One of the two pipeline slots: pinned staging and device buffers for one
+ * batch of reads.  Both slots run on the classifier's one stream, so their
+ * launches reach the shared summary counters in input order; the host packs
+ * and emits one slot while the device works on the other. */
 struct BioBloomClassifier::Batch {
     hipStream_t st = nullptr;
     uint32_t nf = 0;
     bool scores = false;
     uint64_t cap_reads = 0, cap_bytes = 0;
+    uint64_t n = 0; /* reads packed / in flight */
+    uint64_t max_len = 0;
+    bool paired = false;
     char *d_seq = nullptr, *h_seq = nullptr;
     uint64_t *d_off = nullptr, *h_off = nullptr;
     uint8_t *d_hits = nullptr, *h_hits = nullptr;
     double *d_scores = nullptr, *h_scores = nullptr;
     uint32_t *d_cls = nullptr, *h_cls = nullptr;
-    uint64_t *d_summary = nullptr, *d_ctr = nullptr;
 
     void release() {
         (void)hipFree(d_seq);
@@ -211,24 +123,34 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
         std::cerr << std::endl;
     }
     std::cerr << "Filter Loading Complete." << std::endl;
-    m_b = new Batch();
-    m_b->nf = m_nf;
-    m_b->scores = m_opt.mode == BBT_BESTHIT || m_opt.mode == BBT_SCORES;
-    HIP_CHECK(hipStreamCreate(&m_b->st));
-    HIP_CHECK(hipMalloc(&m_b->d_summary, (2 * m_nf + 2) * 8));
-    HIP_CHECK(hipMalloc(&m_b->d_ctr, 4 * 8));
-    HIP_CHECK(hipMemset(m_b->d_summary, 0, (2 * m_nf + 2) * 8));
-    HIP_CHECK(hipMemset(m_b->d_ctr, 0, 4 * 8));
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreate(&st));
+    m_stream = st;
+    for (Batch *&b : m_slot) {
+        b = new Batch();
+        b->nf = m_nf;
+        b->scores = m_opt.mode == BBT_BESTHIT || m_opt.mode == BBT_SCORES;
+        b->st = st;
+    }
+    HIP_CHECK(hipMalloc(&m_dsummary, (2 * m_nf + 2) * 8));
+    HIP_CHECK(hipMalloc(&m_dctr, 4 * 8));
+    HIP_CHECK(hipMemset(m_dsummary, 0, (2 * m_nf + 2) * 8));
+    HIP_CHECK(hipMemset(m_dctr, 0, 4 * 8));
 }
 
 BioBloomClassifier::~BioBloomClassifier() {
-    if (m_b) {
-        m_b->release();
-        (void)hipFree(m_b->d_summary);
-        (void)hipFree(m_b->d_ctr);
-        (void)hipStreamDestroy(m_b->st);
-        delete m_b;
+    if (m_stream)
+        (void)hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
+    for (Batch *&b : m_slot) {
+        if (b)
+            b->release();
+        delete b;
+        b = nullptr;
     }
+    (void)hipFree(m_dsummary);
+    (void)hipFree(m_dctr);
+    if (m_stream)
+        (void)hipStreamDestroy(static_cast<hipStream_t>(m_stream));
     bbt_filters_free(m_fs);
 }
 
@@ -242,26 +164,51 @@ void BioBloomClassifier::setOrderedFilter() {
     m_opt.mode = BBT_ORDERED;
 }
 
-void BioBloomClassifier::run_batch(std::vector<SeqRecord> &recs, bool paired) {
-    Batch &b = *m_b;
-    const uint64_t n = recs.size();
+/* Packs recs into slot s's pinned staging (the device may still be working
+ * on the other slot). */
+void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired) {
+    Batch &b = *m_slot[s];
+    const uint64_t n = recs.v.size();
+    b.n = n;
+    b.paired = paired;
     if (n == 0)
         return;
     uint64_t bytes = 0, maxlen = 0;
-    for (const SeqRecord &r : recs) {
-        bytes += r.seq.size();
-        maxlen = std::max<uint64_t>(maxlen, r.seq.size());
+    for (const SeqView &r : recs.v) {
+        bytes += r.seq_len;
+        maxlen = std::max<uint64_t>(maxlen, r.seq_len);
     }
-    if (maxlen > 0xFFFFFFFFull)
-        die("read longer than 4 Gbp");
     b.reserve(n, bytes);
     uint64_t o = 0;
     for (uint64_t i = 0; i < n; ++i) {
         b.h_off[i] = o;
-        std::memcpy(b.h_seq + o, recs[i].seq.data(), recs[i].seq.size());
-        o += recs[i].seq.size();
+        o += recs.v[i].seq_len;
     }
     b.h_off[n] = o;
+    /* gather the bases into the pinned staging buffer on a few threads */
+    const unsigned T = o < (uint64_t(8) << 20) ? 1u : 8u;
+    const auto gather = [&](unsigned t) {
+        for (uint64_t i = n * t / T, e = n * (t + 1) / T; i < e; ++i)
+            std::memcpy(b.h_seq + b.h_off[i], recs.v[i].seq, recs.v[i].seq_len);
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < T; ++t)
+        th.emplace_back(gather, t);
+    gather(0);
+    for (std::thread &t : th)
+        t.join();
+    b.max_len = maxlen;
+}
+
+/* Enqueues copy-in, classification and copy-out of slot s; returns without
+ * waiting. */
+void BioBloomClassifier::submit(int s) {
+    Batch &b = *m_slot[s];
+    const uint64_t n = b.n;
+    const bool paired = b.paired;
+    if (n == 0)
+        return;
+    const uint64_t o = b.h_off[n];
     HIP_CHECK(hipMemcpyAsync(b.d_seq, b.h_seq, (o + 7) / 8 * 8, hipMemcpyHostToDevice, b.st));
     HIP_CHECK(hipMemcpyAsync(b.d_off, b.h_off, (n + 1) * 8, hipMemcpyHostToDevice, b.st));
     bbt_reads in;
@@ -270,7 +217,7 @@ void BioBloomClassifier::run_batch(std::vector<SeqRecord> &recs, bool paired) {
     in.seq_bytes = b.cap_bytes;
     in.off = b.d_off;
     in.nreads = n;
-    in.max_len = uint32_t(maxlen);
+    in.max_len = uint32_t(b.max_len);
     bbt_eval_params p;
     std::memset(&p, 0, sizeof(p));
     p.mode = m_opt.mode;
@@ -288,8 +235,8 @@ void BioBloomClassifier::run_batch(std::vector<SeqRecord> &recs, bool paired) {
     out.hits = b.d_hits;
     out.scores = b.d_scores;
     out.cls = b.d_cls;
-    out.summary = b.d_summary;
-    out.counters = b.d_ctr;
+    out.summary = m_dsummary;
+    out.counters = m_dctr;
     bbt_check(bbt_eval_batch(m_fs, &in, &p, &out, b.st));
     const uint64_t nrec = paired ? n / 2 : n;
     HIP_CHECK(hipMemcpyAsync(b.h_cls, b.d_cls, nrec * 4, hipMemcpyDeviceToHost, b.st));
@@ -297,7 +244,44 @@ void BioBloomClassifier::run_batch(std::vector<SeqRecord> &recs, bool paired) {
     if (b.scores)
         HIP_CHECK(hipMemcpyAsync(b.h_scores, b.d_scores, n * m_nf * 8,
                                  hipMemcpyDeviceToHost, b.st));
-    HIP_CHECK(hipStreamSynchronize(b.st));
+}
+
+/* Waits for slot s's launch (the stream also finishes any earlier one). */
+void BioBloomClassifier::finish(int s) {
+    if (m_slot[s]->n)
+        HIP_CHECK(hipStreamSynchronize(m_slot[s]->st));
+}
+
+/* Blocks one batch may hold: two batches in flight stay well inside the
+ * reader's pool, so the reader can always parse ahead. */
+constexpr size_t kBatchBlocks = ParallelSeqReader::kMaxBlocks / 2 - 2;
+
+/* Two-slot pipeline over a batch source: next(recs) fills the next batch and
+ * returns false once the input is exhausted (recs may still hold records).
+ * While the device classifies one batch, the host gathers the next one
+ * (parsing runs ahead on the reader's threads) and then emits the finished
+ * batch, so outputs keep input order. */
+void BioBloomClassifier::pipeline(const std::function<bool(RecBatch &)> &next, bool paired,
+                                  const std::string &type) {
+    RecBatch recs[2];
+    int cur = 0;
+    bool more = next(recs[0]);
+    pack(0, recs[0], paired);
+    submit(0);
+    while (true) {
+        const int nx = 1 - cur;
+        recs[nx].clear();
+        if (more)
+            more = next(recs[nx]);
+        pack(nx, recs[nx], paired);
+        finish(cur);
+        emit_batch(cur, recs[cur], paired, type);
+        recs[cur].clear(); /* releases its blocks to the reader */
+        if (recs[nx].v.empty())
+            break;
+        submit(nx);
+        cur = nx;
+    }
 }
 
 /* iostream default formatting of a double, as the reference prints scores */
@@ -344,20 +328,30 @@ static Out *open_one(const std::string &path, bool gz) {
 }
 
 /* header line of one record for the partition files / stdout */
-static std::string record_text(const SeqRecord &r, const std::string &type,
+static std::string record_text(const SeqView &r, const std::string &type,
                                const std::string &extra) {
     std::string s;
-    if (type == "fa") {
-        s = ">" + r.name + " " + r.comment + extra + "\n" + r.seq + "\n";
-    } else {
-        s = "@" + r.name + " " + r.comment + extra + "\n" + r.seq + "\n+\n" + r.qual + "\n";
+    s.reserve(r.name_len + r.comment_len + 2 * r.seq_len + extra.size() + 8);
+    s += type == "fa" ? '>' : '@';
+    s.append(r.name, r.name_len);
+    s += ' ';
+    s.append(r.comment, r.comment_len);
+    s += extra;
+    s += '\n';
+    s.append(r.seq, r.seq_len);
+    s += '\n';
+    if (type != "fa") {
+        s += "+\n";
+        s.append(r.qual, r.qual_len);
+        s += '\n';
     }
     return s;
 }
 
-void BioBloomClassifier::emit_batch(const std::vector<SeqRecord> &recs, bool paired,
+void BioBloomClassifier::emit_batch(int s, const RecBatch &batch, bool paired,
                                     const std::string &type) {
-    const Batch &b = *m_b;
+    const Batch &b = *m_slot[s];
+    const std::vector<SeqView> &recs = batch.v;
     const uint64_t nrec = paired ? recs.size() / 2 : recs.size();
     const uint32_t multi = m_nf + 1;
     for (uint64_t r = 0; r < nrec; ++r) {
@@ -392,17 +386,17 @@ void BioBloomClassifier::emit_batch(const std::vector<SeqRecord> &recs, bool pai
             bool print = m_opt.inverse ? cls != 0 : cls == 0;
             if (print)
                 for (int m = 0; m < nm; ++m) {
-                    const SeqRecord &rec = recs[first + m];
+                    const SeqView &rec = recs[first + m];
                     std::string sc = (!m_opt.inverse && m_opt.mode == BBT_BESTHIT)
                                          ? " " + scoreStr[m] : "";
-                    std::cout << "@" << rec.name << " " << rec.comment << sc << "\n"
-                              << rec.seq << "\n+\n" << rec.qual << "\n";
+                    std::cout << "@" << rec.name_sv() << " " << rec.comment_sv() << sc << "\n"
+                              << rec.seq_sv() << "\n+\n" << rec.qual_sv() << "\n";
                 }
         }
         /* printSingleToFile / printPairToFile (BioBloomClassifier.h:144-209, 252-529) */
         if (!m_outs.empty()) {
             for (int m = 0; m < nm; ++m) {
-                const SeqRecord &rec = recs[first + m];
+                const SeqView &rec = recs[first + m];
                 std::string add;
                 if (m_opt.mode == BBT_SCORES && cls == multi)
                     add = extra[m];
@@ -415,7 +409,8 @@ void BioBloomClassifier::emit_batch(const std::vector<SeqRecord> &recs, bool pai
         if (m_status) {
             const std::string name = cls < m_nf ? m_ids[cls]
                                                 : (cls == m_nf ? "noMatch" : "multiMatch");
-            static_cast<Out *>(m_status)->write(recs[first].name + "\t" + name + "\n");
+            static_cast<Out *>(m_status)->write(std::string(recs[first].name_sv()) + "\t" +
+                                                name + "\n");
         }
     }
 }
@@ -429,8 +424,12 @@ void BioBloomClassifier::progress(uint64_t &total, uint64_t add) {
 void BioBloomClassifier::write_summary(uint64_t totalReads) {
     std::vector<uint64_t> sm(2 * m_nf + 2);
     std::vector<uint64_t> ctr(4);
-    HIP_CHECK(hipMemcpy(sm.data(), m_b->d_summary, sm.size() * 8, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(ctr.data(), m_b->d_ctr, 32, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(m_stream)));
+    HIP_CHECK(hipMemcpy(sm.data(), m_dsummary, sm.size() * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(ctr.data(), m_dctr, 32, hipMemcpyDeviceToHost));
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
+    std::cerr << "Filtering time: " << secs << " s" << std::endl;
     m_probes = ctr[0];
     m_lookups = ctr[1];
     uint64_t need = 0;
@@ -458,87 +457,97 @@ void BioBloomClassifier::filter(const std::vector<std::string> &inputFiles) {
     filterPrint(inputFiles, "");
 }
 
-void BioBloomClassifier::filterPrint(const std::vector<std::string> &inputFiles,
-                                     const std::string &type) {
+void BioBloomClassifier::open_outputs(const std::string &type, bool paired) {
     if (!type.empty()) {
         m_gz = m_postfix == ".gz";
         std::vector<std::string> names = m_ids;
         names.push_back("noMatch");
         names.push_back("multiMatch");
-        for (const std::string &n : names)
-            m_outs.push_back(open_one(m_prefix + "_" + n + "." + type + m_postfix, m_gz));
+        for (const std::string &n : names) {
+            if (paired) {
+                m_outs.push_back(open_one(m_prefix + "_" + n + "_1." + type + m_postfix, m_gz));
+                m_outs.push_back(open_one(m_prefix + "_" + n + "_2." + type + m_postfix, m_gz));
+            } else {
+                m_outs.push_back(open_one(m_prefix + "_" + n + "." + type + m_postfix, m_gz));
+            }
+        }
     }
     if (m_opt.read_status)
         m_status = open_one(m_prefix + "_readStatus.tsv", false);
+}
+
+void BioBloomClassifier::filterPrint(const std::vector<std::string> &inputFiles,
+                                     const std::string &type) {
+    open_outputs(type, false);
     std::cerr << "Filtering Start" << std::endl;
+    m_t0 = std::chrono::steady_clock::now();
     uint64_t total = 0;
-    std::vector<SeqRecord> recs;
     for (const std::string &path : inputFiles) {
-        SeqReader rd(path);
+        ParallelSeqReader rd(path, m_opt.threads, m_opt.ingest_block);
         if (!rd.ok())
             die("file " + path + " cannot be opened");
-        bool more = true;
-        while (more) {
-            recs.resize(m_opt.batch_reads);
-            uint64_t n = 0;
-            while (n < m_opt.batch_reads) {
-                if (rd.read(recs[n]) < 0) {
-                    more = false; /* end of file, or a truncated record as kseq */
-                    break;
+        SeqCursor cur(rd);
+        pipeline(
+            [&](RecBatch &batch) {
+                bool more = true;
+                while (batch.v.size() < m_opt.batch_reads &&
+                       batch.holds.size() < kBatchBlocks) {
+                    /* stops at end of file, or a truncated record as kseq */
+                    const SeqView *v = cur.next();
+                    if (!v) {
+                        more = false;
+                        break;
+                    }
+                    batch.v.push_back(*v);
+                    batch.hold(cur.block());
                 }
-                ++n;
-            }
-            recs.resize(n);
-            progress(total, n);
-            run_batch(recs, false);
-            emit_batch(recs, false, type);
-        }
+                progress(total, batch.v.size());
+                return more;
+            },
+            false, type);
     }
     std::cerr << "Total Reads: " << total << "\n";
     if (!type.empty()) {
         for (size_t i = 0; i < m_outs.size(); ++i) {
-            Out *o = static_cast<Out *>(m_outs[i]);
-            if (o->gz)
-                gzclose(o->gz);
-            if (o->f)
-                std::fclose(o->f);
-            delete o;
             std::string n = i < m_nf ? m_ids[i] : (i == m_nf ? "noMatch" : "multiMatch");
             std::cerr << "File written to: " << m_prefix + "_" + n + "." + type + m_postfix
                       << std::endl;
         }
-        m_outs.clear();
+        close_outputs(type);
     }
     write_summary(total);
 }
 
 void BioBloomClassifier::pair_loop(const std::string &f1, const std::string &f2,
                                    const std::string &type) {
-    SeqReader r1(f1), r2(f2);
+    ParallelSeqReader r1(f1, m_opt.threads, m_opt.ingest_block),
+        r2(f2, m_opt.threads, m_opt.ingest_block);
     if (!r1.ok())
         die("file " + f1 + " cannot be opened");
     if (!r2.ok())
         die("file " + f2 + " cannot be opened");
-    std::vector<SeqRecord> recs;
-    bool more = true;
-    while (more) {
-        recs.resize(2 * m_opt.batch_reads);
-        uint64_t n = 0;
-        while (n < m_opt.batch_reads) {
-            /* the loop stops when either file ends (BioBloomClassifier.cpp:269, 291) */
-            int64_t l1 = r1.read(recs[2 * n]);
-            int64_t l2 = r2.read(recs[2 * n + 1]);
-            if (l1 < 0 || l2 < 0) {
-                more = false;
-                break;
+    SeqCursor c1(r1), c2(r2);
+    pipeline(
+        [&](RecBatch &batch) {
+            bool more = true;
+            while (batch.v.size() < 2 * m_opt.batch_reads &&
+                   batch.holds.size() < kBatchBlocks) {
+                /* the loop stops when either file ends (BioBloomClassifier.cpp:269, 291) */
+                const SeqView *a = c1.next();
+                const SeqView *b = c2.next();
+                if (!a || !b) {
+                    more = false;
+                    break;
+                }
+                batch.v.push_back(*a);
+                batch.hold(c1.block());
+                batch.v.push_back(*b);
+                batch.hold(c2.block());
             }
-            ++n;
-        }
-        recs.resize(2 * n);
-        m_pairs += n;
-        run_batch(recs, true);
-        emit_batch(recs, true, type);
-    }
+            m_pairs += batch.v.size() / 2;
+            return more;
+        },
+        true, type);
 }
 
 /* Single-file ("smart pair") input, BioBloomClassifier.cpp:464-562 and
@@ -550,38 +559,45 @@ void BioBloomClassifier::pair_loop(const std::string &f1, const std::string &f2,
  * thread.  A name shorter than two characters is kept whole (the reference
  * would take a negative length there). */
 void BioBloomClassifier::smart_pair_loop(const std::string &file, const std::string &type) {
-    SeqReader rd(file);
+    ParallelSeqReader rd(file, m_opt.threads, m_opt.ingest_block);
     if (!rd.ok())
         die("file " + file + " cannot be opened");
+    SeqCursor cur(rd);
     std::unordered_map<std::string, SeqRecord> waiting;
-    std::vector<SeqRecord> recs;
-    SeqRecord rec;
-    bool more = true;
-    while (more) {
-        recs.clear();
-        while (recs.size() < 2 * m_opt.batch_reads) {
-            if (rd.read(rec) < 0) {
-                more = false;
-                break;
+    pipeline(
+        [&](RecBatch &batch) {
+            bool more = true;
+            while (batch.v.size() < 2 * m_opt.batch_reads &&
+                   batch.holds.size() < kBatchBlocks) {
+                const SeqView *v = cur.next();
+                if (!v) {
+                    more = false;
+                    break;
+                }
+                uint32_t L = v->name_len;
+                if (L >= 2 && (v->name[L - 1] == '1' || v->name[L - 1] == '2'))
+                    L -= 2;
+                std::string key(v->name, L);
+                auto it = waiting.find(key);
+                if (it == waiting.end()) {
+                    /* the first mate waits, copied out of its block */
+                    SeqRecord &r = waiting[key];
+                    v->copy_to(r);
+                    r.name = key;
+                } else {
+                    batch.owned.push_back(std::move(it->second));
+                    waiting.erase(it);
+                    batch.v.push_back(view_of(batch.owned.back()));
+                    SeqView second = *v;
+                    second.name_len = L;
+                    batch.v.push_back(second);
+                    batch.hold(cur.block());
+                }
             }
-            const size_t L = rec.name.size();
-            if (L >= 2 && (rec.name[L - 1] == '1' || rec.name[L - 1] == '2'))
-                rec.name.resize(L - 2);
-            auto it = waiting.find(rec.name);
-            if (it == waiting.end()) {
-                std::string key = rec.name;
-                waiting.emplace(std::move(key), std::move(rec));
-            } else {
-                recs.push_back(std::move(it->second));
-                recs.push_back(std::move(rec));
-                waiting.erase(it);
-            }
-            rec = SeqRecord();
-        }
-        m_pairs += recs.size() / 2;
-        run_batch(recs, true);
-        emit_batch(recs, true, type);
-    }
+            m_pairs += batch.v.size() / 2;
+            return more;
+        },
+        true, type);
 }
 
 void BioBloomClassifier::filterPair(const std::string &file1, const std::string &file2) {
@@ -603,19 +619,9 @@ void BioBloomClassifier::filterPairPrint(const std::string &file1, const std::st
 
 void BioBloomClassifier::pair_run(const std::string &file1, const std::string *file2,
                                   const std::string &type) {
-    if (!type.empty()) {
-        m_gz = m_postfix == ".gz";
-        std::vector<std::string> names = m_ids;
-        names.push_back("noMatch");
-        names.push_back("multiMatch");
-        for (const std::string &n : names) {
-            m_outs.push_back(open_one(m_prefix + "_" + n + "_1." + type + m_postfix, m_gz));
-            m_outs.push_back(open_one(m_prefix + "_" + n + "_2." + type + m_postfix, m_gz));
-        }
-    }
-    if (m_opt.read_status)
-        m_status = open_one(m_prefix + "_readStatus.tsv", false);
+    open_outputs(type, true);
     std::cerr << "Filtering Start" << "\n";
+    m_t0 = std::chrono::steady_clock::now();
     m_pairs = 0;
     if (file2)
         pair_loop(file1, *file2, type);
@@ -629,9 +635,9 @@ void BioBloomClassifier::pair_run(const std::string &file1, const std::string *f
 void BioBloomClassifier::filterPair(const std::vector<std::string> &files1,
                                     const std::vector<std::string> &files2) {
     /* file-list mode (BioBloomClassifier.cpp:744-942): pairs of files in turn */
-    if (m_opt.read_status)
-        m_status = open_one(m_prefix + "_readStatus.tsv", false);
+    open_outputs("", true);
     std::cerr << "Filtering Start" << "\n";
+    m_t0 = std::chrono::steady_clock::now();
     m_pairs = 0;
     for (size_t i = 0; i < files1.size() && i < files2.size(); ++i)
         pair_loop(files1[i], files2[i], "");

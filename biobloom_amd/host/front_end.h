@@ -13,10 +13,14 @@
 #include <stdint.h>
 #include <zlib.h>
 
+#include <chrono>
+#include <deque>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "bbt_hip.h"
+#include "seq_ingest.h"
 
 namespace bbt_fe {
 
@@ -36,34 +40,28 @@ struct Options {
     bool dust = false;               /* opt::dust (-D) */
     uint32_t dust_t = 20;            /* opt::dustT (-T) */
     uint32_t dust_w = 64;            /* opt::dustWindow (-W) */
+    unsigned threads = 0;            /* opt::threads (-t): parsing threads, 0 = all */
+    uint64_t ingest_block = 16u << 20; /* bytes of input parsed per block */
 };
 
-/* One FASTA/FASTQ record as kseq_read delivers it (Common/kseq.h:177-219). */
-struct SeqRecord {
-    std::string name, comment, seq, qual;
-    bool fastq = false;
-};
-
-/* FASTA/FASTQ(.gz) reader with kseq_read's contract: name up to the first
- * whitespace, comment = rest of the header line, sequence lines joined,
- * FASTQ quality read until it is as long as the sequence.  read() returns
- * the sequence length, -1 at end of file, -2 for a truncated quality, -3 on
- * a stream error. */
-class SeqReader {
-  public:
-    explicit SeqReader(const std::string &path);
-    ~SeqReader();
-    bool ok() const { return m_fp != nullptr; }
-    int64_t read(SeqRecord &rec);
-
-  private:
-    int getc_();
-    bool getline_(std::string &out, bool stop_at_space, int *delim);
-    gzFile m_fp = nullptr;
-    std::vector<char> m_buf;
-    int m_begin = 0, m_end = 0;
-    bool m_eof = false, m_err = false;
-    int m_last = 0; /* header char already consumed */
+/* The records of one device batch: views into parsed blocks, the blocks
+ * they point into, and records copied out of blocks (smart-pair mates). */
+struct RecBatch {
+    std::vector<SeqView> v;
+    std::vector<ParallelSeqReader::BlockRef> holds;
+    std::deque<SeqRecord> owned;
+    void clear() {
+        v.clear();
+        holds.clear();
+        owned.clear();
+    }
+    /* keeps blk alive with the batch (two readers alternate in pair input) */
+    void hold(const ParallelSeqReader::BlockRef &blk) {
+        const size_t n = holds.size();
+        if ((n && holds[n - 1] == blk) || (n > 1 && holds[n - 2] == blk))
+            return;
+        holds.push_back(blk);
+    }
 };
 
 class BioBloomClassifier {
@@ -91,9 +89,13 @@ class BioBloomClassifier {
 
   private:
     struct Batch;
-    void run_batch(std::vector<SeqRecord> &recs, bool paired);
-    void emit_batch(const std::vector<SeqRecord> &recs, bool paired,
-                    const std::string &outputType);
+    void pack(int slot, const RecBatch &recs, bool paired);
+    void submit(int slot);
+    void finish(int slot);
+    void pipeline(const std::function<bool(RecBatch &)> &next, bool paired,
+                  const std::string &type);
+    void open_outputs(const std::string &type, bool paired);
+    void emit_batch(int slot, const RecBatch &recs, bool paired, const std::string &outputType);
     void close_outputs(const std::string &outputType);
     void write_summary(uint64_t totalReads);
     void progress(uint64_t &total, uint64_t add);
@@ -108,12 +110,15 @@ class BioBloomClassifier {
     std::string m_prefix, m_postfix;
     Options m_opt;
     bool m_stdout = false;
-    Batch *m_b = nullptr;
+    Batch *m_slot[2] = {nullptr, nullptr};
+    void *m_stream = nullptr; /* hipStream_t of both slots */
+    uint64_t *m_dsummary = nullptr, *m_dctr = nullptr;
     uint64_t m_probes = 0, m_lookups = 0;
     uint64_t m_pairs = 0;
     std::vector<void *> m_outs; /* gzFile or FILE* per class */
     bool m_gz = false;
     void *m_status = nullptr;
+    std::chrono::steady_clock::time_point m_t0; /* "Filtering Start" */
 };
 
 } // namespace bbt_fe

@@ -1,0 +1,66 @@
+/*
+ * ingest_host.cpp -- C entry points over the front ends' FASTA/FASTQ readers
+ * (biobloom_amd/host/seq_ingest.cpp) for the CPU tests: every record of a
+ * file, serialized, from the sequential or the block-parallel reader.
+ */
+#include <cstring>
+#include <string>
+
+#include "seq_ingest.h"
+
+extern "C" {
+
+/* Writes records as name 0x1f comment 0x1f seq 0x1f qual 0x1f (q|a) 0x1e
+ * into out (cap bytes); *len receives the bytes needed, *status the first
+ * negative return.  mode 0: SeqReader, 1: ParallelSeqReader::read, 2: its
+ * blocks of views (ParallelSeqReader::next_block).  Returns the
+ * record count, or -1 when the file cannot be opened. */
+long ingest_dump(const char *path, int mode, unsigned threads, unsigned long block,
+                 char *out, unsigned long cap, unsigned long *len, long *status,
+                 unsigned long *reparsed) {
+    std::string s;
+    bbt_fe::SeqRecord r;
+    long n = 0, st = 0;
+    auto put = [&] {
+        s += r.name + '\x1f' + r.comment + '\x1f' + r.seq + '\x1f' + r.qual + '\x1f' +
+             (r.fastq ? 'q' : 'a') + '\x1e';
+        ++n;
+    };
+    if (mode == 0) {
+        bbt_fe::SeqReader rd(path);
+        if (!rd.ok())
+            return -1;
+        while ((st = long(rd.read(r))) >= 0)
+            put();
+    } else if (mode == 1) {
+        bbt_fe::ParallelSeqReader rd(path, threads, block);
+        if (!rd.ok())
+            return -1;
+        while ((st = long(rd.read(r))) >= 0)
+            put();
+        if (reparsed)
+            *reparsed = rd.reparsed();
+    } else {
+        /* the views of whole blocks, as the classifier front end takes them */
+        bbt_fe::ParallelSeqReader rd(path, threads, block);
+        if (!rd.ok())
+            return -1;
+        while (auto blk = rd.next_block()) {
+            for (size_t p = 0; p < blk->parts.size(); ++p)
+                for (size_t i = 0; i < blk->count[p]; ++i) {
+                    blk->parts[p][i].copy_to(r);
+                    put();
+                }
+            st = long(blk->status);
+        }
+        if (reparsed)
+            *reparsed = rd.reparsed();
+    }
+    *len = s.size();
+    *status = st;
+    if (out && cap >= s.size())
+        std::memcpy(out, s.data(), s.size());
+    return n;
+}
+
+} // extern "C"

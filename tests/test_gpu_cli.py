@@ -238,3 +238,63 @@ def test_maker_subtract(data, tmp_path):
                         "-s", str(d / "g0.bf"), str(tmp_path / "mix.fa")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "hash number" in r.stderr
+
+
+@pytest.mark.parametrize("fmt", ["fq.gz", "fa_crlf_multiline"])
+def test_input_formats_through_parallel_ingest(data, tmp_path, fmt):
+    """gzip FASTQ and CRLF multi-line FASTA go through the block-parallel
+    reader (small blocks, 4 threads): status and summary equal the oracle on
+    the same sequences."""
+    import gzip
+    d, reads, filters = data
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    if fmt == "fq.gz":
+        path = tmp_path / "in.fq.gz"
+        text = "".join(f"@{n} c\n{s}\n+\n{'I' * len(s)}\n" for n, s in reads)
+        path.write_bytes(gzip.compress(text.encode()))
+    else:
+        path = tmp_path / "in.fa"
+        parts = []
+        for n, s in reads:
+            body = "\r\n".join(s[i:i + 60] for i in range(0, len(s), 60))
+            parts.append(f">{n} some comment\r\n{body}\r\n")
+        path.write_bytes("".join(parts).encode())
+    bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+    print(f"[gpu] biobloomcategorizer ingest {fmt}", flush=True)
+    subprocess.run([CAT, "-p", str(tmp_path / "fmt"), "-f", bfs, "--status", "-t", "4",
+                    "--ingest_block=65536", "--batch_reads=700", str(path)],
+                   check=True, timeout=120)
+    buf, off = O.pack_reads([s for _, s in reads])
+    hits, _, cls, _ = O.classify(filters, buf, off)
+    status = open(tmp_path / "fmt_readStatus.tsv").read().splitlines()
+    assert status == [f"{n}\t{names[c]}" for (n, _), c in zip(reads, cls)]
+    assert open(tmp_path / "fmt_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)
+
+
+def test_long_reads_in_small_blocks(data, tmp_path):
+    """20 kbp reads parsed in 64 KiB blocks: each device batch stops at the
+    blocks it may hold, so many short batches run; status and summary equal
+    the oracle."""
+    d, _, filters = data
+    rng = np.random.default_rng(21)
+    genome = open(d / "g1.fa").read().split(">")[1]
+    g = "".join(genome.split("\n")[1:])
+    reads = []
+    for j in range(300):
+        if j % 2:
+            s = "".join(rng.choice(list("ACGT"), 20_000))
+        else:
+            a = int(rng.integers(0, len(g) - 20_000))
+            s = g[a:a + 20_000]
+        reads.append((f"L{j}", s))
+    write_fastq(tmp_path / "long.fq", reads)
+    bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+    print("[gpu] biobloomcategorizer long reads, 64 KiB blocks", flush=True)
+    subprocess.run([CAT, "-p", str(tmp_path / "long"), "-f", bfs, "--status",
+                    "--ingest_block=65536", str(tmp_path / "long.fq")], check=True, timeout=120)
+    buf, off = O.pack_reads([s for _, s in reads])
+    hits, _, cls, _ = O.classify(filters, buf, off)
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    status = open(tmp_path / "long_readStatus.tsv").read().splitlines()
+    assert status == [f"{n}\t{names[c]}" for (n, _), c in zip(reads, cls)]
+    assert open(tmp_path / "long_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)
