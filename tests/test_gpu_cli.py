@@ -298,3 +298,50 @@ def test_long_reads_in_small_blocks(data, tmp_path):
     status = open(tmp_path / "long_readStatus.tsv").read().splitlines()
     assert status == [f"{n}\t{names[c]}" for (n, _), c in zip(reads, cls)]
     assert open(tmp_path / "long_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)
+
+
+def test_score_outputs_stdout_and_gz(data, tmp_path):
+    """The printing paths (BioBloomClassifier.h:115-209): -w --fa puts every
+    filter's score on multi-match headers, -b --fq the best score on every
+    header, -d / -n print first-filter / other reads to stdout, -g gzips the
+    partition files; scores and classes come from the oracle."""
+    import gzip
+    d, reads, filters = data
+    sub = reads[:600]
+    path = tmp_path / "in.fq"
+    write_fastq(path, sub)
+    buf, off = O.pack_reads([s for _, s in sub])
+    bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    g = lambda v: "%g" % v  # iostream's default double format
+
+    def run(args, prefix):
+        print(f"[gpu] biobloomcategorizer {' '.join(args)}", flush=True)
+        return subprocess.run([CAT, "-p", str(tmp_path / prefix), "-f", bfs] + args +
+                              [str(path)], check=True, timeout=120, capture_output=True,
+                              text=True).stdout
+
+    # -w with --fa: SCORES mode
+    run(["-w", "--fa"], "w")
+    _, sc, cls, _ = O.classify(filters, buf, off, mode=O.SCORES, want_scores=True)
+    for ci, n in enumerate(names):
+        want = "".join(
+            f">{rn} c" + ("".join(" " + g(x) for x in sc[r]) if c == 4 else "") + f"\n{s}\n"
+            for r, ((rn, s), c) in enumerate(zip(sub, cls)) if c == ci)
+        assert open(tmp_path / f"w_{n}.fa").read() == want
+    # -b with --fq and -g: BESTHIT, gzip partitions
+    run(["-b", "--fq", "-g"], "b")
+    _, sc, cls, _ = O.classify(filters, buf, off, mode=O.BESTHIT, want_scores=True)
+    for ci, n in enumerate(names):
+        want = "".join(
+            f"@{rn} c {g(max(0.0, float(sc[r].max())))}\n{s}\n+\n{'I' * len(s)}\n"
+            for r, ((rn, s), c) in enumerate(zip(sub, cls)) if c == ci)
+        assert gzip.open(tmp_path / f"b_{n}.fq.gz", "rt").read() == want
+    # -d and -n: stdout
+    _, _, cls, _ = O.classify(filters, buf, off)
+    out = run(["-d"], "d")
+    assert out == "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
+                          for (rn, s), c in zip(sub, cls) if c == 0)
+    out = run(["-d", "-n"], "n")
+    assert out == "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
+                          for (rn, s), c in zip(sub, cls) if c != 0)
