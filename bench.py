@@ -558,16 +558,53 @@ def main():
                              "gbs_at_64B": lps * LINE_BYTES / 1e9}
             log(f"random gather over {nbytes >> 20} MiB: {lps / 1e9:.1f} G lines/s")
 
+    # the same step with the reads starting in pinned host memory and the hits
+    # and classes returned to it (PCIe-inclusive; not the headline value)
+    pcie = None
+    if main_name in results:
+        hseq = torch.from_numpy(np.ascontiguousarray(buf[: int(off[-1])])).pin_memory()
+        hoff = torch.from_numpy(np.ascontiguousarray(off).view(np.int64)).pin_memory()
+        hhits = torch.empty(out.hits.numel(), dtype=torch.uint8).pin_memory()
+        hcls = torch.empty(out.cls.numel(), dtype=torch.int32).pin_memory()
+        path = paths[main_name]
+
+        def step_pcie():
+            dr.seq[: hseq.numel()].copy_(hseq, non_blocking=True)
+            dr.off.copy_(hoff, non_blocking=True)
+            step(path)
+            hhits.copy_(out.hits, non_blocking=True)
+            hcls.copy_(out.cls, non_blocking=True)
+
+        ms_p = time_steps(step_pcie, args.steps, 1, torch)
+        pcie = {"reads_per_s": nreads / (ms_p / 1e3), "ms_per_step": ms_p,
+                "host_to_device_bytes": int(hseq.numel() + hoff.numel() * 8),
+                "device_to_host_bytes": int(hhits.numel() + hcls.numel() * 4)}
+        log(f"{main_name} with host copies: {ms_p:.3f} ms/step, "
+            f"{pcie['reads_per_s'] / 1e6:.2f} M reads/s")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         sample = min(args.cpu_sample, nrec)
         v, dt, parity = cpu_baseline(fs, buf, off, paired, sample, threads, B, R["hits"])
+        sample1 = max(1, sample // 10)
+        v1, dt1, parity1 = cpu_baseline(fs, buf, off, paired, sample1, 1, B, R["hits"])
+        model = None
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": v, "unit": "reads/s", "cores": threads, "kind": "port",
                "sample": f"first {sample} {'pairs' if paired else 'reads'} of the "
                          f"workload, all filters, {dt:.2f}s",
-               "parity_with_gpu": parity}
-        log(f"cpu oracle: {v / 1e6:.3f} M reads/s on {threads} threads, parity={parity}")
+               "parity_with_gpu": parity and parity1,
+               "single_thread": {"value": v1, "sample": f"first {sample1} "
+                                 f"{'pairs' if paired else 'reads'}, {dt1:.2f}s"},
+               "nproc": os.cpu_count(), "cpu_model": model}
+        log(f"cpu oracle: {v / 1e6:.3f} M reads/s on {threads} threads, "
+            f"{v1 / 1e6:.4f} M reads/s on 1 thread, parity={parity and parity1}")
 
     line = {
         "metric": METRIC,
@@ -624,6 +661,7 @@ def main():
                                       if gather else None),
         },
         "tuning": tuning,
+        "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -285,25 +285,26 @@ ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,
     m_fd = ::open(path.c_str(), O_RDONLY);
     if (m_fd < 0)
         return;
+    /* a plain regular file is parsed in place from a read-only mapping;
+     * anything else (gzip, pipes) goes through zlib, which also passes plain
+     * data through */
+    struct stat sb;
     unsigned char magic[2] = {0, 0};
-    const ssize_t r = ::pread(m_fd, magic, 2, 0);
-    if (r == 2 && magic[0] == 0x1f && magic[1] == 0x8b) {
+    if (::fstat(m_fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0 &&
+        ::pread(m_fd, magic, 2, 0) >= 1 && !(magic[0] == 0x1f && magic[1] == 0x8b)) {
+        void *p = ::mmap(nullptr, size_t(sb.st_size), PROT_READ, MAP_PRIVATE, m_fd, 0);
+        if (p != MAP_FAILED) {
+            m_map = static_cast<const char *>(p);
+            m_map_len = size_t(sb.st_size);
+            ::madvise(p, m_map_len, MADV_SEQUENTIAL);
+        }
+    }
+    if (!m_map) {
         m_fp = gzdopen(m_fd, "r"); /* takes the descriptor */
         m_fd = -1;
         if (!m_fp)
             return;
         gzbuffer(m_fp, 1u << 20);
-    } else {
-        /* a plain file is parsed in place from a read-only mapping */
-        struct stat sb;
-        if (::fstat(m_fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0) {
-            void *p = ::mmap(nullptr, size_t(sb.st_size), PROT_READ, MAP_PRIVATE, m_fd, 0);
-            if (p != MAP_FAILED) {
-                m_map = static_cast<const char *>(p);
-                m_map_len = size_t(sb.st_size);
-                ::madvise(p, m_map_len, MADV_SEQUENTIAL);
-            }
-        }
     }
     m_ok = true;
     m_thread = std::thread([this] { run(); });
@@ -335,8 +336,7 @@ size_t ParallelSeqReader::fill_raw(std::vector<char> &buf, size_t at, size_t wan
     size_t got = 0;
     while (got < want) {
         const size_t ask = std::min<size_t>(want - got, size_t(1) << 30);
-        const long r = m_fp ? long(gzread(m_fp, buf.data() + at + got, unsigned(ask)))
-                            : long(::read(m_fd, buf.data() + at + got, ask));
+        const long r = long(gzread(m_fp, buf.data() + at + got, unsigned(ask)));
         if (r < 0) {
             err = eof = true;
             break;
@@ -533,9 +533,9 @@ void ParallelSeqReader::run_mapped() {
     }
 }
 
-/* A gzip stream (or a file that cannot be mapped): the block's bytes are the
- * unfinished record of the previous block followed by the next decompressed
- * bytes; the block keeps its bytes, which its views point into. */
+/* A zlib stream (gzip, a pipe, or a file that cannot be mapped): the block's
+ * bytes are the unfinished record of the previous block followed by the next
+ * decompressed bytes; the block keeps its bytes, which its views point into. */
 void ParallelSeqReader::run_stream() {
     std::vector<char> A(kHeadroom + m_block), B, next;
     bool eof = false, err = false;

@@ -10,15 +10,18 @@
 
 extern "C" {
 
-/* Writes records as name 0x1f comment 0x1f seq 0x1f qual 0x1f (q|a) 0x1e
- * into out (cap bytes); *len receives the bytes needed, *status the first
- * negative return.  mode 0: SeqReader, 1: ParallelSeqReader::read, 2: its
- * blocks of views (ParallelSeqReader::next_block).  Returns the
- * record count, or -1 when the file cannot be opened. */
+/* Reads the file once and serializes its records as name 0x1f comment 0x1f
+ * seq 0x1f qual 0x1f (q|a) 0x1e; *out / *len receive the text (valid until
+ * the next call), *status the first negative return.  mode 0: SeqReader,
+ * 1: ParallelSeqReader::read, 2: its blocks of views
+ * (ParallelSeqReader::next_block).  Returns the record count, or -1 when the
+ * file cannot be opened. */
+static std::string g_text;
+
 long ingest_dump(const char *path, int mode, unsigned threads, unsigned long block,
-                 char *out, unsigned long cap, unsigned long *len, long *status,
-                 unsigned long *reparsed) {
-    std::string s;
+                 const char **out, unsigned long *len, long *status, unsigned long *reparsed) {
+    std::string &s = g_text;
+    s.clear();
     bbt_fe::SeqRecord r;
     long n = 0, st = 0;
     auto put = [&] {
@@ -56,10 +59,9 @@ long ingest_dump(const char *path, int mode, unsigned threads, unsigned long blo
         if (reparsed)
             *reparsed = rd.reparsed();
     }
+    *out = s.data();
     *len = s.size();
     *status = st;
-    if (out && cap >= s.size())
-        std::memcpy(out, s.data(), s.size());
     return n;
 }
 

@@ -36,22 +36,21 @@ def L():
     build()
     lib = C.CDLL(LIB)
     lib.ingest_dump.restype = C.c_long
-    lib.ingest_dump.argtypes = [C.c_char_p, C.c_int, C.c_uint, C.c_ulong, C.c_void_p,
-                                C.c_ulong, C.POINTER(C.c_ulong), C.POINTER(C.c_long),
-                                C.POINTER(C.c_ulong)]
+    lib.ingest_dump.argtypes = [C.c_char_p, C.c_int, C.c_uint, C.c_ulong,
+                                C.POINTER(C.c_void_p), C.POINTER(C.c_ulong),
+                                C.POINTER(C.c_long), C.POINTER(C.c_ulong)]
     return lib
 
 
 def dump(L, path, mode, threads=4, block=1 << 16):
-    ln, st, rp = C.c_ulong(0), C.c_long(0), C.c_ulong(0)
-    n = L.ingest_dump(str(path).encode(), mode, threads, block, None, 0, C.byref(ln),
+    """(records, final status, shares re-parsed) from one pass over path."""
+    ptr, ln, st, rp = C.c_void_p(), C.c_ulong(0), C.c_long(0), C.c_ulong(0)
+    n = L.ingest_dump(str(path).encode(), mode, threads, block, C.byref(ptr), C.byref(ln),
                       C.byref(st), C.byref(rp))
     assert n >= 0
-    buf = C.create_string_buffer(ln.value + 1)
-    L.ingest_dump(str(path).encode(), mode, threads, block, buf, ln.value + 1, C.byref(ln),
-                  C.byref(st), None)
+    raw = C.string_at(ptr, ln.value) if ln.value else b""
     recs = []
-    for item in buf.raw[:ln.value].split(b"\x1e")[:-1]:
+    for item in raw.split(b"\x1e")[:-1]:
         f = item.split(b"\x1f")
         recs.append((f[0], f[1], f[2], f[3], f[4] == b"q"))
     assert len(recs) == n
@@ -149,6 +148,25 @@ def test_generated_files_equal_model(L, tmp_path, seed):
             reparsed += rp
     # multi-line records and '@'-led quality lines defeat some guesses
     assert reparsed > 0
+
+
+def test_pipe_input(L, tmp_path):
+    """A named pipe (not mappable) carrying gzip or plain text is read through
+    zlib and yields the same records."""
+    import threading
+    rng = np.random.default_rng(4)
+    data = random_file(rng, 800)
+    want, status = kseq_model.read_all(data)
+    for payload in (data, gzip.compress(data)):
+        fifo = tmp_path / "fifo"
+        if fifo.exists():
+            fifo.unlink()
+        os.mkfifo(fifo)
+        t = threading.Thread(target=lambda: fifo.write_bytes(payload))
+        t.start()
+        recs, st, _ = dump(L, fifo, 2, 4, 1 << 16)
+        t.join()
+        assert (recs, st) == (want, status)
 
 
 def test_strict_fastq_splits_without_reparse(L, tmp_path):
