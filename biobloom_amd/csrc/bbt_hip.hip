@@ -619,6 +619,158 @@ __global__ __launch_bounds__(kBlock) void k_insert(const uint8_t *seq, uint64_t 
     }
 }
 
+/* BloomFilter::contains for the canonical hash h0: the h bit tests, stopping
+ * at the first clear bit. */
+__device__ inline bool filter_contains(const FilterDev &f, uint64_t h0, uint64_t kseed) {
+    for (uint32_t j = 0; j < f.h; ++j) {
+        const uint64_t p = fastmod(multi_hash(h0, j, kseed), f.m, f.magic, f.sh1, f.sh2);
+        if (!((f.bits[p >> 3] >> (p & 7)) & 1u))
+            return false;
+    }
+    return true;
+}
+
+/* SeqEval::evalSimple (SeqEval.h:37-114) and evalMinMatchLen (:295-332)
+ * with a subtract filter, one lane per read: the test the progressive
+ * filter builders apply to each read (BloomFilterGenerator.cpp:176-262,
+ * 404-470).  A k-mer that the filter and the subtract filter both contain
+ * keeps a hit's control flow but adds nothing to the score (SIMPLE) and
+ * leaves the match length as it is (LENGTH). */
+struct SubEvalArgs {
+    const uint8_t *seq;
+    const uint64_t *off;
+    uint64_t nreads;
+    FilterDev f, sub;
+    uint32_t has_sub, length_mode;
+    double threshold;
+    uint32_t streak, min_len;
+    uint8_t *pass;
+};
+
+__global__ __launch_bounds__(kBlock) void k_eval_sub(SubEvalArgs a) {
+    const uint32_t kv = a.f.k;
+    const uint64_t kseed = uint64_t(kv) * kMultiSeed;
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < a.nreads;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t b = a.off[r], len = a.off[r + 1] - b;
+        const uint8_t *s = a.seq + b;
+        /* evalSimple: thres, antiThres from the size_t frame count */
+        const double n = double(uint64_t(len - kv + 1));
+        const double thres = a.threshold * n;
+        const double anti_thres = floor((1.0 - a.threshold) * n);
+        double score = 0;
+        uint32_t anti = 0, streak = 0, match = 0;
+        uint64_t prev = 0, skip_end = 0;
+        bool first = true, skipping = false;
+        int res = -1; /* undecided / false / true */
+        uint64_t fwd = 0, rc = 0, win = 0;
+        uint32_t run = 0;
+        for (uint64_t i = 0; i < len && res < 0; ++i) {
+            const unsigned c = s[i];
+            if (!base_valid(c)) {
+                run = 0;
+                fwd = rc = 0;
+                continue;
+            }
+            const unsigned code = base_code(c);
+            if (run >= kv) {
+                const unsigned out = unsigned(win >> (2 * (kv - 1))) & 3u;
+                fwd = rotl1(fwd) ^ rotl64(seed_fwd(out), kv) ^ seed_fwd(code);
+                rc = rotr1(rc ^ seed_rc(out) ^ rotl64(seed_rc(code), kv));
+            } else {
+                fwd = rotl1(fwd) ^ seed_fwd(code);
+                rc = rotr1(rc ^ rotl64(seed_rc(code), kv));
+                ++run;
+            }
+            win = (win << 2) | code;
+            if (run < kv)
+                continue;
+            const uint64_t p = i + 1 - kv;
+            const uint64_t h0 = rc < fwd ? rc : fwd;
+            if (a.length_mode) {
+                if (len - p + match < a.min_len) {
+                    res = 0;
+                    break;
+                }
+                if (p != prev + 1)
+                    match = 0;
+                if (filter_contains(a.f, h0, kseed)) {
+                    if (!a.has_sub || !filter_contains(a.sub, h0, kseed))
+                        match = match == 0 ? kv : match + 1;
+                } else {
+                    match = 0;
+                }
+                if (match >= a.min_len) {
+                    res = 1;
+                    break;
+                }
+                prev = p;
+                continue;
+            }
+            if (skipping) {
+                /* the skip after a miss that ends a streak: every window up
+                 * to skipEnd counts against the read, gaps inside included */
+                if (p < skip_end) {
+                    if (anti_thres <= double(++anti)) {
+                        res = 0;
+                        break;
+                    }
+                    prev = p;
+                    continue;
+                }
+                skipping = false;
+            }
+            if (first) {
+                first = false;
+                if (filter_contains(a.f, h0, kseed)) {
+                    if (!a.has_sub || !filter_contains(a.sub, h0, kseed))
+                        score += 0.5;
+                    if (thres <= score) {
+                        res = 1;
+                        break;
+                    }
+                    ++streak;
+                } else if (anti_thres <= double(++anti)) {
+                    res = 0;
+                    break;
+                }
+                prev = p;
+                continue;
+            }
+            if (p != prev + 1) {
+                anti += uint32_t(p - prev - 1);
+                if (anti_thres <= double(anti)) {
+                    res = 0;
+                    break;
+                }
+                streak = 0;
+            }
+            if (filter_contains(a.f, h0, kseed)) {
+                if (!a.has_sub || !filter_contains(a.sub, h0, kseed))
+                    score += streak == 0 ? 0.5 : 1.0;
+                if (thres <= score) {
+                    res = 1;
+                    break;
+                }
+                prev = p;
+                ++streak;
+            } else {
+                if (anti_thres <= double(++anti)) {
+                    res = 0;
+                    break;
+                }
+                prev = p;
+                if (streak >= a.streak) {
+                    skipping = true;
+                    skip_end = p + kv;
+                }
+                streak = 0;
+            }
+        }
+        a.pass[r] = res == 1 ? 1 : 0;
+    }
+}
+
 /* Independent random byte loads over a power-of-two buffer, 8 in flight per
  * lane: the random-line rate that bounds a Bloom probe. */
 __global__ __launch_bounds__(kBlock) void k_rand_gather(const uint8_t *buf, uint64_t mask,
@@ -973,10 +1125,14 @@ bbt_status bbt_filter_refresh(bbt_filterset *fs, uint32_t idx, void *stream) {
  * hc receives {new, already present, removed}.  Caller holds fs->mu. */
 static bbt_status insert_impl(bbt_filterset *fs, uint32_t idx, const FilterDev *sub,
                               const char *dseq, uint64_t nbytes, unsigned long long hc[3],
-                              hipStream_t st) {
+                              hipStream_t st, bool ordered = false) {
     unsigned long long *cnt = fs->dscratch + 4;
     k_zero_u64<<<1, 64, 0, st>>>(cnt, 3);
-    const uint32_t chunk = 256;
+    if (ordered && nbytes > 0xFFFFFFFFull)
+        return fail(BBT_ERR_ARG, "bbt_filter_insert_ordered: sequence above 4 GB");
+    /* ordered: one thread takes every window in order, so the new / present
+     * split is the one a sequential insert gives */
+    const uint32_t chunk = ordered ? uint32_t(std::max<uint64_t>(nbytes, 1)) : 256;
     const uint64_t threads = (nbytes + chunk - 1) / chunk;
     const uint64_t blocks = (threads + kBlock - 1) / kBlock;
     if (blocks > 0x7FFFFFFFull)
@@ -1006,6 +1162,21 @@ bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
     unsigned long long hc[3] = {0, 0, 0};
     const bbt_status rc = insert_impl(fs, idx, nullptr, dseq, nbytes, hc,
                                       static_cast<hipStream_t>(stream));
+    if (rc == BBT_OK && counts) {
+        counts[0] = hc[0];
+        counts[1] = hc[1];
+    }
+    return rc;
+}
+
+bbt_status bbt_filter_insert_ordered(bbt_filterset *fs, uint32_t idx, const char *dseq,
+                                     uint64_t nbytes, uint64_t *counts, void *stream) {
+    if (!fs || idx >= fs->desc.size() || (!dseq && nbytes))
+        return fail(BBT_ERR_ARG, "bbt_filter_insert_ordered: bad argument");
+    std::lock_guard<std::mutex> g(fs->mu);
+    unsigned long long hc[3] = {0, 0, 0};
+    const bbt_status rc = insert_impl(fs, idx, nullptr, dseq, nbytes, hc,
+                                      static_cast<hipStream_t>(stream), true);
     if (rc == BBT_OK && counts) {
         counts[0] = hc[0];
         counts[1] = hc[1];
@@ -1045,6 +1216,40 @@ bbt_status bbt_filter_insert_subtract(bbt_filterset *fs, uint32_t idx,
         counts[2] = hc[2];
     }
     return rc;
+}
+
+bbt_status bbt_eval_subtract(const bbt_filterset *fs, uint32_t idx, const bbt_filterset *sub,
+                             uint32_t sub_idx, const bbt_reads *in, uint32_t scorer,
+                             double threshold, uint32_t streak, uint8_t *pass, void *stream) {
+    if (!fs || idx >= fs->desc.size() || !in || (in->nreads && (!in->seq || !in->off || !pass)) ||
+        (sub && sub_idx >= sub->desc.size()))
+        return fail(BBT_ERR_ARG, "bbt_eval_subtract: bad argument");
+    if (scorer != BBT_SIMPLE && scorer != BBT_LENGTH)
+        return fail(BBT_ERR_UNSUPPORTED, "bbt_eval_subtract: SIMPLE or LENGTH scoring only");
+    SubEvalArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.f = fs->desc[idx];
+    if (sub) {
+        a.sub = sub->desc[sub_idx];
+        if (a.sub.h != a.f.h || a.sub.k != a.f.k)
+            return fail(BBT_ERR_ARG, "bbt_eval_subtract: the subtract filter's hash number and "
+                                     "k-mer size must equal the filter's");
+        a.has_sub = 1;
+    }
+    if (!in->nreads)
+        return BBT_OK;
+    a.seq = reinterpret_cast<const uint8_t *>(in->seq);
+    a.off = in->off;
+    a.nreads = in->nreads;
+    a.length_mode = scorer == BBT_LENGTH;
+    a.threshold = threshold;
+    a.streak = streak;
+    a.min_len = uint32_t(std::lround(threshold));
+    a.pass = pass;
+    const uint64_t blocks = std::min<uint64_t>((in->nreads + kBlock - 1) / kBlock, 65536);
+    k_eval_sub<<<dim3(uint32_t(blocks)), kBlock, 0, static_cast<hipStream_t>(stream)>>>(a);
+    HIP_OK(hipGetLastError());
+    return BBT_OK;
 }
 
 bbt_status bbt_filter_store(const bbt_filterset *fs, uint32_t idx, const char *prefix,

@@ -109,9 +109,14 @@ _SIGS = {
     "bbt_filter_download": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     "bbt_filter_insert": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                       C.POINTER(C.c_uint64), C.c_void_p]),
+    "bbt_filter_insert_ordered": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_uint64), C.c_void_p]),
     "bbt_filter_insert_subtract": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                                C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                                C.c_void_p]),
+    "bbt_eval_subtract": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.POINTER(Reads), C.c_uint32, C.c_double, C.c_uint32,
+                                      C.c_void_p, C.c_void_p]),
     "bbt_filter_refresh": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "bbt_filter_store": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_double,
                                      C.c_uint64, C.c_uint64, C.c_uint64, C.c_char_p]),
@@ -267,6 +272,14 @@ class FilterSet:
         check(load().bbt_filter_insert_subtract(self._h, i, sub._h, sub_i, C.c_void_p(dseq_ptr),
                                                 nbytes, counts, C.c_void_p(stream)))
         return int(counts[0]), int(counts[1]), int(counts[2])
+
+    def eval_subtract(self, i, sub, sub_i, reads, scorer, threshold, streak, pass_ptr,
+                      stream=0):
+        """evalRead of every read against filter i, k-mers of filter sub_i of
+        FilterSet sub (or None) scoring nothing; one byte per read."""
+        check(load().bbt_eval_subtract(self._h, i, sub._h if sub is not None else None, sub_i,
+                                       C.byref(reads), scorer, threshold, streak,
+                                       C.c_void_p(pass_ptr), C.c_void_p(stream)))
 
     def refresh(self, i, stream=0):
         check(load().bbt_filter_refresh(self._h, i, C.c_void_p(stream)))

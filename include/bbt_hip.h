@@ -21,6 +21,9 @@
  *                      191-237), the text of <prefix>_summary.tsv
  *   bbt_filter_insert  BloomFilterGenerator::loadFilter
  *                      (BioBloomMaker/BloomFilterGenerator.h:143-188)
+ *   bbt_eval_subtract  SeqEval::evalRead with a subtract filter (SeqEval.h:
+ *                      37-114, 295-332), the per-read test of the progressive
+ *                      builders (BloomFilterGenerator.cpp:85-1170)
  *   bbt_filter_insert_subtract
  *                      BloomFilterGenerator::loadFilterSubtract
  *                      (BloomFilterGenerator.h:217-284), the maker's -s mode
@@ -177,10 +180,24 @@ bbt_status bbt_filter_download(const bbt_filterset *fs, uint32_t idx, void *host
  * (host, optional) receives {k-mers newly set, k-mers already present}. */
 bbt_status bbt_filter_insert(bbt_filterset *fs, uint32_t idx, const char *dseq,
                              uint64_t nbytes, uint64_t *counts, void *stream);
+/* BloomFilterGenerator::loadFilter(bf, str) (BloomFilterGenerator.h:193-202)
+ * for one read: as bbt_filter_insert, with the k-mers taken in sequence order
+ * by one device thread, so the {new, present} split is exactly the
+ * sequential count the progressive builders add up. */
+bbt_status bbt_filter_insert_ordered(bbt_filterset *fs, uint32_t idx, const char *dseq,
+                                     uint64_t nbytes, uint64_t *counts, void *stream);
 /* As bbt_filter_insert, but a k-mer that filter sub_idx of sub contains is
  * not inserted.  The two filters must share h and k (BBT_ERR_ARG with the
  * reference's message otherwise).  counts (host, optional) receives {k-mers
  * newly set, k-mers already present, k-mers removed by the subtract filter}. */
+/* pass[r] = SeqEval::evalRead(read r, filter idx, threshold, sub) for every
+ * read of a device batch, with scorer BBT_SIMPLE (threshold = the score, streak
+ * = opt::streakThreshold) or BBT_LENGTH (threshold rounded = the minimum match
+ * length).  A k-mer both filters contain is a hit that scores nothing.  sub may
+ * be null; otherwise it must share h and k.  Asynchronous on stream. */
+bbt_status bbt_eval_subtract(const bbt_filterset *fs, uint32_t idx, const bbt_filterset *sub,
+                             uint32_t sub_idx, const bbt_reads *in, uint32_t scorer,
+                             double threshold, uint32_t streak, uint8_t *pass, void *stream);
 bbt_status bbt_filter_insert_subtract(bbt_filterset *fs, uint32_t idx,
                                       const bbt_filterset *sub, uint32_t sub_idx,
                                       const char *dseq, uint64_t nbytes, uint64_t *counts,

@@ -170,6 +170,9 @@ inline bool bf_contains(const orc_filter &f, const uint64_t *hv, uint64_t &looku
 struct Counters {
     uint64_t probes = 0;
     uint64_t lookups = 0;
+    /* the subtract filter of evalRead (SeqEval.h:38): a hit it also contains
+     * scores nothing */
+    const orc_filter *sub = nullptr;
     /* -D: the read's low-complexity intervals and the in-order query cursor
      * of SDust::isLowComp (Common/SDust.hpp:38-48) */
     const std::vector<uint32_t> *dust_start = nullptr, *dust_finish = nullptr;
@@ -191,6 +194,15 @@ inline bool probe(const orc_filter &f, const NtIter &itr, Counters &c) {
         return false;
     ++c.probes;
     return bf_contains(f, itr.hashes(), c.lookups);
+}
+
+/* `subtract == NULL || !subtract->contains(*itr)` after a hit (SeqEval.h:55,
+ * 81, 84, 316). */
+inline bool scores(const NtIter &itr, const Counters &c) {
+    if (!c.sub)
+        return true;
+    uint64_t l = 0;
+    return !bf_contains(*c.sub, itr.hashes(), l);
 }
 
 /* With -D, evalRead / evalScore build an SDust of the read per call
@@ -239,7 +251,8 @@ bool eval_tiling(const orc_filter &f, const char *s, uint64_t len,
     uint64_t prev = 0;
     if (!itr.end()) {
         if (probe(f, itr, c)) {
-            score += 0.5;
+            if (scores(itr, c))
+                score += 0.5;
             if (thres <= score)
                 return true;
             ++streak;
@@ -257,7 +270,9 @@ bool eval_tiling(const orc_filter &f, const char *s, uint64_t len,
             streak = 0;
         }
         if (probe(f, itr, c)) {
-            if (streak == 0)
+            if (!scores(itr, c))
+                ;
+            else if (streak == 0)
                 score += 0.5;
             else if (harmonic)
                 score += 1.0 - 1.0 / (1.0 + double(streak));
@@ -358,10 +373,12 @@ bool eval_min_match_len(const orc_filter &f, const char *s, uint64_t len,
             return false;
         if (itr.pos() != prev + 1)
             matchLen = 0;
-        if (probe(f, itr, c))
-            matchLen = matchLen == 0 ? f.k : matchLen + 1;
-        else
+        if (probe(f, itr, c)) {
+            if (scores(itr, c))
+                matchLen = matchLen == 0 ? f.k : matchLen + 1;
+        } else {
             matchLen = 0;
+        }
         if (matchLen >= minMatchLen)
             return true;
         prev = itr.pos();
@@ -863,6 +880,95 @@ int orc_eval_read(const orc_filter *f, const char *seq, uint64_t len,
     if (lookups)
         *lookups += c.lookups;
     return r ? 1 : 0;
+}
+
+int orc_eval_read_sub(const orc_filter *f, const orc_filter *sub, const char *seq,
+                      uint64_t len, const orc_eval_params *p) {
+    Counters c;
+    c.sub = sub;
+    return eval_read(*f, seq, len, *p, c) ? 1 : 0;
+}
+
+int orc_progressive(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k, const orc_filter *bait,
+                    const orc_filter *sub, const char *seq, const uint64_t *off,
+                    uint64_t npairs, const orc_prog_params *p, uint64_t *total_entries,
+                    uint64_t *total_reads, uint64_t *tagged) {
+    /* BloomFilterGenerator::generateProgressive (BloomFilterGenerator.cpp:
+     * 117-274) and generateProgressiveBait (:343-509) over one pair of read
+     * files, as one thread runs them: per pair, count it, stop once the
+     * filter holds the expected k-mers, else tag the pair when its reads pass
+     * (PROG_STD: both; PROG_INC: either, if it has more k-mers than the score)
+     * against the growing filter or the fixed bait filter, and insert both
+     * reads of a tagged pair. */
+    orc_filter f{bits, m, h, k, 0};
+    orc_eval_params ep;
+    std::memset(&ep, 0, sizeof(ep));
+    ep.scorer = p->scorer;
+    ep.threshold = p->score;
+    ep.streak = p->streak;
+    orc_eval_params bp = ep;
+    bp.threshold = p->bait;
+    const auto passes = [&](const char *s, uint64_t len) {
+        Counters c;
+        c.sub = sub;
+        if (eval_read(f, s, len, ep, c))
+            return true;
+        if (!bait)
+            return false;
+        Counters cb;
+        cb.sub = sub;
+        return eval_read(*bait, s, len, bp, cb);
+    };
+    if (p->single) {
+        /* generateProgressive over single-end files (BloomFilterGenerator.cpp:
+         * 1038-1170): reads are counted only while the filter is short of the
+         * expected k-mers, and a read is tagged alone */
+        if (*total_entries >= p->expected)
+            return 0;
+        for (uint32_t it = 0; it < p->iterations; ++it) {
+            for (uint64_t i = 0; i < npairs; ++i) {
+                if (*total_entries >= p->expected)
+                    break;
+                ++*total_reads;
+                const char *s1 = seq + off[i];
+                const uint64_t l1 = off[i + 1] - off[i];
+                const uint64_t nk1 = l1 > k ? l1 - k + 1 : 0;
+                Counters c;
+                c.sub = sub;
+                if (!(double(nk1) > p->score && eval_read(f, s1, l1, ep, c)))
+                    continue;
+                ++*tagged;
+                uint64_t n = 0, r = 0;
+                orc_bf_insert_seq(bits, m, h, k, s1, l1, &n, &r);
+                *total_entries += n;
+            }
+        }
+        return 0;
+    }
+    for (uint32_t it = 0; it < p->iterations; ++it) {
+        for (uint64_t i = 0; i < npairs; ++i) {
+            ++*total_reads;
+            if (*total_entries >= p->expected)
+                break;
+            const char *s1 = seq + off[2 * i], *s2 = seq + off[2 * i + 1];
+            const uint64_t l1 = off[2 * i + 1] - off[2 * i], l2 = off[2 * i + 2] - off[2 * i + 1];
+            const uint64_t nk1 = l1 > k ? l1 - k + 1 : 0, nk2 = l2 > k ? l2 - k + 1 : 0;
+            bool tag;
+            if (p->inclusive)
+                tag = (double(nk1) > p->score && passes(s1, l1)) ||
+                      (double(nk2) > p->score && passes(s2, l2));
+            else
+                tag = passes(s1, l1) && passes(s2, l2);
+            if (!tag)
+                continue;
+            ++*tagged;
+            uint64_t n = 0, r = 0;
+            orc_bf_insert_seq(bits, m, h, k, s1, l1, &n, &r);
+            orc_bf_insert_seq(bits, m, h, k, s2, l2, &n, &r);
+            *total_entries += n;
+        }
+    }
+    return 0;
 }
 
 double orc_eval_score(const orc_filter *f, const char *seq, uint64_t len,

@@ -97,6 +97,29 @@ typedef struct {
     uint32_t pad;
 } orc_eval_params;
 
+/* evalRead with a subtract filter (SeqEval.h:37-114, 295-332): a hit that sub
+ * also contains scores nothing; SIMPLE and LENGTH scorers. */
+int orc_eval_read_sub(const orc_filter *f, const orc_filter *sub, const char *seq,
+                      uint64_t len, const orc_eval_params *p);
+
+/* Progressive filter growth over read pairs (reads 2i, 2i+1 of seq/off), or
+ * over single reads (npairs = reads) when p->single. */
+typedef struct {
+    double score;        /* -r: the SIMPLE score, or the minimum match length */
+    double bait;         /* -b: the bait filter's threshold */
+    uint32_t scorer;     /* ORC_SIMPLE or ORC_LENGTH */
+    uint32_t inclusive;  /* -i: PROG_INC (either read), else PROG_STD (both) */
+    uint32_t streak;     /* opt::streakThreshold */
+    uint32_t iterations; /* -e */
+    uint32_t single;     /* reads one by one (the single-end file-list builder) */
+    uint32_t pad;
+    uint64_t expected;   /* the k-mers after which growth stops */
+} orc_prog_params;
+int orc_progressive(uint8_t *bits, uint64_t m, uint32_t h, uint32_t k, const orc_filter *bait,
+                    const orc_filter *sub, const char *seq, const uint64_t *off,
+                    uint64_t npairs, const orc_prog_params *p, uint64_t *total_entries,
+                    uint64_t *total_reads, uint64_t *tagged);
+
 int orc_eval_read(const orc_filter *f, const char *seq, uint64_t len,
                   const orc_eval_params *p, uint64_t *probes, uint64_t *lookups);
 double orc_eval_score(const orc_filter *f, const char *seq, uint64_t len,

@@ -71,6 +71,14 @@ def lib():
         L.orc_binom_isf_up.argtypes = [C.c_double, C.c_uint64, C.c_double]
         L.orc_calc_min_count.restype = C.c_uint64
         L.orc_calc_min_count.argtypes = [C.c_uint64, C.c_double, C.c_double]
+        L.orc_eval_read_sub.restype = C.c_int
+        L.orc_eval_read_sub.argtypes = [C.POINTER(OrcFilter), C.POINTER(OrcFilter), C.c_char_p,
+                                        C.c_uint64, C.POINTER(OrcParams)]
+        L.orc_progressive.restype = C.c_int
+        L.orc_progressive.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                      C.POINTER(OrcFilter), C.POINTER(OrcFilter), C.c_char_p,
+                                      u64p, C.c_uint64, C.POINTER(OrcProgParams), u64p, u64p,
+                                      u64p]
         L.orc_eval_read.restype = C.c_int
         L.orc_eval_read.argtypes = [C.POINTER(OrcFilter), C.c_char_p, C.c_uint64,
                                     C.POINTER(OrcParams), u64p, u64p]
@@ -199,6 +207,43 @@ def eval_read(filt, seq, p):
     r = lib().orc_eval_read(C.byref(fs), seq, len(seq), C.byref(p), C.byref(pr),
                             C.byref(lk))
     return bool(r), pr.value, lk.value
+
+
+def eval_read_sub(filt, sub, seq, p):
+    """evalRead with a subtract filter (None: no subtract)."""
+    if isinstance(seq, str):
+        seq = seq.encode()
+    fs = filt.struct()
+    ss = sub.struct() if sub is not None else None
+    return bool(lib().orc_eval_read_sub(C.byref(fs), C.byref(ss) if ss is not None else None,
+                                        seq, len(seq), C.byref(p)))
+
+
+class OrcProgParams(C.Structure):
+    _fields_ = [("score", C.c_double), ("bait", C.c_double), ("scorer", C.c_uint32),
+                ("inclusive", C.c_uint32), ("streak", C.c_uint32), ("iterations", C.c_uint32),
+                ("single", C.c_uint32), ("pad", C.c_uint32), ("expected", C.c_uint64)]
+
+
+def progressive(filt, pairs, score, expected, total_entries, scorer=SIMPLE, inclusive=False,
+                bait=None, bait_score=0.0, sub=None, streak=3, iterations=1, single=False):
+    """Grows filt (in place) over pairs [(seq1, seq2), ...] (or single reads
+    [seq, ...] when single) as the progressive builder does on one thread
+    -> (total_entries, total_reads, tagged)."""
+    buf, off = pack_reads(list(pairs) if single else [s for pr in pairs for s in pr])
+    pp = OrcProgParams(score, bait_score, scorer, int(inclusive), streak, iterations,
+                       int(single), 0, expected)
+    te, tr, tg = (np.array([v], np.uint64) for v in (total_entries, 0, 0))
+    bs = bait.struct() if bait is not None else None
+    ss = sub.struct() if sub is not None else None
+    rc = lib().orc_progressive(filt.bits.ctypes.data, filt.m, filt.h, filt.k,
+                               C.byref(bs) if bs is not None else None,
+                               C.byref(ss) if ss is not None else None,
+                               buf.ctypes.data_as(C.c_char_p), _u64(off),
+                               len(pairs), C.byref(pp), _u64(te), _u64(tr), _u64(tg))
+    assert rc == 0
+    filt._fpr = None
+    return int(te[0]), int(tr[0]), int(tg[0])
 
 
 def eval_score(filt, seq, p):

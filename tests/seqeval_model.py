@@ -25,9 +25,12 @@ class Walk:
         return self.hits[i]
 
 
-def simple(length, k, positions, hits, t, streak_thr=3, harmonic=False):
-    """evalSimple (SeqEval.h:37-114) / evalHarmonic (:116-193)."""
+def simple(length, k, positions, hits, t, streak_thr=3, harmonic=False, sub=None):
+    """evalSimple (SeqEval.h:37-114) / evalHarmonic (:116-193); sub[i] says
+    whether the subtract filter contains window i (a hit it holds scores
+    nothing, SeqEval.h:55, 81, 84)."""
     w = Walk(positions, hits)
+    sub = sub or [False] * len(w.positions)
     n = float(frames(length, k))
     thres = t * n
     anti_thres = math.floor((1.0 - t) * n)
@@ -36,7 +39,8 @@ def simple(length, k, positions, hits, t, streak_thr=3, harmonic=False):
     prev = 0
     if npos:
         if w.probe(0):
-            score += 0.5
+            if not sub[0]:
+                score += 0.5
             if thres <= score:
                 return True, w
             streak += 1
@@ -54,7 +58,9 @@ def simple(length, k, positions, hits, t, streak_thr=3, harmonic=False):
                 return False, w
             streak = 0
         if w.probe(i):
-            if streak == 0:
+            if sub[i]:
+                pass
+            elif streak == 0:
                 score += 0.5
             elif harmonic:
                 score += 1.0 - 1.0 / (1.0 + float(streak))
@@ -138,9 +144,11 @@ def binomial(length, k, positions, hits, thres, streak_thr=3):
     return False, w
 
 
-def min_match_len(length, k, positions, hits, min_len):
-    """evalMinMatchLen (SeqEval.h:295-332)."""
+def min_match_len(length, k, positions, hits, min_len, sub=None):
+    """evalMinMatchLen (SeqEval.h:295-332); a hit the subtract filter holds
+    (sub[i]) leaves the match length as it is (:316)."""
     w = Walk(positions, hits)
+    sub = sub or [False] * len(w.positions)
     match, prev = 0, 0
     for i, p in enumerate(w.positions):
         if length - p + match < min_len:
@@ -148,7 +156,8 @@ def min_match_len(length, k, positions, hits, min_len):
         if p != prev + 1:
             match = 0
         if w.probe(i):
-            match = k if match == 0 else match + 1
+            if not sub[i]:
+                match = k if match == 0 else match + 1
         else:
             match = 0
         if match >= min_len:
