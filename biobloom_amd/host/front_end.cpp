@@ -354,6 +354,8 @@ void BioBloomClassifier::emit_batch(int s, const RecBatch &batch, bool paired,
     const std::vector<SeqView> &recs = batch.v;
     const uint64_t nrec = paired ? recs.size() / 2 : recs.size();
     const uint32_t multi = m_nf + 1;
+    if (!m_stdout && m_outs.empty() && !m_status)
+        return; /* summary only: the counters are on the device */
     for (uint64_t r = 0; r < nrec; ++r) {
         const uint32_t cls = b.h_cls[r];
         const uint64_t first = paired ? 2 * r : r;
