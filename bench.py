@@ -383,8 +383,9 @@ def bench_e2e(args, torch, D, B):
             wall = time.time() - t
             if r.returncode != 0:
                 raise SystemExit(f"biobloomcategorizer failed: {r.stderr[-2000:]}")
-            ft = [float(x.split()[2]) for x in r.stderr.splitlines()
-                  if x.startswith("Filtering time:")][0]
+            tl = [x for x in r.stderr.splitlines() if x.startswith("Filtering time:")][0]
+            ft = float(tl.split()[2])
+            log(tl)
             got = open(prefix + "_summary.tsv").read()
             item = {"input": ext, "threads": threads, "filtering_s": ft, "wall_s": wall,
                     "reads_per_s": 2 * npairs / ft, "pairs_per_s": npairs / ft,

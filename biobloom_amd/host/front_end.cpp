@@ -43,12 +43,12 @@ static void bbt_check(bbt_status s) {
 /* ===================================================================== */
 /* device batches                                                         */
 
-/* One of the two pipeline slots: pinned staging and device buffers for one
- * batch of reads.  Both slots run on the classifier's one stream, so their
- * launches reach the shared summary counters in input order; the host packs
- * and emits one slot while the device works on the other. */
+/* One of the two pipeline slots: pinned staging, device buffers, a stream and
+ * a workspace for one batch of reads.  The host packs and emits one slot while
+ * the device works on the other; both add to the same summary counters. */
 struct BioBloomClassifier::Batch {
     hipStream_t st = nullptr;
+    bbt_workspace *ws = nullptr;
     uint32_t nf = 0;
     bool scores = false;
     uint64_t cap_reads = 0, cap_bytes = 0;
@@ -123,14 +123,14 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
         std::cerr << std::endl;
     }
     std::cerr << "Filter Loading Complete." << std::endl;
-    hipStream_t st;
-    HIP_CHECK(hipStreamCreate(&st));
-    m_stream = st;
+    /* each slot has its own stream and workspace: one slot's copies overlap
+     * the other's kernels */
     for (Batch *&b : m_slot) {
         b = new Batch();
         b->nf = m_nf;
         b->scores = m_opt.mode == BBT_BESTHIT || m_opt.mode == BBT_SCORES;
-        b->st = st;
+        HIP_CHECK(hipStreamCreate(&b->st));
+        bbt_check(bbt_workspace_create(&b->ws));
     }
     HIP_CHECK(hipMalloc(&m_dsummary, (2 * m_nf + 2) * 8));
     HIP_CHECK(hipMalloc(&m_dctr, 4 * 8));
@@ -139,18 +139,18 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
 }
 
 BioBloomClassifier::~BioBloomClassifier() {
-    if (m_stream)
-        (void)hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
     for (Batch *&b : m_slot) {
-        if (b)
-            b->release();
+        if (!b)
+            continue;
+        (void)hipStreamSynchronize(b->st);
+        b->release();
+        bbt_workspace_free(b->ws);
+        (void)hipStreamDestroy(b->st);
         delete b;
         b = nullptr;
     }
     (void)hipFree(m_dsummary);
     (void)hipFree(m_dctr);
-    if (m_stream)
-        (void)hipStreamDestroy(static_cast<hipStream_t>(m_stream));
     bbt_filters_free(m_fs);
 }
 
@@ -237,7 +237,7 @@ void BioBloomClassifier::submit(int s) {
     out.cls = b.d_cls;
     out.summary = m_dsummary;
     out.counters = m_dctr;
-    bbt_check(bbt_eval_batch(m_fs, &in, &p, &out, b.st));
+    bbt_check(bbt_eval_batch_ws(m_fs, b.ws, &in, &p, &out, b.st));
     const uint64_t nrec = paired ? n / 2 : n;
     HIP_CHECK(hipMemcpyAsync(b.h_cls, b.d_cls, nrec * 4, hipMemcpyDeviceToHost, b.st));
     HIP_CHECK(hipMemcpyAsync(b.h_hits, b.d_hits, n * m_nf, hipMemcpyDeviceToHost, b.st));
@@ -246,7 +246,7 @@ void BioBloomClassifier::submit(int s) {
                                  hipMemcpyDeviceToHost, b.st));
 }
 
-/* Waits for slot s's launch (the stream also finishes any earlier one). */
+/* Waits for slot s's launch. */
 void BioBloomClassifier::finish(int s) {
     if (m_slot[s]->n)
         HIP_CHECK(hipStreamSynchronize(m_slot[s]->st));
@@ -263,23 +263,41 @@ constexpr size_t kBatchBlocks = ParallelSeqReader::kMaxBlocks / 2 - 2;
  * batch, so outputs keep input order. */
 void BioBloomClassifier::pipeline(const std::function<bool(RecBatch &)> &next, bool paired,
                                   const std::string &type) {
+    using clk = std::chrono::steady_clock;
+    const auto since = [](clk::time_point t) {
+        return std::chrono::duration<double>(clk::now() - t).count();
+    };
     RecBatch recs[2];
     int cur = 0;
+    clk::time_point t = clk::now();
     bool more = next(recs[0]);
+    m_phase[0] += since(t);
+    t = clk::now();
     pack(0, recs[0], paired);
     submit(0);
+    m_phase[1] += since(t);
     while (true) {
         const int nx = 1 - cur;
+        t = clk::now();
         recs[nx].clear();
         if (more)
             more = next(recs[nx]);
+        m_phase[0] += since(t);
+        t = clk::now();
         pack(nx, recs[nx], paired);
+        m_phase[1] += since(t);
+        t = clk::now();
         finish(cur);
+        m_phase[2] += since(t);
+        t = clk::now();
         emit_batch(cur, recs[cur], paired, type);
         recs[cur].clear(); /* releases its blocks to the reader */
+        m_phase[3] += since(t);
         if (recs[nx].v.empty())
             break;
+        t = clk::now();
         submit(nx);
+        m_phase[1] += since(t);
         cur = nx;
     }
 }
@@ -426,12 +444,15 @@ void BioBloomClassifier::progress(uint64_t &total, uint64_t add) {
 void BioBloomClassifier::write_summary(uint64_t totalReads) {
     std::vector<uint64_t> sm(2 * m_nf + 2);
     std::vector<uint64_t> ctr(4);
-    HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(m_stream)));
+    for (Batch *b : m_slot)
+        HIP_CHECK(hipStreamSynchronize(b->st));
     HIP_CHECK(hipMemcpy(sm.data(), m_dsummary, sm.size() * 8, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(ctr.data(), m_dctr, 32, hipMemcpyDeviceToHost));
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
-    std::cerr << "Filtering time: " << secs << " s" << std::endl;
+    std::cerr << "Filtering time: " << secs << " s (gather " << m_phase[0] << ", pack+launch "
+              << m_phase[1] << ", device wait " << m_phase[2] << ", output " << m_phase[3]
+              << ")" << std::endl;
     m_probes = ctr[0];
     m_lookups = ctr[1];
     uint64_t need = 0;

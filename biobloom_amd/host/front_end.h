@@ -111,7 +111,6 @@ class BioBloomClassifier {
     Options m_opt;
     bool m_stdout = false;
     Batch *m_slot[2] = {nullptr, nullptr};
-    void *m_stream = nullptr; /* hipStream_t of both slots */
     uint64_t *m_dsummary = nullptr, *m_dctr = nullptr;
     uint64_t m_probes = 0, m_lookups = 0;
     uint64_t m_pairs = 0;
@@ -119,6 +118,7 @@ class BioBloomClassifier {
     bool m_gz = false;
     void *m_status = nullptr;
     std::chrono::steady_clock::time_point m_t0; /* "Filtering Start" */
+    double m_phase[4] = {0, 0, 0, 0}; /* s: gather, pack + launch, device wait, output */
 };
 
 } // namespace bbt_fe
