@@ -452,6 +452,8 @@ def main():
                     help="tune the batched path's scheduling knobs on this batch first")
     ap.add_argument("--no-gather", action="store_true",
                     help="skip the random-gather reference measurement")
+    ap.add_argument("--gather-sweep", action="store_true",
+                    help="only measure the random-line rate over buffers of 1 MiB to 8 GiB")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
@@ -479,6 +481,14 @@ def main():
         print(json.dumps({"copy_bytes": n, "read_bytes": n, "write_bytes": n}))
         return
     B.load()
+    if args.gather_sweep:
+        sweep = []
+        for e in range(20, 34):
+            lps, ms = B.measure_gather(1 << e, 64, 0)
+            sweep.append({"bytes": 1 << e, "lines_per_s": lps, "ms": ms})
+            log(f"random gather over {(1 << e) >> 20} MiB: {lps / 1e9:.1f} G lines/s")
+        print(json.dumps({"random_gather_sweep": sweep}), flush=True)
+        return
     if args.workload == "mibf":
         return bench_mibf(args, torch, D, B)
     if args.workload == "serve":

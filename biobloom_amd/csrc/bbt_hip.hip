@@ -135,12 +135,23 @@ __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uin
         atomicOr(a.counters + 3, 1ull);
 }
 
+/* Item -> (read, filter).  The batched path's lane walks are filter-major
+ * (item = f * nr + r), so the walks in flight at any moment probe one or two
+ * filters, whose lines then stay in each XCD's L2: random 128 B lines come
+ * back about 1.5x faster from a 16 MiB working set than from hundreds of MiB.
+ * The plain path keeps the input order, read-major (nr = 0). */
 template <int SC, bool EXH, int K>
-__device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item,
+__device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item, uint64_t nr,
                                            Walker<SC, EXH, K> &w, uint64_t &read,
                                            uint32_t &f) {
-    const uint64_t r = item / a.nf;
-    f = uint32_t(item - r * a.nf);
+    uint64_t r;
+    if (nr) {
+        f = uint32_t(item / nr);
+        r = item - uint64_t(f) * nr;
+    } else {
+        r = item / a.nf;
+        f = uint32_t(item - r * a.nf);
+    }
     read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
@@ -157,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
     if (item < a.nitems) {
         uint64_t read;
         uint32_t f;
-        start_walk(a, item, w, read, f);
+        start_walk(a, item, 0, w, read, f);
         while (w.advance())
             w.consume(*w.addr);
         store_walk(a, read, f, w);
@@ -281,11 +292,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     w.lookups = 0;
     uint32_t issued = 0;
     uint32_t wave_probes = 0, wave_lookups = 0; /* lane 0 only */
-    const uint64_t nlong = a.list_counts ? a.list_counts[0] * a.nf : 0;
-    const uint64_t nitems = a.list_counts ? a.list_counts[1] * a.nf : a.nitems;
+    const uint64_t nl = a.list_counts ? a.list_counts[0] : 0;
+    const uint64_t nr = a.list_counts ? a.list_counts[1] : (a.nf ? a.nitems / a.nf : 0);
+    const uint64_t nlong = nl * a.nf;
+    const uint64_t nitems = nr * a.nf;
 
-    /* phase 1: long reads, a whole wave per (read, filter), longest first;
-     * every lane holds the same walk state */
+    /* phase 1: long reads, a whole wave per (read, filter), longest first
+     * (read-major, so the longest walks of every filter start first and none
+     * is left for the end); every lane holds the same walk state */
     while (nlong) {
         unsigned long long it = 0;
         if (lane == 0)
@@ -324,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             if (!active) {
                 const uint64_t item = base + uint64_t(__popcll(idle & lt_mask));
                 if (item < nitems) {
-                    start_walk(a, item, w, read, f);
+                    start_walk(a, item, nr, w, read, f);
                     active = true;
                 }
             }
