@@ -16,12 +16,15 @@
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "front_end.h"
@@ -81,44 +84,81 @@ bool fexists(const string &f) {
     return in.good();
 }
 
-/* the device side of one batch: reads, outputs, pinned host copies */
-struct Device {
-    hipStream_t st = nullptr;
-    char *dseq = nullptr;
-    uint64_t *doff = nullptr;
+/* One of two pipeline slots: pinned staging and device buffers for a batch
+ * of reads, and pinned host copies of its results. */
+struct Slot {
+    char *dseq = nullptr, *hseq = nullptr;
+    uint64_t *doff = nullptr, *hoff = nullptr;
     uint32_t *dnh = nullptr, *dhits = nullptr, *dsat = nullptr;
-    uint64_t *dctr = nullptr;
+    uint32_t *hnh = nullptr, *hhits = nullptr, *hsat = nullptr;
+    uint64_t *dctr = nullptr, *hctr = nullptr;
     size_t seq_cap = 0, rec_cap = 0;
-    vector<uint32_t> nh, hits, sat;
+    size_t nreads = 0, bytes = 0;
+    uint32_t max_len = 0;
 
-    /* bytes: the padded sequence bytes of the batch */
-    void reserve(size_t bytes, size_t nreads) {
-        if (bytes > seq_cap) {
-            (void)hipFree(dseq);
-            seq_cap = std::max(bytes, seq_cap * 2);
-            if (hipMalloc(&dseq, seq_cap) != hipSuccess)
-                die("device allocation failed");
-        }
-        if (nreads > rec_cap) {
-            (void)hipFree(doff);
-            (void)hipFree(dnh);
-            (void)hipFree(dhits);
-            (void)hipFree(dsat);
-            rec_cap = std::max(nreads, rec_cap * 2);
-            if (hipMalloc(&doff, (rec_cap + 1) * 8) != hipSuccess ||
-                hipMalloc(&dnh, rec_cap * 4) != hipSuccess ||
-                hipMalloc(&dhits, rec_cap * kMaxHits * 7 * 4) != hipSuccess ||
-                hipMalloc(&dsat, rec_cap * 4) != hipSuccess)
-                die("device allocation failed");
-        }
-        if (!dctr && hipMalloc(&dctr, 32) != hipSuccess)
-            die("device allocation failed");
-    }
     static void die(const string &m) {
         cerr << "Error: " << m << endl;
         exit(1);
     }
+    void release() {
+        (void)hipFree(dseq);
+        (void)hipFree(doff);
+        (void)hipFree(dnh);
+        (void)hipFree(dhits);
+        (void)hipFree(dsat);
+        (void)hipFree(dctr);
+        (void)hipHostFree(hseq);
+        (void)hipHostFree(hoff);
+        (void)hipHostFree(hnh);
+        (void)hipHostFree(hhits);
+        (void)hipHostFree(hsat);
+        (void)hipHostFree(hctr);
+        dseq = hseq = nullptr;
+        doff = hoff = nullptr;
+        dnh = dhits = dsat = hnh = hhits = hsat = nullptr;
+        dctr = hctr = nullptr;
+        seq_cap = rec_cap = 0;
+    }
+    /* bytes: the padded sequence bytes of the batch */
+    void reserve(size_t nbytes, size_t n) {
+        if (nbytes > seq_cap) {
+            (void)hipFree(dseq);
+            (void)hipHostFree(hseq);
+            seq_cap = std::max(nbytes, seq_cap * 2);
+            if (hipMalloc(&dseq, seq_cap) != hipSuccess ||
+                hipHostMalloc(&hseq, seq_cap, hipHostMallocDefault) != hipSuccess)
+                die("device allocation failed");
+        }
+        if (n > rec_cap) {
+            (void)hipFree(doff);
+            (void)hipFree(dnh);
+            (void)hipFree(dhits);
+            (void)hipFree(dsat);
+            (void)hipHostFree(hoff);
+            (void)hipHostFree(hnh);
+            (void)hipHostFree(hhits);
+            (void)hipHostFree(hsat);
+            rec_cap = std::max(n, rec_cap * 2);
+            if (hipMalloc(&doff, (rec_cap + 1) * 8) != hipSuccess ||
+                hipMalloc(&dnh, rec_cap * 4) != hipSuccess ||
+                hipMalloc(&dhits, rec_cap * kMaxHits * 7 * 4) != hipSuccess ||
+                hipMalloc(&dsat, rec_cap * 4) != hipSuccess ||
+                hipHostMalloc(&hoff, (rec_cap + 1) * 8, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&hnh, rec_cap * 4, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&hhits, rec_cap * kMaxHits * 7 * 4, hipHostMallocDefault) !=
+                    hipSuccess ||
+                hipHostMalloc(&hsat, rec_cap * 4, hipHostMallocDefault) != hipSuccess)
+                die("device allocation failed");
+        }
+        if (!dctr && (hipMalloc(&dctr, 32) != hipSuccess ||
+                      hipHostMalloc(&hctr, 32, hipHostMallocDefault) != hipSuccess))
+            die("device allocation failed");
+    }
 };
+
+/* blocks of parsed input one batch may hold (two batches stay well inside
+ * the reader's pool) */
+constexpr size_t kBatchBlocks = bbt_fe::ParallelSeqReader::kMaxBlocks / 2 - 2;
 
 class MIBFClassifier {
   public:
@@ -136,88 +176,156 @@ class MIBFClassifier {
                             : m_info.nseeds - o.frameMatches;
         m_above.assign(m_info.nids + 1, 0);
         m_unique.assign(m_info.nids + 1, 0);
-        if (hipStreamCreateWithFlags(&m_dev.st, hipStreamNonBlocking) != hipSuccess)
-            Device::die("stream creation failed");
+        /* one stream: the filter's query state is shared by its launches */
+        if (hipStreamCreateWithFlags(&m_st, hipStreamNonBlocking) != hipSuccess)
+            Slot::die("stream creation failed");
     }
-    ~MIBFClassifier() { bbt_mibf_free(m_mf); }
+    ~MIBFClassifier() {
+        (void)hipStreamSynchronize(m_st);
+        for (Slot &s : m_slot)
+            s.release();
+        (void)hipStreamDestroy(m_st);
+        bbt_mibf_free(m_mf);
+    }
 
     void filter(const vector<string> &files) {
         cerr << "Filtering Start" << endl;
+        m_t0 = std::chrono::steady_clock::now();
         for (const string &f : files) {
             bbt_fe::ParallelSeqReader rd(f);
             if (!rd.ok()) {
                 cerr << "file " << f << " cannot be opened" << endl;
                 exit(1);
             }
-            vector<bbt_fe::SeqRecord> batch;
-            bbt_fe::SeqRecord rec;
-            while (rd.read(rec) >= 0) {
-                batch.push_back(std::move(rec));
-                if (++m_numRead % m_o.interval == 0)
-                    cerr << "Currently Reading Read Number: " << m_numRead << endl;
-                if (batch.size() == kBatch) {
-                    run(batch, false);
-                    batch.clear();
-                }
-            }
-            if (!batch.empty())
-                run(batch, false);
+            bbt_fe::SeqCursor cur(rd);
+            pipeline(
+                [&](bbt_fe::RecBatch &b) {
+                    while (b.v.size() < kBatch && b.holds.size() < kBatchBlocks) {
+                        const bbt_fe::SeqView *v = cur.next();
+                        if (!v)
+                            return false;
+                        b.v.push_back(*v);
+                        b.hold(cur.block());
+                        if (++m_numRead % m_o.interval == 0)
+                            cerr << "Currently Reading Read Number: " << m_numRead << endl;
+                    }
+                    return true;
+                },
+                false);
         }
         finish();
     }
 
     void filterPair(const string &f1, const string &f2) {
+        m_t0 = std::chrono::steady_clock::now();
         bbt_fe::ParallelSeqReader r1(f1), r2(f2);
         if (!r1.ok() || !r2.ok()) {
             cerr << "file " << (r1.ok() ? f2 : f1) << " cannot be opened" << endl;
             exit(1);
         }
-        vector<bbt_fe::SeqRecord> batch;
-        bbt_fe::SeqRecord a, b;
-        while (r1.read(a) >= 0 && r2.read(b) >= 0) {
-            batch.push_back(std::move(a));
-            batch.push_back(std::move(b));
-            if (++m_numRead % m_o.interval == 0)
-                cerr << "Currently Reading Read Number: " << m_numRead << endl;
-            if (batch.size() >= 2 * kBatch) {
-                run(batch, true);
-                batch.clear();
-            }
-        }
-        if (!batch.empty())
-            run(batch, true);
+        bbt_fe::SeqCursor c1(r1), c2(r2);
+        pipeline(
+            [&](bbt_fe::RecBatch &b) {
+                while (b.v.size() < 2 * kBatch && b.holds.size() < kBatchBlocks) {
+                    const bbt_fe::SeqView *a = c1.next();
+                    if (!a)
+                        return false;
+                    const bbt_fe::SeqView *m = c2.next();
+                    if (!m)
+                        return false;
+                    b.v.push_back(*a);
+                    b.hold(c1.block());
+                    b.v.push_back(*m);
+                    b.hold(c2.block());
+                    if (++m_numRead % m_o.interval == 0)
+                        cerr << "Currently Reading Read Number: " << m_numRead << endl;
+                }
+                return true;
+            },
+            true);
         finish();
     }
 
   private:
-    void run(const vector<bbt_fe::SeqRecord> &recs, bool paired) {
-        string seq;
-        vector<uint64_t> off{0};
-        uint32_t max_len = 0;
-        for (const auto &r : recs) {
-            seq += r.seq;
-            off.push_back(seq.size());
-            max_len = std::max<uint32_t>(max_len, uint32_t(r.seq.size()));
+    /* Two-slot pipeline: the host gathers and packs the next batch and
+     * formats the finished one while the device queries. */
+    void pipeline(const std::function<bool(bbt_fe::RecBatch &)> &next, bool paired) {
+        bbt_fe::RecBatch recs[2];
+        int cur = 0;
+        bool more = next(recs[0]);
+        pack(0, recs[0]);
+        submit(0, paired);
+        while (true) {
+            const int nx = 1 - cur;
+            recs[nx].clear();
+            if (more)
+                more = next(recs[nx]);
+            pack(nx, recs[nx]);
+            if (hipStreamSynchronize(m_st) != hipSuccess)
+                Slot::die("query failed");
+            emit(cur, recs[cur], paired);
+            recs[cur].clear();
+            if (recs[nx].v.empty())
+                break;
+            submit(nx, paired);
+            cur = nx;
         }
-        const size_t nreads = recs.size(), nrec = paired ? nreads / 2 : nreads;
-        Device &D = m_dev;
-        const size_t padded = (seq.size() + 7) / 8 * 8 + 8;
-        D.reserve(padded, nreads);
-        seq.resize(padded, '\0');
-        uint64_t zero[4] = {0, 0, 0, 0};
-        if (hipMemcpyAsync(D.dseq, seq.data(), padded, hipMemcpyHostToDevice, D.st) !=
+    }
+
+    void pack(int si, const bbt_fe::RecBatch &b) {
+        Slot &S = m_slot[si];
+        const size_t n = b.v.size();
+        S.nreads = n;
+        if (!n)
+            return;
+        size_t bytes = 0;
+        uint32_t max_len = 0;
+        for (const auto &v : b.v) {
+            bytes += v.seq_len;
+            max_len = std::max(max_len, v.seq_len);
+        }
+        const size_t padded = (bytes + 7) / 8 * 8 + 8;
+        S.reserve(padded, n);
+        uint64_t o = 0;
+        for (size_t i = 0; i < n; ++i) {
+            S.hoff[i] = o;
+            o += b.v[i].seq_len;
+        }
+        S.hoff[n] = o;
+        const unsigned T = o < (size_t(8) << 20) ? 1u : 8u;
+        const auto gather = [&](unsigned t) {
+            for (size_t i = n * t / T, e = n * (t + 1) / T; i < e; ++i)
+                memcpy(S.hseq + S.hoff[i], b.v[i].seq, b.v[i].seq_len);
+        };
+        vector<std::thread> th;
+        for (unsigned t = 1; t < T; ++t)
+            th.emplace_back(gather, t);
+        gather(0);
+        for (auto &t : th)
+            t.join();
+        memset(S.hseq + o, 0, padded - o);
+        S.bytes = padded;
+        S.max_len = max_len;
+    }
+
+    void submit(int si, bool paired) {
+        Slot &S = m_slot[si];
+        const size_t n = S.nreads, nrec = paired ? n / 2 : n;
+        if (!n)
+            return;
+        memset(S.hctr, 0, 32);
+        if (hipMemcpyAsync(S.dseq, S.hseq, S.bytes, hipMemcpyHostToDevice, m_st) != hipSuccess ||
+            hipMemcpyAsync(S.doff, S.hoff, (n + 1) * 8, hipMemcpyHostToDevice, m_st) !=
                 hipSuccess ||
-            hipMemcpyAsync(D.doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, D.st) !=
-                hipSuccess ||
-            hipMemcpyAsync(D.dctr, zero, 32, hipMemcpyHostToDevice, D.st) != hipSuccess)
-            Device::die("upload failed");
+            hipMemcpyAsync(S.dctr, S.hctr, 32, hipMemcpyHostToDevice, m_st) != hipSuccess)
+            Slot::die("upload failed");
         bbt_reads in;
         std::memset(&in, 0, sizeof(in));
-        in.seq = D.dseq;
-        in.seq_bytes = padded;
-        in.off = D.doff;
-        in.nreads = nreads;
-        in.max_len = max_len;
+        in.seq = S.dseq;
+        in.seq_bytes = S.bytes;
+        in.off = S.doff;
+        in.nreads = n;
+        in.max_len = S.max_len;
         bbt_mibf_params p;
         std::memset(&p, 0, sizeof(p));
         p.score = m_o.score;
@@ -226,69 +334,95 @@ class MIBFClassifier {
         p.min_nonsat = m_o.minNoSat;
         p.max_hits = kMaxHits;
         p.flags = (paired ? BBT_MIBF_PAIRED : 0) | (m_o.bestAgree ? BBT_MIBF_BEST_AGREE : 0);
-        bbt_mibf_out out{D.dnh, D.dhits, D.dsat, D.dctr};
-        if (bbt_mibf_query_batch(m_mf, &in, &p, &out, D.st) != BBT_OK) {
+        bbt_mibf_out out{S.dnh, S.dhits, S.dsat, S.dctr};
+        if (bbt_mibf_query_batch(m_mf, &in, &p, &out, m_st) != BBT_OK) {
             cerr << "Error: " << bbt_last_error() << endl;
             exit(1);
         }
-        D.nh.resize(nrec);
-        D.hits.resize(nrec * kMaxHits * 7);
-        D.sat.resize(nrec);
-        uint64_t ctr[4];
-        if (hipMemcpyAsync(D.nh.data(), D.dnh, nrec * 4, hipMemcpyDeviceToHost, D.st) !=
-                hipSuccess ||
-            hipMemcpyAsync(D.hits.data(), D.dhits, nrec * kMaxHits * 7 * 4,
-                           hipMemcpyDeviceToHost, D.st) != hipSuccess ||
-            hipMemcpyAsync(D.sat.data(), D.dsat, nrec * 4, hipMemcpyDeviceToHost, D.st) !=
-                hipSuccess ||
-            hipMemcpyAsync(ctr, D.dctr, 32, hipMemcpyDeviceToHost, D.st) != hipSuccess ||
-            hipStreamSynchronize(D.st) != hipSuccess)
-            Device::die("download failed");
-        if (ctr[2] && !m_warned) {
+        if (hipMemcpyAsync(S.hnh, S.dnh, nrec * 4, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
+            hipMemcpyAsync(S.hhits, S.dhits, nrec * kMaxHits * 7 * 4, hipMemcpyDeviceToHost,
+                           m_st) != hipSuccess ||
+            hipMemcpyAsync(S.hsat, S.dsat, nrec * 4, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
+            hipMemcpyAsync(S.hctr, S.dctr, 32, hipMemcpyDeviceToHost, m_st) != hipSuccess)
+            Slot::die("download failed");
+    }
+
+    /* the output lines and summary counts of a finished batch: records in
+     * ranges formatted on threads, then written in input order */
+    void emit(int si, const bbt_fe::RecBatch &b, bool paired) {
+        Slot &S = m_slot[si];
+        const size_t n = S.nreads, nrec = paired ? n / 2 : n;
+        if (!n)
+            return;
+        if (S.hctr[2] && !m_warned) {
             cerr << "Warning: reads with more than " << kMaxHits
                  << " reported candidates; the output lists the first " << kMaxHits << endl;
             m_warned = true;
         }
-        string outStr;
-        for (size_t r = 0; r < nrec; ++r) {
-            const uint32_t n = std::min(D.nh[r], kMaxHits);
-            const uint32_t *h = D.hits.data() + r * kMaxHits * 7;
-            if (!paired) {
-                /* filterSingleRead (MIBFClassifier.hpp:738-751) */
-                update_summary(h, n);
-                if (m_o.hitOnly && n == 0)
-                    continue;
-                outStr.clear();
-                format(recs[r], D.sat[r], h, n, outStr);
-            } else {
-                /* filterPairedRead (:753-771): hitOnly returns before the
-                 * summary update */
-                if (m_o.hitOnly && n == 0)
-                    continue;
-                update_summary(h, n);
-                outStr.clear();
-                format(recs[2 * r], D.sat[r], h, n, outStr);
-                format(recs[2 * r + 1], D.sat[r], h, n, outStr);
+        const unsigned T = nrec < 4096 ? 1u : 8u;
+        struct Part {
+            string out;
+            vector<uint64_t> above, unique;
+            uint64_t multi = 0, noMatch = 0;
+        };
+        vector<Part> parts(T);
+        const auto work = [&](unsigned t) {
+            Part &P = parts[t];
+            P.above.assign(m_info.nids + 1, 0);
+            P.unique.assign(m_info.nids + 1, 0);
+            for (size_t r = nrec * t / T, e = nrec * (t + 1) / T; r < e; ++r) {
+                const uint32_t nh = std::min(S.hnh[r], kMaxHits);
+                const uint32_t *h = S.hhits + r * kMaxHits * 7;
+                if (!paired) {
+                    /* filterSingleRead (MIBFClassifier.hpp:738-751) */
+                    count(P, h, nh);
+                    if (m_o.hitOnly && nh == 0)
+                        continue;
+                    format(b.v[r], S.hsat[r], h, nh, P.out);
+                } else {
+                    /* filterPairedRead (:753-771): hitOnly returns before the
+                     * summary update */
+                    if (m_o.hitOnly && nh == 0)
+                        continue;
+                    count(P, h, nh);
+                    format(b.v[2 * r], S.hsat[r], h, nh, P.out);
+                    format(b.v[2 * r + 1], S.hsat[r], h, nh, P.out);
+                }
             }
-            cout << outStr;
+        };
+        vector<std::thread> th;
+        for (unsigned t = 1; t < T; ++t)
+            th.emplace_back(work, t);
+        work(0);
+        for (auto &t : th)
+            t.join();
+        for (const Part &P : parts) {
+            cout.write(P.out.data(), std::streamsize(P.out.size()));
+            for (uint32_t i = 0; i <= m_info.nids; ++i) {
+                m_above[i] += P.above[i];
+                m_unique[i] += P.unique[i];
+            }
+            m_multi += P.multi;
+            m_noMatch += P.noMatch;
         }
     }
 
     /* ResultsManager::updateSummaryData over QueryResults (ResultsManager.hpp:40-64) */
-    void update_summary(const uint32_t *h, uint32_t n) {
+    template <class P>
+    static void count(P &p, const uint32_t *h, uint32_t n) {
         for (uint32_t i = 0; i < n; ++i)
-            ++m_above[h[i * 7]];
+            ++p.above[h[i * 7]];
         if (n == 0)
-            ++m_noMatch;
+            ++p.noMatch;
         else if (n == 1)
-            ++m_unique[h[0]];
+            ++p.unique[h[0]];
         else
-            ++m_multi;
+            ++p.multi;
     }
 
     /* appendResults (MIBFClassifier.hpp:652-673): name, nonSatFrameCount,
      * solidCount, count, nonSatCount, totalNonSatCount, totalCount */
-    void append(const uint32_t *h, uint32_t n, string &o) {
+    void append(const uint32_t *h, uint32_t n, string &o) const {
         o += "\t";
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t *c = h + i * 7;
@@ -301,9 +435,13 @@ class MIBFClassifier {
     }
 
     /* formatOutStr (MIBFClassifier.hpp:675-736) */
-    void format(const bbt_fe::SeqRecord &rd, uint32_t sat, const uint32_t *h, uint32_t n,
-                string &o) {
-        const string head = rd.name + (rd.comment.empty() ? "" : " " + rd.comment);
+    void format(const bbt_fe::SeqView &rd, uint32_t sat, const uint32_t *h, uint32_t n,
+                string &o) const {
+        string head(rd.name, rd.name_len);
+        if (rd.comment_len) {
+            head += ' ';
+            head.append(rd.comment, rd.comment_len);
+        }
         if (m_o.out == FASTA || m_o.out == FASTQ) {
             o += m_o.out == FASTA ? ">" : "@";
             o += head;
@@ -311,9 +449,12 @@ class MIBFClassifier {
                 o += "\t" + m_names[h[0]];
                 append(h, n, o);
             }
-            o += "\n" + rd.seq;
-            if (m_o.out == FASTQ)
-                o += "\n+\n" + rd.qual;
+            o += '\n';
+            o.append(rd.seq, rd.seq_len);
+            if (m_o.out == FASTQ) {
+                o += "\n+\n";
+                o.append(rd.qual, rd.qual_len);
+            }
         } else if (n == 0) {
             o += "*\t" + head + "\t" + to_string(sat) + "\t*";
         } else {
@@ -324,6 +465,10 @@ class MIBFClassifier {
     }
 
     void finish() {
+        cout.flush();
+        cerr << "Filtering time: "
+             << std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count()
+             << " s" << endl;
         cerr << "Total Reads:" << m_numRead << endl;
         cerr << "Writing file: " << m_o.prefix << "_summary.tsv" << endl;
         vector<uint64_t> sm(2 * (m_info.nids + 1) + 2);
@@ -348,7 +493,9 @@ class MIBFClassifier {
     bbt_mibf_info_t m_info{};
     vector<string> m_names;
     uint32_t m_allowedMiss = 0;
-    Device m_dev;
+    hipStream_t m_st = nullptr;
+    Slot m_slot[2];
+    std::chrono::steady_clock::time_point m_t0;
     uint64_t m_numRead = 0, m_multi = 0, m_noMatch = 0;
     vector<uint64_t> m_above, m_unique;
     bool m_warned = false;
