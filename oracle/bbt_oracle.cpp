@@ -21,6 +21,10 @@
  *   Binomial: Loader's saddle-point pmf; pinned against scipy.stats.binom
  *     (which wraps Boost.Math); Boost's edge rounding of the quantile is
  *     "parity unpinned".
+ *   Subtract filters (evalRead's subtract argument, loadFilterSubtract) and
+ *     the progressive builders' one-thread loop: pinned by the independent
+ *     SeqEval model fed both filters' answers, per-window composition of the
+ *     pinned primitives, closed forms and the recruitment-order checks.
  */
 #include "bbt_oracle.h"
 
