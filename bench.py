@@ -361,7 +361,11 @@ def bench_e2e(args, torch, D, B):
     want = fs.summary_text(res["summary"], npairs)
     del dr
     cli = os.path.join(ROOT, "biobloom_amd", "biobloomcategorizer")
-    tmp = tempfile.mkdtemp(prefix="bbt_e2e_", dir=os.environ.get("TMPDIR", "/tmp"))
+    if args.e2e_keep:
+        tmp = os.path.abspath(args.e2e_keep)
+        os.makedirs(tmp, exist_ok=True)
+    else:
+        tmp = tempfile.mkdtemp(prefix="bbt_e2e_", dir=os.environ.get("TMPDIR", "/tmp"))
     bfs = []
     for i in range(len(fs)):
         inf = fs.info(i)
@@ -380,7 +384,8 @@ def bench_e2e(args, torch, D, B):
             f"disk ({time.time() - t:.1f}s)")
         for threads in args.e2e_threads:
             prefix = os.path.join(tmp, f"out_{ext}_{threads}")
-            cmd = [cli, "-e", "-p", prefix, "-f", " ".join(bfs), "-t", str(threads), f1, f2]
+            cmd = ([cli, "-e", "-p", prefix, "-f", " ".join(bfs), "-t", str(threads)]
+                   + args.e2e_args.split() + [f1, f2])
             t = time.time()
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
             wall = time.time() - t
@@ -397,8 +402,11 @@ def bench_e2e(args, torch, D, B):
             log(f"e2e {ext} -t {threads}: filtering {ft:.2f} s (wall {wall:.2f} s), "
                 f"{item['reads_per_s'] / 1e6:.2f} M reads/s, {item['text_gb_per_s']:.2f} GB/s "
                 f"of FASTQ, summary equal={item['summary_equal']}")
-        os.remove(f1)
-        os.remove(f2)
+        if args.e2e_keep:
+            log("kept: " + " ".join(cmd))
+        else:
+            os.remove(f1)
+            os.remove(f2)
     best = max((r for r in runs if r["input"] == "fastq"), key=lambda r: r["reads_per_s"])
     line = {
         "metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
@@ -413,8 +421,9 @@ def bench_e2e(args, torch, D, B):
         "summary_equal": all(r["summary_equal"] for r in runs),
     }
     print(json.dumps(line), flush=True)
-    import shutil
-    shutil.rmtree(tmp, ignore_errors=True)
+    if not args.e2e_keep:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def bench_mibf_e2e(args, torch, D, B):
@@ -488,6 +497,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi",
                     choices=["multi", "single", "uneven", "mibf", "serve", "e2e", "mibf-e2e"])
+    ap.add_argument("--e2e-args", default="",
+                    help="e2e: extra biobloomcategorizer options, e.g. --batch_reads=524288")
+    ap.add_argument("--e2e-keep", default=None, metavar="DIR",
+                    help="e2e: write the filters and FASTQ files into DIR and keep them "
+                         "(to run the CLI again, e.g. under a profiler)")
     ap.add_argument("--e2e-threads", type=lambda v: [int(x) for x in v.split(",")],
                     default=[16, 4, 1], help="e2e: CLI parse-thread counts to run")
     ap.add_argument("--e2e-gz", action="store_true", help="e2e: also gzip inputs")

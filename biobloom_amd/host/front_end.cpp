@@ -48,6 +48,8 @@ static void bbt_check(bbt_status s) {
  * the device works on the other; both add to the same summary counters. */
 struct BioBloomClassifier::Batch {
     hipStream_t st = nullptr;
+    hipEvent_t walked = nullptr; /* recorded after the slot's kernels */
+    bool walk_recorded = false;
     bbt_workspace *ws = nullptr;
     uint32_t nf = 0;
     bool scores = false;
@@ -130,6 +132,7 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
         b->nf = m_nf;
         b->scores = m_opt.mode == BBT_BESTHIT || m_opt.mode == BBT_SCORES;
         HIP_CHECK(hipStreamCreate(&b->st));
+        HIP_CHECK(hipEventCreateWithFlags(&b->walked, hipEventDisableTiming));
         bbt_check(bbt_workspace_create(&b->ws));
     }
     HIP_CHECK(hipMalloc(&m_dsummary, (2 * m_nf + 2) * 8));
@@ -144,6 +147,7 @@ BioBloomClassifier::~BioBloomClassifier() {
             continue;
         (void)hipStreamSynchronize(b->st);
         b->release();
+        (void)hipEventDestroy(b->walked);
         bbt_workspace_free(b->ws);
         (void)hipStreamDestroy(b->st);
         delete b;
@@ -166,7 +170,7 @@ void BioBloomClassifier::setOrderedFilter() {
 
 /* Packs recs into slot s's pinned staging (the device may still be working
  * on the other slot). */
-void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired) {
+void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired, uint64_t scale) {
     Batch &b = *m_slot[s];
     const uint64_t n = recs.v.size();
     b.n = n;
@@ -178,7 +182,12 @@ void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired) {
         bytes += r.seq_len;
         maxlen = std::max<uint64_t>(maxlen, r.seq_len);
     }
-    b.reserve(n, bytes);
+    /* reserve for scale times this batch (the full batch size while the
+     * batches ramp up): growing the pinned and device buffers synchronizes
+     * the device */
+    const auto t0 = std::chrono::steady_clock::now();
+    b.reserve(n * scale, bytes * scale);
+    m_phase[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     uint64_t o = 0;
     for (uint64_t i = 0; i < n; ++i) {
         b.h_off[i] = o;
@@ -201,9 +210,12 @@ void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired) {
 }
 
 /* Enqueues copy-in, classification and copy-out of slot s; returns without
- * waiting. */
+ * waiting.  The copy-in overlaps the other slot's kernels; the kernels start
+ * after the other slot's, so launches run one at a time in submission order
+ * and each keeps the whole device (and its L2) to itself. */
 void BioBloomClassifier::submit(int s) {
     Batch &b = *m_slot[s];
+    const Batch &other = *m_slot[1 - s];
     const uint64_t n = b.n;
     const bool paired = b.paired;
     if (n == 0)
@@ -211,6 +223,8 @@ void BioBloomClassifier::submit(int s) {
     const uint64_t o = b.h_off[n];
     HIP_CHECK(hipMemcpyAsync(b.d_seq, b.h_seq, (o + 7) / 8 * 8, hipMemcpyHostToDevice, b.st));
     HIP_CHECK(hipMemcpyAsync(b.d_off, b.h_off, (n + 1) * 8, hipMemcpyHostToDevice, b.st));
+    if (other.walk_recorded)
+        HIP_CHECK(hipStreamWaitEvent(b.st, other.walked, 0));
     bbt_reads in;
     std::memset(&in, 0, sizeof(in));
     in.seq = b.d_seq;
@@ -238,6 +252,8 @@ void BioBloomClassifier::submit(int s) {
     out.summary = m_dsummary;
     out.counters = m_dctr;
     bbt_check(bbt_eval_batch_ws(m_fs, b.ws, &in, &p, &out, b.st));
+    HIP_CHECK(hipEventRecord(b.walked, b.st));
+    b.walk_recorded = true;
     const uint64_t nrec = paired ? n / 2 : n;
     HIP_CHECK(hipMemcpyAsync(b.h_cls, b.d_cls, nrec * 4, hipMemcpyDeviceToHost, b.st));
     HIP_CHECK(hipMemcpyAsync(b.h_hits, b.d_hits, n * m_nf, hipMemcpyDeviceToHost, b.st));
@@ -252,53 +268,71 @@ void BioBloomClassifier::finish(int s) {
         HIP_CHECK(hipStreamSynchronize(m_slot[s]->st));
 }
 
-/* Blocks one batch may hold: two batches in flight stay well inside the
- * reader's pool, so the reader can always parse ahead. */
-constexpr size_t kBatchBlocks = ParallelSeqReader::kMaxBlocks / 2 - 2;
+/* Blocks one batch may hold: the three batches alive at once (one on the
+ * device, one being packed, one being gathered) stay well inside the reader's
+ * pool, so the reader can always parse ahead. */
+constexpr size_t kBatchBlocks = (ParallelSeqReader::kMaxBlocks - 4) / 3;
 
-/* Two-slot pipeline over a batch source: next(recs) fills the next batch and
- * returns false once the input is exhausted (recs may still hold records).
- * While the device classifies one batch, the host gathers the next one
- * (parsing runs ahead on the reader's threads) and then emits the finished
- * batch, so outputs keep input order. */
+/* Pipeline over a batch source: next(recs) fills the next batch and returns
+ * false once the input is exhausted (recs may still hold records).  While the
+ * device classifies batch i, a helper thread packs and submits batch i+1 and
+ * this thread gathers batch i+2 (parsing runs ahead on the reader's threads);
+ * batch i is then emitted, so outputs keep input order and the device always
+ * has the next launch queued.  The first batch is an eighth of the batch size
+ * and each next one twice the last, so the device starts while the rest of
+ * the input is still being parsed. */
 void BioBloomClassifier::pipeline(const std::function<bool(RecBatch &)> &next, bool paired,
                                   const std::string &type) {
     using clk = std::chrono::steady_clock;
     const auto since = [](clk::time_point t) {
         return std::chrono::duration<double>(clk::now() - t).count();
     };
-    RecBatch recs[2];
-    int cur = 0;
+    RecBatch recs[3];
+    uint64_t scale[3] = {1, 1, 1};
+    bool more = true;
+    uint64_t limit = std::max<uint64_t>(1, m_opt.batch_reads / 8);
+    /* gathers the next batch into recs[k] (this thread only) */
+    const auto gather = [&](int k) {
+        const clk::time_point t = clk::now();
+        recs[k].clear();
+        m_batch_limit = limit;
+        /* while the batches ramp up, pack() reserves for the full size */
+        scale[k] = std::max<uint64_t>(1, m_opt.batch_reads / limit);
+        if (more)
+            more = next(recs[k]);
+        limit = std::min(m_opt.batch_reads, 2 * limit);
+        m_phase[0] += since(t);
+    };
+    gather(0);
     clk::time_point t = clk::now();
-    bool more = next(recs[0]);
-    m_phase[0] += since(t);
-    t = clk::now();
-    pack(0, recs[0], paired);
+    pack(0, recs[0], paired, scale[0]);
     submit(0);
     m_phase[1] += since(t);
-    while (true) {
-        const int nx = 1 - cur;
+    gather(1);
+    for (uint64_t i = 0;; ++i) {
+        /* batch i is on the device (slot i % 2); batch i - 1 has been emitted,
+         * so slot (i + 1) % 2 and recs[(i + 2) % 3] are free */
+        const int b0 = int(i % 3), b1 = int((i + 1) % 3), b2 = int((i + 2) % 3);
+        const int s0 = int(i % 2), s1 = int((i + 1) % 2);
+        double pack_s = 0;
+        std::thread packer([&] {
+            const clk::time_point tp = clk::now();
+            pack(s1, recs[b1], paired, scale[b1]);
+            submit(s1);
+            pack_s = since(tp);
+        });
+        gather(b2);
+        packer.join();
+        m_phase[1] += pack_s;
         t = clk::now();
-        recs[nx].clear();
-        if (more)
-            more = next(recs[nx]);
-        m_phase[0] += since(t);
-        t = clk::now();
-        pack(nx, recs[nx], paired);
-        m_phase[1] += since(t);
-        t = clk::now();
-        finish(cur);
+        finish(s0);
         m_phase[2] += since(t);
         t = clk::now();
-        emit_batch(cur, recs[cur], paired, type);
-        recs[cur].clear(); /* releases its blocks to the reader */
+        emit_batch(s0, recs[b0], paired, type);
+        recs[b0].clear(); /* releases its blocks to the reader */
         m_phase[3] += since(t);
-        if (recs[nx].v.empty())
+        if (recs[b1].v.empty())
             break;
-        t = clk::now();
-        submit(nx);
-        m_phase[1] += since(t);
-        cur = nx;
     }
 }
 
@@ -451,7 +485,8 @@ void BioBloomClassifier::write_summary(uint64_t totalReads) {
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
     std::cerr << "Filtering time: " << secs << " s (gather " << m_phase[0] << ", pack+launch "
-              << m_phase[1] << ", device wait " << m_phase[2] << ", output " << m_phase[3]
+              << m_phase[1] << " incl. staging allocation " << m_phase[4] << ", device wait "
+              << m_phase[2] << ", output " << m_phase[3]
               << ")" << std::endl;
     m_probes = ctr[0];
     m_lookups = ctr[1];
@@ -513,7 +548,7 @@ void BioBloomClassifier::filterPrint(const std::vector<std::string> &inputFiles,
         pipeline(
             [&](RecBatch &batch) {
                 bool more = true;
-                while (batch.v.size() < m_opt.batch_reads &&
+                while (batch.v.size() < m_batch_limit &&
                        batch.holds.size() < kBatchBlocks) {
                     /* stops at end of file, or a truncated record as kseq */
                     const SeqView *v = cur.next();
@@ -553,7 +588,7 @@ void BioBloomClassifier::pair_loop(const std::string &f1, const std::string &f2,
     pipeline(
         [&](RecBatch &batch) {
             bool more = true;
-            while (batch.v.size() < 2 * m_opt.batch_reads &&
+            while (batch.v.size() < 2 * m_batch_limit &&
                    batch.holds.size() < kBatchBlocks) {
                 /* the loop stops when either file ends (BioBloomClassifier.cpp:269, 291) */
                 const SeqView *a = c1.next();
@@ -590,7 +625,7 @@ void BioBloomClassifier::smart_pair_loop(const std::string &file, const std::str
     pipeline(
         [&](RecBatch &batch) {
             bool more = true;
-            while (batch.v.size() < 2 * m_opt.batch_reads &&
+            while (batch.v.size() < 2 * m_batch_limit &&
                    batch.holds.size() < kBatchBlocks) {
                 const SeqView *v = cur.next();
                 if (!v) {

@@ -89,7 +89,7 @@ class BioBloomClassifier {
 
   private:
     struct Batch;
-    void pack(int slot, const RecBatch &recs, bool paired);
+    void pack(int slot, const RecBatch &recs, bool paired, uint64_t scale);
     void submit(int slot);
     void finish(int slot);
     void pipeline(const std::function<bool(RecBatch &)> &next, bool paired,
@@ -111,6 +111,7 @@ class BioBloomClassifier {
     Options m_opt;
     bool m_stdout = false;
     Batch *m_slot[2] = {nullptr, nullptr};
+    uint64_t m_batch_limit = 1; /* reads (or pairs) the batch being gathered may take */
     uint64_t *m_dsummary = nullptr, *m_dctr = nullptr;
     uint64_t m_probes = 0, m_lookups = 0;
     uint64_t m_pairs = 0;
@@ -118,7 +119,9 @@ class BioBloomClassifier {
     bool m_gz = false;
     void *m_status = nullptr;
     std::chrono::steady_clock::time_point m_t0; /* "Filtering Start" */
-    double m_phase[4] = {0, 0, 0, 0}; /* s: gather, pack + launch, device wait, output */
+    /* s: gather, pack + launch, device wait, output; staging allocation (part
+     * of pack + launch) */
+    double m_phase[5] = {0, 0, 0, 0, 0};
 };
 
 } // namespace bbt_fe

@@ -133,7 +133,7 @@ class ParallelSeqReader {
     using BlockRef = std::shared_ptr<SeqBlock>;
     /* parsed blocks alive at once, queued ahead or held by the caller: a
      * caller that holds this many blocks waits forever for the next one */
-    static constexpr unsigned kMaxBlocks = 16;
+    static constexpr unsigned kMaxBlocks = 48;
     /* threads = parse threads (0: the hardware's, at most 16) */
     explicit ParallelSeqReader(const std::string &path, unsigned threads = 0,
                                size_t block_bytes = size_t(16) << 20);
