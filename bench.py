@@ -455,8 +455,9 @@ def bench_mibf_e2e(args, torch, D, B):
         wall = time.time() - t
         if r.returncode != 0:
             raise SystemExit(f"biobloommicategorizer failed: {r.stderr[-2000:]}")
-        ft = [float(x.split()[2]) for x in r.stderr.splitlines()
-              if x.startswith("Filtering time:")][0]
+        tl = [x for x in r.stderr.splitlines() if x.startswith("Filtering time:")][0]
+        log(tl)
+        ft = float(tl.split()[2])
         lines = sum(1 for _ in open(out))
         item = {"threads": threads, "filtering_s": ft, "wall_s": wall,
                 "reads_per_s": nreads / ft, "text_gb_per_s": nbytes / ft / 1e9,

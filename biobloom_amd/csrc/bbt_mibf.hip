@@ -361,12 +361,22 @@ struct QueryArgs {
     uint32_t *nhits;  /* per record */
     uint32_t *hits;   /* per record max_hits x kHitWords */
     uint32_t *sat;    /* per record */
-    unsigned long long *counters; /* bv lookups, id gathers, overflow, not evaluated */
+    unsigned long long *counters; /* bv lookups, id gathers, overflow, wide records */
     unsigned long long *queue;
+    /* records with more than kMaxCand candidates: the first pass lists them
+     * (wide[0] = how many, wide_list the record indices) and the wide pass
+     * evaluates them again with the candidates past kMaxCand in spill
+     * (spill_stride candidates per workgroup) */
+    unsigned long long *wide;
+    unsigned long long *wide_queue;
+    uint32_t *wide_list;
+    uint32_t *spill;
+    uint32_t spill_stride;
 };
 
 constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
 constexpr uint32_t kMaxCand = 64;
+constexpr uint32_t kWideBlocks = 64; /* workgroups of the wide pass */
 /* up to this many IDs (with the repeat ID 0), per-ID totals are accumulated
  * directly in LDS, six 32-bit counters per ID, instead of sorting the frame
  * records; such a filter takes records of any length */
@@ -395,8 +405,12 @@ __device__ __forceinline__ bool ranks_before(const uint32_t *x, const uint32_t *
  * window hashes come from prefix-XOR scans, every lane resolves its frame's
  * S slots (bit + rank from one line, then the ID), and the distinct IDs of
  * each frame are appended to an LDS store that is sorted by ID and reduced
- * per ID at the end of the record. */
-template <uint32_t CAP, uint32_t SMAX, bool DIRECT>
+ * per ID at the end of the record.  A record with more than kMaxCand
+ * candidates is listed for the WIDE pass, which walks the listed records
+ * again with room for every ID as a candidate and ranks them by selection;
+ * the WIDE pass adds nothing to the lookup counters (the first pass counted
+ * the record's lookups). */
+template <uint32_t CAP, uint32_t SMAX, bool DIRECT, bool WIDE>
 __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
     __shared__ uint64_t rec[DIRECT ? 1 : CAP];
     __shared__ uint32_t cand[kMaxCand * kHitWords];
@@ -415,11 +429,11 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
     while (true) {
         unsigned long long it = 0;
         if (lane == 0)
-            it = atomicAdd(a.queue, 1ull);
+            it = atomicAdd(WIDE ? a.wide_queue : a.queue, 1ull);
         it = __shfl(it, 0);
-        if (it >= a.nrec)
+        if (it >= (WIDE ? *a.wide : a.nrec))
             break;
-        const uint64_t r = it;
+        const uint64_t r = WIDE ? uint64_t(a.wide_list[it]) : it;
         const uint64_t first = a.paired ? 2 * r : r;
         const uint32_t nmates = a.paired ? 2 : 1;
         /* frameCount: len - k + 1; for a pair, the nesting of
@@ -638,8 +652,11 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                                         : 0xFFFFFFFFu;
                 if (cnt >= mc && nsat >= a.min_nonsat) {
                     const uint32_t slot = atomicAdd(&ncand_s, 1u);
-                    if (slot < kMaxCand) {
-                        uint32_t *c7 = cand + slot * kHitWords;
+                    if (slot < kMaxCand || (WIDE && slot - kMaxCand < a.spill_stride)) {
+                        uint32_t *c7 = slot < kMaxCand
+                                           ? cand + slot * kHitWords
+                                           : a.spill + (uint64_t(blockIdx.x) * a.spill_stride +
+                                                        (slot - kMaxCand)) * kHitWords;
                         c7[0] = id;
                         c7[1] = cnt;
                         c7[2] = nsat;
@@ -705,8 +722,11 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                                     : 0xFFFFFFFFu;
             if (cnt >= mc && nsat >= a.min_nonsat) {
                 const uint32_t slot = atomicAdd(&ncand_s, 1u);
-                if (slot < kMaxCand) {
-                    uint32_t *c7 = cand + slot * kHitWords;
+                if (slot < kMaxCand || (WIDE && slot - kMaxCand < a.spill_stride)) {
+                    uint32_t *c7 = slot < kMaxCand
+                                       ? cand + slot * kHitWords
+                                       : a.spill + (uint64_t(blockIdx.x) * a.spill_stride +
+                                                    (slot - kMaxCand)) * kHitWords;
                     c7[0] = id;
                     c7[1] = cnt;
                     c7[2] = nsat;
@@ -718,51 +738,103 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
             }
         }
         __syncthreads();
-        /* ranking and output by lane 0 (a handful of candidates) */
+        /* ranking and output by lane 0 */
         if (lane == 0) {
-            uint32_t nc = ncand_s;
-            if (nc > kMaxCand) {
-                over = true;
-                nc = 0;
-            }
-            uint32_t order[kMaxCand];
-            for (uint32_t x = 0; x < nc; ++x) {
-                uint32_t y = x;
-                while (y > 0 && ranks_before(cand + x * kHitWords, cand + order[y - 1] * kHitWords)) {
-                    order[y] = order[y - 1];
-                    --y;
-                }
-                order[y] = x;
-            }
-            if (nc && a.best_agree) {
-                const uint32_t *t = cand + order[0] * kHitWords;
-                for (uint32_t x = 0; x < nc; ++x)
-                    for (uint32_t f = 1; f < kHitWords; ++f)
-                        if (cand[x * kHitWords + f] > t[f])
-                            nc = 0;
-            }
+            const uint32_t ncand = ncand_s;
             uint32_t nout = 0;
-            if (nc) {
-                const double bar = a.multi_thresh * double(cand[order[0] * kHitWords + 2]);
-                for (uint32_t x = 0; x < nc; ++x) {
-                    const uint32_t *cv = cand + order[x] * kHitWords;
-                    if (double(cv[2]) < bar)
-                        break;
-                    if (nout < a.max_hits)
+            bool listed = false;
+            if (over) {
+            } else if (ncand > kMaxCand && !WIDE) {
+                /* evaluated again by the wide pass, which writes the result */
+                const unsigned long long w = atomicAdd(a.wide, 1ull);
+                a.wide_list[w] = uint32_t(r);
+                listed = true;
+                atomicAdd(a.counters + 3, 1ull);
+            } else if (ncand > kMaxCand) {
+                /* wide pass: best first by selection over every candidate */
+                const auto C = [&](uint32_t x) -> const uint32_t * {
+                    return x < kMaxCand ? cand + x * kHitWords
+                                        : a.spill + (uint64_t(blockIdx.x) * a.spill_stride +
+                                                     (x - kMaxCand)) * kHitWords;
+                };
+                const uint32_t nc = ncand < kMaxCand + a.spill_stride ? ncand
+                                                                      : kMaxCand + a.spill_stride;
+                const uint32_t *top = C(0);
+                for (uint32_t x = 1; x < nc; ++x)
+                    if (ranks_before(C(x), top))
+                        top = C(x);
+                bool agree = true;
+                if (a.best_agree)
+                    for (uint32_t x = 0; x < nc; ++x)
+                        for (uint32_t f = 1; f < kHitWords; ++f)
+                            if (C(x)[f] > top[f])
+                                agree = false;
+                if (agree) {
+                    const double bar = a.multi_thresh * double(top[2]);
+                    uint32_t npass = 0;
+                    for (uint32_t x = 0; x < nc; ++x)
+                        npass += double(C(x)[2]) >= bar ? 1 : 0;
+                    const uint32_t nkeep = npass < a.max_hits ? npass : a.max_hits;
+                    const uint32_t *prev = nullptr;
+                    for (uint32_t j = 0; j < nkeep; ++j) {
+                        const uint32_t *sel = nullptr;
+                        for (uint32_t x = 0; x < nc; ++x) {
+                            const uint32_t *cv = C(x);
+                            if (double(cv[2]) < bar || (prev && !ranks_before(prev, cv)))
+                                continue;
+                            if (!sel || ranks_before(cv, sel))
+                                sel = cv;
+                        }
                         for (uint32_t f = 0; f < kHitWords; ++f)
-                            a.hits[(r * a.max_hits + nout) * kHitWords + f] = cv[f];
-                    else
-                        atomicAdd(a.counters + 2, 1ull);
-                    ++nout;
+                            a.hits[(r * a.max_hits + j) * kHitWords + f] = sel[f];
+                        prev = sel;
+                    }
+                    if (npass > nkeep)
+                        atomicAdd(a.counters + 2, (unsigned long long)(npass - nkeep));
+                    nout = npass;
+                }
+            } else {
+                uint32_t nc = ncand;
+                uint32_t order[kMaxCand];
+                for (uint32_t x = 0; x < nc; ++x) {
+                    uint32_t y = x;
+                    while (y > 0 &&
+                           ranks_before(cand + x * kHitWords, cand + order[y - 1] * kHitWords)) {
+                        order[y] = order[y - 1];
+                        --y;
+                    }
+                    order[y] = x;
+                }
+                if (nc && a.best_agree) {
+                    const uint32_t *t = cand + order[0] * kHitWords;
+                    for (uint32_t x = 0; x < nc; ++x)
+                        for (uint32_t f = 1; f < kHitWords; ++f)
+                            if (cand[x * kHitWords + f] > t[f])
+                                nc = 0;
+                }
+                if (nc) {
+                    const double bar = a.multi_thresh * double(cand[order[0] * kHitWords + 2]);
+                    for (uint32_t x = 0; x < nc; ++x) {
+                        const uint32_t *cv = cand + order[x] * kHitWords;
+                        if (double(cv[2]) < bar)
+                            break;
+                        if (nout < a.max_hits)
+                            for (uint32_t f = 0; f < kHitWords; ++f)
+                                a.hits[(r * a.max_hits + nout) * kHitWords + f] = cv[f];
+                        else
+                            atomicAdd(a.counters + 2, 1ull);
+                        ++nout;
+                    }
                 }
             }
-            a.nhits[r] = over ? 0xFFFFFFFFu : nout;
+            /* a listed record's 0 is a placeholder until the wide pass */
+            a.nhits[r] = over ? 0xFFFFFFFFu : (listed ? 0u : nout);
             a.sat[r] = satc;
-            if (over)
-                atomicAdd(a.counters + 3, 1ull);
         }
         __syncthreads();
     }
+    if (WIDE)
+        return;
     for (int o = 32; o > 0; o >>= 1) {
         bvl += __shfl_xor(bvl, o);
         gat += __shfl_xor(gat, o);
@@ -776,6 +848,15 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
 __global__ void k_mibf_zero(unsigned long long *p, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
         p[i] = 0;
+}
+
+/* the first pass over every record, then the wide pass over the records it
+ * listed (with an empty list the wide workgroups end at once) */
+template <uint32_t CAP, uint32_t SMAX, bool DIRECT>
+void query_pair(const QueryArgs &a, uint32_t blocks, size_t shm, hipStream_t st) {
+    k_mibf_query<CAP, SMAX, DIRECT, false><<<blocks, 64, shm, st>>>(a);
+    if (a.spill_stride)
+        k_mibf_query<CAP, SMAX, DIRECT, true><<<kWideBlocks, 64, shm, st>>>(a);
 }
 
 } // namespace
@@ -794,7 +875,14 @@ struct bbt_mibf {
     uint32_t *dmin = nullptr;
     double min_score = -1;
     uint32_t min_miss = 0xFFFFFFFFu, min_frames = 0;
-    unsigned long long *dctl = nullptr; /* [0] scan total, [1] separator errors, [8] queue */
+    /* [0] scan total, [1] separator errors, [8] queue, [9] records listed for
+     * the wide pass, [10] wide-pass queue */
+    unsigned long long *dctl = nullptr;
+    /* the wide pass: listed record indices, and per workgroup room for the
+     * candidates past kMaxCand (nids - kMaxCand of them at most) */
+    uint32_t *dwide_list = nullptr;
+    uint64_t wide_cap = 0;
+    uint32_t *dspill = nullptr;
     int cu = 0;
     std::mutex mu;
 };
@@ -894,6 +982,8 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
     (void)hipFree(mf->ddata);
     (void)hipFree(mf->dmin);
     (void)hipFree(mf->dctl);
+    (void)hipFree(mf->dwide_list);
+    (void)hipFree(mf->dspill);
     delete mf;
     return BBT_OK;
 }
@@ -1269,22 +1359,40 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     a.sat = out->sat;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.queue = mf->dctl + 8;
-    k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 1);
+    a.wide = mf->dctl + 9;
+    a.wide_queue = mf->dctl + 10;
+    /* with more IDs than kMaxCand a record may have more candidates than the
+     * first pass keeps: those records go through the wide pass */
+    a.spill_stride = mf->d.nids > kMaxCand ? mf->d.nids - kMaxCand : 0;
+    if (a.spill_stride) {
+        if (mf->wide_cap < nrec) {
+            (void)hipFree(mf->dwide_list);
+            mf->dwide_list = nullptr;
+            mf->wide_cap = 0;
+            MHIP(hipMalloc(&mf->dwide_list, nrec * sizeof(uint32_t)));
+            mf->wide_cap = nrec;
+        }
+        if (!mf->dspill)
+            MHIP(hipMalloc(&mf->dspill,
+                           size_t(kWideBlocks) * a.spill_stride * kHitWords * sizeof(uint32_t)));
+    }
+    a.wide_list = mf->dwide_list;
+    a.spill = mf->dspill;
+    k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 3);
     const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
     const bool few = mf->d.S <= 4;
     if (direct) {
         const size_t shm = size_t(kDirectWords) * (mf->d.nids + 1) * sizeof(uint32_t);
-        few ? k_mibf_query<1, 4, true><<<blocks, 64, shm, st>>>(a)
-            : k_mibf_query<1, 8, true><<<blocks, 64, shm, st>>>(a);
+        few ? query_pair<1, 4, true>(a, blocks, shm, st) : query_pair<1, 8, true>(a, blocks, shm, st);
     } else if (recs <= 1024)
-        few ? k_mibf_query<1024, 4, false><<<blocks, 64, 0, st>>>(a)
-            : k_mibf_query<1024, 8, false><<<blocks, 64, 0, st>>>(a);
+        few ? query_pair<1024, 4, false>(a, blocks, 0, st)
+            : query_pair<1024, 8, false>(a, blocks, 0, st);
     else if (recs <= 4096)
-        few ? k_mibf_query<4096, 4, false><<<blocks, 64, 0, st>>>(a)
-            : k_mibf_query<4096, 8, false><<<blocks, 64, 0, st>>>(a);
+        few ? query_pair<4096, 4, false>(a, blocks, 0, st)
+            : query_pair<4096, 8, false>(a, blocks, 0, st);
     else
-        few ? k_mibf_query<8192, 4, false><<<blocks, 64, 0, st>>>(a)
-            : k_mibf_query<8192, 8, false><<<blocks, 64, 0, st>>>(a);
+        few ? query_pair<8192, 4, false>(a, blocks, 0, st)
+            : query_pair<8192, 8, false>(a, blocks, 0, st);
     MHIP(hipGetLastError());
     return BBT_OK;
 }

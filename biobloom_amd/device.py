@@ -169,7 +169,10 @@ class MibfOutputs:
         hits = self.hits[: n * mh * w].cpu().numpy().view(np.uint32).reshape(n, mh, w)
         rows = []
         for r in range(n):
-            c = int(nh[r]) if nh[r] != 0xFFFFFFFF else 0
+            if nh[r] == 0xFFFFFFFF:
+                raise RuntimeError(f"MIBF query: record {r} was not evaluated "
+                                   "(its frame records overflowed the per-record store)")
+            c = int(nh[r])
             rows.append([tuple(int(x) for x in hits[r, j]) for j in range(min(c, mh))])
         return (rows, self.sat[:n].cpu().numpy().view(np.uint32).tolist(),
                 self.counters.cpu().numpy().astype(np.uint64), nh.copy())

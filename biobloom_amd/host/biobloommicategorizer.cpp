@@ -44,6 +44,7 @@ struct Opt {
     uint64_t interval = 10000000;
     int out = TSV;
     int verbose = 0;
+    unsigned threads = 0; /* threads that parse the input (0: all, at most 16) */
 };
 
 constexpr uint32_t kMaxHits = 64;
@@ -59,7 +60,7 @@ void usage() {
             "  -f, --filter=N         Path of miBF '.bf' file\n"
             "  -e, --paired_mode      Uses paired-end information.\n"
             "  -s, --min_FPR=N        Minimum -10*log(FPR) threshold for a match. [100.0]\n"
-            "  -t, --threads=N        The number of threads to use. [1]\n"
+            "  -t, --threads=N        Threads that parse the input. [all, at most 16]\n"
             "  -i, --hitOnly          Return only results that hit the filter.\n"
             "      --fa               Output categorized reads in Fasta output.\n"
             "      --fq               Output categorized reads in Fastq output.\n"
@@ -179,11 +180,16 @@ class MIBFClassifier {
         /* one stream: the filter's query state is shared by its launches */
         if (hipStreamCreateWithFlags(&m_st, hipStreamNonBlocking) != hipSuccess)
             Slot::die("stream creation failed");
+        for (hipEvent_t &e : m_done)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                Slot::die("event creation failed");
     }
     ~MIBFClassifier() {
         (void)hipStreamSynchronize(m_st);
         for (Slot &s : m_slot)
             s.release();
+        for (hipEvent_t &e : m_done)
+            (void)hipEventDestroy(e);
         (void)hipStreamDestroy(m_st);
         bbt_mibf_free(m_mf);
     }
@@ -192,7 +198,7 @@ class MIBFClassifier {
         cerr << "Filtering Start" << endl;
         m_t0 = std::chrono::steady_clock::now();
         for (const string &f : files) {
-            bbt_fe::ParallelSeqReader rd(f);
+            bbt_fe::ParallelSeqReader rd(f, m_o.threads);
             if (!rd.ok()) {
                 cerr << "file " << f << " cannot be opened" << endl;
                 exit(1);
@@ -218,7 +224,7 @@ class MIBFClassifier {
 
     void filterPair(const string &f1, const string &f2) {
         m_t0 = std::chrono::steady_clock::now();
-        bbt_fe::ParallelSeqReader r1(f1), r2(f2);
+        bbt_fe::ParallelSeqReader r1(f1, m_o.threads), r2(f2, m_o.threads);
         if (!r1.ok() || !r2.ok()) {
             cerr << "file " << (r1.ok() ? f2 : f1) << " cannot be opened" << endl;
             exit(1);
@@ -247,27 +253,46 @@ class MIBFClassifier {
     }
 
   private:
-    /* Two-slot pipeline: the host gathers and packs the next batch and
-     * formats the finished one while the device queries. */
+    /* Two-slot pipeline: while the device queries one batch, the host
+     * gathers, packs and submits the next (queued behind it on the stream)
+     * and then formats the finished one, so the device always has the next
+     * launch queued and outputs keep input order. */
     void pipeline(const std::function<bool(bbt_fe::RecBatch &)> &next, bool paired) {
+        using clk = std::chrono::steady_clock;
+        const auto since = [](clk::time_point t) {
+            return std::chrono::duration<double>(clk::now() - t).count();
+        };
         bbt_fe::RecBatch recs[2];
         int cur = 0;
+        clk::time_point t = clk::now();
         bool more = next(recs[0]);
+        m_phase[0] += since(t);
+        t = clk::now();
         pack(0, recs[0]);
         submit(0, paired);
+        m_phase[1] += since(t);
         while (true) {
             const int nx = 1 - cur;
+            t = clk::now();
             recs[nx].clear();
             if (more)
                 more = next(recs[nx]);
+            m_phase[0] += since(t);
+            /* slot nx's last batch was waited for in the previous round */
+            t = clk::now();
             pack(nx, recs[nx]);
-            if (hipStreamSynchronize(m_st) != hipSuccess)
+            submit(nx, paired);
+            m_phase[1] += since(t);
+            t = clk::now();
+            if (m_slot[cur].nreads && hipEventSynchronize(m_done[cur]) != hipSuccess)
                 Slot::die("query failed");
+            m_phase[2] += since(t);
+            t = clk::now();
             emit(cur, recs[cur], paired);
             recs[cur].clear();
+            m_phase[3] += since(t);
             if (recs[nx].v.empty())
                 break;
-            submit(nx, paired);
             cur = nx;
         }
     }
@@ -343,7 +368,8 @@ class MIBFClassifier {
             hipMemcpyAsync(S.hhits, S.dhits, nrec * kMaxHits * 7 * 4, hipMemcpyDeviceToHost,
                            m_st) != hipSuccess ||
             hipMemcpyAsync(S.hsat, S.dsat, nrec * 4, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
-            hipMemcpyAsync(S.hctr, S.dctr, 32, hipMemcpyDeviceToHost, m_st) != hipSuccess)
+            hipMemcpyAsync(S.hctr, S.dctr, 32, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
+            hipEventRecord(m_done[si], m_st) != hipSuccess)
             Slot::die("download failed");
     }
 
@@ -371,6 +397,8 @@ class MIBFClassifier {
             P.above.assign(m_info.nids + 1, 0);
             P.unique.assign(m_info.nids + 1, 0);
             for (size_t r = nrec * t / T, e = nrec * (t + 1) / T; r < e; ++r) {
+                if (S.hnh[r] == 0xFFFFFFFFu)
+                    Slot::die("a record was not evaluated (per-record store overflow)");
                 const uint32_t nh = std::min(S.hnh[r], kMaxHits);
                 const uint32_t *h = S.hhits + r * kMaxHits * 7;
                 if (!paired) {
@@ -468,7 +496,8 @@ class MIBFClassifier {
         cout.flush();
         cerr << "Filtering time: "
              << std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count()
-             << " s" << endl;
+             << " s (gather " << m_phase[0] << ", pack+launch " << m_phase[1]
+             << ", device wait " << m_phase[2] << ", output " << m_phase[3] << ")" << endl;
         cerr << "Total Reads:" << m_numRead << endl;
         cerr << "Writing file: " << m_o.prefix << "_summary.tsv" << endl;
         vector<uint64_t> sm(2 * (m_info.nids + 1) + 2);
@@ -494,6 +523,8 @@ class MIBFClassifier {
     vector<string> m_names;
     uint32_t m_allowedMiss = 0;
     hipStream_t m_st = nullptr;
+    hipEvent_t m_done[2] = {nullptr, nullptr}; /* recorded after each slot's copy-out */
+    double m_phase[4] = {0, 0, 0, 0}; /* s: gather, pack + launch, device wait, output */
     Slot m_slot[2];
     std::chrono::steady_clock::time_point m_t0;
     uint64_t m_numRead = 0, m_multi = 0, m_noMatch = 0;
@@ -556,7 +587,12 @@ int main(int argc, char *argv[]) {
             }
             break;
         case 'b': o.bestAgree = true; break;
-        case 't': break;
+        case 't':
+            if (!(cv >> o.threads)) {
+                cerr << "Error - Invalid parameter! t: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
         case 'v': ++o.verbose; break;
         case 'r':
             if (!(cv >> o.streak)) {

@@ -341,16 +341,16 @@ typedef struct {
 
 typedef struct {
     uint32_t *nhits;    /* device, per record (read or pair): reported candidates
-                           (may exceed max_hits; 0xFFFFFFFF = not evaluated,
-                           see counters[3]) */
+                           (may exceed max_hits) */
     uint32_t *hits;     /* device, per record max_hits records of 7 words in rank
                            order: id, count, nonSat, total, totalNonSat, solid,
                            nonSatFrame */
     uint32_t *sat;      /* device, per record: frames whose hit slots are all
                            saturated (satCount) */
     uint64_t *counters; /* device, 4 counters ADDED TO: bit lookups, ID slot
-                           reads, candidates beyond max_hits, records not
-                           evaluated (more than 64 candidates) */
+                           reads, candidates beyond max_hits, records with
+                           more than 64 candidates (ranked by a second pass
+                           that holds every candidate) */
 } bbt_mibf_out;
 
 /* Builds on the device from reference records packed back to back in dseq,

@@ -158,6 +158,41 @@ def test_max_hits_overflow_counted(dev):
     assert wctr[2] > 0 and gctr[2] == wctr[2]
 
 
+@pytest.mark.parametrize("nids,L", [(100, 150), (100, 700), (1100, 300), (1100, 700)])
+def test_more_than_64_candidates(dev, nids, L):
+    """Reads from a segment that many references share have more than 64
+    candidate IDs: such records go through the wide pass (counted in
+    counters[3]) and equal the oracle, which keeps every candidate, in rank
+    order, with the candidates beyond max_hits counted.  100 IDs take the
+    per-ID LDS totals, 1100 the sorted frame records."""
+    D, B = dev
+    rng = random.Random(31)
+    seeds = seeds_for(rng)
+    base = "".join(rng.choice("ACGT") for _ in range(1500))
+    recs = [(i + 1, base) for i in range(nids)]
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = [base[a:a + L] for a in range(0, 1500 - L, 97)] + reads_from(rng, recs, 20)
+    dr = device_reads(D, reads)
+    wide_seen = 0
+    for score, multi, mh, agree in [(1e-10, 1.0, 64, False), (1e-10, 0.5, 8, False),
+                                    (0.5, 0.0, 8, False), (0.5, 0.0, 64, True)]:
+        print(f"[gpu] mibf wide nids={nids} L={L} score={score} multi={multi} max_hits={mh} "
+              f"best_agree={agree}", flush=True)
+        want, wsat, wctr = om.query(reads, score=score, allowed_miss=2, multi=multi,
+                                    max_hits=mh, best_agree=agree)
+        got, gsat, gctr, nh = D.mibf_classify(
+            mf, dr, B.mibf_params(score, multi, 2, 0, mh, best_agree=agree))
+        assert gsat == wsat
+        for r in range(len(reads)):
+            assert got[r] == want[r], (r, len(reads[r]))
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[2] == wctr[2]
+        assert not (nh == 0xFFFFFFFF).any()
+        wide_seen += int(gctr[3])
+    assert wide_seen > 0
+
+
 def test_store_load_roundtrip(dev, tmp_path):
     D, B = dev
     rng = random.Random(11)
