@@ -210,7 +210,8 @@ static double stirling_err(double n) {
                  S4 = 1.0 / 1188;
     if (n <= 15.0) {
         long double x = (long double)n;
-        return double(lgammal(x + 1.0L) - (x + 0.5L) * logl(x) + x -
+        int sign = 0; /* the reentrant form: tables are computed on several threads */
+        return double(lgammal_r(x + 1.0L, &sign) - (x + 0.5L) * logl(x) + x -
                       0.5L * logl(2.0L * 3.14159265358979323846264338327950288L));
     }
     double nn = n * n;

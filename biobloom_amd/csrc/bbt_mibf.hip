@@ -26,6 +26,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bbt_hip.h"
@@ -377,6 +378,9 @@ struct QueryArgs {
 constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
 constexpr uint32_t kMaxCand = 64;
 constexpr uint32_t kWideBlocks = 64; /* workgroups of the wide pass */
+/* candidates a query may keep per record: a record has at most one per
+ * (frame, seed) slot, and at most 8192 of those reach the sorted path */
+constexpr uint32_t kMaxHitsOut = 8192;
 /* up to this many IDs (with the repeat ID 0), per-ID totals are accumulated
  * directly in LDS, six 32-bit counters per ID, instead of sorting the frame
  * records; such a filter takes records of any length */
@@ -873,6 +877,7 @@ struct bbt_mibf {
     std::vector<uint64_t> id_slots;  /* slots per ID (index = ID) */
     /* minCount table for (score, allowed_miss), rows 0..min_frames */
     uint32_t *dmin = nullptr;
+    std::vector<uint32_t> hmin; /* host copy of the table */
     double min_score = -1;
     uint32_t min_miss = 0xFFFFFFFFu, min_frames = 0;
     /* [0] scan total, [1] separator errors, [8] queue, [9] records listed for
@@ -1304,8 +1309,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     const uint64_t nrec = paired ? in->nreads / 2 : in->nreads;
     if (nrec == 0)
         return BBT_OK;
-    if (p->max_hits == 0 || p->max_hits > kMaxCand)
-        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..64");
+    if (p->max_hits == 0 || p->max_hits > kMaxHitsOut)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..8192");
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> g(mf->mu);
     const uint32_t k = mf->d.span;
@@ -1324,14 +1329,33 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     if (!mf->dmin || mf->min_score != p->score || mf->min_miss != miss ||
         mf->min_frames < frames) {
         /* getMinCount (MIBFClassifier.hpp:930-939) for every frame count up to
-         * the longest read of the batch */
+         * the longest read of the batch; rows already computed for the same
+         * score and allowed misses are kept, the others are split over host
+         * threads (one binomial quantile per frame count and ID) */
+        const bool extend = mf->dmin && mf->min_score == p->score && mf->min_miss == miss;
+        const uint32_t from = extend ? mf->min_frames + 1 : 1;
         std::vector<double> probs(mf->d.nids + 1);
         bbt_mibf_frame_probs(mf, miss, probs.data());
         const uint32_t cols = mf->d.nids + 1;
-        std::vector<uint32_t> t(size_t(frames + 1) * cols, 1);
-        for (uint32_t fc = 1; fc <= frames; ++fc)
-            for (uint32_t id = 1; id <= mf->d.nids; ++id)
-                t[size_t(fc) * cols + id] = bbt_mibf_min_count(fc, probs[id], p->score, mf->d.nids);
+        std::vector<uint32_t> &t = mf->hmin;
+        t.resize(size_t(frames + 1) * cols, 1);
+        if (!extend)
+            std::fill(t.begin(), t.begin() + cols, 1u);
+        const uint32_t rows = frames + 1 - from;
+        const unsigned nt = std::max(1u, std::min({16u, std::thread::hardware_concurrency(),
+                                                   (rows * cols + 2047) / 2048}));
+        const auto work = [&](unsigned w) {
+            for (uint32_t fc = from + w; fc <= frames; fc += nt)
+                for (uint32_t id = 1; id <= mf->d.nids; ++id)
+                    t[size_t(fc) * cols + id] =
+                        bbt_mibf_min_count(fc, probs[id], p->score, mf->d.nids);
+        };
+        std::vector<std::thread> th;
+        for (unsigned w = 1; w < nt; ++w)
+            th.emplace_back(work, w);
+        work(0);
+        for (std::thread &x : th)
+            x.join();
         (void)hipFree(mf->dmin);
         mf->dmin = nullptr;
         MHIP(hipMalloc(&mf->dmin, t.size() * 4));

@@ -47,7 +47,9 @@ struct Opt {
     unsigned threads = 0; /* threads that parse the input (0: all, at most 16) */
 };
 
-constexpr uint32_t kMaxHits = 64;
+/* candidates a batch's query keeps per record; the few records with more
+ * are queried again with room for all of them */
+constexpr uint32_t kFirstHits = 4;
 constexpr size_t kBatch = 1 << 17;
 
 void usage() {
@@ -93,8 +95,9 @@ struct Slot {
     uint32_t *dnh = nullptr, *dhits = nullptr, *dsat = nullptr;
     uint32_t *hnh = nullptr, *hhits = nullptr, *hsat = nullptr;
     uint64_t *dctr = nullptr, *hctr = nullptr;
-    size_t seq_cap = 0, rec_cap = 0;
+    size_t seq_cap = 0, rec_cap = 0, hit_cap = 0;
     size_t nreads = 0, bytes = 0;
+    uint32_t stride = 0; /* candidates per record in dhits / hhits */
     uint32_t max_len = 0;
 
     static void die(const string &m) {
@@ -118,9 +121,9 @@ struct Slot {
         doff = hoff = nullptr;
         dnh = dhits = dsat = hnh = hhits = hsat = nullptr;
         dctr = hctr = nullptr;
-        seq_cap = rec_cap = 0;
+        seq_cap = rec_cap = hit_cap = 0;
     }
-    /* bytes: the padded sequence bytes of the batch */
+    /* bytes: the padded sequence bytes of the batch; n reads */
     void reserve(size_t nbytes, size_t n) {
         if (nbytes > seq_cap) {
             (void)hipFree(dseq);
@@ -133,24 +136,32 @@ struct Slot {
         if (n > rec_cap) {
             (void)hipFree(doff);
             (void)hipFree(dnh);
-            (void)hipFree(dhits);
             (void)hipFree(dsat);
             (void)hipHostFree(hoff);
             (void)hipHostFree(hnh);
-            (void)hipHostFree(hhits);
             (void)hipHostFree(hsat);
             rec_cap = std::max(n, rec_cap * 2);
             if (hipMalloc(&doff, (rec_cap + 1) * 8) != hipSuccess ||
                 hipMalloc(&dnh, rec_cap * 4) != hipSuccess ||
-                hipMalloc(&dhits, rec_cap * kMaxHits * 7 * 4) != hipSuccess ||
                 hipMalloc(&dsat, rec_cap * 4) != hipSuccess ||
                 hipHostMalloc(&hoff, (rec_cap + 1) * 8, hipHostMallocDefault) != hipSuccess ||
                 hipHostMalloc(&hnh, rec_cap * 4, hipHostMallocDefault) != hipSuccess ||
-                hipHostMalloc(&hhits, rec_cap * kMaxHits * 7 * 4, hipHostMallocDefault) !=
-                    hipSuccess ||
                 hipHostMalloc(&hsat, rec_cap * 4, hipHostMallocDefault) != hipSuccess)
                 die("device allocation failed");
         }
+    }
+    /* room for max_hits candidates of each of n records */
+    void reserve_hits(size_t n, uint32_t max_hits) {
+        const size_t words = n * max_hits * 7;
+        if (words > hit_cap) {
+            (void)hipFree(dhits);
+            (void)hipHostFree(hhits);
+            hit_cap = std::max(words, hit_cap * 2);
+            if (hipMalloc(&dhits, hit_cap * 4) != hipSuccess ||
+                hipHostMalloc(&hhits, hit_cap * 4, hipHostMallocDefault) != hipSuccess)
+                die("device allocation failed");
+        }
+        stride = max_hits;
         if (!dctr && (hipMalloc(&dctr, 32) != hipSuccess ||
                       hipHostMalloc(&hctr, 32, hipHostMallocDefault) != hipSuccess))
             die("device allocation failed");
@@ -188,6 +199,7 @@ class MIBFClassifier {
         (void)hipStreamSynchronize(m_st);
         for (Slot &s : m_slot)
             s.release();
+        m_redo.release();
         for (hipEvent_t &e : m_done)
             (void)hipEventDestroy(e);
         (void)hipStreamDestroy(m_st);
@@ -268,8 +280,8 @@ class MIBFClassifier {
         bool more = next(recs[0]);
         m_phase[0] += since(t);
         t = clk::now();
-        pack(0, recs[0]);
-        submit(0, paired);
+        pack(m_slot[0], recs[0].v);
+        launch(0, paired);
         m_phase[1] += since(t);
         while (true) {
             const int nx = 1 - cur;
@@ -280,8 +292,8 @@ class MIBFClassifier {
             m_phase[0] += since(t);
             /* slot nx's last batch was waited for in the previous round */
             t = clk::now();
-            pack(nx, recs[nx]);
-            submit(nx, paired);
+            pack(m_slot[nx], recs[nx].v);
+            launch(nx, paired);
             m_phase[1] += since(t);
             t = clk::now();
             if (m_slot[cur].nreads && hipEventSynchronize(m_done[cur]) != hipSuccess)
@@ -297,30 +309,29 @@ class MIBFClassifier {
         }
     }
 
-    void pack(int si, const bbt_fe::RecBatch &b) {
-        Slot &S = m_slot[si];
-        const size_t n = b.v.size();
+    void pack(Slot &S, const vector<bbt_fe::SeqView> &v) {
+        const size_t n = v.size();
         S.nreads = n;
         if (!n)
             return;
         size_t bytes = 0;
         uint32_t max_len = 0;
-        for (const auto &v : b.v) {
-            bytes += v.seq_len;
-            max_len = std::max(max_len, v.seq_len);
+        for (const auto &r : v) {
+            bytes += r.seq_len;
+            max_len = std::max(max_len, r.seq_len);
         }
         const size_t padded = (bytes + 7) / 8 * 8 + 8;
         S.reserve(padded, n);
         uint64_t o = 0;
         for (size_t i = 0; i < n; ++i) {
             S.hoff[i] = o;
-            o += b.v[i].seq_len;
+            o += v[i].seq_len;
         }
         S.hoff[n] = o;
         const unsigned T = o < (size_t(8) << 20) ? 1u : 8u;
         const auto gather = [&](unsigned t) {
             for (size_t i = n * t / T, e = n * (t + 1) / T; i < e; ++i)
-                memcpy(S.hseq + S.hoff[i], b.v[i].seq, b.v[i].seq_len);
+                memcpy(S.hseq + S.hoff[i], v[i].seq, v[i].seq_len);
         };
         vector<std::thread> th;
         for (unsigned t = 1; t < T; ++t)
@@ -333,11 +344,20 @@ class MIBFClassifier {
         S.max_len = max_len;
     }
 
-    void submit(int si, bool paired) {
-        Slot &S = m_slot[si];
+    /* queues slot si's batch and an event after its copy-out */
+    void launch(int si, bool paired) {
+        if (submit(m_slot[si], paired, kFirstHits) &&
+            hipEventRecord(m_done[si], m_st) != hipSuccess)
+            Slot::die("event record failed");
+    }
+
+    /* queues copy-in, query (keeping max_hits candidates per record) and
+     * copy-out of S's batch; false when the batch is empty */
+    bool submit(Slot &S, bool paired, uint32_t max_hits) {
         const size_t n = S.nreads, nrec = paired ? n / 2 : n;
         if (!n)
-            return;
+            return false;
+        S.reserve_hits(nrec, max_hits);
         memset(S.hctr, 0, 32);
         if (hipMemcpyAsync(S.dseq, S.hseq, S.bytes, hipMemcpyHostToDevice, m_st) != hipSuccess ||
             hipMemcpyAsync(S.doff, S.hoff, (n + 1) * 8, hipMemcpyHostToDevice, m_st) !=
@@ -357,7 +377,7 @@ class MIBFClassifier {
         p.multi_thresh = m_o.multi;
         p.allowed_miss = m_allowedMiss;
         p.min_nonsat = m_o.minNoSat;
-        p.max_hits = kMaxHits;
+        p.max_hits = max_hits;
         p.flags = (paired ? BBT_MIBF_PAIRED : 0) | (m_o.bestAgree ? BBT_MIBF_BEST_AGREE : 0);
         bbt_mibf_out out{S.dnh, S.dhits, S.dsat, S.dctr};
         if (bbt_mibf_query_batch(m_mf, &in, &p, &out, m_st) != BBT_OK) {
@@ -365,12 +385,12 @@ class MIBFClassifier {
             exit(1);
         }
         if (hipMemcpyAsync(S.hnh, S.dnh, nrec * 4, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
-            hipMemcpyAsync(S.hhits, S.dhits, nrec * kMaxHits * 7 * 4, hipMemcpyDeviceToHost,
+            hipMemcpyAsync(S.hhits, S.dhits, nrec * max_hits * 7 * 4, hipMemcpyDeviceToHost,
                            m_st) != hipSuccess ||
             hipMemcpyAsync(S.hsat, S.dsat, nrec * 4, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
-            hipMemcpyAsync(S.hctr, S.dctr, 32, hipMemcpyDeviceToHost, m_st) != hipSuccess ||
-            hipEventRecord(m_done[si], m_st) != hipSuccess)
+            hipMemcpyAsync(S.hctr, S.dctr, 32, hipMemcpyDeviceToHost, m_st) != hipSuccess)
             Slot::die("download failed");
+        return true;
     }
 
     /* the output lines and summary counts of a finished batch: records in
@@ -380,10 +400,39 @@ class MIBFClassifier {
         const size_t n = S.nreads, nrec = paired ? n / 2 : n;
         if (!n)
             return;
-        if (S.hctr[2] && !m_warned) {
-            cerr << "Warning: reads with more than " << kMaxHits
-                 << " reported candidates; the output lists the first " << kMaxHits << endl;
-            m_warned = true;
+        /* records with more candidates than the batch's query kept are
+         * queried again, as one small batch, with room for all of them */
+        vector<size_t> big;
+        uint32_t need = 0;
+        for (size_t r = 0; r < nrec; ++r) {
+            if (S.hnh[r] == 0xFFFFFFFFu)
+                Slot::die("a record was not evaluated (per-record store overflow)");
+            if (S.hnh[r] > S.stride) {
+                big.push_back(r);
+                need = std::max(need, S.hnh[r]);
+            }
+        }
+        vector<int64_t> redo_of;
+        if (!big.empty()) {
+            vector<bbt_fe::SeqView> rv;
+            for (size_t r : big) {
+                if (paired) {
+                    rv.push_back(b.v[2 * r]);
+                    rv.push_back(b.v[2 * r + 1]);
+                } else {
+                    rv.push_back(b.v[r]);
+                }
+            }
+            pack(m_redo, rv);
+            submit(m_redo, paired, need);
+            if (hipStreamSynchronize(m_st) != hipSuccess)
+                Slot::die("query failed");
+            redo_of.assign(nrec, -1);
+            for (size_t j = 0; j < big.size(); ++j) {
+                if (m_redo.hnh[j] != S.hnh[big[j]])
+                    Slot::die("the repeated query of a record differs");
+                redo_of[big[j]] = int64_t(j);
+            }
         }
         const unsigned T = nrec < 4096 ? 1u : 8u;
         struct Part {
@@ -396,11 +445,12 @@ class MIBFClassifier {
             Part &P = parts[t];
             P.above.assign(m_info.nids + 1, 0);
             P.unique.assign(m_info.nids + 1, 0);
+            P.out.reserve((nrec * (t + 1) / T - nrec * t / T) * (paired ? 2 : 1) * 96);
             for (size_t r = nrec * t / T, e = nrec * (t + 1) / T; r < e; ++r) {
-                if (S.hnh[r] == 0xFFFFFFFFu)
-                    Slot::die("a record was not evaluated (per-record store overflow)");
-                const uint32_t nh = std::min(S.hnh[r], kMaxHits);
-                const uint32_t *h = S.hhits + r * kMaxHits * 7;
+                const uint32_t nh = S.hnh[r];
+                const uint32_t *h =
+                    nh > S.stride ? m_redo.hhits + size_t(redo_of[r]) * m_redo.stride * 7
+                                  : S.hhits + r * S.stride * 7;
                 if (!paired) {
                     /* filterSingleRead (MIBFClassifier.hpp:738-751) */
                     count(P, h, nh);
@@ -448,48 +498,74 @@ class MIBFClassifier {
             ++p.multi;
     }
 
+    /* decimal digits of v appended to o */
+    static void put_u32(string &o, uint32_t v) {
+        char b[10];
+        int n = 0;
+        do {
+            b[n++] = char('0' + v % 10);
+            v /= 10;
+        } while (v);
+        while (n)
+            o.push_back(b[--n]);
+    }
+
     /* appendResults (MIBFClassifier.hpp:652-673): name, nonSatFrameCount,
      * solidCount, count, nonSatCount, totalNonSatCount, totalCount */
     void append(const uint32_t *h, uint32_t n, string &o) const {
-        o += "\t";
+        static const int kField[6] = {6, 5, 1, 2, 4, 3};
+        o.push_back('\t');
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t *c = h + i * 7;
             if (i)
-                o += ";";
+                o.push_back(';');
             o += m_names[c[0]];
-            o += "," + to_string(c[6]) + "," + to_string(c[5]) + "," + to_string(c[1]) + "," +
-                 to_string(c[2]) + "," + to_string(c[4]) + "," + to_string(c[3]);
+            for (int f : kField) {
+                o.push_back(',');
+                put_u32(o, c[f]);
+            }
         }
     }
 
     /* formatOutStr (MIBFClassifier.hpp:675-736) */
     void format(const bbt_fe::SeqView &rd, uint32_t sat, const uint32_t *h, uint32_t n,
                 string &o) const {
-        string head(rd.name, rd.name_len);
-        if (rd.comment_len) {
-            head += ' ';
-            head.append(rd.comment, rd.comment_len);
-        }
+        const auto head = [&] {
+            o.append(rd.name, rd.name_len);
+            if (rd.comment_len) {
+                o.push_back(' ');
+                o.append(rd.comment, rd.comment_len);
+            }
+        };
         if (m_o.out == FASTA || m_o.out == FASTQ) {
-            o += m_o.out == FASTA ? ">" : "@";
-            o += head;
+            o.push_back(m_o.out == FASTA ? '>' : '@');
+            head();
             if (n) {
-                o += "\t" + m_names[h[0]];
+                o.push_back('\t');
+                o += m_names[h[0]];
                 append(h, n, o);
             }
-            o += '\n';
+            o.push_back('\n');
             o.append(rd.seq, rd.seq_len);
             if (m_o.out == FASTQ) {
                 o += "\n+\n";
                 o.append(rd.qual, rd.qual_len);
             }
         } else if (n == 0) {
-            o += "*\t" + head + "\t" + to_string(sat) + "\t*";
+            o += "*\t";
+            head();
+            o.push_back('\t');
+            put_u32(o, sat);
+            o += "\t*";
         } else {
-            o += m_names[h[0]] + "\t" + head + "\t" + to_string(sat);
+            o += m_names[h[0]];
+            o.push_back('\t');
+            head();
+            o.push_back('\t');
+            put_u32(o, sat);
             append(h, n, o);
         }
-        o += "\n";
+        o.push_back('\n');
     }
 
     void finish() {
@@ -526,10 +602,10 @@ class MIBFClassifier {
     hipEvent_t m_done[2] = {nullptr, nullptr}; /* recorded after each slot's copy-out */
     double m_phase[4] = {0, 0, 0, 0}; /* s: gather, pack + launch, device wait, output */
     Slot m_slot[2];
+    Slot m_redo; /* records queried again with room for all their candidates */
     std::chrono::steady_clock::time_point m_t0;
     uint64_t m_numRead = 0, m_multi = 0, m_noMatch = 0;
     vector<uint64_t> m_above, m_unique;
-    bool m_warned = false;
 };
 
 } // namespace

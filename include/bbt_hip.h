@@ -335,7 +335,7 @@ typedef struct {
     double multi_thresh;    /* report candidates with nonSat >= this x best's */
     uint32_t allowed_miss;  /* seeds of a frame that may miss (S - frameMatches) */
     uint32_t min_nonsat;    /* opt::minCountNonSatCount */
-    uint32_t max_hits;      /* hit records per record in bbt_mibf_out.hits (1..64) */
+    uint32_t max_hits;      /* hit records per record in bbt_mibf_out.hits (1..8192) */
     uint32_t flags;         /* BBT_MIBF_* */
 } bbt_mibf_params;
 

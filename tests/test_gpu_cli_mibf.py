@@ -197,6 +197,48 @@ def test_categorizer_paired(data, hit_only):
     assert summary == expected_summary(full, rows, len(reads) // 2, hit_only_paired=hit_only)
 
 
+@pytest.mark.parametrize("paired", [False, True])
+def test_categorizer_many_candidates(data, tmp_path, paired):
+    """100 references share one segment: reads from it have dozens to more
+    than 64 candidate IDs.  Every candidate is printed, in rank order (the
+    records with more than the first query keeps are queried again), and
+    lines and summary equal the oracle's."""
+    B, d, seeds, *_ = data
+    rng = random.Random(31)
+    base = "".join(rng.choice("ACGT") for _ in range(1500))
+    recs = [(i + 1, base) for i in range(100)]
+    names = [f"s{i}" for i, _ in recs]
+    write_fasta(tmp_path / "shared.fa", [(n, s) for n, (_, s) in zip(names, recs)])
+    print("[gpu] biobloommimaker, 100 shared references", flush=True)
+    subprocess.run([MIMAKER, "-p", "sh", "-S", " ".join(seeds), str(tmp_path / "shared.fa")],
+                   check=True, timeout=120, cwd=tmp_path)
+    reads = []
+    for j, a in enumerate(range(0, 800, 53)):
+        L = [150, 700, 40, 300][j % 4]
+        reads.append((f"m{j}", base[a:a + L]))
+    reads.append(("junk", "".join(rng.choice("ACGT") for _ in range(150))))
+    if len(reads) % 2:
+        reads.append(("junk2", "".join(rng.choice("ACGT") for _ in range(150))))
+    om = O.Mibf(recs, seeds)
+    rows, sats, _ = om.query([s for _, s in reads], score=1e-10, allowed_miss=len(seeds) - 1,
+                             multi=0.0, max_hits=1024, paired=paired)
+    assert max(len(r) for r in rows) > 64
+    if paired:
+        write_fastq(tmp_path / "a1.fq", reads[0::2])
+        write_fastq(tmp_path / "a2.fq", reads[1::2])
+        args = ["-e", "-m", "0", str(tmp_path / "a1.fq"), str(tmp_path / "a2.fq")]
+    else:
+        write_fastq(tmp_path / "a.fq", reads)
+        args = ["-m", "0", str(tmp_path / "a.fq")]
+    print(f"[gpu] biobloommicategorizer many candidates paired={paired}", flush=True)
+    r = subprocess.run([MICAT, "-p", str(tmp_path / "mc"), "-f", str(tmp_path / "sh.bf")] + args,
+                       check=True, timeout=120, capture_output=True, text=True)
+    full = ["repeat"] + names
+    assert r.stdout.splitlines() == expected_lines(full, reads, rows, sats, paired=paired)
+    nrec = len(reads) // 2 if paired else len(reads)
+    assert open(tmp_path / "mc_summary.tsv").read() == expected_summary(full, rows, nrec)
+
+
 def test_maker_kmer_mode_and_ids_by_file(data, tmp_path):
     B, d, seeds, recs, names, _ = data
     write_fasta(tmp_path / "a.fa", [(names[0], recs[0][1]), (names[1], recs[1][1])])
