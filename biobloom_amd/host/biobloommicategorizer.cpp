@@ -45,12 +45,12 @@ struct Opt {
     int out = TSV;
     int verbose = 0;
     unsigned threads = 0; /* threads that parse the input (0: all, at most 16) */
+    uint64_t batch = uint64_t(1) << 17; /* reads (or pairs) per device batch */
 };
 
 /* candidates a batch's query keeps per record; the few records with more
  * are queried again with room for all of them */
 constexpr uint32_t kFirstHits = 4;
-constexpr size_t kBatch = 1 << 17;
 
 void usage() {
     cerr << "Usage: biobloommicategorizer [OPTION]... -f [FILTER] [FILE]...\n"
@@ -70,6 +70,7 @@ void usage() {
             "  -h, --help             Display this dialog.\n"
             "  -v, --verbose          Display verbose output\n"
             "  -I, --interval         the interval to report file processing status [10000000]\n"
+            "      --batch_reads=N    Reads (or pairs) per device batch. [131072]\n"
             "Advanced options:\n"
             "  -a, --frameMatches=N   Min number seed matches needed in a frame to match [1]\n"
             "                         Ignored if k-mers used when indexing.\n"
@@ -168,9 +169,9 @@ struct Slot {
     }
 };
 
-/* blocks of parsed input one batch may hold (two batches stay well inside
- * the reader's pool) */
-constexpr size_t kBatchBlocks = bbt_fe::ParallelSeqReader::kMaxBlocks / 2 - 2;
+/* blocks of parsed input one batch may hold (the three batches alive at
+ * once stay well inside the reader's pool) */
+constexpr size_t kBatchBlocks = (bbt_fe::ParallelSeqReader::kMaxBlocks - 4) / 3;
 
 class MIBFClassifier {
   public:
@@ -218,7 +219,7 @@ class MIBFClassifier {
             bbt_fe::SeqCursor cur(rd);
             pipeline(
                 [&](bbt_fe::RecBatch &b) {
-                    while (b.v.size() < kBatch && b.holds.size() < kBatchBlocks) {
+                    while (b.v.size() < m_o.batch && b.holds.size() < kBatchBlocks) {
                         const bbt_fe::SeqView *v = cur.next();
                         if (!v)
                             return false;
@@ -244,7 +245,7 @@ class MIBFClassifier {
         bbt_fe::SeqCursor c1(r1), c2(r2);
         pipeline(
             [&](bbt_fe::RecBatch &b) {
-                while (b.v.size() < 2 * kBatch && b.holds.size() < kBatchBlocks) {
+                while (b.v.size() < 2 * m_o.batch && b.holds.size() < kBatchBlocks) {
                     const bbt_fe::SeqView *a = c1.next();
                     if (!a)
                         return false;
@@ -265,47 +266,56 @@ class MIBFClassifier {
     }
 
   private:
-    /* Two-slot pipeline: while the device queries one batch, the host
-     * gathers, packs and submits the next (queued behind it on the stream)
-     * and then formats the finished one, so the device always has the next
-     * launch queued and outputs keep input order. */
+    /* Pipeline over batches: while the device queries batch i, one helper
+     * thread packs batch i+1 and queues it behind batch i on the stream,
+     * another waits for batch i and formats it, and this thread gathers
+     * batch i+2; outputs keep input order. */
     void pipeline(const std::function<bool(bbt_fe::RecBatch &)> &next, bool paired) {
         using clk = std::chrono::steady_clock;
         const auto since = [](clk::time_point t) {
             return std::chrono::duration<double>(clk::now() - t).count();
         };
-        bbt_fe::RecBatch recs[2];
-        int cur = 0;
+        bbt_fe::RecBatch recs[3];
+        bool more = true;
+        const auto gather = [&](int k) {
+            const clk::time_point t = clk::now();
+            recs[k].clear();
+            if (more)
+                more = next(recs[k]);
+            m_phase[0] += since(t);
+        };
+        gather(0);
         clk::time_point t = clk::now();
-        bool more = next(recs[0]);
-        m_phase[0] += since(t);
-        t = clk::now();
         pack(m_slot[0], recs[0].v);
         launch(0, paired);
         m_phase[1] += since(t);
-        while (true) {
-            const int nx = 1 - cur;
-            t = clk::now();
-            recs[nx].clear();
-            if (more)
-                more = next(recs[nx]);
-            m_phase[0] += since(t);
-            /* slot nx's last batch was waited for in the previous round */
-            t = clk::now();
-            pack(m_slot[nx], recs[nx].v);
-            launch(nx, paired);
-            m_phase[1] += since(t);
-            t = clk::now();
-            if (m_slot[cur].nreads && hipEventSynchronize(m_done[cur]) != hipSuccess)
-                Slot::die("query failed");
-            m_phase[2] += since(t);
-            t = clk::now();
-            emit(cur, recs[cur], paired);
-            recs[cur].clear();
-            m_phase[3] += since(t);
-            if (recs[nx].v.empty())
+        gather(1);
+        for (uint64_t i = 0;; ++i) {
+            /* slot (i + 1) % 2 and recs[(i + 2) % 3] held batch i - 1, which
+             * has been emitted */
+            const int b0 = int(i % 3), b1 = int((i + 1) % 3), b2 = int((i + 2) % 3);
+            const int s0 = int(i % 2), s1 = int((i + 1) % 2);
+            std::thread packer([&] {
+                const clk::time_point tp = clk::now();
+                pack(m_slot[s1], recs[b1].v);
+                launch(s1, paired);
+                m_phase[1] += since(tp);
+            });
+            std::thread emitter([&] {
+                clk::time_point te = clk::now();
+                if (m_slot[s0].nreads && hipEventSynchronize(m_done[s0]) != hipSuccess)
+                    Slot::die("query failed");
+                m_phase[2] += since(te);
+                te = clk::now();
+                emit(s0, recs[b0], paired);
+                m_phase[3] += since(te);
+            });
+            gather(b2);
+            packer.join();
+            emitter.join();
+            recs[b0].clear();
+            if (recs[b1].v.empty())
                 break;
-            cur = nx;
         }
     }
 
@@ -635,6 +645,7 @@ int main(int argc, char *argv[]) {
                                  {"inverse", no_argument, NULL, 'n'},
                                  {"debug", no_argument, &debug, 1},
                                  {"verbose", no_argument, NULL, 'v'},
+                                 {"batch_reads", required_argument, NULL, 1000},
                                  {NULL, 0, NULL, 0}};
     int c, idx = 0;
     while ((c = getopt_long(argc, argv, "p:f:es:hI:t:f:m:r:dnvc:a:bi", lo, &idx)) != -1) {
@@ -648,6 +659,12 @@ int main(int argc, char *argv[]) {
         }
         case 'f': o.filter = optarg; break;
         case 'p': o.prefix = optarg; break;
+        case 1000:
+            if (!(cv >> o.batch) || o.batch == 0) {
+                cerr << "Error - Invalid parameters! batch_reads: " << optarg << endl;
+                exit(EXIT_FAILURE);
+            }
+            break;
         case 'h': usage(); break;
         case 'I':
             if (!(cv >> o.interval)) {

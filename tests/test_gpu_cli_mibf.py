@@ -226,10 +226,11 @@ def test_categorizer_many_candidates(data, tmp_path, paired):
     if paired:
         write_fastq(tmp_path / "a1.fq", reads[0::2])
         write_fastq(tmp_path / "a2.fq", reads[1::2])
-        args = ["-e", "-m", "0", str(tmp_path / "a1.fq"), str(tmp_path / "a2.fq")]
+        args = ["-e", "-m", "0", "--batch_reads=3", str(tmp_path / "a1.fq"),
+                str(tmp_path / "a2.fq")]
     else:
         write_fastq(tmp_path / "a.fq", reads)
-        args = ["-m", "0", str(tmp_path / "a.fq")]
+        args = ["-m", "0", "--batch_reads=5", str(tmp_path / "a.fq")]
     print(f"[gpu] biobloommicategorizer many candidates paired={paired}", flush=True)
     r = subprocess.run([MICAT, "-p", str(tmp_path / "mc"), "-f", str(tmp_path / "sh.bf")] + args,
                        check=True, timeout=120, capture_output=True, text=True)
@@ -237,6 +238,24 @@ def test_categorizer_many_candidates(data, tmp_path, paired):
     assert r.stdout.splitlines() == expected_lines(full, reads, rows, sats, paired=paired)
     nrec = len(reads) // 2 if paired else len(reads)
     assert open(tmp_path / "mc_summary.tsv").read() == expected_summary(full, rows, nrec)
+
+
+@pytest.mark.parametrize("paired", [False, True])
+def test_categorizer_small_batches(data, paired):
+    """Many small device batches (--batch_reads=37): batches are packed,
+    queried and formatted at the same time, and the lines still come out in
+    input order, equal to the oracle's."""
+    B, d, seeds, recs, names, reads = data
+    args = ["--batch_reads=37", "--fq"] + (["-e", str(d / "r1.fq"), str(d / "r2.fq")]
+                                          if paired else [str(d / "all.fq")])
+    lines, summary = run_cat(d, args, f"small_{int(paired)}")
+    om = O.Mibf(recs, seeds)
+    rows, sats, _ = om.query([s for _, s in reads], score=1e-10, allowed_miss=len(seeds) - 1,
+                             multi=1.0, min_nonsat=0, max_hits=64, paired=paired)
+    full = ["repeat"] + names
+    assert lines == expected_lines(full, reads, rows, sats, "fq", paired=paired)
+    nrec = len(reads) // 2 if paired else len(reads)
+    assert summary == expected_summary(full, rows, nrec)
 
 
 def test_maker_kmer_mode_and_ids_by_file(data, tmp_path):
