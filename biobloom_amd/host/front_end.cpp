@@ -16,7 +16,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
 #include <iostream>
+#include <mutex>
 #include <sstream>
 #include <thread>
 #include <unordered_map>
@@ -142,6 +145,7 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
 }
 
 BioBloomClassifier::~BioBloomClassifier() {
+    finish_writes();
     for (Batch *&b : m_slot) {
         if (!b)
             continue;
@@ -343,26 +347,131 @@ static std::string fmt(double v) {
     return s.str();
 }
 
+/* text compressed into one gzip member (zlib's default level, as gzopen
+ * "wb" writes); a gzip file may hold several members one after another,
+ * and every gzip reader returns their concatenated contents */
+static void gzip_member(const std::string &in, std::string &out) {
+    z_stream z;
+    std::memset(&z, 0, sizeof(z));
+    if (deflateInit2(&z, Z_DEFAULT_COMPRESSION, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) !=
+        Z_OK)
+        die("zlib: deflateInit2 failed");
+    out.resize(deflateBound(&z, uLong(in.size())));
+    z.next_in = reinterpret_cast<Bytef *>(const_cast<char *>(in.data()));
+    z.avail_in = uInt(in.size());
+    z.next_out = reinterpret_cast<Bytef *>(&out[0]);
+    z.avail_out = uInt(out.size());
+    if (deflate(&z, Z_FINISH) != Z_STREAM_END)
+        die("zlib: deflate failed");
+    out.resize(z.total_out);
+    deflateEnd(&z);
+}
+
 namespace {
+/* an output file; gzip files receive whole gzip members */
 struct Out {
-    gzFile gz = nullptr;
     FILE *f = nullptr;
+    bool gz = false, written = false;
     void write(const std::string &s) {
-        if (gz)
-            gzwrite(gz, s.data(), unsigned(s.size()));
-        else if (f)
-            std::fwrite(s.data(), 1, s.size(), f);
+        if (s.empty())
+            return;
+        if (std::fwrite(s.data(), 1, s.size(), f) != s.size())
+            die("write failed");
+        written = true;
+    }
+    void close() {
+        if (gz && !written) {
+            /* an empty gzip file, as closing an unwritten gzFile leaves */
+            std::string m;
+            gzip_member(std::string(), m);
+            write(m);
+        }
+        std::fclose(f);
     }
 };
 } // namespace
 
+/* Writes the formatted text of finished batches, in order, on its own
+ * thread: buf[t][k] is thread range t's text for output k (the partition
+ * files, then the read status, then stdout).  At most one batch waits while
+ * another is written. */
+struct BioBloomClassifier::Writer {
+    using Text = std::vector<std::vector<std::string>>;
+    std::vector<void *> outs;
+    void *status = nullptr;
+    bool to_stdout = false;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Text> q;
+    bool busy = false, stop = false;
+
+    Writer(const std::vector<void *> &o, void *st, bool so) : outs(o), status(st), to_stdout(so) {
+        th = std::thread([this] { loop(); });
+    }
+    ~Writer() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        th.join();
+    }
+    void push(Text &&t) {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return q.empty(); });
+        q.push_back(std::move(t));
+        cv.notify_all();
+    }
+    void drain() {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return q.empty() && !busy; });
+    }
+    void loop() {
+        while (true) {
+            Text t;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return stop || !q.empty(); });
+                if (q.empty())
+                    return;
+                t = std::move(q.front());
+                q.pop_front();
+                busy = true;
+            }
+            cv.notify_all();
+            const size_t n = outs.size();
+            for (const std::vector<std::string> &b : t) {
+                for (size_t k = 0; k < n; ++k)
+                    static_cast<Out *>(outs[k])->write(b[k]);
+                if (status)
+                    static_cast<Out *>(status)->write(b[n]);
+                if (to_stdout)
+                    std::cout.write(b[n + 1].data(), std::streamsize(b[n + 1].size()));
+            }
+            {
+                std::lock_guard<std::mutex> g(mu);
+                busy = false;
+            }
+            cv.notify_all();
+        }
+    }
+};
+
+/* waits for the writer and ends it (before outputs are closed) */
+void BioBloomClassifier::finish_writes() {
+    if (m_writer) {
+        m_writer->drain();
+        delete m_writer;
+        m_writer = nullptr;
+    }
+}
+
 void BioBloomClassifier::close_outputs(const std::string &) {
+    finish_writes();
     for (void *v : m_outs) {
         Out *o = static_cast<Out *>(v);
-        if (o->gz)
-            gzclose(o->gz);
-        if (o->f)
-            std::fclose(o->f);
+        o->close();
         delete o;
     }
     m_outs.clear();
@@ -370,36 +479,36 @@ void BioBloomClassifier::close_outputs(const std::string &) {
 
 static Out *open_one(const std::string &path, bool gz) {
     Out *o = new Out();
-    if (gz)
-        o->gz = gzopen(path.c_str(), "wb");
-    else
-        o->f = std::fopen(path.c_str(), "wb");
-    if (!o->gz && !o->f)
+    o->f = std::fopen(path.c_str(), "wb");
+    o->gz = gz;
+    if (!o->f)
         die("cannot open output file " + path);
     return o;
 }
 
-/* header line of one record for the partition files / stdout */
-static std::string record_text(const SeqView &r, const std::string &type,
-                               const std::string &extra) {
-    std::string s;
-    s.reserve(r.name_len + r.comment_len + 2 * r.seq_len + extra.size() + 8);
-    s += type == "fa" ? '>' : '@';
-    s.append(r.name, r.name_len);
-    s += ' ';
-    s.append(r.comment, r.comment_len);
-    s += extra;
-    s += '\n';
-    s.append(r.seq, r.seq_len);
-    s += '\n';
+/* one record of a partition file appended to o (header with the extra
+ * score text, sequence, and for FASTQ the quality) */
+static void append_record(std::string &o, const SeqView &r, const std::string &type,
+                          const std::string &extra) {
+    o += type == "fa" ? '>' : '@';
+    o.append(r.name, r.name_len);
+    o += ' ';
+    o.append(r.comment, r.comment_len);
+    o += extra;
+    o += '\n';
+    o.append(r.seq, r.seq_len);
+    o += '\n';
     if (type != "fa") {
-        s += "+\n";
-        s.append(r.qual, r.qual_len);
-        s += '\n';
+        o += "+\n";
+        o.append(r.qual, r.qual_len);
+        o += '\n';
     }
-    return s;
 }
 
+/* The output text of a finished batch: records in ranges are formatted on
+ * threads into one buffer per output (partition files, read status,
+ * stdout), gzip outputs compressed there too (one member per buffer), then
+ * the buffers are written in input order. */
 void BioBloomClassifier::emit_batch(int s, const RecBatch &batch, bool paired,
                                     const std::string &type) {
     const Batch &b = *m_slot[s];
@@ -408,65 +517,98 @@ void BioBloomClassifier::emit_batch(int s, const RecBatch &batch, bool paired,
     const uint32_t multi = m_nf + 1;
     if (!m_stdout && m_outs.empty() && !m_status)
         return; /* summary only: the counters are on the device */
-    for (uint64_t r = 0; r < nrec; ++r) {
-        const uint32_t cls = b.h_cls[r];
-        const uint64_t first = paired ? 2 * r : r;
-        const int nm = paired ? 2 : 1;
-        std::string extra[2];
-        std::string scoreStr[2];
-        for (int m = 0; m < nm; ++m) {
-            const uint64_t rd = first + m;
-            if (!b.scores)
-                continue;
-            const double *srow = b.h_scores + rd * m_nf;
-            const uint8_t *hrow = b.h_hits + rd * m_nf;
-            if (m_opt.mode == BBT_BESTHIT) {
-                /* evaluateReadBestHit: max score (from 0), repeated per best hit */
-                double mx = 0;
-                for (uint32_t i = 0; i < m_nf; ++i)
-                    if (mx < srow[i])
-                        mx = srow[i];
-                scoreStr[m] = fmt(mx);
-                for (uint32_t i = 0; i < m_nf; ++i)
-                    if (hrow[i])
-                        extra[m] += " " + fmt(mx);
-            } else {
-                for (uint32_t i = 0; i < m_nf; ++i)
-                    extra[m] += " " + fmt(srow[i]);
+    const size_t nouts = m_outs.size(), sink_status = nouts, sink_stdout = nouts + 1;
+    const unsigned T =
+        nrec < 4096 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<std::string>> buf(T, std::vector<std::string>(nouts + 2));
+    const auto work = [&](unsigned t) {
+        std::vector<std::string> &o = buf[t];
+        for (uint64_t r = nrec * t / T, re = nrec * (t + 1) / T; r < re; ++r) {
+            const uint32_t cls = b.h_cls[r];
+            const uint64_t first = paired ? 2 * r : r;
+            const int nm = paired ? 2 : 1;
+            std::string extra[2];
+            std::string scoreStr[2];
+            for (int m = 0; m < nm; ++m) {
+                const uint64_t rd = first + m;
+                if (!b.scores)
+                    continue;
+                const double *srow = b.h_scores + rd * m_nf;
+                const uint8_t *hrow = b.h_hits + rd * m_nf;
+                if (m_opt.mode == BBT_BESTHIT) {
+                    /* evaluateReadBestHit: max score (from 0), repeated per best hit */
+                    double mx = 0;
+                    for (uint32_t i = 0; i < m_nf; ++i)
+                        if (mx < srow[i])
+                            mx = srow[i];
+                    scoreStr[m] = fmt(mx);
+                    for (uint32_t i = 0; i < m_nf; ++i)
+                        if (hrow[i])
+                            extra[m] += " " + fmt(mx);
+                } else {
+                    for (uint32_t i = 0; i < m_nf; ++i)
+                        extra[m] += " " + fmt(srow[i]);
+                }
             }
-        }
-        /* -d / -n: printSingle / printPair (BioBloomClassifier.h:115-142, 211-250) */
-        if (m_stdout) {
-            bool print = m_opt.inverse ? cls != 0 : cls == 0;
-            if (print)
+            /* -d / -n: printSingle / printPair (BioBloomClassifier.h:115-142, 211-250) */
+            if (m_stdout) {
+                const bool print = m_opt.inverse ? cls != 0 : cls == 0;
+                if (print)
+                    for (int m = 0; m < nm; ++m) {
+                        const SeqView &rec = recs[first + m];
+                        std::string &d = o[sink_stdout];
+                        d += '@';
+                        d.append(rec.name, rec.name_len);
+                        d += ' ';
+                        d.append(rec.comment, rec.comment_len);
+                        if (!m_opt.inverse && m_opt.mode == BBT_BESTHIT) {
+                            d += ' ';
+                            d += scoreStr[m];
+                        }
+                        d += '\n';
+                        d.append(rec.seq, rec.seq_len);
+                        d += "\n+\n";
+                        d.append(rec.qual, rec.qual_len);
+                        d += '\n';
+                    }
+            }
+            /* printSingleToFile / printPairToFile (BioBloomClassifier.h:144-209, 252-529) */
+            if (nouts) {
                 for (int m = 0; m < nm; ++m) {
                     const SeqView &rec = recs[first + m];
-                    std::string sc = (!m_opt.inverse && m_opt.mode == BBT_BESTHIT)
-                                         ? " " + scoreStr[m] : "";
-                    std::cout << "@" << rec.name_sv() << " " << rec.comment_sv() << sc << "\n"
-                              << rec.seq_sv() << "\n+\n" << rec.qual_sv() << "\n";
+                    std::string add;
+                    if (m_opt.mode == BBT_SCORES && cls == multi)
+                        add = extra[m];
+                    else if (m_opt.mode == BBT_BESTHIT)
+                        add = (type == "fa" && cls == multi) ? extra[m] : " " + scoreStr[m];
+                    append_record(o[size_t(cls) * nm + m], rec, type, add);
                 }
-        }
-        /* printSingleToFile / printPairToFile (BioBloomClassifier.h:144-209, 252-529) */
-        if (!m_outs.empty()) {
-            for (int m = 0; m < nm; ++m) {
-                const SeqView &rec = recs[first + m];
-                std::string add;
-                if (m_opt.mode == BBT_SCORES && cls == multi)
-                    add = extra[m];
-                else if (m_opt.mode == BBT_BESTHIT)
-                    add = (type == "fa" && cls == multi) ? extra[m] : " " + scoreStr[m];
-                Out *o = static_cast<Out *>(m_outs[size_t(cls) * nm + m]);
-                o->write(record_text(rec, type, add));
+            }
+            if (m_status) {
+                std::string &d = o[sink_status];
+                d.append(recs[first].name, recs[first].name_len);
+                d += '\t';
+                d += cls < m_nf ? m_ids[cls] : (cls == m_nf ? "noMatch" : "multiMatch");
+                d += '\n';
             }
         }
-        if (m_status) {
-            const std::string name = cls < m_nf ? m_ids[cls]
-                                                : (cls == m_nf ? "noMatch" : "multiMatch");
-            static_cast<Out *>(m_status)->write(std::string(recs[first].name_sv()) + "\t" +
-                                                name + "\n");
-        }
-    }
+        if (m_gz)
+            for (size_t k = 0; k < nouts; ++k)
+                if (!o[k].empty()) {
+                    std::string z;
+                    gzip_member(o[k], z);
+                    o[k].swap(z);
+                }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < T; ++t)
+        th.emplace_back(work, t);
+    work(0);
+    for (std::thread &x : th)
+        x.join();
+    if (!m_writer)
+        m_writer = new Writer(m_outs, m_status, m_stdout);
+    m_writer->push(std::move(buf));
 }
 
 void BioBloomClassifier::progress(uint64_t &total, uint64_t add) {
@@ -476,6 +618,7 @@ void BioBloomClassifier::progress(uint64_t &total, uint64_t add) {
 }
 
 void BioBloomClassifier::write_summary(uint64_t totalReads) {
+    finish_writes();
     std::vector<uint64_t> sm(2 * m_nf + 2);
     std::vector<uint64_t> ctr(4);
     for (Batch *b : m_slot)
@@ -503,8 +646,7 @@ void BioBloomClassifier::write_summary(uint64_t totalReads) {
     std::fclose(f);
     if (m_status) {
         Out *o = static_cast<Out *>(m_status);
-        if (o->f)
-            std::fclose(o->f);
+        o->close();
         delete o;
         m_status = nullptr;
     }

@@ -89,6 +89,7 @@ class BioBloomClassifier {
 
   private:
     struct Batch;
+    struct Writer;
     void pack(int slot, const RecBatch &recs, bool paired, uint64_t scale);
     void submit(int slot);
     void finish(int slot);
@@ -97,6 +98,7 @@ class BioBloomClassifier {
     void open_outputs(const std::string &type, bool paired);
     void emit_batch(int slot, const RecBatch &recs, bool paired, const std::string &outputType);
     void close_outputs(const std::string &outputType);
+    void finish_writes();
     void write_summary(uint64_t totalReads);
     void progress(uint64_t &total, uint64_t add);
     void pair_loop(const std::string &f1, const std::string &f2, const std::string &type);
@@ -118,6 +120,7 @@ class BioBloomClassifier {
     std::vector<void *> m_outs; /* gzFile or FILE* per class */
     bool m_gz = false;
     void *m_status = nullptr;
+    Writer *m_writer = nullptr; /* writes formatted batches in order, off this thread */
     std::chrono::steady_clock::time_point m_t0; /* "Filtering Start" */
     /* s: gather, pack + launch, device wait, output; staging allocation (part
      * of pack + launch) */

@@ -300,6 +300,44 @@ def test_long_reads_in_small_blocks(data, tmp_path):
     assert open(tmp_path / "long_summary.tsv").read() == O.summary(["g0", "g1", "g2"], hits)
 
 
+def test_gz_partitions_from_threads(data, tmp_path):
+    """12k reads in batches that ramp up to 8192: batches of 4096 and more are
+    formatted and gzip-compressed on threads, so each partition file holds
+    several gzip members; the decompressed files, --status and the -d stdout
+    equal the oracle's classes in input order.  Partitions that get no
+    reads are valid, empty gzip files."""
+    import gzip
+    d, reads, filters = data
+    rng = np.random.default_rng(5)
+    many = [(f"m{i}", reads[int(rng.integers(0, len(reads)))][1]) for i in range(12000)]
+    path = tmp_path / "many.fq"
+    write_fastq(path, many)
+    buf, off = O.pack_reads([s for _, s in many])
+    _, _, cls, _ = O.classify(filters, buf, off)
+    bfs = " ".join(str(d / f"g{i}.bf") for i in range(3))
+    names = ["g0", "g1", "g2", "noMatch", "multiMatch"]
+    print("[gpu] biobloomcategorizer --fq -g --status -d, 12k reads", flush=True)
+    out = subprocess.run([CAT, "-p", str(tmp_path / "z"), "-f", bfs, "--fq", "-g", "--status",
+                          "-d", "--batch_reads=8192", str(path)], check=True, timeout=120,
+                         capture_output=True, text=True).stdout
+    for ci, n in enumerate(names):
+        want = "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
+                       for (rn, s), c in zip(many, cls) if c == ci)
+        assert gzip.open(tmp_path / f"z_{n}.fq.gz", "rt").read() == want
+    status = open(tmp_path / "z_readStatus.tsv").read().splitlines()
+    assert status == [f"{rn}\t{names[c]}" for (rn, _), c in zip(many, cls)]
+    assert out == "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
+                          for (rn, s), c in zip(many, cls) if c == 0)
+    # random reads only: every partition but noMatch stays empty
+    junk = [(f"j{i}", "".join(rng.choice(list("ACGT"), 150))) for i in range(50)]
+    write_fastq(tmp_path / "junk.fq", junk)
+    subprocess.run([CAT, "-p", str(tmp_path / "e"), "-f", bfs, "--fq", "-g",
+                    str(tmp_path / "junk.fq")], check=True, timeout=120)
+    for n in names:
+        text = gzip.open(tmp_path / f"e_{n}.fq.gz", "rt").read()
+        assert (text != "") == (n == "noMatch")
+
+
 def test_score_outputs_stdout_and_gz(data, tmp_path):
     """The printing paths (BioBloomClassifier.h:115-209): -w --fa puts every
     filter's score on multi-match headers, -b --fq the best score on every
