@@ -490,16 +490,17 @@ int main(int argc, char *argv[]) {
     uint64_t expected = 0;
     string sources;
     for (const string &f : inputs) {
-        bbt_fe::SeqReader rd(f);
+        /* records as the sequential kseq reader gives them, parsed on threads */
+        bbt_fe::ParallelSeqReader rd(f);
         if (!rd.ok()) {
             cerr << "file " << f << " cannot be opened" << endl;
             exit(1);
         }
         cerr << "Opening File " << f << endl;
-        bbt_fe::SeqRecord rec;
-        while (rd.read(rec) >= 0) {
-            expected += uint64_t(rec.seq.size()) - k + 1; /* size_t arithmetic */
-            all += rec.seq;
+        bbt_fe::SeqCursor cur(rd);
+        while (const bbt_fe::SeqView *v = cur.next()) {
+            expected += uint64_t(v->seq_len) - k + 1; /* size_t arithmetic */
+            all.append(v->seq, v->seq_len);
             all.push_back('\n');
         }
         sources += f + " ";

@@ -165,23 +165,25 @@ int main(int argc, char *argv[]) {
     vector<string> rec_name;
     uint64_t counts = 0;
     for (const string &f : inputs) {
-        bbt_fe::SeqReader rd(f);
+        /* records as the sequential kseq reader gives them, parsed on threads */
+        bbt_fe::ParallelSeqReader rd(f);
         if (!rd.ok()) {
             cerr << "file " << f << " cannot be opened" << endl;
             exit(1);
         }
         if (verbose)
             cerr << "Opening " << f << endl;
-        bbt_fe::SeqRecord rec;
+        bbt_fe::SeqCursor cur(rd);
         const string fname = f.substr(f.find_last_of('/') + 1);
-        while (rd.read(rec) >= 0) {
+        while (const bbt_fe::SeqView *v = cur.next()) {
+            const string name(v->name, v->name_len);
             if (!byFile) {
-                names.push_back(rec.name);
-                nameToID[rec.name] = uint32_t(names.size());
+                names.push_back(name);
+                nameToID[name] = uint32_t(names.size());
             }
-            rec_name.push_back(byFile ? fname : rec.name);
-            counts += rec.seq.size() >= k ? rec.seq.size() - k + 1 : 0;
-            all += rec.seq;
+            rec_name.push_back(byFile ? fname : name);
+            counts += v->seq_len >= k ? v->seq_len - k + 1 : 0;
+            all.append(v->seq, v->seq_len);
             all.push_back('\n'); /* the separator bbt_mibf_build requires */
             rec_off.push_back(all.size());
         }
