@@ -374,6 +374,7 @@ def bench_e2e(args, torch, D, B):
         fs.store(i, pre, w["fpr"], glen - w["k"] + 1, counts[i][0], counts[i][1], f"g{i}.fa")
         bfs.append(pre + ".bf")
     runs = []
+    cpu = None
     variants = [("fastq", False)] + ([("fastq.gz", True)] if args.e2e_gz else [])
     for ext, gz in variants:
         t = time.time()
@@ -402,6 +403,10 @@ def bench_e2e(args, torch, D, B):
             log(f"e2e {ext} -t {threads}: filtering {ft:.2f} s (wall {wall:.2f} s), "
                 f"{item['reads_per_s'] / 1e6:.2f} M reads/s, {item['text_gb_per_s']:.2f} GB/s "
                 f"of FASTQ, summary equal={item['summary_equal']}")
+        if not gz and not args.no_cpu and cpu is None:
+            cpu = cpu_baseline_e2e(fs, f1, f2, 2 * reads.shape[1] + 18, min(npairs, 200_000),
+                                   min(16, os.cpu_count() or 1))
+            log(f"cpu e2e baseline: {cpu['value'] / 1e6:.3f} M reads/s ({cpu['sample']})")
         if args.e2e_keep:
             log("kept: " + " ".join(cmd))
         else:
@@ -419,11 +424,54 @@ def bench_e2e(args, torch, D, B):
                    "scale": args.scale, "pairs": npairs},
         "e2e": runs,
         "summary_equal": all(r["summary_equal"] for r in runs),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     if not args.e2e_keep:
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline_e2e(fs, f1, f2, width, npairs, threads):
+    """The reference's path end to end on the host cores, over the first
+    npairs pairs of the two FASTQ files (records of `width` bytes): read the
+    bytes, parse them with the oracle's kseq_read restatement (one thread, as
+    the reference's parse is a critical section), classify every pair against
+    every filter with the oracle on `threads` threads, and write the summary
+    text."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    ofs, ids = [], []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        f = O.Filter(inf.m, inf.h, inf.k)
+        fs.download(i, f.bits)
+        ofs.append(f)
+        ids.append(f"g{i}")
+    t0 = time.time()
+    mates = []
+    for path in (f1, f2):
+        with open(path, "rb") as fh:
+            data = fh.read(npairs * width)
+        mates.append(O.kseq_seqs(data))
+    t1 = time.time()
+    (b1, o1, _), (b2, o2, _) = mates
+    n = min(len(o1), len(o2)) - 1
+    L = int(o1[1] - o1[0])
+    if not (np.all(np.diff(o1[: n + 1]) == L) and np.all(np.diff(o2[: n + 1]) == L)):
+        raise SystemExit("cpu e2e baseline expects fixed-length reads")
+    buf = np.stack([b1[: n * L].reshape(n, L), b2[: n * L].reshape(n, L)], 1).reshape(-1)
+    buf = np.concatenate([buf, np.zeros(1, np.uint8)])
+    off = np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(L)
+    hits, _, _, _ = O.classify(ofs, buf, off, paired=True, nthreads=threads)
+    t2 = time.time()
+    text = O.summary(ids, hits, paired=True)
+    t3 = time.time()
+    return {"value": 2 * n / (t3 - t0), "unit": "reads/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} pairs of both files: read + kseq parse (1 thread) "
+                      f"{t1 - t0:.2f}s, classify ({threads} threads) {t2 - t1:.2f}s, "
+                      f"summary {t3 - t2:.3f}s",
+            "summary_bytes": len(text)}
 
 
 def bench_mibf_e2e(args, torch, D, B):

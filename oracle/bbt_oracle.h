@@ -19,6 +19,7 @@
  *   binomial thresholds                         Common/SeqEval.h:199-216
  *   dust masking (-D)                           Common/sdust.c, Common/SDust.hpp,
  *                                               SeqEval.h:496-529
+ *   kseq_read (sequences only)                  Common/kseq.h:93-219 (bbt_kseq.cpp)
  *
  * Parity status of each function is recorded beside it in bbt_oracle.cpp.
  */
@@ -151,6 +152,15 @@ uint64_t orc_summary(const char *const *filter_ids, uint32_t nf,
  * returns their number, or -1 if cap is too small. */
 int64_t orc_sdust(const char *seq, uint64_t len, int T, int W, uint32_t *starts,
                   uint32_t *finishes, uint64_t cap);
+
+/* ---- kseq_read (bbt_kseq.cpp) ------------------------------------------- */
+/* Every record's sequence of the FASTA/FASTQ text buf[0..len) appended to
+ * seq_out (cap bytes) with its end in off_out[1..] (off_out[0] = 0, at most
+ * cap_recs - 1 records); *status = kseq_read's return after the last record
+ * (-1 end of input, -2 bad quality).  Returns the records, or -1 if a
+ * capacity is too small. */
+int64_t orc_kseq_seqs(const char *buf, uint64_t len, char *seq_out, uint64_t cap,
+                      uint64_t *off_out, uint64_t cap_recs, int64_t *status);
 
 /* ---- multi-index Bloom filter (bbt_mibf.cpp) ------------------------- */
 /* seeds: nseeds concatenated '0'/'1' strings of length span (spaced-seed

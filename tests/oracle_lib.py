@@ -256,6 +256,24 @@ def eval_score(filt, seq, p):
     return r, pr.value, lk.value
 
 
+def kseq_seqs(data):
+    """orc_kseq_seqs: the sequences kseq_read gives for FASTA/FASTQ text ->
+    (uint8 buffer, uint64 offsets, kseq status after the last record)."""
+    L = lib()
+    L.orc_kseq_seqs.restype = C.c_int64
+    L.orc_kseq_seqs.argtypes = [C.c_char_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                C.c_uint64, C.POINTER(C.c_int64)]
+    data = bytes(data)
+    cap = len(data) + 8
+    buf = np.zeros(cap, np.uint8)
+    off = np.zeros(len(data) // 2 + 2, np.uint64)
+    st = C.c_int64(0)
+    n = L.orc_kseq_seqs(data, len(data), buf.ctypes.data, cap, off.ctypes.data, len(off),
+                        C.byref(st))
+    assert n >= 0
+    return buf[: int(off[n])], off[: n + 1].copy(), int(st.value)
+
+
 def pack_reads(reads):
     """Concatenate reads (bytes/str) into (uint8 buffer, uint64 offsets)."""
     bs = [r.encode() if isinstance(r, str) else bytes(r) for r in reads]
