@@ -18,6 +18,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -484,8 +487,9 @@ class MIBFClassifier {
         work(0);
         for (auto &t : th)
             t.join();
-        for (const Part &P : parts) {
-            cout.write(P.out.data(), std::streamsize(P.out.size()));
+        vector<string> text;
+        for (Part &P : parts) {
+            text.push_back(std::move(P.out));
             for (uint32_t i = 0; i <= m_info.nids; ++i) {
                 m_above[i] += P.above[i];
                 m_unique[i] += P.unique[i];
@@ -493,7 +497,61 @@ class MIBFClassifier {
             m_multi += P.multi;
             m_noMatch += P.noMatch;
         }
+        m_writer.push(std::move(text));
     }
+
+    /* Writes formatted batches to stdout in order on its own thread, so a
+     * batch is written while the next one is formatted; at most one batch
+     * waits. */
+    struct Writer {
+        std::thread th;
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<vector<string>> q;
+        bool busy = false, stop = false;
+        Writer() {
+            th = std::thread([this] {
+                while (true) {
+                    vector<string> t;
+                    {
+                        std::unique_lock<std::mutex> g(mu);
+                        cv.wait(g, [&] { return stop || !q.empty(); });
+                        if (q.empty())
+                            return;
+                        t = std::move(q.front());
+                        q.pop_front();
+                        busy = true;
+                    }
+                    cv.notify_all();
+                    for (const string &x : t)
+                        cout.write(x.data(), std::streamsize(x.size()));
+                    {
+                        std::lock_guard<std::mutex> g(mu);
+                        busy = false;
+                    }
+                    cv.notify_all();
+                }
+            });
+        }
+        ~Writer() {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                stop = true;
+            }
+            cv.notify_all();
+            th.join();
+        }
+        void push(vector<string> &&t) {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return q.empty(); });
+            q.push_back(std::move(t));
+            cv.notify_all();
+        }
+        void drain() {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return q.empty() && !busy; });
+        }
+    };
 
     /* ResultsManager::updateSummaryData over QueryResults (ResultsManager.hpp:40-64) */
     template <class P>
@@ -579,6 +637,7 @@ class MIBFClassifier {
     }
 
     void finish() {
+        m_writer.drain();
         cout.flush();
         cerr << "Filtering time: "
              << std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count()
@@ -613,6 +672,7 @@ class MIBFClassifier {
     double m_phase[4] = {0, 0, 0, 0}; /* s: gather, pack + launch, device wait, output */
     Slot m_slot[2];
     Slot m_redo; /* records queried again with room for all their candidates */
+    Writer m_writer;
     std::chrono::steady_clock::time_point m_t0;
     uint64_t m_numRead = 0, m_multi = 0, m_noMatch = 0;
     vector<uint64_t> m_above, m_unique;
