@@ -1311,6 +1311,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
         return BBT_OK;
     if (p->max_hits == 0 || p->max_hits > kMaxHitsOut)
         return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..8192");
+    if (nrec > 0xFFFFFFFFull)
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: more than 2^32 - 1 records in one batch");
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> g(mf->mu);
     const uint32_t k = mf->d.span;
