@@ -317,8 +317,24 @@ def bench_serve(args, torch, D, B):
     print(json.dumps(line), flush=True)
 
 
+def write_bgzf(f, data, level=1, chunk=65280):
+    """data as BGZF (bgzip's blocked gzip): members of at most `chunk` bytes
+    of text with the 'BC' size subfield, then the empty end-of-file member."""
+    import struct
+    import zlib
+    for i in range(0, len(data), chunk):
+        part = data[i:i + chunk]
+        c = zlib.compressobj(level, zlib.DEFLATED, -15)
+        payload = c.compress(part) + c.flush()
+        f.write(b"\x1f\x8b\x08\x04\0\0\0\0\0\xff" + struct.pack("<H", 6) + b"BC" +
+                struct.pack("<HH", 2, len(payload) + 25) + payload +
+                struct.pack("<II", zlib.crc32(part), len(part)))
+    f.write(b"\x1f\x8b\x08\x04\0\0\0\0\0\xff\x06\0BC\x02\0\x1b\0\x03\0\0\0\0\0\0\0\0\0")
+
+
 def write_fastq(path, reads, mate, gz=False):
-    """FASTQ with fixed-width names @r<9 digits>/<mate>, qualities 'I'."""
+    """FASTQ with fixed-width names @r<9 digits>/<mate>, qualities 'I';
+    gz: False, True (one gzip member) or "bgzf"."""
     import gzip
     n, L = reads.shape
     w = 2 * L + 18
@@ -337,7 +353,10 @@ def write_fastq(path, reads, mate, gz=False):
     a[:, 16 + L] = 10
     a[:, 17 + L:17 + 2 * L] = ord("I")
     a[:, 17 + 2 * L] = 10
-    if gz:
+    if gz == "bgzf":
+        with open(path, "wb") as f:
+            write_bgzf(f, memoryview(a.reshape(-1)))
+    elif gz:
         with gzip.open(path, "wb", compresslevel=1) as f:
             f.write(a.tobytes())
     else:
@@ -375,7 +394,8 @@ def bench_e2e(args, torch, D, B):
         bfs.append(pre + ".bf")
     runs = []
     cpu = None
-    variants = [("fastq", False)] + ([("fastq.gz", True)] if args.e2e_gz else [])
+    variants = [("fastq", False)] + ([("fastq.gz", True), ("fastq.bgz", "bgzf")]
+                                     if args.e2e_gz else [])
     for ext, gz in variants:
         t = time.time()
         f1, f2 = os.path.join(tmp, f"r1.{ext}"), os.path.join(tmp, f"r2.{ext}")

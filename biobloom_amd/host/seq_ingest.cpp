@@ -10,6 +10,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 namespace bbt_fe {
@@ -277,6 +278,244 @@ int64_t SeqReader::read(SeqRecord &rec) {
 /* ===================================================================== */
 /* ParallelSeqReader                                                      */
 
+/* ---- BGZF ---------------------------------------------------------------- */
+
+/* A BGZF stream (RFC 1952 members, each with the extra subfield 'B','C' of
+ * length 2 holding the member's size - 1, at most 64 KiB of text each).
+ * read() fills a buffer by inflating whole members on threads straight into
+ * it (sizes from each member's ISIZE, contents checked against its CRC32);
+ * the part of a member that does not fit waits in a small buffer.  A member
+ * without the subfield switches the rest of the stream to sequential zlib,
+ * as gzread would read it. */
+class BgzfSource {
+  public:
+    BgzfSource(int fd, unsigned threads) : m_fd(fd), m_threads(std::max(1u, threads)) {}
+    ~BgzfSource() {
+        if (m_zinit)
+            inflateEnd(&m_z);
+        ::close(m_fd);
+    }
+    /* true when the file at fd starts with a BGZF member */
+    static bool detect(int fd) {
+        unsigned char h[18];
+        if (::pread(fd, h, sizeof(h), 0) != ssize_t(sizeof(h)))
+            return false;
+        size_t sz = 0;
+        return member_size(h, sizeof(h), sz) == 1;
+    }
+    /* up to want bytes of text into out; eof once the stream is done */
+    size_t read(char *out, size_t want, bool &eof, bool &err) {
+        compact();
+        size_t got = 0;
+        while (got < want && !m_err && !m_done) {
+            if (m_pend_at < m_pend.size()) {
+                const size_t t = std::min(want - got, m_pend.size() - m_pend_at);
+                std::memcpy(out + got, m_pend.data() + m_pend_at, t);
+                m_pend_at += t;
+                got += t;
+                continue;
+            }
+            if (m_stream) {
+                got += inflate_stream(out + got, want - got);
+                continue;
+            }
+            /* the members that fit whole, inflated on threads */
+            std::vector<Member> ms;
+            const size_t room = want - got;
+            size_t cpos = m_cpos, total = 0;
+            Member m;
+            int r = 1;
+            while (ms.size() < 4096 && (r = next_member(cpos, m)) == 1 && m.isize <= room - total) {
+                m.out = total;
+                total += m.isize;
+                ms.push_back(m);
+                cpos += m.csize;
+            }
+            if (!ms.empty()) {
+                /* the text stops before the first member that fails */
+                std::atomic<size_t> first_bad{ms.size()};
+                const unsigned T =
+                    unsigned(std::min<size_t>(m_threads, std::max<size_t>(1, ms.size() / 8)));
+                const auto work = [&](unsigned t) {
+                    for (size_t i = t; i < ms.size(); i += T)
+                        if (!inflate_member(ms[i], out + got + ms[i].out)) {
+                            size_t cur = first_bad.load();
+                            while (i < cur && !first_bad.compare_exchange_weak(cur, i)) {
+                            }
+                        }
+                };
+                std::vector<std::thread> th;
+                for (unsigned t = 1; t < T; ++t)
+                    th.emplace_back(work, t);
+                work(0);
+                for (std::thread &x : th)
+                    x.join();
+                if (first_bad < ms.size()) {
+                    m_err = true;
+                    got += ms[first_bad].out;
+                } else {
+                    got += total;
+                }
+                m_cpos = cpos;
+            } else if (r == 1) {
+                /* the next member does not fit: its text waits to the side */
+                m_pend.resize(m.isize);
+                m_pend_at = 0;
+                if (!inflate_member(m, m_pend.data())) {
+                    m_err = true;
+                    m_pend.clear();
+                }
+                m_cpos = cpos + m.csize;
+            } else if (r < 0) {
+                m_stream = true; /* not a BGZF member: sequential from here */
+            } else {
+                m_done = true; /* no more input */
+            }
+        }
+        if (m_err)
+            err = true;
+        eof = m_err || (m_done && m_pend_at == m_pend.size());
+        return got;
+    }
+
+  private:
+    struct Member {
+        size_t at = 0, csize = 0, isize = 0, payload = 0, out = 0;
+        uint32_t crc = 0;
+    };
+    /* 1: a BGZF member header (flags exactly FEXTRA, a 'B','C' subfield) of
+     * total member size sz at h; 0: more header bytes needed; -1: not one */
+    static int member_size(const unsigned char *h, size_t n, size_t &sz) {
+        if (n < 12)
+            return 0;
+        if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || h[3] != 4)
+            return -1;
+        const size_t xlen = size_t(h[10]) | (size_t(h[11]) << 8);
+        if (n < 12 + xlen)
+            return 0;
+        for (size_t x = 12; x + 4 <= 12 + xlen;) {
+            const size_t slen = size_t(h[x + 2]) | (size_t(h[x + 3]) << 8);
+            if (h[x] == 'B' && h[x + 1] == 'C' && slen == 2 && x + 6 <= 12 + xlen) {
+                sz = (size_t(h[x + 4]) | (size_t(h[x + 5]) << 8)) + 1;
+                return sz >= 12 + xlen + 8 ? 1 : -1;
+            }
+            x += 4 + slen;
+        }
+        return -1;
+    }
+    /* drops consumed compressed bytes once they are most of the buffer (only
+     * between reads: collected members hold offsets into it) */
+    void compact() {
+        if (m_cpos < (size_t(8) << 20) || m_cpos * 2 < m_clen)
+            return;
+        std::memmove(m_cbuf.data(), m_cbuf.data() + m_cpos, m_clen - m_cpos);
+        m_clen -= m_cpos;
+        m_cpos = 0;
+    }
+    /* reads compressed bytes until need of them follow from */
+    bool fill(size_t from, size_t need) {
+        while (m_clen - from < need && !m_src_eof) {
+            if (m_cbuf.size() < m_clen + (size_t(4) << 20))
+                m_cbuf.resize(m_clen + (size_t(4) << 20));
+            const ssize_t r = ::read(m_fd, m_cbuf.data() + m_clen, m_cbuf.size() - m_clen);
+            if (r < 0) {
+                m_err = true;
+                return false;
+            }
+            if (r == 0)
+                m_src_eof = true;
+            m_clen += size_t(r);
+        }
+        return m_clen - from >= need;
+    }
+    /* the member at cpos: 1 found, 0 end of input, -1 not BGZF (or cut short) */
+    int next_member(size_t cpos, Member &m) {
+        if (!fill(cpos, 12))
+            return m_clen > cpos ? -1 : 0;
+        const unsigned char *h0 = reinterpret_cast<unsigned char *>(m_cbuf.data() + cpos);
+        const size_t xlen = size_t(h0[10]) | (size_t(h0[11]) << 8);
+        if (!fill(cpos, 12 + xlen))
+            return -1;
+        size_t sz = 0;
+        if (member_size(reinterpret_cast<unsigned char *>(m_cbuf.data() + cpos), m_clen - cpos,
+                        sz) != 1 ||
+            !fill(cpos, sz))
+            return -1;
+        const unsigned char *h = reinterpret_cast<unsigned char *>(m_cbuf.data() + cpos);
+        m.at = cpos;
+        m.csize = sz;
+        m.payload = 12 + xlen;
+        const unsigned char *t = h + sz - 8;
+        m.crc = uint32_t(t[0]) | (uint32_t(t[1]) << 8) | (uint32_t(t[2]) << 16) |
+                (uint32_t(t[3]) << 24);
+        m.isize = size_t(t[4]) | (size_t(t[5]) << 8) | (size_t(t[6]) << 16) | (size_t(t[7]) << 24);
+        return m.isize > (size_t(1) << 16) ? -1 : 1;
+    }
+    bool inflate_member(const Member &m, char *out) const {
+        z_stream z;
+        std::memset(&z, 0, sizeof(z));
+        if (inflateInit2(&z, -15) != Z_OK)
+            return false;
+        z.next_in =
+            reinterpret_cast<Bytef *>(const_cast<char *>(m_cbuf.data() + m.at + m.payload));
+        z.avail_in = uInt(m.csize - m.payload - 8);
+        z.next_out = reinterpret_cast<Bytef *>(out);
+        z.avail_out = uInt(m.isize);
+        const int r = inflate(&z, Z_FINISH);
+        const bool ok = r == Z_STREAM_END && z.total_out == m.isize;
+        inflateEnd(&z);
+        return ok && uint32_t(crc32(0L, reinterpret_cast<Bytef *>(out), uInt(m.isize))) == m.crc;
+    }
+    /* sequential zlib over the rest of the stream (gzip members of any kind);
+     * fills out to want unless the input ends (m_done) or is bad (m_err) */
+    size_t inflate_stream(char *out, size_t want) {
+        if (!m_zinit) {
+            std::memset(&m_z, 0, sizeof(m_z));
+            if (inflateInit2(&m_z, 15 + 16) != Z_OK) {
+                m_err = true;
+                return 0;
+            }
+            m_zinit = true;
+        }
+        size_t got = 0;
+        while (got < want) {
+            if (m_cpos >= m_clen && !fill(m_cpos, 1)) {
+                m_done = true;
+                if (m_zmid)
+                    m_err = true; /* the input ends inside a member */
+                break;
+            }
+            m_z.next_in = reinterpret_cast<Bytef *>(m_cbuf.data() + m_cpos);
+            m_z.avail_in = uInt(std::min<size_t>(m_clen - m_cpos, size_t(1) << 30));
+            m_z.next_out = reinterpret_cast<Bytef *>(out + got);
+            m_z.avail_out = uInt(std::min<size_t>(want - got, size_t(1) << 30));
+            const uInt in0 = m_z.avail_in, out0 = m_z.avail_out;
+            const int r = inflate(&m_z, Z_NO_FLUSH);
+            m_cpos += in0 - m_z.avail_in;
+            got += out0 - m_z.avail_out;
+            m_zmid = true;
+            if (r == Z_STREAM_END) {
+                inflateReset(&m_z);
+                m_zmid = false;
+            } else if (r != Z_OK && r != Z_BUF_ERROR) {
+                m_err = true;
+                break;
+            }
+        }
+        return got;
+    }
+
+    int m_fd;
+    unsigned m_threads;
+    std::vector<char> m_cbuf; /* compressed bytes [0, m_clen), unread from m_cpos */
+    size_t m_clen = 0, m_cpos = 0;
+    std::vector<char> m_pend; /* text of a member that did not fit, from m_pend_at */
+    size_t m_pend_at = 0;
+    bool m_src_eof = false, m_done = false, m_err = false, m_stream = false;
+    z_stream m_z;
+    bool m_zinit = false, m_zmid = false;
+};
+
 ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,
                                      size_t block_bytes)
     : m_threads(threads), m_block(std::max<size_t>(block_bytes, size_t(1) << 16)) {
@@ -299,7 +538,10 @@ ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,
             ::madvise(p, m_map_len, MADV_SEQUENTIAL);
         }
     }
-    if (!m_map) {
+    if (!m_map && S_ISREG(sb.st_mode) && BgzfSource::detect(m_fd)) {
+        m_bgzf.reset(new BgzfSource(m_fd, m_threads)); /* takes the descriptor */
+        m_fd = -1;
+    } else if (!m_map) {
         m_fp = gzdopen(m_fd, "r"); /* takes the descriptor */
         m_fd = -1;
         if (!m_fp)
@@ -333,6 +575,8 @@ ParallelSeqReader::~ParallelSeqReader() {
 
 size_t ParallelSeqReader::fill_raw(std::vector<char> &buf, size_t at, size_t want, bool &eof,
                                    bool &err) {
+    if (m_bgzf)
+        return m_bgzf->read(buf.data() + at, want, eof, err);
     size_t got = 0;
     while (got < want) {
         const size_t ask = std::min<size_t>(want - got, size_t(1) << 30);

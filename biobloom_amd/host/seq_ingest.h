@@ -25,7 +25,9 @@
  * per-share arena), so parsing copies nothing for one-line records.
  * Decompression of the next block overlaps the parse of the current one, and
  * parsed blocks queue up ahead of the caller, so parsing also overlaps the
- * caller's device work.
+ * caller's device work.  A BGZF file (blocked gzip, as bgzip writes: every
+ * member at most 64 KiB and its compressed size in the header) is inflated
+ * member by member on the parse threads.
  */
 #ifndef BBT_SEQ_INGEST_H
 #define BBT_SEQ_INGEST_H
@@ -127,6 +129,8 @@ struct SeqBlock {
     }
 };
 
+class BgzfSource;
+
 /* Multi-threaded reader with SeqReader's exact output. */
 class ParallelSeqReader {
   public:
@@ -161,6 +165,7 @@ class ParallelSeqReader {
     void parse_block(const char *b, size_t n, bool eof, SeqBlock &out, size_t &carry_from);
 
     gzFile m_fp = nullptr;
+    std::unique_ptr<BgzfSource> m_bgzf; /* BGZF file, inflated on threads */
     const char *m_map = nullptr; /* plain file, mapped */
     size_t m_map_len = 0;
     bool m_ok = false;

@@ -182,3 +182,69 @@ def test_strict_fastq_splits_without_reparse(L, tmp_path):
         recs, st, rp = dump(L, p, mode, threads, block)
         assert st == -1 and recs == ref
         assert rp == 0
+
+
+def bgzf(data, chunk=65280, level=6):
+    """data as BGZF: gzip members of at most `chunk` bytes of text, each with
+    the 'BC' extra subfield holding the member size - 1, then the empty
+    end-of-file member that bgzip writes."""
+    import struct
+    import zlib
+    out = []
+    for i in range(0, len(data), chunk):
+        part = data[i:i + chunk]
+        c = zlib.compressobj(level, zlib.DEFLATED, -15)
+        payload = c.compress(part) + c.flush()
+        out.append(b"\x1f\x8b\x08\x04\0\0\0\0\0\xff" + struct.pack("<H", 6) + b"BC" +
+                   struct.pack("<HH", 2, len(payload) + 25) + payload +
+                   struct.pack("<II", zlib.crc32(part), len(part)))
+    out.append(b"\x1f\x8b\x08\x04\0\0\0\0\0\xff\x06\0BC\x02\0\x1b\0\x03\0\0\0\0\0\0\0\0\0")
+    return b"".join(out)
+
+
+@pytest.mark.parametrize("chunk", [700, 9000, 65280])
+def test_bgzf_equal_model(L, tmp_path, chunk):
+    """A BGZF file is inflated member by member on threads: records and
+    status equal the sequential model for every reader mode, thread count
+    and block size (blocks of 64 KiB hold parts of members)."""
+    rng = np.random.default_rng(chunk)
+    data = random_file(rng, 1500)
+    want, status = kseq_model.read_all(data)
+    p = tmp_path / "in.bgz"
+    p.write_bytes(bgzf(data, chunk))
+    assert dump(L, p, 0)[:2] == (want, status)
+    for threads, block in [(1, 1 << 16), (4, 1 << 16), (8, 100_003), (16, 1 << 20)]:
+        for mode in (1, 2):
+            assert dump(L, p, mode, threads, block)[:2] == (want, status), (threads, block, mode)
+
+
+def test_bgzf_then_gzip_member(L, tmp_path):
+    """BGZF members followed by an ordinary gzip member: the reader goes on
+    with sequential zlib from that member, and the records are those of the
+    concatenated text."""
+    rng = np.random.default_rng(41)
+    a, b = random_file(rng, 600, strict=True), random_file(rng, 600, strict=True)
+    want, status = kseq_model.read_all(a + b)
+    p = tmp_path / "mix.gz"
+    p.write_bytes(bgzf(a, 5000)[:-28] + gzip.compress(b))
+    for mode, threads in [(0, 1), (1, 4), (2, 8)]:
+        assert dump(L, p, mode, threads)[:2] == (want, status)
+
+
+def test_bgzf_corrupt_member(L, tmp_path):
+    """A member whose contents fail their CRC ends the input with an error
+    status; the records before it are the model's."""
+    rng = np.random.default_rng(43)
+    data = random_file(rng, 1200, strict=True)
+    want, _ = kseq_model.read_all(data)
+    z = bytearray(bgzf(data, 4000))
+    z[len(z) // 2] ^= 0x55
+    p = tmp_path / "bad.bgz"
+    p.write_bytes(bytes(z))
+    for mode, threads in [(1, 1), (2, 4), (2, 16)]:
+        recs, st, _ = dump(L, p, mode, threads)
+        assert st < -1
+        assert 0 < len(recs) < len(want)
+        # the text ends inside a record: the last one may be cut short
+        assert recs[:-1] == want[:len(recs) - 1]
+        assert recs[-1][0] == want[len(recs) - 1][0]
