@@ -347,45 +347,65 @@ static std::string fmt(double v) {
     return s.str();
 }
 
-/* text compressed into one gzip member (zlib's default level, as gzopen
- * "wb" writes); a gzip file may hold several members one after another,
- * and every gzip reader returns their concatenated contents */
-static void gzip_member(const std::string &in, std::string &out) {
-    z_stream z;
-    std::memset(&z, 0, sizeof(z));
-    if (deflateInit2(&z, Z_DEFAULT_COMPRESSION, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) !=
-        Z_OK)
-        die("zlib: deflateInit2 failed");
-    out.resize(deflateBound(&z, uLong(in.size())));
-    z.next_in = reinterpret_cast<Bytef *>(const_cast<char *>(in.data()));
-    z.avail_in = uInt(in.size());
-    z.next_out = reinterpret_cast<Bytef *>(&out[0]);
-    z.avail_out = uInt(out.size());
-    if (deflate(&z, Z_FINISH) != Z_STREAM_END)
-        die("zlib: deflate failed");
-    out.resize(z.total_out);
-    deflateEnd(&z);
+/* text compressed as BGZF members (gzip members of at most 65280 bytes of
+ * text, each with the 'B','C' extra subfield holding the member's size - 1),
+ * at zlib's default level as gzopen "wb" writes: any gzip reader returns the
+ * concatenated text, and BGZF readers (this one included) can inflate the
+ * members in parallel */
+static void bgzf_members(const std::string &in, std::string &out) {
+    constexpr size_t kText = 65280;
+    out.clear();
+    for (size_t i = 0; i < in.size(); i += kText) {
+        const size_t n = std::min(kText, in.size() - i);
+        z_stream z;
+        std::memset(&z, 0, sizeof(z));
+        if (deflateInit2(&z, Z_DEFAULT_COMPRESSION, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) !=
+            Z_OK)
+            die("zlib: deflateInit2 failed");
+        const size_t h = out.size();
+        out.resize(h + 18 + deflateBound(&z, uLong(n)) + 8);
+        z.next_in = reinterpret_cast<Bytef *>(const_cast<char *>(in.data() + i));
+        z.avail_in = uInt(n);
+        z.next_out = reinterpret_cast<Bytef *>(&out[h + 18]);
+        z.avail_out = uInt(out.size() - h - 26);
+        if (deflate(&z, Z_FINISH) != Z_STREAM_END)
+            die("zlib: deflate failed");
+        const size_t total = 18 + z.total_out + 8;
+        deflateEnd(&z);
+        static const unsigned char kHead[16] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0,
+                                                0,    0xff, 6, 0, 'B', 'C', 2, 0};
+        std::memcpy(&out[h], kHead, 16);
+        out[h + 16] = char((total - 1) & 0xff);
+        out[h + 17] = char((total - 1) >> 8);
+        const uint32_t crc = uint32_t(crc32(0L, reinterpret_cast<const Bytef *>(in.data() + i),
+                                            uInt(n)));
+        unsigned char *t = reinterpret_cast<unsigned char *>(&out[h + total - 8]);
+        for (int b = 0; b < 4; ++b) {
+            t[b] = (unsigned char)(crc >> (8 * b));
+            t[4 + b] = (unsigned char)(uint32_t(n) >> (8 * b));
+        }
+        out.resize(h + total);
+    }
 }
 
+/* the empty member that ends a BGZF file */
+static const char kBgzfEof[28] = {'\x1f', '\x8b', 8, 4, 0, 0, 0, 0, 0, '\xff', 6, 0, 'B', 'C',
+                                  2, 0, 27, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
 namespace {
-/* an output file; gzip files receive whole gzip members */
+/* an output file; gzip files receive whole BGZF members */
 struct Out {
     FILE *f = nullptr;
-    bool gz = false, written = false;
+    bool gz = false;
     void write(const std::string &s) {
-        if (s.empty())
-            return;
-        if (std::fwrite(s.data(), 1, s.size(), f) != s.size())
+        if (!s.empty() && std::fwrite(s.data(), 1, s.size(), f) != s.size())
             die("write failed");
-        written = true;
     }
     void close() {
-        if (gz && !written) {
-            /* an empty gzip file, as closing an unwritten gzFile leaves */
-            std::string m;
-            gzip_member(std::string(), m);
-            write(m);
-        }
+        /* a gzip file ends with the empty member, so even an unwritten one is
+         * a valid (empty) gzip file */
+        if (gz && std::fwrite(kBgzfEof, 1, sizeof(kBgzfEof), f) != sizeof(kBgzfEof))
+            die("write failed");
         std::fclose(f);
     }
 };
@@ -596,7 +616,7 @@ void BioBloomClassifier::emit_batch(int s, const RecBatch &batch, bool paired,
             for (size_t k = 0; k < nouts; ++k)
                 if (!o[k].empty()) {
                     std::string z;
-                    gzip_member(o[k], z);
+                    bgzf_members(o[k], z);
                     o[k].swap(z);
                 }
     };

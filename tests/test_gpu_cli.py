@@ -320,10 +320,21 @@ def test_gz_partitions_from_threads(data, tmp_path):
     out = subprocess.run([CAT, "-p", str(tmp_path / "z"), "-f", bfs, "--fq", "-g", "--status",
                           "-d", "--batch_reads=8192", str(path)], check=True, timeout=120,
                          capture_output=True, text=True).stdout
+    eof = b"\x1f\x8b\x08\x04\0\0\0\0\0\xff\x06\0BC\x02\0\x1b\0\x03\0\0\0\0\0\0\0\0\0"
     for ci, n in enumerate(names):
         want = "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
                        for (rn, s), c in zip(many, cls) if c == ci)
         assert gzip.open(tmp_path / f"z_{n}.fq.gz", "rt").read() == want
+        raw = (tmp_path / f"z_{n}.fq.gz").read_bytes()
+        # BGZF: members with the 'BC' size subfield, then the empty end member
+        assert raw[:4] == b"\x1f\x8b\x08\x04" and raw[12:14] == b"BC" and raw.endswith(eof)
+    # a partition read back (BGZF input, members inflated on threads): every
+    # read lands in its own class again
+    g0 = [(rn, s) for (rn, s), c in zip(many, cls) if c == 0]
+    subprocess.run([CAT, "-p", str(tmp_path / "back"), "-f", bfs, "--status", "-t", "8",
+                    str(tmp_path / "z_g0.fq.gz")], check=True, timeout=120)
+    back = open(tmp_path / "back_readStatus.tsv").read().splitlines()
+    assert back == [f"{rn}\tg0" for rn, _ in g0]
     status = open(tmp_path / "z_readStatus.tsv").read().splitlines()
     assert status == [f"{rn}\t{names[c]}" for (rn, _), c in zip(many, cls)]
     assert out == "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
@@ -336,6 +347,8 @@ def test_gz_partitions_from_threads(data, tmp_path):
     for n in names:
         text = gzip.open(tmp_path / f"e_{n}.fq.gz", "rt").read()
         assert (text != "") == (n == "noMatch")
+        if n != "noMatch":
+            assert (tmp_path / f"e_{n}.fq.gz").read_bytes() == eof
 
 
 def test_score_outputs_stdout_and_gz(data, tmp_path):
