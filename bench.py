@@ -260,6 +260,8 @@ def serve_run(B, fs, params, pool, mode, qps, seconds, max_requests, seed, want)
             "submit_ms": stats["submit_ms"], "pack_ms": stats["pack_ms"],
             "device_ms": stats["device_ms"], "span_ms": (last - first) / 1e3,
             "max_waves_in_flight": stats["max_waves_in_flight"],
+            "probes": stats["probes"], "lookups": stats["lookups"],
+            "lookups_per_s": stats["lookups"] / ((last - first) / 1e6),
             "equal_to_alone": equal}
 
 
@@ -300,6 +302,12 @@ def bench_serve(args, torch, D, B):
                 f"span {r['span_ms']:.0f} ms: submit {r['submit_ms']:.0f}, pack "
                 f"{r['pack_ms']:.0f}, device {r['device_ms']:.0f} ms")
     top = runs["batched"][-1]
+    # the random-line rate over a buffer of the big filter's size: what the
+    # saturated server's lookups (one random 64 B line each) are measured against
+    gbytes = 1 << max(12, int(np.ceil(np.log2(max(big.m // 8, 4096)))))
+    glps, _ = B.measure_gather(gbytes, 64, 0)
+    log(f"random gather over {gbytes >> 20} MiB: {glps / 1e9:.1f} G lines/s; saturated "
+        f"server: {top['lookups_per_s'] / 1e9:.1f} G lookups/s")
     line = {
         "metric": METRIC, "value": top["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
         "steps": top["requests"], "warmup": 0,
@@ -311,6 +319,9 @@ def bench_serve(args, torch, D, B):
                    "scale": args.scale, "qps": qps_list, "filters": len(fs),
                    "big_filter_bits": big.m},
         "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+        "saturated_lookups_per_s": top["lookups_per_s"],
+        "random_gather": {"bytes": gbytes, "lines_per_s": glps},
+        "saturated_frac_of_random_gather": top["lookups_per_s"] / glps,
         "serving": runs,
         "alone_equals_mixed": all(r["equal_to_alone"] for m in runs.values() for r in m),
     }

@@ -267,6 +267,7 @@ struct bbt_server {
     std::condition_variable cv_dispatch, cv_complete, cv_done;
     std::thread dispatcher, completer;
     std::unique_ptr<CopyPool> copier;
+    uint64_t *dctr = nullptr; /* device: contains() calls, lookups, lookups issued, flag */
     bool stop = false;
     uint64_t next_id = 1, waves = 0, done_reqs = 0, done_reads = 0, max_inflight = 0;
     uint64_t wave_reads_sum = 0, wave_reqs_sum = 0;
@@ -348,6 +349,7 @@ bbt_status bbt_server::launch(Slot &s) {
     std::memset(&out, 0, sizeof(out));
     out.hits = reinterpret_cast<uint8_t *>(blk + L.hits);
     out.cls = reinterpret_cast<uint32_t *>(blk + L.cls);
+    out.counters = dctr; /* every launch adds to the server's totals */
     bbt_eval_params p = params;
     p.path = cfg.mode == BBT_SERVE_PLAIN ? BBT_PATH_PLAIN : BBT_PATH_BATCHED;
     p.flags = 0;
@@ -534,6 +536,8 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
         return sfail(BBT_ERR_ARG, "bbt_server_create: device_bytes below one full wave (" +
                                       std::to_string(Lmax.total) + " bytes)");
     SHIP(hipMalloc(&S->darena, cfg->device_bytes));
+    SHIP(hipMalloc(&S->dctr, 4 * sizeof(uint64_t)));
+    SHIP(hipMemset(S->dctr, 0, 4 * sizeof(uint64_t)));
     S->arena.reset(cfg->device_bytes);
     const uint64_t hb = align_up(cfg->host_bytes ? cfg->host_bytes
                                                  : bs * (uint64_t(S->cfg.slots) + 2),
@@ -705,6 +709,13 @@ bbt_status bbt_server_get_stats(bbt_server *S, bbt_server_stats *out) {
     out->submit_ms = S->submit_ms;
     out->pack_ms = S->pack_ms;
     out->device_ms = S->device_ms;
+    /* the counters as the finished launches left them (read while none runs
+     * for exact totals, e.g. after bbt_server_drain) */
+    uint64_t c[4] = {0, 0, 0, 0};
+    SHIP(hipMemcpy(c, S->dctr, sizeof(c), hipMemcpyDeviceToHost));
+    out->probes = c[0];
+    out->lookups = c[1];
+    out->lookups_issued = c[2];
     return BBT_OK;
 }
 
@@ -746,6 +757,7 @@ bbt_status bbt_server_destroy(bbt_server *S) {
         (void)hipHostFree(s.h_cls);
     }
     (void)hipFree(S->darena);
+    (void)hipFree(S->dctr);
     (void)hipHostFree(S->harena);
     delete S;
     return BBT_OK;

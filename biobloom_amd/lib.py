@@ -88,7 +88,8 @@ class ServerStats(C.Structure):
                 ("reads_done", C.c_uint64), ("max_waves_in_flight", C.c_uint64),
                 ("arena_bytes", C.c_uint64), ("arena_high_water", C.c_uint64),
                 ("mean_wave_reads", C.c_double), ("mean_wave_requests", C.c_double),
-                ("submit_ms", C.c_double), ("pack_ms", C.c_double), ("device_ms", C.c_double)]
+                ("submit_ms", C.c_double), ("pack_ms", C.c_double), ("device_ms", C.c_double),
+                ("probes", C.c_uint64), ("lookups", C.c_uint64), ("lookups_issued", C.c_uint64)]
 
 
 # exported symbols and their signatures: (restype, argtypes)

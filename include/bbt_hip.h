@@ -289,6 +289,9 @@ typedef struct {
     double submit_ms;  /* host time spent copying requests in bbt_server_submit */
     double pack_ms;    /* host time packing launches into pinned staging */
     double device_ms;  /* device time of the launches: copies in, kernels, copies out */
+    uint64_t probes;   /* contains() calls of the reference walk, over all launches */
+    uint64_t lookups;  /* their bit lookups (one random 64 B line each) */
+    uint64_t lookups_issued; /* bit lookups the kernels made (>= lookups) */
 } bbt_server_stats;
 
 /* p: the classification parameters of every request (BBT_STD or BBT_ORDERED;

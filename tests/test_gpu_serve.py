@@ -48,9 +48,11 @@ def make_requests(synth, genomes, n, seed, paired=False):
     return reqs
 
 
-def alone(D, B, fs, buf, off, params):
+def alone(D, B, fs, buf, off, params, counters=False):
     dr = D.DeviceReads(buf, off)
     res = D.classify(fs, dr, params)
+    if counters:
+        return res["hits"], res["cls"], res["counters"]
     return res["hits"], res["cls"]
 
 
@@ -78,10 +80,16 @@ def test_alone_equals_mixed(env, mode):
     st = srv.stats()
     srv.close()
     assert st["requests_done"] == len(reqs)
+    probes = lookups = 0
     for (buf, off), (h, c, info) in zip(reqs, got):
-        wh, wc = alone(D, B, fs, buf, off, params)
+        wh, wc, ctr = alone(D, B, fs, buf, off, params, counters=True)
         assert np.array_equal(h, wh) and np.array_equal(c, wc)
         assert info["done_us"] >= info["admit_us"] >= info["submit_us"]
+        probes += int(ctr[0])
+        lookups += int(ctr[1])
+    # the reference walk's contains() calls and lookups do not depend on how
+    # requests were merged into launches
+    assert st["probes"] == probes and st["lookups"] == lookups > 0
     if mode == "plain":
         assert st["mean_wave_requests"] == 1.0
     else:
