@@ -613,7 +613,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(reads shard across GPUs with replicated filters, see DESIGN.md)",
+              "(across GPUs, reads would be sharded over replicated filters)",
               file=sys.stderr)
         sys.exit(2)
 
