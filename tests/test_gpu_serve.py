@@ -185,3 +185,28 @@ def test_host_memory_admission(env):
     for (buf, off), (h, c, _) in zip(reqs, got):
         wh, wc = alone(D, B, fs, buf, off, params)
         assert np.array_equal(h, wh) and np.array_equal(c, wc)
+
+
+def test_large_request_copy(env):
+    """A request of 9 MB of bases goes into the pinned arena through the
+    server's split copy (requests of 4 MiB and more): its results, and those
+    of a small request behind it, equal the requests alone."""
+    torch, D, B, synth, fs, genomes = env
+    params = B.params()
+    rng = np.random.default_rng(17)
+    g = genomes[1]
+    starts = rng.integers(0, len(g) - 150, size=60_000)
+    items = [g[a:a + 150].copy() for a in starts[:30_000]]
+    items += [synth.codes_to_ascii(rng.integers(0, 4, size=150, dtype=np.uint8))
+              for _ in range(30_000)]
+    big = synth.pack(items)
+    small = make_requests(synth, genomes, 1, 18)[0]
+    print("[gpu] server: one 9 MB request and one small one", flush=True)
+    srv = B.Server(fs, params, slots=2, device_bytes=256 << 20, max_wave_reads=1 << 17,
+                   max_wave_bases=16 << 20)
+    got = run_server(srv, [big, small])
+    srv.close()
+    for (buf, off), (h, c, _) in zip([big, small], got):
+        wh, wc = alone(D, B, fs, buf, off, params)
+        assert np.array_equal(h, wh) and np.array_equal(c, wc)
+    assert int(got[0][1].size) == 60_000 and (got[0][1][:30_000] == 1).mean() > 0.9
