@@ -480,9 +480,9 @@ class BgzfSource {
         size_t got = 0;
         while (got < want) {
             if (m_cpos >= m_clen && !fill(m_cpos, 1)) {
+                /* the input ends (inside a member too: a stream cut short ends
+                 * there with the text inflated so far, as gzread ends it) */
                 m_done = true;
-                if (m_zmid)
-                    m_err = true; /* the input ends inside a member */
                 break;
             }
             m_z.next_in = reinterpret_cast<Bytef *>(m_cbuf.data() + m_cpos);
@@ -493,10 +493,8 @@ class BgzfSource {
             const int r = inflate(&m_z, Z_NO_FLUSH);
             m_cpos += in0 - m_z.avail_in;
             got += out0 - m_z.avail_out;
-            m_zmid = true;
             if (r == Z_STREAM_END) {
                 inflateReset(&m_z);
-                m_zmid = false;
             } else if (r != Z_OK && r != Z_BUF_ERROR) {
                 m_err = true;
                 break;
@@ -513,7 +511,7 @@ class BgzfSource {
     size_t m_pend_at = 0;
     bool m_src_eof = false, m_done = false, m_err = false, m_stream = false;
     z_stream m_z;
-    bool m_zinit = false, m_zmid = false;
+    bool m_zinit = false;
 };
 
 ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,

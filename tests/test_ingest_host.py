@@ -248,3 +248,18 @@ def test_bgzf_corrupt_member(L, tmp_path):
         # the text ends inside a record: the last one may be cut short
         assert recs[:-1] == want[:len(recs) - 1]
         assert recs[-1][0] == want[len(recs) - 1][0]
+
+
+def test_bgzf_cut_short_equals_sequential(L, tmp_path):
+    """A BGZF file cut anywhere (inside a header, a payload or a trailer, or
+    between members): the block-parallel reader gives the sequential
+    reader's (gzread's) records and status."""
+    rng = np.random.default_rng(47)
+    data = random_file(rng, 400, strict=True)
+    z = bgzf(data, 3000)
+    p = tmp_path / "cut.bgz"
+    for cut in sorted(set(int(x) for x in rng.integers(1, len(z), 40)) | {18, len(z) - 28}):
+        p.write_bytes(z[:cut])
+        want = dump(L, p, 0)[:2]
+        for mode, threads in [(1, 4), (2, 16)]:
+            assert dump(L, p, mode, threads)[:2] == want, (cut, mode)
