@@ -132,6 +132,50 @@ def cpu_baseline_mibf(mf, buf, off, nsample, params, gpu_rows, gpu_sat):
     return nsample / dt, dt, bool(parity)
 
 
+def cpu_baseline_mibf_e2e(mf, path, width, nsample):
+    """biobloommicategorizer's path on the host cores over the first nsample
+    reads of the FASTQ file (records of `width` bytes): read the bytes, parse
+    them with the oracle's kseq_read restatement (one thread), query the
+    oracle MIBF (OpenMP over reads) with the CLI's defaults, and count the
+    summary (hits, unique hits, multi, none per ID)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    info = mf.info()
+    bv = np.zeros(info.m // 64, np.uint64)
+    data = np.zeros(max(1, info.pop), np.uint32)
+    mf.download(bv, data)
+    seeds = mf.seeds() or O.Kmer(info.span, info.nseeds)
+    om = O.Mibf.from_arrays(bv, data, info.m, seeds, info.nids)
+    kw = dict(score=1e-10, allowed_miss=info.nseeds - 1, multi=1.0, min_nonsat=0, max_hits=64)
+    t0 = time.time()
+    with open(path, "rb") as fh:
+        text = fh.read(nsample * width)
+    sbuf, soff, _ = O.kseq_seqs(text)
+    t1 = time.time()
+    n = len(soff) - 1
+    reads = [bytes(sbuf[int(soff[i]):int(soff[i + 1])]) for i in range(n)]
+    rows, _, _ = om.query(reads, **kw)
+    t2 = time.time()
+    above = np.zeros(info.nids + 1, np.int64)
+    unique = np.zeros(info.nids + 1, np.int64)
+    multi = none = 0
+    for row in rows:
+        for h in row:
+            above[h[0]] += 1
+        if not row:
+            none += 1
+        elif len(row) == 1:
+            unique[row[0][0]] += 1
+        else:
+            multi += 1
+    t3 = time.time()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"value": n / (t3 - t0), "unit": "reads/s",
+            "cores": int(omp) if omp and omp.isdigit() else os.cpu_count(), "kind": "port",
+            "sample": f"first {n} reads: read + kseq parse (1 thread) {t1 - t0:.2f}s, MIBF "
+                      f"query (OpenMP) {t2 - t1:.2f}s, summary counts {t3 - t2:.2f}s"}
+
+
 def bench_mibf(args, torch, D, B):
     from biobloom_amd import synth
     t = time.time()
@@ -545,6 +589,10 @@ def bench_mibf_e2e(args, torch, D, B):
         log(f"mibf e2e -t {threads}: filtering {ft:.2f} s (wall {wall:.2f} s), "
             f"{item['reads_per_s'] / 1e6:.2f} M reads/s, {lines} lines")
         os.remove(out)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_mibf_e2e(mf, fq, 2 * reads.shape[1] + 18, min(nreads, 200_000))
+        log(f"cpu e2e baseline: {cpu['value'] / 1e6:.3f} M reads/s ({cpu['sample']})")
     best = max(runs, key=lambda r: r["reads_per_s"])
     line = {
         "metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1,
@@ -556,6 +604,7 @@ def bench_mibf_e2e(args, torch, D, B):
                                "device batches + per-read TSV lines + summary.tsv",
                    "scale": args.scale, "reads": nreads},
         "e2e": runs,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     import shutil
