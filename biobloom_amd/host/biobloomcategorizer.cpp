@@ -32,7 +32,8 @@ static void printHelpDialog() {
             "  -s, --score=N          Score threshold (0..1) or minimum match length (>1).\n"
             "  -b, --best_hit         The best hit is used rather than the score threshold.\n"
             "  -w, --with_score       Output multimatches with scores in the order of filter.\n"
-            "  -t, --threads=N        Threads that parse the input. [all, at most 16]\n"
+            "  -t, --threads=N        Threads that parse each input file. [all; at most 8\n"
+            "                         for plain files, 16 for gzip]\n"
             "  -g, --gz_output        Outputs all output files in compressed gzip.\n"
             "      --fa               Output categorized reads in Fasta files.\n"
             "      --fq               Output categorized reads in Fastq files.\n"

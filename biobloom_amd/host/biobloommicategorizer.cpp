@@ -47,7 +47,7 @@ struct Opt {
     uint64_t interval = 10000000;
     int out = TSV;
     int verbose = 0;
-    unsigned threads = 0; /* threads that parse the input (0: all, at most 16) */
+    unsigned threads = 0; /* parse threads per file (0: the reader's default) */
     uint64_t batch = uint64_t(1) << 17; /* reads (or pairs) per device batch */
 };
 
@@ -65,7 +65,8 @@ void usage() {
             "  -f, --filter=N         Path of miBF '.bf' file\n"
             "  -e, --paired_mode      Uses paired-end information.\n"
             "  -s, --min_FPR=N        Minimum -10*log(FPR) threshold for a match. [100.0]\n"
-            "  -t, --threads=N        Threads that parse the input. [all, at most 16]\n"
+            "  -t, --threads=N        Threads that parse each input file. [all; at most 8\n"
+            "                         for plain files, 16 for gzip]\n"
             "  -i, --hitOnly          Return only results that hit the filter.\n"
             "      --fa               Output categorized reads in Fasta output.\n"
             "      --fq               Output categorized reads in Fastq output.\n"

@@ -40,7 +40,7 @@ struct Options {
     bool dust = false;               /* opt::dust (-D) */
     uint32_t dust_t = 20;            /* opt::dustT (-T) */
     uint32_t dust_w = 64;            /* opt::dustWindow (-W) */
-    unsigned threads = 0;            /* opt::threads (-t): parsing threads, 0 = all */
+    unsigned threads = 0;            /* opt::threads (-t): parse threads per file, 0 = the reader's default */
     uint64_t ingest_block = 16u << 20; /* bytes of input parsed per block */
 };
 

@@ -517,8 +517,10 @@ class BgzfSource {
 ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,
                                      size_t block_bytes)
     : m_threads(threads), m_block(std::max<size_t>(block_bytes, size_t(1) << 16)) {
-    if (m_threads == 0)
-        m_threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const bool chosen = m_threads != 0;
+    if (!chosen)
+        m_threads = std::min(16u, hw);
     m_fd = ::open(path.c_str(), O_RDONLY);
     if (m_fd < 0)
         return;
@@ -536,6 +538,10 @@ ParallelSeqReader::ParallelSeqReader(const std::string &path, unsigned threads,
             ::madvise(p, m_map_len, MADV_SEQUENTIAL);
         }
     }
+    /* parsing mapped text is fast: past 8 threads the shares mostly wait
+     * for memory and take cores from the rest of the pipeline */
+    if (m_map && !chosen)
+        m_threads = std::min(8u, hw);
     if (!m_map && S_ISREG(sb.st_mode) && BgzfSource::detect(m_fd)) {
         m_bgzf.reset(new BgzfSource(m_fd, m_threads)); /* takes the descriptor */
         m_fd = -1;

@@ -138,7 +138,8 @@ class ParallelSeqReader {
     /* parsed blocks alive at once, queued ahead or held by the caller: a
      * caller that holds this many blocks waits forever for the next one */
     static constexpr unsigned kMaxBlocks = 48;
-    /* threads = parse threads (0: the hardware's, at most 16) */
+    /* threads = parse threads (0: the hardware's, at most 8 for a mapped
+     * plain file and 16 for compressed or streamed input) */
     explicit ParallelSeqReader(const std::string &path, unsigned threads = 0,
                                size_t block_bytes = size_t(16) << 20);
     /* every BlockRef must be released before the reader is destroyed */
