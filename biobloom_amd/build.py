@@ -51,7 +51,7 @@ def build_cli():
         _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-Wall",
               "-o", out, src, os.path.join(HOST, "front_end.cpp"),
               os.path.join(HOST, "seq_ingest.cpp"), "-pthread",
-              "-Wl,-rpath,$ORIGIN", "-L", PKG, "-lbbt_hip", "-lz"])
+              "-Wl,-rpath,$ORIGIN", "-L", PKG, "-lbbt_hip", "-lz", "-ldl"])
 
 
 def build():

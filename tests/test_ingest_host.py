@@ -28,7 +28,7 @@ def build():
         return
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread",
                     "-I", os.path.join(ROOT, "biobloom_amd", "host"), "-o", LIB,
-                    SRCS[0], SRCS[1], "-lz"], check=True)
+                    SRCS[0], SRCS[1], "-lz", "-ldl"], check=True)
 
 
 @pytest.fixture(scope="module")
@@ -202,11 +202,14 @@ def bgzf(data, chunk=65280, level=6):
     return b"".join(out)
 
 
+@pytest.mark.parametrize("zlib_only", [False, True])
 @pytest.mark.parametrize("chunk", [700, 9000, 65280])
-def test_bgzf_equal_model(L, tmp_path, chunk):
-    """A BGZF file is inflated member by member on threads: records and
-    status equal the sequential model for every reader mode, thread count
-    and block size (blocks of 64 KiB hold parts of members)."""
+def test_bgzf_equal_model(L, tmp_path, monkeypatch, chunk, zlib_only):
+    """A BGZF file is inflated member by member on threads (with libdeflate
+    where the system has it, and with zlib): records and status equal the
+    sequential model for every reader mode, thread count and block size
+    (blocks of 64 KiB hold parts of members)."""
+    monkeypatch.setenv("BBT_NO_LIBDEFLATE", "1" if zlib_only else "0")
     rng = np.random.default_rng(chunk)
     data = random_file(rng, 1500)
     want, status = kseq_model.read_all(data)
@@ -231,9 +234,11 @@ def test_bgzf_then_gzip_member(L, tmp_path):
         assert dump(L, p, mode, threads)[:2] == (want, status)
 
 
-def test_bgzf_corrupt_member(L, tmp_path):
+@pytest.mark.parametrize("zlib_only", [False, True])
+def test_bgzf_corrupt_member(L, tmp_path, monkeypatch, zlib_only):
     """A member whose contents fail their CRC ends the input with an error
     status; the records before it are the model's."""
+    monkeypatch.setenv("BBT_NO_LIBDEFLATE", "1" if zlib_only else "0")
     rng = np.random.default_rng(43)
     data = random_file(rng, 1200, strict=True)
     want, _ = kseq_model.read_all(data)
