@@ -9,6 +9,8 @@
  */
 #include "front_end.h"
 
+#include "deflate_rt.h"
+
 #include <hip/hip_runtime.h>
 
 #include <cctype>
@@ -349,36 +351,48 @@ static std::string fmt(double v) {
 
 /* text compressed as BGZF members (gzip members of at most 65280 bytes of
  * text, each with the 'B','C' extra subfield holding the member's size - 1),
- * at zlib's default level as gzopen "wb" writes: any gzip reader returns the
- * concatenated text, and BGZF readers (this one included) can inflate the
- * members in parallel */
+ * at zlib's default level (6) as gzopen "wb" writes, with libdeflate when the
+ * system has it: any gzip reader returns the concatenated text, and BGZF
+ * readers (this one included) can inflate the members in parallel */
 static void bgzf_members(const std::string &in, std::string &out) {
     constexpr size_t kText = 65280;
+    constexpr int kLevel = 6; /* Z_DEFAULT_COMPRESSION */
+    const Deflate *fast = Deflate::get();
     out.clear();
     for (size_t i = 0; i < in.size(); i += kText) {
         const size_t n = std::min(kText, in.size() - i);
-        z_stream z;
-        std::memset(&z, 0, sizeof(z));
-        if (deflateInit2(&z, Z_DEFAULT_COMPRESSION, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) !=
-            Z_OK)
-            die("zlib: deflateInit2 failed");
         const size_t h = out.size();
-        out.resize(h + 18 + deflateBound(&z, uLong(n)) + 8);
-        z.next_in = reinterpret_cast<Bytef *>(const_cast<char *>(in.data() + i));
-        z.avail_in = uInt(n);
-        z.next_out = reinterpret_cast<Bytef *>(&out[h + 18]);
-        z.avail_out = uInt(out.size() - h - 26);
-        if (deflate(&z, Z_FINISH) != Z_STREAM_END)
-            die("zlib: deflate failed");
-        const size_t total = 18 + z.total_out + 8;
-        deflateEnd(&z);
+        size_t total = 0;
+        if (fast) {
+            const size_t room = fast->bound(kLevel, n);
+            out.resize(h + 18 + room + 8);
+            const size_t c = fast->deflate(kLevel, in.data() + i, n, &out[h + 18], room);
+            if (!c)
+                die("libdeflate: compression failed");
+            total = 18 + c + 8;
+        } else {
+            z_stream z;
+            std::memset(&z, 0, sizeof(z));
+            if (deflateInit2(&z, kLevel, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK)
+                die("zlib: deflateInit2 failed");
+            out.resize(h + 18 + deflateBound(&z, uLong(n)) + 8);
+            z.next_in = reinterpret_cast<Bytef *>(const_cast<char *>(in.data() + i));
+            z.avail_in = uInt(n);
+            z.next_out = reinterpret_cast<Bytef *>(&out[h + 18]);
+            z.avail_out = uInt(out.size() - h - 26);
+            if (deflate(&z, Z_FINISH) != Z_STREAM_END)
+                die("zlib: deflate failed");
+            total = 18 + z.total_out + 8;
+            deflateEnd(&z);
+        }
         static const unsigned char kHead[16] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0,
                                                 0,    0xff, 6, 0, 'B', 'C', 2, 0};
         std::memcpy(&out[h], kHead, 16);
         out[h + 16] = char((total - 1) & 0xff);
         out[h + 17] = char((total - 1) >> 8);
-        const uint32_t crc = uint32_t(crc32(0L, reinterpret_cast<const Bytef *>(in.data() + i),
-                                            uInt(n)));
+        const uint32_t crc =
+            fast ? fast->crc32(in.data() + i, n)
+                 : uint32_t(crc32(0L, reinterpret_cast<const Bytef *>(in.data() + i), uInt(n)));
         unsigned char *t = reinterpret_cast<unsigned char *>(&out[h + total - 8]);
         for (int b = 0; b < 4; ++b) {
             t[b] = (unsigned char)(crc >> (8 * b));

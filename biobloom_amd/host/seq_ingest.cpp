@@ -4,6 +4,8 @@
  */
 #include "seq_ingest.h"
 
+#include "deflate_rt.h"
+
 #include <dlfcn.h>
 #include <fcntl.h>
 #include <sys/mman.h>
@@ -282,51 +284,6 @@ int64_t SeqReader::read(SeqRecord &rec) {
 
 /* ---- BGZF ---------------------------------------------------------------- */
 
-/* libdeflate, when the system has it (loaded at run time; its public API:
- * a decompressor object, whole-buffer raw deflate decoding, CRC32), inflates
- * BGZF members faster than zlib; without it zlib does.  BBT_NO_LIBDEFLATE=1
- * keeps zlib. */
-struct FastInflate {
-    void *(*alloc)() = nullptr;
-    int (*raw)(void *, const void *, size_t, void *, size_t, size_t *) = nullptr;
-    void (*release)(void *) = nullptr;
-    uint32_t (*crc)(uint32_t, const void *, size_t) = nullptr;
-    bool ok = false;
-    FastInflate() {
-        void *h = ::dlopen("libdeflate.so.0", RTLD_NOW | RTLD_LOCAL);
-        if (!h)
-            return;
-        alloc = reinterpret_cast<void *(*)()>(::dlsym(h, "libdeflate_alloc_decompressor"));
-        raw = reinterpret_cast<int (*)(void *, const void *, size_t, void *, size_t, size_t *)>(
-            ::dlsym(h, "libdeflate_deflate_decompress"));
-        release = reinterpret_cast<void (*)(void *)>(::dlsym(h, "libdeflate_free_decompressor"));
-        crc = reinterpret_cast<uint32_t (*)(uint32_t, const void *, size_t)>(
-            ::dlsym(h, "libdeflate_crc32"));
-        ok = alloc && raw && release && crc;
-    }
-    static const FastInflate &get() {
-        static const FastInflate f;
-        return f;
-    }
-    /* this thread's decompressor (freed when the thread ends) */
-    void *decompressor() const {
-        struct Holder {
-            void *d = nullptr;
-            void (*rel)(void *) = nullptr;
-            ~Holder() {
-                if (d)
-                    rel(d);
-            }
-        };
-        thread_local Holder t;
-        if (!t.d) {
-            t.d = alloc();
-            t.rel = release;
-        }
-        return t.d;
-    }
-};
-
 /* A BGZF stream (RFC 1952 members, each with the extra subfield 'B','C' of
  * length 2 holding the member's size - 1, at most 64 KiB of text each).
  * read() fills a buffer by inflating whole members on threads straight into
@@ -336,10 +293,8 @@ struct FastInflate {
  * as gzread would read it. */
 class BgzfSource {
   public:
-    BgzfSource(int fd, unsigned threads) : m_fd(fd), m_threads(std::max(1u, threads)) {
-        const char *no = std::getenv("BBT_NO_LIBDEFLATE");
-        m_fast = FastInflate::get().ok && !(no && no[0] == '1') ? &FastInflate::get() : nullptr;
-    }
+    BgzfSource(int fd, unsigned threads)
+        : m_fd(fd), m_threads(std::max(1u, threads)), m_fast(Deflate::get()) {}
     ~BgzfSource() {
         if (m_zinit)
             inflateEnd(&m_z);
@@ -504,12 +459,9 @@ class BgzfSource {
     bool inflate_member(const Member &m, char *out) const {
         const char *in = m_cbuf.data() + m.at + m.payload;
         const size_t in_n = m.csize - m.payload - 8;
-        if (m_fast) {
-            void *d = m_fast->decompressor();
-            size_t n = 0;
-            return d && m_fast->raw(d, in, in_n, out, m.isize, &n) == 0 && n == m.isize &&
-                   m_fast->crc(0, out, m.isize) == m.crc;
-        }
+        if (m_fast)
+            return m_fast->inflate(in, in_n, out, m.isize) &&
+                   m_fast->crc32(out, m.isize) == m.crc;
         z_stream z;
         std::memset(&z, 0, sizeof(z));
         if (inflateInit2(&z, -15) != Z_OK)
@@ -563,7 +515,7 @@ class BgzfSource {
 
     int m_fd;
     unsigned m_threads;
-    const FastInflate *m_fast = nullptr;
+    const Deflate *m_fast = nullptr;
     std::vector<char> m_cbuf; /* compressed bytes [0, m_clen), unread from m_cpos */
     size_t m_clen = 0, m_cpos = 0;
     std::vector<char> m_pend; /* text of a member that did not fit, from m_pend_at */

@@ -328,6 +328,13 @@ def test_gz_partitions_from_threads(data, tmp_path):
         raw = (tmp_path / f"z_{n}.fq.gz").read_bytes()
         # BGZF: members with the 'BC' size subfield, then the empty end member
         assert raw[:4] == b"\x1f\x8b\x08\x04" and raw[12:14] == b"BC" and raw.endswith(eof)
+    # zlib instead of libdeflate for the members: the same text
+    env = dict(os.environ, BBT_NO_LIBDEFLATE="1")
+    subprocess.run([CAT, "-p", str(tmp_path / "zz"), "-f", bfs, "--fq", "-g",
+                    "--batch_reads=8192", str(path)], check=True, timeout=120, env=env)
+    for n in names:
+        assert gzip.open(tmp_path / f"zz_{n}.fq.gz").read() == \
+            gzip.open(tmp_path / f"z_{n}.fq.gz").read()
     # a partition read back (BGZF input, members inflated on threads): every
     # read lands in its own class again
     g0 = [(rn, s) for (rn, s), c in zip(many, cls) if c == 0]

@@ -20,7 +20,8 @@ NATIVE = os.path.join(HERE, "native")
 LIB = os.path.join(NATIVE, "libingest_host.so")
 SRCS = [os.path.join(NATIVE, "ingest_host.cpp"),
         os.path.join(ROOT, "biobloom_amd", "host", "seq_ingest.cpp"),
-        os.path.join(ROOT, "biobloom_amd", "host", "seq_ingest.h")]
+        os.path.join(ROOT, "biobloom_amd", "host", "seq_ingest.h"),
+        os.path.join(ROOT, "biobloom_amd", "host", "deflate_rt.h")]
 
 
 def build():
