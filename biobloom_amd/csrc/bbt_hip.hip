@@ -881,6 +881,7 @@ struct bbt_filterset {
     std::vector<uint32_t *> dbinom;
     double binom_min_fpr = -1;
     uint32_t binom_frames = 0;
+    std::vector<double> binom_fpr; /* each filter's FPR the thresholds were made for */
     /* BINOMIAL evalScore tables per filter (and the FPR they were made for) */
     std::vector<double *> dbscore;
     std::vector<double> bscore_fpr;
@@ -1438,15 +1439,21 @@ bbt_status launch_ordered(const Launch &L, uint32_t scorer, uint64_t nrec, uint3
 }
 
 /* calcMinCount tables for every frame length up to max_frames */
+/* (remade when min_fpr, the frame range or any filter's FPR changed: uploads
+ * and refreshes change the FPR, (popcount / m)^h) */
 bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) {
-    if (!fs->dbinom.empty() && fs->binom_min_fpr == min_fpr &&
-        fs->binom_frames >= max_frames)
+    bool fresh = !fs->dbinom.empty() && fs->binom_min_fpr == min_fpr &&
+                 fs->binom_frames >= max_frames && fs->binom_fpr.size() == fs->desc.size();
+    for (size_t f = 0; fresh && f < fs->desc.size(); ++f)
+        fresh = fs->binom_fpr[f] == fs->info[f].fpr;
+    if (fresh)
         return BBT_OK;
     HIP_OK(hipDeviceSynchronize()); /* in-flight batches read the old tables */
     for (uint32_t *p : fs->dbinom)
         if (p)
             (void)hipFree(p);
     fs->dbinom.clear();
+    fs->binom_fpr.clear();
     for (size_t f = 0; f < fs->desc.size(); ++f) {
         std::vector<uint32_t> t(size_t(max_frames) + 1, 1u);
         for (uint32_t fl = 1; fl <= max_frames; ++fl)
@@ -1455,6 +1462,7 @@ bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) 
         HIP_OK(hipMalloc(&d, t.size() * sizeof(uint32_t)));
         HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         fs->dbinom.push_back(d);
+        fs->binom_fpr.push_back(fs->info[f].fpr);
     }
     fs->binom_min_fpr = min_fpr;
     fs->binom_frames = max_frames;
@@ -1698,11 +1706,18 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     return BBT_OK;
 }
 
-/* identity of a captured batch: every pointer and value the kernels see */
-static std::string graph_key(const bbt_reads *in, const bbt_eval_params *p,
-                             const bbt_eval_out *out, const Prepared &P, hipStream_t st) {
+/* identity of a captured batch: every pointer and value the kernels see,
+ * the filter set's descriptors included (a workspace may serve several
+ * filter sets with the same buffers) */
+static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
+                             const bbt_eval_params *p, const bbt_eval_out *out,
+                             const Prepared &P, hipStream_t st) {
     std::string k;
     auto add = [&k](const void *v, size_t n) { k.append(static_cast<const char *>(v), n); };
+    add(&fs, sizeof(fs));
+    add(&P.L.a.filters, sizeof(P.L.a.filters));
+    add(&P.nf, sizeof(P.nf));
+    add(&P.kk, sizeof(P.kk));
     add(in, sizeof(*in));
     add(p, sizeof(*p));
     add(out, sizeof(*out));
@@ -1733,7 +1748,7 @@ static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads
         if (s != BBT_OK)
             return s;
     } else {
-        const std::string key = graph_key(in, p, out, P, st);
+        const std::string key = graph_key(fs, in, p, out, P, st);
         if (!W.graph_exec || key != W.graph_key) {
             if (W.graph_exec) {
                 (void)hipGraphExecDestroy(W.graph_exec);

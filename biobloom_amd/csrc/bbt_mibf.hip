@@ -889,6 +889,12 @@ struct bbt_mibf {
     uint64_t wide_cap = 0;
     uint32_t *dspill = nullptr;
     int cu = 0;
+    /* recorded after each query's kernels: the work queue, the wide-pass
+     * lists and the minimum-count table belong to the filter, so the next
+     * query (on any stream) waits for it, and the host waits for it before
+     * freeing or refilling those buffers */
+    hipEvent_t last = nullptr;
+    bool has_last = false;
     std::mutex mu;
 };
 
@@ -930,6 +936,7 @@ bbt_status mibf_init(bbt_mibf *mf, const char *const *seeds, uint32_t nseeds,
     MHIP(hipGetDevice(&dev));
     MHIP(hipDeviceGetAttribute(&mf->cu, hipDeviceAttributeMultiprocessorCount, dev));
     MHIP(hipMalloc(&mf->dctl, 16 * sizeof(unsigned long long)));
+    MHIP(hipEventCreateWithFlags(&mf->last, hipEventDisableTiming));
     return BBT_OK;
 }
 
@@ -989,6 +996,8 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
     (void)hipFree(mf->dctl);
     (void)hipFree(mf->dwide_list);
     (void)hipFree(mf->dspill);
+    if (mf->last)
+        (void)hipEventDestroy(mf->last);
     delete mf;
     return BBT_OK;
 }
@@ -1328,8 +1337,17 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
                      "bbt_mibf_query_batch: minimum-count table above 1 GiB (reads too long "
                      "for this many IDs)");
     const uint32_t miss = p->allowed_miss;
+    /* the previous query (on any stream) may still read the buffers that are
+     * refilled or replaced below */
+    const auto wait_last = [&]() -> bbt_status {
+        if (mf->has_last)
+            MHIP(hipEventSynchronize(mf->last));
+        return BBT_OK;
+    };
     if (!mf->dmin || mf->min_score != p->score || mf->min_miss != miss ||
         mf->min_frames < frames) {
+        if (wait_last() != BBT_OK)
+            return BBT_ERR_HIP;
         /* getMinCount (MIBFClassifier.hpp:930-939) for every frame count up to
          * the longest read of the batch; rows already computed for the same
          * score and allowed misses are kept, the others are split over host
@@ -1392,6 +1410,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     a.spill_stride = mf->d.nids > kMaxCand ? mf->d.nids - kMaxCand : 0;
     if (a.spill_stride) {
         if (mf->wide_cap < nrec) {
+            if (wait_last() != BBT_OK)
+                return BBT_ERR_HIP;
             (void)hipFree(mf->dwide_list);
             mf->dwide_list = nullptr;
             mf->wide_cap = 0;
@@ -1404,6 +1424,10 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     }
     a.wide_list = mf->dwide_list;
     a.spill = mf->dspill;
+    /* the queue counters are shared by every query of this filter: this
+     * query's kernels start after the previous query's have finished */
+    if (mf->has_last)
+        MHIP(hipStreamWaitEvent(st, mf->last, 0));
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 3);
     const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
     const bool few = mf->d.S <= 4;
@@ -1420,6 +1444,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
         few ? query_pair<8192, 4, false>(a, blocks, 0, st)
             : query_pair<8192, 8, false>(a, blocks, 0, st);
     MHIP(hipGetLastError());
+    MHIP(hipEventRecord(mf->last, st));
+    mf->has_last = true;
     return BBT_OK;
 }
 

@@ -10,7 +10,8 @@
  * Differences kept on purpose: reads are counted for the summary in every
  * thread setting (the reference counts them only in its multi-threaded
  * reader); -r (streak) does not stop a query early, every frame is
- * evaluated; --debug is not available.
+ * evaluated, and a warning says so when -r is given (the streak rule lives in
+ * the absent btl_bloomfilter MIBFQuerySupport); --debug is not available.
  */
 #include <getopt.h>
 #include <hip/hip_runtime.h>
@@ -80,6 +81,8 @@ void usage() {
             "                         Ignored if k-mers used when indexing.\n"
             "  -m, --multi=N          Multi Match threshold. [1.0]\n"
             "  -r, --streak=N         Number of additional hits needed to skip classification. [3]\n"
+            "                         Accepted and checked against -c; it has no other effect\n"
+            "                         here: every frame of a read is evaluated.\n"
             "  -c, --minNoSat=N       Minimum count of non saturated matches. Increasing this value\n"
             "                         will filter out repetitive or low quality sequences.[0]\n"
             "  -b, --bestHitAgree     Filters out all matches where best hit is ambiguous because\n"
@@ -684,7 +687,7 @@ class MIBFClassifier {
 int main(int argc, char *argv[]) {
     Opt o;
     int fq = 0, fa = 0, tsv = 0, version = 0, debug = 0;
-    bool die = false;
+    bool die = false, streak_given = false;
     static struct option lo[] = {{"prefix", required_argument, NULL, 'p'},
                                  {"filter", required_argument, NULL, 'f'},
                                  {"paired", no_argument, NULL, 'e'},
@@ -753,6 +756,7 @@ int main(int argc, char *argv[]) {
                 cerr << "Error - Invalid parameter! r: " << optarg << endl;
                 exit(EXIT_FAILURE);
             }
+            streak_given = true;
             break;
         case 'c':
             if (!(cv >> o.minNoSat)) {
@@ -791,6 +795,10 @@ int main(int argc, char *argv[]) {
         cerr << "-r should be greater or equalt to -c" << endl;
         exit(1);
     }
+    if (streak_given)
+        cerr << "Warning: -r (streak) does not stop a query early here; every frame of "
+                "a read is evaluated (the reference's streak rule is not available)"
+             << endl;
     if (inputs.empty()) {
         cerr << "Error: Need Input File" << endl;
         die = true;

@@ -372,7 +372,14 @@ class BgzfSource {
                 }
                 m_cpos = cpos + m.csize;
             } else if (r < 0) {
-                m_stream = true; /* not a BGZF member: sequential from here */
+                /* not a BGZF member.  Bytes after a member that do not start
+                 * another gzip member are trailing garbage, which gzread
+                 * ignores: the input ends there.  Any other gzip member is
+                 * inflated sequentially from here. */
+                if (cpos > 0 && !gzip_magic(cpos))
+                    m_done = true;
+                else
+                    m_stream = true;
             } else {
                 m_done = true; /* no more input */
             }
@@ -432,6 +439,14 @@ class BgzfSource {
             m_clen += size_t(r);
         }
         return m_clen - from >= need;
+    }
+    /* true when the compressed bytes at pos start a gzip member (two magic
+     * bytes; a lone last byte does not) */
+    bool gzip_magic(size_t pos) {
+        if (!fill(pos, 2))
+            return false;
+        const unsigned char *h = reinterpret_cast<unsigned char *>(m_cbuf.data() + pos);
+        return h[0] == 0x1f && h[1] == 0x8b;
     }
     /* the member at cpos: 1 found, 0 end of input, -1 not BGZF (or cut short) */
     int next_member(size_t cpos, Member &m) {
@@ -505,6 +520,11 @@ class BgzfSource {
             got += out0 - m_z.avail_out;
             if (r == Z_STREAM_END) {
                 inflateReset(&m_z);
+                /* trailing garbage after a member ends the input (gzread) */
+                if (!gzip_magic(m_cpos)) {
+                    m_done = true;
+                    break;
+                }
             } else if (r != Z_OK && r != Z_BUF_ERROR) {
                 m_err = true;
                 break;

@@ -394,7 +394,11 @@ bbt_status bbt_mibf_frame_probs(const bbt_mibf *mf, uint32_t allowed_miss, doubl
 uint32_t bbt_mibf_min_count(uint64_t frames, double prob, double score, uint32_t nids);
 /* Classifies a batch of reads or pairs (bbt_reads as for bbt_eval_batch;
  * order is ignored); one wavefront per record.  Records of up to 8192
- * frames x hashes (both mates of a pair together). */
+ * frames x hashes (both mates of a pair together).  The work queue and the
+ * wide-pass lists belong to the filter: calls on one filter run one after
+ * another on the device (each call's kernels wait for the previous call's,
+ * whichever stream either was given), so calls from several streams are
+ * correct but do not overlap. */
 bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mibf_params *p,
                                 bbt_mibf_out *out, void *stream);
 

@@ -377,3 +377,40 @@ def test_query_reads_beyond_the_record_store(dev):
         assert got == want
         assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[3] == 0
     assert all(row for row in want)
+
+
+def test_queries_on_two_streams(dev):
+    """Two batches queried on two streams without a host wait between them:
+    the filter's shared work queue must not mix them (the second call's
+    kernels wait for the first's); both equal the oracle."""
+    D, B = dev
+    import torch
+    rng = random.Random(77)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=3000)
+    om = O.Mibf(recs, seeds, occupancy=0.5)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    ra = reads_from(rng, recs, 3000)
+    rb = reads_from(rng, recs, 2000, lens=(150, 250))
+    da, db = device_reads(D, ra), device_reads(D, rb)
+    p = B.mibf_params(1e-10, 1.0, 2, 0, 8)
+    oa, ob = D.MibfOutputs(len(ra), 8), D.MibfOutputs(len(rb), 8)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        print(f"[gpu] mibf two streams rep={rep}", flush=True)
+        oa.counters.zero_()
+        ob.counters.zero_()
+        oa.nhits.fill_(-1)
+        ob.nhits.fill_(-1)
+        torch.cuda.synchronize()
+        mf.query_batch(da.struct(), p, oa.struct(), int(sa.cuda_stream))
+        mf.query_batch(db.struct(), p, ob.struct(), int(sb.cuda_stream))
+        torch.cuda.synchronize()
+        for reads, out in ((ra, oa), (rb, ob)):
+            want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=1.0,
+                                        min_nonsat=0, max_hits=8)
+            got, gsat, gctr, _ = out.host()
+            assert got == want and gsat == wsat
+            assert gctr[0] == wctr[0] and gctr[1] == wctr[1]

@@ -281,3 +281,29 @@ def test_dust(dev, multi):
     check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), mode=O.BESTHIT)
     check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), mode=O.SCORES, scorer=O.LENGTH,
                 thr=30.0)
+
+
+def test_binomial_thresholds_follow_uploads(dev):
+    """BINOMIAL thresholds depend on each filter's FPR ((popcount / m)^h):
+    after new bits are uploaded into a filter, the next batch must use
+    thresholds made for the new FPR (checked against the oracle on the new
+    bits)."""
+    D, B = dev
+    rng = np.random.default_rng(31)
+    genomes = small_genomes(rng, 2, 60_000)
+    fs, _ = D.build_filters(genomes, 25, 0.075)
+    ofs = oracle_filters_like(fs, B)
+    from biobloom_amd import synth
+    reads, _, _ = synth.fixed_len_reads(synth.rng_for(9), genomes, 600, 150, 0.03, 0.5)
+    buf, off = O.pack_reads([bytes(r) for r in reads])
+    check_batch(D, B, fs, ofs, buf, off, scorer=O.BINOMIAL, thr=1e-10)
+    # a denser filter 0: its bits OR'ed with 10% random bits (a higher FPR)
+    extra = np.packbits(rng.random(ofs[0].bits.size * 8) < 0.1, bitorder="little")
+    denser = ofs[0].bits | extra.astype(ofs[0].bits.dtype)
+    fs.upload(0, denser)
+    assert fs.info(0).fpr > ofs[0].fpr
+    ofs2 = oracle_filters_like(fs, B)
+    assert ofs2[0].fpr == fs.info(0).fpr
+    print("[gpu] binomial after upload", flush=True)
+    check_batch(D, B, fs, ofs2, buf, off, scorer=O.BINOMIAL, thr=1e-10)
+    check_batch(D, B, fs, ofs2, buf, off, scorer=O.BINOMIAL, thr=1e-10, mode=O.BESTHIT)

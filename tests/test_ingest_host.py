@@ -269,3 +269,15 @@ def test_bgzf_cut_short_equals_sequential(L, tmp_path):
         want = dump(L, p, 0)[:2]
         for mode, threads in [(1, 4), (2, 16)]:
             assert dump(L, p, mode, threads)[:2] == want, (cut, mode)
+    # bytes after the last member that do not start a gzip member: gzread
+    # ignores them and the input ends normally, with every record
+    p.write_bytes(z)
+    full = dump(L, p, 0)[:2]
+    assert full[1] == -1 and len(full[0]) == 400
+    for pad in (b"\0" * 64, b"\x1f", b"garbage\n", b"\x1f\x00" + b"\0" * 30):
+        for body in (z, z[:-28]):  # with and without bgzip's empty last member
+            p.write_bytes(body + pad)
+            want = dump(L, p, 0)[:2]
+            assert want == full, pad
+            for mode, threads in [(1, 4), (2, 16)]:
+                assert dump(L, p, mode, threads)[:2] == want, (pad, mode)
