@@ -915,8 +915,8 @@ static bbt_status sync_desc(bbt_filterset *fs) {
     /* kernels of earlier batches on other streams may still read them */
     HIP_OK(hipDeviceSynchronize());
     const size_t n = fs->desc.size();
-    if (!fs->ddesc)
-        HIP_OK(hipMalloc(&fs->ddesc, sizeof(FilterDev) * std::max<size_t>(n, 1)));
+    if (!fs->ddesc) /* allocated with the set */
+        return fail(BBT_ERR_ARG, "filter set without device descriptors");
     for (size_t i = 0; i < n; ++i) {
         fs->desc[i].fpr = fs->info[i].fpr;
         fs->desc[i].binom = i < fs->dbinom.size() ? fs->dbinom[i] : nullptr;
@@ -972,7 +972,8 @@ static bbt_status alloc_set(uint32_t n, const uint64_t *m, const uint32_t *h,
         fs->info.push_back(inf);
         fs->ids.push_back("filter" + std::to_string(i));
     }
-    if (hipMalloc(&fs->dscratch, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&fs->dscratch, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&fs->ddesc, sizeof(FilterDev) * n) != hipSuccess) {
         bbt_filters_free(fs);
         return fail(BBT_ERR_NOMEM, "device allocation for scratch failed");
     }
@@ -1441,12 +1442,26 @@ bbt_status launch_ordered(const Launch &L, uint32_t scorer, uint64_t nrec, uint3
 /* calcMinCount tables for every frame length up to max_frames */
 /* (remade when min_fpr, the frame range or any filter's FPR changed: uploads
  * and refreshes change the FPR, (popcount / m)^h) */
-bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) {
+/* true when the tables are made for these parameters and every filter's
+ * current FPR (uploads and refreshes change it) */
+bool binom_fresh(const bbt_filterset *fs, double min_fpr, uint32_t max_frames) {
     bool fresh = !fs->dbinom.empty() && fs->binom_min_fpr == min_fpr &&
                  fs->binom_frames >= max_frames && fs->binom_fpr.size() == fs->desc.size();
     for (size_t f = 0; fresh && f < fs->desc.size(); ++f)
         fresh = fs->binom_fpr[f] == fs->info[f].fpr;
-    if (fresh)
+    return fresh;
+}
+
+bool bscore_fresh(const bbt_filterset *fs, uint32_t max_frames) {
+    bool fresh = !fs->dbscore.empty() && fs->bscore_frames >= max_frames &&
+                 fs->bscore_fpr.size() == fs->desc.size();
+    for (size_t f = 0; fresh && f < fs->desc.size(); ++f)
+        fresh = fs->bscore_fpr[f] == fs->info[f].fpr;
+    return fresh;
+}
+
+bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) {
+    if (binom_fresh(fs, min_fpr, max_frames))
         return BBT_OK;
     HIP_OK(hipDeviceSynchronize()); /* in-flight batches read the old tables */
     for (uint32_t *p : fs->dbinom)
@@ -1476,11 +1491,7 @@ bbt_status ensure_binom(bbt_filterset *fs, double min_fpr, uint32_t max_frames) 
 constexpr uint32_t kBinomScoreMaxFrames = 1024;
 
 bbt_status ensure_bscore(bbt_filterset *fs, uint32_t max_frames) {
-    bool fresh = !fs->dbscore.empty() && fs->bscore_frames >= max_frames &&
-                 fs->bscore_fpr.size() == fs->desc.size();
-    for (size_t f = 0; fresh && f < fs->desc.size(); ++f)
-        fresh = fs->bscore_fpr[f] == fs->info[f].fpr;
-    if (fresh)
+    if (bscore_fresh(fs, max_frames))
         return BBT_OK;
     HIP_OK(hipDeviceSynchronize()); /* in-flight batches read the old tables */
     for (double *p : fs->dbscore)
@@ -1519,10 +1530,23 @@ struct Prepared {
     uint32_t wave_min;
 };
 
+/* noalloc: the call may not allocate (bbt_eval_batch_noalloc): W already
+ * holds its control words and lists, and what would need an allocation (the
+ * -D pre-pass, graphs, timing events, BINOMIAL tables not yet built) is
+ * rejected instead. */
 static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
                           const bbt_eval_params *p, const bbt_eval_out *out, hipStream_t st,
-                          Prepared &P) {
+                          Prepared &P, bool noalloc) {
     const uint32_t nf = uint32_t(fs->desc.size());
+    if (noalloc) {
+        if (p->dust)
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "bbt_eval_batch_noalloc: -D (dust) needs the sdust pre-pass's "
+                        "scratch; use bbt_eval_batch");
+        if (p->flags & (BBT_FLAG_GRAPH | BBT_FLAG_TIME_WALK))
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "bbt_eval_batch_noalloc: graph and timing flags are not available");
+    }
     if (!in->seq || !in->off)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: reads not given");
     if (reinterpret_cast<uintptr_t>(in->seq) % 8 != 0 || in->seq_bytes % 8 != 0)
@@ -1546,6 +1570,10 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             return fail(BBT_ERR_UNSUPPORTED,
                         "binomial evalScore (best-hit / with-score modes) covers reads of up "
                         "to 1024 k-mers");
+        if (noalloc && !bscore_fresh(fs, frames))
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "bbt_eval_batch_noalloc: the binomial score tables for these reads "
+                        "are not built; run bbt_eval_batch once with the same parameters");
         bbt_status s = ensure_bscore(fs, frames);
         if (s != BBT_OK)
             return s;
@@ -1561,6 +1589,10 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         for (const FilterDev &d : fs->desc)
             kmin = std::min(kmin, d.k);
         const uint32_t frames = in->max_len >= kmin ? in->max_len - kmin + 1 : 0;
+        if (noalloc && !binom_fresh(fs, p->threshold, frames))
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "bbt_eval_batch_noalloc: the binomial thresholds for these reads are "
+                        "not built; run bbt_eval_batch once with the same parameters");
         bbt_status s = ensure_binom(fs, p->threshold, frames);
         if (s != BBT_OK)
             return s;
@@ -1729,10 +1761,11 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
 }
 
 static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
-                            const bbt_eval_params *p, bbt_eval_out *out, hipStream_t st) {
+                            const bbt_eval_params *p, bbt_eval_out *out, hipStream_t st,
+                            bool noalloc = false) {
     std::lock_guard<std::mutex> g(fs->mu);
     Prepared P;
-    bbt_status s = prepare(fs, W, in, p, out, st, P);
+    bbt_status s = prepare(fs, W, in, p, out, st, P, noalloc);
     if (s != BBT_OK)
         return s;
     const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
@@ -1797,6 +1830,67 @@ extern "C" bbt_status bbt_eval_batch(bbt_filterset *fs, const bbt_reads *in,
     if (in->nreads == 0)
         return BBT_OK;
     return eval_impl(fs, fs->ws, in, p, out, static_cast<hipStream_t>(stream));
+}
+
+/* ---- bbt_eval_batch_noalloc ---------------------------------------------
+ * The control words and length-bucket lists of one batch live in this
+ * array instead of a workspace: kNoAllocCtlWords 64-bit words, then the two
+ * read lists of kNoAllocMaxReads 32-bit entries each (8 MiB + 1 KiB).  Each
+ * GPU holds its own copy from the library's first use there until the
+ * process ends.  In host code the name `scratch` is not a device address:
+ * its address on a device comes from hipGetSymbolAddress, taken once per
+ * device with that device current, and only that address reaches kernels. */
+constexpr uint64_t kNoAllocMaxReads = uint64_t(1) << 20;
+constexpr uint64_t kNoAllocCtlWords = 128;
+static __device__ __align__(256) unsigned long long
+    scratch[kNoAllocCtlWords + kNoAllocMaxReads]; /* lists: 2 x kNoAllocMaxReads x 4 B */
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_ptr[kMaxDevices] = {};
+
+/* scratch's address on the current device dev, taken on its first use there */
+static bbt_status noalloc_scratch(int dev, unsigned long long **ptr) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch_noalloc: device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return fail(BBT_ERR_HIP, "bbt_eval_batch_noalloc: no device address for scratch");
+        g_scratch_ptr[dev] = static_cast<unsigned long long *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_eval_batch_noalloc(bbt_filterset *fs, const bbt_reads *in,
+                                             const bbt_eval_params *p, bbt_eval_out *out,
+                                             void *stream) {
+    if (!fs || !in || !p || !out)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch_noalloc: null argument");
+    if (in->nreads > kNoAllocMaxReads)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch_noalloc: at most " +
+                                     std::to_string(kNoAllocMaxReads) +
+                                     " reads per call (the size of its scratch); split the "
+                                     "batch or use bbt_eval_batch");
+    if (in->nreads == 0)
+        return BBT_OK;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != fs->device)
+        return fail(BBT_ERR_ARG,
+                    "bbt_eval_batch_noalloc: the filter set belongs to another device");
+    unsigned long long *base = nullptr;
+    bbt_status s = noalloc_scratch(dev, &base);
+    if (s != BBT_OK)
+        return s;
+    bbt_workspace W; /* views into scratch; nothing of it is freed */
+    W.dctl = base;
+    W.dlists = reinterpret_cast<uint32_t *>(base + kNoAllocCtlWords);
+    W.lists_cap = kNoAllocMaxReads;
+    return eval_impl(fs, W, in, p, out, static_cast<hipStream_t>(stream), true);
 }
 
 extern "C" bbt_status bbt_workspace_create(bbt_workspace **out) {

@@ -128,6 +128,9 @@ _SIGS = {
     "bbt_last_walk_ms": (C.c_double, [C.c_void_p]),
     "bbt_measure_gather": (C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bbt_eval_batch_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(Reads),
+                                           C.POINTER(EvalParams), C.POINTER(EvalOut),
+                                           C.c_void_p]),
     "bbt_workspace_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
     "bbt_workspace_free": (C.c_int32, [C.c_void_p]),
     "bbt_eval_batch_ws": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Reads),

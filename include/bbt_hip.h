@@ -226,6 +226,21 @@ bbt_status bbt_workspace_create(bbt_workspace **out);
 bbt_status bbt_workspace_free(bbt_workspace *ws);
 bbt_status bbt_eval_batch_ws(bbt_filterset *fs, bbt_workspace *ws, const bbt_reads *in,
                              const bbt_eval_params *p, bbt_eval_out *out, void *stream);
+/* bbt_eval_batch without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results as bbt_eval_batch; the batch's control words and
+ * read lists live in a library-owned device array of 8 MiB + 1 KiB per GPU,
+ * held until the process ends, instead of the filter set's workspace.
+ * Limits: at most 1048576 reads per call (BBT_ERR_ARG beyond); -D (dust),
+ * BBT_FLAG_GRAPH and BBT_FLAG_TIME_WALK are rejected (BBT_ERR_UNSUPPORTED);
+ * BINOMIAL scoring needs its tables built by an earlier bbt_eval_batch with
+ * the same threshold and no longer reads (BBT_ERR_UNSUPPORTED otherwise).
+ * Every call on one GPU shares that GPU's array: calls whose kernels
+ * overlap on one GPU (other streams, other threads, other filter sets) race
+ * and give wrong results; order them (one stream, or events). */
+bbt_status bbt_eval_batch_noalloc(bbt_filterset *fs, const bbt_reads *in,
+                                  const bbt_eval_params *p, bbt_eval_out *out, void *stream);
 /* Scheduling knobs of a filter set (no counterpart in the reference). */
 bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t);
 bbt_status bbt_get_tuning(const bbt_filterset *fs, bbt_tuning *t);
