@@ -1687,7 +1687,9 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         HIP_OK(hipGetLastError());
         a.dust = W.dmask;
     }
-    if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED) {
+    /* the length buckets are needed only when some read may reach the
+     * whole-wave length; otherwise every read is a lane walk in input order */
+    if (p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED && in->max_len >= P.wave_min) {
         const uint32_t gb =
             uint32_t(std::min<uint64_t>(1024, (in->nreads + kBlock - 1) / kBlock));
         k_len_count<<<gb, kBlock, 0, st>>>(in->off, in->nreads, P.wave_min, W.dctl + 8);

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -64,6 +65,9 @@ struct MibfDev {
     uint32_t care[kMaxSeeds]; /* spaced seeds: bit j set = position j is used */
     uint32_t dc_union;        /* positions that are don't-care in some seed */
     uint32_t pad;
+    /* direct-mapped table, or null: one 32-bit word per bit position, 0 for a
+     * clear bit, else the slot's ID word (ID 0 stored as the all-ones ID) */
+    const uint32_t *direct;
 };
 
 __device__ __forceinline__ uint32_t mix32(uint64_t x) {
@@ -294,6 +298,37 @@ __global__ __launch_bounds__(256) void k_mibf_sat(const uint8_t *seq, uint64_t n
                        for (uint32_t j = 0; j < nseen; ++j)
                            atomicOr(data + ranks[j], kSatBit);
                });
+}
+
+/* The direct-mapped table from the lines and the rank-ordered ID array:
+ * thread w fills the 64 words of bit-vector word w (rank of its first set
+ * bit from the line's sample and the words before it in the line). */
+__global__ __launch_bounds__(256) void k_mibf_direct(const uint64_t *lines, uint64_t nwords,
+                                                     const uint32_t *data, uint32_t *direct) {
+    for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < nwords;
+         w += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t line = w / kWordsPerLine;
+        const uint32_t slot = uint32_t(w - line * kWordsPerLine);
+        const uint64_t *L = lines + line * 8;
+        uint64_t r = L[0];
+        for (uint32_t t = 0; t < slot; ++t)
+            r += uint64_t(__popcll(L[1 + t]));
+        const uint64_t word = L[1 + slot];
+        uint4 *out = reinterpret_cast<uint4 *>(direct + w * 64);
+        for (uint32_t q = 0; q < 16; ++q) {
+            uint32_t v[4];
+            for (uint32_t e = 0; e < 4; ++e) {
+                v[e] = 0;
+                if ((word >> (4 * q + e)) & 1) {
+                    uint32_t x = data[r++];
+                    if ((x & kIdMask) == 0)
+                        x |= kIdMask;
+                    v[e] = x;
+                }
+            }
+            out[q] = make_uint4(v[0], v[1], v[2], v[3]);
+        }
+    }
 }
 
 /* slots per ID (hist[id]) and saturated slots (hist[nids + 1]); with few
@@ -532,22 +567,39 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                         hs[sd] = rv[sd] < fw[sd] ? rv[sd] : fw[sd];
                 }
                 cprev = code;
-                uint64_t rk[SMAX];
                 uint32_t present = 0;
-#pragma unroll
-                for (uint32_t sd = 0; sd < SMAX; ++sd) {
-                    rk[sd] = 0;
-                    if (sd < S && valid) {
-                        const bool bit = bv_rank_line(
-                            a.d.lines, fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
-                            rk[sd]);
-                        present |= uint32_t(bit) << sd;
-                    }
-                }
                 uint32_t raw[SMAX];
+                if (a.d.direct) {
+                    /* one load per seed: the ID word, or 0 for a clear bit */
 #pragma unroll
-                for (uint32_t sd = 0; sd < SMAX; ++sd)
-                    raw[sd] = ((present >> sd) & 1) ? a.d.data[rk[sd]] : 0u;
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        raw[sd] = (sd < S && valid)
+                                      ? a.d.direct[fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1,
+                                                           a.d.sh2)]
+                                      : 0u;
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                        present |= uint32_t(raw[sd] != 0) << sd;
+                        if ((raw[sd] & kIdMask) == kIdMask)
+                            raw[sd] &= kSatBit; /* ID 0 */
+                    }
+                } else {
+                    /* bit and rank from the line, then the ID slot */
+                    uint64_t rk[SMAX];
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                        rk[sd] = 0;
+                        if (sd < S && valid) {
+                            const bool bit = bv_rank_line(
+                                a.d.lines, fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
+                                rk[sd]);
+                            present |= uint32_t(bit) << sd;
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        raw[sd] = ((present >> sd) & 1) ? a.d.data[rk[sd]] : 0u;
+                }
                 const uint32_t nh = uint32_t(__popc(present));
                 bvl += valid ? S : 0;
                 gat += nh;
@@ -871,6 +923,7 @@ struct bbt_mibf {
     MibfDev d;
     uint64_t *dlines = nullptr;
     uint32_t *ddata = nullptr;
+    uint32_t *ddirect = nullptr; /* direct-mapped table (d.direct), or null */
     uint64_t nlines = 0, pop = 0, pop_sat = 0;
     std::vector<std::string> seeds;
     std::vector<std::string> names;  /* names[0] = "repeat", then IDs 1..n */
@@ -969,6 +1022,33 @@ bbt_status mibf_stats(bbt_mibf *mf, hipStream_t st) {
     return BBT_OK;
 }
 
+/* The direct-mapped table (m 32-bit words) is built when it fits in half
+ * of the free device memory, unless BBT_MIBF_DIRECT=0: on a GPU with
+ * hundreds of GB of HBM it turns each query lookup (a line load for the bit
+ * and its rank, then a dependent ID load) into one random load. */
+bbt_status mibf_make_direct(bbt_mibf *mf, hipStream_t st) {
+    const char *env = std::getenv("BBT_MIBF_DIRECT");
+    if (env && std::strcmp(env, "0") == 0)
+        return BBT_OK;
+    size_t free_b = 0, total_b = 0;
+    MHIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t bytes = mf->d.m * 4;
+    if (bytes > free_b / 2)
+        return BBT_OK;
+    if (hipMalloc(&mf->ddirect, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        mf->ddirect = nullptr;
+        return BBT_OK; /* the lines and ID array serve queries as before */
+    }
+    const uint64_t nwords = mf->d.m / 64;
+    k_mibf_direct<<<uint32_t(std::min<uint64_t>((nwords + 255) / 256, 1u << 16)), 256, 0, st>>>(
+        mf->dlines, nwords, mf->ddata, mf->ddirect);
+    MHIP(hipGetLastError());
+    MHIP(hipStreamSynchronize(st));
+    mf->d.direct = mf->ddirect;
+    return BBT_OK;
+}
+
 constexpr char kMagic[8] = {'B', 'B', 'T', 'M', 'I', 'B', 'F', '1'};
 
 struct MibfHeader {
@@ -992,6 +1072,7 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
         return BBT_OK;
     (void)hipFree(mf->dlines);
     (void)hipFree(mf->ddata);
+    (void)hipFree(mf->ddirect);
     (void)hipFree(mf->dmin);
     (void)hipFree(mf->dctl);
     (void)hipFree(mf->dwide_list);
@@ -1092,7 +1173,12 @@ bbt_status bbt_mibf_build(const char *dseq, const uint64_t *drec_off, const uint
                                        chunk);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         return done(mfail(BBT_ERR_HIP, "bbt_mibf_build: ID passes failed"));
-    return done(mibf_stats(mf, st));
+    (void)hipFree(best);
+    best = nullptr;
+    bbt_status s2 = mibf_stats(mf, st);
+    if (s2 == BBT_OK)
+        s2 = mibf_make_direct(mf, st);
+    return done(s2);
 }
 
 bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info) {
@@ -1105,6 +1191,8 @@ bbt_status bbt_mibf_info(const bbt_mibf *mf, bbt_mibf_info_t *info) {
     info->nseeds = mf->d.S;
     info->span = mf->d.span;
     info->kmer_mode = mf->d.kmer;
+    info->direct = mf->d.direct ? 1 : 0;
+    info->reserved = 0;
     return BBT_OK;
 }
 
@@ -1257,6 +1345,8 @@ bbt_status bbt_mibf_load(const char *path, bbt_mibf **out, void *stream) {
         return mfail(BBT_ERR_HIP, "bbt_mibf_load: upload failed");
     }
     s = mibf_stats(mf, st);
+    if (s == BBT_OK)
+        s = mibf_make_direct(mf, st);
     if (s != BBT_OK) {
         bbt_mibf_free(mf);
         return s;
@@ -1387,6 +1477,8 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     QueryArgs a;
     std::memset(&a, 0, sizeof(a));
     a.d = mf->d;
+    if (p->flags & BBT_MIBF_LINES)
+        a.d.direct = nullptr;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
     a.off = in->off;
     a.nrec = nrec;

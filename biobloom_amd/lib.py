@@ -55,7 +55,7 @@ class EvalOut(C.Structure):
 class MibfInfo(C.Structure):
     _fields_ = [("m", C.c_uint64), ("pop", C.c_uint64), ("pop_saturated", C.c_uint64),
                 ("nids", C.c_uint32), ("nseeds", C.c_uint32), ("span", C.c_uint32),
-                ("kmer_mode", C.c_uint32)]
+                ("kmer_mode", C.c_uint32), ("direct", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class MibfParams(C.Structure):
@@ -431,12 +431,16 @@ class Mibf:
 
 MIBF_PAIRED = 1
 MIBF_BEST_AGREE = 2
+MIBF_LINES = 4
 MIBF_HIT_WORDS = 7
 
 
 def mibf_params(score=0.1, multi_thresh=1.0, allowed_miss=0, min_nonsat=1, max_hits=8,
-                paired=False, best_agree=False):
-    flags = (MIBF_PAIRED if paired else 0) | (MIBF_BEST_AGREE if best_agree else 0)
+                paired=False, best_agree=False, lines=False):
+    """lines: look up through the lines and ID array even when the filter has
+    its direct-mapped table."""
+    flags = ((MIBF_PAIRED if paired else 0) | (MIBF_BEST_AGREE if best_agree else 0)
+             | (MIBF_LINES if lines else 0))
     return MibfParams(score, multi_thresh, allowed_miss, min_nonsat, max_hits, flags)
 
 

@@ -332,7 +332,13 @@ bbt_status bbt_server_destroy(bbt_server *s);
 /* One bit vector of m bits shared by all references; each set bit owns one
  * 32-bit slot (rank order) holding a reference ID (bits 0..30, IDs 1..n) and
  * a saturation flag (bit 31).  Hashes are spaced-seed ntHash values, one per
- * seed per window; a seed is a string of '0'/'1' ('1' = position used). */
+ * seed per window; a seed is a string of '0'/'1' ('1' = position used).
+ * Device layout: the bit vector in 64-byte lines (a rank sample and seven
+ * words) and the rank-ordered ID array; and, when m x 4 bytes fit in half of
+ * the free device memory (BBT_MIBF_DIRECT=0 in the environment turns it off),
+ * a direct-mapped table of one 32-bit word per bit position (0 = clear bit,
+ * else the slot's ID word), which makes each query lookup one random load
+ * instead of a line load and a dependent ID load.  Results are the same. */
 typedef struct bbt_mibf bbt_mibf;
 
 typedef struct {
@@ -343,10 +349,15 @@ typedef struct {
     uint32_t nseeds;
     uint32_t span;          /* seed length / k-mer size (window size) */
     uint32_t kmer_mode;     /* 1: nseeds ntHash values per k-mer, no seeds */
+    uint32_t direct;        /* 1: queries read the direct-mapped table (above) */
+    uint32_t reserved;
 } bbt_mibf_info_t;
 
 #define BBT_MIBF_PAIRED 1     /* reads 2j, 2j+1 are mates: one record per pair */
 #define BBT_MIBF_BEST_AGREE 2 /* -b: drop results whose best hit is ambiguous */
+#define BBT_MIBF_LINES 4      /* look up through the lines and the ID array even
+                                 when the direct-mapped table exists (same
+                                 results; for comparison) */
 
 typedef struct {
     double score;           /* opt::score: significance spread over the IDs */

@@ -38,7 +38,9 @@
  *              to satCount.  A pair pools the frames of both mates; its
  *              frameCount follows the nesting of MIBFClassifier.hpp:804-805
  *              (mate 1's frames, or mate 2's when mate 1 is shorter than k).
- *  candidates  count >= minCount[id] (binomial quantile at score/(#IDs) on
+ *  candidates  IDs 1..#IDs only (a slot holding ID 0, which no build writes,
+ *              counts toward its frame but is never a candidate);
+ *              count >= minCount[id] (binomial quantile at score/(#IDs) on
  *              the ID's per-frame probability, at least 1, frameCount =
  *              len - span + 1) and nonSat >= minNoSat; ranked by nonSat,
  *              count, solid, totalNonSat, total (descending) then ID; the best
@@ -294,7 +296,7 @@ int orc_mibf_query(const uint64_t *bv, const uint32_t *data, uint64_t m, const c
         std::vector<orc_mibf_hit> cand;
         for (auto &kv : acc) {
             const orc_mibf_hit &h = kv.second;
-            if (h.id <= nids && h.count >= mc[h.id] && h.nonsat >= min_nonsat)
+            if (h.id >= 1 && h.id <= nids && h.count >= mc[h.id] && h.nonsat >= min_nonsat)
                 cand.push_back(h);
         }
         if (cand.empty())

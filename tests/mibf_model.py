@@ -110,6 +110,8 @@ def query(order, data, m, seeds, probs, nids, score, a, multi, min_nonsat, seq, 
                 h[5] += 1 if clean else 0
     cand = []
     for iid, (cnt, nsat, tot, tns, solid, nsf) in acc.items():
+        if not 1 <= iid <= nids:  # ID 0 (no build writes it) is never a candidate
+            continue
         mc = max(1, int(O.lib().orc_binom_isf_up(score / nids, fc, probs[iid])))
         if cnt >= mc and nsat >= min_nonsat:
             cand.append((iid, cnt, nsat, tot, tns, solid, nsf))

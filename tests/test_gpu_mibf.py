@@ -41,6 +41,20 @@ def reads_from(rng, recs, n, lens=(40, 100, 150), err=0.02, junk=0.3):
     return reads
 
 
+def classify_both(D, B, mf, dr, params):
+    """D.mibf_classify through the direct-mapped table and again through the
+    lines and the ID array: both layouts must give the same per-record
+    results and lookup counters."""
+    got = D.mibf_classify(mf, dr, params)
+    if mf.info().direct:
+        lp = B.MibfParams(params.score, params.multi_thresh, params.allowed_miss,
+                          params.min_nonsat, params.max_hits, params.flags | B.MIBF_LINES)
+        other = D.mibf_classify(mf, dr, lp)
+        assert other[0] == got[0] and other[1] == got[1], "layouts disagree"
+        assert np.array_equal(other[2], got[2]) and np.array_equal(other[3], got[3])
+    return got
+
+
 def device_reads(D, reads):
     buf, off = O.pack_reads(reads)
     return D.DeviceReads(buf, off)
@@ -110,7 +124,7 @@ def test_query_matches_oracle(dev, seed):
         print(f"[gpu] mibf query seed={seed} score={score} miss={a} multi={multi}", flush=True)
         want, wsat, wctr = om.query(reads, score=score, allowed_miss=a, multi=multi,
                                     min_nonsat=mns, max_hits=8)
-        got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(score, multi, a, mns, 8))
+        got, gsat, gctr, _ = classify_both(D, B, mf, dr, B.mibf_params(score, multi, a, mns, 8))
         assert gsat == wsat
         for r in range(len(reads)):
             assert got[r] == want[r], (r, reads[r])
@@ -134,7 +148,7 @@ def test_query_long_reads(dev, L):
     dr = device_reads(D, reads)
     print(f"[gpu] mibf long reads L={L}", flush=True)
     want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=1, multi=0.5, max_hits=8)
-    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 0.5, 1, 0, 8))
+    got, gsat, gctr, _ = classify_both(D, B, mf, dr, B.mibf_params(1e-10, 0.5, 1, 0, 8))
     assert gsat == wsat
     assert got == want
     assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
@@ -153,7 +167,7 @@ def test_max_hits_overflow_counted(dev):
     reads = [base[a:a + 150] for a in range(0, 600, 50)]
     dr = device_reads(D, reads)
     want, wsat, wctr = om.query(reads, score=0.5, allowed_miss=2, multi=0.0, max_hits=2)
-    got, gsat, gctr, nh = D.mibf_classify(mf, dr, B.mibf_params(0.5, 0.0, 2, 0, 2))
+    got, gsat, gctr, nh = classify_both(D, B, mf, dr, B.mibf_params(0.5, 0.0, 2, 0, 2))
     assert got == want and gsat == wsat
     assert wctr[2] > 0 and gctr[2] == wctr[2]
 
@@ -182,8 +196,8 @@ def test_more_than_64_candidates(dev, nids, L):
               f"best_agree={agree}", flush=True)
         want, wsat, wctr = om.query(reads, score=score, allowed_miss=2, multi=multi,
                                     max_hits=mh, best_agree=agree)
-        got, gsat, gctr, nh = D.mibf_classify(
-            mf, dr, B.mibf_params(score, multi, 2, 0, mh, best_agree=agree))
+        got, gsat, gctr, nh = classify_both(
+            D, B, mf, dr, B.mibf_params(score, multi, 2, 0, mh, best_agree=agree))
         assert gsat == wsat
         for r in range(len(reads)):
             assert got[r] == want[r], (r, len(reads[r]))
@@ -261,13 +275,13 @@ def test_kmer_mode_matches_oracle(dev, tmp_path):
     dr = device_reads(D, reads)
     print("[gpu] mibf k-mer mode query", flush=True)
     want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=0, multi=1.0, max_hits=8)
-    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))
+    got, gsat, gctr, _ = classify_both(D, B, mf, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))
     assert got == want and gsat == wsat
     assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
     mf.store(str(tmp_path / "km"))
     mf2 = B.Mibf.load(str(tmp_path / "km.bf"))
     assert mf2.info().kmer_mode == 1
-    assert D.mibf_classify(mf2, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))[:2] == (got, gsat)
+    assert classify_both(D, B, mf2, dr, B.mibf_params(1e-10, 1.0, 0, 0, 8))[:2] == (got, gsat)
 
 
 @pytest.mark.parametrize("best_agree", [False, True])
@@ -294,8 +308,8 @@ def test_pairs_match_oracle(dev, best_agree):
         print(f"[gpu] mibf pairs best_agree={best_agree} score={score} miss={a}", flush=True)
         want, wsat, wctr = om.query(reads, score=score, allowed_miss=a, multi=multi,
                                     max_hits=8, paired=True, best_agree=best_agree)
-        got, gsat, gctr, _ = D.mibf_classify(
-            mf, dr, B.mibf_params(score, multi, a, 0, 8, paired=True, best_agree=best_agree))
+        got, gsat, gctr, _ = classify_both(
+            D, B, mf, dr, B.mibf_params(score, multi, a, 0, 8, paired=True, best_agree=best_agree))
         assert len(got) == len(reads) // 2
         assert gsat == wsat
         assert got == want
@@ -340,7 +354,7 @@ def test_many_ids_sort_path(dev):
     dr = device_reads(D, reads)
     print("[gpu] mibf 1100 ids", flush=True)
     want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5, max_hits=8)
-    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8))
+    got, gsat, gctr, _ = classify_both(D, B, mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8))
     assert gsat == wsat
     assert got == want
     assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
@@ -371,8 +385,8 @@ def test_query_reads_beyond_the_record_store(dev):
     for paired in [False, True]:
         want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5,
                                     max_hits=8, paired=paired)
-        got, gsat, gctr, _ = D.mibf_classify(
-            mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8, paired=paired))
+        got, gsat, gctr, _ = classify_both(
+            D, B, mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8, paired=paired))
         assert gsat == wsat
         assert got == want
         assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[3] == 0
@@ -414,3 +428,47 @@ def test_queries_on_two_streams(dev):
             got, gsat, gctr, _ = out.host()
             assert got == want and gsat == wsat
             assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+
+
+def test_direct_table_and_id_zero_slots(dev, tmp_path, monkeypatch):
+    """The direct-mapped table is built by default and not with
+    BBT_MIBF_DIRECT=0; a loaded filter whose slots hold ID 0 (saturated or
+    not) gives the same results through the table as through the lines and
+    the ID array, and as the oracle on the same arrays."""
+    import struct
+    D, B = dev
+    rng = random.Random(91)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=5, L=3000)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, occupancy=0.5)
+    assert mf.info().direct == 1
+    monkeypatch.setenv("BBT_MIBF_DIRECT", "0")
+    mf0 = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                       [str(i) for i, _ in recs], seeds, occupancy=0.5)
+    assert mf0.info().direct == 0
+    monkeypatch.delenv("BBT_MIBF_DIRECT")
+    prefix = str(tmp_path / "z")
+    mf.store(prefix)
+    raw = bytearray(open(prefix + ".bf", "rb").read())
+    nseeds, span, nids, kmer, m, pop = struct.unpack_from("<IIIIQQ", raw, 8)
+    data_at = 40 + nseeds * span + (m // 64) * 8
+    data = np.frombuffer(raw, np.uint32, count=pop, offset=data_at).copy()
+    pick = np.random.default_rng(5).choice(pop, size=pop // 5, replace=False)
+    data[pick[: len(pick) // 2]] = 0                 # ID 0
+    data[pick[len(pick) // 2:]] = 0x80000000        # ID 0, saturated
+    raw[data_at:data_at + 4 * pop] = data.tobytes()
+    open(prefix + ".bf", "wb").write(bytes(raw))
+    mz = B.Mibf.load(prefix + ".bf")
+    assert mz.info().direct == 1
+    info, bv, dz = download(mz)
+    assert np.array_equal(dz[:pop], data)
+    om = O.Mibf.from_arrays(bv, dz, info.m, seeds, info.nids)
+    reads = reads_from(rng, recs, 400)
+    dr = device_reads(D, reads)
+    for a in (0, 2):
+        want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=a, multi=1.0,
+                                    min_nonsat=0, max_hits=8)
+        got, gsat, gctr, _ = classify_both(D, B, mz, dr, B.mibf_params(1e-10, 1.0, a, 0, 8))
+        assert got == want and gsat == wsat
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1]

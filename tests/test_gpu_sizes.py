@@ -147,7 +147,7 @@ def test_bloom_filter_above_2_34_bits(dev):
 
 def test_mibf_above_2_32_bits(dev):
     D, B = dev
-    from test_gpu_mibf import download, reads_from
+    from test_gpu_mibf import classify_both, download, reads_from
     from test_oracle_mibf import make_refs, seeds_for
     rng = random.Random(2024)
     seeds = seeds_for(rng, span=25, weight=20)
@@ -169,7 +169,8 @@ def test_mibf_above_2_32_bits(dev):
     print("[gpu] mibf 5 Gbit query", flush=True)
     want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=1.0, min_nonsat=0,
                                 max_hits=8)
-    got, gsat, gctr, _ = D.mibf_classify(mf, dr, B.mibf_params(1e-10, 1.0, 2, 0, 8))
+    got, gsat, gctr, _ = classify_both(D, B, mf, dr, B.mibf_params(1e-10, 1.0, 2, 0, 8))
+    assert mf.info().direct == 1
     assert gsat == wsat
     assert got == want
     assert gctr[0] == wctr[0] and gctr[1] == wctr[1]

@@ -184,3 +184,34 @@ def test_kmer_mode_matches_model():
         want, wsat = M.query(order, data, mb.m, km, probs, mb.nids, 1e-10, 0, 1.0, 0, r)
         assert st == wsat and row == [tuple(c) for c in want]
     assert sum(len(r) for r in res) >= 25
+
+
+def test_id_zero_slots_are_never_candidates():
+    """Slots holding ID 0 (no build writes it; a loaded file may hold it)
+    count toward their frame's distinct IDs and saturation but never become
+    a candidate: oracle and model agree on planted ID-0 slots."""
+    rng = random.Random(77)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=5, L=1500)
+    mb = O.Mibf(recs, seeds)
+    order, data = M.build(recs, seeds, mb.m)
+    pick = random.Random(3).sample(range(mb.pop), mb.pop // 4)
+    for j, i in enumerate(pick):
+        v = 0 if j % 2 else 0x80000000
+        mb.data[i] = v
+        data[i] = v
+    reads = []
+    for _ in range(80):
+        _, ref = rng.choice(recs)
+        a = rng.randrange(0, len(ref) - 150)
+        reads.append(ref[a:a + 150])
+    for a in (0, 2):
+        res, sat, _ = mb.query(reads, score=1e-10, allowed_miss=a, multi=1.0)
+        probs = M.frame_probs(data, mb.pop, mb.m, 3, a, mb.nids)
+        assert np.allclose(mb.frame_probs(a), probs, rtol=1e-12, atol=0)
+        for r, row, st in zip(reads, res, sat):
+            want, wsat = M.query(order, data, mb.m, seeds, probs, mb.nids, 1e-10, a, 1.0, 0, r)
+            assert st == wsat
+            assert row == [tuple(c) for c in want]
+            assert all(c[0] >= 1 for c in row)
+    assert sum(len(r) for r in res) > 0
