@@ -454,17 +454,26 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
     __shared__ uint64_t rec[DIRECT ? 1 : CAP];
     __shared__ uint32_t cand[kMaxCand * kHitWords];
     __shared__ uint32_t ncand_s;
+    /* spaced seeds: for window offset j and base code c, the contributions
+     * the don't-care base c at j made to the full window hashes, which the
+     * seed hash XORs back out: {rotl(seed_fwd(c), k-1-j), rotl(seed_rc(c), j)} */
+    __shared__ ulonglong2 dct[32 * 4];
+    __shared__ uint8_t order_s[kMaxCand]; /* lane 0's candidate ranking */
     /* DIRECT: per ID, count, nonSat, total, totalNonSat, solid, nonSatFrame */
     extern __shared__ uint32_t acc[];
     const uint32_t lane = threadIdx.x;
     const uint32_t S = a.d.S, k = a.d.span;
     const uint32_t nid1 = a.d.nids + 1;
     unsigned long long bvl = 0, gat = 0;
-    if (DIRECT) {
+    if (DIRECT)
         for (uint32_t x = lane; x < kDirectWords * nid1; x += 64)
             acc[x] = 0;
-        __syncthreads();
+    for (uint32_t x = lane; x < 32 * 4; x += 64) {
+        const uint32_t j = x >> 2, c = x & 3;
+        dct[x] = j < k ? make_ulonglong2(rotl64(seed_fwd(c), k - 1 - j), rotl64(seed_rc(c), j))
+                       : make_ulonglong2(0, 0);
     }
+    __syncthreads();
     while (true) {
         unsigned long long it = 0;
         if (lane == 0)
@@ -492,7 +501,9 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 continue;
             uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
             int lastN = -1;
-            unsigned cprev = 0; /* base codes of the previous 64 bases, one per lane */
+            /* 2-bit codes of the 32 bases before this chunk, the last one in
+             * bits 0-1 (lane 63's packed window of the previous chunk) */
+            uint64_t wprev = 0;
             for (uint32_t b0 = 0; b0 < len; b0 += 64) {
                 const uint32_t i = b0 + lane;
                 const unsigned c = i < len ? unsigned(s[i]) : 0u;
@@ -530,9 +541,8 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 Hp = H;
                 Gp = G;
                 /* this lane's frame: the S hashes (spaced seeds: the window
-                 * values with the don't-care bases XORed out, each base taken
-                 * from the lane that holds it), all S line loads, then the ID
-                 * gathers */
+                 * values with the don't-care bases XORed out), then one load
+                 * per seed */
                 uint64_t hs[SMAX];
                 if (a.d.kmer) {
                     const uint64_t h0 = wr < wf ? wr : wf;
@@ -540,6 +550,19 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                     for (uint32_t sd = 0; sd < SMAX; ++sd)
                         hs[sd] = multi_hash(h0, sd, uint64_t(k) * kMultiSeed);
                 } else {
+                    /* the window's bases as 2-bit codes, base i in bits 0-1 and
+                     * base i - t in bits 2t..2t+1 (t < 32): doubling steps over
+                     * the lanes, then the previous chunk's bases above those of
+                     * this chunk */
+                    uint64_t w = code;
+#pragma unroll
+                    for (uint32_t dd = 1; dd < 32; dd <<= 1) {
+                        const uint64_t t = __shfl_up(w, dd);
+                        w |= (lane >= dd ? t : 0) << (2 * dd);
+                    }
+                    if (lane < 31)
+                        w |= wprev << (2 * (lane + 1));
+                    wprev = __shfl(w, 63);
                     uint64_t fw[SMAX], rv[SMAX];
 #pragma unroll
                     for (uint32_t sd = 0; sd < SMAX; ++sd) {
@@ -548,25 +571,19 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                     }
                     for (uint32_t dcm = a.d.dc_union; dcm; dcm &= dcm - 1) {
                         const uint32_t j = uint32_t(__builtin_ctz(dcm));
-                        const uint32_t back = k - 1 - j; /* base pw + j = i - back */
-                        const unsigned cur = unsigned(__shfl_up(int(code), back));
-                        const unsigned prv =
-                            unsigned(__shfl(int(cprev), (int(lane) - int(back)) & 63));
-                        const unsigned cj = lane >= back ? cur : prv;
-                        const uint64_t tf = rotl64(seed_fwd(cj), back);
-                        const uint64_t tr = rotl64(seed_rc(cj), j);
+                        const uint32_t cj = uint32_t(w >> (2 * (k - 1 - j))) & 3u;
+                        const ulonglong2 t = dct[j * 4 + cj];
 #pragma unroll
                         for (uint32_t sd = 0; sd < SMAX; ++sd)
                             if (sd < S && !((a.d.care[sd] >> j) & 1)) {
-                                fw[sd] ^= tf;
-                                rv[sd] ^= tr;
+                                fw[sd] ^= t.x;
+                                rv[sd] ^= t.y;
                             }
                     }
 #pragma unroll
                     for (uint32_t sd = 0; sd < SMAX; ++sd)
                         hs[sd] = rv[sd] < fw[sd] ? rv[sd] : fw[sd];
                 }
-                cprev = code;
                 uint32_t present = 0;
                 uint32_t raw[SMAX];
                 if (a.d.direct) {
@@ -641,15 +658,54 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                     }
                     return n;
                 };
-                uint32_t myrec = 0;
-                if (eval) {
+                if (eval)
                     satc += allsat ? 1 : 0;
-                    myrec = each_id([](uint32_t, uint64_t) {});
-                }
                 if (DIRECT) {
+                    /* The lanes whose frame's first distinct ID is the first
+                     * evaluated lane's (most frames of a read: its
+                     * reference) add their six statistics by ballots and
+                     * one lane's update; every other distinct ID of a frame
+                     * goes through LDS atomics.  The block is one wave, so
+                     * its LDS operations happen in program order. */
+                    uint32_t lid = 0xFFFFFFFFu, locc = 0, lns = 0;
                     if (eval)
-                        each_id([&](uint32_t, uint64_t v) {
+                        each_id([&](uint32_t x, uint64_t v) {
+                            if (x == 0) {
+                                lid = uint32_t(v >> 32);
+                                locc = (uint32_t(v) >> 8) & 0xFF;
+                                lns = uint32_t(v) & 0xFF;
+                            }
+                        });
+                    const uint64_t em = __ballot(lid != 0xFFFFFFFFu);
+                    uint32_t leader = 0xFFFFFFFFu;
+                    if (em) {
+                        leader = uint32_t(__shfl(int(lid), int(__ffsll((unsigned long long)em) - 1)));
+                        const bool mine = lid == leader;
+                        const uint64_t bm = __ballot(mine);
+                        uint32_t tot = 0, tns = 0;
+                        for (uint32_t b = 0; (1u << b) <= S; ++b) {
+                            tot += uint32_t(__popcll(__ballot(mine && ((locc >> b) & 1)))) << b;
+                            tns += uint32_t(__popcll(__ballot(mine && ((lns >> b) & 1)))) << b;
+                        }
+                        const uint32_t cnt = uint32_t(__popcll(bm));
+                        const uint32_t nsat = uint32_t(__popcll(__ballot(mine && lns != 0)));
+                        const uint32_t solid = uint32_t(__popcll(__ballot(mine && locc == S)));
+                        const uint32_t nsf = uint32_t(__popcll(__ballot(mine && !anysat)));
+                        if (lane == uint32_t(__ffsll((unsigned long long)em) - 1) && leader < nid1) {
+                            uint32_t *A = acc + kDirectWords * leader;
+                            A[0] += cnt;
+                            A[1] += nsat;
+                            A[2] += tot;
+                            A[3] += tns;
+                            A[4] += solid;
+                            A[5] += nsf;
+                        }
+                    }
+                    if (eval)
+                        each_id([&](uint32_t x, uint64_t v) {
                             const uint32_t id = uint32_t(v >> 32);
+                            if (x == 0 && id == leader)
+                                return;
                             const uint32_t occ = (uint32_t(v) >> 8) & 0xFF, ns = uint32_t(v) & 0xFF;
                             if (id < nid1) {
                                 uint32_t *A = acc + kDirectWords * id;
@@ -667,6 +723,9 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                         });
                     continue;
                 }
+                uint32_t myrec = 0;
+                if (eval)
+                    myrec = each_id([](uint32_t, uint64_t) {});
                 /* append this step's records: exclusive scan of myrec */
                 uint32_t pre = myrec;
                 for (int dd = 1; dd < 64; dd <<= 1) {
@@ -851,7 +910,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 }
             } else {
                 uint32_t nc = ncand;
-                uint32_t order[kMaxCand];
+                uint8_t *order = order_s;
                 for (uint32_t x = 0; x < nc; ++x) {
                     uint32_t y = x;
                     while (y > 0 &&
@@ -909,7 +968,16 @@ __global__ void k_mibf_zero(unsigned long long *p, uint32_t n) {
 /* the first pass over every record, then the wide pass over the records it
  * listed (with an empty list the wide workgroups end at once) */
 template <uint32_t CAP, uint32_t SMAX, bool DIRECT>
-void query_pair(const QueryArgs &a, uint32_t blocks, size_t shm, hipStream_t st) {
+void query_pair(const QueryArgs &a, int cu, size_t shm, hipStream_t st) {
+    /* as many one-wave workgroups as fit on every CU (registers and LDS),
+     * each pulling records from the queue */
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_mibf_query<CAP, SMAX, DIRECT, false>, 64, shm) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 16;
+    const uint32_t blocks =
+        uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(a.nrec, uint64_t(cu) * per_cu)));
     k_mibf_query<CAP, SMAX, DIRECT, false><<<blocks, 64, shm, st>>>(a);
     if (a.spill_stride)
         k_mibf_query<CAP, SMAX, DIRECT, true><<<kWideBlocks, 64, shm, st>>>(a);
@@ -1521,7 +1589,7 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     if (mf->has_last)
         MHIP(hipStreamWaitEvent(st, mf->last, 0));
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 3);
-    const uint32_t blocks = uint32_t(std::min<uint64_t>(nrec, uint64_t(mf->cu) * 16));
+    const int blocks = mf->cu;
     const bool few = mf->d.S <= 4;
     if (direct) {
         const size_t shm = size_t(kDirectWords) * (mf->d.nids + 1) * sizeof(uint32_t);
