@@ -28,6 +28,7 @@
 #include "../../include/bbt_hip.h"
 #include "bbt_device.h"
 #include "bbt_dust.h"
+#include "bbt_reads.h"
 #include "bbt_host.h"
 
 using namespace bbt;
@@ -847,6 +848,9 @@ struct bbt_workspace {
     uint32_t *dmask = nullptr;
     uint64_t dmask_words = 0;
     DustScratch *dscratch = nullptr;
+    /* BBT_READS_2BIT: the batch unpacked to ASCII */
+    uint8_t *dascii = nullptr;
+    uint64_t ascii_cap = 0;
 };
 
 static void workspace_release(bbt_workspace *w) {
@@ -866,6 +870,8 @@ static void workspace_release(bbt_workspace *w) {
         (void)hipFree(w->dmask);
     if (w->dscratch)
         (void)hipFree(w->dscratch);
+    if (w->dascii)
+        (void)hipFree(w->dascii);
     *w = bbt_workspace();
 }
 
@@ -1240,6 +1246,8 @@ bbt_status bbt_eval_subtract(const bbt_filterset *fs, uint32_t idx, const bbt_fi
     if (!fs || idx >= fs->desc.size() || !in || (in->nreads && (!in->seq || !in->off || !pass)) ||
         (sub && sub_idx >= sub->desc.size()))
         return fail(BBT_ERR_ARG, "bbt_eval_subtract: bad argument");
+    if (in->format != BBT_READS_ASCII)
+        return fail(BBT_ERR_UNSUPPORTED, "bbt_eval_subtract: ASCII reads only");
     if (scorer != BBT_SIMPLE && scorer != BBT_LENGTH)
         return fail(BBT_ERR_UNSUPPORTED, "bbt_eval_subtract: SIMPLE or LENGTH scoring only");
     SubEvalArgs a;
@@ -1528,6 +1536,8 @@ struct Prepared {
     uint64_t nrec;
     uint32_t kk;
     uint32_t wave_min;
+    const uint8_t *codes; /* BBT_READS_2BIT: the packed batch, unpacked first */
+    uint64_t code_bytes;
 };
 
 /* noalloc: the call may not allocate (bbt_eval_batch_noalloc): W already
@@ -1552,6 +1562,26 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     if (reinterpret_cast<uintptr_t>(in->seq) % 8 != 0 || in->seq_bytes % 8 != 0)
         return fail(BBT_ERR_ARG,
                     "bbt_eval_batch: seq must be 8-byte aligned with a size multiple of 8");
+    if (in->format > BBT_READS_2BIT)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: unknown read format");
+    if (in->format == BBT_READS_2BIT) {
+        if (noalloc)
+            return fail(BBT_ERR_UNSUPPORTED,
+                        "bbt_eval_batch_noalloc: 2-bit reads need an ASCII staging buffer; "
+                        "use bbt_eval_batch");
+        /* the unpacked batch: 4 bases per code byte, and a word of padding */
+        const uint64_t need = 4 * in->seq_bytes + 8;
+        if (W.ascii_cap < need) {
+            if (W.dascii) {
+                HIP_OK(hipStreamSynchronize(st)); /* the previous batch may read it */
+                HIP_OK(hipFree(W.dascii));
+            }
+            W.dascii = nullptr;
+            W.ascii_cap = 0;
+            HIP_OK(hipMalloc(&W.dascii, need));
+            W.ascii_cap = need;
+        }
+    }
     if (p->paired && (in->nreads & 1))
         return fail(BBT_ERR_ARG, "bbt_eval_batch: paired batch needs an even read count");
     if (p->mode > BBT_SCORES || p->scorer > BBT_BINOMIAL || p->path > BBT_PATH_BATCHED)
@@ -1605,7 +1635,10 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     if (p->dust) {
         if (p->dust_w < 3 || p->dust_w > uint32_t(kDustMaxW) || p->dust_t > 1000000)
             return fail(BBT_ERR_UNSUPPORTED, "dust window (-W) must be 3..64");
-        const uint64_t words = in->seq_bytes / 32 + 2;
+        /* one bit per base position of the batch (4 per code byte when packed) */
+        const uint64_t positions =
+            in->format == BBT_READS_2BIT ? 4 * in->seq_bytes : in->seq_bytes;
+        const uint64_t words = positions / 32 + 2;
         if (W.dmask_words < words) {
             if (W.dmask) {
                 HIP_OK(hipStreamSynchronize(st));
@@ -1638,6 +1671,11 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     L.blocks_per_cu = fs->tune.blocks_per_cu;
     EvalArgs &a = L.a;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
+    if (in->format == BBT_READS_2BIT) {
+        P.codes = a.seq;
+        P.code_bytes = in->seq_bytes;
+        a.seq = W.dascii;
+    }
     a.off = in->off;
     a.order = in->order;
     a.filters = fs->ddesc;
@@ -1672,6 +1710,12 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     /* work queues of the (up to two) walks of this call, and for the batched
      * path the length buckets: long reads walked by whole waves */
     k_zero_u64<<<1, 64, 0, st>>>(W.dctl, 48);
+    if (P.code_bytes) {
+        const uint64_t nw = P.code_bytes / 8;
+        k_unpack_2bit<<<uint32_t(std::min<uint64_t>((nw + kBlock - 1) / kBlock, 65536)), kBlock,
+                        0, st>>>(P.codes, P.code_bytes, const_cast<uint8_t *>(a.seq));
+        HIP_OK(hipGetLastError());
+    }
     a.queue = W.dctl + 0;
     a.lqueue = W.dctl + 2;
     a.dust = nullptr;
@@ -1752,6 +1796,7 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     add(&P.L.a.filters, sizeof(P.L.a.filters));
     add(&P.nf, sizeof(P.nf));
     add(&P.kk, sizeof(P.kk));
+    add(&P.L.a.seq, sizeof(P.L.a.seq)); /* the staging buffer of 2-bit reads */
     add(in, sizeof(*in));
     add(p, sizeof(*p));
     add(out, sizeof(*out));

@@ -33,6 +33,7 @@
 #include "../../include/bbt_hip.h"
 #include "bbt_device.h"
 #include "bbt_host.h"
+#include "bbt_reads.h"
 
 using namespace bbt;
 
@@ -1016,6 +1017,9 @@ struct bbt_mibf {
      * freeing or refilling those buffers */
     hipEvent_t last = nullptr;
     bool has_last = false;
+    /* BBT_READS_2BIT queries: the batch unpacked to ASCII */
+    uint8_t *dascii = nullptr;
+    uint64_t ascii_cap = 0;
     std::mutex mu;
 };
 
@@ -1145,6 +1149,7 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
     (void)hipFree(mf->dctl);
     (void)hipFree(mf->dwide_list);
     (void)hipFree(mf->dspill);
+    (void)hipFree(mf->dascii);
     if (mf->last)
         (void)hipEventDestroy(mf->last);
     delete mf;
@@ -1480,6 +1485,10 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
         return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: max_hits must be 1..8192");
     if (nrec > 0xFFFFFFFFull)
         return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: more than 2^32 - 1 records in one batch");
+    if (in->format > BBT_READS_2BIT || reinterpret_cast<uintptr_t>(in->seq) % 8 ||
+        (in->format == BBT_READS_2BIT && in->seq_bytes % 8))
+        return mfail(BBT_ERR_ARG, "bbt_mibf_query_batch: unknown read format, or 2-bit reads "
+                                  "not 8-byte aligned");
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> g(mf->mu);
     const uint32_t k = mf->d.span;
@@ -1584,11 +1593,30 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     }
     a.wide_list = mf->dwide_list;
     a.spill = mf->dspill;
+    if (in->format == BBT_READS_2BIT) {
+        const uint64_t need = 4 * in->seq_bytes + 8;
+        if (mf->ascii_cap < need) {
+            if (wait_last() != BBT_OK)
+                return BBT_ERR_HIP;
+            (void)hipFree(mf->dascii);
+            mf->dascii = nullptr;
+            mf->ascii_cap = 0;
+            MHIP(hipMalloc(&mf->dascii, need));
+            mf->ascii_cap = need;
+        }
+        a.seq = mf->dascii;
+    }
     /* the queue counters are shared by every query of this filter: this
      * query's kernels start after the previous query's have finished */
     if (mf->has_last)
         MHIP(hipStreamWaitEvent(st, mf->last, 0));
     k_mibf_zero<<<1, 64, 0, st>>>(mf->dctl + 8, 3);
+    if (in->format == BBT_READS_2BIT) {
+        const uint64_t nw = in->seq_bytes / 8;
+        k_unpack_2bit<<<uint32_t(std::min<uint64_t>((nw + 255) / 256, 65536)), 256, 0, st>>>(
+            reinterpret_cast<const uint8_t *>(in->seq), in->seq_bytes, mf->dascii);
+        MHIP(hipGetLastError());
+    }
     const int blocks = mf->cu;
     const bool few = mf->d.S <= 4;
     if (direct) {

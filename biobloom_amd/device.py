@@ -30,6 +30,21 @@ class DeviceReads:
         self.order = None
         if order is not None:
             self.order = torch.from_numpy(np.ascontiguousarray(order, np.int32)).to(device)
+        self.format, self.seq_bytes = B.READS_ASCII, self.seq.numel()
+
+    @classmethod
+    def packed(cls, buf, off, device="cuda", order=None):
+        """The same reads in the BBT_READS_2BIT layout (3/8 byte per base)."""
+        data, cb = pack_2bit(buf, off)
+        self = cls(np.zeros(0, np.uint8), np.zeros(1, np.uint64), device, order)
+        off = np.ascontiguousarray(off, np.uint64)
+        self.seq = torch.from_numpy(data).to(device)
+        self.off = torch.from_numpy(off.view(np.int64)).to(device)
+        self.nreads = len(off) - 1
+        self.lens = np.diff(off)
+        self.max_len = int(self.lens.max()) if len(self.lens) else 0
+        self.format, self.seq_bytes = B.READS_2BIT, cb
+        return self
 
     @classmethod
     def from_matrix(cls, reads2d, device="cuda"):
@@ -38,9 +53,23 @@ class DeviceReads:
         return cls(reads2d.reshape(-1), off, device)
 
     def struct(self):
-        return B.Reads(self.seq.data_ptr(), self.seq.numel(), self.off.data_ptr(),
-                       self.nreads, self.max_len, 0,
+        return B.Reads(self.seq.data_ptr(), self.seq_bytes, self.off.data_ptr(),
+                       self.nreads, self.max_len, self.format,
                        self.order.data_ptr() if self.order is not None else None)
+
+
+def pack_2bit(buf, off):
+    """Reads (uint8 buffer, nreads + 1 offsets) in the BBT_READS_2BIT layout
+    of include/bbt_hip.h: (codes then N mask as one uint8 array, code bytes)."""
+    n = int(off[-1])
+    cb = (n + 31) // 32 * 8
+    padded = np.full(cb * 4, ord("A"), np.uint8)
+    padded[:n] = buf[:n]
+    c = ((padded >> 1) & 3).reshape(-1, 4)
+    codes = (c[:, 0] | (c[:, 1] << 2) | (c[:, 2] << 4) | (c[:, 3] << 6)).astype(np.uint8)
+    up = padded & 0xDF
+    bad = ~((up == ord("A")) | (up == ord("C")) | (up == ord("G")) | (up == ord("T")))
+    return np.concatenate([codes, np.packbits(bad, bitorder="little")]), cb
 
 
 def length_order(lens):

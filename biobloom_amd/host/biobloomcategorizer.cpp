@@ -46,6 +46,8 @@ static void printHelpDialog() {
             "  -S, --score_type=N     'harmonic' or 'binomial' scoring. [simple]\n"
             "      --status           Also write <prefix>_readStatus.tsv (read -> class).\n"
             "      --plain            Use the plain one-launch-per-batch kernel.\n"
+            "      --ascii_reads      Send reads to the device as text (default: 2-bit\n"
+            "                         codes and an N mask, 3/8 of the bytes).\n"
             "      --batch_reads=N    Reads (or pairs) per device batch. [262144]\n"
             "      --ingest_block=N   Bytes of input parsed per block. [16777216]\n"
             "  -h, --help             Display this dialog.\n";
@@ -82,7 +84,7 @@ static bool fexists(const string &f) {
 int main(int argc, char *argv[]) {
     bbt_fe::Options opt;
     bool die = false, paired = false, stdoutFilter = false, collab = false;
-    int fastq = 0, fasta = 0, status = 0, plain = 0;
+    int fastq = 0, fasta = 0, status = 0, plain = 0, ascii_reads = 0;
     string filtersFile, prefix, postfix, outputType, fileListFilename;
     double binomialScore = 100;
     static struct option long_options[] = {
@@ -99,6 +101,7 @@ int main(int argc, char *argv[]) {
         {"fa", no_argument, &fasta, 1},
         {"status", no_argument, &status, 1},
         {"plain", no_argument, &plain, 1},
+        {"ascii_reads", no_argument, &ascii_reads, 1},
         {"file_list", required_argument, NULL, 'l'},
         {"version", no_argument, NULL, 'v'},
         {"multi", required_argument, NULL, 'm'},
@@ -223,6 +226,7 @@ int main(int argc, char *argv[]) {
     opt.read_status = status != 0;
     if (plain)
         opt.path = BBT_PATH_PLAIN;
+    opt.pack2bit = ascii_reads == 0;
     vector<string> filterFilePaths = convertInputString(filtersFile);
     vector<string> inputFiles;
     while (optind < argc)

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "front_end.h"
+#include "read_pack.h"
 
 using namespace std;
 
@@ -50,6 +51,7 @@ struct Opt {
     int verbose = 0;
     unsigned threads = 0; /* parse threads per file (0: the reader's default) */
     uint64_t batch = uint64_t(1) << 17; /* reads (or pairs) per device batch */
+    bool pack2bit = true; /* send reads 2-bit packed (false: ASCII) */
 };
 
 /* candidates a batch's query keeps per record; the few records with more
@@ -76,6 +78,8 @@ void usage() {
             "  -v, --verbose          Display verbose output\n"
             "  -I, --interval         the interval to report file processing status [10000000]\n"
             "      --batch_reads=N    Reads (or pairs) per device batch. [131072]\n"
+            "      --ascii_reads      Send reads to the device as text (default: 2-bit\n"
+            "                         codes and an N mask, 3/8 of the bytes).\n"
             "Advanced options:\n"
             "  -a, --frameMatches=N   Min number seed matches needed in a frame to match [1]\n"
             "                         Ignored if k-mers used when indexing.\n"
@@ -105,6 +109,7 @@ struct Slot {
     uint64_t *dctr = nullptr, *hctr = nullptr;
     size_t seq_cap = 0, rec_cap = 0, hit_cap = 0;
     size_t nreads = 0, bytes = 0;
+    size_t code_bytes = 0; /* 2-bit batch: bytes of codes (the mask follows); 0: ASCII */
     uint32_t stride = 0; /* candidates per record in dhits / hhits */
     uint32_t max_len = 0;
 
@@ -337,7 +342,8 @@ class MIBFClassifier {
             bytes += r.seq_len;
             max_len = std::max(max_len, r.seq_len);
         }
-        const size_t padded = (bytes + 7) / 8 * 8 + 8;
+        const size_t padded = m_o.pack2bit ? bbt_host::packed_bytes(bytes)
+                                           : (bytes + 7) / 8 * 8 + 8;
         S.reserve(padded, n);
         uint64_t o = 0;
         for (size_t i = 0; i < n; ++i) {
@@ -345,6 +351,16 @@ class MIBFClassifier {
             o += v[i].seq_len;
         }
         S.hoff[n] = o;
+        S.max_len = max_len;
+        if (m_o.pack2bit) {
+            /* 2-bit codes and the N mask straight from the records' views */
+            const auto get = [&](uint64_t r) { return v[r].seq; };
+            S.code_bytes = bbt_host::pack_2bit(get, S.hoff, n,
+                                               reinterpret_cast<uint8_t *>(S.hseq), 8);
+            S.bytes = padded;
+            return;
+        }
+        S.code_bytes = 0;
         const unsigned T = o < (size_t(8) << 20) ? 1u : 8u;
         const auto gather = [&](unsigned t) {
             for (size_t i = n * t / T, e = n * (t + 1) / T; i < e; ++i)
@@ -384,7 +400,8 @@ class MIBFClassifier {
         bbt_reads in;
         std::memset(&in, 0, sizeof(in));
         in.seq = S.dseq;
-        in.seq_bytes = S.bytes;
+        in.seq_bytes = S.code_bytes ? S.code_bytes : S.bytes;
+        in.format = S.code_bytes ? BBT_READS_2BIT : BBT_READS_ASCII;
         in.off = S.doff;
         in.nreads = n;
         in.max_len = S.max_len;
@@ -710,6 +727,7 @@ int main(int argc, char *argv[]) {
                                  {"debug", no_argument, &debug, 1},
                                  {"verbose", no_argument, NULL, 'v'},
                                  {"batch_reads", required_argument, NULL, 1000},
+                                 {"ascii_reads", no_argument, NULL, 1001},
                                  {NULL, 0, NULL, 0}};
     int c, idx = 0;
     while ((c = getopt_long(argc, argv, "p:f:es:hI:t:f:m:r:dnvc:a:bi", lo, &idx)) != -1) {
@@ -729,6 +747,7 @@ int main(int argc, char *argv[]) {
                 exit(EXIT_FAILURE);
             }
             break;
+        case 1001: o.pack2bit = false; break;
         case 'h': usage(); break;
         case 'I':
             if (!(cv >> o.interval)) {

@@ -8,6 +8,7 @@
  * BioBloomClassifier.h:115-529 and ResultsManager.hpp:191-237.
  */
 #include "front_end.h"
+#include "read_pack.h"
 
 #include "deflate_rt.h"
 
@@ -61,6 +62,7 @@ struct BioBloomClassifier::Batch {
     uint64_t cap_reads = 0, cap_bytes = 0;
     uint64_t n = 0; /* reads packed / in flight */
     uint64_t max_len = 0;
+    uint64_t code_bytes = 0; /* 2-bit batch: bytes of codes (the mask follows) */
     bool paired = false;
     char *d_seq = nullptr, *h_seq = nullptr;
     uint64_t *d_off = nullptr, *h_off = nullptr;
@@ -192,7 +194,8 @@ void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired, uint64_t
      * batches ramp up): growing the pinned and device buffers synchronizes
      * the device */
     const auto t0 = std::chrono::steady_clock::now();
-    b.reserve(n * scale, bytes * scale);
+    const uint64_t need = m_opt.pack2bit ? bbt_host::packed_bytes(bytes) : bytes;
+    b.reserve(n * scale, need * scale);
     m_phase[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     uint64_t o = 0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -200,6 +203,14 @@ void BioBloomClassifier::pack(int s, const RecBatch &recs, bool paired, uint64_t
         o += recs.v[i].seq_len;
     }
     b.h_off[n] = o;
+    b.max_len = maxlen;
+    if (m_opt.pack2bit) {
+        /* 2-bit codes and the N mask straight from the records' views */
+        const auto get = [&](uint64_t r) { return recs.v[r].seq; };
+        b.code_bytes = bbt_host::pack_2bit(get, b.h_off, n, reinterpret_cast<uint8_t *>(b.h_seq),
+                                           8);
+        return;
+    }
     /* gather the bases into the pinned staging buffer on a few threads */
     const unsigned T = o < (uint64_t(8) << 20) ? 1u : 8u;
     const auto gather = [&](unsigned t) {
@@ -227,14 +238,16 @@ void BioBloomClassifier::submit(int s) {
     if (n == 0)
         return;
     const uint64_t o = b.h_off[n];
-    HIP_CHECK(hipMemcpyAsync(b.d_seq, b.h_seq, (o + 7) / 8 * 8, hipMemcpyHostToDevice, b.st));
+    const uint64_t copy = m_opt.pack2bit ? b.code_bytes * 3 / 2 : (o + 7) / 8 * 8;
+    HIP_CHECK(hipMemcpyAsync(b.d_seq, b.h_seq, copy, hipMemcpyHostToDevice, b.st));
     HIP_CHECK(hipMemcpyAsync(b.d_off, b.h_off, (n + 1) * 8, hipMemcpyHostToDevice, b.st));
     if (other.walk_recorded)
         HIP_CHECK(hipStreamWaitEvent(b.st, other.walked, 0));
     bbt_reads in;
     std::memset(&in, 0, sizeof(in));
     in.seq = b.d_seq;
-    in.seq_bytes = b.cap_bytes;
+    in.seq_bytes = m_opt.pack2bit ? b.code_bytes : b.cap_bytes;
+    in.format = m_opt.pack2bit ? BBT_READS_2BIT : BBT_READS_ASCII;
     in.off = b.d_off;
     in.nreads = n;
     in.max_len = uint32_t(b.max_len);

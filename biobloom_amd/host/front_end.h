@@ -42,6 +42,7 @@ struct Options {
     uint32_t dust_w = 64;            /* opt::dustWindow (-W) */
     unsigned threads = 0;            /* opt::threads (-t): parse threads per file, 0 = the reader's default */
     uint64_t ingest_block = 16u << 20; /* bytes of input parsed per block */
+    bool pack2bit = true;            /* send reads 2-bit packed (false: ASCII) */
 };
 
 /* The records of one device batch: views into parsed blocks, the blocks

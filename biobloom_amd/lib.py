@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(PKG, "libbbt_hip.so")
 SIMPLE, LENGTH, HARMONIC, BINOMIAL = 0, 1, 2, 3
 STD, ORDERED, BESTHIT, SCORES = 0, 1, 2, 3
 PATH_PLAIN, PATH_BATCHED = 0, 1
+READS_ASCII, READS_2BIT = 0, 1
 FLAG_TIME_WALK = 1
 FLAG_GRAPH = 2
 
@@ -44,7 +45,7 @@ class Tuning(C.Structure):
 class Reads(C.Structure):
     _fields_ = [("seq", C.c_void_p), ("seq_bytes", C.c_uint64), ("off", C.c_void_p),
                 ("nreads", C.c_uint64), ("max_len", C.c_uint32),
-                ("reserved", C.c_uint32), ("order", C.c_void_p)]
+                ("format", C.c_uint32), ("order", C.c_void_p)]
 
 
 class EvalOut(C.Structure):

@@ -129,14 +129,27 @@ typedef struct {
     uint32_t reserved;
 } bbt_eval_params;
 
+/* read formats (bbt_reads.format) */
+#define BBT_READS_ASCII 0 /* one byte per base, as read from FASTA/FASTQ */
+#define BBT_READS_2BIT 1  /* 2-bit codes and an N mask, 3/8 byte per base:
+                             base j of the batch is bits 2(j%4)..2(j%4)+1 of
+                             byte j/4 (A 0, C 1, T 2, G 3, i.e. (ASCII >> 1) & 3),
+                             and bit j%8 of byte seq_bytes + j/8 is set when
+                             base j is not A/C/G/T in either case (its code is
+                             then ignored).  Case is not kept: no kernel
+                             depends on it.  Unpacked on the device. */
+
 typedef struct {
-    const char *seq;       /* device: bases of all reads back to back (ASCII) */
-    uint64_t seq_bytes;    /* readable bytes at seq; must be >= off[nreads]
-                              rounded up to a multiple of 8 */
-    const uint64_t *off;   /* device: nreads + 1 offsets into seq */
+    const char *seq;       /* device: bases of all reads back to back, in format */
+    uint64_t seq_bytes;    /* ASCII: readable bytes at seq, >= off[nreads]
+                              rounded up to a multiple of 8.  2BIT: bytes of
+                              codes, a multiple of 8 with 4 x seq_bytes >=
+                              off[nreads]; the mask (seq_bytes / 2 bytes)
+                              follows them at seq + seq_bytes */
+    const uint64_t *off;   /* device: nreads + 1 offsets (in bases) */
     uint64_t nreads;
     uint32_t max_len;      /* upper bound on read length */
-    uint32_t reserved;
+    uint32_t format;       /* BBT_READS_ASCII or BBT_READS_2BIT */
     const uint32_t *order; /* device, optional: processing order of reads
                               (a permutation of 0..nreads-1), or NULL */
 } bbt_reads;

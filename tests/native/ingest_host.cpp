@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 
+#include "read_pack.h"
 #include "seq_ingest.h"
 
 extern "C" {
@@ -64,5 +65,17 @@ long ingest_dump(const char *path, int mode, unsigned threads, unsigned long blo
     *status = st;
     return n;
 }
+
+/* The 2-bit packer of the front ends (read_pack.h) over reads given back to
+ * back in seq with nreads + 1 offsets: writes packed_bytes(off[nreads])
+ * bytes into out and returns the code bytes. */
+unsigned long pack_2bit_reads(const char *seq, const unsigned long *off, unsigned long nreads,
+                              unsigned char *out, unsigned threads) {
+    const uint64_t *o = reinterpret_cast<const uint64_t *>(off);
+    const auto get = [&](uint64_t r) { return seq + o[r]; };
+    return bbt_host::pack_2bit(get, o, nreads, out, threads);
+}
+
+unsigned long packed_bytes_for(unsigned long nbases) { return bbt_host::packed_bytes(nbases); }
 
 } // extern "C"

@@ -90,10 +90,11 @@ def test_maker_matches_oracle_filter(data):
         assert filters[i].m == O.lib().orc_calc_optimal_size(60_000 - 48, 0.075, 3)
 
 
-@pytest.mark.parametrize("plain", [False, True])
-def test_single_end_summary_and_status(data, plain):
+@pytest.mark.parametrize("extra", [[], ["--plain"], ["--ascii_reads"]])
+def test_single_end_summary_and_status(data, extra):
+    """Reads go to the device 2-bit packed by default, as text with
+    --ascii_reads; both kernel paths."""
     d, reads, filters = data
-    extra = ["--plain"] if plain else []
     got = run_cat(d, extra + ["--status", str(d / "all.fq")], "se")
     buf, off = O.pack_reads([s for _, s in reads])
     hits, _, cls, _ = O.classify(filters, buf, off)

@@ -172,9 +172,10 @@ def test_categorizer_single_end(data, out, hit_only):
 
 
 def test_categorizer_options(data):
-    """-s, -a, -m, -c and -b reach the query."""
+    """-s, -a, -m, -c and -b reach the query (reads sent as text here: the
+    other tests send them 2-bit packed)."""
     B, d, seeds, recs, names, reads = data
-    lines, summary = run_cat(d, ["-s", "30", "-a", "2", "-m", "0.5", "-c", "2", "-b",
+    lines, summary = run_cat(d, ["--ascii_reads", "-s", "30", "-a", "2", "-m", "0.5", "-c", "2", "-b",
                                  str(d / "all.fq")], "opts")
     om = O.Mibf(recs, seeds)
     rows, sats, _ = om.query([s for _, s in reads], score=10 ** -3.0, allowed_miss=1,

@@ -472,3 +472,27 @@ def test_direct_table_and_id_zero_slots(dev, tmp_path, monkeypatch):
         got, gsat, gctr, _ = classify_both(D, B, mz, dr, B.mibf_params(1e-10, 1.0, a, 0, 8))
         assert got == want and gsat == wsat
         assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+
+
+def test_2bit_reads_query_like_ascii(dev):
+    """MIBF queries of BBT_READS_2BIT batches equal the ASCII batch's and the
+    oracle's, single and paired, with lower case and N in the reads."""
+    D, B = dev
+    rng = random.Random(55)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=6, L=3000)
+    om = O.Mibf(recs, seeds, occupancy=0.5)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = [r.lower() if i % 3 == 0 else r for i, r in enumerate(reads_from(rng, recs, 800))]
+    buf, off = O.pack_reads(reads)
+    da, dp = D.DeviceReads(buf, off), D.DeviceReads.packed(buf, off)
+    for paired in (False, True):
+        print(f"[gpu] mibf 2-bit reads paired={paired}", flush=True)
+        p = B.mibf_params(1e-10, 1.0, 2, 0, 8, paired=paired)
+        want, wsat, _ = om.query(reads, score=1e-10, allowed_miss=2, multi=1.0, min_nonsat=0,
+                                 max_hits=8, paired=paired)
+        ga = D.mibf_classify(mf, da, p)
+        gp = classify_both(D, B, mf, dp, p)
+        assert gp[0] == ga[0] == want and gp[1] == ga[1] == wsat
+        assert np.array_equal(gp[2], ga[2])

@@ -307,3 +307,53 @@ def test_binomial_thresholds_follow_uploads(dev):
     print("[gpu] binomial after upload", flush=True)
     check_batch(D, B, fs, ofs2, buf, off, scorer=O.BINOMIAL, thr=1e-10)
     check_batch(D, B, fs, ofs2, buf, off, scorer=O.BINOMIAL, thr=1e-10, mode=O.BESTHIT)
+
+
+def test_2bit_reads_classify_like_ascii(dev, multi):
+    """BBT_READS_2BIT batches (codes + N mask, unpacked on the device) give the
+    ASCII batch's hits, scores, classes, summary and counters, against the
+    oracle on the ASCII text: lower case, N and other bytes included, every
+    mode and scorer, both paths, -D and paired end."""
+    D, B = dev
+    from test_ingest_host import pack_model
+    fs, ofs, buf, off = multi
+    rng = np.random.default_rng(21)
+    reads = [bytes(buf[int(off[i]):int(off[i + 1])]) for i in range(len(off) - 1)]
+    odd = []
+    for i, r in enumerate(reads[:600]):
+        a = bytearray(r)
+        for j in rng.integers(0, max(1, len(a)), 3) if len(a) else []:
+            a[j] = rng.choice(list(b"acgtnNRY-"))
+        odd.append(bytes(a))
+    reads = odd + reads[600:]
+    if len(reads) % 2:
+        reads.append(b"ACGT" * 40)
+    pbuf, poff = O.pack_reads(reads)
+    data, cb = D.pack_2bit(pbuf, poff)
+    want_cb, want = pack_model(reads)
+    assert cb == want_cb and np.array_equal(data, want)
+    da, dp = D.DeviceReads(pbuf, poff), D.DeviceReads.packed(pbuf, poff)
+    assert dp.seq.numel() * 8 < da.seq.numel() * 3 + 64
+    cases = [dict(), dict(scorer=O.HARMONIC, thr=0.2), dict(scorer=O.BINOMIAL, thr=1e-10),
+             dict(mode=O.BESTHIT, scorer=O.LENGTH, thr=30.0), dict(mode=O.ORDERED),
+             dict(paired=True), dict(dust=(20, 64)), dict(mode=O.SCORES)]
+    for c in cases:
+        mode, scorer, thr = c.get("mode", 0), c.get("scorer", 0), c.get("thr", 0.15)
+        paired, dust = c.get("paired", False), c.get("dust")
+        dt, dw = dust if dust else (20, 64)
+        want_scores = mode in (O.BESTHIT, O.SCORES)
+        oh, osc, ocls, octr = O.classify(ofs, pbuf, poff, paired, False, mode,
+                                         O.params(scorer, thr, 3, bool(dust), dt, dw),
+                                         want_scores=want_scores)
+        for path in PATHS:
+            print(f"[gpu] 2-bit reads {c} path={path}", flush=True)
+            p = B.params(mode, scorer, thr, 3, paired, False, path, dust=bool(dust),
+                         dust_t=dt, dust_w=dw)
+            ra, rp = D.classify(fs, da, p), D.classify(fs, dp, p)
+            for key in ("hits", "cls", "summary", "counters"):
+                assert np.array_equal(ra[key], rp[key]), (c, key)
+            if want_scores:
+                assert np.array_equal(ra["scores"].view(np.uint64), rp["scores"].view(np.uint64))
+                assert np.array_equal(rp["scores"].view(np.uint64), osc.view(np.uint64))
+            assert np.array_equal(rp["hits"], oh) and np.array_equal(rp["cls"], ocls)
+            assert int(rp["counters"][1]) == int(octr[1])

@@ -21,7 +21,8 @@ LIB = os.path.join(NATIVE, "libingest_host.so")
 SRCS = [os.path.join(NATIVE, "ingest_host.cpp"),
         os.path.join(ROOT, "biobloom_amd", "host", "seq_ingest.cpp"),
         os.path.join(ROOT, "biobloom_amd", "host", "seq_ingest.h"),
-        os.path.join(ROOT, "biobloom_amd", "host", "deflate_rt.h")]
+        os.path.join(ROOT, "biobloom_amd", "host", "deflate_rt.h"),
+        os.path.join(ROOT, "biobloom_amd", "host", "read_pack.h")]
 
 
 def build():
@@ -281,3 +282,63 @@ def test_bgzf_cut_short_equals_sequential(L, tmp_path):
             assert want == full, pad
             for mode, threads in [(1, 4), (2, 16)]:
                 assert dump(L, p, mode, threads)[:2] == want, (pad, mode)
+
+
+def pack_model(reads):
+    """The BBT_READS_2BIT layout (include/bbt_hip.h) restated with numpy:
+    code (b >> 1) & 3 per base, four per byte; then one N-mask bit per base."""
+    buf = np.frombuffer(b"".join(reads), np.uint8)
+    n = len(buf)
+    cb = (n + 31) // 32 * 8
+    padded = np.full(cb * 4, ord("A"), np.uint8)
+    padded[:n] = buf
+    codes = (padded >> 1) & 3
+    packed = (codes.reshape(-1, 4) << np.array([0, 2, 4, 6], np.uint8)).sum(1).astype(np.uint8)
+    up = padded & 0xDF
+    bad = ~np.isin(up, np.frombuffer(b"ACGT", np.uint8))
+    mask = np.packbits(bad, bitorder="little")
+    return cb, np.concatenate([packed, mask])
+
+
+def unpack_model(packed, cb, n):
+    """What the device's unpacking gives back: 'A' 'C' 'T' 'G' by code, 'N'
+    for masked bases."""
+    codes = np.unpackbits(packed[:cb], bitorder="little").reshape(-1, 2)
+    codes = codes[:, 0] | (codes[:, 1] << 1)
+    mask = np.unpackbits(packed[cb:cb + cb // 2], bitorder="little").astype(bool)
+    out = np.frombuffer(b"ACTG", np.uint8)[codes]
+    out[mask] = ord("N")
+    return out[:n]
+
+
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_pack_2bit_matches_model(L, threads):
+    """The front ends' SWAR packer equals the layout's restatement on reads of
+    every length 0..80 (unaligned starts), upper and lower case, N and other
+    bytes; unpacking gives the text with case folded and non-ACGT as N."""
+    L.pack_2bit_reads.restype = C.c_ulong
+    L.pack_2bit_reads.argtypes = [C.c_char_p, C.POINTER(C.c_ulong), C.c_ulong, C.c_void_p,
+                                  C.c_uint]
+    L.packed_bytes_for.restype = C.c_ulong
+    rng = np.random.default_rng(threads)
+    alphabet = np.frombuffer(b"ACGTacgtNnRY-.*\0", np.uint8)
+    reads = []
+    for i in range(3000):
+        ln = i % 81 if i < 1000 else int(rng.integers(0, 400))
+        p = np.full(16, 0.0)
+        p[:8] = 0.118
+        p[8:] = 0.056 / 8
+        reads.append(bytes(rng.choice(alphabet, ln, p=p / p.sum())))
+    off = np.zeros(len(reads) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    seq = b"".join(reads)
+    n = int(off[-1])
+    out = np.zeros(L.packed_bytes_for(n), np.uint8)
+    cb = L.pack_2bit_reads(seq, off.ctypes.data_as(C.POINTER(C.c_ulong)), len(reads),
+                           out.ctypes.data, threads)
+    want_cb, want = pack_model(reads)
+    assert cb == want_cb and np.array_equal(out, want)
+    text = np.frombuffer(seq, np.uint8)
+    folded = np.where(np.isin(text & 0xDF, np.frombuffer(b"ACGT", np.uint8)), text & 0xDF,
+                      ord("N")).astype(np.uint8)
+    assert np.array_equal(unpack_model(out, cb, n), folded)
