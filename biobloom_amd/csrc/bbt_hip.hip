@@ -58,6 +58,8 @@ constexpr int kBlock = 256;
 constexpr uint32_t kDefaultWaveMin = 256;
 /* idle lanes of a wave that trigger a refill from the queue */
 constexpr uint32_t kDefaultRefillMin = 32;
+/* probe unroll: lookups of one contains() call issued together */
+constexpr uint32_t kAhead = 4;
 
 struct EvalArgs {
     const uint8_t *seq;
@@ -76,6 +78,7 @@ struct EvalArgs {
     uint64_t nitems;
     uint32_t nf;
     uint32_t refill_min;
+    uint32_t probe_ahead; /* lane walks: issue a contains() call's lookups together */
     WalkParams wp;
 };
 
@@ -325,6 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     bool drained = false;
     uint64_t read = 0;
     uint32_t f = 0;
+    uint32_t ahead_extra = 0; /* probe unroll: lookups issued past a call's end */
     while (true) {
         const uint64_t idle = __ballot(!active);
         const uint32_t nidle = uint32_t(__popcll(idle));
@@ -351,15 +355,40 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
         }
         if (active) {
             if (w.advance()) {
-                const unsigned byte = *w.addr;
-                w.consume(byte);
+                if (a.probe_ahead && w.j == 0) {
+                    /* probe unroll: the first kAhead lookups of this contains()
+                     * call as independent loads, consumed in order until the
+                     * call ends (a clear bit or the last hash); the loads past
+                     * that are issued-only */
+                    const uint32_t hh = w.h < kAhead ? w.h : kAhead;
+                    const uint64_t ks = w.kseed();
+                    unsigned bytes[kAhead];
+#pragma unroll
+                    for (uint32_t jj = 0; jj < kAhead; ++jj) {
+                        bytes[jj] = 0;
+                        if (jj < hh) {
+                            const uint64_t pos =
+                                fastmod(multi_hash(w.h0, jj, ks), w.m, w.magic, w.sh1, w.sh2);
+                            bytes[jj] = w.bits[pos >> 3];
+                        }
+                    }
+                    const uint32_t before = w.lookups;
+#pragma unroll
+                    for (uint32_t jj = 0; jj < kAhead; ++jj)
+                        if (jj < hh && w.pending)
+                            w.consume(bytes[jj]);
+                    ahead_extra += hh - (w.lookups - before);
+                } else {
+                    const unsigned byte = *w.addr;
+                    w.consume(byte);
+                }
             } else {
                 store_walk(a, read, f, w);
                 active = false;
             }
         }
     }
-    issued += w.lookups;
+    issued += w.lookups + ahead_extra;
     add_counters(a.counters, w.probes + wave_probes, w.lookups + wave_lookups);
     add_issued(a.counters, issued);
 }
@@ -895,8 +924,10 @@ struct bbt_filterset {
     /* work queue counter and per-call scratch */
     unsigned long long *dscratch = nullptr; /* [3] popcount, [4..6] insert counts */
     int cu_count = 0;
-    /* scheduling knobs (bbt_set_tuning / bbt_autotune) */
+    /* scheduling knobs (bbt_set_tuning / bbt_autotune); without them
+     * (tuned false) a batch takes the tuning table's entry for its geometry */
     bbt_tuning tune = {0, 0, 0, 0};
+    bool tuned = false;
     bbt_workspace ws; /* of bbt_eval_batch */
     std::mutex mu;
 };
@@ -1528,6 +1559,41 @@ bbt_status ensure_bscore(bbt_filterset *fs, uint32_t max_frames) {
 
 } // namespace
 
+/* ---- the tuning table ---------------------------------------------------
+ * Knobs bbt_autotune chose, by batch geometry: filter-set size (log2 of its
+ * bytes), filter count, common k (0 if mixed), largest h, longest read
+ * (log2), mode, scorer, pairing. */
+static std::mutex g_tune_mu;
+static std::map<std::string, bbt_tuning> g_tune_table;
+
+static unsigned floor_log2(uint64_t v) {
+    unsigned l = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++l;
+    }
+    return l;
+}
+
+static std::string tune_key(const bbt_filterset *fs, const bbt_reads *in,
+                            const bbt_eval_params *p) {
+    uint64_t bytes = 0;
+    uint32_t hmax = 0, kc = fs->desc.empty() ? 0 : fs->desc[0].k;
+    for (size_t i = 0; i < fs->desc.size(); ++i) {
+        bytes += fs->info[i].m / 8;
+        hmax = std::max(hmax, fs->desc[i].h);
+        if (fs->desc[i].k != kc)
+            kc = 0;
+    }
+    char buf[192];
+    std::snprintf(buf, sizeof(buf),
+                  "bytes_log2=%u nf=%zu k=%u h=%u len_log2=%u mode=%u scorer=%u paired=%u",
+                  floor_log2(bytes), fs->desc.size(), kc, hmax,
+                  floor_log2(std::max<uint64_t>(1, in->max_len)), p->mode, p->scorer,
+                  p->paired ? 1u : 0u);
+    return buf;
+}
+
 /* Everything of one bbt_eval_batch that is host work (checks, tables,
  * allocations) and the argument block of its kernels. */
 struct Prepared {
@@ -1668,7 +1734,16 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     L.st = st;
     L.path = p->path;
     L.cu = fs->cu_count;
-    L.blocks_per_cu = fs->tune.blocks_per_cu;
+    bbt_tuning tune = fs->tune;
+    if (!fs->tuned && !noalloc) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        if (!g_tune_table.empty()) {
+            const auto it = g_tune_table.find(tune_key(fs, in, p));
+            if (it != g_tune_table.end())
+                tune = it->second;
+        }
+    }
+    L.blocks_per_cu = tune.blocks_per_cu;
     EvalArgs &a = L.a;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
     if (in->format == BBT_READS_2BIT) {
@@ -1683,7 +1758,8 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.scores = out->scores;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.nf = nf;
-    a.refill_min = fs->tune.refill_min ? fs->tune.refill_min : kDefaultRefillMin;
+    a.refill_min = tune.refill_min ? tune.refill_min : kDefaultRefillMin;
+    a.probe_ahead = tune.probe_ahead ? 1u : 0u;
     a.wp.threshold = p->threshold;
     a.wp.streak = p->streak;
     a.wp.min_len = uint32_t(std::round(p->threshold));
@@ -1692,8 +1768,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     P.nrec = p->paired ? in->nreads / 2 : in->nreads;
     P.kk = common_k == 25 ? 25u : 0u;
     P.wave_min = p->wave_min_len ? p->wave_min_len
-                                 : (fs->tune.wave_min_len ? fs->tune.wave_min_len
-                                                          : kDefaultWaveMin);
+                                 : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
     return BBT_OK;
 }
 
@@ -1802,6 +1877,7 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     add(out, sizeof(*out));
     add(&P.wave_min, sizeof(P.wave_min));
     add(&P.L.a.refill_min, sizeof(P.L.a.refill_min));
+    add(&P.L.a.probe_ahead, sizeof(P.L.a.probe_ahead));
     add(&P.L.blocks_per_cu, sizeof(P.L.blocks_per_cu));
     add(&st, sizeof(st));
     return k;
@@ -2009,6 +2085,7 @@ extern "C" bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t) {
         return fail(BBT_ERR_ARG, "bbt_set_tuning: null argument");
     std::lock_guard<std::mutex> g(fs->mu);
     fs->tune = *t;
+    fs->tuned = true;
     return BBT_OK;
 }
 
@@ -2045,9 +2122,11 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t waves[] = {64, 128, 256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
     const uint32_t refills[] = {1, 4, 8, 16, 32};
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
-    for (int dim = 0; dim < 3; ++dim) {
-        const uint32_t *vals = dim == 0 ? waves : (dim == 1 ? refills : bpcs);
-        const int nv = dim == 0 ? 8 : 5;
+    const uint32_t aheads[] = {0, 1};
+    for (int dim = 0; dim < 4; ++dim) {
+        const uint32_t *vals =
+            dim == 0 ? waves : (dim == 1 ? refills : (dim == 2 ? bpcs : aheads));
+        const int nv = dim == 0 ? 8 : (dim == 3 ? 2 : 5);
         for (int i = 0; i < nv; ++i) {
             bbt_tuning t = cur;
             if (dim == 0) {
@@ -2056,8 +2135,10 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
                 t.wave_min_len = vals[i];
             } else if (dim == 1) {
                 t.refill_min = vals[i];
-            } else {
+            } else if (dim == 2) {
                 t.blocks_per_cu = vals[i];
+            } else {
+                t.probe_ahead = vals[i];
             }
             double ms = 0;
             s = trial(t, ms);
@@ -2073,7 +2154,85 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     if (best)
         *best = cur;
     fs->ws.last_walk_ms = best_ms;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_table[tune_key(fs, in, p)] = cur;
+    }
     return s;
+}
+
+extern "C" bbt_status bbt_tuning_lookup(const bbt_filterset *fs, const bbt_reads *in,
+                                        const bbt_eval_params *p, bbt_tuning *t) {
+    if (!fs || !in || !p || !t)
+        return fail(BBT_ERR_ARG, "bbt_tuning_lookup: null argument");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    const auto it = g_tune_table.find(tune_key(fs, in, p));
+    if (it == g_tune_table.end())
+        return fail(BBT_ERR_ARG, "bbt_tuning_lookup: no entry for this geometry");
+    *t = it->second;
+    return BBT_OK;
+}
+
+/* one line per entry: the geometry, a tab, then the four knobs */
+extern "C" bbt_status bbt_tuning_save(const char *path) {
+    if (!path)
+        return fail(BBT_ERR_ARG, "bbt_tuning_save: null path");
+    std::string text;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        for (const auto &kv : g_tune_table) {
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "\t%u %u %u %u\n", kv.second.wave_min_len,
+                          kv.second.refill_min, kv.second.blocks_per_cu,
+                          kv.second.probe_ahead);
+            text += kv.first + buf;
+        }
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f)
+        return fail(BBT_ERR_IO, std::string("bbt_tuning_save: cannot write ") + path);
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    if (std::fclose(f) != 0 || !ok)
+        return fail(BBT_ERR_IO, std::string("bbt_tuning_save: cannot write ") + path);
+    return BBT_OK;
+}
+
+extern "C" bbt_status bbt_tuning_load(const char *path) {
+    if (!path)
+        return fail(BBT_ERR_ARG, "bbt_tuning_load: null path");
+    FILE *f = std::fopen(path, "r");
+    if (!f)
+        return fail(BBT_ERR_IO, std::string("bbt_tuning_load: cannot read ") + path);
+    std::map<std::string, bbt_tuning> got;
+    char line[512];
+    unsigned lineno = 0;
+    bbt_status st = BBT_OK;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++lineno;
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r'))
+            l.pop_back();
+        if (l.empty())
+            continue;
+        const size_t tab = l.find('\t');
+        bbt_tuning t = {0, 0, 0, 0};
+        char extra = 0;
+        if (tab == std::string::npos || tab == 0 ||
+            std::sscanf(l.c_str() + tab + 1, "%u %u %u %u %c", &t.wave_min_len, &t.refill_min,
+                        &t.blocks_per_cu, &t.probe_ahead, &extra) != 4) {
+            st = fail(BBT_ERR_FORMAT, std::string("bbt_tuning_load: malformed line ") +
+                                          std::to_string(lineno) + " of " + path);
+            break;
+        }
+        got[l.substr(0, tab)] = t;
+    }
+    std::fclose(f);
+    if (st != BBT_OK)
+        return st;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const auto &kv : got)
+        g_tune_table[kv.first] = kv.second;
+    return BBT_OK;
 }
 
 extern "C" double bbt_last_walk_ms(const bbt_filterset *fs) {

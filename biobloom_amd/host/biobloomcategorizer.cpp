@@ -48,6 +48,10 @@ static void printHelpDialog() {
             "      --plain            Use the plain one-launch-per-batch kernel.\n"
             "      --ascii_reads      Send reads to the device as text (default: 2-bit\n"
             "                         codes and an N mask, 3/8 of the bytes).\n"
+            "      --tune_file=FILE   Tuning table of the batched kernels: the knobs for\n"
+            "                         this filter size and read length are taken from it.\n"
+            "      --autotune         Time the batched kernels' knobs on the first batch\n"
+            "                         and keep the fastest (saved to --tune_file).\n"
             "      --batch_reads=N    Reads (or pairs) per device batch. [262144]\n"
             "      --ingest_block=N   Bytes of input parsed per block. [16777216]\n"
             "  -h, --help             Display this dialog.\n";
@@ -84,7 +88,7 @@ static bool fexists(const string &f) {
 int main(int argc, char *argv[]) {
     bbt_fe::Options opt;
     bool die = false, paired = false, stdoutFilter = false, collab = false;
-    int fastq = 0, fasta = 0, status = 0, plain = 0, ascii_reads = 0;
+    int fastq = 0, fasta = 0, status = 0, plain = 0, ascii_reads = 0, autotune = 0;
     string filtersFile, prefix, postfix, outputType, fileListFilename;
     double binomialScore = 100;
     static struct option long_options[] = {
@@ -102,6 +106,8 @@ int main(int argc, char *argv[]) {
         {"status", no_argument, &status, 1},
         {"plain", no_argument, &plain, 1},
         {"ascii_reads", no_argument, &ascii_reads, 1},
+        {"autotune", no_argument, &autotune, 1},
+        {"tune_file", required_argument, NULL, 1002},
         {"file_list", required_argument, NULL, 'l'},
         {"version", no_argument, NULL, 'v'},
         {"multi", required_argument, NULL, 'm'},
@@ -155,6 +161,7 @@ int main(int argc, char *argv[]) {
             }
             break;
         }
+        case 1002: opt.tune_file = optarg; break;
         case 'e': paired = true; break;
         case 'i': opt.inclusive = true; break;
         case 't': {
@@ -227,6 +234,7 @@ int main(int argc, char *argv[]) {
     if (plain)
         opt.path = BBT_PATH_PLAIN;
     opt.pack2bit = ascii_reads == 0;
+    opt.autotune = autotune != 0;
     vector<string> filterFilePaths = convertInputString(filtersFile);
     vector<string> inputFiles;
     while (optind < argc)

@@ -13,6 +13,7 @@
 #include "deflate_rt.h"
 
 #include <hip/hip_runtime.h>
+#include <unistd.h>
 
 #include <cctype>
 #include <chrono>
@@ -132,6 +133,11 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
         std::cerr << std::endl;
     }
     std::cerr << "Filter Loading Complete." << std::endl;
+    if (!m_opt.tune_file.empty() && access(m_opt.tune_file.c_str(), R_OK) == 0) {
+        /* knobs tuned earlier for this filter size and read length */
+        bbt_check(bbt_tuning_load(m_opt.tune_file.c_str()));
+        std::cerr << "Loaded tuning table: " << m_opt.tune_file << std::endl;
+    }
     /* each slot has its own stream and workspace: one slot's copies overlap
      * the other's kernels */
     for (Batch *&b : m_slot) {
@@ -270,6 +276,24 @@ void BioBloomClassifier::submit(int s) {
     out.cls = b.d_cls;
     out.summary = m_dsummary;
     out.counters = m_dctr;
+    if (m_opt.autotune && !m_autotuned && m_opt.path == BBT_PATH_BATCHED &&
+        m_opt.mode != BBT_ORDERED) {
+        /* the knobs timed on this first batch; outputs only (the trials
+         * must not add to the summary) */
+        m_autotuned = true;
+        bbt_eval_out t_out;
+        std::memset(&t_out, 0, sizeof(t_out));
+        t_out.hits = b.d_hits;
+        t_out.scores = b.d_scores;
+        bbt_tuning best;
+        bbt_check(bbt_autotune(m_fs, &in, &p, &t_out, b.st, &best));
+        std::cerr << "Autotuned: reads of " << best.wave_min_len
+                  << "+ bases per wavefront, refill at " << best.refill_min
+                  << " idle lanes, " << best.blocks_per_cu << " blocks per CU, probe unroll "
+                  << best.probe_ahead << std::endl;
+        if (!m_opt.tune_file.empty())
+            bbt_check(bbt_tuning_save(m_opt.tune_file.c_str()));
+    }
     bbt_check(bbt_eval_batch_ws(m_fs, b.ws, &in, &p, &out, b.st));
     HIP_CHECK(hipEventRecord(b.walked, b.st));
     b.walk_recorded = true;

@@ -43,6 +43,8 @@ struct Options {
     unsigned threads = 0;            /* opt::threads (-t): parse threads per file, 0 = the reader's default */
     uint64_t ingest_block = 16u << 20; /* bytes of input parsed per block */
     bool pack2bit = true;            /* send reads 2-bit packed (false: ASCII) */
+    bool autotune = false;           /* tune the batched path's knobs on the first batch */
+    std::string tune_file;           /* tuning table: loaded at start, saved after tuning */
 };
 
 /* The records of one device batch: views into parsed blocks, the blocks
@@ -126,6 +128,7 @@ class BioBloomClassifier {
     /* s: gather, pack + launch, device wait, output; staging allocation (part
      * of pack + launch) */
     double m_phase[5] = {0, 0, 0, 0, 0};
+    bool m_autotuned = false; /* --autotune ran on a batch */
 };
 
 } // namespace bbt_fe

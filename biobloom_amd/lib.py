@@ -39,7 +39,7 @@ class EvalParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wave_min_len", C.c_uint32), ("refill_min", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("probe_ahead", C.c_uint32)]
 
 
 class Reads(C.Structure):
@@ -151,6 +151,10 @@ _SIGS = {
     "bbt_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     "bbt_autotune": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
                                  C.POINTER(EvalOut), C.c_void_p, C.POINTER(Tuning)]),
+    "bbt_tuning_save": (C.c_int32, [C.c_char_p]),
+    "bbt_tuning_load": (C.c_int32, [C.c_char_p]),
+    "bbt_tuning_lookup": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
+                                      C.POINTER(Tuning)]),
     "bbt_summary_text": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64,
                                      C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bbt_calc_min_count": (C.c_uint64, [C.c_uint64, C.c_double, C.c_double]),
@@ -297,22 +301,28 @@ class FilterSet:
         check(load().bbt_eval_batch(self._h, C.byref(reads), C.byref(params),
                                     C.byref(out), C.c_void_p(stream)))
 
-    def set_tuning(self, wave_min_len=0, refill_min=0, blocks_per_cu=0):
+    def set_tuning(self, wave_min_len=0, refill_min=0, blocks_per_cu=0, probe_ahead=0):
         check(load().bbt_set_tuning(self._h, C.byref(Tuning(wave_min_len, refill_min,
-                                                             blocks_per_cu, 0))))
+                                                             blocks_per_cu, probe_ahead))))
 
     def get_tuning(self):
         t = Tuning()
         check(load().bbt_get_tuning(self._h, C.byref(t)))
-        return {"wave_min_len": t.wave_min_len, "refill_min": t.refill_min,
-                "blocks_per_cu": t.blocks_per_cu}
+        return tuning_dict(t)
 
     def autotune(self, reads, params, out, stream=0):
         t = Tuning()
         check(load().bbt_autotune(self._h, C.byref(reads), C.byref(params), C.byref(out),
                                   C.c_void_p(stream), C.byref(t)))
-        return {"wave_min_len": t.wave_min_len, "refill_min": t.refill_min,
-                "blocks_per_cu": t.blocks_per_cu, "walk_ms": self.last_walk_ms()}
+        d = tuning_dict(t)
+        d["walk_ms"] = self.last_walk_ms()
+        return d
+
+    def tuning_lookup(self, reads, params):
+        """The tuning table's knobs for this batch's geometry, or None."""
+        t = Tuning()
+        st = load().bbt_tuning_lookup(self._h, C.byref(reads), C.byref(params), C.byref(t))
+        return tuning_dict(t) if st == 0 else None
 
     def last_walk_ms(self):
         return load().bbt_last_walk_ms(self._h)
@@ -325,6 +335,19 @@ class FilterSet:
         buf = C.create_string_buffer(need.value + 1)
         check(L.bbt_summary_text(self._h, arr, nrec, buf, need.value + 1, None))
         return buf.value.decode()
+
+
+def tuning_dict(t):
+    return {"wave_min_len": t.wave_min_len, "refill_min": t.refill_min,
+            "blocks_per_cu": t.blocks_per_cu, "probe_ahead": t.probe_ahead}
+
+
+def tuning_save(path):
+    check(load().bbt_tuning_save(str(path).encode()))
+
+
+def tuning_load(path):
+    check(load().bbt_tuning_load(str(path).encode()))
 
 
 def measure_gather(nbytes, rounds=64, stream=0):

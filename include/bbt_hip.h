@@ -92,12 +92,17 @@ typedef int32_t bbt_status;
 typedef struct bbt_filterset bbt_filterset;
 
 /* Scheduling knobs of the batched path; 0 = library default.  They change
- * only how work is spread over the GPU, never a result. */
+ * only how work is spread over the GPU, never a result (nor the reference
+ * lookup counters; probe_ahead adds to the issued-lookup counter only). */
 typedef struct {
-    uint32_t wave_min_len;  /* reads this long are walked by a whole wavefront */
+    uint32_t wave_min_len;  /* reads per wavefront: reads this long are walked
+                               by a whole wavefront, shorter ones 64 per wave */
     uint32_t refill_min;    /* idle lanes a wave gathers before refilling */
     uint32_t blocks_per_cu; /* persistent blocks (256 threads) per CU */
-    uint32_t reserved;
+    uint32_t probe_ahead;   /* probe unroll of the lane walks: 1 = a contains()
+                               call's first 4 lookups are issued together
+                               (independent loads; those past the first clear
+                               bit are wasted), 0 = one lookup at a time */
 } bbt_tuning;
 
 typedef struct {
@@ -259,9 +264,23 @@ bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t);
 bbt_status bbt_get_tuning(const bbt_filterset *fs, bbt_tuning *t);
 /* Times candidate knob settings of the batched path on this batch (each
  * candidate runs the full batch; outputs are overwritten, counters added
- * to), keeps the fastest in fs and returns it in *best. */
+ * to), keeps the fastest in fs and returns it in *best.  The choice is also
+ * kept in the process's tuning table under the batch's geometry: filter-set
+ * size (log2 of its bytes), filter count, k, h, longest read (log2), mode,
+ * scorer and pairing.  A batch on a filter set without knobs of its own
+ * (bbt_set_tuning, bbt_autotune) takes the table's entry for its geometry. */
 bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in, const bbt_eval_params *p,
                         bbt_eval_out *out, void *stream, bbt_tuning *best);
+/* The tuning table as text (one geometry and its knobs per line), so a
+ * later process reuses the tuned knobs: save writes every entry, load adds
+ * the file's entries (BBT_ERR_IO if it cannot be read, BBT_ERR_FORMAT for a
+ * malformed line).  No counterpart in the reference. */
+bbt_status bbt_tuning_save(const char *path);
+bbt_status bbt_tuning_load(const char *path);
+/* Knobs the table holds for this batch's geometry on fs: BBT_OK and *t, or
+ * BBT_ERR_ARG when there is no entry. */
+bbt_status bbt_tuning_lookup(const bbt_filterset *fs, const bbt_reads *in,
+                             const bbt_eval_params *p, bbt_tuning *t);
 /* Measurement utility (no counterpart in the reference): rate of
  * independent random byte loads over a fresh nbytes device buffer (power of
  * two), 8 in flight per lane, rounds x 8 loads per lane; each load touches

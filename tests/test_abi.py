@@ -88,3 +88,27 @@ def test_host_helpers_match_reference_formulas():
     assert L.bbt_calc_optimal_size(5_000_000 - 24, 0.075, 3) == 27_387_648
     for n, p, q in [(126, 0.075, 1e-10), (1976, 0.0237, 1e-5), (30, 0.3, 0.01)]:
         assert L.bbt_calc_min_count(n, p, q) == O.lib().orc_calc_min_count(n, p, q)
+
+
+def test_tuning_table_save_and_load(tmp_path):
+    """The tuning table's text form: loaded entries are saved back line for
+    line (sorted by geometry); a malformed line is a format error; a missing
+    file an I/O error."""
+    from biobloom_amd import lib as B
+    L = B.load()
+    lines = ["bytes_log2=26 nf=10 k=25 h=3 len_log2=7 mode=0 scorer=0 paired=1\t"
+             "2147483647 0 4 0",
+             "bytes_log2=28 nf=4 k=25 h=4 len_log2=15 mode=0 scorer=0 paired=0\t256 16 0 1"]
+    src = tmp_path / "tune.tsv"
+    src.write_text("\n".join(lines) + "\n")
+    assert L.bbt_tuning_load(str(src).encode()) == 0
+    out = tmp_path / "saved.tsv"
+    assert L.bbt_tuning_save(str(out).encode()) == 0
+    saved = out.read_text().splitlines()
+    for ln in lines:
+        assert ln in saved
+    bad = tmp_path / "bad.tsv"
+    bad.write_text("bytes_log2=1 nf=1\t1 2 x\n")
+    assert L.bbt_tuning_load(str(bad).encode()) == 3
+    assert "malformed line 1" in L.bbt_last_error().decode()
+    assert L.bbt_tuning_load(str(tmp_path / "missing.tsv").encode()) == 2

@@ -404,3 +404,20 @@ def test_score_outputs_stdout_and_gz(data, tmp_path):
     out = run(["-d", "-n"], "n")
     assert out == "".join(f"@{rn} c\n{s}\n+\n{'I' * len(s)}\n"
                           for (rn, s), c in zip(sub, cls) if c != 0)
+
+
+def test_autotune_and_tuning_file(data, tmp_path):
+    """--autotune times the knobs on the first batch and saves them to
+    --tune_file; a later run takes them from the file.  Summaries equal the
+    oracle's both times."""
+    d, reads, filters = data
+    tune = tmp_path / "tune.tsv"
+    buf, off = O.pack_reads([s for _, s in reads])
+    hits, _, _, _ = O.classify(filters, buf, off)
+    want = O.summary(["g0", "g1", "g2"], hits)
+    got = run_cat(d, ["--autotune", f"--tune_file={tune}", str(d / "all.fq")], "tuned")
+    assert got == want
+    lines = tune.read_text().splitlines()
+    assert len(lines) >= 1 and all("\t" in ln for ln in lines)
+    got = run_cat(d, [f"--tune_file={tune}", str(d / "all.fq")], "tuned2")
+    assert got == want

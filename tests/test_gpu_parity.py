@@ -357,3 +357,51 @@ def test_2bit_reads_classify_like_ascii(dev, multi):
                 assert np.array_equal(rp["scores"].view(np.uint64), osc.view(np.uint64))
             assert np.array_equal(rp["hits"], oh) and np.array_equal(rp["cls"], ocls)
             assert int(rp["counters"][1]) == int(octr[1])
+
+
+def test_probe_unroll_keeps_results(dev, multi):
+    """probe_ahead (a contains() call's lookups issued together) changes no
+    hit, score, class or reference counter; it only issues more lookups."""
+    D, B = dev
+    fs, ofs, buf, off = multi
+    try:
+        for refill in (1, 32):
+            fs.set_tuning(refill_min=refill, probe_ahead=1)
+            for scorer, thr in [(O.SIMPLE, 0.15), (O.LENGTH, 40.0), (O.BINOMIAL, 1e-10)]:
+                check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
+            check_batch(D, B, fs, ofs, buf, off, paired=True)
+            check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=O.HARMONIC, thr=0.15)
+    finally:
+        fs.set_tuning()
+
+
+def test_tuning_table_carries_knobs(dev, tmp_path):
+    """bbt_autotune records its choice under the batch's geometry; another
+    filter set of the same geometry without knobs of its own takes it, also
+    after a save and load; results stay the oracle's."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(3, 9)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 120_000)) for _ in range(2)]
+    longs, _ = synth.long_reads(rng, genomes, 30, median=2500, hi=8000)
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 1500, 150, 0.01, 0.7)
+    buf, off = O.pack_reads([bytes(x) for x in longs + list(shorts)])
+    fs, _ = D.build_filters(genomes, 25, 0.05)
+    dr = D.DeviceReads(buf, off)
+    out = D.Outputs(dr.nreads, len(fs), False, False)
+    p = B.params()
+    print("[gpu] tuning table", flush=True)
+    best = fs.autotune(dr.struct(), p, out.struct(), int(torch.cuda.current_stream().cuda_stream))
+    got = fs.tuning_lookup(dr.struct(), p)
+    assert got == {k: best[k] for k in got}
+    path = tmp_path / "tune.tsv"
+    B.tuning_save(path)
+    B.tuning_load(path)
+    fs2, _ = D.build_filters(genomes, 25, 0.05)
+    assert fs2.tuning_lookup(dr.struct(), p) == got
+    assert fs2.tuning_lookup(dr.struct(), B.params(paired=True)) is None
+    ofs = oracle_filters_like(fs2, B)
+    oh, _, ocls, octr = O.classify(ofs, buf, off)
+    res = D.classify(fs2, dr, p)
+    assert np.array_equal(res["hits"], oh) and np.array_equal(res["cls"], ocls)
+    assert int(res["counters"][1]) == int(octr[1])
