@@ -19,24 +19,43 @@
 
 namespace bbt {
 
-constexpr int kDustMaxW = 64;
-/* the perfect-interval list holds distinct (start, finish) word pairs of one
- * window: at most (W-2)(W-1)/2 of them */
-constexpr uint32_t kDustPCap = (kDustMaxW - 2) * (kDustMaxW - 1) / 2 + 64;
-/* threads of the pre-pass (each owns one DustScratch) */
+/* largest -W: word counts are 16-bit, and the perfect-interval list of one
+ * window grows as W^2 (below) */
+constexpr int kDustMaxW = 4096;
+/* threads of the pre-pass at most (each owns one scratch region) */
 constexpr uint32_t kDustThreads = 8192;
+/* the pre-pass's scratch for all threads at most (large W: fewer threads) */
+constexpr uint64_t kDustScratchBudget = uint64_t(2) << 30;
 
 struct DustPerf {
-    int32_t start, finish;
-    uint16_t r, l;
+    int32_t start, finish, r, l;
 };
 
-/* per-thread state of one sdust pass, in device memory */
-struct DustScratch {
-    uint8_t cv[64], cw[64], c[64];
-    uint8_t ring[64]; /* the window's words, a ring of W - 2 <= 62 */
-    DustPerf P[kDustPCap];
+/* Per-thread scratch of one sdust pass, in device memory: counts of the 64
+ * words in the window (cv: from the last perfect boundary, cw: whole
+ * window) and a work copy c, the window's words as a ring of ring_cap >=
+ * W - 2 entries, then the perfect-interval list, which holds distinct
+ * (start, finish) word pairs of one window: at most (W-2)(W-1)/2 of them. */
+struct DustLayout {
+    uint32_t ring_cap; /* a power of two */
+    uint32_t pcap;
+    uint64_t perf_at;  /* byte offset of the list in a thread's region */
+    uint64_t stride;   /* bytes per thread */
+    uint32_t threads;  /* threads of the pre-pass */
 };
+
+inline DustLayout dust_layout(int W) {
+    DustLayout d;
+    d.ring_cap = 64;
+    while (int(d.ring_cap) < W - 2)
+        d.ring_cap <<= 1;
+    d.pcap = uint32_t(W - 2) * uint32_t(W - 1) / 2 + 64;
+    d.perf_at = ((3 * 64 + uint64_t(d.ring_cap)) * 2 + 15) / 16 * 16;
+    d.stride = (d.perf_at + uint64_t(d.pcap) * sizeof(DustPerf) + 255) / 256 * 256;
+    const uint64_t fit = kDustScratchBudget / d.stride / 64 * 64;
+    d.threads = uint32_t(fit < 64 ? 64 : (fit > kDustThreads ? kDustThreads : fit));
+    return d;
+}
 
 __device__ __forceinline__ int dust_nt4(unsigned c) {
     switch (c) {
@@ -48,8 +67,20 @@ __device__ __forceinline__ int dust_nt4(unsigned c) {
     }
 }
 
-__device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, DustScratch &S,
-                                 uint32_t *mask, uint64_t gbase, bool &overflow) {
+__device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, uint8_t *region,
+                                 const DustLayout &lay, uint32_t *mask, uint64_t gbase,
+                                 bool &overflow) {
+    struct View {
+        uint16_t *cv, *cw, *c, *ring;
+        DustPerf *P;
+    } S;
+    S.cv = reinterpret_cast<uint16_t *>(region);
+    S.cw = S.cv + 64;
+    S.c = S.cw + 64;
+    S.ring = S.c + 64;
+    S.P = reinterpret_cast<DustPerf *>(region + lay.perf_at);
+    const uint32_t rmask = lay.ring_cap - 1;
+    const uint32_t kDustPCap = lay.pcap;
     for (int t = 0; t < 64; ++t) {
         S.cv[t] = 0;
         S.cw[t] = 0;
@@ -90,7 +121,7 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
         int r = rv, max_r = 0, max_l = 0;
         const int wn = int(cnt);
         for (int i = wn - L - 1; i >= 0; --i) {
-            const unsigned tt = S.ring[(head + uint32_t(i)) & 63];
+            const unsigned tt = S.ring[(head + uint32_t(i)) & rmask];
             r += S.c[tt]++;
             const int new_r = r, new_l = wn - i - 1;
             if (new_r * 10 > T * new_l) {
@@ -111,8 +142,7 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
                     }
                     for (uint32_t x = np; x > j; --x)
                         S.P[x] = S.P[x - 1];
-                    S.P[j] = DustPerf{i + start, wn + 2 + start, uint16_t(new_r),
-                                      uint16_t(new_l)};
+                    S.P[j] = DustPerf{i + start, wn + 2 + start, new_r, new_l};
                     ++np;
                 }
             }
@@ -130,7 +160,7 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
                 flush(start);
                 if (int(cnt) >= W - 2) {
                     const unsigned s0 = S.ring[head];
-                    head = (head + 1) & 63;
+                    head = (head + 1) & rmask;
                     --cnt;
                     rw -= --S.cw[s0];
                     if (L > int(cnt)) {
@@ -138,7 +168,7 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
                         rv -= --S.cv[s0];
                     }
                 }
-                S.ring[(head + cnt) & 63] = uint8_t(t);
+                S.ring[(head + cnt) & rmask] = uint16_t(t);
                 ++cnt;
                 ++L;
                 rw += S.cw[t]++;
@@ -146,7 +176,7 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
                 if (S.cv[t] * 10 > T * 2) {
                     unsigned s0;
                     do {
-                        s0 = S.ring[(head + cnt - uint32_t(L)) & 63];
+                        s0 = S.ring[(head + cnt - uint32_t(L)) & rmask];
                         rv -= --S.cv[s0];
                         --L;
                     } while (s0 != t);
@@ -166,20 +196,21 @@ __device__ inline void dust_read(const uint8_t *s, uint32_t n, int T, int W, Dus
         emit(res_s, res_f);
 }
 
-/* One thread per read from a queue; each thread owns scratch[thread]. */
+/* One thread per read from a queue; each thread owns lay.stride bytes of
+ * scratch. */
 __global__ __launch_bounds__(64) void k_dust(const uint8_t *seq, const uint64_t *off,
-                                             uint64_t nreads, int T, int W,
-                                             DustScratch *scratch, uint32_t *mask,
+                                             uint64_t nreads, int T, int W, uint8_t *scratch,
+                                             DustLayout lay, uint32_t *mask,
                                              unsigned long long *queue,
                                              unsigned long long *err) {
-    DustScratch &S = scratch[uint64_t(blockIdx.x) * blockDim.x + threadIdx.x];
+    uint8_t *S = scratch + (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * lay.stride;
     bool overflow = false;
     while (true) {
         const unsigned long long r = atomicAdd(queue, 1ull);
         if (r >= nreads)
             break;
         const uint64_t b = off[r];
-        dust_read(seq + b, uint32_t(off[r + 1] - b), T, W, S, mask, b, overflow);
+        dust_read(seq + b, uint32_t(off[r + 1] - b), T, W, S, lay, mask, b, overflow);
     }
     if (overflow && err)
         atomicOr(err, 1ull);

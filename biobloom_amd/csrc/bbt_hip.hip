@@ -23,6 +23,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bbt_hip.h"
@@ -61,6 +62,15 @@ constexpr uint32_t kDefaultRefillMin = 32;
 /* probe unroll: lookups of one contains() call issued together */
 constexpr uint32_t kAhead = 4;
 
+/* A (read, filter) item whose BINOMIAL evalScore the host computes: the
+ * walk's hit count over its frame count (the read has more frames than the
+ * score table); the host writes the score's bits back over frames/count. */
+struct ScoreFix {
+    uint64_t idx; /* read * nf + filter */
+    uint32_t frames;
+    uint32_t count;
+};
+
 struct EvalArgs {
     const uint8_t *seq;
     const uint64_t *off;
@@ -79,6 +89,10 @@ struct EvalArgs {
     uint32_t nf;
     uint32_t refill_min;
     uint32_t probe_ahead; /* lane walks: issue a contains() call's lookups together */
+    /* BINOMIAL evalScore past the score table: the walk stores the hit count
+     * and lists the item; the host turns the counts into scores */
+    unsigned long long *fix_count;
+    ScoreFix *fix_list;
     WalkParams wp;
 };
 
@@ -125,10 +139,19 @@ __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uin
                 v = -0.0;
             } else {
                 const uint64_t fl = uint64_t(w.len) - kv + 1;
-                if (F.bscore && fl <= F.bscore_frames)
+                if (F.bscore && fl <= F.bscore_frames) {
                     v = F.bscore[fl * (fl + 1) / 2 + w.score2];
-                else
+                } else if (a.fix_list) {
+                    const unsigned long long slot = atomicAdd(a.fix_count, 1ull);
+                    ScoreFix fx;
+                    fx.idx = idx;
+                    fx.frames = uint32_t(fl);
+                    fx.count = w.score2;
+                    a.fix_list[slot] = fx;
+                    v = double(w.score2);
+                } else {
                     bad = true;
+                }
             }
         }
         a.scores[idx] = v;
@@ -515,6 +538,17 @@ __global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v) {
         p[i] = v;
 }
 
+/* the host-computed scores of the listed items (value bits in place of
+ * frames/count) into their rows */
+__global__ void k_fix_scores(const ScoreFix *fix, uint64_t n, double *scores) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const ScoreFix f = fix[i];
+        const uint64_t bits = uint64_t(f.frames) | (uint64_t(f.count) << 32);
+        scores[f.idx] = __longlong_as_double((long long)bits);
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
         p[i] = 0;
@@ -876,10 +910,14 @@ struct bbt_workspace {
     /* -D: the batch's low-complexity bitmap and the pre-pass's scratch */
     uint32_t *dmask = nullptr;
     uint64_t dmask_words = 0;
-    DustScratch *dscratch = nullptr;
+    uint8_t *dscratch = nullptr; /* sized for dscratch_w */
+    uint32_t dscratch_w = 0;
     /* BBT_READS_2BIT: the batch unpacked to ASCII */
     uint8_t *dascii = nullptr;
     uint64_t ascii_cap = 0;
+    /* BINOMIAL scores past the table: the listed items */
+    ScoreFix *dfix = nullptr;
+    uint64_t fix_cap = 0;
 };
 
 static void workspace_release(bbt_workspace *w) {
@@ -901,6 +939,8 @@ static void workspace_release(bbt_workspace *w) {
         (void)hipFree(w->dscratch);
     if (w->dascii)
         (void)hipFree(w->dascii);
+    if (w->dfix)
+        (void)hipFree(w->dfix);
     *w = bbt_workspace();
 }
 
@@ -1604,6 +1644,7 @@ struct Prepared {
     uint32_t wave_min;
     const uint8_t *codes; /* BBT_READS_2BIT: the packed batch, unpacked first */
     uint64_t code_bytes;
+    bool fix_scores;      /* BINOMIAL scores past the table, finished on the host */
 };
 
 /* noalloc: the call may not allocate (bbt_eval_batch_noalloc): W already
@@ -1655,6 +1696,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     if (!out->hits)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: hits output is required");
     const bool need_scores = p->mode == BBT_BESTHIT || p->mode == BBT_SCORES;
+    bool fix_scores = false; /* BINOMIAL scores of reads past the table */
     if (need_scores && !out->scores)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: scores output required for this mode");
     if (need_scores && p->scorer == BBT_BINOMIAL) {
@@ -1662,15 +1704,31 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         for (const FilterDev &d : fs->desc)
             kmin = std::min(kmin, d.k);
         const uint32_t frames = in->max_len >= kmin ? in->max_len - kmin + 1 : 0;
-        if (frames > kBinomScoreMaxFrames)
-            return fail(BBT_ERR_UNSUPPORTED,
-                        "binomial evalScore (best-hit / with-score modes) covers reads of up "
-                        "to 1024 k-mers");
-        if (noalloc && !bscore_fresh(fs, frames))
+        if (frames > kBinomScoreMaxFrames) {
+            /* reads past the table: the device counts, the host scores */
+            fix_scores = true;
+            if (noalloc || (p->flags & BBT_FLAG_GRAPH))
+                return fail(BBT_ERR_UNSUPPORTED,
+                            "binomial evalScore of reads over 1024 k-mers is finished on the "
+                            "host: not available without allocation or in a graph");
+            const uint64_t need = in->nreads * uint64_t(nf);
+            if (W.fix_cap < need) {
+                if (W.dfix) {
+                    HIP_OK(hipStreamSynchronize(st));
+                    HIP_OK(hipFree(W.dfix));
+                }
+                W.dfix = nullptr;
+                W.fix_cap = 0;
+                HIP_OK(hipMalloc(&W.dfix, need * sizeof(ScoreFix)));
+                W.fix_cap = need;
+            }
+        }
+        const uint32_t table = std::min(frames, kBinomScoreMaxFrames);
+        if (noalloc && !bscore_fresh(fs, table))
             return fail(BBT_ERR_UNSUPPORTED,
                         "bbt_eval_batch_noalloc: the binomial score tables for these reads "
                         "are not built; run bbt_eval_batch once with the same parameters");
-        bbt_status s = ensure_bscore(fs, frames);
+        bbt_status s = ensure_bscore(fs, table);
         if (s != BBT_OK)
             return s;
     }
@@ -1700,7 +1758,8 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         HIP_OK(hipMalloc(&W.dctl, 128 * sizeof(unsigned long long)));
     if (p->dust) {
         if (p->dust_w < 3 || p->dust_w > uint32_t(kDustMaxW) || p->dust_t > 1000000)
-            return fail(BBT_ERR_UNSUPPORTED, "dust window (-W) must be 3..64");
+            return fail(BBT_ERR_UNSUPPORTED, "dust window (-W) must be 3.." +
+                                                 std::to_string(kDustMaxW));
         /* one bit per base position of the batch (4 per code byte when packed) */
         const uint64_t positions =
             in->format == BBT_READS_2BIT ? 4 * in->seq_bytes : in->seq_bytes;
@@ -1715,8 +1774,18 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             HIP_OK(hipMalloc(&W.dmask, words * sizeof(uint32_t)));
             W.dmask_words = words;
         }
-        if (!W.dscratch)
-            HIP_OK(hipMalloc(&W.dscratch, size_t(kDustThreads) * sizeof(DustScratch)));
+        if (W.dscratch_w < p->dust_w) {
+            /* the scratch grows with the window (W^2 for the interval list) */
+            const DustLayout lay = dust_layout(int(p->dust_w));
+            if (W.dscratch) {
+                HIP_OK(hipStreamSynchronize(st));
+                HIP_OK(hipFree(W.dscratch));
+            }
+            W.dscratch = nullptr;
+            W.dscratch_w = 0;
+            HIP_OK(hipMalloc(&W.dscratch, uint64_t(lay.threads) * lay.stride));
+            W.dscratch_w = p->dust_w;
+        }
     }
     if (p->path == BBT_PATH_BATCHED && W.lists_cap < in->nreads) {
         /* the lists may still be read by this workspace's previous batch */
@@ -1760,6 +1829,11 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.nf = nf;
     a.refill_min = tune.refill_min ? tune.refill_min : kDefaultRefillMin;
     a.probe_ahead = tune.probe_ahead ? 1u : 0u;
+    if (fix_scores) {
+        P.fix_scores = true;
+        a.fix_list = W.dfix;
+        a.fix_count = W.dctl + 102;
+    }
     a.wp.threshold = p->threshold;
     a.wp.streak = p->streak;
     a.wp.min_len = uint32_t(std::round(p->threshold));
@@ -1774,6 +1848,9 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
 
 /* The kernels of one batch, in stream order; capturable (no memset nodes:
  * counters are cleared by a fill kernel). */
+static bbt_status enqueue_classify(const bbt_eval_params *p, const bbt_eval_out *out,
+                                   const Prepared &P);
+
 static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
                           const bbt_eval_params *p, const bbt_eval_out *out, Prepared &P,
                           bool timed) {
@@ -1785,6 +1862,8 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     /* work queues of the (up to two) walks of this call, and for the batched
      * path the length buckets: long reads walked by whole waves */
     k_zero_u64<<<1, 64, 0, st>>>(W.dctl, 48);
+    if (P.fix_scores)
+        k_zero_u64<<<1, 64, 0, st>>>(W.dctl + 102, 1);
     if (P.code_bytes) {
         const uint64_t nw = P.code_bytes / 8;
         k_unpack_2bit<<<uint32_t(std::min<uint64_t>((nw + kBlock - 1) / kBlock, 65536)), kBlock,
@@ -1799,8 +1878,10 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         k_fill_u8<<<1024, kBlock, 0, st>>>(reinterpret_cast<uint8_t *>(W.dmask),
                                             W.dmask_words * sizeof(uint32_t), 0);
         k_zero_u64<<<1, 64, 0, st>>>(W.dctl + 100, 2);
-        k_dust<<<kDustThreads / 64, 64, 0, st>>>(
-            a.seq, in->off, in->nreads, int(p->dust_t), int(p->dust_w), W.dscratch, W.dmask,
+        /* the layout the scratch was allocated with (its window >= this one's) */
+        const DustLayout lay = dust_layout(int(W.dscratch_w));
+        k_dust<<<lay.threads / 64, 64, 0, st>>>(
+            a.seq, in->off, in->nreads, int(p->dust_t), int(p->dust_w), W.dscratch, lay, W.dmask,
             W.dctl + 100, out->counters ? reinterpret_cast<unsigned long long *>(out->counters) + 3
                                         : W.dctl + 101);
         HIP_OK(hipGetLastError());
@@ -1847,6 +1928,16 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         return s;
     if (timed)
         HIP_OK(hipEventRecord(W.ev1, st));
+    if (P.fix_scores)
+        return BBT_OK; /* eval_impl finishes the scores, then classifies */
+    return enqueue_classify(p, out, P);
+}
+
+/* hit rows (and best hits from the scores) -> classes and summary */
+static bbt_status enqueue_classify(const bbt_eval_params *p, const bbt_eval_out *out,
+                                   const Prepared &P) {
+    const uint32_t nf = P.nf;
+    hipStream_t st = P.L.st;
     if (out->cls || out->summary || p->mode == BBT_BESTHIT) {
         const uint64_t blocks = (P.nrec + kBlock - 1) / kBlock;
         const size_t shm = out->summary ? (2 * size_t(nf) + 2) * sizeof(unsigned int) : 0;
@@ -1883,6 +1974,50 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     return k;
 }
 
+/* BINOMIAL evalScore of the listed items on the host: -10 log10 P(X > count),
+ * X ~ Bin(frames, FPR of the item's filter), the function that fills the
+ * device table (SeqEval.h:450-491, 558), then written into the rows. */
+static bbt_status finish_scores(bbt_filterset *fs, bbt_workspace &W, const bbt_eval_out *out,
+                                const Prepared &P, hipStream_t st) {
+    unsigned long long n = 0;
+    HIP_OK(hipMemcpyAsync(&n, W.dctl + 102, sizeof(n), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (n == 0)
+        return BBT_OK;
+    if (n > W.fix_cap)
+        return fail(BBT_ERR_HIP, "binomial score list overflow");
+    std::vector<ScoreFix> fx(n);
+    HIP_OK(hipMemcpyAsync(fx.data(), W.dfix, n * sizeof(ScoreFix), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const uint32_t nf = P.nf;
+    const unsigned T = unsigned(std::max<uint64_t>(
+        1, std::min<uint64_t>({16, std::max(1u, std::thread::hardware_concurrency()), n / 256})));
+    const auto work = [&](unsigned t) {
+        for (uint64_t i = t; i < n; i += T) {
+            const double fpr = fs->info[fx[i].idx % nf].fpr;
+            const double v =
+                std::log10(binom_sf(int64_t(fx[i].count), fx[i].frames, fpr)) * -10;
+            uint64_t bits;
+            std::memcpy(&bits, &v, 8);
+            fx[i].frames = uint32_t(bits);
+            fx[i].count = uint32_t(bits >> 32);
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < T; ++t)
+        th.emplace_back(work, t);
+    work(0);
+    for (std::thread &x : th)
+        x.join();
+    HIP_OK(hipMemcpyAsync(W.dfix, fx.data(), n * sizeof(ScoreFix), hipMemcpyHostToDevice, st));
+    k_fix_scores<<<uint32_t(std::min<uint64_t>((n + kBlock - 1) / kBlock, 4096)), kBlock, 0,
+                   st>>>(W.dfix, n, out->scores);
+    HIP_OK(hipGetLastError());
+    /* the host vector is read by the copy: wait before it goes away */
+    HIP_OK(hipStreamSynchronize(st));
+    return BBT_OK;
+}
+
 static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *in,
                             const bbt_eval_params *p, bbt_eval_out *out, hipStream_t st,
                             bool noalloc = false) {
@@ -1903,6 +2038,13 @@ static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads
         s = enqueue(fs, W, in, p, out, P, timed);
         if (s != BBT_OK)
             return s;
+        if (P.fix_scores) {
+            s = finish_scores(fs, W, out, P, st);
+            if (s == BBT_OK)
+                s = enqueue_classify(p, out, P);
+            if (s != BBT_OK)
+                return s;
+        }
     } else {
         const std::string key = graph_key(fs, in, p, out, P, st);
         if (!W.graph_exec || key != W.graph_key) {

@@ -409,6 +409,11 @@ struct QueryArgs {
     uint32_t *wide_list;
     uint32_t *spill;
     uint32_t spill_stride;
+    /* CAP = 0: each workgroup's frame records in device memory, grec_cap
+     * (a power of two) of them per workgroup */
+    uint64_t *grec;
+    uint32_t grec_cap;
+    uint32_t grec_blocks; /* workgroups the area has room for */
 };
 
 constexpr uint32_t kHitWords = 7; /* id count nonSat total totalNonSat solid nonSatFrame */
@@ -452,7 +457,10 @@ __device__ __forceinline__ bool ranks_before(const uint32_t *x, const uint32_t *
  * the record's lookups). */
 template <uint32_t CAP, uint32_t SMAX, bool DIRECT, bool WIDE>
 __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
-    __shared__ uint64_t rec[DIRECT ? 1 : CAP];
+    __shared__ uint64_t rec_lds[DIRECT || CAP == 0 ? 1 : CAP];
+    /* the record store: LDS, or with CAP = 0 this workgroup's device area */
+    uint64_t *const rec = CAP ? rec_lds : a.grec + uint64_t(blockIdx.x) * a.grec_cap;
+    const uint32_t cap = CAP ? CAP : a.grec_cap;
     __shared__ uint32_t cand[kMaxCand * kHitWords];
     __shared__ uint32_t ncand_s;
     /* spaced seeds: for window offset j and base code c, the contributions
@@ -738,7 +746,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 const uint32_t base = nrec + pre - myrec;
                 if (eval)
                     each_id([&](uint32_t x, uint64_t v) {
-                        if (base + x < CAP)
+                        if (base + x < cap)
                             rec[base + x] = v;
                     });
                 nrec += total;
@@ -784,7 +792,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                 }
             }
             nrec = 0;
-        } else if (nrec > CAP) { /* cannot happen when the host sized CAP by frames x S */
+        } else if (nrec > cap) { /* cannot happen when the host sized cap by frames x S */
             over = true;
             nrec = 0;
         }
@@ -977,8 +985,10 @@ void query_pair(const QueryArgs &a, int cu, size_t shm, hipStream_t st) {
             &per_cu, k_mibf_query<CAP, SMAX, DIRECT, false>, 64, shm) != hipSuccess ||
         per_cu < 1)
         per_cu = 16;
-    const uint32_t blocks =
-        uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(a.nrec, uint64_t(cu) * per_cu)));
+    uint64_t nb = std::max<uint64_t>(1, std::min<uint64_t>(a.nrec, uint64_t(cu) * per_cu));
+    if (CAP == 0)
+        nb = std::min<uint64_t>(nb, a.grec_blocks); /* the device record area's room */
+    const uint32_t blocks = uint32_t(nb);
     k_mibf_query<CAP, SMAX, DIRECT, false><<<blocks, 64, shm, st>>>(a);
     if (a.spill_stride)
         k_mibf_query<CAP, SMAX, DIRECT, true><<<kWideBlocks, 64, shm, st>>>(a);
@@ -1020,6 +1030,9 @@ struct bbt_mibf {
     /* BBT_READS_2BIT queries: the batch unpacked to ASCII */
     uint8_t *dascii = nullptr;
     uint64_t ascii_cap = 0;
+    /* frame records of records too long for LDS (more than 1023 IDs) */
+    uint64_t *dgrec = nullptr;
+    uint32_t grec_cap = 0, grec_blocks = 0;
     std::mutex mu;
 };
 
@@ -1150,6 +1163,7 @@ bbt_status bbt_mibf_free(bbt_mibf *mf) {
     (void)hipFree(mf->dwide_list);
     (void)hipFree(mf->dspill);
     (void)hipFree(mf->dascii);
+    (void)hipFree(mf->dgrec);
     if (mf->last)
         (void)hipEventDestroy(mf->last);
     delete mf;
@@ -1495,10 +1509,9 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     const uint32_t frames = in->max_len >= k ? in->max_len - k + 1 : 0;
     const uint64_t recs = uint64_t(frames) * mf->d.S * (paired ? 2 : 1);
     const bool direct = mf->d.nids + 1 <= kDirectIds;
-    if (!direct && recs > 8192)
+    if (!direct && recs > (uint64_t(1) << 26))
         return mfail(BBT_ERR_UNSUPPORTED,
-                     "bbt_mibf_query_batch: with more than 1023 IDs, frames x hashes of the "
-                     "longest record may not exceed 8192 (reads up to ~2.7 kbp with 3 seeds)");
+                     "bbt_mibf_query_batch: frames x hashes of the longest record above 2^26");
     if (uint64_t(frames + 1) * (mf->d.nids + 1) * 4 > (uint64_t(1) << 30))
         return mfail(BBT_ERR_UNSUPPORTED,
                      "bbt_mibf_query_batch: minimum-count table above 1 GiB (reads too long "
@@ -1593,6 +1606,30 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     }
     a.wide_list = mf->dwide_list;
     a.spill = mf->dspill;
+    if (!direct && recs > 8192) {
+        /* records too long for the LDS store: a device area per workgroup,
+         * for as many workgroups as fit in about 1 GiB (at least the wide
+         * pass's) */
+        uint32_t cap = 1;
+        while (cap < recs)
+            cap <<= 1;
+        const uint64_t per = uint64_t(cap) * 8;
+        const uint64_t nblk = std::max<uint64_t>(kWideBlocks, (uint64_t(1) << 30) / per);
+        if (mf->grec_cap != cap || mf->grec_blocks != nblk) {
+            if (wait_last() != BBT_OK)
+                return BBT_ERR_HIP;
+            (void)hipFree(mf->dgrec);
+            mf->dgrec = nullptr;
+            mf->grec_cap = 0;
+            mf->grec_blocks = 0;
+            MHIP(hipMalloc(&mf->dgrec, nblk * per));
+            mf->grec_cap = cap;
+            mf->grec_blocks = uint32_t(nblk);
+        }
+        a.grec = mf->dgrec;
+        a.grec_cap = cap;
+        a.grec_blocks = mf->grec_blocks;
+    }
     if (in->format == BBT_READS_2BIT) {
         const uint64_t need = 4 * in->seq_bytes + 8;
         if (mf->ascii_cap < need) {
@@ -1628,9 +1665,12 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     else if (recs <= 4096)
         few ? query_pair<4096, 4, false>(a, blocks, 0, st)
             : query_pair<4096, 8, false>(a, blocks, 0, st);
-    else
+    else if (recs <= 8192)
         few ? query_pair<8192, 4, false>(a, blocks, 0, st)
             : query_pair<8192, 8, false>(a, blocks, 0, st);
+    else
+        few ? query_pair<0, 4, false>(a, blocks, 0, st)
+            : query_pair<0, 8, false>(a, blocks, 0, st);
     MHIP(hipGetLastError());
     MHIP(hipEventRecord(mf->last, st));
     mf->has_last = true;

@@ -130,7 +130,7 @@ typedef struct {
     uint32_t dust;      /* -D: windows starting in an sdust low-complexity interval
                            are misses without a contains() call (SeqEval.h:496) */
     uint32_t dust_t;    /* -T, sdust threshold [20] */
-    uint32_t dust_w;    /* -W, sdust window [64], at most 64 here */
+    uint32_t dust_w;    /* -W, sdust window [64], 3..4096 */
     uint32_t reserved;
 } bbt_eval_params;
 

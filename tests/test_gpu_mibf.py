@@ -496,3 +496,59 @@ def test_2bit_reads_query_like_ascii(dev):
         gp = classify_both(D, B, mf, dp, p)
         assert gp[0] == ga[0] == want and gp[1] == ga[1] == wsat
         assert np.array_equal(gp[2], ga[2])
+
+
+def test_long_records_with_many_ids(dev):
+    """More than 1023 IDs (the sorted frame-record path) and records of more
+    than 8192 frames x seeds: the records go to a device-memory store per
+    workgroup; results equal the oracle's, single and paired."""
+    D, B = dev
+    rng = random.Random(73)
+    seeds = seeds_for(rng)
+    recs = make_refs(rng, n=4, L=20000)
+    recs += [(5 + i, "".join(rng.choice("ACGT") for _ in range(120))) for i in range(1100)]
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    assert mf.info().nids > 1023
+    reads = []
+    for L in [4000, 9000, 150, 3500, 12000, 150, 6000, 300]:
+        _, ref = recs[rng.randrange(0, 4)]
+        a = rng.randrange(0, len(ref) - L)
+        r = list(ref[a:a + L])
+        for i in range(len(r)):
+            if rng.random() < 0.03:
+                r[i] = rng.choice("ACGTN")
+        reads.append("".join(r))
+    dr = device_reads(D, reads)
+    print("[gpu] mibf long records, 1104 IDs", flush=True)
+    for paired in [False, True]:
+        want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5,
+                                    max_hits=8, paired=paired)
+        got, gsat, gctr, _ = classify_both(
+            D, B, mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 8, paired=paired))
+        assert gsat == wsat and got == want
+        assert gctr[0] == wctr[0] and gctr[1] == wctr[1]
+    assert all(row for row in want)
+
+
+def test_long_records_many_candidates(dev):
+    """Long reads (past the LDS record store) from a segment that 1100
+    references share: more than 64 candidates, so the wide pass runs over the
+    device-memory store too; equal to the oracle."""
+    D, B = dev
+    rng = random.Random(37)
+    seeds = seeds_for(rng)
+    base = "".join(rng.choice("ACGT") for _ in range(4200))
+    recs = [(i + 1, base) for i in range(1100)]
+    om = O.Mibf(recs, seeds)
+    mf = D.build_mibf([s for _, s in recs], [i for i, _ in recs],
+                      [str(i) for i, _ in recs], seeds, m=om.m)
+    reads = [base[a:a + 3500] for a in range(0, 600, 150)] + reads_from(rng, recs, 10)
+    dr = device_reads(D, reads)
+    print("[gpu] mibf long records, 1100 shared IDs", flush=True)
+    want, wsat, wctr = om.query(reads, score=1e-10, allowed_miss=2, multi=0.5, max_hits=16)
+    got, gsat, gctr, nh = classify_both(D, B, mf, dr, B.mibf_params(1e-10, 0.5, 2, 0, 16))
+    assert gsat == wsat and got == want
+    assert gctr[0] == wctr[0] and gctr[1] == wctr[1] and gctr[2] == wctr[2]
+    assert int(gctr[3]) > 0 and not (nh == 0xFFFFFFFF).any()

@@ -184,6 +184,13 @@ def test_uneven_lengths(dev):
     check_batch(D, B, fs, ofs, buf, off, order=D.length_order(np.diff(off)))
     for scorer, thr in [(O.HARMONIC, 0.15), (O.LENGTH, 60.0), (O.BINOMIAL, 1e-10)]:
         check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
+    # BINOMIAL evalScore of reads past the device table (1024 k-mers): the
+    # device counts hits, the host computes -10 log10 of the tail
+    assert max(len(x) for x in longs) > 1100
+    for mode in (O.BESTHIT, O.SCORES):
+        check_batch(D, B, fs, ofs, buf, off, mode=mode, scorer=O.BINOMIAL, thr=1e-10)
+        check_batch(D, B, fs, ofs, buf, off, mode=mode, scorer=O.BINOMIAL, thr=1e-10,
+                    paired=True)
 
 
 def test_store_and_load_roundtrip(dev, tmp_path):
@@ -271,7 +278,9 @@ def test_dust(dev, multi):
     if len(planted) % 2:
         planted.append(b"CA" * 40)
     pbuf, poff = O.pack_reads(planted)
-    for dust in [(20, 64), (10, 32)]:
+    # windows above 64 too (the scratch grows as W^2; sdust's work per base
+    # on repetitive sequence grows with it, on the host as on the device)
+    for dust in [(20, 64), (10, 32), (20, 100)]:
         check_batch(D, B, fs, ofs, pbuf, poff, dust=dust)
         check_batch(D, B, fs, ofs, pbuf, poff, dust=dust, wave_min=26)
     check_batch(D, B, fs, ofs, pbuf, poff, dust=(20, 64), paired=True)
