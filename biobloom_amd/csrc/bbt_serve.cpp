@@ -131,6 +131,7 @@ struct Request {
     uint32_t max_len = 0;
     std::vector<uint8_t> hits;
     std::vector<uint32_t> cls;
+    std::vector<double> scores; /* BESTHIT / SCORES: nreads * nf evalScore values */
     Clock::time_point t_submit, t_admit, t_done;
     uint64_t wave = 0;
     int state = PENDING;
@@ -139,9 +140,10 @@ struct Request {
 
 /* byte layout of one wave's device block */
 struct WaveLayout {
-    uint64_t seq = 0, off = 0, hits = 0, cls = 0, total = 0;
+    uint64_t seq = 0, off = 0, hits = 0, cls = 0, scores = 0, total = 0;
     uint64_t seq_bytes = 0;
-    static WaveLayout of(uint64_t nreads, uint64_t bases, uint64_t nrec, uint32_t nf) {
+    static WaveLayout of(uint64_t nreads, uint64_t bases, uint64_t nrec, uint32_t nf,
+                         bool with_scores) {
         WaveLayout L;
         L.seq_bytes = align_up(bases, 8) + 8;
         uint64_t o = 0;
@@ -153,6 +155,9 @@ struct WaveLayout {
         o = align_up(o + std::max<uint64_t>(1, nreads * nf), kAlign);
         L.cls = o;
         o = align_up(o + std::max<uint64_t>(1, nrec) * 4, kAlign);
+        L.scores = o;
+        if (with_scores)
+            o = align_up(o + std::max<uint64_t>(1, nreads * nf) * 8, kAlign);
         L.total = o;
         return L;
     }
@@ -165,6 +170,7 @@ struct Slot {
     uint64_t *h_off = nullptr;
     uint8_t *h_hits = nullptr;
     uint32_t *h_cls = nullptr;
+    double *h_scores = nullptr; /* BESTHIT / SCORES */
     bool busy = false;
     std::vector<std::shared_ptr<Request>> reqs;
     uint64_t block_off = 0, block_len = 0;
@@ -254,6 +260,7 @@ struct bbt_server {
     bbt_eval_params params{};
     bbt_server_config cfg{};
     uint32_t nf = 0;
+    bool scores = false; /* BESTHIT / SCORES: evalScore values are returned too */
     char *darena = nullptr;
     Arena arena;
     char *harena = nullptr; /* pinned: bases of queued and in-flight requests */
@@ -320,7 +327,7 @@ bbt_status bbt_server::launch(Slot &s) {
         max_len = std::max(max_len, r->max_len);
     }
     const uint64_t nrec = paired ? nreads / 2 : nreads;
-    const WaveLayout L = WaveLayout::of(nreads, bases, nrec, nf);
+    const WaveLayout L = WaveLayout::of(nreads, bases, nrec, nf, scores);
     char *blk = darena + s.block_off;
     const double packed = std::chrono::duration<double, std::milli>(Clock::now() - tp).count();
     {
@@ -349,6 +356,8 @@ bbt_status bbt_server::launch(Slot &s) {
     std::memset(&out, 0, sizeof(out));
     out.hits = reinterpret_cast<uint8_t *>(blk + L.hits);
     out.cls = reinterpret_cast<uint32_t *>(blk + L.cls);
+    if (scores)
+        out.scores = reinterpret_cast<double *>(blk + L.scores);
     out.counters = dctr; /* every launch adds to the server's totals */
     bbt_eval_params p = params;
     p.path = cfg.mode == BBT_SERVE_PLAIN ? BBT_PATH_PLAIN : BBT_PATH_BATCHED;
@@ -360,6 +369,9 @@ bbt_status bbt_server::launch(Slot &s) {
                         hipMemcpyDeviceToHost, s.st));
     SHIP(hipMemcpyAsync(s.h_cls, out.cls, std::max<uint64_t>(1, nrec) * 4,
                         hipMemcpyDeviceToHost, s.st));
+    if (scores)
+        SHIP(hipMemcpyAsync(s.h_scores, out.scores, std::max<uint64_t>(1, nreads * nf) * 8,
+                            hipMemcpyDeviceToHost, s.st));
     SHIP(hipEventRecord(s.done, s.st));
     s.nreads = nreads;
     s.nrec = nrec;
@@ -399,7 +411,7 @@ void bbt_server::dispatch_loop() {
                 break;
             const WaveLayout L = WaveLayout::of(nreads + r.nreads, bases + r.bases,
                                                 paired ? (nreads + r.nreads) / 2 : nreads + r.nreads,
-                                                nf);
+                                                nf, scores);
             uint64_t off = 0;
             if (!arena.alloc(L.total, off))
                 break;
@@ -482,6 +494,9 @@ void bbt_server::complete_loop() {
                 for (auto &r : s.reqs) {
                     r->hits.assign(s.h_hits + rd * nf, s.h_hits + (rd + r->nreads) * nf);
                     r->cls.assign(s.h_cls + rc, s.h_cls + rc + r->nrec);
+                    if (scores)
+                        r->scores.assign(s.h_scores + rd * nf,
+                                         s.h_scores + (rd + r->nreads) * nf);
                     rd += r->nreads;
                     rc += r->nrec;
                     r->t_done = now;
@@ -516,10 +531,6 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
     if (cfg->slots == 0 || cfg->slots > 8 || cfg->max_wave_reads == 0 ||
         cfg->max_wave_bases == 0 || cfg->device_bytes == 0 || cfg->mode > BBT_SERVE_PLAIN)
         return sfail(BBT_ERR_ARG, "bbt_server_create: bad configuration");
-    if (p->mode == BBT_BESTHIT || p->mode == BBT_SCORES)
-        return sfail(BBT_ERR_UNSUPPORTED,
-                     "bbt_server_create: serving returns hits and classes (standard and "
-                     "ordered modes)");
     *out = nullptr;
     std::unique_ptr<bbt_server> S(new bbt_server());
     S->fs = fs;
@@ -528,10 +539,11 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
     if (S->cfg.mode == BBT_SERVE_PLAIN)
         S->cfg.slots = 1;
     S->nf = bbt_filterset_size(fs);
+    S->scores = p->mode == BBT_BESTHIT || p->mode == BBT_SCORES;
     S->t0 = Clock::now();
     SHIP(hipGetDevice(&S->device));
     const uint64_t rd = S->cfg.max_wave_reads, bs = S->cfg.max_wave_bases;
-    const WaveLayout Lmax = WaveLayout::of(rd, bs, rd, S->nf);
+    const WaveLayout Lmax = WaveLayout::of(rd, bs, rd, S->nf, S->scores);
     if (Lmax.total > cfg->device_bytes)
         return sfail(BBT_ERR_ARG, "bbt_server_create: device_bytes below one full wave (" +
                                       std::to_string(Lmax.total) + " bytes)");
@@ -557,6 +569,9 @@ bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_off), (rd + 1) * 8));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_hits), std::max<uint64_t>(1, rd * S->nf)));
         SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_cls), std::max<uint64_t>(1, rd) * 4));
+        if (S->scores)
+            SHIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_scores),
+                               std::max<uint64_t>(1, rd * S->nf) * 8));
     }
     const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
     S->copier.reset(new CopyPool(std::min(3u, hc - 1)));
@@ -651,6 +666,24 @@ bbt_status bbt_server_drain(bbt_server *S, double timeout_s) {
         return sfail(BBT_ERR_HIP, "bbt_server_drain: timed out");
     if (!S->error.empty())
         return sfail(BBT_ERR_HIP, "server error: " + S->error);
+    return BBT_OK;
+}
+
+bbt_status bbt_server_scores(bbt_server *S, uint64_t req_id, double *scores) {
+    if (!S || !scores)
+        return sfail(BBT_ERR_ARG, "bbt_server_scores: null argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->reqs.find(req_id);
+    if (it == S->reqs.end())
+        return sfail(BBT_ERR_ARG, "bbt_server_scores: unknown request");
+    const Request &r = *it->second;
+    if (r.state != DONE)
+        return sfail(r.state == FAILED ? BBT_ERR_HIP : BBT_ERR_ARG,
+                     r.state == FAILED ? "request failed: " + r.error
+                                       : std::string("bbt_server_scores: request not done"));
+    if (!S->scores)
+        return sfail(BBT_ERR_ARG, "bbt_server_scores: the server's mode has no scores");
+    std::memcpy(scores, r.scores.data(), r.scores.size() * 8);
     return BBT_OK;
 }
 
@@ -755,6 +788,8 @@ bbt_status bbt_server_destroy(bbt_server *S) {
         (void)hipHostFree(s.h_off);
         (void)hipHostFree(s.h_hits);
         (void)hipHostFree(s.h_cls);
+        if (s.h_scores)
+            (void)hipHostFree(s.h_scores);
     }
     (void)hipFree(S->darena);
     (void)hipFree(S->dctr);

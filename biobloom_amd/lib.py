@@ -141,6 +141,7 @@ _SIGS = {
     "bbt_server_submit": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                       C.POINTER(C.c_uint64)]),
     "bbt_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double]),
+    "bbt_server_scores": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "bbt_server_drain": (C.c_int32, [C.c_void_p, C.c_double]),
     "bbt_server_result": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                       C.POINTER(RequestStats)]),
@@ -541,6 +542,13 @@ class Server:
         return (hits[: nreads * self.nf].reshape(nreads, self.nf), cls[:nrec],
                 {"wave": st.wave, "submit_us": st.submit_us, "admit_us": st.admit_us,
                  "done_us": st.done_us})
+
+    def scores(self, rid, nreads):
+        """evalScore values (nreads x nf) of a request to a BESTHIT / SCORES server."""
+        import numpy as np
+        out = np.zeros(max(1, nreads * self.nf), np.float64)
+        check(load().bbt_server_scores(self._h, rid, out.ctypes.data))
+        return out[: nreads * self.nf].reshape(nreads, self.nf)
 
     def release(self, rid):
         check(load().bbt_server_release(self._h, rid))

@@ -341,8 +341,9 @@ typedef struct {
     uint64_t lookups_issued; /* bit lookups the kernels made (>= lookups) */
 } bbt_server_stats;
 
-/* p: the classification parameters of every request (BBT_STD or BBT_ORDERED;
- * path and flags are set by the server).  fs must outlive the server. */
+/* p: the classification parameters of every request, any mode (path and
+ * flags are set by the server; BBT_BESTHIT and BBT_SCORES also return each
+ * read's evalScore values, bbt_server_scores).  fs must outlive the server. */
 bbt_status bbt_server_create(bbt_filterset *fs, const bbt_eval_params *p,
                              const bbt_server_config *cfg, bbt_server **out);
 /* Copies the request (host seq, nreads + 1 offsets into it) and queues it. */
@@ -355,6 +356,9 @@ bbt_status bbt_server_drain(bbt_server *s, double timeout_s);
 /* hits: nreads * nf bytes, cls: per record; either may be NULL. */
 bbt_status bbt_server_result(bbt_server *s, uint64_t req_id, uint8_t *hits, uint32_t *cls,
                              bbt_request_stats *stats);
+/* BBT_BESTHIT / BBT_SCORES servers: the request's nreads * nf evalScore
+ * values (the scores output of bbt_eval_batch). */
+bbt_status bbt_server_scores(bbt_server *s, uint64_t req_id, double *scores);
 bbt_status bbt_server_release(bbt_server *s, uint64_t req_id);
 bbt_status bbt_server_get_stats(bbt_server *s, bbt_server_stats *out);
 /* Stops admitting, finishes the launches in flight, frees everything. */

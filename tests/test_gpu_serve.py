@@ -210,3 +210,44 @@ def test_large_request_copy(env):
         wh, wc = alone(D, B, fs, buf, off, params)
         assert np.array_equal(h, wh) and np.array_equal(c, wc)
     assert int(got[0][1].size) == 60_000 and (got[0][1][:30_000] == 1).mean() > 0.9
+
+
+@pytest.mark.parametrize("mode,scorer,thr", [("besthit", 0, 0.15), ("scores", 2, 0.15),
+                                             ("besthit", 3, 1e-10)])
+def test_score_modes_served(env, mode, scorer, thr):
+    """BESTHIT and SCORES requests through the server: hits, classes and the
+    evalScore values of every request equal the request classified alone and
+    the oracle's (BioBloomClassifier.cpp:1194-1237), merged into shared
+    launches."""
+    torch, D, B, synth, fs, genomes = env
+    m = B.BESTHIT if mode == "besthit" else B.SCORES
+    params = B.params(mode=m, scorer=scorer, threshold=thr)
+    reqs = make_requests(synth, genomes, 10, 71 + scorer)
+    ofs = []
+    for i in range(len(fs)):
+        inf = fs.info(i)
+        f = O.Filter(inf.m, inf.h, inf.k)
+        fs.download(i, f.bits)
+        ofs.append(f)
+    srv = B.Server(fs, params, slots=2, device_bytes=256 << 20, max_wave_reads=1 << 14,
+                   max_wave_bases=16 << 20)
+    ids = [srv.submit(buf, off) for buf, off in reqs]
+    srv.drain(120.0)
+    waves = set()
+    for rid, (buf, off) in zip(ids, reqs):
+        n = len(off) - 1
+        h, c, st = srv.result(rid, n)
+        sc = srv.scores(rid, n)
+        srv.release(rid)
+        waves.add(st["wave"])
+        print(f"[gpu] served {mode} scorer={scorer} request of {n} reads", flush=True)
+        dr = D.DeviceReads(buf, off)
+        res = D.classify(fs, dr, params)
+        assert np.array_equal(h, res["hits"]) and np.array_equal(c, res["cls"])
+        assert np.array_equal(sc.view(np.uint64), res["scores"].view(np.uint64))
+        oh, osc, ocls, _ = O.classify(ofs, buf, off, False, False, m,
+                                      O.params(scorer, thr, 3), want_scores=True)
+        assert np.array_equal(h, oh) and np.array_equal(c, ocls)
+        assert np.array_equal(sc.view(np.uint64), osc.view(np.uint64))
+    srv.close()
+    assert len(waves) < len(reqs)  # requests shared launches
