@@ -937,7 +937,7 @@ def main():
     ap.add_argument("--no-gather", action="store_true",
                     help="skip the random-gather reference measurements")
     ap.add_argument("--gather-sweep", action="store_true",
-                    help="only measure the random-line rate over buffers of 1 MiB to 8 GiB")
+                    help="only measure the random-line rate over buffers of 1 MiB to 32 GiB")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
@@ -978,7 +978,7 @@ def main():
     B.load()
     if args.gather_sweep:
         sweep = []
-        for e in range(20, 34):
+        for e in range(20, 36):
             lps, ms = B.measure_gather(1 << e, 64, 0)
             sweep.append({"bytes": 1 << e, "lines_per_s": lps, "ms": ms})
             log(f"random gather over {(1 << e) >> 20} MiB: {lps / 1e9:.1f} G lines/s")

@@ -446,17 +446,122 @@ __device__ __forceinline__ bool ranks_before(const uint32_t *x, const uint32_t *
     return x[0] < y[0];
 }
 
-/* One wavefront per record.  Each mate is walked 64 windows per step: the
- * window hashes come from prefix-XOR scans, every lane resolves its frame's
- * S slots (bit + rank from one line, then the ID), and the distinct IDs of
- * each frame are appended to an LDS store that is sorted by ID and reduced
- * per ID at the end of the record.  A record with more than kMaxCand
- * candidates is listed for the WIDE pass, which walks the listed records
- * again with room for every ID as a candidate and ranks them by selection;
- * the WIDE pass adds nothing to the lookup counters (the first pass counted
- * the record's lookups). */
-template <uint32_t CAP, uint32_t SMAX, bool DIRECT, bool WIDE>
+/* Bases 64c .. 64c + 63 of a sequence, one per lane, in prefix form: H and G
+ * (XOR prefixes of the position-rotated seeds through this base, as in
+ * bbt_device.h), N (the last position at or before this base that is not
+ * A/C/G/T, or past the sequence; -1 if none) and W (2-bit codes of this base
+ * and the 31 before it, this base in bits 0-1). */
+struct BaseChunk {
+    uint64_t H, G, W;
+    int N;
+};
+
+/* the same values for the last base scanned, carried into the next chunk */
+struct ScanCarry {
+    uint64_t H, G, W;
+    int N;
+};
+
+__device__ __forceinline__ BaseChunk scan_chunk(unsigned c, uint32_t i, uint32_t len,
+                                                uint32_t lane, bool spaced, ScanCarry &cy) {
+    const bool ok = i < len && base_valid(c);
+    const unsigned code = base_code(c);
+    BaseChunk x;
+    x.H = ok ? pfx_fwd(code, i) : 0;
+    x.G = ok ? pfx_rc(code, i) : 0;
+    x.N = ok ? -1 : int(i);
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const uint64_t th = __shfl_up(x.H, unsigned(dd));
+        const uint64_t tg = __shfl_up(x.G, unsigned(dd));
+        const int tl = __shfl_up(x.N, unsigned(dd));
+        if (int(lane) >= dd) {
+            x.H ^= th;
+            x.G ^= tg;
+            x.N = x.N > tl ? x.N : tl;
+        }
+    }
+    x.H ^= cy.H;
+    x.G ^= cy.G;
+    x.N = x.N > cy.N ? x.N : cy.N;
+    cy.H = __shfl(x.H, 63);
+    cy.G = __shfl(x.G, 63);
+    cy.N = __shfl(x.N, 63);
+    x.W = 0;
+    if (spaced) {
+        /* doubling steps over the lanes, then the previous chunk's bases
+         * above those of this chunk */
+        uint64_t w = code;
+#pragma unroll
+        for (uint32_t dd = 1; dd < 32; dd <<= 1) {
+            const uint64_t t = __shfl_up(w, dd);
+            w |= (lane >= dd ? t : 0) << (2 * dd);
+        }
+        if (lane < 31)
+            w |= cy.W << (2 * (lane + 1));
+        cy.W = __shfl(w, 63);
+        x.W = w;
+    }
+    return x;
+}
+
+/* Frame p = f0 + lane (the window [p, p + k - 1], k <= 64) from the base
+ * chunk X starting at f0, the next chunk Y and the prefixes (hx, gx) through
+ * base f0 - 1: the full window values, the packed codes of its bases (W at
+ * its last base) and whether it lies inside the sequence's nfr frames
+ * without an N. */
+struct Frame {
+    uint64_t fwd, rc, W;
+    bool valid;
+};
+
+__device__ __forceinline__ Frame frame_at(const BaseChunk &X, const BaseChunk &Y, uint64_t hx,
+                                          uint64_t gx, uint32_t f0, uint32_t lane, uint32_t k,
+                                          uint32_t nfr, bool spaced) {
+    uint64_t Hp = __shfl_up(X.H, 1u), Gp = __shfl_up(X.G, 1u);
+    if (lane == 0) {
+        Hp = hx;
+        Gp = gx;
+    }
+    const uint32_t t = lane + k - 1;
+    const int src = int(t & 63);
+    const bool inX = t < 64;
+    const uint64_t hX = __shfl(X.H, src), hY = __shfl(Y.H, src);
+    const uint64_t gX = __shfl(X.G, src), gY = __shfl(Y.G, src);
+    const int nX = __shfl(X.N, src), nY = __shfl(Y.N, src);
+    const uint32_t p = f0 + lane;
+    Frame fr;
+    fr.valid = p < nfr && (inX ? nX : nY) < int(p);
+    fr.fwd = pfx_win_fwd(inX ? hX : hY, Hp, p + k - 1);
+    fr.rc = pfx_win_rc(inX ? gX : gY, Gp, p);
+    fr.W = 0;
+    if (spaced) {
+        const uint64_t wX = __shfl(X.W, src), wY = __shfl(Y.W, src);
+        fr.W = inX ? wX : wY;
+    }
+    return fr;
+}
+
+/* records claimed per queue atomic (first pass): their offsets come in one
+ * load, and each sequence's first bases are loaded while the one before it
+ * waits for its lookups */
+constexpr uint32_t kRecBatch = 8;
+
+/* One wavefront per record.  Each sequence is taken 64 frames per chunk
+ * (lane = frame): the window values from prefix-XOR scans of 64-base chunks,
+ * the S seed hashes per frame, then the slots -- with the direct-mapped
+ * table (TABLE) one load per seed, two chunks' loads in flight together;
+ * with the lines, bit + rank from one line, then the ID.  Each frame's
+ * distinct IDs either add to per-ID totals in LDS (DIRECT: up to kDirectIds
+ * IDs) or are appended to a record store (LDS, or device memory with
+ * CAP = 0) that is sorted by ID and reduced per ID at the end of the record.
+ * A record with more than kMaxCand candidates is listed for the WIDE pass,
+ * which walks the listed records again with room for every ID as a
+ * candidate and ranks them by selection; the WIDE pass adds nothing to the
+ * lookup counters (the first pass counted the record's lookups). */
+template <uint32_t CAP, uint32_t SMAX, bool DIRECT, bool WIDE, bool TABLE>
 __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
+    /* frame chunks whose lookups are in flight together */
+    constexpr uint32_t FG = TABLE ? 2 : 1;
     __shared__ uint64_t rec_lds[DIRECT || CAP == 0 ? 1 : CAP];
     /* the record store: LDS, or with CAP = 0 this workgroup's device area */
     uint64_t *const rec = CAP ? rec_lds : a.grec + uint64_t(blockIdx.x) * a.grec_cap;
@@ -468,11 +573,15 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
      * seed hash XORs back out: {rotl(seed_fwd(c), k-1-j), rotl(seed_rc(c), j)} */
     __shared__ ulonglong2 dct[32 * 4];
     __shared__ uint8_t order_s[kMaxCand]; /* lane 0's candidate ranking */
-    /* DIRECT: per ID, count, nonSat, total, totalNonSat, solid, nonSatFrame */
+    /* DIRECT: per ID, count, nonSat, total, totalNonSat, solid, nonSatFrame;
+     * then the minimum counts of the current frame count, per ID */
     extern __shared__ uint32_t acc[];
     const uint32_t lane = threadIdx.x;
     const uint32_t S = a.d.S, k = a.d.span;
     const uint32_t nid1 = a.d.nids + 1;
+    const bool spaced = !a.d.kmer;
+    uint32_t *const mcs = acc + kDirectWords * nid1;
+    uint32_t mc_row = 0xFFFFFFFFu; /* the frame count whose minimum counts mcs holds */
     unsigned long long bvl = 0, gat = 0;
     if (DIRECT)
         for (uint32_t x = lane; x < kDirectWords * nid1; x += 64)
@@ -483,149 +592,54 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                        : make_ulonglong2(0, 0);
     }
     __syncthreads();
+    const uint32_t per = a.paired ? 2 : 1;
     while (true) {
-        unsigned long long it = 0;
+        const uint32_t rb = WIDE ? 1 : kRecBatch;
+        unsigned long long it0 = 0;
         if (lane == 0)
-            it = atomicAdd(WIDE ? a.wide_queue : a.queue, 1ull);
-        it = __shfl(it, 0);
-        if (it >= (WIDE ? *a.wide : a.nrec))
+            it0 = atomicAdd(WIDE ? a.wide_queue : a.queue, (unsigned long long)rb);
+        it0 = __shfl(it0, 0);
+        const uint64_t lim = WIDE ? *a.wide : a.nrec;
+        if (it0 >= lim)
             break;
-        const uint64_t r = WIDE ? uint64_t(a.wide_list[it]) : it;
-        const uint64_t first = a.paired ? 2 * r : r;
-        const uint32_t nmates = a.paired ? 2 : 1;
-        /* frameCount: len - k + 1; for a pair, the nesting of
-         * MIBFClassifier.hpp:804-805 (mate 2 only when mate 1 is too short) */
-        const uint32_t len1 = uint32_t(a.off[first + 1] - a.off[first]);
-        uint32_t fc = len1 >= k ? len1 - k + 1 : 0;
-        if (a.paired && len1 < k) {
-            const uint32_t len2 = uint32_t(a.off[first + 2] - a.off[first + 1]);
-            fc = len2 >= k ? len2 - k + 1 : 0;
-        }
-        uint32_t satc = 0, nrec = 0;
-        for (uint32_t mate = 0; mate < nmates && fc; ++mate) {
-            const uint64_t b = a.off[first + mate];
-            const uint32_t len = uint32_t(a.off[first + mate + 1] - b);
+        const uint32_t nb = uint32_t(lim - it0 < rb ? lim - it0 : rb);
+        /* the batch's sequences q = 0 .. per * nb - 1 start at offs[q] (lane
+         * q's value) and end at offs[q + 1] */
+        const uint64_t r0 = WIDE ? uint64_t(a.wide_list[it0]) : it0;
+        uint64_t offs = 0;
+        if (lane <= per * nb)
+            offs = a.off[per * r0 + lane];
+        const uint32_t nseq = per * nb;
+        /* the first three 64-base chunks of sequence pf_q, loaded ahead */
+        uint32_t pf_q = 0xFFFFFFFFu;
+        unsigned pf[3] = {0, 0, 0};
+        const auto prefetch = [&](uint32_t q) {
+            const uint64_t b = __shfl(offs, int(q));
+            const uint32_t len = uint32_t(__shfl(offs, int(q + 1)) - b);
             const uint8_t *s = a.seq + b;
-            if (len < k)
-                continue;
-            uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
-            int lastN = -1;
-            /* 2-bit codes of the 32 bases before this chunk, the last one in
-             * bits 0-1 (lane 63's packed window of the previous chunk) */
-            uint64_t wprev = 0;
-            for (uint32_t b0 = 0; b0 < len; b0 += 64) {
-                const uint32_t i = b0 + lane;
-                const unsigned c = i < len ? unsigned(s[i]) : 0u;
-                const bool ok = i < len && base_valid(c);
-                const unsigned code = base_code(c);
-                uint64_t H = ok ? pfx_fwd(code, i) : 0;
-                uint64_t G = ok ? pfx_rc(code, i) : 0;
-                int ln = ok ? -1 : int(i);
-                for (int dd = 1; dd < 64; dd <<= 1) {
-                    const uint64_t th = __shfl_up(H, unsigned(dd));
-                    const uint64_t tg = __shfl_up(G, unsigned(dd));
-                    const int tl = __shfl_up(ln, unsigned(dd));
-                    if (int(lane) >= dd) {
-                        H ^= th;
-                        G ^= tg;
-                        ln = ln > tl ? ln : tl;
-                    }
-                }
-                H ^= Hc;
-                G ^= Gc;
-                ln = ln > lastN ? ln : lastN;
-                const int src = int(lane) - int(k);
-                const uint64_t hb_cur = __shfl(H, src & 63), gb_cur = __shfl(G, src & 63);
-                const uint64_t hb_prev = __shfl(Hp, (src + 64) & 63);
-                const uint64_t gb_prev = __shfl(Gp, (src + 64) & 63);
-                const uint64_t Hb = src >= 0 ? hb_cur : hb_prev;
-                const uint64_t Gb = src >= 0 ? gb_cur : gb_prev;
-                const int64_t pw = int64_t(i) - int64_t(k) + 1;
-                const bool valid = pw >= 0 && i < len && int64_t(ln) < pw;
-                const uint64_t wf = pfx_win_fwd(H, Hb, i);
-                const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
-                Hc = __shfl(H, 63);
-                Gc = __shfl(G, 63);
-                lastN = __shfl(ln, 63);
-                Hp = H;
-                Gp = G;
-                /* this lane's frame: the S hashes (spaced seeds: the window
-                 * values with the don't-care bases XORed out), then one load
-                 * per seed */
-                uint64_t hs[SMAX];
-                if (a.d.kmer) {
-                    const uint64_t h0 = wr < wf ? wr : wf;
 #pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd)
-                        hs[sd] = multi_hash(h0, sd, uint64_t(k) * kMultiSeed);
-                } else {
-                    /* the window's bases as 2-bit codes, base i in bits 0-1 and
-                     * base i - t in bits 2t..2t+1 (t < 32): doubling steps over
-                     * the lanes, then the previous chunk's bases above those of
-                     * this chunk */
-                    uint64_t w = code;
-#pragma unroll
-                    for (uint32_t dd = 1; dd < 32; dd <<= 1) {
-                        const uint64_t t = __shfl_up(w, dd);
-                        w |= (lane >= dd ? t : 0) << (2 * dd);
-                    }
-                    if (lane < 31)
-                        w |= wprev << (2 * (lane + 1));
-                    wprev = __shfl(w, 63);
-                    uint64_t fw[SMAX], rv[SMAX];
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
-                        fw[sd] = wf;
-                        rv[sd] = wr;
-                    }
-                    for (uint32_t dcm = a.d.dc_union; dcm; dcm &= dcm - 1) {
-                        const uint32_t j = uint32_t(__builtin_ctz(dcm));
-                        const uint32_t cj = uint32_t(w >> (2 * (k - 1 - j))) & 3u;
-                        const ulonglong2 t = dct[j * 4 + cj];
-#pragma unroll
-                        for (uint32_t sd = 0; sd < SMAX; ++sd)
-                            if (sd < S && !((a.d.care[sd] >> j) & 1)) {
-                                fw[sd] ^= t.x;
-                                rv[sd] ^= t.y;
-                            }
-                    }
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd)
-                        hs[sd] = rv[sd] < fw[sd] ? rv[sd] : fw[sd];
-                }
-                uint32_t present = 0;
-                uint32_t raw[SMAX];
-                if (a.d.direct) {
-                    /* one load per seed: the ID word, or 0 for a clear bit */
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd)
-                        raw[sd] = (sd < S && valid)
-                                      ? a.d.direct[fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1,
-                                                           a.d.sh2)]
-                                      : 0u;
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
-                        present |= uint32_t(raw[sd] != 0) << sd;
-                        if ((raw[sd] & kIdMask) == kIdMask)
-                            raw[sd] &= kSatBit; /* ID 0 */
-                    }
-                } else {
-                    /* bit and rank from the line, then the ID slot */
-                    uint64_t rk[SMAX];
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd) {
-                        rk[sd] = 0;
-                        if (sd < S && valid) {
-                            const bool bit = bv_rank_line(
-                                a.d.lines, fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
-                                rk[sd]);
-                            present |= uint32_t(bit) << sd;
-                        }
-                    }
-#pragma unroll
-                    for (uint32_t sd = 0; sd < SMAX; ++sd)
-                        raw[sd] = ((present >> sd) & 1) ? a.d.data[rk[sd]] : 0u;
-                }
+            for (uint32_t x = 0; x < 3; ++x) {
+                const uint32_t i = 64 * x + lane;
+                pf[x] = i < len ? unsigned(s[i]) : 0u;
+            }
+            pf_q = q;
+        };
+        for (uint32_t j = 0; j < nb; ++j) {
+            const uint64_t r = r0 + j;
+            /* frameCount: len - k + 1; for a pair, the nesting of
+             * MIBFClassifier.hpp:804-805 (mate 2 only when mate 1 is too short) */
+            const uint32_t q1 = per * j;
+            const uint32_t len1 = uint32_t(__shfl(offs, int(q1 + 1)) - __shfl(offs, int(q1)));
+            uint32_t fc = len1 >= k ? len1 - k + 1 : 0;
+            if (a.paired && len1 < k) {
+                const uint32_t len2 =
+                    uint32_t(__shfl(offs, int(q1 + 2)) - __shfl(offs, int(q1 + 1)));
+                fc = len2 >= k ? len2 - k + 1 : 0;
+            }
+            uint32_t satc = 0, nrec = 0;
+            /* one frame chunk's slots into the per-ID totals or the record
+             * store */
+            const auto absorb = [&](bool valid, uint32_t present, uint32_t *raw) {
                 const uint32_t nh = uint32_t(__popc(present));
                 bvl += valid ? S : 0;
                 gat += nh;
@@ -688,7 +702,8 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                     const uint64_t em = __ballot(lid != 0xFFFFFFFFu);
                     uint32_t leader = 0xFFFFFFFFu;
                     if (em) {
-                        leader = uint32_t(__shfl(int(lid), int(__ffsll((unsigned long long)em) - 1)));
+                        const int first = int(__ffsll((unsigned long long)em) - 1);
+                        leader = uint32_t(__shfl(int(lid), first));
                         const bool mine = lid == leader;
                         const uint64_t bm = __ballot(mine);
                         uint32_t tot = 0, tns = 0;
@@ -700,7 +715,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                         const uint32_t nsat = uint32_t(__popcll(__ballot(mine && lns != 0)));
                         const uint32_t solid = uint32_t(__popcll(__ballot(mine && locc == S)));
                         const uint32_t nsf = uint32_t(__popcll(__ballot(mine && !anysat)));
-                        if (lane == uint32_t(__ffsll((unsigned long long)em) - 1) && leader < nid1) {
+                        if (lane == uint32_t(first) && leader < nid1) {
                             uint32_t *A = acc + kDirectWords * leader;
                             A[0] += cnt;
                             A[1] += nsat;
@@ -730,12 +745,12 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                                     atomicAdd(A + 5, 1u);
                             }
                         });
-                    continue;
+                    return;
                 }
                 uint32_t myrec = 0;
                 if (eval)
                     myrec = each_id([](uint32_t, uint64_t) {});
-                /* append this step's records: exclusive scan of myrec */
+                /* append this chunk's records: exclusive scan of myrec */
                 uint32_t pre = myrec;
                 for (int dd = 1; dd < 64; dd <<= 1) {
                     const uint32_t t = __shfl_up(pre, unsigned(dd));
@@ -750,8 +765,137 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
                             rec[base + x] = v;
                     });
                 nrec += total;
+            };
+            /* the S seed hashes of a frame: the don't-care bases XORed out of
+             * the window values (spaced seeds), or ntHash's multi-hash of the
+             * canonical value (k-mer mode) */
+            const auto seed_hashes = [&](const Frame &fr, uint64_t *hs) {
+                if (!spaced) {
+                    const uint64_t h0 = fr.rc < fr.fwd ? fr.rc : fr.fwd;
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        hs[sd] = multi_hash(h0, sd, uint64_t(k) * kMultiSeed);
+                    return;
+                }
+                uint64_t fw[SMAX], rv[SMAX];
+#pragma unroll
+                for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                    fw[sd] = fr.fwd;
+                    rv[sd] = fr.rc;
+                }
+                for (uint32_t dcm = a.d.dc_union; dcm; dcm &= dcm - 1) {
+                    const uint32_t jj = uint32_t(__builtin_ctz(dcm));
+                    const uint32_t cj = uint32_t(fr.W >> (2 * (k - 1 - jj))) & 3u;
+                    const ulonglong2 t = dct[jj * 4 + cj];
+#pragma unroll
+                    for (uint32_t sd = 0; sd < SMAX; ++sd)
+                        if (sd < S && !((a.d.care[sd] >> jj) & 1)) {
+                            fw[sd] ^= t.x;
+                            rv[sd] ^= t.y;
+                        }
+                }
+#pragma unroll
+                for (uint32_t sd = 0; sd < SMAX; ++sd)
+                    hs[sd] = rv[sd] < fw[sd] ? rv[sd] : fw[sd];
+            };
+            for (uint32_t mate = 0; mate < per && fc; ++mate) {
+                const uint32_t q = q1 + mate;
+                const uint64_t b = __shfl(offs, int(q));
+                const uint32_t len = uint32_t(__shfl(offs, int(q + 1)) - b);
+                if (len < k)
+                    continue;
+                const uint8_t *s = a.seq + b;
+                if (pf_q != q)
+                    prefetch(q);
+                unsigned c0 = pf[0], c1 = pf[1], c2 = pf[2];
+                pf_q = 0xFFFFFFFFu;
+                const uint32_t nfr = len - k + 1;
+                ScanCarry cy;
+                cy.H = cy.G = cy.W = 0;
+                cy.N = -1;
+                BaseChunk X = scan_chunk(c0, lane, len, lane, spaced, cy);
+                uint64_t hx = 0, gx = 0; /* prefixes through the base before X */
+                for (uint32_t c = 0; 64 * c < nfr; c += FG) {
+                    /* base chunks c + 1 .. c + FG */
+                    BaseChunk Y[FG];
+                    unsigned cb[FG];
+#pragma unroll
+                    for (uint32_t f = 0; f < FG; ++f) {
+                        const uint32_t i = 64 * (c + 1 + f) + lane;
+                        if (c == 0)
+                            cb[f] = f == 0 ? c1 : c2;
+                        else
+                            cb[f] = i < len ? unsigned(s[i]) : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t f = 0; f < FG; ++f)
+                        Y[f] = scan_chunk(cb[f], 64 * (c + 1 + f) + lane, len, lane, spaced, cy);
+                    /* frame chunks c .. c + FG - 1: hashes, then every lookup */
+                    uint32_t raw[FG][SMAX];
+                    uint32_t present[FG];
+                    bool valid[FG];
+#pragma unroll
+                    for (uint32_t f = 0; f < FG; ++f) {
+                        const BaseChunk &Xf = f == 0 ? X : Y[f > 0 ? f - 1 : 0];
+                        /* prefixes through the last base of the chunk before Xf */
+                        const BaseChunk &Wf = f == 1 ? X : Y[f > 1 ? f - 2 : 0];
+                        const uint64_t hp = f == 0 ? hx : __shfl(Wf.H, 63);
+                        const uint64_t gp = f == 0 ? gx : __shfl(Wf.G, 63);
+                        const Frame fr = frame_at(Xf, Y[f], hp, gp, 64 * (c + f), lane, k, nfr,
+                                                  spaced);
+                        valid[f] = fr.valid;
+                        uint64_t hs[SMAX];
+                        seed_hashes(fr, hs);
+                        present[f] = 0;
+                        if (TABLE) {
+                            /* one load per seed: the ID word, or 0 for a clear bit */
+#pragma unroll
+                            for (uint32_t sd = 0; sd < SMAX; ++sd)
+                                raw[f][sd] = (sd < S && fr.valid)
+                                                 ? a.d.direct[fastmod(hs[sd], a.d.m, a.d.magic,
+                                                                      a.d.sh1, a.d.sh2)]
+                                                 : 0u;
+                        } else {
+                            /* bit and rank from the line, then the ID slot */
+                            uint64_t rk[SMAX];
+#pragma unroll
+                            for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                                rk[sd] = 0;
+                                if (sd < S && fr.valid) {
+                                    const bool bit = bv_rank_line(
+                                        a.d.lines,
+                                        fastmod(hs[sd], a.d.m, a.d.magic, a.d.sh1, a.d.sh2),
+                                        rk[sd]);
+                                    present[f] |= uint32_t(bit) << sd;
+                                }
+                            }
+#pragma unroll
+                            for (uint32_t sd = 0; sd < SMAX; ++sd)
+                                raw[f][sd] = ((present[f] >> sd) & 1) ? a.d.data[rk[sd]] : 0u;
+                        }
+                    }
+                    /* the next sequence's first bases load while these
+                     * lookups are in flight */
+                    if (c == 0 && q + 1 < nseq)
+                        prefetch(q + 1);
+#pragma unroll
+                    for (uint32_t f = 0; f < FG; ++f) {
+                        if (TABLE) {
+#pragma unroll
+                            for (uint32_t sd = 0; sd < SMAX; ++sd) {
+                                present[f] |= uint32_t(raw[f][sd] != 0) << sd;
+                                if ((raw[f][sd] & kIdMask) == kIdMask)
+                                    raw[f][sd] &= kSatBit; /* ID 0 */
+                            }
+                        }
+                        if (64 * (c + f) < nfr)
+                            absorb(valid[f], present[f], raw[f]);
+                    }
+                    hx = __shfl(FG == 1 ? X.H : Y[FG - 2].H, 63);
+                    gx = __shfl(FG == 1 ? X.G : Y[FG - 2].G, 63);
+                    X = Y[FG - 1];
+                }
             }
-        }
         for (int o = 32; o > 0; o >>= 1)
             satc += __shfl_xor(satc, o);
         bool over = false;
@@ -759,6 +903,12 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
         if (DIRECT) {
             if (lane == 0)
                 ncand_s = 0;
+            if (row != mc_row) {
+                for (uint32_t id = lane; id < nid1; id += 64)
+                    mcs[id] = (id >= 1 && row) ? a.min_count[uint64_t(row) * nid1 + id]
+                                               : 0xFFFFFFFFu;
+                mc_row = row;
+            }
             __syncthreads();
             /* candidates straight from the per-ID totals; each lane clears
              * the IDs it read for the next record */
@@ -771,9 +921,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
 #pragma unroll
                 for (uint32_t x = 0; x < kDirectWords; ++x)
                     A[x] = 0;
-                const uint32_t mc = (id >= 1 && row)
-                                        ? a.min_count[uint64_t(row) * nid1 + id]
-                                        : 0xFFFFFFFFu;
+                const uint32_t mc = mcs[id];
                 if (cnt >= mc && nsat >= a.min_nonsat) {
                     const uint32_t slot = atomicAdd(&ncand_s, 1u);
                     if (slot < kMaxCand || (WIDE && slot - kMaxCand < a.spill_stride)) {
@@ -956,6 +1104,7 @@ __global__ __launch_bounds__(64) void k_mibf_query(QueryArgs a) {
             a.sat[r] = satc;
         }
         __syncthreads();
+        }
     }
     if (WIDE)
         return;
@@ -976,22 +1125,44 @@ __global__ void k_mibf_zero(unsigned long long *p, uint32_t n) {
 
 /* the first pass over every record, then the wide pass over the records it
  * listed (with an empty list the wide workgroups end at once) */
-template <uint32_t CAP, uint32_t SMAX, bool DIRECT>
+template <uint32_t CAP, uint32_t SMAX, bool DIRECT, bool TABLE>
 void query_pair(const QueryArgs &a, int cu, size_t shm, hipStream_t st) {
     /* as many one-wave workgroups as fit on every CU (registers and LDS),
      * each pulling records from the queue */
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_mibf_query<CAP, SMAX, DIRECT, false>, 64, shm) != hipSuccess ||
+            &per_cu, k_mibf_query<CAP, SMAX, DIRECT, false, TABLE>, 64, shm) != hipSuccess ||
         per_cu < 1)
         per_cu = 16;
     uint64_t nb = std::max<uint64_t>(1, std::min<uint64_t>(a.nrec, uint64_t(cu) * per_cu));
     if (CAP == 0)
         nb = std::min<uint64_t>(nb, a.grec_blocks); /* the device record area's room */
     const uint32_t blocks = uint32_t(nb);
-    k_mibf_query<CAP, SMAX, DIRECT, false><<<blocks, 64, shm, st>>>(a);
+    k_mibf_query<CAP, SMAX, DIRECT, false, TABLE><<<blocks, 64, shm, st>>>(a);
     if (a.spill_stride)
-        k_mibf_query<CAP, SMAX, DIRECT, true><<<kWideBlocks, 64, shm, st>>>(a);
+        k_mibf_query<CAP, SMAX, DIRECT, true, TABLE><<<kWideBlocks, 64, shm, st>>>(a);
+}
+
+/* the kernel for the filter's ID count (per-ID totals in LDS, or the record
+ * store sized by the longest record), seed count and layout */
+template <bool TABLE>
+void query_launch(const QueryArgs &a, bool direct, uint64_t recs, bool few, int blocks, size_t shm,
+                  hipStream_t st) {
+    if (direct)
+        few ? query_pair<1, 4, true, TABLE>(a, blocks, shm, st)
+            : query_pair<1, 8, true, TABLE>(a, blocks, shm, st);
+    else if (recs <= 1024)
+        few ? query_pair<1024, 4, false, TABLE>(a, blocks, 0, st)
+            : query_pair<1024, 8, false, TABLE>(a, blocks, 0, st);
+    else if (recs <= 4096)
+        few ? query_pair<4096, 4, false, TABLE>(a, blocks, 0, st)
+            : query_pair<4096, 8, false, TABLE>(a, blocks, 0, st);
+    else if (recs <= 8192)
+        few ? query_pair<8192, 4, false, TABLE>(a, blocks, 0, st)
+            : query_pair<8192, 8, false, TABLE>(a, blocks, 0, st);
+    else
+        few ? query_pair<0, 4, false, TABLE>(a, blocks, 0, st)
+            : query_pair<0, 8, false, TABLE>(a, blocks, 0, st);
 }
 
 } // namespace
@@ -1656,21 +1827,12 @@ bbt_status bbt_mibf_query_batch(bbt_mibf *mf, const bbt_reads *in, const bbt_mib
     }
     const int blocks = mf->cu;
     const bool few = mf->d.S <= 4;
-    if (direct) {
-        const size_t shm = size_t(kDirectWords) * (mf->d.nids + 1) * sizeof(uint32_t);
-        few ? query_pair<1, 4, true>(a, blocks, shm, st) : query_pair<1, 8, true>(a, blocks, shm, st);
-    } else if (recs <= 1024)
-        few ? query_pair<1024, 4, false>(a, blocks, 0, st)
-            : query_pair<1024, 8, false>(a, blocks, 0, st);
-    else if (recs <= 4096)
-        few ? query_pair<4096, 4, false>(a, blocks, 0, st)
-            : query_pair<4096, 8, false>(a, blocks, 0, st);
-    else if (recs <= 8192)
-        few ? query_pair<8192, 4, false>(a, blocks, 0, st)
-            : query_pair<8192, 8, false>(a, blocks, 0, st);
+    /* per-ID totals and one row of minimum counts */
+    const size_t shm = direct ? size_t(kDirectWords + 1) * (mf->d.nids + 1) * sizeof(uint32_t) : 0;
+    if (a.d.direct)
+        query_launch<true>(a, direct, recs, few, blocks, shm, st);
     else
-        few ? query_pair<0, 4, false>(a, blocks, 0, st)
-            : query_pair<0, 8, false>(a, blocks, 0, st);
+        query_launch<false>(a, direct, recs, few, blocks, shm, st);
     MHIP(hipGetLastError());
     MHIP(hipEventRecord(mf->last, st));
     mf->has_last = true;
