@@ -535,6 +535,54 @@ def cpu_baseline_serve(fs, pool, target_s, want):
             "parity_with_gpu": parity, "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
+def serve_resident(torch, D, B, fs, pool, want, steps=3):
+    """The serving pool's reads as one batch already in HBM, through the
+    batched and the plain path with the library's default knobs: the walk
+    kernel on the HBM-resident 3 Gbp filter without host copies or queueing.
+    Each path's hits and classes are checked against the requests alone."""
+    bufs, offs, base = [], [], 0
+    for buf, off in pool:
+        n = int(off[-1])
+        bufs.append(np.asarray(buf[:n]))
+        offs.append(np.asarray(off[:-1], np.uint64) + np.uint64(base))
+        base += n
+    rbuf = np.concatenate(bufs)
+    roff = np.concatenate(offs + [np.array([base], np.uint64)])
+    want_hits = np.concatenate([h for h, _ in want])
+    want_cls = np.concatenate([c for _, c in want])
+    dr = D.DeviceReads(rbuf, roff)
+    out = D.Outputs(dr.nreads, len(fs), False, False)
+    rs, os_ = dr.struct(), out.struct()
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    res = {"reads": dr.nreads, "bases": base, "max_len": dr.max_len,
+           "knobs": "library defaults"}
+    for label, path in (("batched", B.PATH_BATCHED), ("plain", B.PATH_PLAIN)):
+        def step(flags=0):
+            out.zero_counts()
+            fs.eval_batch(rs, B.params(path=path, flags=flags), os_, stream)
+
+        ms = time_steps(step, steps, 1, torch)
+        walk = []
+        for _ in range(3):
+            step(B.FLAG_TIME_WALK)
+            walk.append(fs.last_walk_ms())
+        torch.cuda.synchronize()
+        host = out.host()
+        assert int(host["counters"][3]) == 0, "device error flag set"
+        res[label] = {"ms": ms, "walk_ms": float(np.mean(walk)),
+                      "reads_per_s": dr.nreads / (ms / 1e3),
+                      "probes": int(host["counters"][0]), "lookups": int(host["counters"][1]),
+                      "issued": int(host["counters"][2]),
+                      "equal_to_alone": bool(np.array_equal(host["hits"], want_hits)
+                                             and np.array_equal(host["cls"], want_cls))}
+        log(f"serve resident {label}: {ms:.3f} ms/step, walk {np.mean(walk):.3f} ms, "
+            f"{res[label]['reads_per_s'] / 1e6:.2f} M reads/s, "
+            f"{res[label]['lookups'] / (np.mean(walk) / 1e3) / 1e9:.1f} G lookups/s, "
+            f"equal={res[label]['equal_to_alone']}")
+    res["speedup_vs_plain"] = res["plain"]["ms"] / res["batched"]["ms"]
+    return res
+
+
 def bench_serve(args, torch, D, B, w, gathers):
     pool = w["requests"]
     t = time.time()
@@ -569,16 +617,35 @@ def bench_serve(args, torch, D, B, w, gathers):
     low = min(runs["batched"], key=lambda r: r["qps"])
     low_plain = (min(runs["plain"], key=lambda r: r["qps"]) if runs["plain"] else None)
     # the random-line rate over a buffer of the big filter's size: what the
-    # saturated server's lookups (one random 64 B line each) are measured against
+    # lookups (one random 64 B line each) are measured against
     gbytes, glps = gathers.rate(max(big.m // 8, 4096) * 2 - 1)
-    achieved = top["lookups_per_s"] * LINE_BYTES / 1e9
-    rl = {"bound": "hbm", "kernel": "k_walk_persistent (served launches)",
+    served = {"achieved": top["lookups_per_s"] * LINE_BYTES / 1e9,
+              "frac": top["lookups_per_s"] * LINE_BYTES / 1e9 / HBM_PEAK_GBS,
+              "frac_of_random_gather": top["lookups_per_s"] / glps if glps else None,
+              "note": "lookups x 64 B of the saturated batched run over its wall span "
+                      "(host copies included)"}
+    res = serve_resident(torch, D, B, fs, pool, want)
+    walk_ms = res["batched"]["walk_ms"]
+    lookups = res["batched"]["lookups"]
+    achieved = lookups * LINE_BYTES / (walk_ms / 1e3) / 1e9
+    traffic = load_traffic("serve")
+    rl = {"bound": "hbm", "kernel": "k_walk_persistent",
           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-          "note": "lookups x 64 B of the saturated batched run over its wall span "
-                  "(host copies included)",
+          "frac": achieved / HBM_PEAK_GBS,
+          "traffic": traffic["bytes_per_launch"] if traffic else None,
+          "traffic_source": ("profiles/traffic_walk_serve.json: rocprofv3 --pmc FETCH_SIZE + "
+                             "WRITE_SIZE per launch, scaled by a device copy of known size"
+                             if traffic else None),
+          "note": "the request pool as one batch resident in HBM: reference lookups x 64 B "
+                  "over the walk kernel's event-timed duration",
+          "walk_ms": walk_ms, "algorithmic_bytes": lookups * LINE_BYTES,
+          "filter_set_bytes": sum(fs.info(i).m // 8 for i in range(len(fs))),
           "random_gather": {"big_filter_pow2": {"bytes": gbytes, "lines_per_s": glps}},
-          "frac_of_random_gather": top["lookups_per_s"] / glps if glps else None}
+          "frac_of_random_gather": (lookups / (walk_ms / 1e3) / glps if glps else None),
+          "served": served}
+    if traffic:
+        rl["traffic_gbs"] = traffic["bytes_per_launch"] / (walk_ms / 1e3) / 1e9
+        rl["traffic_frac"] = rl["traffic_gbs"] / HBM_PEAK_GBS
     sub = {"workload": WORKLOAD_TEXT["serve"], "filters": len(fs), "big_filter_bits": big.m,
            "value": top["reads_per_s"], "unit": "reads/s", "qps_at_value": top["qps"],
            "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
@@ -590,7 +657,7 @@ def bench_serve(args, torch, D, B, w, gathers):
            "speedup_vs_plain": (top["reads_per_s"] / plain_top["reads_per_s"]
                                 if plain_top else None),
            "saturated_lookups_per_s": top["lookups_per_s"], "roofline": rl,
-           "serving": runs,
+           "resident": res, "serving": runs,
            "alone_equals_mixed": all(r["equal_to_alone"] for m in runs.values() for r in m)}
 
     def cpu():
