@@ -61,6 +61,8 @@ constexpr uint32_t kDefaultWaveMin = 256;
 constexpr uint32_t kDefaultRefillMin = 32;
 /* probe unroll: lookups of one contains() call issued together */
 constexpr uint32_t kAhead = 4;
+/* LDS of one CU (gfx950) */
+constexpr uint32_t kLdsPerCu = 160 * 1024;
 
 /* A (read, filter) item whose BINOMIAL evalScore the host computes: the
  * walk's hit count over its frame count (the read has more frames than the
@@ -912,6 +914,8 @@ struct bbt_workspace {
     uint64_t dmask_words = 0;
     uint8_t *dscratch = nullptr; /* sized for dscratch_w */
     uint32_t dscratch_w = 0;
+    uint32_t *dfall = nullptr; /* reads the LDS pass leaves to the scratch pass */
+    uint64_t dfall_cap = 0;
     /* BBT_READS_2BIT: the batch unpacked to ASCII */
     uint8_t *dascii = nullptr;
     uint64_t ascii_cap = 0;
@@ -937,6 +941,8 @@ static void workspace_release(bbt_workspace *w) {
         (void)hipFree(w->dmask);
     if (w->dscratch)
         (void)hipFree(w->dscratch);
+    if (w->dfall)
+        (void)hipFree(w->dfall);
     if (w->dascii)
         (void)hipFree(w->dascii);
     if (w->dfix)
@@ -1786,6 +1792,16 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             HIP_OK(hipMalloc(&W.dscratch, uint64_t(lay.threads) * lay.stride));
             W.dscratch_w = p->dust_w;
         }
+        if (p->dust_w <= uint32_t(kDustFastW) && W.dfall_cap < in->nreads) {
+            if (W.dfall) {
+                HIP_OK(hipStreamSynchronize(st));
+                HIP_OK(hipFree(W.dfall));
+            }
+            W.dfall = nullptr;
+            W.dfall_cap = 0;
+            HIP_OK(hipMalloc(&W.dfall, in->nreads * sizeof(uint32_t)));
+            W.dfall_cap = in->nreads;
+        }
     }
     if (p->path == BBT_PATH_BATCHED && W.lists_cap < in->nreads) {
         /* the lists may still be read by this workspace's previous batch */
@@ -1848,6 +1864,30 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
 
 /* The kernels of one batch, in stream order; capturable (no memset nodes:
  * counters are cleared by a fill kernel). */
+/* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
+ * current device (asked once per device). */
+static bool dust_big_ok(uint64_t bytes) {
+    static std::mutex mu;
+    static std::map<int, int> limit; /* device -> bytes, 0 if unavailable */
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return false;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = limit.find(dev);
+    if (it == limit.end()) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess ||
+            hipFuncSetAttribute(reinterpret_cast<const void *>(k_dust_big),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess) {
+            (void)hipGetLastError();
+            v = 0;
+        }
+        it = limit.emplace(dev, v).first;
+    }
+    return bytes <= uint64_t(it->second);
+}
+
 static bbt_status enqueue_classify(const bbt_eval_params *p, const bbt_eval_out *out,
                                    const Prepared &P);
 
@@ -1877,13 +1917,40 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         /* -D pre-pass: clear the bitmap, then sdust every read */
         k_fill_u8<<<1024, kBlock, 0, st>>>(reinterpret_cast<uint8_t *>(W.dmask),
                                             W.dmask_words * sizeof(uint32_t), 0);
-        k_zero_u64<<<1, 64, 0, st>>>(W.dctl + 100, 2);
+        /* [104] LDS-pass queue, [105] error sink, [106] scratch-pass queue,
+         * [107] reads listed for the scratch pass */
+        k_zero_u64<<<1, 64, 0, st>>>(W.dctl + 104, 4);
+        unsigned long long *err = out->counters
+                                      ? reinterpret_cast<unsigned long long *>(out->counters) + 3
+                                      : W.dctl + 105;
         /* the layout the scratch was allocated with (its window >= this one's) */
         const DustLayout lay = dust_layout(int(W.dscratch_w));
-        k_dust<<<lay.threads / 64, 64, 0, st>>>(
-            a.seq, in->off, in->nreads, int(p->dust_t), int(p->dust_w), W.dscratch, lay, W.dmask,
-            W.dctl + 100, out->counters ? reinterpret_cast<unsigned long long *>(out->counters) + 3
-                                        : W.dctl + 101);
+        if (p->dust_w <= uint32_t(kDustFastW)) {
+            /* LDS pass over every read, then the reads whose interval lists
+             * outgrew it: one per workgroup with the whole list in LDS when
+             * the device allows it, else on the scratch pass */
+            const uint32_t grid = uint32_t(std::min<uint64_t>(
+                (in->nreads + 63) / 64, uint64_t(fs->cu_count) * (kLdsPerCu / kDustFastLds)));
+            k_dust_lds<<<grid, 64, 0, st>>>(a.seq, in->off, in->nreads, int(p->dust_t),
+                                             int(p->dust_w), W.dmask, W.dctl + 104, W.dfall,
+                                             W.dctl + 107);
+            HIP_OK(hipGetLastError());
+            const DustLayout big = dust_layout(int(p->dust_w));
+            if (dust_big_ok(big.stride)) {
+                k_dust_big<<<uint32_t(fs->cu_count), 64, size_t(big.stride), st>>>(
+                    a.seq, in->off, int(p->dust_t), int(p->dust_w), big, W.dmask, W.dctl + 106,
+                    W.dfall, W.dctl + 107, err);
+            } else {
+                k_dust<<<(lay.threads + 63) / 64, 64, 0, st>>>(
+                    a.seq, in->off, in->nreads, int(p->dust_t), int(p->dust_w), W.dscratch, lay,
+                    W.dmask, W.dctl + 106, W.dfall, W.dctl + 107, err);
+            }
+        } else {
+            k_dust<<<(lay.threads + 63) / 64, 64, 0, st>>>(a.seq, in->off, in->nreads,
+                                                    int(p->dust_t), int(p->dust_w), W.dscratch,
+                                                    lay, W.dmask, W.dctl + 106, nullptr,
+                                                    nullptr, err);
+        }
         HIP_OK(hipGetLastError());
         a.dust = W.dmask;
     }

@@ -145,7 +145,8 @@ typedef struct {
                              depends on it.  Unpacked on the device. */
 
 typedef struct {
-    const char *seq;       /* device: bases of all reads back to back, in format */
+    const char *seq;       /* device, 8-byte aligned: bases of all reads back to
+                              back, in format */
     uint64_t seq_bytes;    /* ASCII: readable bytes at seq, >= off[nreads]
                               rounded up to a multiple of 8.  2BIT: bytes of
                               codes, a multiple of 8 with 4 x seq_bytes >=

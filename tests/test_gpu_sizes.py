@@ -12,6 +12,7 @@
   built on the device and probed by both paths, against the oracle.
 * A multi-index Bloom filter of more than 2^32 bits, built and queried on the
   device, against the oracle.
+* -D on the full single-filter workload with planted low-complexity runs.
 """
 import os
 import random
@@ -103,6 +104,41 @@ def test_single_filter_workload_full_size(dev, tmp_path):
                        + extra + [str(tmp_path / "reads.fq")], check=True, timeout=300)
         assert open(f"{prefix}_summary.tsv").read() == want_summary
         assert open(f"{prefix}_readStatus.tsv").read().splitlines() == want_status
+
+
+def test_dust_single_workload_full_size(dev):
+    """-D on the single-filter workload at full size (100k reads), every
+    20th read with a planted low-complexity run: the LDS sdust kernel takes
+    the reads whose interval lists stay short and hands the planted runs to
+    the device-memory kernel; hits, classes and counters equal the oracle's
+    for the default window, a short one and one past the LDS kernel."""
+    D, B = dev
+    from biobloom_amd import synth
+    w = synth.workload_single(1.0)
+    fs, _ = D.build_filters(w["genomes"], 25, 0.075)
+    of = oracle_copy(fs, 0)
+    rng = np.random.default_rng(31)
+    units = [b"A", b"AC", b"CAG", b"ACGT", b"TTAGGG"]
+    reads = []
+    for i, r in enumerate(w["reads"]):
+        r = r.tobytes()
+        if i % 20 == 0:
+            a = int(rng.integers(0, len(r)))
+            r = r[:a] + units[i // 20 % len(units)] * int(rng.integers(4, 30)) + r[a:]
+        reads.append(r)
+    buf, off = O.pack_reads(reads)
+    dr = D.DeviceReads(buf, off)
+    for dt, dw in ((20, 64), (10, 32), (20, 80)):
+        oh, _, ocls, octr = O.classify([of], buf, off, p=O.params(dust=True, dust_t=dt,
+                                                                    dust_w=dw))
+        for path in (B.PATH_PLAIN, B.PATH_BATCHED):
+            print(f"[gpu] dust T={dt} W={dw} path={path} reads={len(reads)}", flush=True)
+            res = D.classify(fs, dr, B.params(path=path, dust=True, dust_t=dt, dust_w=dw))
+            assert np.array_equal(res["hits"], oh), "hits differ"
+            assert np.array_equal(res["cls"], ocls), "classes differ"
+            assert int(res["counters"][0]) == int(octr[0]), "contains() calls differ"
+            assert int(res["counters"][1]) == int(octr[1]), "bit lookups differ"
+            assert int(res["counters"][3]) == 0
 
 
 def test_bloom_filter_above_2_34_bits(dev):
