@@ -25,6 +25,7 @@ eager launch of the same path and the plain per-batch path are beside it.
 
 --workload {single,uneven,mibf,serve}: one workload alone, as its own line.
 --workload e2e / mibf-e2e: the CLIs end to end on FASTQ files.
+--workload dust: -D (the sdust pre-pass) against the same batches without it.
 """
 import argparse
 import json
@@ -666,6 +667,42 @@ def bench_serve(args, torch, D, B, w, gathers):
     return sub, cpu
 
 
+def bench_dust(args, torch, D, B):
+    """-D (sdust pre-pass, T = 20, W = 64) against the same batch without it:
+    the single-filter workload and a quarter of the multi-filter reads
+    (1M x 150 bp), batched path, steps timed by events around the call."""
+    from biobloom_amd import synth
+    res = {}
+    for name, scale in (("single", 1.0), ("multi", 0.25)):
+        w = getattr(synth, f"workload_{name}")(scale=scale)
+        fs, _ = D.build_filters(w["genomes"], w["k"], w["fpr"])
+        dr = D.DeviceReads.from_matrix(w["reads"])
+        paired = bool(w["paired"])
+        out = D.Outputs(dr.nreads, len(fs), paired, False)
+        stream = int(torch.cuda.current_stream().cuda_stream)
+        rs, os_ = dr.struct(), out.struct()
+        r = {"reads": dr.nreads}
+        for dust in (False, True):
+            def step(flags=0):
+                out.zero_counts()
+                fs.eval_batch(rs, B.params(paired=paired, dust=dust, flags=flags), os_, stream)
+            ms = time_steps(step, args.steps, args.warmup, torch)
+            step(B.FLAG_TIME_WALK)
+            torch.cuda.synchronize()
+            key = "dust" if dust else "plain"
+            r[key] = {"ms_per_step": ms, "walk_ms": fs.last_walk_ms(),
+                      "reads_per_s": dr.nreads / (ms / 1e3)}
+            log(f"dust {name} dust={dust}: {ms:.3f} ms/step, walk {fs.last_walk_ms():.3f} ms")
+        r["dust_ms_per_million_reads"] = ((r["dust"]["ms_per_step"] - r["plain"]["ms_per_step"])
+                                          * 1e6 / dr.nreads)
+        res[name] = r
+    line = {"metric": "-D step time (sdust pre-pass + walks)", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "single (100k reads) and multi (1M reads), T=20 W=64"},
+            "results": res}
+    print(json.dumps(line), flush=True)
+
+
 def as_line(sub, steps, warmup, dtype="u64"):
     """A sub-object as a full bench line of its own (--workload runs)."""
     line = {"metric": METRIC, "value": sub["value"], "unit": sub["unit"], "n_gpus": 1,
@@ -967,7 +1004,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="multi",
-                    choices=["multi", "single", "uneven", "mibf", "serve", "e2e", "mibf-e2e"])
+                    choices=["multi", "single", "uneven", "mibf", "serve", "e2e", "mibf-e2e",
+                             "dust"])
     ap.add_argument("--no-suite", action="store_true",
                     help="multi: the headline workload only, without the other configs")
     ap.add_argument("--suite-steps", type=int, default=3,
@@ -1055,6 +1093,8 @@ def main():
         return bench_e2e(args, torch, D, B)
     if args.workload == "mibf-e2e":
         return bench_mibf_e2e(args, torch, D, B)
+    if args.workload == "dust":
+        return bench_dust(args, torch, D, B)
     gathers = GatherCache(B, not args.no_gather)
     if args.workload in ("single", "uneven", "mibf", "serve"):
         w = generate(args.workload, args.scale)
