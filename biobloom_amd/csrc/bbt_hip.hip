@@ -458,36 +458,70 @@ __global__ void k_len_scan(unsigned long long *cnt, unsigned long long *base,
         cnt[b] = 0; /* reused as scatter cursors */
 }
 
+/* Claims, for every lane with a bucket b >= 0, the next free slot of that
+ * bucket's counter in cnt (LDS), one atomic per distinct bucket in the wave;
+ * returns the lane's slot. */
+__device__ __forceinline__ uint32_t wave_claim(int b, unsigned int *cnt) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    uint64_t pending = __ballot(b >= 0);
+    uint32_t mine = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int bl = __shfl(b, leader);
+        const uint64_t m = __ballot(b == bl);
+        unsigned int base = 0;
+        if (int(lane) == leader)
+            base = atomicAdd(cnt + bl, unsigned(__popcll(m)));
+        base = __shfl(base, leader);
+        if (b == bl)
+            mine = base + uint32_t(__popcll(m & lt_mask));
+        pending &= ~m;
+    }
+    return mine;
+}
+
+/* Each workgroup takes a contiguous share of the reads: it counts its reads
+ * per bucket in LDS, reserves that many list slots per bucket with one
+ * global atomic each, then writes every read into its bucket's slots (the
+ * order inside a bucket is not defined).  Buckets 0..31 are the wave list by
+ * floor(log2 len) at base[b]; 32 is the lane list. */
 __global__ __launch_bounds__(kBlock) void k_len_scatter(const uint64_t *off, uint64_t nreads,
                                                         uint32_t wave_min,
                                                         const unsigned long long *base,
                                                         unsigned long long *cursor,
                                                         uint32_t *long_list,
                                                         uint32_t *short_list) {
-    const uint32_t lane = lane_id();
-    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    const uint64_t start = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (uint64_t r0 = start - lane; r0 < nreads; r0 += stride) {
-        const uint64_t r = r0 + lane;
-        const bool in = r < nreads;
-        const uint64_t len = in ? off[r + 1] - off[r] : 0;
-        const bool lng = in && len >= wave_min && len > 0;
-        const bool sht = in && !lng;
-        const uint64_t sm = __ballot(sht);
-        unsigned long long sb = 0;
-        if (sm) {
-            const uint32_t leader = uint32_t(__ffsll((unsigned long long)sm) - 1);
-            if (lane == leader)
-                sb = atomicAdd(cursor + 32, (unsigned long long)__popcll(sm));
-            sb = __shfl(sb, int(leader));
-        }
-        if (sht)
-            short_list[sb + __popcll(sm & lt_mask)] = uint32_t(r);
-        if (lng) {
-            const int bk = min(31, 63 - __clzll((long long)len));
-            long_list[base[bk] + atomicAdd(cursor + bk, 1ull)] = uint32_t(r);
-        }
+    __shared__ unsigned int cnt[33];
+    __shared__ unsigned long long at[33];
+    const uint64_t share = (nreads + gridDim.x - 1) / gridDim.x;
+    const uint64_t r_begin = uint64_t(blockIdx.x) * share;
+    const uint64_t r_end = r_begin + share < nreads ? r_begin + share : nreads;
+    const auto bucket = [&](uint64_t r) -> int {
+        if (r >= r_end)
+            return -1;
+        const uint64_t len = off[r + 1] - off[r];
+        return len >= wave_min && len > 0 ? min(31, 63 - __clzll((long long)len)) : 32;
+    };
+    for (int t = threadIdx.x; t < 33; t += blockDim.x)
+        cnt[t] = 0;
+    __syncthreads();
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += blockDim.x)
+        (void)wave_claim(bucket(r0 + threadIdx.x), cnt);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 33; t += blockDim.x) {
+        at[t] = cnt[t] ? atomicAdd(cursor + t, (unsigned long long)cnt[t]) : 0;
+        cnt[t] = 0;
+    }
+    __syncthreads();
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += blockDim.x) {
+        const uint64_t r = r0 + threadIdx.x;
+        const int b = bucket(r);
+        const uint32_t slot = wave_claim(b, cnt);
+        if (b == 32)
+            short_list[at[32] + slot] = uint32_t(r);
+        else if (b >= 0)
+            long_list[base[b] + at[b] + slot] = uint32_t(r);
     }
 }
 
@@ -1651,7 +1685,32 @@ struct Prepared {
     const uint8_t *codes; /* BBT_READS_2BIT: the packed batch, unpacked first */
     uint64_t code_bytes;
     bool fix_scores;      /* BINOMIAL scores past the table, finished on the host */
+    bool dust_big = false; /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
 };
+
+/* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
+ * current device (asked once per device). */
+static bool dust_big_ok(uint64_t bytes) {
+    static std::mutex mu;
+    static std::map<int, int> limit; /* device -> bytes, 0 if unavailable */
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return false;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = limit.find(dev);
+    if (it == limit.end()) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess ||
+            hipFuncSetAttribute(reinterpret_cast<const void *>(k_dust_big),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess) {
+            (void)hipGetLastError();
+            v = 0;
+        }
+        it = limit.emplace(dev, v).first;
+    }
+    return bytes <= uint64_t(it->second);
+}
 
 /* noalloc: the call may not allocate (bbt_eval_batch_noalloc): W already
  * holds its control words and lists, and what would need an allocation (the
@@ -1677,6 +1736,9 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
                     "bbt_eval_batch: seq must be 8-byte aligned with a size multiple of 8");
     if (in->format > BBT_READS_2BIT)
         return fail(BBT_ERR_ARG, "bbt_eval_batch: unknown read format");
+    if (in->nreads > 0xFFFFFFFFull)
+        return fail(BBT_ERR_ARG, "bbt_eval_batch: more than 2^32 - 1 reads in one batch "
+                                 "(read lists hold 32-bit indices); split the batch");
     if (in->format == BBT_READS_2BIT) {
         if (noalloc)
             return fail(BBT_ERR_UNSUPPORTED,
@@ -1792,6 +1854,9 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             HIP_OK(hipMalloc(&W.dscratch, uint64_t(lay.threads) * lay.stride));
             W.dscratch_w = p->dust_w;
         }
+        /* asked here, before any capture of the batch */
+        P.dust_big = p->dust_w <= uint32_t(kDustFastW) &&
+                     dust_big_ok(dust_layout(int(p->dust_w)).stride);
         if (p->dust_w <= uint32_t(kDustFastW) && W.dfall_cap < in->nreads) {
             if (W.dfall) {
                 HIP_OK(hipStreamSynchronize(st));
@@ -1864,30 +1929,6 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
 
 /* The kernels of one batch, in stream order; capturable (no memset nodes:
  * counters are cleared by a fill kernel). */
-/* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
- * current device (asked once per device). */
-static bool dust_big_ok(uint64_t bytes) {
-    static std::mutex mu;
-    static std::map<int, int> limit; /* device -> bytes, 0 if unavailable */
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess)
-        return false;
-    std::lock_guard<std::mutex> g(mu);
-    auto it = limit.find(dev);
-    if (it == limit.end()) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
-                hipSuccess ||
-            hipFuncSetAttribute(reinterpret_cast<const void *>(k_dust_big),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess) {
-            (void)hipGetLastError();
-            v = 0;
-        }
-        it = limit.emplace(dev, v).first;
-    }
-    return bytes <= uint64_t(it->second);
-}
-
 static bbt_status enqueue_classify(const bbt_eval_params *p, const bbt_eval_out *out,
                                    const Prepared &P);
 
@@ -1936,7 +1977,7 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
                                              W.dctl + 107);
             HIP_OK(hipGetLastError());
             const DustLayout big = dust_layout(int(p->dust_w));
-            if (dust_big_ok(big.stride)) {
+            if (P.dust_big) {
                 k_dust_big<<<uint32_t(fs->cu_count), 64, size_t(big.stride), st>>>(
                     a.seq, in->off, int(p->dust_t), int(p->dust_w), big, W.dmask, W.dctl + 106,
                     W.dfall, W.dctl + 107, err);
