@@ -326,9 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     const uint64_t nlong = nl * a.nf;
     const uint64_t nitems = nr * a.nf;
 
-    /* phase 1: long reads, a whole wave per (read, filter), longest first
-     * (read-major, so the longest walks of every filter start first and none
-     * is left for the end); every lane holds the same walk state */
+    /* phase 1: long reads, a whole wave per (read, filter), filter-major and
+     * longest first within a filter (the walks in flight then probe one or two
+     * filters, as the lane walks do: 88.5 -> 85.5 ms on the uneven workload);
+     * every lane holds the same walk state */
     while (nlong) {
         unsigned long long it = 0;
         if (lane == 0)
@@ -336,8 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
         it = __shfl(it, 0);
         if (it >= nlong)
             break;
-        const uint64_t r = it / a.nf;
-        const uint32_t f = uint32_t(it - r * a.nf);
+        const uint32_t f = uint32_t(it / nl);
+        const uint64_t r = it - uint64_t(f) * nl;
         const uint64_t read = a.long_list[r];
         Walker<SC, EXH, K> ww;
         wave_walk(a, read, f, ww, issued);
