@@ -86,6 +86,10 @@ struct EvalArgs {
     const uint32_t *long_list;  /* reads walked by a whole wave, longest first */
     const uint32_t *short_list; /* reads walked one per lane */
     const unsigned long long *list_counts; /* [0] long reads, [1] short reads */
+    /* length buckets of the wave list: bucket b's reads are at
+     * long_list[bucket_base[b] ..) and number bucket_cnt[b] (b = 0..31) */
+    const unsigned long long *bucket_base;
+    const unsigned long long *bucket_cnt;
     const uint32_t *dust;       /* -D: bit (off[read] + p) = window p is low complexity */
     uint64_t nitems;
     uint32_t nf;
@@ -326,19 +330,44 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     const uint64_t nlong = nl * a.nf;
     const uint64_t nitems = nr * a.nf;
 
-    /* phase 1: long reads, a whole wave per (read, filter), filter-major and
-     * longest first within a filter (the walks in flight then probe one or two
-     * filters, as the lane walks do: 88.5 -> 85.5 ms on the uneven workload);
-     * every lane holds the same walk state */
+    /* phase 1: long reads, a whole wave per (read, filter): length bucket by
+     * length bucket, longest first, and filter-major inside a bucket.  The
+     * longest walks of every filter start first (none is left for the end of
+     * a small launch), and the walks in flight probe one or two filters at a
+     * time, as the lane walks do (uneven workload: 88.5 -> 85.4 ms against
+     * read-major order) */
+    __shared__ unsigned long long s_bbase[32], s_bcnt[32];
+    if (nlong) {
+        for (int t = threadIdx.x; t < 32; t += blockDim.x) {
+            s_bbase[t] = a.bucket_base[t];
+            s_bcnt[t] = a.bucket_cnt[t];
+        }
+    }
+    __syncthreads();
     while (nlong) {
-        unsigned long long it = 0;
-        if (lane == 0)
+        unsigned long long it = 0, fr = 0;
+        if (lane == 0) {
             it = atomicAdd(a.lqueue, 1ull);
+            if (it < nlong) {
+                /* the bucket whose item range [base nf, (base + cnt) nf) holds it */
+                for (int b = 31; b >= 0; --b) {
+                    const unsigned long long c = s_bcnt[b];
+                    const unsigned long long lo = s_bbase[b] * a.nf;
+                    if (c && it < lo + c * a.nf) {
+                        const unsigned long long q = it - lo;
+                        const unsigned long long fb = q / c;
+                        fr = (fb << 40) | (s_bbase[b] + (q - fb * c));
+                        break;
+                    }
+                }
+            }
+        }
         it = __shfl(it, 0);
         if (it >= nlong)
             break;
-        const uint32_t f = uint32_t(it / nl);
-        const uint64_t r = it - uint64_t(f) * nl;
+        fr = __shfl(fr, 0);
+        const uint32_t f = uint32_t(fr >> 40);
+        const uint64_t r = fr & ((uint64_t(1) << 40) - 1);
         const uint64_t read = a.long_list[r];
         Walker<SC, EXH, K> ww;
         wave_walk(a, read, f, ww, issued);
@@ -1686,7 +1715,7 @@ struct Prepared {
     const uint8_t *codes; /* BBT_READS_2BIT: the packed batch, unpacked first */
     uint64_t code_bytes;
     bool fix_scores;      /* BINOMIAL scores past the table, finished on the host */
-    bool dust_big = false; /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
+    bool dust_big;        /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
 };
 
 /* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
@@ -1825,6 +1854,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         return s;
     if (!W.dctl)
         HIP_OK(hipMalloc(&W.dctl, 128 * sizeof(unsigned long long)));
+    bool dust_big = false; /* -D: handed-on reads in a workgroup's LDS */
     if (p->dust) {
         if (p->dust_w < 3 || p->dust_w > uint32_t(kDustMaxW) || p->dust_t > 1000000)
             return fail(BBT_ERR_UNSUPPORTED, "dust window (-W) must be 3.." +
@@ -1856,8 +1886,8 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             W.dscratch_w = p->dust_w;
         }
         /* asked here, before any capture of the batch */
-        P.dust_big = p->dust_w <= uint32_t(kDustFastW) &&
-                     dust_big_ok(dust_layout(int(p->dust_w)).stride);
+        dust_big = p->dust_w <= uint32_t(kDustFastW) &&
+                   dust_big_ok(dust_layout(int(p->dust_w)).stride);
         if (p->dust_w <= uint32_t(kDustFastW) && W.dfall_cap < in->nreads) {
             if (W.dfall) {
                 HIP_OK(hipStreamSynchronize(st));
@@ -1881,6 +1911,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         W.lists_cap = in->nreads;
     }
     std::memset(&P, 0, sizeof(P));
+    P.dust_big = dust_big;
     Launch &L = P.L;
     L.st = st;
     L.path = p->path;
@@ -2010,6 +2041,8 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
         a.long_list = W.dlists;
         a.short_list = W.dlists + W.lists_cap;
         a.list_counts = W.dctl + 4;
+        a.bucket_base = W.dctl + 48;
+        a.bucket_cnt = W.dctl + 8; /* the scatter cursors end at the counts */
     }
     if (p->mode == BBT_ORDERED)
         k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
