@@ -1528,7 +1528,13 @@ bbt_status launch_walk(const Launch &L) {
         if (L.blocks_per_cu)
             per_cu = std::min<int>(per_cu, int(L.blocks_per_cu));
         uint64_t blocks = uint64_t(L.cu) * uint64_t(per_cu);
-        const uint64_t need = (L.a.nitems + kBlock - 1) / kBlock;
+        /* lane walks take 64 items per wave; when length buckets exist, any
+         * item may be a whole-wave walk, which takes a wave of its own, so a
+         * small batch of long reads still gets one wave per item */
+        const uint64_t waves_per_block = kBlock / 64;
+        const uint64_t need = L.a.list_counts
+                                  ? (L.a.nitems + waves_per_block - 1) / waves_per_block
+                                  : (L.a.nitems + kBlock - 1) / kBlock;
         blocks = std::max<uint64_t>(1, std::min(blocks, need));
         k_walk_persistent<SC, EXH, K><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a);
     }
