@@ -134,6 +134,22 @@ BBT_HD uint64_t fastmod(uint64_t n, uint64_t d, uint64_t magic, uint32_t sh1,
     return n - q * d;
 }
 
+/* Position of the n-th set bit (n >= 1) of x, or 64 when x has fewer. */
+BBT_HD uint32_t nth_bit(uint64_t x, uint32_t n) {
+    if (n == 0 || uint32_t(__builtin_popcountll(x)) < n)
+        return 64;
+    uint32_t pos = 0;
+    for (uint32_t w = 32; w > 0; w >>= 1) {
+        const uint32_t c = uint32_t(__builtin_popcountll(x & ((uint64_t(1) << w) - 1)));
+        if (c < n) {
+            n -= c;
+            x >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 /* Device-resident description of one Bloom filter. */
 struct FilterDev {
     const uint8_t *bits;     /* m/8 bytes, bit p = byte p>>3, mask 1<<(p&7) */
@@ -566,6 +582,50 @@ struct Walker {
                     const uint32_t r = x ? uint32_t(__builtin_ctzll(x)) : 64 - L;
                     const uint64_t run =
                         r >= 64 ? ~uint64_t(0) : ((uint64_t(1) << r) - 1) << L;
+                    if ((SC == SIMPLE || SC == BINOMIAL) && !hit && streak < streak_thr &&
+                        streak_thr >= 2) {
+                        /* a miss, then isolated hits among misses: each hit
+                         * follows a miss, so it scores as the first of a
+                         * streak (SIMPLE +0.5, BINOMIAL +1), and the miss after
+                         * it finds streak 1 < streakThreshold (SeqEval.h:78-98).
+                         * Consumed in one step up to the first of two adjacent
+                         * hits (included) or the end of the contiguous valid
+                         * windows, stopping at the hit that accepts or the
+                         * miss that rejects. */
+                        const uint64_t cv = ~(vm >> L);
+                        const uint32_t cl = cv ? uint32_t(__builtin_ctzll(cv)) : 64 - L;
+                        const uint64_t cm = cl >= 64 ? ~uint64_t(0) : (uint64_t(1) << cl) - 1;
+                        const uint64_t hm = (hmask >> L) & cm;
+                        const uint64_t adj = hm & (hm >> 1);
+                        const uint32_t R = adj ? uint32_t(__builtin_ctzll(adj)) + 1 : cl;
+                        const uint64_t rm = R >= 64 ? ~uint64_t(0) : (uint64_t(1) << R) - 1;
+                        const uint64_t hits_r = hm & rm, miss_r = ~hm & rm;
+                        const uint64_t dust_r = (dmask >> L) & rm;
+                        if (!EXH) {
+                            const uint32_t nh = thr2 > score2 ? thr2 - score2 : 1u;
+                            const uint32_t nm = anti_thr > anti ? anti_thr - anti : 1u;
+                            const uint32_t ph = nth_bit(hits_r, nh), pm = nth_bit(miss_r, nm);
+                            if (ph < 64 || pm < 64) {
+                                const uint32_t e = ph < pm ? ph : pm;
+                                const uint64_t em =
+                                    e >= 63 ? ~uint64_t(0) : (uint64_t(1) << (e + 1)) - 1;
+                                probes += e + 1 - uint32_t(__builtin_popcountll(dust_r & em));
+                                lookups += lk(L, L + e + 1);
+                                score2 += uint32_t(__builtin_popcountll(hits_r & em));
+                                anti += uint32_t(__builtin_popcountll(miss_r & em));
+                                state = ph < pm ? W_ACCEPT : W_REJECT;
+                                return;
+                            }
+                            anti += uint32_t(__builtin_popcountll(miss_r));
+                        }
+                        score2 += uint32_t(__builtin_popcountll(hits_r));
+                        probes += R - uint32_t(__builtin_popcountll(dust_r));
+                        lookups += lk(L, L + R);
+                        streak = uint32_t((hits_r >> (R - 1)) & 1);
+                        prev = pw + R - 1;
+                        vm &= ~(rm << L);
+                        continue;
+                    }
                     if (!hit && streak < streak_thr) {
                         if (!EXH && anti_thr <= anti + r) {
                             const uint32_t n = anti_thr > anti ? anti_thr - anti : 1u;
