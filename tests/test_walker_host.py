@@ -200,3 +200,52 @@ def test_walker_dust_matches_oracle(W, wave):
                 ws = [O.eval_score(f, r, p)[0] for r in reads]
                 got = scores.view(np.uint64).tolist()
                 assert got == np.array(ws, np.float64).view(np.uint64).tolist()
+
+
+@pytest.mark.parametrize("wave", [False, True])
+@pytest.mark.parametrize("m,h", [(8192, 2), (12288, 2), (6000, 1)])
+def test_walker_sparse_hits_matches_oracle(W, m, h, wave):
+    """Isolated hits among misses, the pattern the whole-wave walk replays in one
+    step for SIMPLE and BINOMIAL when streakThreshold >= 2: a dense filter gives
+    false positives at 20-50% of windows, and genome reads mutated every few
+    bases give runs of single hits.  Decisions and probe/lookup counts equal the
+    oracle's for streak thresholds 2..6 and thresholds that accept or reject
+    on the first, middle or last hits of a read."""
+    rng = random.Random(m + h)
+    k = 21
+    genome = "".join(rng.choice("ACGT") for _ in range(3000))
+    f = O.Filter(m, h, k)
+    f.insert_seq(genome)
+    reads = []
+    for i in range(90):
+        L = rng.choice([30, 64, 100, 150, 300, 700, 1300])
+        if i % 3 == 0:
+            reads.append("".join(rng.choice("ACGT") for _ in range(L)))
+            continue
+        a = rng.randrange(0, 3000 - L)
+        s = list(genome[a:a + L])
+        step = rng.choice([k - 1, k, k + 2, 2 * k])
+        for p in range(rng.randrange(step), L, step):
+            s[p] = "ACGT"[("ACGT".index(s[p]) + 1) % 4]
+            if rng.random() < 0.1:
+                s[p] = "N"
+        reads.append("".join(s))
+    top = max(len(r) for r in reads)
+    for scorer, thr in [(O.SIMPLE, 0.05), (O.SIMPLE, 0.15), (O.SIMPLE, 0.3),
+                        (O.SIMPLE, 0.6), (O.BINOMIAL, 1e-2), (O.BINOMIAL, 1e-6)]:
+        binom = None
+        if scorer == O.BINOMIAL:
+            binom = np.array([1] + [O.lib().orc_calc_min_count(fl, f.fpr, thr)
+                                    for fl in range(1, top + 1)], np.uint32)
+        for streak in [2, 3, 4, 6]:
+            p = O.params(scorer, thr, streak)
+            hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, streak, binom, wave)
+            want, probes, lookups = [], 0, 0
+            for r in reads:
+                ok, pr, lk = O.eval_read(f, r, p)
+                want.append(int(ok))
+                probes += pr
+                lookups += lk
+            assert hits.tolist() == want, (scorer, thr, streak)
+            assert (int(ctr[0]), int(ctr[1])) == (probes, lookups), (scorer, thr, streak)
+            assert 0 < sum(want) < len(want) or thr in (0.05, 0.6, 1e-2, 1e-6)
