@@ -47,6 +47,22 @@ constexpr unsigned kMaxHash = 16;
 
 enum Scorer : uint32_t { SIMPLE = 0, LENGTH = 1, HARMONIC = 2, BINOMIAL = 3 };
 
+/* A load from device (global) memory.  Pointers that reach the walk through
+ * descriptors or integer arithmetic are generic to the compiler, which then
+ * emits flat loads; those also count against the LDS wait counter, so an LDS
+ * read behind them would wait for the memory load. */
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ T ld_global(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)(p);
+}
+#else
+template <class T>
+BBT_HD T ld_global(const T *p) {
+    return *p;
+}
+#endif
+
 BBT_HD uint64_t rotl64(uint64_t v, unsigned s) {
     s &= 63;
     return s ? (v << s) | (v >> (64 - s)) : v;
@@ -86,6 +102,32 @@ BBT_HD uint64_t pfx_fwd(unsigned code, uint32_t i) { return rotr64(seed_fwd(code
 BBT_HD uint64_t pfx_rc(unsigned code, uint32_t i) { return rotl64(seed_rc(code), i & 63); }
 BBT_HD uint64_t pfx_win_fwd(uint64_t Hi, uint64_t Hb, uint32_t i) { return rotl64(Hi ^ Hb, i & 63); }
 BBT_HD uint64_t pfx_win_rc(uint64_t Gi, uint64_t Gb, uint32_t p) { return rotr64(Gi ^ Gb, p & 63); }
+
+/*
+ * Rolling ntHash terms of one base, precomputed for a fixed k.  Entry
+ * e = (o << 2) | c, o and c codes, is the pair of terms when base c enters the
+ * window and base o (k places back) leaves it:
+ *   fwd' = rotl1(fwd) ^ rotl(seed_fwd(o), k) ^ seed_fwd(c)
+ *   rc'  = rotr1(rc ^ seed_rc(o) ^ rotl(seed_rc(c), k));
+ * entries 16 + c are the terms while the window fills (no base leaves).
+ * Stored as t[2e] (forward term) and t[2e + 1] (reverse-complement term), so a
+ * step is one 16-byte table read, two one-bit rotations and four XORs.
+ */
+constexpr unsigned kRollEntries = 20;
+
+BBT_HD void roll_fill(uint64_t *t, unsigned k) {
+    for (unsigned e = 0; e < kRollEntries; ++e) {
+        const unsigned c = e & 3u;
+        uint64_t tf = seed_fwd(c), tr = rotl64(seed_rc(c), k);
+        if (e < 16) {
+            const unsigned o = e >> 2;
+            tf ^= rotl64(seed_fwd(o), k);
+            tr ^= seed_rc(o);
+        }
+        t[2 * e] = tf;
+        t[2 * e + 1] = tr;
+    }
+}
 
 /* j-th value of ntHash's multi-hash expansion of the canonical hash h0:
  * t = h0 * (j ^ k*multiSeed); t ^ (t >> 27); j = 0 is h0 itself.  kseed is
@@ -165,6 +207,7 @@ struct FilterDev {
      * [fl (fl + 1) / 2 + c] for 1 <= fl <= bscore_frames, 0 <= c <= fl */
     const double *bscore;
     uint64_t mult[kMaxHash]; /* mult[j] = j ^ (k * kMultiSeed), j >= 1 */
+    uint64_t roll[2 * kRollEntries]; /* roll_fill(roll, k) */
 };
 
 /* Per-walk parameters that are uniform over a launch. */
@@ -189,13 +232,16 @@ struct Walker {
     const uint8_t *s;
     uint32_t len;
     uint32_t i; /* next base */
-    uint64_t word;  /* 8 aligned bytes holding base i */
-    uint64_t next;  /* the following 8 bytes, loaded one word ahead */
-    uint64_t waddr; /* address of word */
+    const uint8_t *sw; /* s rounded down to 8 bytes */
+    uint32_t mis;      /* s - sw */
+    uint32_t wi;       /* index of word from sw */
+    uint64_t word;     /* 8 aligned bytes holding base i */
+    uint64_t next;     /* the following 8 bytes, loaded one word ahead */
     /* rolling hash */
     uint64_t fwd, rc, win;
     uint32_t run;
     uint32_t k;
+    const uint64_t *rt; /* the filter's roll terms for k */
     /* filter */
     const uint8_t *bits;
     uint64_t m, magic;
@@ -232,13 +278,14 @@ struct Walker {
         s = seq;
         len = length;
         i = 0;
-        waddr = reinterpret_cast<uint64_t>(seq) & ~uint64_t(7);
-        word = length ? *reinterpret_cast<const uint64_t *>(waddr) : 0;
-        next = waddr + 8 < reinterpret_cast<uint64_t>(seq) + length
-                   ? *reinterpret_cast<const uint64_t *>(waddr + 8)
-                   : 0;
+        mis = uint32_t(reinterpret_cast<uintptr_t>(seq) & 7u);
+        sw = seq - mis;
+        wi = 0;
+        word = length ? ld_global(reinterpret_cast<const uint64_t *>(sw)) : 0;
+        next = 8u < length + mis ? ld_global(reinterpret_cast<const uint64_t *>(sw + 8)) : 0;
         fwd = rc = win = 0;
         run = 0;
+        rt = f->roll;
         bits = f->bits;
         m = f->m;
         magic = f->magic;
@@ -298,21 +345,21 @@ struct Walker {
         }
     }
 
-    /* byte i of the read through an 8-byte aligned register window */
-    /* byte idx of the read; idx advances by one per call, so the word
-     * after the current one is loaded a word ahead of its use (loads stay
-     * inside the read) */
+    /* byte idx of the read through an 8-byte aligned register window; idx
+     * advances by one per call, so the word after the current one is loaded
+     * a word ahead of its use.  Only words that hold bytes of the read are
+     * loaded. */
     BBT_HD unsigned base_at(uint32_t idx) {
-        const uint64_t a = reinterpret_cast<uint64_t>(s) + idx;
-        const uint64_t wa = a & ~uint64_t(7);
-        if (wa != waddr) {
-            waddr = wa;
+        const uint32_t a = idx + mis;
+        const uint32_t w = a >> 3;
+        if (w != wi) {
+            wi = w;
             word = next;
-            next = wa + 8 < reinterpret_cast<uint64_t>(s) + len
-                       ? *reinterpret_cast<const uint64_t *>(wa + 8)
+            next = 8u * (w + 1) < len + mis
+                       ? ld_global(reinterpret_cast<const uint64_t *>(sw + 8u * (w + 1)))
                        : 0;
         }
-        return unsigned(word >> ((a & 7) * 8)) & 0xFFu;
+        return unsigned(word >> ((a & 7u) * 8u)) & 0xFFu;
     }
 
     BBT_HD bool low_complexity(uint32_t pw) const {
@@ -359,15 +406,12 @@ struct Walker {
                 continue;
             }
             const unsigned code = base_code(c);
-            if (run >= kv) {
-                const unsigned out = unsigned(win >> (2 * (kv - 1))) & 3u;
-                fwd = rotl1(fwd) ^ rotl64(seed_fwd(out), kv) ^ seed_fwd(code);
-                rc = rotr1(rc ^ seed_rc(out) ^ rotl64(seed_rc(code), kv));
-            } else {
-                fwd = rotl1(fwd) ^ seed_fwd(code);
-                rc = rotr1(rc ^ rotl64(seed_rc(code), kv));
-                ++run;
-            }
+            const bool full = run >= kv;
+            const unsigned e =
+                (full ? (unsigned(win >> (2 * (kv - 1))) & 3u) << 2 : 16u) | code;
+            fwd = rotl1(fwd) ^ rt[2 * e];
+            rc = rotr1(rc ^ rt[2 * e + 1]);
+            run += full ? 0u : 1u;
             win = (win << 2) | code;
             if (run < kv)
                 continue;

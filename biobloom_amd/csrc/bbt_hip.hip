@@ -173,10 +173,23 @@ __device__ __forceinline__ void store_walk(const EvalArgs &a, uint64_t read, uin
  * filters, whose lines then stay in each XCD's L2: random 128 B lines come
  * back about 1.5x faster from a 16 MiB working set than from hundreds of MiB.
  * The plain path keeps the input order, read-major (nr = 0). */
+/* The roll terms of a launch whose filters share a compile-time k, copied
+ * into LDS: the lane walks read one entry per base, and LDS keeps those reads
+ * off the vector memory path that carries the filter loads.  With a runtime
+ * k every walk reads its own filter's table (FilterDev::roll). */
+template <int K>
+__device__ __forceinline__ const uint64_t *roll_lds(const EvalArgs &a, uint64_t *s_roll) {
+    if (K) {
+        for (int t = threadIdx.x; t < int(2 * kRollEntries); t += blockDim.x)
+            s_roll[t] = a.filters[0].roll[t];
+    }
+    return s_roll;
+}
+
 template <int SC, bool EXH, int K>
 __device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item, uint64_t nr,
                                            Walker<SC, EXH, K> &w, uint64_t &read,
-                                           uint32_t &f) {
+                                           uint32_t &f, const uint64_t *s_roll) {
     uint64_t r;
     if (nr) {
         f = uint32_t(item / nr);
@@ -188,12 +201,17 @@ __device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item, uin
     read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+    if (K)
+        w.rt = s_roll;
     w.dm = a.dust;
     w.dbase = b;
 }
 
 template <int SC, bool EXH, int K>
 __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
+    __shared__ uint64_t s_roll[2 * kRollEntries];
+    const uint64_t *roll = roll_lds<K>(a, s_roll);
+    __syncthreads();
     const uint64_t item = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     Walker<SC, EXH, K> w;
     w.probes = 0;
@@ -201,9 +219,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
     if (item < a.nitems) {
         uint64_t read;
         uint32_t f;
-        start_walk(a, item, 0, w, read, f);
+        start_walk(a, item, 0, w, read, f, roll);
         while (w.advance())
-            w.consume(*w.addr);
+            w.consume(ld_global(w.addr));
         store_walk(a, read, f, w);
     }
     add_counters(a.counters, w.probes, w.lookups);
@@ -286,7 +304,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
             if (need) {
                 const uint64_t hv = multi_hash(h0, jj, w.kseed());
                 const uint64_t pos = fastmod(hv, w.m, w.magic, w.sh1, w.sh2);
-                const unsigned byte = w.bits[pos >> 3];
+                const unsigned byte = ld_global(w.bits + (pos >> 3));
                 ++lk;
                 if (!(byte & (1u << unsigned(pos & 7)))) {
                     allset = false;
@@ -337,6 +355,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
      * time, as the lane walks do (uneven workload: 88.5 -> 85.4 ms against
      * read-major order) */
     __shared__ unsigned long long s_bbase[32], s_bcnt[32];
+    __shared__ uint64_t s_roll[2 * kRollEntries];
+    const uint64_t *roll = roll_lds<K>(a, s_roll);
     if (nlong) {
         for (int t = threadIdx.x; t < 32; t += blockDim.x) {
             s_bbase[t] = a.bucket_base[t];
@@ -398,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             if (!active) {
                 const uint64_t item = base + uint64_t(__popcll(idle & lt_mask));
                 if (item < nitems) {
-                    start_walk(a, item, nr, w, read, f);
+                    start_walk(a, item, nr, w, read, f, roll);
                     active = true;
                 }
             }
@@ -424,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                         if (jj < hh) {
                             const uint64_t pos =
                                 fastmod(multi_hash(w.h0, jj, ks), w.m, w.magic, w.sh1, w.sh2);
-                            bytes[jj] = w.bits[pos >> 3];
+                            bytes[jj] = ld_global(w.bits + (pos >> 3));
                         }
                     }
                     const uint32_t before = w.lookups;
@@ -434,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                             w.consume(bytes[jj]);
                     ahead_extra += hh - (w.lookups - before);
                 } else {
-                    const unsigned byte = *w.addr;
+                    const unsigned byte = ld_global(w.addr);
                     w.consume(byte);
                 }
             } else {
@@ -557,13 +577,15 @@ __global__ __launch_bounds__(kBlock) void k_len_scatter(const uint64_t *off, uin
 
 template <int SC, int K>
 __device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
-                         Walker<SC, false, K> &w) {
+                         Walker<SC, false, K> &w, const uint64_t *s_roll) {
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
+    if (K)
+        w.rt = s_roll;
     w.dm = a.dust;
     w.dbase = b;
     while (w.advance())
-        w.consume(*w.addr);
+        w.consume(ld_global(w.addr));
     return w.accepted();
 }
 
@@ -572,6 +594,9 @@ __device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
 template <int SC, int K>
 __global__ __launch_bounds__(kBlock) void k_ordered(EvalArgs a, uint64_t nrec,
                                                     uint32_t paired, uint32_t inclusive) {
+    __shared__ uint64_t s_roll[2 * kRollEntries];
+    const uint64_t *roll = roll_lds<K>(a, s_roll);
+    __syncthreads();
     const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     Walker<SC, false, K> w;
     w.probes = 0;
@@ -581,11 +606,11 @@ __global__ __launch_bounds__(kBlock) void k_ordered(EvalArgs a, uint64_t nrec,
         for (uint32_t f = 0; f < a.nf; ++f) {
             bool pass;
             if (!paired)
-                pass = walk_one(a, r1, f, w);
+                pass = walk_one(a, r1, f, w, roll);
             else if (inclusive)
-                pass = walk_one(a, r1, f, w) || walk_one(a, r1 + 1, f, w);
+                pass = walk_one(a, r1, f, w, roll) || walk_one(a, r1 + 1, f, w, roll);
             else
-                pass = walk_one(a, r1, f, w) && walk_one(a, r1 + 1, f, w);
+                pass = walk_one(a, r1, f, w, roll) && walk_one(a, r1 + 1, f, w, roll);
             if (pass) {
                 a.hits[r1 * a.nf + f] = 1;
                 if (paired)
@@ -1054,6 +1079,7 @@ static void fill_desc(FilterDev &d, void *bits, uint64_t m, uint32_t h, uint32_t
     d.k = k;
     for (uint32_t j = 1; j < kMaxHash; ++j)
         d.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
+    roll_fill(d.roll, k);
 }
 
 static bbt_status sync_desc(bbt_filterset *fs) {

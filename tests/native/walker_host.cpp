@@ -153,6 +153,7 @@ int walker_host_wave_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t 
     f.binom_len = binom_len;
     for (uint32_t j = 1; j < kMaxHash; ++j)
         f.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
+    roll_fill(f.roll, k);
     WalkParams wp;
     wp.threshold = threshold;
     wp.streak = streak;
@@ -196,6 +197,7 @@ int walker_host_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
     f.binom_len = binom_len;
     for (uint32_t j = 1; j < kMaxHash; ++j)
         f.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
+    roll_fill(f.roll, k);
     WalkParams wp;
     wp.threshold = threshold;
     wp.streak = streak;
