@@ -77,8 +77,10 @@ BBT_HD uint64_t rotr1(uint64_t v) { return (v >> 1) | (v << 63); }
 /* ASCII -> 2-bit code with (c >> 1) & 3: A 0, C 1, T 2, G 3 in either case;
  * the complement of a code is code ^ 2. */
 BBT_HD bool base_valid(unsigned c) {
-    unsigned u = c & 0xDFu;
-    return u == 'A' || u == 'C' || u == 'G' || u == 'T';
+    /* (c & 0xDF) - 'A' is 0, 2, 6 or 19 for A, C, G, T in either case: one
+     * bit test of 0x80045, branch-free (a chain of compares becomes branches) */
+    const unsigned d = (c & 0xDFu) - unsigned('A');
+    return (d < 20u) & ((0x80045u >> (d & 31u)) & 1u);
 }
 BBT_HD unsigned base_code(unsigned c) { return (c >> 1) & 3u; }
 
@@ -235,6 +237,7 @@ struct Walker {
     const uint8_t *sw; /* s rounded down to 8 bytes */
     uint32_t mis;      /* s - sw */
     uint32_t wi;       /* index of word from sw */
+    uint32_t lastw;    /* index of the word holding the read's last byte */
     uint64_t word;     /* 8 aligned bytes holding base i */
     uint64_t next;     /* the following 8 bytes, loaded one word ahead */
     /* rolling hash */
@@ -281,8 +284,12 @@ struct Walker {
         mis = uint32_t(reinterpret_cast<uintptr_t>(seq) & 7u);
         sw = seq - mis;
         wi = 0;
-        word = length ? ld_global(reinterpret_cast<const uint64_t *>(sw)) : 0;
-        next = 8u < length + mis ? ld_global(reinterpret_cast<const uint64_t *>(sw + 8)) : 0;
+        lastw = length ? (length + mis - 1) >> 3 : 0;
+        word = next = 0;
+        if (length) {
+            word = ld_global(reinterpret_cast<const uint64_t *>(sw));
+            next = ld_global(reinterpret_cast<const uint64_t *>(sw + 8u * (lastw ? 1u : 0u)));
+        }
         fwd = rc = win = 0;
         run = 0;
         rt = f->roll;
@@ -348,16 +355,16 @@ struct Walker {
     /* byte idx of the read through an 8-byte aligned register window; idx
      * advances by one per call, so the word after the current one is loaded
      * a word ahead of its use.  Only words that hold bytes of the read are
-     * loaded. */
+     * loaded: past the last one the last is loaded again, so the load has no
+     * alternative value and stays in flight until the word is used. */
     BBT_HD unsigned base_at(uint32_t idx) {
         const uint32_t a = idx + mis;
         const uint32_t w = a >> 3;
         if (w != wi) {
             wi = w;
             word = next;
-            next = 8u * (w + 1) < len + mis
-                       ? ld_global(reinterpret_cast<const uint64_t *>(sw + 8u * (w + 1)))
-                       : 0;
+            const uint32_t nw = w + 1 < lastw ? w + 1 : lastw;
+            next = ld_global(reinterpret_cast<const uint64_t *>(sw + 8u * nw));
         }
         return unsigned(word >> ((a & 7u) * 8u)) & 0xFFu;
     }
@@ -385,6 +392,66 @@ struct Walker {
             state = W_REJECT;
     }
 
+    /* Rolls the base with 2-bit code into the window hash; true once the
+     * window holds k bases. */
+    BBT_HD bool roll_in(unsigned code) {
+        const unsigned kv = kk();
+        const bool full = run >= kv;
+        const unsigned e = (full ? (unsigned(win >> (2 * (kv - 1))) & 3u) << 2 : 16u) | code;
+        fwd = rotl1(fwd) ^ rt[2 * e];
+        rc = rotr1(rc ^ rt[2 * e + 1]);
+        run += full ? 0u : 1u;
+        win = (win << 2) | code;
+        return run >= kv;
+    }
+
+    /* One base of the walk, the flat form of advance(): true when the window
+     * ending at this base needs a contains() call (h0 set, pending); the
+     * caller then hands the call's bits back with take(), in hash order,
+     * until pending clears.  Otherwise the base only rolled the hash, or
+     * consumed a window without a probe, or the walk ended (state). */
+    BBT_HD bool step() {
+        if (state != W_RUNNING)
+            return false;
+        if (i >= len) {
+            finish_end();
+            return false;
+        }
+        const unsigned kv = kk();
+        const unsigned c = base_at(i);
+        ++i;
+        if (!base_valid(c)) {
+            run = 0;
+            fwd = rc = 0;
+            return false;
+        }
+        if (!roll_in(base_code(c)))
+            return false;
+        p = i - kv;
+        if (window_pre(p) != PRE_PROBE)
+            return false;
+        if (low_complexity(p)) {
+            on_result(false); /* -D: a miss without a contains() call */
+            return false;
+        }
+        h0 = rc < fwd ? rc : fwd;
+        j = 0;
+        ++probes;
+        pending = 1;
+        return true;
+    }
+
+    /* The bit of hash j of the pending contains() call (step()). */
+    BBT_HD void take(bool bit) {
+        ++lookups;
+        if (bit && j + 1 < h) {
+            ++j;
+            return;
+        }
+        pending = 0;
+        on_result(bit);
+    }
+
     /* Advance until a filter byte is needed (returns true, addr/mask set) or
      * the walk has ended (returns false). */
     BBT_HD bool advance() {
@@ -405,15 +472,7 @@ struct Walker {
                 fwd = rc = 0;
                 continue;
             }
-            const unsigned code = base_code(c);
-            const bool full = run >= kv;
-            const unsigned e =
-                (full ? (unsigned(win >> (2 * (kv - 1))) & 3u) << 2 : 16u) | code;
-            fwd = rotl1(fwd) ^ rt[2 * e];
-            rc = rotr1(rc ^ rt[2 * e + 1]);
-            run += full ? 0u : 1u;
-            win = (win << 2) | code;
-            if (run < kv)
+            if (!roll_in(base_code(c)))
                 continue;
             p = i - kv; /* start of the N-free window ending at base i-1 */
             const int pre = window_pre(p);

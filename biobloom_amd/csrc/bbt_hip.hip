@@ -429,35 +429,36 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             continue;
         }
         if (active) {
-            if (w.advance()) {
-                if (a.probe_ahead && w.j == 0) {
-                    /* probe unroll: the first kAhead lookups of this contains()
-                     * call as independent loads, consumed in order until the
-                     * call ends (a clear bit or the last hash); the loads past
-                     * that are issued-only */
-                    const uint32_t hh = w.h < kAhead ? w.h : kAhead;
-                    const uint64_t ks = w.kseed();
-                    unsigned bytes[kAhead];
+            /* one base per lane per pass; a window that needs a contains()
+             * call gets its bits na at a time (probe unroll: kAhead
+             * independent loads), taken in hash order until the call ends at
+             * a clear bit or the last hash; loads past that end are
+             * issued-only */
+            if (w.step()) {
+                const uint32_t na = a.probe_ahead ? kAhead : 1u;
+                const uint64_t ks = w.kseed();
+                for (uint32_t j0 = 0; w.pending; j0 += na) {
+                    bool bit[kAhead];
+                    uint32_t n = 0;
 #pragma unroll
                     for (uint32_t jj = 0; jj < kAhead; ++jj) {
-                        bytes[jj] = 0;
-                        if (jj < hh) {
-                            const uint64_t pos =
-                                fastmod(multi_hash(w.h0, jj, ks), w.m, w.magic, w.sh1, w.sh2);
-                            bytes[jj] = ld_global(w.bits + (pos >> 3));
+                        bit[jj] = false;
+                        if (jj < na && j0 + jj < w.h) {
+                            const uint64_t pos = fastmod(multi_hash(w.h0, j0 + jj, ks), w.m,
+                                                         w.magic, w.sh1, w.sh2);
+                            bit[jj] = (ld_global(w.bits + (pos >> 3)) >> unsigned(pos & 7)) & 1u;
+                            ++n;
                         }
                     }
                     const uint32_t before = w.lookups;
 #pragma unroll
                     for (uint32_t jj = 0; jj < kAhead; ++jj)
-                        if (jj < hh && w.pending)
-                            w.consume(bytes[jj]);
-                    ahead_extra += hh - (w.lookups - before);
-                } else {
-                    const unsigned byte = ld_global(w.addr);
-                    w.consume(byte);
+                        if (jj < n && w.pending)
+                            w.take(bit[jj]);
+                    ahead_extra += n - (w.lookups - before);
                 }
-            } else {
+            }
+            if (w.state != W_RUNNING) {
                 store_walk(a, read, f, w);
                 active = false;
             }

@@ -13,6 +13,9 @@ using namespace bbt;
 namespace {
 /* -D bitmap of the batch (bit off[read] + p), or null */
 const uint32_t *g_dust = nullptr;
+/* lane walk form: 0 advance()/consume(); n > 0 step()/take() with the bits of
+ * a contains() call fetched n at a time, as the persistent kernel does */
+unsigned g_flat = 0;
 
 template <int SC, bool EXH>
 void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t nreads,
@@ -24,8 +27,25 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
         w.init(seq + off[r], uint32_t(off[r + 1] - off[r]), &f, wp);
         w.dm = g_dust;
         w.dbase = off[r];
-        while (w.advance())
-            w.consume(*w.addr);
+        if (g_flat) {
+            while (w.state == W_RUNNING) {
+                if (!w.step())
+                    continue;
+                for (uint32_t j0 = 0; w.pending; j0 += g_flat) {
+                    bool bit[16];
+                    for (uint32_t jj = 0; jj < g_flat && j0 + jj < f.h; ++jj) {
+                        const uint64_t pos = fastmod(multi_hash(w.h0, j0 + jj, w.kseed()), f.m,
+                                                     f.magic, f.sh1, f.sh2);
+                        bit[jj] = (f.bits[pos >> 3] >> (pos & 7)) & 1u;
+                    }
+                    for (uint32_t jj = 0; jj < g_flat && j0 + jj < f.h && w.pending; ++jj)
+                        w.take(bit[jj]);
+                }
+            }
+        } else {
+            while (w.advance())
+                w.consume(*w.addr);
+        }
         if (EXH)
             scores[r] = w.score_value();
         else
@@ -231,4 +251,17 @@ void walker_host_fastmod(uint64_t d, const uint64_t *n, uint64_t count, uint64_t
     for (uint64_t i = 0; i < count; ++i)
         out[i] = fastmod(n[i], fm.d, fm.magic, fm.sh1, fm.sh2);
 }
+
+void walker_host_set_flat(unsigned n) { g_flat = n < 16 ? n : 16; }
+
+/* base_valid and base_code for every byte value */
+void walker_host_bases(uint8_t *valid, uint8_t *code) {
+    for (unsigned c = 0; c < 256; ++c) {
+        valid[c] = base_valid(c) ? 1 : 0;
+        code[c] = uint8_t(base_code(c));
+    }
+}
+
+/* the roll table of k */
+void walker_host_roll(unsigned k, uint64_t *t) { roll_fill(t, k); }
 }

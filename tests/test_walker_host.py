@@ -40,6 +40,9 @@ def W():
     L.walker_host_wave_eval.restype = C.c_int
     L.walker_host_wave_eval.argtypes = L.walker_host_eval.argtypes
     L.walker_host_fastmod.argtypes = [C.c_uint64, p, C.c_uint64, p]
+    L.walker_host_bases.argtypes = [p, p]
+    L.walker_host_set_flat.argtypes = [C.c_uint]
+    L.walker_host_roll.argtypes = [C.c_uint, p]
     return L
 
 
@@ -82,6 +85,50 @@ def test_fastmod_exact(W):
         assert [int(x) for x in out] == [n % d for n in ns], d
 
 
+def test_bases_every_byte(W):
+    """Only A, C, G, T in either case are bases (ntHashIterator restarts at
+    any other byte), coded A 0, C 1, T 2, G 3."""
+    valid = np.zeros(256, np.uint8)
+    code = np.zeros(256, np.uint8)
+    W.walker_host_bases(valid.ctypes.data_as(C.c_void_p), code.ctypes.data_as(C.c_void_p))
+    want = {ord(b): i for i, b in enumerate("ACTG")}
+    want.update({ord(b.lower()): i for b, i in list(zip("ACTG", range(4)))})
+    assert [c for c in range(256) if valid[c]] == sorted(want)
+    assert all(int(code[c]) == v for c, v in want.items())
+
+
+def rotl(v, s):
+    s %= 64
+    return ((v << s) | (v >> (64 - s))) & (2**64 - 1) if s else v
+
+
+def test_roll_table_rolls(W):
+    """Rolling with the table's terms equals ntHash computed from scratch on
+    every window (forward: XOR of rotl(seed(b_j), k-1-j); reverse complement:
+    XOR of rotl(seed(comp b_j), j)), for k below, at and above 64."""
+    seeds = {"A": 0x3c8bfbb395c60474, "C": 0x3193c18562a02b4c,
+             "G": 0x20323ed082572324, "T": 0x295549f54be24456}
+    comp = {"A": "T", "C": "G", "G": "C", "T": "A"}
+    code = {"A": 0, "C": 1, "T": 2, "G": 3}
+    rng = random.Random(5)
+    for k in (1, 5, 25, 31, 63, 64, 65, 96):
+        t = np.zeros(40, np.uint64)
+        W.walker_host_roll(k, t.ctypes.data_as(C.c_void_p))
+        s = "".join(rng.choice("ACGT") for _ in range(k + 40))
+        fwd = rc = 0
+        for i, b in enumerate(s):
+            e = (code[s[i - k]] << 2 | code[b]) if i >= k else 16 + code[b]
+            fwd = rotl(fwd, 1) ^ int(t[2 * e])
+            rc = rotl(rc ^ int(t[2 * e + 1]), 63)
+            if i >= k - 1:
+                w = s[i - k + 1:i + 1]
+                f0 = r0 = 0
+                for j, x in enumerate(w):
+                    f0 ^= rotl(seeds[x], k - 1 - j)
+                    r0 ^= rotl(seeds[comp[x]], j)
+                assert (fwd, rc) == (f0, r0), (k, i)
+
+
 def make_filter_and_reads(rng, k, h, m, nreads=150):
     genome = "".join(rng.choice("ACGT") for _ in range(3000))
     f = O.Filter(m, h, k)
@@ -107,10 +154,21 @@ def make_filter_and_reads(rng, k, h, m, nreads=150):
     return f, reads
 
 
+@pytest.fixture(params=[0, 1, 4], ids=["advance", "step1", "step4"])
+def flat(W, request):
+    """lane walk form: advance()/consume(), or step()/take() with a call's
+    bits fetched one or four at a time"""
+    W.walker_host_set_flat(request.param)
+    yield request.param
+    W.walker_host_set_flat(0)
+
+
 @pytest.mark.parametrize("wave", [False, True])
 @pytest.mark.parametrize("k,h,m", [(25, 3, 1 << 16), (25, 4, 1 << 14), (11, 2, 4096),
                                    (5, 1, 1 << 12), (31, 3, 1 << 18), (32, 5, 1 << 17)])
-def test_walker_matches_oracle(W, k, h, m, wave):
+def test_walker_matches_oracle(W, k, h, m, wave, flat):
+    if wave and flat:
+        pytest.skip("the lane walk form does not apply to whole-wave walks")
     rng = random.Random(k * 1000 + h)
     f, reads = make_filter_and_reads(rng, k, h, m)
     if wave:
@@ -170,7 +228,9 @@ def dust_bitmap(reads, T, W):
 
 
 @pytest.mark.parametrize("wave", [False, True])
-def test_walker_dust_matches_oracle(W, wave):
+def test_walker_dust_matches_oracle(W, wave, flat):
+    if wave and flat:
+        pytest.skip("the lane walk form does not apply to whole-wave walks")
     """-D: the device walk (lane and whole-wave) with the sdust bitmap equals
     the oracle's dust walks, every scorer, counts included."""
     rng = random.Random(808 + wave)
@@ -204,7 +264,9 @@ def test_walker_dust_matches_oracle(W, wave):
 
 @pytest.mark.parametrize("wave", [False, True])
 @pytest.mark.parametrize("m,h", [(8192, 2), (12288, 2), (6000, 1)])
-def test_walker_sparse_hits_matches_oracle(W, m, h, wave):
+def test_walker_sparse_hits_matches_oracle(W, m, h, wave, flat):
+    if wave and flat:
+        pytest.skip("the lane walk form does not apply to whole-wave walks")
     """Isolated hits among misses, the pattern the whole-wave walk replays in one
     step for SIMPLE and BINOMIAL when streakThreshold >= 2: a dense filter gives
     false positives at 20-50% of windows, and genome reads mutated every few
