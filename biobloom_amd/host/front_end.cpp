@@ -56,6 +56,7 @@ static void bbt_check(bbt_status s) {
 struct BioBloomClassifier::Batch {
     hipStream_t st = nullptr;
     hipEvent_t walked = nullptr; /* recorded after the slot's kernels */
+    hipEvent_t started = nullptr; /* before them, once the other slot's are done */
     bool walk_recorded = false;
     bbt_workspace *ws = nullptr;
     uint32_t nf = 0;
@@ -145,7 +146,8 @@ BioBloomClassifier::BioBloomClassifier(const std::vector<std::string> &paths,
         b->nf = m_nf;
         b->scores = m_opt.mode == BBT_BESTHIT || m_opt.mode == BBT_SCORES;
         HIP_CHECK(hipStreamCreate(&b->st));
-        HIP_CHECK(hipEventCreateWithFlags(&b->walked, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreate(&b->walked));
+        HIP_CHECK(hipEventCreate(&b->started));
         bbt_check(bbt_workspace_create(&b->ws));
     }
     HIP_CHECK(hipMalloc(&m_dsummary, (2 * m_nf + 2) * 8));
@@ -162,6 +164,7 @@ BioBloomClassifier::~BioBloomClassifier() {
         (void)hipStreamSynchronize(b->st);
         b->release();
         (void)hipEventDestroy(b->walked);
+        (void)hipEventDestroy(b->started);
         bbt_workspace_free(b->ws);
         (void)hipStreamDestroy(b->st);
         delete b;
@@ -249,6 +252,7 @@ void BioBloomClassifier::submit(int s) {
     HIP_CHECK(hipMemcpyAsync(b.d_off, b.h_off, (n + 1) * 8, hipMemcpyHostToDevice, b.st));
     if (other.walk_recorded)
         HIP_CHECK(hipStreamWaitEvent(b.st, other.walked, 0));
+    HIP_CHECK(hipEventRecord(b.started, b.st));
     bbt_reads in;
     std::memset(&in, 0, sizeof(in));
     in.seq = b.d_seq;
@@ -307,8 +311,15 @@ void BioBloomClassifier::submit(int s) {
 
 /* Waits for slot s's launch. */
 void BioBloomClassifier::finish(int s) {
-    if (m_slot[s]->n)
-        HIP_CHECK(hipStreamSynchronize(m_slot[s]->st));
+    Batch &b = *m_slot[s];
+    if (!b.n)
+        return;
+    HIP_CHECK(hipStreamSynchronize(b.st));
+    /* the device time of the batch's kernels (from the copy-in or the end of
+     * the other slot's kernels, whichever is later, to the walk's end) */
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, b.started, b.walked) == hipSuccess)
+        m_phase[5] += ms / 1e3;
 }
 
 /* Blocks one batch may hold: the three batches alive at once (one on the
@@ -700,7 +711,7 @@ void BioBloomClassifier::write_summary(uint64_t totalReads) {
         std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
     std::cerr << "Filtering time: " << secs << " s (gather " << m_phase[0] << ", pack+launch "
               << m_phase[1] << " incl. staging allocation " << m_phase[4] << ", device wait "
-              << m_phase[2] << ", output " << m_phase[3]
+              << m_phase[2] << ", output " << m_phase[3] << "; device kernels " << m_phase[5]
               << ")" << std::endl;
     m_probes = ctr[0];
     m_lookups = ctr[1];

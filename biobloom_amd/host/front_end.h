@@ -126,8 +126,8 @@ class BioBloomClassifier {
     Writer *m_writer = nullptr; /* writes formatted batches in order, off this thread */
     std::chrono::steady_clock::time_point m_t0; /* "Filtering Start" */
     /* s: gather, pack + launch, device wait, output; staging allocation (part
-     * of pack + launch) */
-    double m_phase[5] = {0, 0, 0, 0, 0};
+     * of pack + launch); device time of the batches' kernels */
+    double m_phase[6] = {0, 0, 0, 0, 0, 0};
     bool m_autotuned = false; /* --autotune ran on a batch */
 };
 

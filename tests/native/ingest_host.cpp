@@ -78,4 +78,10 @@ unsigned long pack_2bit_reads(const char *seq, const unsigned long *off, unsigne
 
 unsigned long packed_bytes_for(unsigned long nbases) { return bbt_host::packed_bytes(nbases); }
 
+/* the packer's form: -1 automatic, 0 SWAR, 1 AVX2; returns whether AVX2 runs here */
+int pack_set_simd(int mode) {
+    bbt_host::pack_simd_mode() = mode;
+    return bbt_host::pack_avx2_available() ? 1 : 0;
+}
+
 } // extern "C"

@@ -311,11 +311,18 @@ def unpack_model(packed, cb, n):
     return out[:n]
 
 
+@pytest.mark.parametrize("simd", [0, 1], ids=["swar", "avx2"])
 @pytest.mark.parametrize("threads", [1, 3, 16])
-def test_pack_2bit_matches_model(L, threads):
-    """The front ends' SWAR packer equals the layout's restatement on reads of
-    every length 0..80 (unaligned starts), upper and lower case, N and other
-    bytes; unpacking gives the text with case folded and non-ACGT as N."""
+def test_pack_2bit_matches_model(L, threads, simd):
+    """The front ends' packer (SWAR and AVX2 forms) equals the layout's
+    restatement on reads of every length 0..80 (unaligned starts), upper and
+    lower case, N and other bytes; unpacking gives the text with case folded
+    and non-ACGT as N."""
+    L.pack_set_simd.restype = C.c_int
+    L.pack_set_simd.argtypes = [C.c_int]
+    if simd and not L.pack_set_simd(-1):
+        pytest.skip("no AVX2 on this host")
+    L.pack_set_simd(simd)
     L.pack_2bit_reads.restype = C.c_ulong
     L.pack_2bit_reads.argtypes = [C.c_char_p, C.POINTER(C.c_ulong), C.c_ulong, C.c_void_p,
                                   C.c_uint]
@@ -342,3 +349,4 @@ def test_pack_2bit_matches_model(L, threads):
     folded = np.where(np.isin(text & 0xDF, np.frombuffer(b"ACGT", np.uint8)), text & 0xDF,
                       ord("N")).astype(np.uint8)
     assert np.array_equal(unpack_model(out, cb, n), folded)
+    L.pack_set_simd(-1)
