@@ -193,6 +193,40 @@ def test_uneven_lengths(dev):
                     paired=True)
 
 
+def test_runtime_k_filters(dev):
+    """Filter sets whose k is not the compiled one: all at k = 31, and k = 21,
+    31 and 25 in one set (each walk rolls its own filter's k).  Both paths,
+    lane and whole-wave walks, every scorer, against the oracle."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(31)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 120_000)) for _ in range(3)]
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 600, 150, 0.01, 0.6)
+    longs, _ = synth.long_reads(rng, genomes, 20, median=900, hi=3000)
+    reads = [bytes(x) for x in shorts] + [bytes(x) for x in longs] + [b"", b"ACGTN" * 12]
+    buf, off = O.pack_reads(reads)
+    L = B.load()
+    for ks in ([31, 31, 31], [21, 31, 25]):
+        print(f"[gpu] runtime k {ks}", flush=True)
+        h = L.bbt_calc_optimal_hashnum(0.05)
+        ms = [L.bbt_calc_optimal_size(len(g) - k + 1, 0.05, h) for g, k in zip(genomes, ks)]
+        fs = B.FilterSet.create(ms, [h] * 3, ks)
+        for i, g in enumerate(genomes):
+            dg = torch.from_numpy(np.ascontiguousarray(g)).to("cuda")
+            fs.insert(i, dg.data_ptr(), dg.numel())
+            fs.refresh(i)
+            fs.set_id(i, f"g{i}")
+            del dg
+        ofs = oracle_filters_like(fs, B)
+        for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.2), (O.LENGTH, 40.0),
+                            (O.BINOMIAL, 1e-10)]:
+            check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
+            check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr, wave_min=100)
+        check_batch(D, B, fs, ofs, buf, off, paired=True)
+        check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=O.HARMONIC, thr=0.15)
+        check_batch(D, B, fs, ofs, buf, off, mode=O.ORDERED)
+
+
 def test_store_and_load_roundtrip(dev, tmp_path):
     D, B = dev
     rng = np.random.default_rng(5)
