@@ -585,8 +585,16 @@ __device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
         w.rt = s_roll;
     w.dm = a.dust;
     w.dbase = b;
-    while (w.advance())
-        w.consume(ld_global(w.addr));
+    /* flat form: one base per pass, a contains() call's bits one at a time */
+    const uint64_t ks = w.kseed();
+    while (w.state == W_RUNNING) {
+        if (!w.step())
+            continue;
+        while (w.pending) {
+            const uint64_t pos = fastmod(multi_hash(w.h0, w.j, ks), w.m, w.magic, w.sh1, w.sh2);
+            w.take((ld_global(w.bits + (pos >> 3)) >> unsigned(pos & 7)) & 1u);
+        }
+    }
     return w.accepted();
 }
 
