@@ -212,9 +212,9 @@ def bench_walk(args, torch, D, B, name, w, steps, warmup, gathers, autotune=True
                                    wave_min_len=args.wave_min), os_, stream)
 
     tuning = None
-    if args.refill_min or args.blocks_per_cu:
+    if args.refill_min or args.blocks_per_cu or args.probe_ahead >= 0:
         fs.set_tuning(wave_min_len=args.wave_min, refill_min=args.refill_min,
-                      blocks_per_cu=args.blocks_per_cu)
+                      blocks_per_cu=args.blocks_per_cu, probe_ahead=max(0, args.probe_ahead))
         tuning = fs.get_tuning()
     elif autotune and not args.no_autotune:
         t = time.time()
@@ -1037,6 +1037,9 @@ def main():
                     help="batched path: idle lanes that trigger a queue refill")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="batched path: persistent blocks per CU (0 = occupancy)")
+    ap.add_argument("--probe-ahead", type=int, default=-1,
+                    help="batched path: lane-walk load scheduling (bit 0: a call's lookups "
+                         "4 at a time; bit 1: the next window's issued ahead)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched path: keep the library's default knobs")
     ap.add_argument("--no-gather", action="store_true",

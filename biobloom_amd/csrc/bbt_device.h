@@ -417,7 +417,19 @@ struct Walker {
             finish_end();
             return false;
         }
-        const unsigned kv = kk();
+        uint32_t pw;
+        uint64_t hv;
+        if (!roll_step(pw, hv))
+            return false;
+        return decide(pw, hv);
+    }
+
+    /* The two halves of step(), for a caller that rolls the next base (and
+     * issues its window's loads) before the pending call's bits are taken.
+     * roll_step (i < len): one base in; true when it completes an N-free
+     * window, whose start and canonical hash it returns.  It touches only the
+     * rolling hash, never the automaton. */
+    BBT_HD bool roll_step(uint32_t &pw, uint64_t &hv) {
         const unsigned c = base_at(i);
         ++i;
         if (!base_valid(c)) {
@@ -427,14 +439,21 @@ struct Walker {
         }
         if (!roll_in(base_code(c)))
             return false;
-        p = i - kv;
-        if (window_pre(p) != PRE_PROBE)
+        pw = i - kk();
+        hv = rc < fwd ? rc : fwd;
+        return true;
+    }
+
+    /* decide: window pw (hash hv) through the automaton, after every earlier
+     * window's result; true when it needs a contains() call (pending). */
+    BBT_HD bool decide(uint32_t pw, uint64_t hv) {
+        if (window_pre(pw) != PRE_PROBE)
             return false;
-        if (low_complexity(p)) {
+        if (low_complexity(pw)) {
             on_result(false); /* -D: a miss without a contains() call */
             return false;
         }
-        h0 = rc < fwd ? rc : fwd;
+        h0 = hv;
         j = 0;
         ++probes;
         pending = 1;

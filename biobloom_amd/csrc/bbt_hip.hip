@@ -404,6 +404,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     uint64_t read = 0;
     uint32_t f = 0;
     uint32_t ahead_extra = 0; /* probe unroll: lookups issued past a call's end */
+    /* probe_ahead bit 0: a call's bits kAhead at a time; bit 1: lookahead */
+    const uint32_t na = (a.probe_ahead & 1u) ? kAhead : 1u;
+    const bool look = (a.probe_ahead & 2u) != 0;
+    unsigned pbyte[kAhead] = {0, 0, 0, 0}; /* lookahead: the pending call's loaded bytes */
+    uint32_t psh = 0, pn = 0;              /* their bit positions (3 bits each), count */
     while (true) {
         const uint64_t idle = __ballot(!active);
         const uint32_t nidle = uint32_t(__popcll(idle));
@@ -429,15 +434,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             continue;
         }
         if (active) {
-            /* one base per lane per pass; a window that needs a contains()
-             * call gets its bits na at a time (probe unroll: kAhead
-             * independent loads), taken in hash order until the call ends at
-             * a clear bit or the last hash; loads past that end are
-             * issued-only */
-            if (w.step()) {
-                const uint32_t na = a.probe_ahead ? kAhead : 1u;
-                const uint64_t ks = w.kseed();
-                for (uint32_t j0 = 0; w.pending; j0 += na) {
+            const uint64_t ks = w.kseed();
+            /* the rest of a pending contains() call: na bits per round (probe
+             * unroll: kAhead independent loads), taken in hash order until the
+             * call ends at a clear bit or the last hash; loads past that end
+             * are issued-only */
+            const auto finish_call = [&]() {
+                while (w.pending) {
+                    const uint32_t j0 = w.j;
                     bool bit[kAhead];
                     uint32_t n = 0;
 #pragma unroll
@@ -457,6 +461,53 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                             w.take(bit[jj]);
                     ahead_extra += n - (w.lookups - before);
                 }
+            };
+            if (look) {
+                /* lookahead: roll the next base and issue its window's first
+                 * bits, then take the pending call's bits (issued a pass
+                 * earlier, so their loads overlapped this pass's work), then
+                 * decide the new window on the state they leave; its loads
+                 * are issued-only if it is not probed */
+                uint32_t cp = 0;
+                uint64_t ch = 0;
+                const bool cand = w.state == W_RUNNING && w.i < w.len && w.roll_step(cp, ch);
+                unsigned cbyte[kAhead];
+                uint32_t csh = 0, cn = 0;
+#pragma unroll
+                for (uint32_t jj = 0; jj < kAhead; ++jj) {
+                    cbyte[jj] = 0;
+                    if (cand && jj < na && jj < w.h) {
+                        const uint64_t pos =
+                            fastmod(multi_hash(ch, jj, ks), w.m, w.magic, w.sh1, w.sh2);
+                        cbyte[jj] = ld_global(w.bits + (pos >> 3));
+                        csh |= uint32_t(pos & 7) << (3 * jj);
+                        ++cn;
+                    }
+                }
+                if (w.pending) {
+                    const uint32_t before = w.lookups;
+#pragma unroll
+                    for (uint32_t jj = 0; jj < kAhead; ++jj)
+                        if (jj < pn && w.pending)
+                            w.take((pbyte[jj] >> ((psh >> (3 * jj)) & 7u)) & 1u);
+                    ahead_extra += pn - (w.lookups - before);
+                    finish_call();
+                }
+                pn = 0;
+                if (cand && w.state == W_RUNNING && w.decide(cp, ch)) {
+#pragma unroll
+                    for (uint32_t jj = 0; jj < kAhead; ++jj)
+                        pbyte[jj] = cbyte[jj];
+                    psh = csh;
+                    pn = cn;
+                } else {
+                    ahead_extra += cn;
+                }
+                if (w.state == W_RUNNING && !w.pending && w.i >= w.len)
+                    w.finish_end();
+            } else if (w.step()) {
+                /* one base per lane per pass */
+                finish_call();
             }
             if (w.state != W_RUNNING) {
                 store_walk(a, read, f, w);
@@ -1982,7 +2033,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.nf = nf;
     a.refill_min = tune.refill_min ? tune.refill_min : kDefaultRefillMin;
-    a.probe_ahead = tune.probe_ahead ? 1u : 0u;
+    a.probe_ahead = tune.probe_ahead & 3u;
     if (fix_scores) {
         P.fix_scores = true;
         a.fix_list = W.dfix;
@@ -2447,11 +2498,11 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t waves[] = {64, 128, 256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
     const uint32_t refills[] = {1, 4, 8, 16, 32};
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
-    const uint32_t aheads[] = {0, 1};
+    const uint32_t aheads[] = {0, 1, 2, 3};
     for (int dim = 0; dim < 4; ++dim) {
         const uint32_t *vals =
             dim == 0 ? waves : (dim == 1 ? refills : (dim == 2 ? bpcs : aheads));
-        const int nv = dim == 0 ? 8 : (dim == 3 ? 2 : 5);
+        const int nv = dim == 0 ? 8 : (dim == 3 ? 4 : 5);
         for (int i = 0; i < nv; ++i) {
             bbt_tuning t = cur;
             if (dim == 0) {

@@ -99,10 +99,14 @@ typedef struct {
                                by a whole wavefront, shorter ones 64 per wave */
     uint32_t refill_min;    /* idle lanes a wave gathers before refilling */
     uint32_t blocks_per_cu; /* persistent blocks (256 threads) per CU */
-    uint32_t probe_ahead;   /* probe unroll of the lane walks: 1 = a contains()
-                               call's first 4 lookups are issued together
-                               (independent loads; those past the first clear
-                               bit are wasted), 0 = one lookup at a time */
+    uint32_t probe_ahead;   /* load scheduling of the lane walks, two bits:
+                               bit 0 (probe unroll): a contains() call's
+                               lookups are issued 4 at a time (independent
+                               loads; those past the first clear bit are
+                               wasted), else one at a time; bit 1
+                               (lookahead): the next window's first lookups
+                               are issued before the pending call's bits are
+                               taken (wasted when that window is not probed) */
 } bbt_tuning;
 
 typedef struct {

@@ -27,7 +27,48 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
         w.init(seq + off[r], uint32_t(off[r + 1] - off[r]), &f, wp);
         w.dm = g_dust;
         w.dbase = off[r];
-        if (g_flat) {
+        if (g_flat > 100) {
+            /* the lookahead order of the persistent kernel: roll the next
+             * base and fetch its window's first bits, then take the pending
+             * call's bits, then decide the new window */
+            const unsigned na = g_flat - 100;
+            const auto bit_of = [&](uint64_t hv, uint32_t jj) {
+                const uint64_t pos = fastmod(multi_hash(hv, jj, w.kseed()), f.m, f.magic,
+                                             f.sh1, f.sh2);
+                return bool((f.bits[pos >> 3] >> (pos & 7)) & 1u);
+            };
+            bool pb[16], cb[16];
+            uint32_t pn = 0;
+            while (w.state == W_RUNNING) {
+                uint32_t cp = 0;
+                uint64_t ch = 0;
+                const bool cand = w.i < w.len && w.roll_step(cp, ch);
+                uint32_t cn = 0;
+                if (cand)
+                    for (; cn < na && cn < f.h; ++cn)
+                        cb[cn] = bit_of(ch, cn);
+                if (w.pending) {
+                    for (uint32_t jj = 0; jj < pn && w.pending; ++jj)
+                        w.take(pb[jj]);
+                    while (w.pending) {
+                        const uint32_t j0 = w.j;
+                        uint32_t n = 0;
+                        for (; n < na && j0 + n < f.h; ++n)
+                            pb[n] = bit_of(w.h0, j0 + n);
+                        for (uint32_t jj = 0; jj < n && w.pending; ++jj)
+                            w.take(pb[jj]);
+                    }
+                }
+                pn = 0;
+                if (cand && w.state == W_RUNNING && w.decide(cp, ch)) {
+                    for (uint32_t jj = 0; jj < cn; ++jj)
+                        pb[jj] = cb[jj];
+                    pn = cn;
+                }
+                if (w.state == W_RUNNING && !w.pending && w.i >= w.len)
+                    w.finish_end();
+            }
+        } else if (g_flat) {
             while (w.state == W_RUNNING) {
                 if (!w.step())
                     continue;
@@ -252,7 +293,7 @@ void walker_host_fastmod(uint64_t d, const uint64_t *n, uint64_t count, uint64_t
         out[i] = fastmod(n[i], fm.d, fm.magic, fm.sh1, fm.sh2);
 }
 
-void walker_host_set_flat(unsigned n) { g_flat = n < 16 ? n : 16; }
+void walker_host_set_flat(unsigned n) { g_flat = n > 100 ? (n < 116 ? n : 116) : (n < 16 ? n : 16); }
 
 /* base_valid and base_code for every byte value */
 void walker_host_bases(uint8_t *valid, uint8_t *code) {

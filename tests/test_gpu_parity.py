@@ -402,14 +402,18 @@ def test_2bit_reads_classify_like_ascii(dev, multi):
             assert int(rp["counters"][1]) == int(octr[1])
 
 
-def test_probe_unroll_keeps_results(dev, multi):
-    """probe_ahead (a contains() call's lookups issued together) changes no
-    hit, score, class or reference counter; it only issues more lookups."""
+@pytest.mark.parametrize("ahead", [1, 2, 3])
+def test_probe_unroll_keeps_results(dev, multi, ahead):
+    """probe_ahead (1: a contains() call's lookups issued together; 2: the
+    next window's lookups issued before the pending call's bits are taken; 3:
+    both) changes no hit, score, class or reference counter; it only issues
+    more lookups."""
     D, B = dev
     fs, ofs, buf, off = multi
     try:
         for refill in (1, 32):
-            fs.set_tuning(refill_min=refill, probe_ahead=1)
+            fs.set_tuning(refill_min=refill, probe_ahead=ahead)
+            check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
             for scorer, thr in [(O.SIMPLE, 0.15), (O.LENGTH, 40.0), (O.BINOMIAL, 1e-10)]:
                 check_batch(D, B, fs, ofs, buf, off, scorer=scorer, thr=thr)
             check_batch(D, B, fs, ofs, buf, off, paired=True)

@@ -154,10 +154,12 @@ def make_filter_and_reads(rng, k, h, m, nreads=150):
     return f, reads
 
 
-@pytest.fixture(params=[0, 1, 4], ids=["advance", "step1", "step4"])
+@pytest.fixture(params=[0, 1, 4, 101, 104],
+                ids=["advance", "step1", "step4", "lookahead1", "lookahead4"])
 def flat(W, request):
     """lane walk form: advance()/consume(), or step()/take() with a call's
-    bits fetched one or four at a time"""
+    bits fetched one or four at a time, or the lookahead order (the next
+    window rolled and its bits fetched before the pending call's are taken)"""
     W.walker_host_set_flat(request.param)
     yield request.param
     W.walker_host_set_flat(0)
