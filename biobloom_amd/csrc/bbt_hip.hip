@@ -2459,6 +2459,8 @@ extern "C" bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void 
 extern "C" bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t) {
     if (!fs || !t)
         return fail(BBT_ERR_ARG, "bbt_set_tuning: null argument");
+    if (t->probe_ahead > 3)
+        return fail(BBT_ERR_ARG, "bbt_set_tuning: probe_ahead takes two bits (0..3)");
     std::lock_guard<std::mutex> g(fs->mu);
     fs->tune = *t;
     fs->tuned = true;
@@ -2595,7 +2597,8 @@ extern "C" bbt_status bbt_tuning_load(const char *path) {
         char extra = 0;
         if (tab == std::string::npos || tab == 0 ||
             std::sscanf(l.c_str() + tab + 1, "%u %u %u %u %c", &t.wave_min_len, &t.refill_min,
-                        &t.blocks_per_cu, &t.probe_ahead, &extra) != 4) {
+                        &t.blocks_per_cu, &t.probe_ahead, &extra) != 4 ||
+            t.probe_ahead > 3) {
             st = fail(BBT_ERR_FORMAT, std::string("bbt_tuning_load: malformed line ") +
                                           std::to_string(lineno) + " of " + path);
             break;

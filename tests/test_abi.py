@@ -98,7 +98,9 @@ def test_tuning_table_save_and_load(tmp_path):
     L = B.load()
     lines = ["bytes_log2=26 nf=10 k=25 h=3 len_log2=7 mode=0 scorer=0 paired=1\t"
              "2147483647 0 4 0",
-             "bytes_log2=28 nf=4 k=25 h=4 len_log2=15 mode=0 scorer=0 paired=0\t256 16 0 1"]
+             "bytes_log2=28 nf=4 k=25 h=4 len_log2=15 mode=0 scorer=0 paired=0\t256 16 0 1",
+             "bytes_log2=22 nf=1 k=25 h=3 len_log2=7 mode=0 scorer=0 paired=0\t"
+             "2147483647 32 3 3"]
     src = tmp_path / "tune.tsv"
     src.write_text("\n".join(lines) + "\n")
     assert L.bbt_tuning_load(str(src).encode()) == 0
@@ -111,4 +113,6 @@ def test_tuning_table_save_and_load(tmp_path):
     bad.write_text("bytes_log2=1 nf=1\t1 2 x\n")
     assert L.bbt_tuning_load(str(bad).encode()) == 3
     assert "malformed line 1" in L.bbt_last_error().decode()
+    bad.write_text("bytes_log2=1 nf=1\t256 16 0 4\n")  # probe_ahead has two bits
+    assert L.bbt_tuning_load(str(bad).encode()) == 3
     assert L.bbt_tuning_load(str(tmp_path / "missing.tsv").encode()) == 2
