@@ -198,6 +198,10 @@ BBT_HD uint32_t nth_bit(uint64_t x, uint32_t n) {
 struct FilterDev {
     const uint8_t *bits;     /* m/8 bytes, bit p = byte p>>3, mask 1<<(p&7) */
     uint64_t m;              /* bits */
+    /* byte p>>3 is at bits[(p >> 3) << bshift]: 0 for a filter's own array;
+     * the filters of one geometry interleaved byte by byte, filter f's bytes
+     * at f, f + 2^bshift, ... (bits points at its first) */
+    uint32_t bshift;
     uint64_t magic;          /* FastMod of m */
     uint32_t sh1, sh2;
     uint32_t h, k;
@@ -247,6 +251,7 @@ struct Walker {
     const uint64_t *rt; /* the filter's roll terms for k */
     /* filter */
     const uint8_t *bits;
+    uint32_t bshift; /* byte p>>3 at bits[(p >> 3) << bshift] */
     uint64_t m, magic;
     uint32_t sh1, sh2, h;
     /* probe in progress */
@@ -294,6 +299,7 @@ struct Walker {
         run = 0;
         rt = f->roll;
         bits = f->bits;
+        bshift = f->bshift;
         m = f->m;
         magic = f->magic;
         sh1 = f->sh1;
@@ -381,7 +387,7 @@ struct Walker {
 
     BBT_HD void set_addr() {
         uint64_t pos = fastmod(hash_j(j), m, magic, sh1, sh2);
-        addr = bits + (pos >> 3);
+        addr = bits + ((pos >> 3) << bshift);
         mask = 1u << unsigned(pos & 7);
     }
 

@@ -95,6 +95,9 @@ struct EvalArgs {
     uint32_t nf;
     uint32_t refill_min;
     uint32_t probe_ahead; /* lane walks: issue a contains() call's lookups together */
+    /* lane items read by read (item = read * nf + filter): the lanes of one
+     * read walk its filters side by side over interleaved bits */
+    uint32_t read_major;
     /* BINOMIAL evalScore past the score table: the walk stores the hit count
      * and lists the item; the host turns the counts into scores */
     unsigned long long *fix_count;
@@ -304,7 +307,7 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
             if (need) {
                 const uint64_t hv = multi_hash(h0, jj, w.kseed());
                 const uint64_t pos = fastmod(hv, w.m, w.magic, w.sh1, w.sh2);
-                const unsigned byte = ld_global(w.bits + (pos >> 3));
+                const unsigned byte = ld_global(w.bits + ((pos >> 3) << w.bshift));
                 ++lk;
                 if (!(byte & (1u << unsigned(pos & 7)))) {
                     allset = false;
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
             if (!active) {
                 const uint64_t item = base + uint64_t(__popcll(idle & lt_mask));
                 if (item < nitems) {
-                    start_walk(a, item, nr, w, read, f, roll);
+                    start_walk(a, item, a.read_major ? 0 : nr, w, read, f, roll);
                     active = true;
                 }
             }
@@ -450,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                         if (jj < na && j0 + jj < w.h) {
                             const uint64_t pos = fastmod(multi_hash(w.h0, j0 + jj, ks), w.m,
                                                          w.magic, w.sh1, w.sh2);
-                            bit[jj] = (ld_global(w.bits + (pos >> 3)) >> unsigned(pos & 7)) & 1u;
+                            bit[jj] = (ld_global(w.bits + ((pos >> 3) << w.bshift)) >> unsigned(pos & 7)) & 1u;
                             ++n;
                         }
                     }
@@ -479,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                     if (cand && jj < na && jj < w.h) {
                         const uint64_t pos =
                             fastmod(multi_hash(ch, jj, ks), w.m, w.magic, w.sh1, w.sh2);
-                        cbyte[jj] = ld_global(w.bits + (pos >> 3));
+                        cbyte[jj] = ld_global(w.bits + ((pos >> 3) << w.bshift));
                         csh |= uint32_t(pos & 7) << (3 * jj);
                         ++cn;
                     }
@@ -643,7 +646,7 @@ __device__ bool walk_one(const EvalArgs &a, uint64_t read, uint32_t f,
             continue;
         while (w.pending) {
             const uint64_t pos = fastmod(multi_hash(w.h0, w.j, ks), w.m, w.magic, w.sh1, w.sh2);
-            w.take((ld_global(w.bits + (pos >> 3)) >> unsigned(pos & 7)) & 1u);
+            w.take((ld_global(w.bits + ((pos >> 3) << w.bshift)) >> unsigned(pos & 7)) & 1u);
         }
     }
     return w.accepted();
@@ -697,6 +700,31 @@ __global__ void k_fix_scores(const ScoreFix *fix, uint64_t n, double *scores) {
         const ScoreFix f = fix[i];
         const uint64_t bits = uint64_t(f.frames) | (uint64_t(f.count) << 32);
         scores[f.idx] = __longlong_as_double((long long)bits);
+    }
+}
+
+/* The filters of one geometry interleaved byte by byte: byte i of filter f
+ * at dst[(i << sh) + f], the unused slots of each 2^sh group zero.  The
+ * descriptors give the filters' own arrays. */
+__global__ __launch_bounds__(kBlock) void k_interleave(const FilterDev *src, uint32_t nf,
+                                                       uint64_t nbytes, uint32_t sh,
+                                                       uint8_t *dst) {
+    const uint32_t S = 1u << sh;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nbytes;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        uint8_t *o = dst + (i << sh);
+        if (S < 4) {
+            for (uint32_t f = 0; f < S; ++f)
+                o[f] = f < nf ? src[f].bits[i] : 0;
+            continue;
+        }
+        for (uint32_t f0 = 0; f0 < S; f0 += 4) {
+            uint32_t w = 0;
+            for (uint32_t q = 0; q < 4; ++q)
+                if (f0 + q < nf)
+                    w |= uint32_t(src[f0 + q].bits[i]) << (8 * q);
+            *reinterpret_cast<uint32_t *>(o + f0) = w;
+        }
     }
 }
 
@@ -1125,6 +1153,14 @@ struct bbt_filterset {
     bool tuned = false;
     bbt_workspace ws; /* of bbt_eval_batch */
     std::mutex mu;
+    /* the filters interleaved byte by byte when they share one geometry
+     * (probe_ahead bit 2): rebuilt from their own arrays by every batch that
+     * walks it, with its own descriptors (bits + f, bshift) */
+    uint8_t *dilv = nullptr;
+    uint64_t ilv_bytes = 0;
+    FilterDev *ddesc_ilv = nullptr;
+    uint32_t ilv_shift = 0;
+    bool ilv_desc_fresh = false;
 };
 
 static void fill_desc(FilterDev &d, void *bits, uint64_t m, uint32_t h, uint32_t k) {
@@ -1160,6 +1196,65 @@ static bbt_status sync_desc(bbt_filterset *fs) {
     HIP_OK(hipMemcpy(fs->ddesc, fs->desc.data(), sizeof(FilterDev) * n,
                      hipMemcpyHostToDevice));
     fs->dirty = false;
+    fs->ilv_desc_fresh = false;
+    return BBT_OK;
+}
+
+/* The interleave of a filter set: log2 of the group size (the filter count
+ * rounded up to a power of two), or 0 when the set cannot be interleaved
+ * (fewer than 2 or more than 64 filters, not one m, h and k, or a copy
+ * above kIlvMaxBytes). */
+constexpr uint64_t kIlvMaxBytes = uint64_t(32) << 30;
+
+static uint32_t ilv_shift_for(const bbt_filterset *fs) {
+    const size_t n = fs->desc.size();
+    if (n < 2 || n > 64)
+        return 0;
+    const FilterDev &d0 = fs->desc[0];
+    for (const FilterDev &d : fs->desc)
+        if (d.m != d0.m || d.h != d0.h || d.k != d0.k)
+            return 0;
+    uint32_t sh = 1;
+    while ((size_t(1) << sh) < n)
+        ++sh;
+    if ((d0.m / 8) > (kIlvMaxBytes >> sh))
+        return 0;
+    return sh;
+}
+
+/* The interleaved copy's memory and descriptors (not its contents: the
+ * batch's k_interleave writes them); false when the set does not qualify. */
+static bbt_status ensure_ilv(bbt_filterset *fs, bool &ok) {
+    ok = false;
+    const uint32_t sh = ilv_shift_for(fs);
+    if (!sh)
+        return BBT_OK;
+    const size_t n = fs->desc.size();
+    const uint64_t bytes = (fs->desc[0].m / 8) << sh;
+    if (fs->ilv_bytes < bytes || !fs->ddesc_ilv) {
+        HIP_OK(hipDeviceSynchronize()); /* batches of other streams may read them */
+        if (fs->dilv)
+            (void)hipFree(fs->dilv);
+        fs->dilv = nullptr;
+        fs->ilv_bytes = 0;
+        HIP_OK(hipMalloc(&fs->dilv, bytes));
+        fs->ilv_bytes = bytes;
+        if (!fs->ddesc_ilv)
+            HIP_OK(hipMalloc(&fs->ddesc_ilv, sizeof(FilterDev) * n));
+        fs->ilv_desc_fresh = false;
+    }
+    if (!fs->ilv_desc_fresh || fs->ilv_shift != sh) {
+        HIP_OK(hipDeviceSynchronize());
+        std::vector<FilterDev> v = fs->desc;
+        for (size_t f = 0; f < n; ++f) {
+            v[f].bits = fs->dilv + f;
+            v[f].bshift = sh;
+        }
+        HIP_OK(hipMemcpy(fs->ddesc_ilv, v.data(), sizeof(FilterDev) * n, hipMemcpyHostToDevice));
+        fs->ilv_shift = sh;
+        fs->ilv_desc_fresh = true;
+    }
+    ok = true;
     return BBT_OK;
 }
 
@@ -1289,6 +1384,10 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->ddesc);
     if (fs->dscratch)
         (void)hipFree(fs->dscratch);
+    if (fs->dilv)
+        (void)hipFree(fs->dilv);
+    if (fs->ddesc_ilv)
+        (void)hipFree(fs->ddesc_ilv);
     workspace_release(&fs->ws);
     delete fs;
     return BBT_OK;
@@ -1808,6 +1907,7 @@ struct Prepared {
     uint64_t code_bytes;
     bool fix_scores;      /* BINOMIAL scores past the table, finished on the host */
     bool dust_big;        /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
+    bool group;           /* read-major lane walks over the interleaved filters */
 };
 
 /* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
@@ -2028,6 +2128,18 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.off = in->off;
     a.order = in->order;
     a.filters = fs->ddesc;
+    if ((tune.probe_ahead & 4u) && p->path == BBT_PATH_BATCHED && p->mode != BBT_ORDERED &&
+        !noalloc) {
+        bool ok = false;
+        bbt_status s2 = ensure_ilv(fs, ok);
+        if (s2 != BBT_OK)
+            return s2;
+        if (ok) {
+            P.group = true;
+            a.filters = fs->ddesc_ilv;
+            a.read_major = 1;
+        }
+    }
     a.hits = out->hits;
     a.scores = out->scores;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
@@ -2138,6 +2250,13 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     }
     if (p->mode == BBT_ORDERED)
         k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
+    if (P.group) {
+        /* the interleaved copy, from the filters' own arrays as they are now */
+        const uint64_t nbytes = fs->desc[0].m / 8;
+        k_interleave<<<uint32_t(std::min<uint64_t>((nbytes + kBlock - 1) / kBlock, 65536)),
+                       kBlock, 0, st>>>(fs->ddesc, nf, nbytes, fs->ilv_shift, fs->dilv);
+        HIP_OK(hipGetLastError());
+    }
     if (timed)
         HIP_OK(hipEventRecord(W.ev0, st));
     switch (p->mode) {
@@ -2203,6 +2322,7 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     add(&P.wave_min, sizeof(P.wave_min));
     add(&P.L.a.refill_min, sizeof(P.L.a.refill_min));
     add(&P.L.a.probe_ahead, sizeof(P.L.a.probe_ahead));
+    add(&P.L.a.read_major, sizeof(P.L.a.read_major));
     add(&P.L.blocks_per_cu, sizeof(P.L.blocks_per_cu));
     add(&st, sizeof(st));
     return k;
@@ -2459,8 +2579,8 @@ extern "C" bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void 
 extern "C" bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t) {
     if (!fs || !t)
         return fail(BBT_ERR_ARG, "bbt_set_tuning: null argument");
-    if (t->probe_ahead > 3)
-        return fail(BBT_ERR_ARG, "bbt_set_tuning: probe_ahead takes two bits (0..3)");
+    if (t->probe_ahead > 7)
+        return fail(BBT_ERR_ARG, "bbt_set_tuning: probe_ahead takes three bits (0..7)");
     std::lock_guard<std::mutex> g(fs->mu);
     fs->tune = *t;
     fs->tuned = true;
@@ -2500,11 +2620,13 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t waves[] = {64, 128, 256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
     const uint32_t refills[] = {1, 4, 8, 16, 32};
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
-    const uint32_t aheads[] = {0, 1, 2, 3};
+    const uint32_t aheads[] = {0, 1, 2, 3, 4, 5, 6, 7};
+    /* the interleaved forms only for a set of one geometry */
+    const int nahead = ilv_shift_for(fs) && p->mode != BBT_ORDERED ? 8 : 4;
     for (int dim = 0; dim < 4; ++dim) {
         const uint32_t *vals =
             dim == 0 ? waves : (dim == 1 ? refills : (dim == 2 ? bpcs : aheads));
-        const int nv = dim == 0 ? 8 : (dim == 3 ? 4 : 5);
+        const int nv = dim == 0 ? 8 : (dim == 3 ? nahead : 5);
         for (int i = 0; i < nv; ++i) {
             bbt_tuning t = cur;
             if (dim == 0) {
@@ -2598,7 +2720,7 @@ extern "C" bbt_status bbt_tuning_load(const char *path) {
         if (tab == std::string::npos || tab == 0 ||
             std::sscanf(l.c_str() + tab + 1, "%u %u %u %u %c", &t.wave_min_len, &t.refill_min,
                         &t.blocks_per_cu, &t.probe_ahead, &extra) != 4 ||
-            t.probe_ahead > 3) {
+            t.probe_ahead > 7) {
             st = fail(BBT_ERR_FORMAT, std::string("bbt_tuning_load: malformed line ") +
                                           std::to_string(lineno) + " of " + path);
             break;

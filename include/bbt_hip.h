@@ -99,14 +99,21 @@ typedef struct {
                                by a whole wavefront, shorter ones 64 per wave */
     uint32_t refill_min;    /* idle lanes a wave gathers before refilling */
     uint32_t blocks_per_cu; /* persistent blocks (256 threads) per CU */
-    uint32_t probe_ahead;   /* load scheduling of the lane walks, two bits:
+    uint32_t probe_ahead;   /* load scheduling of the lane walks, three bits:
                                bit 0 (probe unroll): a contains() call's
                                lookups are issued 4 at a time (independent
                                loads; those past the first clear bit are
                                wasted), else one at a time; bit 1
                                (lookahead): the next window's first lookups
                                are issued before the pending call's bits are
-                               taken (wasted when that window is not probed) */
+                               taken (wasted when that window is not probed);
+                               bit 2 (interleave): for filters of one m, h
+                               and k (2 to 64 of them), lane walks go read by
+                               read over a copy of the filters interleaved
+                               byte by byte, so the lanes of one read fetch
+                               one line per lookup for all its filters (the
+                               copy is made per batch; bbt_eval_batch only,
+                               ignored for other sets and by ORDERED mode) */
 } bbt_tuning;
 
 typedef struct {

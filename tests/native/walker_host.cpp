@@ -5,6 +5,7 @@
  */
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "bbt_device.h"
 
@@ -16,6 +17,24 @@ const uint32_t *g_dust = nullptr;
 /* lane walk form: 0 advance()/consume(); n > 0 step()/take() with the bits of
  * a contains() call fetched n at a time, as the persistent kernel does */
 unsigned g_flat = 0;
+/* log2 of an interleave group (0: the filter's own array): the filter's
+ * bytes sit in the last slot of each 2^g_ilv-byte group, the others hold
+ * unrelated bytes */
+unsigned g_ilv = 0;
+std::vector<uint8_t> g_ilv_buf;
+
+void use_interleaved(FilterDev &f) {
+    if (!g_ilv)
+        return;
+    const size_t S = size_t(1) << g_ilv, nb = f.m / 8;
+    g_ilv_buf.assign(nb * S, 0);
+    for (size_t i = 0; i < nb * S; ++i)
+        g_ilv_buf[i] = uint8_t(i * 2654435761u >> 13);
+    for (size_t i = 0; i < nb; ++i)
+        g_ilv_buf[i * S + S - 1] = f.bits[i];
+    f.bits = g_ilv_buf.data() + S - 1;
+    f.bshift = g_ilv;
+}
 
 template <int SC, bool EXH>
 void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t nreads,
@@ -35,7 +54,7 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
             const auto bit_of = [&](uint64_t hv, uint32_t jj) {
                 const uint64_t pos = fastmod(multi_hash(hv, jj, w.kseed()), f.m, f.magic,
                                              f.sh1, f.sh2);
-                return bool((f.bits[pos >> 3] >> (pos & 7)) & 1u);
+                return bool((f.bits[(pos >> 3) << f.bshift] >> (pos & 7)) & 1u);
             };
             bool pb[16], cb[16];
             uint32_t pn = 0;
@@ -77,7 +96,7 @@ void run(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint64_t n
                     for (uint32_t jj = 0; jj < g_flat && j0 + jj < f.h; ++jj) {
                         const uint64_t pos = fastmod(multi_hash(w.h0, j0 + jj, w.kseed()), f.m,
                                                      f.magic, f.sh1, f.sh2);
-                        bit[jj] = (f.bits[pos >> 3] >> (pos & 7)) & 1u;
+                        bit[jj] = (f.bits[(pos >> 3) << f.bshift] >> (pos & 7)) & 1u;
                     }
                     for (uint32_t jj = 0; jj < g_flat && j0 + jj < f.h && w.pending; ++jj)
                         w.take(bit[jj]);
@@ -152,7 +171,7 @@ void run_wave(const FilterDev &f, const uint8_t *seq, const uint64_t *off, uint6
                     }
                     const uint64_t pos = fastmod(hv, f.m, f.magic, f.sh1, f.sh2);
                     ++lk[L];
-                    if (!(f.bits[pos >> 3] & (1u << (pos & 7)))) {
+                    if (!(f.bits[(pos >> 3) << f.bshift] & (1u << (pos & 7)))) {
                         allset[L] = false;
                         need = false;
                     } else if (j + 1 >= f.h) {
@@ -215,6 +234,7 @@ int walker_host_wave_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t 
     for (uint32_t j = 1; j < kMaxHash; ++j)
         f.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
     roll_fill(f.roll, k);
+    use_interleaved(f);
     WalkParams wp;
     wp.threshold = threshold;
     wp.streak = streak;
@@ -259,6 +279,7 @@ int walker_host_eval(const uint8_t *bits, uint64_t m, uint32_t h, uint32_t k,
     for (uint32_t j = 1; j < kMaxHash; ++j)
         f.mult[j] = uint64_t(j) ^ (uint64_t(k) * kMultiSeed);
     roll_fill(f.roll, k);
+    use_interleaved(f);
     WalkParams wp;
     wp.threshold = threshold;
     wp.streak = streak;
@@ -292,6 +313,8 @@ void walker_host_fastmod(uint64_t d, const uint64_t *n, uint64_t count, uint64_t
     for (uint64_t i = 0; i < count; ++i)
         out[i] = fastmod(n[i], fm.d, fm.magic, fm.sh1, fm.sh2);
 }
+
+void walker_host_set_ilv(unsigned sh) { g_ilv = sh < 7 ? sh : 6; }
 
 void walker_host_set_flat(unsigned n) { g_flat = n > 100 ? (n < 116 ? n : 116) : (n < 16 ? n : 16); }
 

@@ -225,6 +225,12 @@ def test_runtime_k_filters(dev):
         check_batch(D, B, fs, ofs, buf, off, paired=True)
         check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=O.HARMONIC, thr=0.15)
         check_batch(D, B, fs, ofs, buf, off, mode=O.ORDERED)
+        if ks[0] == ks[1] == ks[2]:
+            # one geometry: the interleaved read-major walks, lane and whole-wave
+            fs.set_tuning(probe_ahead=5)
+            check_batch(D, B, fs, ofs, buf, off, scorer=O.BINOMIAL, thr=1e-10)
+            check_batch(D, B, fs, ofs, buf, off, wave_min=100, paired=True)
+            fs.set_tuning()
 
 
 def test_store_and_load_roundtrip(dev, tmp_path):
@@ -250,26 +256,34 @@ def test_store_and_load_roundtrip(dev, tmp_path):
         assert fs2.info(i).fpr == fs.info(i).fpr
 
 
-def test_graph_replay_matches_eager(dev, multi):
+@pytest.mark.parametrize("ahead", [0, 7])
+def test_graph_replay_matches_eager(dev, multi, ahead):
+    """Replays of a captured batch (ahead 7: with the interleaved copy made
+    inside the graph) equal the eager batch and the oracle, replay after
+    replay, with eager batches between them."""
     D, B = dev
     fs, ofs, buf, off = multi
     dr = D.DeviceReads(buf, off)
     oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True)
     out = D.Outputs(dr.nreads, len(fs), True, False)
-    for streak in [3, 1]:
-        if streak == 1:
-            oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True,
-                                           p=O.params(O.SIMPLE, 0.15, 1))
-        for rep in range(4):
-            print(f"[gpu] graph replay streak={streak} rep={rep}", flush=True)
-            out.hits.fill_(7)
-            res = D.classify(fs, dr, B.params(paired=True, streak=streak,
-                                              flags=B.FLAG_GRAPH), out=out)
-            assert np.array_equal(res["hits"], oh)
-            assert np.array_equal(res["cls"], ocls)
-            assert int(res["counters"][1]) == int(octr[1])
-            eager = D.classify(fs, dr, B.params(paired=True, streak=streak))
-            assert np.array_equal(eager["summary"], res["summary"])
+    fs.set_tuning(probe_ahead=ahead)
+    try:
+        for streak in [3, 1]:
+            if streak == 1:
+                oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True,
+                                               p=O.params(O.SIMPLE, 0.15, 1))
+            for rep in range(4):
+                print(f"[gpu] graph replay ahead={ahead} streak={streak} rep={rep}", flush=True)
+                out.hits.fill_(7)
+                res = D.classify(fs, dr, B.params(paired=True, streak=streak,
+                                                  flags=B.FLAG_GRAPH), out=out)
+                assert np.array_equal(res["hits"], oh)
+                assert np.array_equal(res["cls"], ocls)
+                assert int(res["counters"][1]) == int(octr[1])
+                eager = D.classify(fs, dr, B.params(paired=True, streak=streak))
+                assert np.array_equal(eager["summary"], res["summary"])
+    finally:
+        fs.set_tuning()
 
 
 def test_autotune_keeps_results(dev):
@@ -402,12 +416,13 @@ def test_2bit_reads_classify_like_ascii(dev, multi):
             assert int(rp["counters"][1]) == int(octr[1])
 
 
-@pytest.mark.parametrize("ahead", [1, 2, 3])
+@pytest.mark.parametrize("ahead", [1, 2, 3, 4, 5, 6, 7])
 def test_probe_unroll_keeps_results(dev, multi, ahead):
-    """probe_ahead (1: a contains() call's lookups issued together; 2: the
-    next window's lookups issued before the pending call's bits are taken; 3:
-    both) changes no hit, score, class or reference counter; it only issues
-    more lookups."""
+    """probe_ahead (bit 0: a contains() call's lookups issued together; bit
+    1: the next window's lookups issued before the pending call's bits are
+    taken; bit 2: read-major walks over the filters interleaved byte by byte,
+    which this set of one geometry takes) changes no hit, score, class or
+    reference counter; it only issues more lookups."""
     D, B = dev
     fs, ofs, buf, off = multi
     try:

@@ -42,6 +42,7 @@ def W():
     L.walker_host_fastmod.argtypes = [C.c_uint64, p, C.c_uint64, p]
     L.walker_host_bases.argtypes = [p, p]
     L.walker_host_set_flat.argtypes = [C.c_uint]
+    L.walker_host_set_ilv.argtypes = [C.c_uint]
     L.walker_host_roll.argtypes = [C.c_uint, p]
     return L
 
@@ -313,3 +314,31 @@ def test_walker_sparse_hits_matches_oracle(W, m, h, wave, flat):
             assert hits.tolist() == want, (scorer, thr, streak)
             assert (int(ctr[0]), int(ctr[1])) == (probes, lookups), (scorer, thr, streak)
             assert 0 < sum(want) < len(want) or thr in (0.05, 0.6, 1e-2, 1e-6)
+
+
+@pytest.mark.parametrize("sh", [1, 2, 4, 6])
+def test_walker_interleaved_layout(W, sh):
+    """A filter read through an interleaved copy (its bytes every 2^sh bytes,
+    the other slots unrelated): every lane-walk form and the whole-wave walk
+    equal the oracle, counts included."""
+    rng = random.Random(40 + sh)
+    f, reads = make_filter_and_reads(rng, 25, 3, 1 << 15)
+    W.walker_host_set_ilv(sh)
+    try:
+        for mode in (0, 4, 104):
+            W.walker_host_set_flat(mode)
+            for wave in (False, True):
+                for scorer, thr in [(O.SIMPLE, 0.15), (O.HARMONIC, 0.15), (O.LENGTH, 30.0)]:
+                    p = O.params(scorer, thr, 3)
+                    hits, _, ctr = run_walker(W, f, reads, scorer, False, thr, 3, None, wave)
+                    want, probes, lookups = [], 0, 0
+                    for r in reads:
+                        ok, pr, lk = O.eval_read(f, r, p)
+                        want.append(int(ok))
+                        probes += pr
+                        lookups += lk
+                    assert hits.tolist() == want, (mode, wave, scorer)
+                    assert (int(ctr[0]), int(ctr[1])) == (probes, lookups)
+    finally:
+        W.walker_host_set_ilv(0)
+        W.walker_host_set_flat(0)
