@@ -113,7 +113,9 @@ typedef struct {
                                byte by byte, so the lanes of one read fetch
                                one line per lookup for all its filters (the
                                copy is made per batch; bbt_eval_batch only,
-                               ignored for other sets and by ORDERED mode) */
+                               ignored for other sets and by ORDERED mode).
+                               A set of one geometry with no knobs of its own
+                               (and none in the tuning table) takes 7. */
 } bbt_tuning;
 
 typedef struct {
