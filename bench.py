@@ -592,6 +592,8 @@ def bench_serve(args, torch, D, B, w, gathers):
     big = fs.info(0)
     log(f"serve: filters built on device ({time.time() - t:.1f}s): big m={big.m} "
         f"({big.m / 8 / 2**30:.2f} GiB), h={big.h}; pool of {len(pool)} requests")
+    if args.probe_ahead >= 0:
+        fs.set_tuning(probe_ahead=args.probe_ahead)
     params = B.params()
     # each pool request alone through bbt_eval_batch: the results every
     # served copy must reproduce

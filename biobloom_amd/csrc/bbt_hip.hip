@@ -703,10 +703,17 @@ __global__ void k_fix_scores(const ScoreFix *fix, uint64_t n, double *scores) {
     }
 }
 
-/* The filters of one geometry interleaved byte by byte: byte i of filter f
- * at dst[(i << sh) + f], the unused slots of each 2^sh group zero.  The
+/* The members of a group of filters of one geometry: filter set indices,
+ * slot q = the q-th member. */
+struct IlvMembers {
+    uint32_t n;
+    uint8_t f[64];
+};
+
+/* A group interleaved byte by byte: byte i of its q-th member at
+ * dst[(i << sh) + q], the unused slots of each 2^sh group zero.  The
  * descriptors give the filters' own arrays. */
-__global__ __launch_bounds__(kBlock) void k_interleave(const FilterDev *src, uint32_t nf,
+__global__ __launch_bounds__(kBlock) void k_interleave(const FilterDev *src, IlvMembers mem,
                                                        uint64_t nbytes, uint32_t sh,
                                                        uint8_t *dst) {
     const uint32_t S = 1u << sh;
@@ -714,16 +721,16 @@ __global__ __launch_bounds__(kBlock) void k_interleave(const FilterDev *src, uin
          i += uint64_t(gridDim.x) * blockDim.x) {
         uint8_t *o = dst + (i << sh);
         if (S < 4) {
-            for (uint32_t f = 0; f < S; ++f)
-                o[f] = f < nf ? src[f].bits[i] : 0;
+            for (uint32_t q = 0; q < S; ++q)
+                o[q] = q < mem.n ? src[mem.f[q]].bits[i] : 0;
             continue;
         }
-        for (uint32_t f0 = 0; f0 < S; f0 += 4) {
+        for (uint32_t q0 = 0; q0 < S; q0 += 4) {
             uint32_t w = 0;
             for (uint32_t q = 0; q < 4; ++q)
-                if (f0 + q < nf)
-                    w |= uint32_t(src[f0 + q].bits[i]) << (8 * q);
-            *reinterpret_cast<uint32_t *>(o + f0) = w;
+                if (q0 + q < mem.n)
+                    w |= uint32_t(src[mem.f[q0 + q]].bits[i]) << (8 * q);
+            *reinterpret_cast<uint32_t *>(o + q0) = w;
         }
     }
 }
@@ -1153,13 +1160,19 @@ struct bbt_filterset {
     bool tuned = false;
     bbt_workspace ws; /* of bbt_eval_batch */
     std::mutex mu;
-    /* the filters interleaved byte by byte when they share one geometry
-     * (probe_ahead bit 2): rebuilt from their own arrays by every batch that
-     * walks it, with its own descriptors (bits + f, bshift) */
-    uint8_t *dilv = nullptr;
-    uint64_t ilv_bytes = 0;
+    /* groups of filters of one geometry, each interleaved byte by byte
+     * (probe_ahead bit 2): rebuilt from the filters' own arrays by every batch
+     * that walks them; ddesc_ilv describes every filter, group members at
+     * their slot of the group's copy (bits + q, bshift), the others as in
+     * ddesc */
+    struct IlvGroup {
+        IlvMembers mem;
+        uint32_t shift = 0;
+        uint8_t *buf = nullptr;
+        uint64_t bytes = 0;
+    };
+    std::vector<IlvGroup> ilv;
     FilterDev *ddesc_ilv = nullptr;
-    uint32_t ilv_shift = 0;
     bool ilv_desc_fresh = false;
 };
 
@@ -1200,59 +1213,90 @@ static bbt_status sync_desc(bbt_filterset *fs) {
     return BBT_OK;
 }
 
-/* The interleave of a filter set: log2 of the group size (the filter count
- * rounded up to a power of two), or 0 when the set cannot be interleaved
- * (fewer than 2 or more than 64 filters, not one m, h and k, or a copy
- * above kIlvMaxBytes). */
+/* The interleave plan of a filter set: its filters grouped by (m, h, k) in
+ * set order, each group of at least 2 (up to 64 per group; more of one
+ * geometry make further groups) whose copy, the member count rounded up to a
+ * power of two times m/8 bytes, stays within kIlvMaxBytes. */
 constexpr uint64_t kIlvMaxBytes = uint64_t(32) << 30;
 constexpr uint32_t kDefaultIlvAhead = 7;
 
-static uint32_t ilv_shift_for(const bbt_filterset *fs) {
+static std::vector<bbt_filterset::IlvGroup> ilv_plan(const bbt_filterset *fs) {
+    std::vector<bbt_filterset::IlvGroup> out;
     const size_t n = fs->desc.size();
-    if (n < 2 || n > 64)
-        return 0;
-    const FilterDev &d0 = fs->desc[0];
-    for (const FilterDev &d : fs->desc)
-        if (d.m != d0.m || d.h != d0.h || d.k != d0.k)
-            return 0;
-    uint32_t sh = 1;
-    while ((size_t(1) << sh) < n)
-        ++sh;
-    if ((d0.m / 8) > (kIlvMaxBytes >> sh))
-        return 0;
-    return sh;
+    if (n > 256) /* members are byte indices */
+        return out;
+    std::vector<bool> used(n, false);
+    for (size_t a = 0; a < n; ++a) {
+        if (used[a])
+            continue;
+        const FilterDev &da = fs->desc[a];
+        bbt_filterset::IlvGroup g;
+        g.mem.n = 0;
+        for (size_t b = a; b < n && g.mem.n < 64; ++b) {
+            const FilterDev &db = fs->desc[b];
+            if (!used[b] && db.m == da.m && db.h == da.h && db.k == da.k) {
+                used[b] = true;
+                g.mem.f[g.mem.n++] = uint8_t(b);
+            }
+        }
+        if (g.mem.n < 2)
+            continue;
+        uint32_t sh = 1;
+        while ((1u << sh) < g.mem.n)
+            ++sh;
+        if (da.m / 8 > (kIlvMaxBytes >> sh))
+            continue;
+        g.shift = sh;
+        g.bytes = (da.m / 8) << sh;
+        out.push_back(g);
+    }
+    return out;
 }
 
-/* The interleaved copy's memory and descriptors (not its contents: the
- * batch's k_interleave writes them); false when the set does not qualify. */
+static bool ilv_any(const bbt_filterset *fs) { return !ilv_plan(fs).empty(); }
+
+/* every filter of the set in one group */
+static bool ilv_whole(const bbt_filterset *fs) {
+    const auto plan = ilv_plan(fs);
+    return plan.size() == 1 && plan[0].mem.n == fs->desc.size();
+}
+
+/* The interleaved copies' memory and descriptors (not their contents: the
+ * batch's k_interleave writes them); false when no group qualifies. */
 static bbt_status ensure_ilv(bbt_filterset *fs, bool &ok) {
     ok = false;
-    const uint32_t sh = ilv_shift_for(fs);
-    if (!sh)
+    std::vector<bbt_filterset::IlvGroup> plan = ilv_plan(fs);
+    if (plan.empty())
         return BBT_OK;
     const size_t n = fs->desc.size();
-    const uint64_t bytes = (fs->desc[0].m / 8) << sh;
-    if (fs->ilv_bytes < bytes || !fs->ddesc_ilv) {
+    bool same = plan.size() == fs->ilv.size();
+    for (size_t g = 0; same && g < plan.size(); ++g)
+        same = plan[g].bytes <= fs->ilv[g].bytes && plan[g].shift == fs->ilv[g].shift &&
+               plan[g].mem.n == fs->ilv[g].mem.n &&
+               std::memcmp(plan[g].mem.f, fs->ilv[g].mem.f, plan[g].mem.n) == 0;
+    if (!same || !fs->ddesc_ilv) {
         HIP_OK(hipDeviceSynchronize()); /* batches of other streams may read them */
-        if (fs->dilv)
-            (void)hipFree(fs->dilv);
-        fs->dilv = nullptr;
-        fs->ilv_bytes = 0;
-        HIP_OK(hipMalloc(&fs->dilv, bytes));
-        fs->ilv_bytes = bytes;
+        for (auto &g : fs->ilv)
+            if (g.buf)
+                (void)hipFree(g.buf);
+        fs->ilv.clear();
+        for (auto &g : plan) {
+            HIP_OK(hipMalloc(&g.buf, g.bytes));
+            fs->ilv.push_back(g);
+        }
         if (!fs->ddesc_ilv)
             HIP_OK(hipMalloc(&fs->ddesc_ilv, sizeof(FilterDev) * n));
         fs->ilv_desc_fresh = false;
     }
-    if (!fs->ilv_desc_fresh || fs->ilv_shift != sh) {
+    if (!fs->ilv_desc_fresh) {
         HIP_OK(hipDeviceSynchronize());
         std::vector<FilterDev> v = fs->desc;
-        for (size_t f = 0; f < n; ++f) {
-            v[f].bits = fs->dilv + f;
-            v[f].bshift = sh;
-        }
+        for (const auto &g : fs->ilv)
+            for (uint32_t q = 0; q < g.mem.n; ++q) {
+                v[g.mem.f[q]].bits = g.buf + q;
+                v[g.mem.f[q]].bshift = g.shift;
+            }
         HIP_OK(hipMemcpy(fs->ddesc_ilv, v.data(), sizeof(FilterDev) * n, hipMemcpyHostToDevice));
-        fs->ilv_shift = sh;
         fs->ilv_desc_fresh = true;
     }
     ok = true;
@@ -1385,8 +1429,9 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
         (void)hipFree(fs->ddesc);
     if (fs->dscratch)
         (void)hipFree(fs->dscratch);
-    if (fs->dilv)
-        (void)hipFree(fs->dilv);
+    for (auto &g : fs->ilv)
+        if (g.buf)
+            (void)hipFree(g.buf);
     if (fs->ddesc_ilv)
         (void)hipFree(fs->ddesc_ilv);
     workspace_release(&fs->ws);
@@ -2124,8 +2169,11 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     /* a set of one geometry without knobs of its own or in the table: the
      * interleaved read-major walks, four lookups at a time, with lookahead
      * (10 x 20 Mbp filters: 95.2 -> 52.4 ms per 4M reads); without
-     * allocation the same load scheduling over the filters' own arrays */
-    if (!fs->tuned && !from_table && ilv_shift_for(fs))
+     * allocation the same load scheduling over the filters' own arrays.  A
+     * set with only part of its filters in a group keeps filter-major walks
+     * over their own arrays (the serving set, one big filter and eight of one
+     * geometry: 57.9 ms against 67.2 ms interleaved) */
+    if (!fs->tuned && !from_table && ilv_whole(fs))
         tune.probe_ahead = noalloc ? (kDefaultIlvAhead & 3u) : kDefaultIlvAhead;
     L.blocks_per_cu = tune.blocks_per_cu;
     EvalArgs &a = L.a;
@@ -2261,10 +2309,12 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     if (p->mode == BBT_ORDERED)
         k_fill_u8<<<1024, kBlock, 0, st>>>(out->hits, in->nreads * nf, 0);
     if (P.group) {
-        /* the interleaved copy, from the filters' own arrays as they are now */
-        const uint64_t nbytes = fs->desc[0].m / 8;
-        k_interleave<<<uint32_t(std::min<uint64_t>((nbytes + kBlock - 1) / kBlock, 65536)),
-                       kBlock, 0, st>>>(fs->ddesc, nf, nbytes, fs->ilv_shift, fs->dilv);
+        /* the interleaved copies, from the filters' own arrays as they are now */
+        for (const auto &g : fs->ilv) {
+            const uint64_t nbytes = fs->desc[g.mem.f[0]].m / 8;
+            k_interleave<<<uint32_t(std::min<uint64_t>((nbytes + kBlock - 1) / kBlock, 65536)),
+                           kBlock, 0, st>>>(fs->ddesc, g.mem, nbytes, g.shift, g.buf);
+        }
         HIP_OK(hipGetLastError());
     }
     if (timed)
@@ -2632,7 +2682,7 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
     const uint32_t aheads[] = {0, 1, 2, 3, 4, 5, 6, 7};
     /* the interleaved forms only for a set of one geometry */
-    const int nahead = ilv_shift_for(fs) && p->mode != BBT_ORDERED ? 8 : 4;
+    const int nahead = ilv_any(fs) && p->mode != BBT_ORDERED ? 8 : 4;
     for (int dim = 0; dim < 4; ++dim) {
         const uint32_t *vals =
             dim == 0 ? waves : (dim == 1 ? refills : (dim == 2 ? bpcs : aheads));

@@ -107,15 +107,17 @@ typedef struct {
                                (lookahead): the next window's first lookups
                                are issued before the pending call's bits are
                                taken (wasted when that window is not probed);
-                               bit 2 (interleave): for filters of one m, h
-                               and k (2 to 64 of them), lane walks go read by
-                               read over a copy of the filters interleaved
-                               byte by byte, so the lanes of one read fetch
-                               one line per lookup for all its filters (the
-                               copy is made per batch; bbt_eval_batch only,
-                               ignored for other sets and by ORDERED mode).
-                               A set of one geometry with no knobs of its own
-                               (and none in the tuning table) takes 7. */
+                               bit 2 (interleave): lane walks go read by
+                               read, and each group of 2 to 64 filters of one
+                               m, h and k is read through a copy of its
+                               filters interleaved byte by byte, so the lanes
+                               of one read fetch one line per lookup for all
+                               its filters of a group (the copies are made per
+                               batch; bbt_eval_batch only, ignored by ORDERED
+                               mode and for sets without such a group).  A set
+                               whose filters all share one geometry, with no
+                               knobs of its own (and none in the tuning
+                               table), takes 7. */
 } bbt_tuning;
 
 typedef struct {

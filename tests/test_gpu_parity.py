@@ -233,6 +233,42 @@ def test_runtime_k_filters(dev):
             fs.set_tuning()
 
 
+def test_partial_interleave_groups(dev):
+    """Sets with some filters of one geometry: two of one size and one of
+    another; two groups (k 21 and k 25, two filters each).  The interleaved
+    read-major walks (probe_ahead 4..7) read each group through its copy and
+    the others through their own arrays; both paths equal the oracle."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(44)
+    L = B.load()
+    for lens, ks in (([90_000, 90_000, 140_000], [25, 25, 25]),
+                     ([80_000, 80_000, 80_000, 80_000], [21, 25, 21, 25])):
+        genomes = [synth.codes_to_ascii(synth.random_codes(rng, n)) for n in lens]
+        shorts, _, _ = synth.fixed_len_reads(rng, genomes, 500, 150, 0.01, 0.6)
+        longs, _ = synth.long_reads(rng, genomes, 10, median=800, hi=2500)
+        reads = [bytes(x) for x in shorts] + [bytes(x) for x in longs] + [b"", b"N" * 40]
+        buf, off = O.pack_reads(reads)
+        h = L.bbt_calc_optimal_hashnum(0.05)
+        ms = [L.bbt_calc_optimal_size(len(g) - k + 1, 0.05, h) for g, k in zip(genomes, ks)]
+        fs = B.FilterSet.create(ms, [h] * len(ms), ks)
+        for i, g in enumerate(genomes):
+            dg = torch.from_numpy(np.ascontiguousarray(g)).to("cuda")
+            fs.insert(i, dg.data_ptr(), dg.numel())
+            fs.refresh(i)
+            fs.set_id(i, f"p{i}")
+            del dg
+        ofs = oracle_filters_like(fs, B)
+        for ahead in (4, 7):
+            print(f"[gpu] partial groups lens={lens} k={ks} ahead={ahead}", flush=True)
+            fs.set_tuning(probe_ahead=ahead)
+            check_batch(D, B, fs, ofs, buf, off)
+            check_batch(D, B, fs, ofs, buf, off, paired=True, wave_min=100)
+            check_batch(D, B, fs, ofs, buf, off, mode=O.SCORES, scorer=O.LENGTH, thr=40.0)
+        fs.set_tuning()
+        check_batch(D, B, fs, ofs, buf, off)
+
+
 def test_store_and_load_roundtrip(dev, tmp_path):
     D, B = dev
     rng = np.random.default_rng(5)
