@@ -378,8 +378,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                     const unsigned long long lo = s_bbase[b] * a.nf;
                     if (c && it < lo + c * a.nf) {
                         const unsigned long long q = it - lo;
-                        const unsigned long long fb = q / c;
-                        fr = (fb << 40) | (s_bbase[b] + (q - fb * c));
+                        if (a.read_major) {
+                            /* the filters of one read side by side */
+                            const unsigned long long rb = q / a.nf;
+                            fr = ((q - rb * a.nf) << 40) | (s_bbase[b] + rb);
+                        } else {
+                            const unsigned long long fb = q / c;
+                            fr = (fb << 40) | (s_bbase[b] + (q - fb * c));
+                        }
                         break;
                     }
                 }
