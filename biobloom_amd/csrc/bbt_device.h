@@ -27,6 +27,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define BBT_HD __host__ __device__ __forceinline__
@@ -820,6 +822,166 @@ struct Walker {
         if (SC == HARMONIC)
             return dscore / n;
         return double(score2);
+    }
+};
+
+/* The contains() bits of one position for every filter of an interleaved
+ * group: byte q of a 2^SH-byte group holds filter q's byte, bit b of it the
+ * filter's bit; returns bit q = filter q's bit. */
+template <unsigned SH>
+BBT_HD uint32_t group_bits(const uint32_t *w, unsigned b) {
+    uint32_t out = 0;
+#pragma unroll
+    for (unsigned i = 0; i < ((1u << SH) + 3) / 4; ++i) {
+        const uint32_t x = (w[i] >> b) & 0x01010101u;
+        out |= (((x * 0x01020408u) >> 24) & 0xFu) << (4 * i);
+    }
+    return (1u << SH) < 4 ? out & ((1u << (1u << SH)) - 1u) : out;
+}
+
+/*
+ * One read against every filter of an interleaved group at once, SIMPLE
+ * evalRead (SeqEval.h:37-114): the read's windows are hashed once (the
+ * Walker's rolling hash, its own automaton unused), each of the h positions
+ * of a window is one load of the group's 2^SH bytes, and each filter keeps
+ * its own automaton -- the Walker's window_pre / on_result for SIMPLE,
+ * restated over per-filter fields.  Reads shorter than 65536 bases (the
+ * fields are 16-bit).  fetch(hv, j) returns the group bits of hash j.
+ */
+template <int K, unsigned MAXF>
+struct RowsWalk {
+    Walker<SIMPLE, false, K> w; /* hashing, thresholds, -D bitmap */
+    uint32_t nf;
+    uint32_t alive;             /* filters whose walk runs */
+    uint32_t acc;               /* filters that accepted */
+    uint32_t probes, lookups;   /* contains() calls and bit lookups, all filters */
+    uint32_t issued;            /* group loads */
+    /* per filter: prev | anti << 16; score2 | skipEnd << 16; streak | first << 30 | skipping << 31 */
+    uint32_t pa[MAXF], ss[MAXF], st[MAXF];
+
+    BBT_HD void init(const uint8_t *seq, uint32_t length, const FilterDev *f0, uint32_t n,
+                     const WalkParams &wp) {
+        w.init(seq, length, f0, wp);
+        nf = n;
+        alive = w.state == W_RUNNING ? (n >= 32 ? ~0u : (1u << n) - 1u) : 0u;
+        acc = 0;
+        probes = lookups = issued = 0;
+#pragma unroll
+        for (unsigned f = 0; f < MAXF; ++f) {
+            pa[f] = 0;
+            ss[f] = 0;
+            st[f] = 1u << 30; /* first */
+        }
+    }
+
+    /* window pw through filter f's automaton before contains():
+     * 0 probe, 1 consumed without a probe, 2 the walk ended */
+    template <unsigned F>
+    BBT_HD int pre(uint32_t pw) {
+        uint32_t prev = pa[F] & 0xFFFFu, anti = pa[F] >> 16;
+        if (st[F] >> 31) { /* skipping */
+            if (pw < (ss[F] >> 16)) {
+                ++anti;
+                pa[F] = pw | (anti << 16);
+                if (w.anti_thr <= anti) {
+                    alive &= ~(1u << F);
+                    return 2;
+                }
+                return 1;
+            }
+            st[F] &= ~(1u << 31);
+        }
+        if (!((st[F] >> 30) & 1u) && pw != prev + 1) {
+            anti += pw - prev - 1;
+            pa[F] = prev | (anti << 16);
+            if (w.anti_thr <= anti) {
+                alive &= ~(1u << F);
+                return 2;
+            }
+            st[F] &= 0xC0000000u; /* streak 0 */
+        }
+        return 0;
+    }
+
+    /* the contains() answer of window pw for filter f */
+    template <unsigned F>
+    BBT_HD void result(uint32_t pw, bool hit) {
+        uint32_t anti = pa[F] >> 16, score2 = ss[F] & 0xFFFFu;
+        uint32_t streak = st[F] & 0x3FFFFFFFu;
+        const bool first = (st[F] >> 30) & 1u;
+        if (hit) {
+            score2 += streak == 0 ? 1u : 2u;
+            ss[F] = (ss[F] & 0xFFFF0000u) | score2;
+            if (w.thr2 <= score2) {
+                alive &= ~(1u << F);
+                acc |= 1u << F;
+                return;
+            }
+            st[F] = (st[F] & 0x80000000u) | (first ? 1u : streak + 1u);
+            pa[F] = pw | (anti << 16);
+            return;
+        }
+        ++anti;
+        if (!first && streak >= w.streak_thr) {
+            /* jump-skip: windows below pw + k count as misses (SeqEval.h:99-109) */
+            ss[F] = score2 | ((pw + w.kk()) << 16);
+            st[F] = 0x80000000u;
+        } else {
+            st[F] = st[F] & 0x80000000u;
+        }
+        pa[F] = pw | (anti << 16);
+        if (w.anti_thr <= anti)
+            alive &= ~(1u << F);
+    }
+
+    template <unsigned F, class Fn>
+    BBT_HD void each_alive(const Fn &fn) {
+        if constexpr (F < MAXF) {
+            if (F < nf && ((alive >> F) & 1u))
+                fn(std::integral_constant<unsigned, F>());
+            each_alive<F + 1, Fn>(fn);
+        }
+    }
+
+    /* one base; false once every filter's walk has ended */
+    template <class Fetch>
+    BBT_HD bool step(const Fetch &fetch) {
+        if (!alive)
+            return false;
+        if (w.i >= w.len) {
+            alive = 0; /* the end of the read rejects the running walks */
+            return false;
+        }
+        uint32_t pw;
+        uint64_t hv;
+        if (!w.roll_step(pw, hv))
+            return true;
+        const bool low = w.low_complexity(pw);
+        uint32_t probe = 0;
+        each_alive<0>([&](auto fc) {
+            constexpr unsigned F = decltype(fc)::value;
+            if (this->template pre<F>(pw) == 0) {
+                if (low)
+                    this->template result<F>(pw, false); /* -D: a miss, no contains() */
+                else
+                    probe |= 1u << F;
+            }
+        });
+        if (!probe)
+            return alive != 0;
+        probes += uint32_t(__builtin_popcount(probe));
+        uint32_t ones = probe; /* filters whose bits so far were all set */
+        for (uint32_t j = 0; j < w.h && ones; ++j) {
+            lookups += uint32_t(__builtin_popcount(ones));
+            ++issued;
+            ones &= fetch(hv, j);
+        }
+        each_alive<0>([&](auto fc) {
+            constexpr unsigned F = decltype(fc)::value;
+            if ((probe >> F) & 1u)
+                this->template result<F>(pw, (ones >> F) & 1u);
+        });
+        return alive != 0;
     }
 };
 

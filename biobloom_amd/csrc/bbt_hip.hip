@@ -529,6 +529,93 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     add_issued(a.counters, issued);
 }
 
+/*
+ * One read per lane against every filter of a set interleaved as one group,
+ * SIMPLE evalRead (RowsWalk, bbt_device.h): the read's windows are hashed
+ * once, each bit position is one load of the group's 2^SH bytes (16 for 9
+ * to 16 filters), and each filter keeps its own automaton in registers.
+ * Lanes take reads from the queue; a read's hits are written for all its
+ * filters when the last of its walks ends.  a.filters: the filters' own
+ * descriptors (m, h, k, roll table; their bits are not read).
+ */
+template <int K, unsigned SH>
+__global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t *ilv) {
+    __shared__ uint64_t s_roll[2 * kRollEntries];
+    const uint64_t *roll = roll_lds<K>(a, s_roll);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    const uint64_t nreads = a.nitems / a.nf;
+    const FilterDev *F0 = a.filters;
+    const uint64_t m = F0->m, magic = F0->magic;
+    const uint32_t sh1 = F0->sh1, sh2 = F0->sh2;
+    const uint64_t ks = uint64_t(K ? K : F0->k) * kMultiSeed;
+    const uint32_t nfmask = a.nf >= 32 ? ~0u : (1u << a.nf) - 1u;
+    const auto fetch = [&](uint64_t hv, uint32_t j) -> uint32_t {
+        const uint64_t pos = fastmod(multi_hash(hv, j, ks), m, magic, sh1, sh2);
+        const uint8_t *g = ilv + ((pos >> 3) << SH);
+        uint32_t w[4];
+        if (SH == 4) {
+            const uint4 v = ld_global(reinterpret_cast<const uint4 *>(g));
+            w[0] = v.x;
+            w[1] = v.y;
+            w[2] = v.z;
+            w[3] = v.w;
+        } else if (SH == 3) {
+            const uint2 v = ld_global(reinterpret_cast<const uint2 *>(g));
+            w[0] = v.x;
+            w[1] = v.y;
+        } else {
+            w[0] = ld_global(reinterpret_cast<const uint32_t *>(g));
+        }
+        return group_bits<SH>(w, unsigned(pos & 7)) & nfmask;
+    };
+    RowsWalk<K, 16> rw;
+    uint32_t probes = 0, lookups = 0;
+    bool active = false, drained = false;
+    uint64_t read = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (!drained && nidle != 0 && (nidle >= a.refill_min || nidle == 64)) {
+            const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
+            unsigned long long base = 0;
+            if (lane == leader)
+                base = atomicAdd(a.queue, (unsigned long long)nidle);
+            base = __shfl(base, int(leader));
+            if (base + nidle >= nreads)
+                drained = true;
+            if (!active) {
+                const uint64_t r = base + uint64_t(__popcll(idle & lt_mask));
+                if (r < nreads) {
+                    read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
+                    const uint64_t b = a.off[read];
+                    rw.init(a.seq + b, uint32_t(a.off[read + 1] - b), F0, a.nf, a.wp);
+                    if (K)
+                        rw.w.rt = roll;
+                    rw.w.dm = a.dust;
+                    rw.w.dbase = b;
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained)
+                break;
+            continue;
+        }
+        if (active && !rw.step(fetch)) {
+            for (uint32_t f = 0; f < a.nf; ++f)
+                a.hits[read * a.nf + f] = uint8_t((rw.acc >> f) & 1u);
+            probes += rw.probes;
+            lookups += rw.lookups;
+            active = false;
+        }
+    }
+    add_counters(a.counters, probes, lookups);
+    add_issued(a.counters, lookups);
+}
+
 /* Length buckets for the batched path: reads of at least wave_min bases go
  * to the wave list by floor(log2(len)), longest bucket first; the rest to
  * the lane list.  cnt[0..31] bucket sizes, cnt[32] short reads. */
@@ -1779,6 +1866,37 @@ bbt_status launch_walk(const Launch &L) {
     return BBT_OK;
 }
 
+template <int K, unsigned SH>
+bbt_status launch_rows_k(const Launch &L, const uint8_t *ilv) {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_walk_rows<K, SH>, kBlock, 0) !=
+            hipSuccess)
+            v = 1;
+        per_cu = std::max(1, v);
+    }
+    int bpc = per_cu;
+    if (L.blocks_per_cu)
+        bpc = std::min<int>(bpc, int(L.blocks_per_cu));
+    const uint64_t nreads = L.a.nitems / L.a.nf;
+    const uint64_t blocks = std::max<uint64_t>(
+        1, std::min<uint64_t>(uint64_t(L.cu) * uint64_t(bpc), (nreads + kBlock - 1) / kBlock));
+    k_walk_rows<K, SH><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a, ilv);
+    HIP_OK(hipGetLastError());
+    return BBT_OK;
+}
+
+bbt_status launch_rows(const Launch &L, const uint8_t *ilv, uint32_t sh, uint32_t k) {
+    if (L.a.nitems == 0)
+        return BBT_OK;
+    if (k == 25)
+        return sh == 4 ? launch_rows_k<25, 4>(L, ilv)
+                       : (sh == 3 ? launch_rows_k<25, 3>(L, ilv) : launch_rows_k<25, 2>(L, ilv));
+    return sh == 4 ? launch_rows_k<0, 4>(L, ilv)
+                   : (sh == 3 ? launch_rows_k<0, 3>(L, ilv) : launch_rows_k<0, 2>(L, ilv));
+}
+
 template <int SC, bool EXH>
 bbt_status launch_k(const Launch &L, uint32_t k) {
     return k == 25 ? launch_walk<SC, EXH, 25>(L) : launch_walk<SC, EXH, 0>(L);
@@ -1960,6 +2078,7 @@ struct Prepared {
     bool fix_scores;      /* BINOMIAL scores past the table, finished on the host */
     bool dust_big;        /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
     bool group;           /* read-major lane walks over the interleaved filters */
+    bool rows;            /* one read per lane against every filter (k_walk_rows) */
 };
 
 /* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
@@ -2224,6 +2343,14 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     P.kk = common_k == 25 ? 25u : 0u;
     P.wave_min = p->wave_min_len ? p->wave_min_len
                                  : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
+    /* probe_ahead bit 3: one read per lane against every filter, for a set
+     * interleaved as one group of 3 to 16, SIMPLE evalRead, reads that are
+     * all lane walks and shorter than 32768 bases */
+    P.rows = (tune.probe_ahead & 8u) && P.group && fs->ilv.size() == 1 &&
+             fs->ilv[0].mem.n == nf && nf >= 3 && nf <= 16 && p->scorer == BBT_SIMPLE &&
+             p->mode == BBT_STD && in->max_len < 32768 && in->max_len < P.wave_min;
+    if (P.rows)
+        a.filters = fs->ddesc;
     return BBT_OK;
 }
 
@@ -2340,7 +2467,8 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             s = launch_scorer(L, p->scorer, true, P.kk);
         break;
     default:
-        s = launch_scorer(L, p->scorer, false, P.kk);
+        s = P.rows ? launch_rows(L, fs->ilv[0].buf, fs->ilv[0].shift, P.kk)
+                   : launch_scorer(L, p->scorer, false, P.kk);
         break;
     }
     if (s != BBT_OK)
@@ -2389,6 +2517,7 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     add(&P.L.a.refill_min, sizeof(P.L.a.refill_min));
     add(&P.L.a.probe_ahead, sizeof(P.L.a.probe_ahead));
     add(&P.L.a.read_major, sizeof(P.L.a.read_major));
+    add(&P.rows, sizeof(P.rows));
     add(&P.L.blocks_per_cu, sizeof(P.L.blocks_per_cu));
     add(&st, sizeof(st));
     return k;
@@ -2645,8 +2774,8 @@ extern "C" bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void 
 extern "C" bbt_status bbt_set_tuning(bbt_filterset *fs, const bbt_tuning *t) {
     if (!fs || !t)
         return fail(BBT_ERR_ARG, "bbt_set_tuning: null argument");
-    if (t->probe_ahead > 7)
-        return fail(BBT_ERR_ARG, "bbt_set_tuning: probe_ahead takes three bits (0..7)");
+    if (t->probe_ahead > 15)
+        return fail(BBT_ERR_ARG, "bbt_set_tuning: probe_ahead takes four bits (0..15)");
     std::lock_guard<std::mutex> g(fs->mu);
     fs->tune = *t;
     fs->tuned = true;
@@ -2686,9 +2815,13 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t waves[] = {64, 128, 256, 512, 1024, 2048, 4096, 0x7FFFFFFFu};
     const uint32_t refills[] = {1, 4, 8, 16, 32};
     const uint32_t bpcs[] = {0, 1, 2, 3, 4};
-    const uint32_t aheads[] = {0, 1, 2, 3, 4, 5, 6, 7};
-    /* the interleaved forms only for a set of one geometry */
-    const int nahead = ilv_any(fs) && p->mode != BBT_ORDERED ? 8 : 4;
+    const uint32_t aheads[] = {0, 1, 2, 3, 4, 5, 6, 7, 12};
+    /* the interleaved forms only for a set with a group of one geometry; the
+     * read-per-lane walk (12) where it applies */
+    const bool rows_ok = ilv_whole(fs) && fs->desc.size() >= 3 && fs->desc.size() <= 16 &&
+                         p->scorer == BBT_SIMPLE && p->mode == BBT_STD;
+    const int nahead =
+        ilv_any(fs) && p->mode != BBT_ORDERED ? (rows_ok ? 9 : 8) : 4;
     for (int dim = 0; dim < 4; ++dim) {
         const uint32_t *vals =
             dim == 0 ? waves : (dim == 1 ? refills : (dim == 2 ? bpcs : aheads));
@@ -2786,7 +2919,7 @@ extern "C" bbt_status bbt_tuning_load(const char *path) {
         if (tab == std::string::npos || tab == 0 ||
             std::sscanf(l.c_str() + tab + 1, "%u %u %u %u %c", &t.wave_min_len, &t.refill_min,
                         &t.blocks_per_cu, &t.probe_ahead, &extra) != 4 ||
-            t.probe_ahead > 7) {
+            t.probe_ahead > 15) {
             st = fail(BBT_ERR_FORMAT, std::string("bbt_tuning_load: malformed line ") +
                                           std::to_string(lineno) + " of " + path);
             break;

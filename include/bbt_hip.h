@@ -99,7 +99,7 @@ typedef struct {
                                by a whole wavefront, shorter ones 64 per wave */
     uint32_t refill_min;    /* idle lanes a wave gathers before refilling */
     uint32_t blocks_per_cu; /* persistent blocks (256 threads) per CU */
-    uint32_t probe_ahead;   /* load scheduling of the lane walks, three bits:
+    uint32_t probe_ahead;   /* load scheduling of the lane walks, four bits:
                                bit 0 (probe unroll): a contains() call's
                                lookups are issued 4 at a time (independent
                                loads; those past the first clear bit are
@@ -114,10 +114,16 @@ typedef struct {
                                of one read fetch one line per lookup for all
                                its filters of a group (the copies are made per
                                batch; bbt_eval_batch only, ignored by ORDERED
-                               mode and for sets without such a group).  A set
-                               whose filters all share one geometry, with no
-                               knobs of its own (and none in the tuning
-                               table), takes 7. */
+                               mode and for sets without such a group); bit 3
+                               with bit 2 (read per lane): a set of 3 to 16
+                               filters of one geometry, SIMPLE evalRead in STD
+                               mode, reads all shorter than wave_min_len and
+                               32768: each lane hashes one read once and reads
+                               all its filters with one load per position.
+                               Values above 15 are rejected.  A set whose
+                               filters all share one geometry, with no knobs of
+                               its own (and none in the tuning table), takes
+                               7. */
 } bbt_tuning;
 
 typedef struct {

@@ -314,6 +314,67 @@ void walker_host_fastmod(uint64_t d, const uint64_t *n, uint64_t count, uint64_t
         out[i] = fastmod(n[i], fm.d, fm.magic, fm.sh1, fm.sh2);
 }
 
+/* RowsWalk (one read against every filter of an interleaved group, SIMPLE
+ * evalRead) over G filters of one m, h and k: bits[g] their arrays; hits
+ * read-major (r * G + g); ctr[0] contains() calls, ctr[1] bit lookups. */
+int walker_host_rows(const uint8_t *const *bits, uint32_t G, uint64_t m, uint32_t h, uint32_t k,
+                     const uint8_t *seq, const uint64_t *off, uint64_t nreads, double threshold,
+                     uint32_t streak, uint8_t *hits, uint64_t *ctr, const uint32_t *dust) {
+    if (G < 3 || G > 16)
+        return 1;
+    const unsigned SH = G <= 4 ? 2 : (G <= 8 ? 3 : 4);
+    const size_t S = size_t(1) << SH, nb = m / 8;
+    std::vector<uint8_t> ilv(nb * S + 16, 0);
+    for (size_t i = 0; i < nb; ++i)
+        for (uint32_t g = 0; g < G; ++g)
+            ilv[i * S + g] = bits[g][i];
+    FilterDev f;
+    std::memset(&f, 0, sizeof(f));
+    f.bits = bits[0];
+    f.m = m;
+    FastMod fm = fastmod_make(m);
+    f.magic = fm.magic;
+    f.sh1 = fm.sh1;
+    f.sh2 = fm.sh2;
+    f.h = h;
+    f.k = k;
+    roll_fill(f.roll, k);
+    WalkParams wp;
+    wp.threshold = threshold;
+    wp.streak = streak;
+    wp.min_len = uint32_t(threshold + 0.5);
+    const uint64_t ks = uint64_t(k) * kMultiSeed;
+    const auto fetch = [&](uint64_t hv, uint32_t j) {
+        const uint64_t pos = fastmod(multi_hash(hv, j, ks), m, fm.magic, fm.sh1, fm.sh2);
+        uint32_t w[4];
+        std::memcpy(w, ilv.data() + ((pos >> 3) << SH), 4 * ((S + 3) / 4));
+        uint32_t b = 0;
+        if (SH == 2)
+            b = group_bits<2>(w, unsigned(pos & 7));
+        else if (SH == 3)
+            b = group_bits<3>(w, unsigned(pos & 7));
+        else
+            b = group_bits<4>(w, unsigned(pos & 7));
+        return b & ((1u << G) - 1u);
+    };
+    for (uint64_t r = 0; r < nreads; ++r) {
+        const uint32_t len = uint32_t(off[r + 1] - off[r]);
+        if (len >= 32768)
+            return 2;
+        RowsWalk<0, 16> rw;
+        rw.init(seq + off[r], len, &f, G, wp);
+        rw.w.dm = dust;
+        rw.w.dbase = off[r];
+        while (rw.step(fetch)) {
+        }
+        for (uint32_t g = 0; g < G; ++g)
+            hits[r * G + g] = (rw.acc >> g) & 1u;
+        ctr[0] += rw.probes;
+        ctr[1] += rw.lookups;
+    }
+    return 0;
+}
+
 void walker_host_set_ilv(unsigned sh) { g_ilv = sh < 7 ? sh : 6; }
 
 void walker_host_set_flat(unsigned n) { g_flat = n > 100 ? (n < 116 ? n : 116) : (n < 16 ? n : 16); }

@@ -113,6 +113,6 @@ def test_tuning_table_save_and_load(tmp_path):
     bad.write_text("bytes_log2=1 nf=1\t1 2 x\n")
     assert L.bbt_tuning_load(str(bad).encode()) == 3
     assert "malformed line 1" in L.bbt_last_error().decode()
-    bad.write_text("bytes_log2=1 nf=1\t256 16 0 8\n")  # probe_ahead has three bits
+    bad.write_text("bytes_log2=1 nf=1\t256 16 0 16\n")  # probe_ahead has four bits
     assert L.bbt_tuning_load(str(bad).encode()) == 3
     assert L.bbt_tuning_load(str(tmp_path / "missing.tsv").encode()) == 2

@@ -269,6 +269,39 @@ def test_partial_interleave_groups(dev):
         check_batch(D, B, fs, ofs, buf, off)
 
 
+@pytest.mark.parametrize("nf", [3, 6, 12, 16])
+def test_rows_walk_group_sizes(dev, nf):
+    """The read-per-lane walk (probe_ahead 12) over sets of 3 to 16 filters of
+    one geometry, i.e. interleaved groups of 4, 8 and 16 bytes per position:
+    hits, classes and reference counters equal the oracle's with and without
+    -D, single-end and paired; reads that reach the whole-wave length leave
+    the batch on the other walks, with the same results."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(45, nf)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 40_000)) for _ in range(nf)]
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 600, 150, 0.01, 0.6)
+    odd = [b"", b"ACGT", b"N" * 100, b"A" * 24, b"C" * 25, b"ACGTN" * 40,
+           bytes(shorts[0]).lower(), bytes(shorts[1][:70]) + b"NN" + bytes(shorts[1][70:])]
+    reads = [bytes(x) for x in shorts] + odd
+    buf, off = O.pack_reads(reads)
+    fs, _ = D.build_filters(genomes, 25, 0.05)
+    ofs = oracle_filters_like(fs, B)
+    longs, _ = synth.long_reads(rng, genomes, 8, median=600, hi=1500)
+    lbuf, loff = O.pack_reads(reads + [bytes(x) for x in longs])
+    try:
+        fs.set_tuning(probe_ahead=12)
+        print(f"[gpu] rows walk nf={nf}", flush=True)
+        check_batch(D, B, fs, ofs, buf, off)
+        check_batch(D, B, fs, ofs, buf, off, thr=0.0, streak=0)
+        check_batch(D, B, fs, ofs, buf, off, thr=0.9, streak=5)
+        check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
+        check_batch(D, B, fs, ofs, buf, off, paired=True, inclusive=True)
+        check_batch(D, B, fs, ofs, lbuf, loff)
+    finally:
+        fs.set_tuning()
+
+
 def test_store_and_load_roundtrip(dev, tmp_path):
     D, B = dev
     rng = np.random.default_rng(5)
@@ -292,10 +325,10 @@ def test_store_and_load_roundtrip(dev, tmp_path):
         assert fs2.info(i).fpr == fs.info(i).fpr
 
 
-@pytest.mark.parametrize("ahead", [0, 7])
+@pytest.mark.parametrize("ahead", [0, 7, 15])
 def test_graph_replay_matches_eager(dev, multi, ahead):
-    """Replays of a captured batch (ahead 7: with the interleaved copy made
-    inside the graph) equal the eager batch and the oracle, replay after
+    """Replays of a captured batch (ahead 7 and 15: with the interleaved copy
+    made inside the graph; 15: read-per-lane walks) equal the eager batch and the oracle, replay after
     replay, with eager batches between them."""
     D, B = dev
     fs, ofs, buf, off = multi
@@ -452,13 +485,14 @@ def test_2bit_reads_classify_like_ascii(dev, multi):
             assert int(rp["counters"][1]) == int(octr[1])
 
 
-@pytest.mark.parametrize("ahead", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("ahead", [1, 2, 3, 4, 5, 6, 7, 12, 15])
 def test_probe_unroll_keeps_results(dev, multi, ahead):
     """probe_ahead (bit 0: a contains() call's lookups issued together; bit
     1: the next window's lookups issued before the pending call's bits are
     taken; bit 2: read-major walks over the filters interleaved byte by byte,
-    which this set of one geometry takes) changes no hit, score, class or
-    reference counter; it only issues more lookups."""
+    which this set of one geometry takes; bit 3 with bit 2: one read per lane
+    against all four filters, for SIMPLE evalRead in STD mode) changes no hit,
+    score, class or reference counter; it only issues more lookups."""
     D, B = dev
     fs, ofs, buf, off = multi
     try:

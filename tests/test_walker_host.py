@@ -342,3 +342,65 @@ def test_walker_interleaved_layout(W, sh):
     finally:
         W.walker_host_set_ilv(0)
         W.walker_host_set_flat(0)
+
+
+@pytest.mark.parametrize("G", [3, 4, 7, 10, 16])
+def test_rows_walk_matches_oracle(W, G):
+    """One read against every filter of an interleaved group (SIMPLE evalRead,
+    the read hashed once, one group load per bit position, an automaton per
+    filter): every filter's decision and the contains() calls and bit lookups
+    summed over the filters equal the oracle's, streak thresholds 0-5, with
+    and without -D."""
+    rng = random.Random(70 + G)
+    k, h, m = 21, 3, 1 << 16
+    genome = "".join(rng.choice("ACGT") for _ in range(6000))
+    filters = []
+    for g in range(G):
+        f = O.Filter(m, h, k)
+        a = rng.randrange(0, 4000)
+        f.insert_seq(genome[a:a + 1800])
+        filters.append(f)
+    reads = []
+    for i in range(150):
+        L = rng.choice([0, 10, 21, 22, 60, 150, 151, 300, 700])
+        if i % 3 and L < 5000:
+            a = rng.randrange(0, 6000 - L)
+            r = list(genome[a:a + L])
+            for q in range(L):
+                x = rng.random()
+                if x < 0.02:
+                    r[q] = rng.choice("ACGT")
+                elif x < 0.03:
+                    r[q] = "N"
+            reads.append("".join(r))
+        else:
+            reads.append("".join(rng.choice("ACGTN") if rng.random() < 0.03 else rng.choice("ACGT")
+                                 for _ in range(L)))
+    reads.append(genome[100:130] + "AC" * 40 + genome[200:300])
+    W.walker_host_rows.restype = C.c_int
+    W.walker_host_rows.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                   C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_uint32,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]
+    buf, off = O.pack_reads(reads)
+    buf = padded(buf)
+    ptrs = (C.c_void_p * G)(*[f.bits.ctypes.data for f in filters])
+    for dust in (None, (20, 32)):
+        bm = dust_bitmap(reads, *dust) if dust else None
+        for thr, streak in [(0.15, 3), (0.15, 0), (0.5, 1), (0.05, 2), (0.9, 5), (0.0, 3)]:
+            hits = np.zeros(len(reads) * G, np.uint8)
+            ctr = np.zeros(3, np.uint64)
+            rc = W.walker_host_rows(ptrs, G, m, h, k, buf.ctypes.data, off.ctypes.data,
+                                    len(reads), thr, streak, hits.ctypes.data, ctr.ctypes.data,
+                                    None if bm is None else bm.ctypes.data)
+            assert rc == 0
+            p = O.params(O.SIMPLE, thr, streak, bool(dust), *(dust or (20, 64)))
+            want, probes, lookups = [], 0, 0
+            for r in reads:
+                for f in filters:
+                    ok, pr, lk = O.eval_read(f, r, p)
+                    want.append(int(ok))
+                    probes += pr
+                    lookups += lk
+            assert hits.tolist() == want, (thr, streak, dust)
+            assert (int(ctr[0]), int(ctr[1])) == (probes, lookups), (thr, streak, dust)
+            assert 0 < sum(want) < len(want) or thr in (0.0, 0.9)
