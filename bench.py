@@ -239,6 +239,7 @@ def bench_walk(args, torch, D, B, name, w, steps, warmup, gathers, autotune=True
                 walk.append(fs.last_walk_ms())
         torch.cuda.synchronize()
         res[label] = {"ms": ms, "walk_ms": float(np.mean(walk)) if walk else None,
+                      "kernel": fs.last_walk_kernel(),
                       "probes": int(host["counters"][0]), "lookups": int(host["counters"][1]),
                       "issued": int(host["counters"][2]), "hits": host["hits"],
                       "summary": host["summary"]}
@@ -259,7 +260,7 @@ def bench_walk(args, torch, D, B, name, w, steps, warmup, gathers, autotune=True
     gbytes, glps = gathers.rate(fs_bytes)
     _, g8 = gathers.rate(8 << 30)
     traffic = load_traffic(name)
-    rl = {"bound": "hbm", "kernel": "k_walk_persistent" if "batched" in res else "k_walk_plain",
+    rl = {"bound": "hbm", "kernel": (res.get("batched") or res["plain"])["kernel"],
           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": achieved / HBM_PEAK_GBS,
           "traffic": traffic["bytes_per_launch"] if traffic else None,

@@ -1186,6 +1186,7 @@ struct bbt_workspace {
     /* walk-kernel timing (BBT_FLAG_TIME_WALK) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_walk_ms = -1;
+    const char *last_walk_kernel = ""; /* the walk kernel of the last batch */
     /* -D: the batch's low-complexity bitmap and the pre-pass's scratch */
     uint32_t *dmask = nullptr;
     uint64_t dmask_words = 0;
@@ -1311,7 +1312,7 @@ static bbt_status sync_desc(bbt_filterset *fs) {
  * geometry make further groups) whose copy, the member count rounded up to a
  * power of two times m/8 bytes, stays within kIlvMaxBytes. */
 constexpr uint64_t kIlvMaxBytes = uint64_t(32) << 30;
-constexpr uint32_t kDefaultIlvAhead = 7;
+constexpr uint32_t kDefaultIlvAhead = 15;
 
 static std::vector<bbt_filterset::IlvGroup> ilv_plan(const bbt_filterset *fs) {
     std::vector<bbt_filterset::IlvGroup> out;
@@ -2293,8 +2294,9 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     }
     /* a set of one geometry without knobs of its own or in the table: the
      * interleaved read-major walks, four lookups at a time, with lookahead
-     * (10 x 20 Mbp filters: 95.2 -> 52.4 ms per 4M reads); without
-     * allocation the same load scheduling over the filters' own arrays.  A
+     * (10 x 20 Mbp filters: 95.2 -> 52.4 ms per 4M reads), and one read per
+     * lane where that applies (24.2 ms); without allocation the same load
+     * scheduling over the filters' own arrays.  A
      * set with only part of its filters in a group keeps filter-major walks
      * over their own arrays (the serving set, one big filter and eight of one
      * geometry: 57.9 ms against 67.2 ms interleaved) */
@@ -2575,6 +2577,10 @@ static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads
     bbt_status s = prepare(fs, W, in, p, out, st, P, noalloc);
     if (s != BBT_OK)
         return s;
+    W.last_walk_kernel = p->mode == BBT_ORDERED         ? "k_ordered"
+                         : P.rows                        ? "k_walk_rows"
+                         : p->path == BBT_PATH_PLAIN     ? "k_walk_plain"
+                                                         : "k_walk_persistent";
     const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
     if (timed) {
         if (!W.ev0)
@@ -2937,4 +2943,8 @@ extern "C" bbt_status bbt_tuning_load(const char *path) {
 
 extern "C" double bbt_last_walk_ms(const bbt_filterset *fs) {
     return fs ? fs->ws.last_walk_ms : -1.0;
+}
+
+extern "C" const char *bbt_last_walk_kernel(const bbt_filterset *fs) {
+    return fs ? fs->ws.last_walk_kernel : "";
 }

@@ -127,6 +127,7 @@ _SIGS = {
     "bbt_eval_batch": (C.c_int32, [C.c_void_p, C.POINTER(Reads), C.POINTER(EvalParams),
                                    C.POINTER(EvalOut), C.c_void_p]),
     "bbt_last_walk_ms": (C.c_double, [C.c_void_p]),
+    "bbt_last_walk_kernel": (C.c_char_p, [C.c_void_p]),
     "bbt_measure_gather": (C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bbt_eval_batch_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(Reads),
@@ -327,6 +328,9 @@ class FilterSet:
 
     def last_walk_ms(self):
         return load().bbt_last_walk_ms(self._h)
+
+    def last_walk_kernel(self):
+        return (load().bbt_last_walk_kernel(self._h) or b"").decode()
 
     def summary_text(self, summary_counts, nrec):
         L = load()

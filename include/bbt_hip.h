@@ -123,7 +123,7 @@ typedef struct {
                                Values above 15 are rejected.  A set whose
                                filters all share one geometry, with no knobs of
                                its own (and none in the tuning table), takes
-                               7. */
+                               15. */
 } bbt_tuning;
 
 typedef struct {
@@ -312,6 +312,11 @@ bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void *stream,
 /* Duration of the walk kernel(s) of the last bbt_eval_batch on fs that was
  * called with BBT_FLAG_TIME_WALK, in milliseconds (-1 if none). */
 double bbt_last_walk_ms(const bbt_filterset *fs);
+/* Measurement utility (no counterpart in the reference): name of the walk
+ * kernel the last bbt_eval_batch or bbt_eval_batch_noalloc on fs ran
+ * ("k_walk_persistent", "k_walk_rows", "k_walk_plain" or "k_ordered"; "" before
+ * the first batch), for profiles that name the kernel they time. */
+const char *bbt_last_walk_kernel(const bbt_filterset *fs);
 /* summary.tsv text from the host copy of bbt_eval_out.summary counters;
  * writes at most cap bytes (NUL-terminated), *needed = full length. */
 bbt_status bbt_summary_text(const bbt_filterset *fs, const uint64_t *summary,
