@@ -231,32 +231,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_plain(EvalArgs a) {
     add_issued(a.counters, w.lookups);
 }
 
-/*
- * One (read, filter) walk by a whole wavefront, 64 windows per step: the
- * window hashes come from prefix-XOR scans across lanes (bbt_device.h,
- * pfx_*), every window the walk may probe is probed at once (each lane
- * stops at its first clear bit), and the walk is then replayed in window
- * order over the chunk's valid / hit masks with the same window_pre /
- * on_result steps as the lane walk.  The replay counts contains() calls and
- * lookups exactly as the reference would make them; lookups of windows the
- * walk skips or never reaches are counted separately as issued-only.
- */
-template <int SC, bool EXH, int K>
-__device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
-                          Walker<SC, EXH, K> &w, uint32_t &issued) {
-    const uint32_t lane = lane_id();
-    const uint64_t b = a.off[read];
-    const uint32_t len = uint32_t(a.off[read + 1] - b);
-    const uint8_t *s = a.seq + b;
-    w.init(s, len, a.filters + f, a.wp);
-    w.dm = a.dust;
-    w.dbase = b;
-    w.probes = 0;
-    w.lookups = 0;
-    const unsigned kv = w.kk();
+/* The windows of a read for a whole wavefront, 64 per step: lane L of the
+ * chunk that starts at base b0 takes the window ending at base b0 + L, whose
+ * hash comes from prefix-XOR scans across lanes (bbt_device.h, pfx_*) and
+ * the prefixes carried from the previous chunk.  next() returns whether the
+ * window is N-free and complete, with its start and canonical hash. */
+struct ChunkHash {
     uint64_t Hc = 0, Gc = 0, Hp = 0, Gp = 0;
     int lastN = -1;
-    for (uint32_t b0 = 0; b0 < len && w.state == W_RUNNING; b0 += 64) {
+
+    __device__ __forceinline__ bool next(const uint8_t *s, uint32_t len, uint32_t b0,
+                                         unsigned kv, uint32_t lane, uint32_t &pw_out,
+                                         uint64_t &h0) {
         const uint32_t i = b0 + lane;
         const unsigned c = i < len ? unsigned(s[i]) : 0u;
         const bool ok = i < len && base_valid(c);
@@ -287,20 +273,53 @@ __device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
         const bool valid = pw >= 0 && i < len && int64_t(ln) < pw;
         const uint64_t wf = pfx_win_fwd(H, Hb, i);
         const uint64_t wr = pfx_win_rc(G, Gb, uint32_t(pw));
-        const uint64_t h0 = wr < wf ? wr : wf;
+        h0 = wr < wf ? wr : wf;
         Hc = __shfl(H, 63);
         Gc = __shfl(G, 63);
         lastN = __shfl(ln, 63);
         Hp = H;
         Gp = G;
+        pw_out = uint32_t(pw);
+        return valid;
+    }
+};
+
+/*
+ * One (read, filter) walk by a whole wavefront, 64 windows per step: the
+ * window hashes come from prefix-XOR scans across lanes (ChunkHash), every
+ * window the walk may probe is probed at once (each lane
+ * stops at its first clear bit), and the walk is then replayed in window
+ * order over the chunk's valid / hit masks with the same window_pre /
+ * on_result steps as the lane walk.  The replay counts contains() calls and
+ * lookups exactly as the reference would make them; lookups of windows the
+ * walk skips or never reaches are counted separately as issued-only.
+ */
+template <int SC, bool EXH, int K>
+__device__ void wave_walk(const EvalArgs &a, uint64_t read, uint32_t f,
+                          Walker<SC, EXH, K> &w, uint32_t &issued) {
+    const uint32_t lane = lane_id();
+    const uint64_t b = a.off[read];
+    const uint32_t len = uint32_t(a.off[read + 1] - b);
+    const uint8_t *s = a.seq + b;
+    w.init(s, len, a.filters + f, a.wp);
+    w.dm = a.dust;
+    w.dbase = b;
+    w.probes = 0;
+    w.lookups = 0;
+    const unsigned kv = w.kk();
+    ChunkHash ch;
+    for (uint32_t b0 = 0; b0 < len && w.state == W_RUNNING; b0 += 64) {
+        uint32_t pw;
+        uint64_t h0;
+        const bool valid = ch.next(s, len, b0, kv, lane, pw, h0);
         const uint64_t vmask = __ballot(valid);
         if (!vmask)
             continue;
         /* windows inside the current jump-skip are consumed without a probe,
          * and so are -D's low-complexity windows */
-        const bool low = valid && w.low_complexity(uint32_t(pw));
+        const bool low = valid && w.low_complexity(pw);
         const uint64_t dmask = __ballot(low);
-        bool need = valid && !low && !(w.skipping && uint32_t(pw) < w.skipEnd);
+        bool need = valid && !low && !(w.skipping && pw < w.skipEnd);
         bool allset = need;
         uint32_t lk = 0;
         for (uint32_t jj = 0; __any(need); ++jj) {
@@ -529,6 +548,123 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     add_issued(a.counters, issued);
 }
 
+/* Lookup counts of up to four filters packed 16 bits each: one more for
+ * every filter whose bit is set in need. */
+__device__ __forceinline__ uint64_t spread16(uint32_t need) {
+    return uint64_t(need & 1u) | (uint64_t(need & 2u) << 15) | (uint64_t(need & 4u) << 30) |
+           (uint64_t(need & 8u) << 45);
+}
+
+/*
+ * One read against every filter of a set of 3 or 4 filters interleaved as
+ * one group (4 bytes per bit position), by a whole wavefront, SIMPLE
+ * evalRead: the window hashes are computed once (ChunkHash), each lane's
+ * window is probed for every filter whose walk may probe it with one load
+ * per hash function (the group's bytes; a filter stops at its first clear
+ * bit, the lane when no filter needs the next one), and each filter's walk is
+ * then replayed over its own hit mask (Walker::replay), so its contains()
+ * calls and lookups are the reference's.  Only the walkers' automaton fields
+ * are used.  Lane 0 stores the hits and returns the walks' probes and
+ * lookups; issued counts per filter every bit loaded for it.
+ */
+template <int K>
+__device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t *ilv,
+                                uint32_t &issued, uint32_t &probes, uint32_t &lookups) {
+    constexpr uint32_t G = 4;
+    const uint32_t lane = lane_id();
+    const uint32_t nf = a.nf;
+    const uint64_t b = a.off[read];
+    const uint32_t len = uint32_t(a.off[read + 1] - b);
+    const uint8_t *s = a.seq + b;
+    Walker<SIMPLE, false, K> w[G];
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+        w[g].init(s, len, a.filters + (g < nf ? g : 0), a.wp);
+        w[g].dm = a.dust;
+        w[g].dbase = b;
+        w[g].probes = 0;
+        w[g].lookups = 0;
+        if (g >= nf)
+            w[g].state = W_REJECT;
+    }
+    const FilterDev *F0 = a.filters;
+    const uint64_t m = F0->m, magic = F0->magic;
+    const uint32_t sh1 = F0->sh1, sh2 = F0->sh2, h = F0->h;
+    const unsigned kv = w[0].kk();
+    const uint64_t ks = w[0].kseed();
+    const auto running = [&]() {
+        bool r = false;
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g)
+            r = r || w[g].state == W_RUNNING;
+        return r;
+    };
+    ChunkHash ch;
+    for (uint32_t b0 = 0; b0 < len && running(); b0 += 64) {
+        uint32_t pw;
+        uint64_t h0;
+        const bool valid = ch.next(s, len, b0, kv, lane, pw, h0);
+        const uint64_t vmask = __ballot(valid);
+        if (!vmask)
+            continue;
+        /* -D's low-complexity windows are misses without a probe for all */
+        const bool low = valid && w[0].low_complexity(pw);
+        const uint64_t dmask = __ballot(low);
+        uint32_t need = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g)
+            if (valid && !low && w[g].state == W_RUNNING && !(w[g].skipping && pw < w[g].skipEnd))
+                need |= 1u << g;
+        uint32_t all = need;
+        uint64_t lk = 0;
+        for (uint32_t jj = 0; __any(need != 0); ++jj) {
+            if (need) {
+                const uint64_t pos = fastmod(multi_hash(h0, jj, ks), m, magic, sh1, sh2);
+                const uint32_t word =
+                    ld_global(reinterpret_cast<const uint32_t *>(ilv + ((pos >> 3) << 2)));
+                const uint32_t bit = group_bits<2>(&word, unsigned(pos & 7));
+                lk += spread16(need);
+                issued += uint32_t(__popc(need));
+                all &= bit | ~need;
+                need &= bit;
+                if (jj + 1 >= h)
+                    need = 0;
+            }
+        }
+        /* per-filter lookups of lanes [x, y) from one packed prefix sum */
+        uint64_t lkp = lk;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t t = __shfl_up(lkp, unsigned(d));
+            if (int(lane) >= d)
+                lkp += t;
+        }
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g) {
+            const uint64_t hmask = __ballot((all >> g) & 1u);
+            if (w[g].state != W_RUNNING)
+                continue;
+            const auto lk_range = [&](uint32_t x, uint32_t y) -> uint32_t {
+                const uint64_t hi = __shfl(lkp, int(y - 1));
+                const uint64_t lo = x ? __shfl(lkp, int(x - 1)) : uint64_t(0);
+                return uint32_t(((hi - lo) >> (16 * g)) & 0xFFFFu);
+            };
+            w[g].replay(b0, vmask, hmask, dmask, lk_range);
+        }
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+        if (g >= nf)
+            continue;
+        if (w[g].state == W_RUNNING)
+            w[g].finish_end();
+        if (lane == 0) {
+            store_walk(a, read, g, w[g]);
+            probes += w[g].probes;
+            lookups += w[g].lookups;
+        }
+    }
+}
+
 /*
  * One read per lane against every filter of a set interleaved as one group,
  * SIMPLE evalRead (RowsWalk, bbt_device.h): the read's windows are hashed
@@ -545,7 +681,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
     __syncthreads();
     const uint32_t lane = lane_id();
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
-    const uint64_t nreads = a.nitems / a.nf;
     const FilterDev *F0 = a.filters;
     const uint64_t m = F0->m, magic = F0->magic;
     const uint32_t sh1 = F0->sh1, sh2 = F0->sh2;
@@ -571,7 +706,22 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
         return group_bits<SH>(w, unsigned(pos & 7)) & nfmask;
     };
     RowsWalk<K, 16> rw;
-    uint32_t probes = 0, lookups = 0;
+    uint32_t probes = 0, lookups = 0, issued = 0;
+    /* with length buckets (sets of 3 or 4 filters): the long reads first, a
+     * whole wave per read against all filters, longest bucket first */
+    if (SH == 2 && a.list_counts) {
+        const uint64_t nl = a.list_counts[0];
+        while (true) {
+            unsigned long long r = 0;
+            if (lane == 0)
+                r = atomicAdd(a.lqueue, 1ull);
+            r = __shfl(r, 0);
+            if (r >= nl)
+                break;
+            wave_group_walk<K>(a, a.long_list[r], ilv, issued, probes, lookups);
+        }
+    }
+    const uint64_t nreads = a.list_counts ? a.list_counts[1] : a.nitems / a.nf;
     bool active = false, drained = false;
     uint64_t read = 0;
     while (true) {
@@ -609,11 +759,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
                 a.hits[read * a.nf + f] = uint8_t((rw.acc >> f) & 1u);
             probes += rw.probes;
             lookups += rw.lookups;
+            issued += rw.lookups;
             active = false;
         }
     }
     add_counters(a.counters, probes, lookups);
-    add_issued(a.counters, lookups);
+    add_issued(a.counters, issued);
 }
 
 /* Length buckets for the batched path: reads of at least wave_min bases go
@@ -2346,11 +2497,15 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     P.wave_min = p->wave_min_len ? p->wave_min_len
                                  : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
     /* probe_ahead bit 3: one read per lane against every filter, for a set
-     * interleaved as one group of 3 to 16, SIMPLE evalRead, reads that are
-     * all lane walks and shorter than 32768 bases */
+     * interleaved as one group of 3 to 16, SIMPLE evalRead, lane-walked reads
+     * shorter than 32768 bases; with reads that reach the whole-wave length
+     * (length buckets), a set of 3 or 4 whose long reads a whole wave walks
+     * against all filters at once */
+    const bool buckets = in->max_len >= P.wave_min;
     P.rows = (tune.probe_ahead & 8u) && P.group && fs->ilv.size() == 1 &&
              fs->ilv[0].mem.n == nf && nf >= 3 && nf <= 16 && p->scorer == BBT_SIMPLE &&
-             p->mode == BBT_STD && in->max_len < 32768 && in->max_len < P.wave_min;
+             p->mode == BBT_STD &&
+             (buckets ? nf <= 4 && P.wave_min <= 32768 : in->max_len < 32768);
     if (P.rows)
         a.filters = fs->ddesc;
     return BBT_OK;

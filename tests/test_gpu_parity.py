@@ -269,13 +269,15 @@ def test_partial_interleave_groups(dev):
         check_batch(D, B, fs, ofs, buf, off)
 
 
-@pytest.mark.parametrize("nf", [3, 6, 12, 16])
+@pytest.mark.parametrize("nf", [3, 4, 6, 12, 16])
 def test_rows_walk_group_sizes(dev, nf):
     """The read-per-lane walk (probe_ahead 12) over sets of 3 to 16 filters of
     one geometry, i.e. interleaved groups of 4, 8 and 16 bytes per position:
     hits, classes and reference counters equal the oracle's with and without
-    -D, single-end and paired; reads that reach the whole-wave length leave
-    the batch on the other walks, with the same results."""
+    -D, single-end and paired.  With reads that reach the whole-wave length,
+    sets of 3 and 4 walk each long read by a whole wave against all filters
+    at once (the others leave the batch on the other walks); the same
+    results."""
     D, B = dev
     from biobloom_amd import synth
     rng = synth.rng_for(45, nf)
@@ -298,6 +300,14 @@ def test_rows_walk_group_sizes(dev, nf):
         check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
         check_batch(D, B, fs, ofs, buf, off, paired=True, inclusive=True)
         check_batch(D, B, fs, ofs, lbuf, loff)
+        if nf <= 4:
+            print(f"[gpu] rows walk with long reads nf={nf}", flush=True)
+            check_batch(D, B, fs, ofs, lbuf, loff, thr=0.0, streak=0)
+            check_batch(D, B, fs, ofs, lbuf, loff, thr=0.9, streak=5)
+            check_batch(D, B, fs, ofs, lbuf, loff, streak=1)
+            check_batch(D, B, fs, ofs, lbuf, loff, dust=(20, 64))
+            check_batch(D, B, fs, ofs, lbuf, loff, paired=True, wave_min=100)
+            check_batch(D, B, fs, ofs, lbuf, loff, wave_min=64, thr=0.5)
     finally:
         fs.set_tuning()
 
