@@ -92,7 +92,11 @@ struct EvalArgs {
     const unsigned long long *bucket_cnt;
     const uint32_t *dust;       /* -D: bit (off[read] + p) = window p is low complexity */
     uint64_t nitems;
-    uint32_t nf;
+    uint32_t nf;  /* filters of the set: the stride of hits and scores */
+    /* filters this launch walks: nw of them, walk index i is filter fmap[i]
+     * (fmap null: all nf, in set order) */
+    const uint32_t *fmap;
+    uint32_t nw;
     uint32_t refill_min;
     uint32_t probe_ahead; /* lane walks: issue a contains() call's lookups together */
     /* lane items read by read (item = read * nf + filter): the lanes of one
@@ -198,9 +202,11 @@ __device__ __forceinline__ void start_walk(const EvalArgs &a, uint64_t item, uin
         f = uint32_t(item / nr);
         r = item - uint64_t(f) * nr;
     } else {
-        r = item / a.nf;
-        f = uint32_t(item - r * a.nf);
+        r = item / a.nw;
+        f = uint32_t(item - r * a.nw);
     }
+    if (a.fmap)
+        f = a.fmap[f];
     read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
     const uint64_t b = a.off[read];
     w.init(a.seq + b, uint32_t(a.off[read + 1] - b), a.filters + f, a.wp);
@@ -366,9 +372,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     uint32_t issued = 0;
     uint32_t wave_probes = 0, wave_lookups = 0; /* lane 0 only */
     const uint64_t nl = a.list_counts ? a.list_counts[0] : 0;
-    const uint64_t nr = a.list_counts ? a.list_counts[1] : (a.nf ? a.nitems / a.nf : 0);
-    const uint64_t nlong = nl * a.nf;
-    const uint64_t nitems = nr * a.nf;
+    const uint64_t nr = a.list_counts ? a.list_counts[1] : (a.nw ? a.nitems / a.nw : 0);
+    const uint64_t nlong = nl * a.nw;
+    const uint64_t nitems = nr * a.nw;
 
     /* phase 1: long reads, a whole wave per (read, filter): length bucket by
      * length bucket, longest first, and filter-major inside a bucket.  The
@@ -394,13 +400,13 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
                 /* the bucket whose item range [base nf, (base + cnt) nf) holds it */
                 for (int b = 31; b >= 0; --b) {
                     const unsigned long long c = s_bcnt[b];
-                    const unsigned long long lo = s_bbase[b] * a.nf;
-                    if (c && it < lo + c * a.nf) {
+                    const unsigned long long lo = s_bbase[b] * a.nw;
+                    if (c && it < lo + c * a.nw) {
                         const unsigned long long q = it - lo;
                         if (a.read_major) {
                             /* the filters of one read side by side */
-                            const unsigned long long rb = q / a.nf;
-                            fr = ((q - rb * a.nf) << 40) | (s_bbase[b] + rb);
+                            const unsigned long long rb = q / a.nw;
+                            fr = ((q - rb * a.nw) << 40) | (s_bbase[b] + rb);
                         } else {
                             const unsigned long long fb = q / c;
                             fr = (fb << 40) | (s_bbase[b] + (q - fb * c));
@@ -414,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
         if (it >= nlong)
             break;
         fr = __shfl(fr, 0);
-        const uint32_t f = uint32_t(fr >> 40);
+        const uint32_t f = a.fmap ? a.fmap[fr >> 40] : uint32_t(fr >> 40);
         const uint64_t r = fr & ((uint64_t(1) << 40) - 1);
         const uint64_t read = a.long_list[r];
         Walker<SC, EXH, K> ww;
@@ -548,17 +554,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(EvalArgs a) {
     add_issued(a.counters, issued);
 }
 
-/* Lookup counts of up to four filters packed 16 bits each: one more for
- * every filter whose bit is set in need. */
+/* Lookup counts of four filters packed 16 bits each: one more for every
+ * filter whose bit is set in the low four bits of need. */
 __device__ __forceinline__ uint64_t spread16(uint32_t need) {
     return uint64_t(need & 1u) | (uint64_t(need & 2u) << 15) | (uint64_t(need & 4u) << 30) |
            (uint64_t(need & 8u) << 45);
 }
 
 /*
- * One read against every filter of a set of 3 or 4 filters interleaved as
- * one group (4 bytes per bit position), by a whole wavefront, SIMPLE
- * evalRead: the window hashes are computed once (ChunkHash), each lane's
+ * One read against the 3 to 2^SH filters of an interleaved group (2^SH = 4
+ * or 8 bytes per bit position; walk index g is filter fmap[g], a.nw of
+ * them), by a whole wavefront, SIMPLE evalRead: the window hashes are
+ * computed once (ChunkHash), each lane's
  * window is probed for every filter whose walk may probe it with one load
  * per hash function (the group's bytes; a filter stops at its first clear
  * bit, the lane when no filter needs the next one), and each filter's walk is
@@ -567,19 +574,20 @@ __device__ __forceinline__ uint64_t spread16(uint32_t need) {
  * are used.  Lane 0 stores the hits and returns the walks' probes and
  * lookups; issued counts per filter every bit loaded for it.
  */
-template <int K>
+template <int K, unsigned SH>
 __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t *ilv,
                                 uint32_t &issued, uint32_t &probes, uint32_t &lookups) {
-    constexpr uint32_t G = 4;
+    constexpr uint32_t G = 1u << SH;
     const uint32_t lane = lane_id();
-    const uint32_t nf = a.nf;
+    const uint32_t nf = a.nw;
     const uint64_t b = a.off[read];
     const uint32_t len = uint32_t(a.off[read + 1] - b);
     const uint8_t *s = a.seq + b;
+    const auto fidx = [&](uint32_t g) -> uint32_t { return a.fmap ? a.fmap[g] : g; };
     Walker<SIMPLE, false, K> w[G];
 #pragma unroll
     for (uint32_t g = 0; g < G; ++g) {
-        w[g].init(s, len, a.filters + (g < nf ? g : 0), a.wp);
+        w[g].init(s, len, a.filters + fidx(g < nf ? g : 0), a.wp);
         w[g].dm = a.dust;
         w[g].dbase = b;
         w[g].probes = 0;
@@ -587,7 +595,7 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
         if (g >= nf)
             w[g].state = W_REJECT;
     }
-    const FilterDev *F0 = a.filters;
+    const FilterDev *F0 = a.filters + fidx(0);
     const uint64_t m = F0->m, magic = F0->magic;
     const uint32_t sh1 = F0->sh1, sh2 = F0->sh2, h = F0->h;
     const unsigned kv = w[0].kk();
@@ -616,14 +624,22 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
             if (valid && !low && w[g].state == W_RUNNING && !(w[g].skipping && pw < w[g].skipEnd))
                 need |= 1u << g;
         uint32_t all = need;
-        uint64_t lk = 0;
+        uint64_t lk = 0, lk2 = 0; /* filters 0-3 and 4-7, 16 bits each */
         for (uint32_t jj = 0; __any(need != 0); ++jj) {
             if (need) {
                 const uint64_t pos = fastmod(multi_hash(h0, jj, ks), m, magic, sh1, sh2);
-                const uint32_t word =
-                    ld_global(reinterpret_cast<const uint32_t *>(ilv + ((pos >> 3) << 2)));
-                const uint32_t bit = group_bits<2>(&word, unsigned(pos & 7));
+                uint32_t word[2];
+                if (SH == 3) {
+                    const uint2 v = ld_global(reinterpret_cast<const uint2 *>(ilv + ((pos >> 3) << 3)));
+                    word[0] = v.x;
+                    word[1] = v.y;
+                } else {
+                    word[0] = ld_global(reinterpret_cast<const uint32_t *>(ilv + ((pos >> 3) << 2)));
+                }
+                const uint32_t bit = group_bits<SH>(word, unsigned(pos & 7));
                 lk += spread16(need);
+                if (SH == 3)
+                    lk2 += spread16(need >> 4);
                 issued += uint32_t(__popc(need));
                 all &= bit | ~need;
                 need &= bit;
@@ -631,12 +647,15 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
                     need = 0;
             }
         }
-        /* per-filter lookups of lanes [x, y) from one packed prefix sum */
-        uint64_t lkp = lk;
+        /* per-filter lookups of lanes [x, y) from packed prefix sums */
+        uint64_t lkp = lk, lkp2 = lk2;
         for (int d = 1; d < 64; d <<= 1) {
             const uint64_t t = __shfl_up(lkp, unsigned(d));
-            if (int(lane) >= d)
+            const uint64_t t2 = SH == 3 ? __shfl_up(lkp2, unsigned(d)) : uint64_t(0);
+            if (int(lane) >= d) {
                 lkp += t;
+                lkp2 += t2;
+            }
         }
 #pragma unroll
         for (uint32_t g = 0; g < G; ++g) {
@@ -644,9 +663,10 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
             if (w[g].state != W_RUNNING)
                 continue;
             const auto lk_range = [&](uint32_t x, uint32_t y) -> uint32_t {
-                const uint64_t hi = __shfl(lkp, int(y - 1));
-                const uint64_t lo = x ? __shfl(lkp, int(x - 1)) : uint64_t(0);
-                return uint32_t(((hi - lo) >> (16 * g)) & 0xFFFFu);
+                const uint64_t src = g < 4 ? lkp : lkp2;
+                const uint64_t hi = __shfl(src, int(y - 1));
+                const uint64_t lo = x ? __shfl(src, int(x - 1)) : uint64_t(0);
+                return uint32_t(((hi - lo) >> (16 * (g & 3))) & 0xFFFFu);
             };
             w[g].replay(b0, vmask, hmask, dmask, lk_range);
         }
@@ -658,11 +678,32 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
         if (w[g].state == W_RUNNING)
             w[g].finish_end();
         if (lane == 0) {
-            store_walk(a, read, g, w[g]);
+            store_walk(a, read, fidx(g), w[g]);
             probes += w[g].probes;
             lookups += w[g].lookups;
         }
     }
+}
+
+/* The long reads of a batch with length buckets (a.long_list, longest bucket
+ * first) against a group of 3 to 2^SH filters, a whole wave per read
+ * (wave_group_walk); the short reads are k_walk_rows's. */
+template <int K, unsigned SH>
+__global__ __launch_bounds__(kBlock) void k_walk_group_long(EvalArgs a, const uint8_t *ilv) {
+    const uint32_t lane = lane_id();
+    uint32_t probes = 0, lookups = 0, issued = 0;
+    const uint64_t nl = a.list_counts[0];
+    while (true) {
+        unsigned long long r = 0;
+        if (lane == 0)
+            r = atomicAdd(a.lqueue, 1ull);
+        r = __shfl(r, 0);
+        if (r >= nl)
+            break;
+        wave_group_walk<K, SH>(a, a.long_list[r], ilv, issued, probes, lookups);
+    }
+    add_counters(a.counters, probes, lookups);
+    add_issued(a.counters, issued);
 }
 
 /*
@@ -681,11 +722,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
     __syncthreads();
     const uint32_t lane = lane_id();
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
-    const FilterDev *F0 = a.filters;
+    const FilterDev *F0 = a.filters + (a.fmap ? a.fmap[0] : 0u);
     const uint64_t m = F0->m, magic = F0->magic;
     const uint32_t sh1 = F0->sh1, sh2 = F0->sh2;
     const uint64_t ks = uint64_t(K ? K : F0->k) * kMultiSeed;
-    const uint32_t nfmask = a.nf >= 32 ? ~0u : (1u << a.nf) - 1u;
+    const uint32_t nfmask = a.nw >= 32 ? ~0u : (1u << a.nw) - 1u;
     const auto fetch = [&](uint64_t hv, uint32_t j) -> uint32_t {
         const uint64_t pos = fastmod(multi_hash(hv, j, ks), m, magic, sh1, sh2);
         const uint8_t *g = ilv + ((pos >> 3) << SH);
@@ -707,21 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
     };
     RowsWalk<K, 16> rw;
     uint32_t probes = 0, lookups = 0, issued = 0;
-    /* with length buckets (sets of 3 or 4 filters): the long reads first, a
-     * whole wave per read against all filters, longest bucket first */
-    if (SH == 2 && a.list_counts) {
-        const uint64_t nl = a.list_counts[0];
-        while (true) {
-            unsigned long long r = 0;
-            if (lane == 0)
-                r = atomicAdd(a.lqueue, 1ull);
-            r = __shfl(r, 0);
-            if (r >= nl)
-                break;
-            wave_group_walk<K>(a, a.long_list[r], ilv, issued, probes, lookups);
-        }
-    }
-    const uint64_t nreads = a.list_counts ? a.list_counts[1] : a.nitems / a.nf;
+    const uint64_t nreads = a.list_counts ? a.list_counts[1] : a.nitems / a.nw;
     bool active = false, drained = false;
     uint64_t read = 0;
     while (true) {
@@ -740,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
                 if (r < nreads) {
                     read = a.short_list ? a.short_list[r] : (a.order ? a.order[r] : r);
                     const uint64_t b = a.off[read];
-                    rw.init(a.seq + b, uint32_t(a.off[read + 1] - b), F0, a.nf, a.wp);
+                    rw.init(a.seq + b, uint32_t(a.off[read + 1] - b), F0, a.nw, a.wp);
                     if (K)
                         rw.w.rt = roll;
                     rw.w.dm = a.dust;
@@ -755,8 +782,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t 
             continue;
         }
         if (active && !rw.step(fetch)) {
-            for (uint32_t f = 0; f < a.nf; ++f)
-                a.hits[read * a.nf + f] = uint8_t((rw.acc >> f) & 1u);
+            for (uint32_t g = 0; g < a.nw; ++g)
+                a.hits[read * a.nf + (a.fmap ? a.fmap[g] : g)] = uint8_t((rw.acc >> g) & 1u);
             probes += rw.probes;
             lookups += rw.lookups;
             issued += rw.lookups;
@@ -1419,6 +1446,9 @@ struct bbt_filterset {
     std::vector<IlvGroup> ilv;
     FilterDev *ddesc_ilv = nullptr;
     bool ilv_desc_fresh = false;
+    /* with one group: its members, then the other filters (walked-filter
+     * maps of the read-per-lane walk and of the walk of the rest) */
+    uint32_t *dmap = nullptr;
 };
 
 static void fill_desc(FilterDev &d, void *bits, uint64_t m, uint32_t h, uint32_t k) {
@@ -1463,7 +1493,13 @@ static bbt_status sync_desc(bbt_filterset *fs) {
  * geometry make further groups) whose copy, the member count rounded up to a
  * power of two times m/8 bytes, stays within kIlvMaxBytes. */
 constexpr uint64_t kIlvMaxBytes = uint64_t(32) << 30;
-constexpr uint32_t kDefaultIlvAhead = 15;
+constexpr uint32_t kDefaultIlvAhead = 7;
+/* ... and with the read-per-lane walk, for batches of at least
+ * kRowsMinReads reads: a lone long read is walked against all of a group's
+ * filters by one wave, which sets the latency of a small launch (a serving
+ * request of 10k reads against the serving set: 2.7 -> 7.4 ms) */
+constexpr uint32_t kDefaultRowsAhead = 15;
+constexpr uint64_t kRowsMinReads = 65536;
 
 static std::vector<bbt_filterset::IlvGroup> ilv_plan(const bbt_filterset *fs) {
     std::vector<bbt_filterset::IlvGroup> out;
@@ -1506,6 +1542,24 @@ static bool ilv_whole(const bbt_filterset *fs) {
     return plan.size() == 1 && plan[0].mem.n == fs->desc.size();
 }
 
+/* The read-per-lane walk (probe_ahead bit 3) applies to a batch: the set's
+ * plan has one group, of 3 to 16 filters, SIMPLE evalRead in STD mode on the
+ * batched path, and the lane-walked reads shorter than 32768 bases; when
+ * some reads reach the whole-wave length the group has at most 8 filters
+ * (the whole-wave walk of a long read against all of them).  Returns the
+ * group's size, or 0. */
+static uint32_t rows_group(const std::vector<bbt_filterset::IlvGroup> &plan,
+                           const bbt_eval_params *p, uint64_t max_len, uint32_t wave_min) {
+    if (plan.size() != 1 || p->scorer != BBT_SIMPLE || p->mode != BBT_STD ||
+        p->path != BBT_PATH_BATCHED)
+        return 0;
+    const uint32_t gn = plan[0].mem.n;
+    const bool buckets = max_len >= wave_min;
+    const bool ok = gn >= 3 && gn <= 16 &&
+                    (buckets ? gn <= 8 && wave_min <= 32768 : max_len < 32768);
+    return ok ? gn : 0;
+}
+
 /* The interleaved copies' memory and descriptors (not their contents: the
  * batch's k_interleave writes them); false when no group qualifies. */
 static bbt_status ensure_ilv(bbt_filterset *fs, bool &ok) {
@@ -1542,6 +1596,20 @@ static bbt_status ensure_ilv(bbt_filterset *fs, bool &ok) {
                 v[g.mem.f[q]].bshift = g.shift;
             }
         HIP_OK(hipMemcpy(fs->ddesc_ilv, v.data(), sizeof(FilterDev) * n, hipMemcpyHostToDevice));
+        if (fs->ilv.size() == 1) {
+            std::vector<uint32_t> map;
+            std::vector<bool> in_group(n, false);
+            for (uint32_t q = 0; q < fs->ilv[0].mem.n; ++q) {
+                map.push_back(fs->ilv[0].mem.f[q]);
+                in_group[fs->ilv[0].mem.f[q]] = true;
+            }
+            for (uint32_t f = 0; f < n; ++f)
+                if (!in_group[f])
+                    map.push_back(f);
+            if (!fs->dmap)
+                HIP_OK(hipMalloc(&fs->dmap, sizeof(uint32_t) * n));
+            HIP_OK(hipMemcpy(fs->dmap, map.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+        }
         fs->ilv_desc_fresh = true;
     }
     ok = true;
@@ -1679,6 +1747,8 @@ bbt_status bbt_filters_free(bbt_filterset *fs) {
             (void)hipFree(g.buf);
     if (fs->ddesc_ilv)
         (void)hipFree(fs->ddesc_ilv);
+    if (fs->dmap)
+        (void)hipFree(fs->dmap);
     workspace_release(&fs->ws);
     delete fs;
     return BBT_OK;
@@ -2031,7 +2101,7 @@ bbt_status launch_rows_k(const Launch &L, const uint8_t *ilv) {
     int bpc = per_cu;
     if (L.blocks_per_cu)
         bpc = std::min<int>(bpc, int(L.blocks_per_cu));
-    const uint64_t nreads = L.a.nitems / L.a.nf;
+    const uint64_t nreads = L.a.nitems / L.a.nw;
     const uint64_t blocks = std::max<uint64_t>(
         1, std::min<uint64_t>(uint64_t(L.cu) * uint64_t(bpc), (nreads + kBlock - 1) / kBlock));
     k_walk_rows<K, SH><<<dim3(uint32_t(blocks)), kBlock, 0, L.st>>>(L.a, ilv);
@@ -2039,9 +2109,38 @@ bbt_status launch_rows_k(const Launch &L, const uint8_t *ilv) {
     return BBT_OK;
 }
 
+template <int K, unsigned SH>
+bbt_status launch_group_long_k(const Launch &L, const uint8_t *ilv) {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_walk_group_long<K, SH>, kBlock,
+                                                         0) != hipSuccess)
+            v = 1;
+        per_cu = std::max(1, v);
+    }
+    int bpc = per_cu;
+    if (L.blocks_per_cu)
+        bpc = std::min<int>(bpc, int(L.blocks_per_cu));
+    /* the long-read count is on the device: a full persistent grid, whose
+     * waves leave at once when the list is short */
+    k_walk_group_long<K, SH><<<dim3(uint32_t(L.cu) * uint32_t(bpc)), kBlock, 0, L.st>>>(L.a, ilv);
+    HIP_OK(hipGetLastError());
+    return BBT_OK;
+}
+
 bbt_status launch_rows(const Launch &L, const uint8_t *ilv, uint32_t sh, uint32_t k) {
     if (L.a.nitems == 0)
         return BBT_OK;
+    if (L.a.list_counts) {
+        /* length buckets: groups of up to 8 filters (rows_group) */
+        const bbt_status s = k == 25 ? (sh == 3 ? launch_group_long_k<25, 3>(L, ilv)
+                                                : launch_group_long_k<25, 2>(L, ilv))
+                                     : (sh == 3 ? launch_group_long_k<0, 3>(L, ilv)
+                                                : launch_group_long_k<0, 2>(L, ilv));
+        if (s != BBT_OK)
+            return s;
+    }
     if (k == 25)
         return sh == 4 ? launch_rows_k<25, 4>(L, ilv)
                        : (sh == 3 ? launch_rows_k<25, 3>(L, ilv) : launch_rows_k<25, 2>(L, ilv));
@@ -2231,6 +2330,7 @@ struct Prepared {
     bool dust_big;        /* -D: handed-on reads in a workgroup's LDS (k_dust_big) */
     bool group;           /* read-major lane walks over the interleaved filters */
     bool rows;            /* one read per lane against every filter (k_walk_rows) */
+    uint32_t rows_n;      /* the filters it walks: the group's, fs->dmap[0 .. rows_n) */
 };
 
 /* Whether k_dust_big can have `bytes` of dynamic LDS per workgroup on the
@@ -2446,13 +2546,25 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     /* a set of one geometry without knobs of its own or in the table: the
      * interleaved read-major walks, four lookups at a time, with lookahead
      * (10 x 20 Mbp filters: 95.2 -> 52.4 ms per 4M reads), and one read per
-     * lane where that applies (24.2 ms); without allocation the same load
-     * scheduling over the filters' own arrays.  A
-     * set with only part of its filters in a group keeps filter-major walks
-     * over their own arrays (the serving set, one big filter and eight of one
-     * geometry: 57.9 ms against 67.2 ms interleaved) */
-    if (!fs->tuned && !from_table && ilv_whole(fs))
-        tune.probe_ahead = noalloc ? (kDefaultIlvAhead & 3u) : kDefaultIlvAhead;
+     * lane where that applies to a large batch (24.2 ms); without allocation
+     * the same load scheduling over the filters' own arrays.  A
+     * set with only part of its filters in a group takes the read-per-lane
+     * walk for the group and filter-major walks over their own arrays for
+     * the others where the read-per-lane walk applies to a large batch
+     * (the serving set, one big filter and eight of one geometry, 516k
+     * reads: 57.8 -> 36.8 ms), else filter-major walks for all (57.9 ms
+     * against 67.2 ms with the read-major walks) */
+    P.wave_min = p->wave_min_len ? p->wave_min_len
+                                 : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
+    if (!fs->tuned && !from_table) {
+        const bool rows_fit = !noalloc && in->nreads >= kRowsMinReads &&
+                              rows_group(ilv_plan(fs), p, in->max_len, P.wave_min) != 0;
+        if (ilv_whole(fs))
+            tune.probe_ahead =
+                noalloc ? (kDefaultIlvAhead & 3u) : (rows_fit ? kDefaultRowsAhead : kDefaultIlvAhead);
+        else if (rows_fit)
+            tune.probe_ahead = kDefaultRowsAhead;
+    }
     L.blocks_per_cu = tune.blocks_per_cu;
     EvalArgs &a = L.a;
     a.seq = reinterpret_cast<const uint8_t *>(in->seq);
@@ -2480,6 +2592,8 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     a.scores = out->scores;
     a.counters = reinterpret_cast<unsigned long long *>(out->counters);
     a.nf = nf;
+    a.nw = nf;
+    a.fmap = nullptr;
     a.refill_min = tune.refill_min ? tune.refill_min : kDefaultRefillMin;
     a.probe_ahead = tune.probe_ahead & 3u;
     if (fix_scores) {
@@ -2494,18 +2608,14 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
     P.nf = nf;
     P.nrec = p->paired ? in->nreads / 2 : in->nreads;
     P.kk = common_k == 25 ? 25u : 0u;
-    P.wave_min = p->wave_min_len ? p->wave_min_len
-                                 : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
-    /* probe_ahead bit 3: one read per lane against every filter, for a set
-     * interleaved as one group of 3 to 16, SIMPLE evalRead, lane-walked reads
-     * shorter than 32768 bases; with reads that reach the whole-wave length
-     * (length buckets), a set of 3 or 4 whose long reads a whole wave walks
-     * against all filters at once */
-    const bool buckets = in->max_len >= P.wave_min;
-    P.rows = (tune.probe_ahead & 8u) && P.group && fs->ilv.size() == 1 &&
-             fs->ilv[0].mem.n == nf && nf >= 3 && nf <= 16 && p->scorer == BBT_SIMPLE &&
-             p->mode == BBT_STD &&
-             (buckets ? nf <= 4 && P.wave_min <= 32768 : in->max_len < 32768);
+    /* probe_ahead bit 3: the group's filters by the read-per-lane walk (and
+     * whole-wave walks of long reads against all of them); the set's other
+     * filters, if any, by the lane and wave walks over their own arrays,
+     * filter-major, in a second launch */
+    P.rows_n = P.group && (tune.probe_ahead & 8u)
+                   ? rows_group(fs->ilv, p, in->max_len, P.wave_min)
+                   : 0;
+    P.rows = P.rows_n != 0;
     if (P.rows)
         a.filters = fs->ddesc;
     return BBT_OK;
@@ -2624,8 +2734,27 @@ static bbt_status enqueue(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
             s = launch_scorer(L, p->scorer, true, P.kk);
         break;
     default:
-        s = P.rows ? launch_rows(L, fs->ilv[0].buf, fs->ilv[0].shift, P.kk)
-                   : launch_scorer(L, p->scorer, false, P.kk);
+        if (P.rows) {
+            Launch Lr = L;
+            Lr.a.fmap = fs->dmap;
+            Lr.a.nw = P.rows_n;
+            Lr.a.nitems = in->nreads * P.rows_n;
+            s = launch_rows(Lr, fs->ilv[0].buf, fs->ilv[0].shift, P.kk);
+            if (s == BBT_OK && P.rows_n < nf) {
+                /* the others: filter-major walks over their own arrays, with
+                 * queues of their own */
+                Launch Lo = L;
+                Lo.a.fmap = fs->dmap + P.rows_n;
+                Lo.a.nw = nf - P.rows_n;
+                Lo.a.nitems = in->nreads * (nf - P.rows_n);
+                Lo.a.read_major = 0;
+                Lo.a.queue = W.dctl + 1;
+                Lo.a.lqueue = W.dctl + 3;
+                s = launch_scorer(Lo, p->scorer, false, P.kk);
+            }
+        } else {
+            s = launch_scorer(L, p->scorer, false, P.kk);
+        }
         break;
     }
     if (s != BBT_OK)
@@ -2675,6 +2804,7 @@ static std::string graph_key(const bbt_filterset *fs, const bbt_reads *in,
     add(&P.L.a.probe_ahead, sizeof(P.L.a.probe_ahead));
     add(&P.L.a.read_major, sizeof(P.L.a.read_major));
     add(&P.rows, sizeof(P.rows));
+    add(&P.rows_n, sizeof(P.rows_n));
     add(&P.L.blocks_per_cu, sizeof(P.L.blocks_per_cu));
     add(&st, sizeof(st));
     return k;
@@ -2979,7 +3109,8 @@ extern "C" bbt_status bbt_autotune(bbt_filterset *fs, const bbt_reads *in,
     const uint32_t aheads[] = {0, 1, 2, 3, 4, 5, 6, 7, 12};
     /* the interleaved forms only for a set with a group of one geometry; the
      * read-per-lane walk (12) where it applies */
-    const bool rows_ok = ilv_whole(fs) && fs->desc.size() >= 3 && fs->desc.size() <= 16 &&
+    const auto plan = ilv_plan(fs);
+    const bool rows_ok = plan.size() == 1 && plan[0].mem.n >= 3 && plan[0].mem.n <= 16 &&
                          p->scorer == BBT_SIMPLE && p->mode == BBT_STD;
     const int nahead =
         ilv_any(fs) && p->mode != BBT_ORDERED ? (rows_ok ? 9 : 8) : 4;

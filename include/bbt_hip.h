@@ -115,15 +115,24 @@ typedef struct {
                                its filters of a group (the copies are made per
                                batch; bbt_eval_batch only, ignored by ORDERED
                                mode and for sets without such a group); bit 3
-                               with bit 2 (read per lane): a set of 3 to 16
-                               filters of one geometry, SIMPLE evalRead in STD
-                               mode, reads all shorter than wave_min_len and
-                               32768: each lane hashes one read once and reads
-                               all its filters with one load per position.
-                               Values above 15 are rejected.  A set whose
-                               filters all share one geometry, with no knobs of
-                               its own (and none in the tuning table), takes
-                               15. */
+                               with bit 2 (read per lane): a group of 3 to 16
+                               filters of one geometry (the set's only
+                               group), SIMPLE evalRead in STD mode, lane-walked
+                               reads shorter than 32768: each lane hashes one
+                               read once and reads all the group's filters
+                               with one load per position; with reads of
+                               wave_min_len or more (at most 32768) and a group
+                               of at most 8, each long read is walked by one
+                               wave against all of them; the set's other
+                               filters are walked over their own arrays.
+                               Values above 15 are rejected.  Without knobs
+                               of its own (or in the tuning table) a set whose
+                               filters all share one geometry takes 7, and 15
+                               for a batch of 65536 reads or more where bit 3
+                               applies; a set with one such group of 3 to 16
+                               filters and others takes 15 for such a batch
+                               (the group read per lane, the others over their
+                               own arrays). */
 } bbt_tuning;
 
 typedef struct {

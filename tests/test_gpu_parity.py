@@ -275,8 +275,8 @@ def test_rows_walk_group_sizes(dev, nf):
     one geometry, i.e. interleaved groups of 4, 8 and 16 bytes per position:
     hits, classes and reference counters equal the oracle's with and without
     -D, single-end and paired.  With reads that reach the whole-wave length,
-    sets of 3 and 4 walk each long read by a whole wave against all filters
-    at once (the others leave the batch on the other walks); the same
+    sets of 3 to 8 walk each long read by a whole wave against all filters
+    at once (larger sets leave the batch on the other walks); the same
     results."""
     D, B = dev
     from biobloom_amd import synth
@@ -300,7 +300,7 @@ def test_rows_walk_group_sizes(dev, nf):
         check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
         check_batch(D, B, fs, ofs, buf, off, paired=True, inclusive=True)
         check_batch(D, B, fs, ofs, lbuf, loff)
-        if nf <= 4:
+        if nf <= 8:
             print(f"[gpu] rows walk with long reads nf={nf}", flush=True)
             check_batch(D, B, fs, ofs, lbuf, loff, thr=0.0, streak=0)
             check_batch(D, B, fs, ofs, lbuf, loff, thr=0.9, streak=5)
@@ -308,6 +308,50 @@ def test_rows_walk_group_sizes(dev, nf):
             check_batch(D, B, fs, ofs, lbuf, loff, dust=(20, 64))
             check_batch(D, B, fs, ofs, lbuf, loff, paired=True, wave_min=100)
             check_batch(D, B, fs, ofs, lbuf, loff, wave_min=64, thr=0.5)
+    finally:
+        fs.set_tuning()
+
+
+@pytest.mark.parametrize("ng", [3, 5, 8])
+def test_rows_walk_partial_group(dev, ng):
+    """A set of one larger filter and ng filters of one geometry (the serving
+    set's shape): the group by the read-per-lane walk (and whole-wave walks
+    of long reads against all of it), the larger filter by the lane and wave
+    walks over its own array in a second launch.  With probe_ahead 12, and
+    by default (no knobs: the library takes 15 for SIMPLE in STD mode on a
+    batch of 65536 reads or more, here a batch of 66k reads; smaller batches
+    keep the lane and wave walks), hits, classes and reference counters equal
+    the oracle's; LENGTH, which the read-per-lane walk does not take, too."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(46, ng)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 150_000))]
+    genomes += [synth.codes_to_ascii(synth.random_codes(rng, 40_000)) for _ in range(ng)]
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 800, 150, 0.01, 0.7)
+    longs, _ = synth.long_reads(rng, genomes, 24, median=800, hi=4000)
+    reads = [bytes(x) for x in shorts] + [bytes(x) for x in longs]
+    reads += [b"", b"ACGT", b"N" * 300, bytes(longs[0]).lower(), b"ACGTN" * 80, b"C" * 25]
+    buf, off = O.pack_reads(reads)  # an even count: pairs
+    sbuf, soff = O.pack_reads([bytes(x) for x in shorts])
+    fs, _ = D.build_filters(genomes, 25, 0.05)
+    ofs = oracle_filters_like(fs, B)
+    if ng == 8:
+        print(f"[gpu] rows walk partial group ng={ng} default, 66k reads", flush=True)
+        many, _, _ = synth.fixed_len_reads(rng, genomes, 66_000, 150, 0.01, 0.5)
+        mbuf, moff = O.pack_reads([bytes(x) for x in many] + reads)
+        check_batch(D, B, fs, ofs, mbuf, moff)
+        check_batch(D, B, fs, ofs, mbuf, moff, paired=True)
+    try:
+        for ahead in (None, 12):
+            if ahead is not None:  # first the set as built, with no knobs of its own
+                fs.set_tuning(probe_ahead=ahead)
+            print(f"[gpu] rows walk partial group ng={ng} ahead={ahead}", flush=True)
+            check_batch(D, B, fs, ofs, buf, off)
+            check_batch(D, B, fs, ofs, sbuf, soff)
+            check_batch(D, B, fs, ofs, buf, off, thr=0.5, streak=1)
+            check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
+            check_batch(D, B, fs, ofs, buf, off, paired=True, wave_min=100)
+            check_batch(D, B, fs, ofs, buf, off, scorer=O.LENGTH, thr=40.0)
     finally:
         fs.set_tuning()
 
