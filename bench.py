@@ -571,7 +571,7 @@ def serve_resident(torch, D, B, fs, pool, want, steps=3):
         torch.cuda.synchronize()
         host = out.host()
         assert int(host["counters"][3]) == 0, "device error flag set"
-        res[label] = {"ms": ms, "walk_ms": float(np.mean(walk)),
+        res[label] = {"ms": ms, "walk_ms": float(np.mean(walk)), "kernel": fs.last_walk_kernel(),
                       "reads_per_s": dr.nreads / (ms / 1e3),
                       "probes": int(host["counters"][0]), "lookups": int(host["counters"][1]),
                       "issued": int(host["counters"][2]),
@@ -633,7 +633,7 @@ def bench_serve(args, torch, D, B, w, gathers):
     lookups = res["batched"]["lookups"]
     achieved = lookups * LINE_BYTES / (walk_ms / 1e3) / 1e9
     traffic = load_traffic("serve")
-    rl = {"bound": "hbm", "kernel": "k_walk_persistent",
+    rl = {"bound": "hbm", "kernel": res["batched"]["kernel"],
           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": achieved / HBM_PEAK_GBS,
           "traffic": traffic["bytes_per_launch"] if traffic else None,

@@ -608,6 +608,7 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
         return r;
     };
     ChunkHash ch;
+    uint32_t np = 0, nl = 0;
     for (uint32_t b0 = 0; b0 < len && running(); b0 += 64) {
         uint32_t pw;
         uint64_t h0;
@@ -669,6 +670,12 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
                 return uint32_t(((hi - lo) >> (16 * (g & 3))) & 0xFFFFu);
             };
             w[g].replay(b0, vmask, hmask, dmask, lk_range);
+            /* the walks' counts are summed chunk by chunk, which keeps them
+             * out of the state carried from one chunk to the next */
+            np += w[g].probes;
+            nl += w[g].lookups;
+            w[g].probes = 0;
+            w[g].lookups = 0;
         }
     }
 #pragma unroll
@@ -677,11 +684,12 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
             continue;
         if (w[g].state == W_RUNNING)
             w[g].finish_end();
-        if (lane == 0) {
+        if (lane == 0)
             store_walk(a, read, fidx(g), w[g]);
-            probes += w[g].probes;
-            lookups += w[g].lookups;
-        }
+    }
+    if (lane == 0) {
+        probes += np;
+        lookups += nl;
     }
 }
 
@@ -689,7 +697,7 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
  * first) against a group of 3 to 2^SH filters, a whole wave per read
  * (wave_group_walk); the short reads are k_walk_rows's. */
 template <int K, unsigned SH>
-__global__ __launch_bounds__(kBlock) void k_walk_group_long(EvalArgs a, const uint8_t *ilv) {
+__global__ __launch_bounds__(kBlock, 3) void k_walk_group_long(EvalArgs a, const uint8_t *ilv) {
     const uint32_t lane = lane_id();
     uint32_t probes = 0, lookups = 0, issued = 0;
     const uint64_t nl = a.list_counts[0];
@@ -2862,10 +2870,17 @@ static bbt_status eval_impl(bbt_filterset *fs, bbt_workspace &W, const bbt_reads
     bbt_status s = prepare(fs, W, in, p, out, st, P, noalloc);
     if (s != BBT_OK)
         return s;
-    W.last_walk_kernel = p->mode == BBT_ORDERED         ? "k_ordered"
-                         : P.rows                        ? "k_walk_rows"
-                         : p->path == BBT_PATH_PLAIN     ? "k_walk_plain"
+    if (P.rows) {
+        static const char *const names[4] = {
+            "k_walk_rows", "k_walk_group_long+k_walk_rows", "k_walk_rows+k_walk_persistent",
+            "k_walk_group_long+k_walk_rows+k_walk_persistent"};
+        W.last_walk_kernel =
+            names[(in->max_len >= P.wave_min ? 1 : 0) + (P.rows_n < P.nf ? 2 : 0)];
+    } else {
+        W.last_walk_kernel = p->mode == BBT_ORDERED     ? "k_ordered"
+                             : p->path == BBT_PATH_PLAIN ? "k_walk_plain"
                                                          : "k_walk_persistent";
+    }
     const bool timed = (p->flags & BBT_FLAG_TIME_WALK) != 0;
     if (timed) {
         if (!W.ev0)

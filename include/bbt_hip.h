@@ -322,9 +322,10 @@ bbt_status bbt_measure_gather(uint64_t nbytes, uint32_t rounds, void *stream,
  * called with BBT_FLAG_TIME_WALK, in milliseconds (-1 if none). */
 double bbt_last_walk_ms(const bbt_filterset *fs);
 /* Measurement utility (no counterpart in the reference): name of the walk
- * kernel the last bbt_eval_batch or bbt_eval_batch_noalloc on fs ran
- * ("k_walk_persistent", "k_walk_rows", "k_walk_plain" or "k_ordered"; "" before
- * the first batch), for profiles that name the kernel they time. */
+ * kernel(s) the last bbt_eval_batch or bbt_eval_batch_noalloc on fs ran, in
+ * launch order joined by '+' ("k_walk_persistent", "k_walk_rows",
+ * "k_walk_group_long+k_walk_rows", "k_walk_plain", "k_ordered", ...; "" before
+ * the first batch), for profiles that name the kernels they time. */
 const char *bbt_last_walk_kernel(const bbt_filterset *fs);
 /* summary.tsv text from the host copy of bbt_eval_out.summary counters;
  * writes at most cap bytes (NUL-terminated), *needed = full length. */
