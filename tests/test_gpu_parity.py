@@ -356,6 +356,44 @@ def test_rows_walk_partial_group(dev, ng):
         fs.set_tuning()
 
 
+@pytest.mark.parametrize("k,ng,extra", [(21, 5, False), (31, 4, True)])
+def test_rows_walk_runtime_k(dev, k, ng, extra):
+    """The group walks with a k other than 25 (the roll table and k read at
+    run time): a whole set of 5 filters with k 21, and a group of 4 with k 31
+    beside one larger filter; short and long reads, single-end, paired and
+    graph-replayed batches equal the oracle."""
+    D, B = dev
+    from biobloom_amd import synth
+    rng = synth.rng_for(47, k)
+    genomes = [synth.codes_to_ascii(synth.random_codes(rng, 120_000))] if extra else []
+    genomes += [synth.codes_to_ascii(synth.random_codes(rng, 40_000)) for _ in range(ng)]
+    shorts, _, _ = synth.fixed_len_reads(rng, genomes, 600, 150, 0.01, 0.7)
+    longs, _ = synth.long_reads(rng, genomes, 16, median=700, hi=3000)
+    reads = [bytes(x) for x in shorts] + [bytes(x) for x in longs] + [b"", b"N" * 64]
+    buf, off = O.pack_reads(reads)
+    fs, _ = D.build_filters(genomes, k, 0.05)
+    ofs = oracle_filters_like(fs, B)
+    try:
+        fs.set_tuning(probe_ahead=15)
+        print(f"[gpu] rows walk k={k} ng={ng} extra={extra}", flush=True)
+        check_batch(D, B, fs, ofs, buf, off)
+        check_batch(D, B, fs, ofs, buf, off, paired=True, thr=0.3)
+        check_batch(D, B, fs, ofs, buf, off, dust=(20, 64), wave_min=100)
+        dr = D.DeviceReads(buf, off)
+        oh, _, ocls, octr = O.classify(ofs, buf, off, paired=True)
+        out = D.Outputs(dr.nreads, len(fs), True, False)
+        for rep in range(3):
+            print(f"[gpu] rows walk k={k} graph replay {rep}", flush=True)
+            out.hits.fill_(7)
+            res = D.classify(fs, dr, B.params(paired=True, flags=B.FLAG_GRAPH), out=out)
+            assert np.array_equal(res["hits"], oh) and np.array_equal(res["cls"], ocls)
+            assert int(res["counters"][1]) == int(octr[1])
+            eager = D.classify(fs, dr, B.params(paired=True))
+            assert np.array_equal(eager["summary"], res["summary"])
+    finally:
+        fs.set_tuning()
+
+
 def test_store_and_load_roundtrip(dev, tmp_path):
     D, B = dev
     rng = np.random.default_rng(5)
