@@ -626,7 +626,47 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
                 need |= 1u << g;
         uint32_t all = need;
         uint64_t lk = 0, lk2 = 0; /* filters 0-3 and 4-7, 16 bits each */
-        for (uint32_t jj = 0; __any(need != 0); ++jj) {
+        /* takes the group's bits of hash jj for the filters in need */
+        const auto take = [&](const uint32_t *word, unsigned b, uint32_t jj) {
+            const uint32_t bit = group_bits<SH>(word, b);
+            lk += spread16(need);
+            if (SH == 3)
+                lk2 += spread16(need >> 4);
+            all &= bit | ~need;
+            need &= bit;
+            if (jj + 1 >= h)
+                need = 0;
+        };
+        uint32_t jj = 0;
+        if (SH == 3) {
+            /* a group of 8: some filter almost always needs the second and
+             * third hash of a window it probes, so the first kSpec loads are
+             * issued together (one load latency instead of kSpec in a row) */
+            constexpr uint32_t kSpec = 3;
+            const uint32_t ns = h < kSpec ? h : kSpec;
+            if (need) {
+                uint32_t wv[kSpec][2];
+                uint32_t bsh = 0;
+#pragma unroll
+                for (uint32_t q = 0; q < kSpec; ++q) {
+                    if (q < ns) {
+                        const uint64_t pos = fastmod(multi_hash(h0, q, ks), m, magic, sh1, sh2);
+                        const uint2 v =
+                            ld_global(reinterpret_cast<const uint2 *>(ilv + ((pos >> 3) << 3)));
+                        wv[q][0] = v.x;
+                        wv[q][1] = v.y;
+                        bsh |= uint32_t(pos & 7) << (3 * q);
+                    }
+                }
+                issued += uint32_t(__popc(need)) * ns;
+#pragma unroll
+                for (uint32_t q = 0; q < kSpec; ++q)
+                    if (q < ns && need)
+                        take(wv[q], (bsh >> (3 * q)) & 7u, q);
+            }
+            jj = ns;
+        }
+        for (; __any(need != 0); ++jj) {
             if (need) {
                 const uint64_t pos = fastmod(multi_hash(h0, jj, ks), m, magic, sh1, sh2);
                 uint32_t word[2];
@@ -637,15 +677,8 @@ __device__ void wave_group_walk(const EvalArgs &a, uint64_t read, const uint8_t 
                 } else {
                     word[0] = ld_global(reinterpret_cast<const uint32_t *>(ilv + ((pos >> 3) << 2)));
                 }
-                const uint32_t bit = group_bits<SH>(word, unsigned(pos & 7));
-                lk += spread16(need);
-                if (SH == 3)
-                    lk2 += spread16(need >> 4);
                 issued += uint32_t(__popc(need));
-                all &= bit | ~need;
-                need &= bit;
-                if (jj + 1 >= h)
-                    need = 0;
+                take(word, unsigned(pos & 7), jj);
             }
         }
         /* per-filter lookups of lanes [x, y) from packed prefix sums */
