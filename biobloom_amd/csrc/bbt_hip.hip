@@ -752,9 +752,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk_group_long(EvalArgs a, const
  * SIMPLE evalRead (RowsWalk, bbt_device.h): the read's windows are hashed
  * once, each bit position is one load of the group's 2^SH bytes (16 for 9
  * to 16 filters), and each filter keeps its own automaton in registers.
- * Lanes take reads from the queue; a read's hits are written for all its
- * filters when the last of its walks ends.  a.filters: the filters' own
- * descriptors (m, h, k, roll table; their bits are not read).
+ * Lanes take reads from the queue (with length buckets, the short list);
+ * a read's hits are written for all its filters when the last of its walks
+ * ends.  The group's filters are a.fmap[0 .. a.nw) (null: the set's first
+ * a.nw); a.filters: the filters' own descriptors (m, h, k, roll table;
+ * their bits are not read).
  */
 template <int K, unsigned SH>
 __global__ __launch_bounds__(kBlock) void k_walk_rows(EvalArgs a, const uint8_t *ilv) {
