@@ -2089,17 +2089,20 @@ struct Launch {
     uint32_t blocks_per_cu; /* persistent grid: 0 = as many as fit */
 };
 
-/* resident blocks per CU of one persistent-kernel instantiation, queried once */
+/* resident blocks per CU of a kernel of kBlock threads (at least 1) */
+template <typename KERNEL>
+int blocks_per_cu_of(KERNEL kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess)
+        per_cu = 1;
+    return std::max(1, per_cu);
+}
+
+/* resident blocks per CU of one persistent-kernel instantiation, queried once
+ * (a function-local static: initialised once, also under concurrent calls) */
 template <int SC, bool EXH, int K>
 int persistent_occupancy() {
-    static int cached = -1;
-    if (cached < 0) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, k_walk_persistent<SC, EXH, K>, kBlock, 0) != hipSuccess)
-            per_cu = 1;
-        cached = std::max(1, per_cu);
-    }
+    static const int cached = blocks_per_cu_of(k_walk_persistent<SC, EXH, K>);
     return cached;
 }
 
@@ -2133,14 +2136,7 @@ bbt_status launch_walk(const Launch &L) {
 
 template <int K, unsigned SH>
 bbt_status launch_rows_k(const Launch &L, const uint8_t *ilv) {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_walk_rows<K, SH>, kBlock, 0) !=
-            hipSuccess)
-            v = 1;
-        per_cu = std::max(1, v);
-    }
+    static const int per_cu = blocks_per_cu_of(k_walk_rows<K, SH>);
     int bpc = per_cu;
     if (L.blocks_per_cu)
         bpc = std::min<int>(bpc, int(L.blocks_per_cu));
@@ -2154,14 +2150,7 @@ bbt_status launch_rows_k(const Launch &L, const uint8_t *ilv) {
 
 template <int K, unsigned SH>
 bbt_status launch_group_long_k(const Launch &L, const uint8_t *ilv) {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_walk_group_long<K, SH>, kBlock,
-                                                         0) != hipSuccess)
-            v = 1;
-        per_cu = std::max(1, v);
-    }
+    static const int per_cu = blocks_per_cu_of(k_walk_group_long<K, SH>);
     int bpc = per_cu;
     if (L.blocks_per_cu)
         bpc = std::min<int>(bpc, int(L.blocks_per_cu));
