@@ -1591,10 +1591,15 @@ static bool ilv_whole(const bbt_filterset *fs) {
  * some reads reach the whole-wave length the group has at most 8 filters
  * (the whole-wave walk of a long read against all of them).  Returns the
  * group's size, or 0. */
-static uint32_t rows_group(const std::vector<bbt_filterset::IlvGroup> &plan,
+static uint32_t rows_group(const bbt_filterset *fs,
+                           const std::vector<bbt_filterset::IlvGroup> &plan,
                            const bbt_eval_params *p, uint64_t max_len, uint32_t wave_min) {
     if (plan.size() != 1 || p->scorer != BBT_SIMPLE || p->mode != BBT_STD ||
         p->path != BBT_PATH_BATCHED)
+        return 0;
+    /* the long-read walk sums a chunk's per-filter lookups in 16-bit fields:
+     * up to 64 h */
+    if (fs->desc[plan[0].mem.f[0]].h > 1023)
         return 0;
     const uint32_t gn = plan[0].mem.n;
     const bool buckets = max_len >= wave_min;
@@ -2590,7 +2595,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
                                  : (tune.wave_min_len ? tune.wave_min_len : kDefaultWaveMin);
     if (!fs->tuned && !from_table) {
         const bool rows_fit = !noalloc && in->nreads >= kRowsMinReads &&
-                              rows_group(ilv_plan(fs), p, in->max_len, P.wave_min) != 0;
+                              rows_group(fs, ilv_plan(fs), p, in->max_len, P.wave_min) != 0;
         if (ilv_whole(fs))
             tune.probe_ahead =
                 noalloc ? (kDefaultIlvAhead & 3u) : (rows_fit ? kDefaultRowsAhead : kDefaultIlvAhead);
@@ -2645,7 +2650,7 @@ static bbt_status prepare(bbt_filterset *fs, bbt_workspace &W, const bbt_reads *
      * filters, if any, by the lane and wave walks over their own arrays,
      * filter-major, in a second launch */
     P.rows_n = P.group && (tune.probe_ahead & 8u)
-                   ? rows_group(fs->ilv, p, in->max_len, P.wave_min)
+                   ? rows_group(fs, fs->ilv, p, in->max_len, P.wave_min)
                    : 0;
     P.rows = P.rows_n != 0;
     if (P.rows)
