@@ -295,6 +295,11 @@ def test_rows_walk_group_sizes(dev, nf):
         fs.set_tuning(probe_ahead=12)
         print(f"[gpu] rows walk nf={nf}", flush=True)
         check_batch(D, B, fs, ofs, buf, off)
+        D.classify(fs, D.DeviceReads(buf, off), B.params())
+        assert fs.last_walk_kernel() == "k_walk_rows"
+        D.classify(fs, D.DeviceReads(lbuf, loff), B.params())
+        assert fs.last_walk_kernel() == (
+            "k_walk_group_long+k_walk_rows" if nf <= 8 else "k_walk_persistent")
         check_batch(D, B, fs, ofs, buf, off, thr=0.0, streak=0)
         check_batch(D, B, fs, ofs, buf, off, thr=0.9, streak=5)
         check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
@@ -341,12 +346,19 @@ def test_rows_walk_partial_group(dev, ng):
         mbuf, moff = O.pack_reads([bytes(x) for x in many] + reads)
         check_batch(D, B, fs, ofs, mbuf, moff)
         check_batch(D, B, fs, ofs, mbuf, moff, paired=True)
+        # the plain path ran last; a batched call names the group walks
+        D.classify(fs, D.DeviceReads(mbuf, moff), B.params())
+        assert fs.last_walk_kernel() == "k_walk_group_long+k_walk_rows+k_walk_persistent"
     try:
         for ahead in (None, 12):
             if ahead is not None:  # first the set as built, with no knobs of its own
                 fs.set_tuning(probe_ahead=ahead)
             print(f"[gpu] rows walk partial group ng={ng} ahead={ahead}", flush=True)
             check_batch(D, B, fs, ofs, buf, off)
+            D.classify(fs, D.DeviceReads(buf, off), B.params())
+            assert fs.last_walk_kernel() == (
+                "k_walk_persistent" if ahead is None  # a small batch: no group walks
+                else "k_walk_group_long+k_walk_rows+k_walk_persistent")
             check_batch(D, B, fs, ofs, sbuf, soff)
             check_batch(D, B, fs, ofs, buf, off, thr=0.5, streak=1)
             check_batch(D, B, fs, ofs, buf, off, dust=(20, 64))
